@@ -1,0 +1,217 @@
+// Device-side scalar helpers shared by the gfx950 kernels: a ComplexF64
+// type with the semantics the restated algorithm depends on (abs = hypot,
+// abs1 = |re| + |im|, principal sqrt, Smith division), block reductions over
+// 64-wide wavefronts, and the Householder reflector of xLARFG form used by
+// GenericSchur.jl (householder.jl:12-102).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GS_DEV __device__ __forceinline__
+
+struct zd {
+  double x, y;
+};
+
+GS_DEV zd zmk(double a, double b) { zd r; r.x = a; r.y = b; return r; }
+GS_DEV zd zr(double a) { return zmk(a, 0.0); }
+GS_DEV zd operator+(zd a, zd b) { return zmk(a.x + b.x, a.y + b.y); }
+GS_DEV zd operator-(zd a, zd b) { return zmk(a.x - b.x, a.y - b.y); }
+GS_DEV zd operator-(zd a) { return zmk(-a.x, -a.y); }
+GS_DEV zd operator*(zd a, zd b) { return zmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+GS_DEV zd operator*(double s, zd a) { return zmk(s * a.x, s * a.y); }
+GS_DEV zd operator*(zd a, double s) { return zmk(a.x * s, a.y * s); }
+GS_DEV zd operator/(zd a, double s) { return zmk(a.x / s, a.y / s); }
+GS_DEV zd zconj(zd a) { return zmk(a.x, -a.y); }
+GS_DEV double zabs1(zd a) { return fabs(a.x) + fabs(a.y); }
+GS_DEV double zabs(zd a) { return hypot(a.x, a.y); }
+GS_DEV double zabs2(zd a) { return a.x * a.x + a.y * a.y; }
+GS_DEV bool zeq(zd a, zd b) { return a.x == b.x && a.y == b.y; }
+
+// Smith's algorithm (R. L. Smith, CACM 5(8), 1962).
+GS_DEV zd zdiv(zd a, zd b) {
+  const double ar = fabs(b.x), ai = fabs(b.y);
+  if (!(ai > ar)) {
+    if (b.x == 0.0 && b.y == 0.0) return zmk(a.x / b.x, a.y / b.x);
+    const double r = b.y / b.x;
+    const double d = b.x + b.y * r;
+    return zmk((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+  } else {
+    const double r = b.x / b.y;
+    const double d = b.x * r + b.y;
+    return zmk((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+  }
+}
+GS_DEV zd operator/(zd a, zd b) { return zdiv(a, b); }
+
+// principal square root, with power-of-four scaling near the thresholds
+GS_DEV zd zsqrt(zd z) {
+  const double x = z.x, y = z.y;
+  if (x == 0.0 && y == 0.0) return zmk(0.0, y);
+  const double s = fmax(fabs(x), fabs(y));
+  double scale = 1.0, unscale = 1.0;
+  const double big = 1.7976931348623157e308 / 16.0;
+  const double tiny = 2.2250738585072014e-308 * 16.0 / 2.220446049250313e-16;
+  if (s > big) { scale = 0.25; unscale = 2.0; }
+  else if (s < tiny) { scale = 1.0 / (2.220446049250313e-16 * 2.220446049250313e-16); unscale = 2.220446049250313e-16; }
+  const double xs = x * scale, ys = y * scale;
+  const double t = sqrt((fabs(xs) + hypot(xs, ys)) * 0.5);
+  if (!(xs < 0.0)) return zmk(t * unscale, (ys / (2.0 * t)) * unscale);
+  return zmk((fabs(ys) / (2.0 * t)) * unscale, copysign(t, ys) * unscale);
+}
+
+#define GS_EPS 2.220446049250313e-16
+#define GS_FMIN 2.2250738585072014e-308
+#define GS_FMAX 1.7976931348623157e308
+
+// util.jl:562-570 (dlapy3 scaling)
+GS_DEV double hypot3(double x, double y, double z) {
+  const double xa = fabs(x), ya = fabs(y), za = fabs(z);
+  const double w = fmax(fmax(xa, ya), za);
+  const double rw = 1.0 / w;
+  const double a = rw * xa, b = rw * ya, c = rw * za;
+  return w * sqrt(a * a + b * b + c * c);
+}
+
+// ------------------------------------------------------------ reductions
+// Butterfly reductions leave a bitwise identical result in every lane, so
+// control flow that depends on them stays uniform.
+GS_DEV double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+GS_DEV double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+  return v;
+}
+GS_DEV zd wave_zsum(zd v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x = v.x + __shfl_xor(v.x, o, 64);
+    v.y = v.y + __shfl_xor(v.y, o, 64);
+  }
+  return v;
+}
+
+// Block-wide reductions for NT threads; `red` is LDS scratch of NT/64
+// doubles.  Every thread must call them (they contain barriers when NT > 64).
+template <int NT> GS_DEV double block_max(double v, double* red) {
+  v = wave_max(v);
+  if (NT > 64) {
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    v = red[0];
+#pragma unroll
+    for (int i = 1; i < NT / 64; ++i) v = fmax(v, red[i]);
+  }
+  return v;
+}
+template <int NT> GS_DEV double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  if (NT > 64) {
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    v = red[0];
+#pragma unroll
+    for (int i = 1; i < NT / 64; ++i) v = v + red[i];
+  }
+  return v;
+}
+
+// ------------------------------------------------------------ reflectors
+// Complex xLARFG on the 2-vector (a, x), householder.jl:56-102 with n = 2.
+// On exit a = beta (real), x = v2; returns tau.
+GS_DEV zd zreflector2(zd& a, zd& x) {
+  const double sfmin = GS_FMIN / GS_EPS;
+  double ar = a.x, ai = a.y;
+  double xnorm = zabs(x);
+  if (xnorm == 0.0 && ai == 0.0) return zr(0.0);
+  double beta = -copysign(hypot3(ar, ai, xnorm), ar);
+  int kount = 0;
+  zd alpha = a;
+  if (fabs(beta) < sfmin) {
+    const double rsfmin = 1.0 / sfmin;
+    bool smallb = true;
+    while (smallb) {
+      kount += 1;
+      x = x * rsfmin;
+      beta *= rsfmin;
+      ar *= rsfmin;
+      ai *= rsfmin;
+      smallb = (fabs(beta) < sfmin) && (kount < 20);
+    }
+    xnorm = zabs(x);
+    alpha = zmk(ar, ai);
+    beta = -copysign(hypot3(ar, ai, xnorm), ar);
+  }
+  const zd tau = zmk((beta - ar) / beta, -ai / beta);
+  const zd t = zdiv(zr(1.0), alpha - zr(beta));
+  x = x * t;
+  for (int j = 0; j < kount; ++j) beta *= sfmin;
+  a = zr(beta);
+  return tau;
+}
+
+// Real xLARFG on (a, x1[, x2]) of length nr = 2 or 3, householder.jl:12-54.
+// The norm of the tail follows _norm2 (util.jl:506-525) for 1 or 2 entries.
+GS_DEV double norm2_small(double x1, double x2, int m) {
+  if (m == 1) return fabs(x1);
+  double scale = 0.0, ssq = 0.0;
+  const double xs[2] = {x1, x2};
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    if (xs[i] != 0.0) {
+      const double a = fabs(xs[i]);
+      if (scale < a) {
+        const double q = scale / a;
+        ssq = 1.0 + ssq * (q * q);
+        scale = a;
+      } else {
+        const double q = a / scale;
+        ssq = ssq + q * q;
+      }
+    }
+  }
+  return scale * sqrt(ssq);
+}
+
+GS_DEV double dreflector(int nr, double& a, double& x1, double& x2) {
+  if (nr <= 1) return 0.0;
+  const double sfmin = 2.0 * GS_FMIN / GS_EPS;
+  const int m = nr - 1;
+  double alpha = a;
+  double xnorm = norm2_small(x1, x2, m);
+  if (xnorm == 0.0) return 0.0;
+  double beta = -copysign(hypot(alpha, xnorm), alpha);
+  int kount = 0;
+  if (fabs(beta) < sfmin) {
+    const double rsfmin = 1.0 / sfmin;
+    bool smallb = true;
+    while (smallb) {
+      kount += 1;
+      x1 *= rsfmin;
+      if (m > 1) x2 *= rsfmin;
+      beta *= rsfmin;
+      alpha *= rsfmin;
+      smallb = (fabs(beta) < sfmin) && (kount < 20);
+    }
+    xnorm = norm2_small(x1, x2, m);
+    beta = -copysign(hypot(alpha, xnorm), alpha);
+  }
+  const double tau = (beta - alpha) / beta;
+  const double t = 1.0 / (alpha - beta);
+  x1 *= t;
+  if (m > 1) x2 *= t;
+  for (int j = 0; j < kount; ++j) beta *= sfmin;
+  a = beta;
+  return tau;
+}
+
+// safemin(Float64) (util.jl:5-12) is floatmin for IEEE double.
+#define GS_SAFEMIN GS_FMIN

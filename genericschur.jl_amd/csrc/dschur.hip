@@ -1,0 +1,724 @@
+// Real Schur decomposition on gfx950 to the standard-form quasi-triangular
+// T, following GenericSchur.jl's real path:
+//   driver        GenericSchur.jl:805-835
+//   QR driver     GenericSchur.jl:513-699   (dlahqr deflation, double shifts)
+//   2x2 blocks    GenericSchur.jl:716-803   (_gs2x2!, dlanv2 standard form)
+//   sweep         GenericSchur.jl:837-952   (doubleShiftQR!)
+//   Hessenberg    hessenberg.jl:3-17, householder.jl:12-54,123-172, util.jl:100
+//   Q formation   hessenberg.jl:150-166
+//
+// Same structure as the complex kernel: one workgroup per matrix, H in LDS
+// (batched) or HBM (single), Z in HBM with each sweep's 3-element
+// reflectors applied afterwards as one chain per row of Z.
+#include "dev_common.h"
+#include "internal.h"
+
+namespace gs {
+
+#define HR(i, j) H[((i) - 1) + (int64_t)((j) - 1) * ld]
+#define ZR(i, j) Zg[((i) - 1) + (int64_t)((j) - 1) * ldz]
+
+template <int NT> GS_DEV void bsync_d() {
+  if (NT > 64) __syncthreads();
+  else __builtin_amdgcn_wave_barrier();
+}
+
+GS_DEV int safescale_muls_d(double cfrom, double cto, double* mul) {
+  const double smlnum = GS_SAFEMIN;
+  const double bignum = 1.0 / smlnum;
+  double cfromc = cfrom, ctoc = cto;
+  bool done = false;
+  int k = 0;
+  while (!done && k < 64) {
+    double m;
+    const double cfrom1 = cfromc * smlnum;
+    if (cfrom1 == cfromc) {
+      m = ctoc / cfromc;
+      done = true;
+    } else {
+      const double cto1 = ctoc / bignum;
+      if (cto1 == ctoc) {
+        m = cto;
+        done = true;
+        cfromc = 1.0;
+      } else if (fabs(cfrom1) > fabs(ctoc) && ctoc != 0.0) {
+        m = smlnum;
+        cfromc = 1.0;
+      } else if (fabs(cto1) > fabs(cfromc)) {
+        m = bignum;
+        cfromc = cfrom1;
+      } else {
+        m = ctoc / cfromc;
+        done = true;
+      }
+    }
+    if (k < 4) mul[k] = m;
+    ++k;
+  }
+  return k < 4 ? k : 4;
+}
+
+// _gs2x2! (GenericSchur.jl:716-803)
+GS_DEV void gs2x2_dev(double& a, double& b, double& c, double& d, double& cs, double& sn, zd& w1, zd& w2) {
+  auto sgn = [](double x) { return (x < 0.0) ? -1.0 : 1.0; };
+  const double half = 0.5;
+  const double small = 4.0 * GS_EPS;
+  if (c == 0.0) {
+    cs = 1.0;
+    sn = 0.0;
+  } else if (b == 0.0) {
+    cs = 0.0;
+    sn = 1.0;
+    const double a0 = a, c0 = c, d0 = d;
+    a = d0;
+    b = -c0;
+    c = 0.0;
+    d = a0;
+  } else if ((a - d) == 0.0 && (b * c < 0.0)) {
+    cs = 1.0;
+    sn = 0.0;
+  } else {
+    const double asubd = a - d;
+    double p = half * asubd;
+    const double bcmax = fmax(fabs(b), fabs(c));
+    const double bcmis = fmin(fabs(b), fabs(c)) * sgn(b) * sgn(c);
+    const double scale = fmax(fabs(p), bcmax);
+    double z = (p / scale) * p + (bcmax / scale) * bcmis;
+    if (z >= small) {
+      z = p + sqrt(scale) * sqrt(z) * sgn(p);
+      a = d + z;
+      d = d - (bcmax / z) * bcmis;
+      const double tau = hypot(c, z);
+      cs = z / tau;
+      sn = c / tau;
+      b = b - c;
+      c = 0.0;
+    } else {
+      const double sigma = b + c;
+      double tau = hypot(sigma, asubd);
+      cs = sqrt(half * (1.0 + fabs(sigma) / tau));
+      sn = -(p / (tau * cs)) * sgn(sigma);
+      const double aa = a * cs + b * sn;
+      const double bb = -a * sn + b * cs;
+      const double cc = c * cs + d * sn;
+      const double dd = -c * sn + d * cs;
+      a = aa * cs + cc * sn;
+      b = bb * cs + dd * sn;
+      c = -aa * sn + cc * cs;
+      d = -bb * sn + dd * cs;
+      const double midad = half * (a + d);
+      a = midad;
+      d = a;
+      if (c != 0.0) {
+        if (b != 0.0) {
+          if (b * c >= 0.0) {
+            const double sab = sqrt(fabs(b));
+            const double sac = sqrt(fabs(c));
+            p = sab * sac * sgn(c);
+            tau = 1.0 / sqrt(fabs(b + c));
+            a = midad + p;
+            d = midad - p;
+            b = b - c;
+            c = 0.0;
+            const double cs1 = sab * tau;
+            const double sn1 = sac * tau;
+            const double csn = cs * cs1 - sn * sn1;
+            const double snn = cs * sn1 + sn * cs1;
+            cs = csn;
+            sn = snn;
+          }
+        } else {
+          b = -c;
+          c = 0.0;
+          const double tmp = cs;
+          cs = -sn;
+          sn = tmp;
+        }
+      }
+    }
+  }
+  if (c == 0.0) {
+    w1 = zr(a);
+    w2 = zr(d);
+  } else {
+    const double rti = sqrt(fabs(b)) * sqrt(fabs(c));
+    w1 = zmk(a, rti);
+    w2 = zmk(d, -rti);
+  }
+}
+
+// ------------------------------------------------------------------------
+// hessenberg.jl:3-17 for real input
+template <int NT> GS_DEV void d_hessenberg(double* H, int64_t ld, int n, double* tau, double* red) {
+  const int tid = threadIdx.x;
+  const double sfmin = 2.0 * GS_FMIN / GS_EPS;
+  for (int p = 0; p <= n - 2; ++p) {
+    const int r0 = p + 2;
+    double t_ = 0.0;
+    if (n - 1 - p > 1) {  // reflector of length >= 2 (householder.jl:15)
+      double amax = 0.0;
+      for (int r = r0 + tid; r < n; r += NT) amax = fmax(amax, fabs(H[r + p * ld]));
+      amax = block_max<NT>(amax, red);
+      double xnorm = 0.0;
+      if (amax > 0.0) {
+        const double ra = 1.0 / amax;
+        double s = 0.0;
+        for (int r = r0 + tid; r < n; r += NT) {
+          const double a = H[r + p * ld] * ra;
+          s += a * a;
+        }
+        s = block_sum<NT>(s, red);
+        xnorm = amax * sqrt(s);
+      }
+      if (xnorm != 0.0) {
+        double alpha = H[(p + 1) + p * ld];
+        double beta = -copysign(hypot(alpha, xnorm), alpha);
+        int kount = 0;
+        if (fabs(beta) < sfmin) {
+          const double rsfmin = 1.0 / sfmin;
+          bool smallb = true;
+          while (smallb) {
+            kount += 1;
+            bsync_d<NT>();
+            for (int r = r0 + tid; r < n; r += NT) H[r + p * ld] *= rsfmin;
+            beta *= rsfmin;
+            alpha *= rsfmin;
+            smallb = (fabs(beta) < sfmin) && (kount < 20);
+          }
+          __syncthreads();
+          double am2 = 0.0;
+          for (int r = r0 + tid; r < n; r += NT) am2 = fmax(am2, fabs(H[r + p * ld]));
+          am2 = block_max<NT>(am2, red);
+          xnorm = 0.0;
+          if (am2 > 0.0) {
+            const double ra = 1.0 / am2;
+            double s = 0.0;
+            for (int r = r0 + tid; r < n; r += NT) {
+              const double a = H[r + p * ld] * ra;
+              s += a * a;
+            }
+            s = block_sum<NT>(s, red);
+            xnorm = am2 * sqrt(s);
+          }
+          beta = -copysign(hypot(alpha, xnorm), alpha);
+        }
+        t_ = (beta - alpha) / beta;
+        const double sc = 1.0 / (alpha - beta);
+        __syncthreads();
+        for (int r = r0 + tid; r < n; r += NT) H[r + p * ld] *= sc;
+        for (int j = 0; j < kount; ++j) beta *= sfmin;
+        if (tid == 0) H[(p + 1) + p * ld] = beta;
+      }
+    }
+    if (tid == 0) tau[p] = t_;
+    __syncthreads();
+    // lmul!(H', A[p+1:n, p+1:n])
+    for (int j = p + 1 + tid; j < n; j += NT) {
+      double va = H[(p + 1) + (int64_t)j * ld];
+      for (int r = r0; r < n; ++r) va += H[r + p * ld] * H[r + (int64_t)j * ld];
+      va = t_ * va;
+      H[(p + 1) + (int64_t)j * ld] -= va;
+      for (int r = r0; r < n; ++r) H[r + (int64_t)j * ld] -= va * H[r + p * ld];
+    }
+    __syncthreads();
+    // rmul!(A[:, p+1:n], H, x) with BLAS ger for the rank-1 part
+    for (int r = tid; r < n; r += NT) {
+      double x = H[r + (int64_t)(p + 1) * ld];
+      for (int c = r0; c < n; ++c) x += H[r + (int64_t)c * ld] * H[c + p * ld];
+      H[r + (int64_t)(p + 1) * ld] -= t_ * x;
+      const double xm = -t_ * x;
+      for (int c = r0; c < n; ++c) H[r + (int64_t)c * ld] += xm * H[c + p * ld];
+    }
+    __syncthreads();
+  }
+}
+
+template <int NT> GS_DEV void d_form_q_inplace(double* H, int64_t ld, int n, const double* tau) {
+  const int tid = threadIdx.x;
+  for (int k = n - 2; k >= 0; --k) {
+    const int c = k + 1;
+    const double t = tau[k];
+    for (int j = c + 1 + tid; j < n; j += NT) {
+      double s = H[c + (int64_t)j * ld];
+      for (int r = c + 1; r < n; ++r) s += H[r + (int64_t)k * ld] * H[r + (int64_t)j * ld];
+      s = t * s;
+      H[c + (int64_t)j * ld] -= s;
+      for (int r = c + 1; r < n; ++r) H[r + (int64_t)j * ld] -= s * H[r + (int64_t)k * ld];
+    }
+    __syncthreads();
+    for (int r = tid; r < n; r += NT) {
+      double q;
+      if (r < c) q = 0.0;
+      else if (r == c) q = 1.0 - t;
+      else q = -(t * H[r + (int64_t)k * ld]);
+      H[r + (int64_t)c * ld] = q;
+    }
+    __syncthreads();
+  }
+  for (int r = tid; r < n; r += NT) H[r] = (r == 0) ? 1.0 : 0.0;
+  __syncthreads();
+}
+
+// Chain application of a double-shift sweep's reflectors (steps k0..k1) to Z.
+template <int NT> GS_DEV void d_apply_chain(double* Zg, int64_t ldz, int n, const double* rot, int k0, int k1) {
+  for (int r = 1 + threadIdx.x; r <= n; r += NT) {
+    double c0 = ZR(r, k0), c1 = ZR(r, k0 + 1);
+    for (int k = k0; k <= k1; ++k) {
+      const double tau1 = rot[3 * (k - k0)];
+      const double v2 = rot[3 * (k - k0) + 1];
+      const double v3 = rot[3 * (k - k0) + 2];
+      const double tau2 = tau1 * v2;
+      if (k + 2 <= k1 + 1) {  // nr == 3
+        const double tau3 = tau1 * v3;
+        const double b = ZR(r, k + 2);
+        const double ss = c0 + v2 * c1 + v3 * b;
+        ZR(r, k) = c0 - ss * tau1;
+        c0 = c1 - ss * tau2;
+        c1 = b - ss * tau3;
+      } else {  // nr == 2
+        const double ss = c0 + v2 * c1;
+        ZR(r, k) = c0 - ss * tau1;
+        c0 = c1 - ss * tau2;
+        c1 = 0.0;
+      }
+    }
+    ZR(r, k1 + 1) = c0;
+  }
+}
+
+// doubleShiftQR! (GenericSchur.jl:837-952) with deferred Z
+template <int NT> GS_DEV void d_sweep(double* H, int64_t ld, int n, double* Zg, int64_t ldz, bool wantz, double* rot,
+                                      zd shift1, zd shift2, int istart, int iend, int64_t& nstep) {
+  const int tid = threadIdx.x;
+  const double r1r = shift1.x, r1i = shift1.y, r2r = shift2.x, r2i = shift2.y;
+  const double eps = GS_EPS;
+  double v0 = 0.0, v1 = 0.0, v2s = 0.0;
+  int mx = istart;
+  for (int m = iend - 2; m >= istart; --m) {
+    double H21s = HR(m + 1, m);
+    double s = fabs(HR(m, m) - r2r) + fabs(r2i) + fabs(H21s);
+    H21s = H21s / s;
+    v0 = H21s * HR(m, m + 1) + (HR(m, m) - r1r) * ((HR(m, m) - r2r) / s) - r1i * (r2i / s);
+    v1 = H21s * (HR(m, m) + HR(m + 1, m + 1) - r1r - r2r);
+    v2s = H21s * HR(m + 2, m + 1);
+    s = fabs(v0) + fabs(v1) + fabs(v2s);
+    v0 = v0 / s;
+    v1 = v1 / s;
+    v2s = v2s / s;
+    if (m > istart && fabs(HR(m, m - 1)) * (fabs(v1) + fabs(v2s)) <=
+                          eps * fabs(v0) * (fabs(HR(m - 1, m - 1)) + fabs(HR(m, m)) + fabs(HR(m + 1, m + 1)))) {
+      mx = m;
+      break;
+    }
+  }
+  for (int k = mx; k <= iend - 1; ++k) {
+    const int nr = min(3, iend - k + 1);
+    if (k > mx) {
+      v0 = HR(k, k - 1);
+      v1 = HR(k + 1, k - 1);
+      v2s = (nr == 3) ? HR(k + 2, k - 1) : 0.0;
+    }
+    double x1 = v1, x2 = (nr == 3) ? v2s : 0.0;
+    const double tau1 = dreflector(nr, v0, x1, x2);
+    bsync_d<NT>();
+    if (tid == 0) {
+      if (k > mx) {
+        HR(k, k - 1) = v0;
+        HR(k + 1, k - 1) = 0.0;
+        if (k < iend - 1) HR(k + 2, k - 1) = 0.0;
+      } else if (mx > istart) {
+        HR(k, k - 1) = HR(k, k - 1) * (1.0 - tau1);
+      }
+    }
+    const double vv2 = x1;
+    const double tau2 = tau1 * vv2;
+    if (nr == 3) {
+      const double vv3 = x2;
+      const double tau3 = tau1 * vv3;
+      for (int j = k + tid; j <= n; j += NT) {
+        const double a = HR(k, j), b = HR(k + 1, j), c = HR(k + 2, j);
+        const double ss = a + vv2 * b + vv3 * c;
+        HR(k, j) = a - ss * tau1;
+        HR(k + 1, j) = b - ss * tau2;
+        HR(k + 2, j) = c - ss * tau3;
+      }
+      __syncthreads();
+      const int jm = min(k + 3, iend);
+      for (int j = 1 + tid; j <= jm; j += NT) {
+        const double a = HR(j, k), b = HR(j, k + 1), c = HR(j, k + 2);
+        const double ss = a + vv2 * b + vv3 * c;
+        HR(j, k) = a - ss * tau1;
+        HR(j, k + 1) = b - ss * tau2;
+        HR(j, k + 2) = c - ss * tau3;
+      }
+    } else {
+      for (int j = k + tid; j <= n; j += NT) {
+        const double a = HR(k, j), b = HR(k + 1, j);
+        const double ss = a + vv2 * b;
+        HR(k, j) = a - ss * tau1;
+        HR(k + 1, j) = b - ss * tau2;
+      }
+      __syncthreads();
+      for (int j = 1 + tid; j <= iend; j += NT) {
+        const double a = HR(j, k), b = HR(j, k + 1);
+        const double ss = a + vv2 * b;
+        HR(j, k) = a - ss * tau1;
+        HR(j, k + 1) = b - ss * tau2;
+      }
+    }
+    if (wantz && tid == 0) {
+      rot[3 * (k - mx)] = tau1;
+      rot[3 * (k - mx) + 1] = vv2;
+      rot[3 * (k - mx) + 2] = (nr == 3) ? x2 : 0.0;
+    }
+    __syncthreads();
+  }
+  nstep += iend - mx;
+  if (wantz) d_apply_chain<NT>(Zg, ldz, n, rot, mx, iend - 1);
+  __syncthreads();
+}
+
+// gschur!(H::Hessenberg{<:AbstractFloat}, Z) (GenericSchur.jl:513-699)
+template <int NT> GS_DEV int64_t d_qr(double* H, int64_t ld, int n, double* Zg, int64_t ldz, bool wantz, double* rot,
+                                      zd* Wg, int64_t maxiter, int64_t& nsweep, int64_t& nstep) {
+  const int tid = threadIdx.x;
+  const double eps = GS_EPS;
+  const double smallnum = GS_FMIN * ((double)n / eps);
+  const double threeq = 0.75;
+  const double m7_16 = -7.0 / 16.0;
+  int iwcur = n;
+  int64_t iter = 0;
+  int iend = n, istart = 1;
+  while (iend >= 1) {
+    istart = 1;
+    int iterqr = 0;
+    bool deflate = false;
+    while (true) {
+      iter += 1;
+      if (iter > maxiter) return iter;
+      bool split = false;
+      for (int k = iend; k >= istart + 1; --k) {
+        if (fabs(HR(k, k - 1)) < smallnum) {
+          split = true;
+        } else {
+          const double Hkk = HR(k, k);
+          const double Hkm1km1 = HR(k - 1, k - 1);
+          double t = fabs(Hkm1km1) + fabs(Hkk);
+          if (t == 0.0) {
+            if (k > 2) t += fabs(HR(k - 1, k - 2));
+            if (k + 1 <= n) t += fabs(HR(k + 1, k));
+          }
+          const double aHkkm1 = fabs(HR(k, k - 1));
+          if (aHkkm1 <= t * eps) {
+            const double aHkm1k = fabs(HR(k - 1, k));
+            const double ab = fmax(aHkkm1, aHkm1k);
+            const double ba = fmin(aHkkm1, aHkm1k);
+            const double aa = fmax(fabs(Hkk), fabs(Hkm1km1 - Hkk));
+            const double bb = fmin(fabs(Hkk), fabs(Hkm1km1 - Hkk));
+            const double s = aa + bb;
+            if (ba * (ab / s) <= fmax(smallnum, eps * (bb * (aa / s)))) split = true;
+          }
+        }
+        if (split) {
+          istart = k;
+          break;
+        }
+      }
+      if (!split) istart = 1;
+      if (istart > 1) {
+        bsync_d<NT>();
+        if (tid == 0) HR(istart, istart - 1) = 0.0;
+        __syncthreads();
+      }
+      if (istart >= iend - 1) {
+        deflate = true;
+        break;
+      }
+      iterqr += 1;
+      double H11, H12, H21, H22;
+      if (iterqr == 10) {
+        const double s = fabs(HR(istart + 1, istart)) + fabs(HR(istart + 2, istart + 1));
+        H11 = threeq * s + HR(istart, istart);
+        H12 = m7_16 * s;
+        H21 = s;
+        H22 = H11;
+      } else if (iterqr == 20) {
+        const double s = fabs(HR(iend, iend - 1)) + fabs(HR(iend - 1, iend - 2));
+        H11 = threeq * s + HR(iend, iend);
+        H12 = m7_16 * s;
+        H21 = s;
+        H22 = H11;
+      } else {
+        H11 = HR(iend - 1, iend - 1);
+        H21 = HR(iend, iend - 1);
+        H12 = HR(iend - 1, iend);
+        H22 = HR(iend, iend);
+      }
+      const double s = fabs(H11) + fabs(H12) + fabs(H21) + fabs(H22);
+      double r1r, r2r, r1i, r2i;
+      if (s == 0.0) {
+        r1r = r2r = r1i = r2i = 0.0;
+      } else {
+        H11 /= s;
+        H12 /= s;
+        H21 /= s;
+        H22 /= s;
+        const double tr = (H11 + H22) / 2.0;
+        const double d = (H11 - tr) * (H22 - tr) - H12 * H21;
+        const double rtd = sqrt(fabs(d));
+        if (d >= 0.0) {
+          r1r = tr * s;
+          r2r = r1r;
+          r1i = rtd * s;
+          r2i = -r1i;
+        } else {
+          r1r = tr + rtd;
+          r2r = tr - rtd;
+          if (fabs(r1r - H22) <= fabs(r2r - H22)) {
+            r1r = r1r * s;
+            r2r = r1r;
+          } else {
+            r2r = r2r * s;
+            r1r = r2r;
+          }
+          r1i = r2i = 0.0;
+        }
+      }
+      nsweep += 1;
+      d_sweep<NT>(H, ld, n, Zg, ldz, wantz, rot, zmk(r1r, r1i), zmk(r2r, r2i), istart, iend, nstep);
+    }
+    if (deflate && istart >= iend) {
+      if (tid == 0) Wg[iwcur - 1] = zr(HR(iend, iend));
+      iwcur -= 1;
+      iend = istart - 1;
+    } else if (deflate && istart + 1 == iend) {
+      double a = HR(iend - 1, iend - 1), b = HR(iend - 1, iend), c = HR(iend, iend - 1), d = HR(iend, iend);
+      double cs, sn;
+      zd w1, w2;
+      gs2x2_dev(a, b, c, d, cs, sn, w1, w2);
+      if (tid == 0) {
+        Wg[iwcur - 1] = w2;
+        Wg[iwcur - 2] = w1;
+      }
+      iwcur -= 2;
+      bsync_d<NT>();
+      // lmul!(G, HH[:, istart:n]) with G = Givens(iend-1, iend, cs, sn)
+      for (int j = istart + tid; j <= n; j += NT) {
+        const double a1 = HR(iend - 1, j), a2 = HR(iend, j);
+        HR(iend - 1, j) = cs * a1 + sn * a2;
+        HR(iend, j) = -sn * a1 + cs * a2;
+      }
+      __syncthreads();
+      // rmul!(HH[1:iend, :], G')
+      for (int i = 1 + tid; i <= iend; i += NT) {
+        const double a1 = HR(i, iend - 1), a2 = HR(i, iend);
+        HR(i, iend - 1) = a1 * cs + a2 * sn;
+        HR(i, iend) = -a1 * sn + a2 * cs;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        HR(iend - 1, iend - 1) = a;
+        HR(iend - 1, iend) = b;
+        HR(iend, iend - 1) = c;
+        HR(iend, iend) = d;
+        if (iend > 2) HR(iend - 1, iend - 2) = 0.0;
+      }
+      if (wantz) {
+        for (int i = 1 + tid; i <= n; i += NT) {
+          const double a1 = ZR(i, iend - 1), a2 = ZR(i, iend);
+          ZR(i, iend - 1) = a1 * cs + a2 * sn;
+          ZR(i, iend) = -a1 * sn + a2 * cs;
+        }
+      }
+      __syncthreads();
+    }
+    iend = istart - 1;
+  }
+  return 0;
+}
+
+template <int NT> GS_DEV void d_driver(const double* Ag, int64_t lda, double* H, int64_t ld, int n, double* Tg,
+                                       int64_t ldt, double* Zg, int64_t ldz, zd* Wg, double* tmp, int64_t ldtmp,
+                                       double* tau, double* rot, double* red, int flags, int64_t maxiter,
+                                       int32_t* info, int64_t* stats) {
+  const int tid = threadIdx.x;
+  const bool wantz = (flags & 0x01) != 0;
+  const bool scale = (flags & 0x02) != 0;
+  const bool hessin = (flags & 0x04) != 0;
+  if (maxiter <= 0) maxiter = 100 * (int64_t)n;
+  if ((const void*)Ag != (const void*)H) {
+    for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+      H[r + c * ld] = Ag[r + c * lda];
+    }
+  }
+  __syncthreads();
+  bool scaled = false;
+  double cscale = 1.0, anrm = 0.0;
+  if (scale) {
+    double m = 0.0;
+    for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+      m = fmax(m, fabs(H[r + c * ld]));
+    }
+    anrm = block_max<NT>(m, red);
+    const double smlnum = sqrt(GS_SAFEMIN) / GS_EPS;
+    const double bignum = 1.0 / smlnum;
+    if (anrm > 0.0 && anrm < smlnum) { scaled = true; cscale = smlnum; }
+    else if (anrm > bignum) { scaled = true; cscale = bignum; }
+    if (scaled) {
+      double mul[4];
+      const int nm = safescale_muls_d(anrm, cscale, mul);
+      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        double v = H[r + c * ld];
+        for (int q = 0; q < nm; ++q) v *= mul[q];
+        H[r + c * ld] = v;
+      }
+      __syncthreads();
+    }
+  }
+  if (!hessin) {
+    d_hessenberg<NT>(H, ld, n, tau, red);
+    if (wantz) {
+      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        tmp[r + c * ldtmp] = H[r + c * ld];
+      }
+      __syncthreads();
+      d_form_q_inplace<NT>(H, ld, n, tau);
+      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        const double q = H[r + c * ld];
+        const double h = tmp[r + c * ldtmp];
+        Zg[r + c * ldz] = q;
+        H[r + c * ld] = h;
+      }
+      __syncthreads();
+    }
+  }
+  // triu!(HH, -1) (GenericSchur.jl:529)
+  for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+    const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+    if (r > c + 1) H[r + c * ld] = 0.0;
+  }
+  __syncthreads();
+  int64_t nsweep = 0, nstep = 0;
+  const int64_t st = d_qr<NT>(H, ld, n, Zg, ldz, wantz, rot, Wg, maxiter, nsweep, nstep);
+  __syncthreads();
+  if (stats && tid == 0) {
+    stats[0] = nsweep;
+    stats[1] = nstep;
+  }
+  if (st != 0) {
+    if (tid == 0) *info = (int32_t)(st > 0x7fffffff ? 0x7fffffff : st);
+    return;
+  }
+  double umul[4];
+  int nu = 0;
+  if (scaled) nu = safescale_muls_d(cscale, anrm, umul);
+  for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+    const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+    double v = (r > c + 1) ? 0.0 : H[r + c * ld];
+    for (int q = 0; q < nu; ++q) v *= umul[q];
+    Tg[r + c * ldt] = v;
+  }
+  if (scaled) {
+    for (int r = tid; r < n; r += NT) {
+      zd w = Wg[r];
+      for (int q = 0; q < nu; ++q) w = w * umul[q];
+      Wg[r] = w;
+    }
+  }
+  if (tid == 0) *info = 0;
+}
+
+template <int NT> __global__ void __launch_bounds__(NT) dschur_lds_kernel(BatchArgs a) {
+  extern __shared__ double smem[];
+  const int b = blockIdx.x;
+  if (b >= a.count) return;
+  const int m = a.order[a.first + b];
+  const int n = a.n[m];
+  const int64_t ld = (int64_t)(n | 1);
+  double* H = smem;
+  double* tau = H + ld * n;
+  double* rot = tau + (n > 0 ? n : 1);
+  double* red = rot + 3 * (n > 0 ? n : 1);
+  const int64_t o = a.off[m];
+  const double* A = a.A + o;
+  double* T = a.T + o;
+  double* Z = a.Z + o;
+  zd* W = reinterpret_cast<zd*>(a.W) + a.woff[m];
+  if (n <= 0) {
+    if (threadIdx.x == 0) a.info[m] = (n == 0) ? 0 : -1;
+    return;
+  }
+  d_driver<NT>(A, n, H, ld, n, T, n, Z, n, W, T, n, tau, rot, red, a.flags, a.maxiter, a.info + m,
+               a.stats ? a.stats + 2 * m : nullptr);
+}
+
+template <int NT> __global__ void __launch_bounds__(NT) dschur_global_kernel(SingleArgs a) {
+  __shared__ double red[NT / 64 + 1];
+  const int n = (int)a.n;
+  double* tau = a.ws;
+  double* rot = tau + n;
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+    const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+    a.T[r + c * a.ldt] = a.A[r + c * a.lda];
+  }
+  __syncthreads();
+  d_driver<NT>(a.T, a.ldt, a.T, a.ldt, n, a.T, a.ldt, a.Z, a.ldz, reinterpret_cast<zd*>(a.W), a.Z, a.ldz, tau,
+               rot, red, a.flags, a.maxiter, a.info, a.stats);
+}
+
+size_t dschur_lds_bytes(int n, int nt) {
+  const size_t ld = (size_t)(n | 1);
+  return 8 * (ld * n + 4 * (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
+}
+
+int dschur_max_lds_n() {
+  int n = 1;
+  while (dschur_lds_bytes(n + 1, 256) <= 160 * 1024) ++n;
+  return n;
+}
+
+hipError_t launch_dschur_lds(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
+  const size_t lds = dschur_lds_bytes(nmax, nt);
+  if (a.count <= 0) return hipSuccess;
+  switch (nt) {
+    case 64:
+      (void)hipFuncSetAttribute((const void*)dschur_lds_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(dschur_lds_kernel<64>, dim3(a.count), dim3(64), lds, s, a);
+      break;
+    case 128:
+      (void)hipFuncSetAttribute((const void*)dschur_lds_kernel<128>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(dschur_lds_kernel<128>, dim3(a.count), dim3(128), lds, s, a);
+      break;
+    case 256:
+      (void)hipFuncSetAttribute((const void*)dschur_lds_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(dschur_lds_kernel<256>, dim3(a.count), dim3(256), lds, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_dschur_global(const SingleArgs& a, int nt, hipStream_t s) {
+  switch (nt) {
+    case 64:
+      hipLaunchKernelGGL(dschur_global_kernel<64>, dim3(1), dim3(64), 0, s, a);
+      break;
+    case 256:
+      hipLaunchKernelGGL(dschur_global_kernel<256>, dim3(1), dim3(256), 0, s, a);
+      break;
+    case 1024:
+      hipLaunchKernelGGL(dschur_global_kernel<1024>, dim3(1), dim3(1024), 0, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace gs

@@ -1,0 +1,531 @@
+// Host implementation of the C ABI in include/gschur_mi355x.h.
+//
+// The handle owns a HIP stream, grow-only device staging buffers and a
+// cache of batch plans.  A batch plan sorts the matrices into size bins
+// (largest first inside each bin, so the hardware dispatcher hands the long
+// jobs out first), uploads the per-matrix metadata once, and records the
+// per-bin launch shape (threads per workgroup, LDS bytes).  Repeated calls
+// with the same sizes and offsets reuse the plan and, for device pointers,
+// a captured hipGraph of the whole batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gschur_mi355x.h"
+#include "internal.h"
+
+using namespace gs;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct Bin {
+  int first;  // index into the order array
+  int count;
+  int nmax;
+  int nt;
+};
+
+struct Plan {
+  bool is_complex = false;
+  int64_t count = 0;
+  std::vector<int32_t> n;
+  std::vector<int64_t> off;
+  std::vector<int64_t> woff;
+  std::vector<int32_t> order;
+  std::vector<Bin> bins;
+  int64_t total_elems = 0;  // elements spanned by A/T/Z (max off + n^2)
+  int64_t total_w = 0;
+  // device metadata
+  DevBuf d_order, d_n, d_off, d_woff, d_stats;
+  // captured graph for device-pointer execution
+  hipGraphExec_t gexec = nullptr;
+  const void* g_ptrs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  int g_flags = -1;
+  ~Plan() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    d_order.release();
+    d_n.release();
+    d_off.release();
+    d_woff.release();
+    d_stats.release();
+  }
+};
+
+}  // namespace
+
+struct gs_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  DevBuf bA, bT, bZ, bW, bInfo, bWs, bStats, bMeta;
+  std::map<std::string, std::unique_ptr<Plan>> plans;
+  Plan* last_plan = nullptr;
+  int64_t last_stats[3] = {0, 0, 0};
+  bool last_stats_from_plan = false;
+  bool last_single_stats = false;
+  std::mutex mu;
+};
+
+static int fail_hip(gs_handle* h, hipError_t e, const char* where) {
+  if (h) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%s: %s", where, hipGetErrorString(e));
+    h->err = buf;
+  }
+  return GS_E_HIP - (int)e;
+}
+
+static int fail(gs_handle* h, int code, const char* msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, call)                                  \
+  do {                                                   \
+    hipError_t e_ = (call);                              \
+    if (e_ != hipSuccess) return fail_hip((h), e_, #call); \
+  } while (0)
+
+extern "C" {
+
+int gs_version(void) { return 10000; }
+
+int gs_create(gs_handle** out, int device) {
+  if (!out) return GS_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess) return GS_E_HIP - (int)e;
+  if (device < 0 || device >= ndev) return GS_E_ARG;
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return GS_E_HIP - (int)e;
+  gs_handle* h = new gs_handle();
+  h->device = device;
+  e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e != hipSuccess) {
+    delete h;
+    return GS_E_HIP - (int)e;
+  }
+  *out = h;
+  return GS_OK;
+}
+
+int gs_destroy(gs_handle* h) {
+  if (!h) return GS_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  h->plans.clear();
+  h->bA.release();
+  h->bT.release();
+  h->bZ.release();
+  h->bW.release();
+  h->bInfo.release();
+  h->bWs.release();
+  h->bStats.release();
+  h->bMeta.release();
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return GS_OK;
+}
+
+const char* gs_last_error(const gs_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int gs_synchronize(gs_handle* h) {
+  if (!h) return GS_E_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return GS_OK;
+}
+
+int gs_batched_max_n(int is_complex) { return is_complex ? zschur_max_lds_n() : dschur_max_lds_n(); }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ plans
+static int default_nt(bool cplx, int nmax) {
+  (void)cplx;
+  (void)nmax;
+  return 64;
+}
+
+static int bin_width(bool cplx) { return cplx ? 16 : 16; }
+
+static std::string plan_key(bool cplx, int64_t count, const int32_t* n, const int64_t* off) {
+  std::string k;
+  k.reserve(16 + (size_t)count * 12);
+  k.push_back(cplx ? 'z' : 'd');
+  k.append(reinterpret_cast<const char*>(&count), sizeof count);
+  k.append(reinterpret_cast<const char*>(n), sizeof(int32_t) * (size_t)count);
+  k.append(reinterpret_cast<const char*>(off), sizeof(int64_t) * (size_t)count);
+  return k;
+}
+
+static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, const int64_t* off, Plan** out) {
+  const std::string key = plan_key(cplx, count, n, off);
+  auto it = h->plans.find(key);
+  if (it != h->plans.end()) {
+    *out = it->second.get();
+    return GS_OK;
+  }
+  const int nmax_lds = cplx ? zschur_max_lds_n() : dschur_max_lds_n();
+  std::unique_ptr<Plan> p(new Plan());
+  p->is_complex = cplx;
+  p->count = count;
+  p->n.assign(n, n + count);
+  p->off.assign(off, off + count);
+  p->woff.resize(count);
+  int64_t w = 0, span = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    if (n[i] < 0 || n[i] > nmax_lds) return fail(h, GS_E_DIM, "matrix order outside the batched range");
+    p->woff[i] = w;
+    w += n[i];
+    span = std::max<int64_t>(span, off[i] + (int64_t)n[i] * n[i]);
+  }
+  p->total_w = w;
+  p->total_elems = span;
+  // bins of width bin_width, largest bin first; inside a bin largest n first
+  const int bw = bin_width(cplx);
+  const int nb = (nmax_lds + bw - 1) / bw + 1;
+  std::vector<std::vector<int32_t>> byb(nb);
+  for (int64_t i = 0; i < count; ++i) {
+    const int b = n[i] <= 0 ? 0 : (n[i] - 1) / bw;
+    byb[b].push_back((int32_t)i);
+  }
+  for (int b = nb - 1; b >= 0; --b) {
+    auto& v = byb[b];
+    if (v.empty()) continue;
+    std::stable_sort(v.begin(), v.end(), [&](int32_t x, int32_t y) { return n[x] > n[y]; });
+    Bin bin;
+    bin.first = (int)p->order.size();
+    bin.count = (int)v.size();
+    bin.nmax = std::max(1, n[v.front()]);
+    bin.nt = default_nt(cplx, bin.nmax);
+    p->bins.push_back(bin);
+    p->order.insert(p->order.end(), v.begin(), v.end());
+  }
+  HIPCHK(h, p->d_order.ensure(sizeof(int32_t) * std::max<int64_t>(count, 1)));
+  HIPCHK(h, p->d_n.ensure(sizeof(int32_t) * std::max<int64_t>(count, 1)));
+  HIPCHK(h, p->d_off.ensure(sizeof(int64_t) * std::max<int64_t>(count, 1)));
+  HIPCHK(h, p->d_woff.ensure(sizeof(int64_t) * std::max<int64_t>(count, 1)));
+  HIPCHK(h, p->d_stats.ensure(sizeof(int64_t) * 2 * std::max<int64_t>(count, 1)));
+  if (count > 0) {
+    HIPCHK(h, hipMemcpy(p->d_order.p, p->order.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(p->d_n.p, p->n.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(p->d_off.p, p->off.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(p->d_woff.p, p->woff.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice));
+  }
+  Plan* raw = p.get();
+  if (h->plans.size() > 64) h->plans.clear();
+  h->plans[key] = std::move(p);
+  *out = raw;
+  return GS_OK;
+}
+
+static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T, double* Z, double* W,
+                              int32_t* info, int flags) {
+  BatchArgs a;
+  a.order = static_cast<const int32_t*>(p->d_order.p);
+  a.n = static_cast<const int32_t*>(p->d_n.p);
+  a.off = static_cast<const int64_t*>(p->d_off.p);
+  a.woff = static_cast<const int64_t*>(p->d_woff.p);
+  a.A = A;
+  a.T = T;
+  a.Z = Z;
+  a.W = W;
+  a.info = info;
+  a.stats = static_cast<int64_t*>(p->d_stats.p);
+  a.flags = flags;
+  a.maxiter = 0;
+  a.maxinner = 0;
+  for (const Bin& b : p->bins) {
+    a.first = b.first;
+    a.count = b.count;
+    hipError_t e = p->is_complex ? launch_zschur_lds(a, b.nmax, b.nt, h->stream)
+                                 : launch_dschur_lds(a, b.nmax, b.nt, h->stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n, const int64_t* off,
+                       const double* A, double* T, double* Z, double* W, int32_t* info, int flags) {
+  if (!h || count < 0 || (count > 0 && (!n || !off || !A || !T || !W || !info))) return fail(h, GS_E_ARG, "bad argument");
+  if ((flags & GS_WANTZ) && count > 0 && !Z) return fail(h, GS_E_ARG, "Z required with GS_WANTZ");
+  if (flags & GS_HESSENBERG) return fail(h, GS_E_UNSUPPORTED, "GS_HESSENBERG is single-matrix only");
+  std::lock_guard<std::mutex> lk(h->mu);
+  HIPCHK(h, hipSetDevice(h->device));
+  Plan* p = nullptr;
+  int rc = build_plan(h, cplx, count, n, off, &p);
+  if (rc != GS_OK) return rc;
+  h->last_plan = p;
+  h->last_stats_from_plan = true;
+  if (count == 0) return GS_OK;
+  const int kflags = flags & (GS_WANTZ | GS_SCALE);
+  const size_t es = cplx ? 16 : 8;
+  double* dZdummy = nullptr;
+  if (flags & GS_DEVICE_PTRS) {
+    if (!(flags & GS_WANTZ)) {
+      HIPCHK(h, h->bZ.ensure(es * (size_t)p->total_elems));
+      dZdummy = static_cast<double*>(h->bZ.p);
+    }
+    double* Zd = (flags & GS_WANTZ) ? Z : dZdummy;
+    const void* ptrs[6] = {A, T, Zd, W, info, nullptr};
+    if (!p->gexec || std::memcmp(ptrs, p->g_ptrs, sizeof ptrs) != 0 || p->g_flags != kflags) {
+      if (p->gexec) {
+        (void)hipGraphExecDestroy(p->gexec);
+        p->gexec = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t le = launch_plan(h, p, A, T, Zd, W, info, kflags);
+      hipError_t ce = hipStreamEndCapture(h->stream, &g);
+      if (le != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail_hip(h, le, "kernel launch (capture)");
+      }
+      if (ce != hipSuccess) return fail_hip(h, ce, "hipStreamEndCapture");
+      hipError_t ie = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        p->gexec = nullptr;
+        return fail_hip(h, ie, "hipGraphInstantiate");
+      }
+      std::memcpy(p->g_ptrs, ptrs, sizeof ptrs);
+      p->g_flags = kflags;
+    }
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    HIPCHK(h, hipGraphLaunch(p->gexec, h->stream));
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+    if (!(flags & GS_ASYNC)) HIPCHK(h, hipStreamSynchronize(h->stream));
+    return GS_OK;
+  }
+  // host pointers: stage through device buffers
+  const size_t nel = (size_t)p->total_elems;
+  HIPCHK(h, h->bA.ensure(es * nel));
+  HIPCHK(h, h->bT.ensure(es * nel));
+  HIPCHK(h, h->bZ.ensure(es * nel));
+  HIPCHK(h, h->bW.ensure(16 * (size_t)std::max<int64_t>(p->total_w, 1)));
+  HIPCHK(h, h->bInfo.ensure(sizeof(int32_t) * (size_t)count));
+  HIPCHK(h, hipMemcpyAsync(h->bA.p, A, es * nel, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  hipError_t le = launch_plan(h, p, static_cast<double*>(h->bA.p), static_cast<double*>(h->bT.p),
+                              static_cast<double*>(h->bZ.p), static_cast<double*>(h->bW.p),
+                              static_cast<int32_t*>(h->bInfo.p), kflags);
+  if (le != hipSuccess) return fail_hip(h, le, "kernel launch");
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  // copy back only the matrices' own elements (leave caller memory between
+  // matrices untouched)
+  std::vector<double> hostT(nel * (es / 8)), hostZ;
+  HIPCHK(h, hipMemcpyAsync(hostT.data(), h->bT.p, es * nel, hipMemcpyDeviceToHost, h->stream));
+  if (flags & GS_WANTZ) {
+    hostZ.resize(nel * (es / 8));
+    HIPCHK(h, hipMemcpyAsync(hostZ.data(), h->bZ.p, es * nel, hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHK(h, hipMemcpyAsync(W, h->bW.p, 16 * (size_t)p->total_w, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(info, h->bInfo.p, sizeof(int32_t) * count, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const size_t ed = es / 8;
+  for (int64_t i = 0; i < count; ++i) {
+    const size_t o = (size_t)off[i] * ed, len = (size_t)n[i] * n[i] * ed;
+    std::memcpy(T + o, hostT.data() + o, 8 * len);
+    if (flags & GS_WANTZ) std::memcpy(Z + o, hostZ.data() + o, 8 * len);
+  }
+  return GS_OK;
+}
+
+extern "C" int gs_schur_batched_z(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off, const double* A,
+                                  double* T, double* Z, double* W, int32_t* info, int flags) {
+  return run_batched(h, true, count, n, off, A, T, Z, W, info, flags);
+}
+
+extern "C" int gs_schur_batched_d(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off, const double* A,
+                                  double* T, double* Z, double* W, int32_t* info, int flags) {
+  return run_batched(h, false, count, n, off, A, T, Z, W, info, flags);
+}
+
+// ------------------------------------------------------------ single matrix
+static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda, double* T, int64_t ldt, double* Z,
+                      int64_t ldz, double* W, int flags, int64_t maxiter, int64_t maxinner, int32_t* info) {
+  if (!h || !info) return fail(h, GS_E_ARG, "bad argument");
+  if (n < 0 || lda < std::max<int64_t>(n, 1) || ldt < std::max<int64_t>(n, 1) ||
+      ((flags & GS_WANTZ) && ldz < std::max<int64_t>(n, 1))) {
+    *info = GS_INFO_DIM;
+    return fail(h, GS_E_DIM, "DimensionMismatch: leading dimension smaller than n");
+  }
+  if (n > 0 && (!A || !T || !W || ((flags & GS_WANTZ) && !Z))) return fail(h, GS_E_ARG, "null pointer");
+  std::lock_guard<std::mutex> lk(h->mu);
+  HIPCHK(h, hipSetDevice(h->device));
+  h->last_stats_from_plan = false;
+  if (n == 0) {
+    *info = 0;
+    return GS_OK;
+  }
+  const size_t es = cplx ? 16 : 8;
+  const size_t ed = es / 8;
+  const bool dev = (flags & GS_DEVICE_PTRS) != 0;
+  const int kflags = flags & (GS_WANTZ | GS_SCALE | GS_HESSENBERG | GS_CHECK_SUBDIAG);
+  // device operands, packed with ld = n when staged
+  double *dA, *dT, *dZ, *dW;
+  int32_t* dInfo;
+  int64_t dlda = lda, dldt = ldt, dldz = ldz;
+  const size_t nn = (size_t)n * n;
+  HIPCHK(h, h->bInfo.ensure(sizeof(int32_t)));
+  HIPCHK(h, h->bStats.ensure(2 * sizeof(int64_t)));
+  HIPCHK(h, h->bWs.ensure(es * 4 * (size_t)n + 64));
+  if (dev) {
+    dA = A;
+    dT = T;
+    dW = W;
+    dInfo = static_cast<int32_t*>(h->bInfo.p);
+    if (flags & GS_WANTZ) {
+      dZ = Z;
+    } else {
+      HIPCHK(h, h->bZ.ensure(es * nn));
+      dZ = static_cast<double*>(h->bZ.p);
+      dldz = n;
+    }
+  } else {
+    HIPCHK(h, h->bA.ensure(es * nn));
+    HIPCHK(h, h->bT.ensure(es * nn));
+    HIPCHK(h, h->bZ.ensure(es * nn));
+    HIPCHK(h, h->bW.ensure(16 * (size_t)n));
+    dA = static_cast<double*>(h->bA.p);
+    dT = static_cast<double*>(h->bT.p);
+    dZ = static_cast<double*>(h->bZ.p);
+    dW = static_cast<double*>(h->bW.p);
+    dInfo = static_cast<int32_t*>(h->bInfo.p);
+    dlda = dldt = dldz = n;
+    HIPCHK(h, hipMemcpy2DAsync(dA, es * n, A, es * lda, es * n, n, hipMemcpyHostToDevice, h->stream));
+    if ((flags & GS_HESSENBERG) && (flags & GS_WANTZ))
+      HIPCHK(h, hipMemcpy2DAsync(dZ, es * n, Z, es * ldz, es * n, n, hipMemcpyHostToDevice, h->stream));
+  }
+  const int nlds = cplx ? zschur_max_lds_n() : dschur_max_lds_n();
+  const bool use_lds = !(flags & GS_PLAIN) && n <= nlds && dlda == n && dldt == n && dldz == n;
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  hipError_t le;
+  if (use_lds) {
+    HIPCHK(h, h->bMeta.ensure(64));
+    // metadata for a batch of one: order = 0, n, off = 0, woff = 0
+    struct Meta {
+      int32_t order, n;
+      int64_t off, woff;
+    } meta = {0, (int32_t)n, 0, 0};
+    HIPCHK(h, hipMemcpyAsync(h->bMeta.p, &meta, sizeof meta, hipMemcpyHostToDevice, h->stream));
+    char* mp = static_cast<char*>(h->bMeta.p);
+    BatchArgs a;
+    a.order = reinterpret_cast<const int32_t*>(mp + offsetof(Meta, order));
+    a.n = reinterpret_cast<const int32_t*>(mp + offsetof(Meta, n));
+    a.off = reinterpret_cast<const int64_t*>(mp + offsetof(Meta, off));
+    a.woff = reinterpret_cast<const int64_t*>(mp + offsetof(Meta, woff));
+    a.first = 0;
+    a.count = 1;
+    a.A = dA;
+    a.T = dT;
+    a.Z = dZ;
+    a.W = dW;
+    a.info = dInfo;
+    a.stats = static_cast<int64_t*>(h->bStats.p);
+    a.flags = kflags;
+    a.maxiter = maxiter;
+    a.maxinner = maxinner;
+    le = cplx ? launch_zschur_lds(a, (int)n, 64, h->stream) : launch_dschur_lds(a, (int)n, 64, h->stream);
+  } else {
+    SingleArgs a;
+    a.n = n;
+    a.A = dA;
+    a.lda = dlda;
+    a.T = dT;
+    a.ldt = dldt;
+    a.Z = dZ;
+    a.ldz = dldz;
+    a.W = dW;
+    a.ws = static_cast<double*>(h->bWs.p);
+    a.info = dInfo;
+    a.stats = static_cast<int64_t*>(h->bStats.p);
+    a.flags = kflags;
+    a.maxiter = maxiter;
+    a.maxinner = maxinner;
+    const int nt = n <= 64 ? 64 : (n <= 512 ? 256 : 1024);
+    le = cplx ? launch_zschur_global(a, nt, h->stream) : launch_dschur_global(a, nt, h->stream);
+  }
+  if (le != hipSuccess) return fail_hip(h, le, "kernel launch");
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipMemcpyAsync(info, dInfo, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+  if (!dev) {
+    HIPCHK(h, hipMemcpy2DAsync(T, es * ldt, dT, es * n, es * n, n, hipMemcpyDeviceToHost, h->stream));
+    if (flags & GS_WANTZ)
+      HIPCHK(h, hipMemcpy2DAsync(Z, es * ldz, dZ, es * n, es * n, n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(W, dW, 16 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  (void)ed;
+  return GS_OK;
+}
+
+extern "C" int gs_schur_z(gs_handle* h, int64_t n, double* A, int64_t lda, double* T, int64_t ldt, double* Z,
+                          int64_t ldz, double* W, int flags, int64_t maxiter, int64_t maxinner, int32_t* info) {
+  return run_single(h, true, n, A, lda, T, ldt, Z, ldz, W, flags, maxiter, maxinner, info);
+}
+
+extern "C" int gs_schur_d(gs_handle* h, int64_t n, double* A, int64_t lda, double* T, int64_t ldt, double* Z,
+                          int64_t ldz, double* W, int flags, int64_t maxiter, int64_t maxinner, int32_t* info) {
+  return run_single(h, false, n, A, lda, T, ldt, Z, ldz, W, flags, maxiter, maxinner, info);
+}
+
+extern "C" int gs_last_stats(const gs_handle* hc, int64_t* out, int len) {
+  gs_handle* h = const_cast<gs_handle*>(hc);
+  if (!h || !out || len <= 0) return 0;
+  int64_t sweeps = 0, steps = 0;
+  if (h->last_stats_from_plan && h->last_plan && h->last_plan->count > 0) {
+    std::vector<int64_t> st(2 * (size_t)h->last_plan->count);
+    if (hipMemcpy(st.data(), h->last_plan->d_stats.p, sizeof(int64_t) * st.size(), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      return 0;
+    for (size_t i = 0; i < st.size(); i += 2) {
+      sweeps += st[i];
+      steps += st[i + 1];
+    }
+  } else if (h->bStats.p) {
+    int64_t st[2] = {0, 0};
+    if (hipMemcpy(st, h->bStats.p, sizeof st, hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    sweeps = st[0];
+    steps = st[1];
+  }
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess) ms = 0.f;
+  const int64_t vals[3] = {sweeps, steps, (int64_t)(ms * 1000.0f)};
+  int k = 0;
+  for (; k < len && k < 3; ++k) out[k] = vals[k];
+  return k;
+}

@@ -1,0 +1,60 @@
+// Internal declarations shared by the host API and the kernel translation
+// units of libgschur_mi355x.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gs {
+
+// Arguments of the batched Schur kernels.  Matrix m has order n[m], lives at
+// element offset off[m] of A/T/Z (column-major, ld = n[m]) and owns the
+// eigenvalue slots woff[m] .. woff[m] + n[m] - 1 of W.  Launch item b
+// processes matrix order[first + b].
+struct BatchArgs {
+  const int32_t* order;
+  int first;
+  int count;
+  const int32_t* n;
+  const int64_t* off;
+  const int64_t* woff;
+  const double* A;
+  double* T;
+  double* Z;
+  double* W;
+  int32_t* info;
+  int64_t* stats;  // 2 per matrix (sweeps, bulge steps) or null
+  int flags;
+  int64_t maxiter;   // <= 0: 100 n
+  int64_t maxinner;  // <= 0: 30 n
+};
+
+// Arguments of the single-matrix global-memory kernels (any n).
+struct SingleArgs {
+  int64_t n;
+  double* A;  // input (device), ld = lda
+  int64_t lda;
+  double* T;
+  int64_t ldt;
+  double* Z;
+  int64_t ldz;
+  double* W;
+  double* ws;  // device scratch: >= 4 n doubles (complex) / 4 n (real)
+  int32_t* info;
+  int64_t* stats;
+  int flags;
+  int64_t maxiter;
+  int64_t maxinner;
+};
+
+// LDS bytes needed by the batched kernels for order n with NT threads.
+size_t zschur_lds_bytes(int n, int nt);
+size_t dschur_lds_bytes(int n, int nt);
+int zschur_max_lds_n();
+int dschur_max_lds_n();
+
+hipError_t launch_zschur_lds(const BatchArgs& a, int nmax, int nt, hipStream_t s);
+hipError_t launch_dschur_lds(const BatchArgs& a, int nmax, int nt, hipStream_t s);
+hipError_t launch_zschur_global(const SingleArgs& a, int nt, hipStream_t s);
+hipError_t launch_dschur_global(const SingleArgs& a, int nt, hipStream_t s);
+
+}  // namespace gs

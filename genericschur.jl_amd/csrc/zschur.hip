@@ -1,0 +1,634 @@
+// Complex Schur decomposition on gfx950: Householder Hessenberg reduction,
+// formation of Z, and single-shift Francis QR with deflation, following
+// GenericSchur.jl's complex path:
+//   driver        GenericSchur.jl:350-372   (scale, Hessenberg, Z, QR, unscale)
+//   QR driver     GenericSchur.jl:194-335   (zlahqr deflation and shifts)
+//   sweep         GenericSchur.jl:374-504   (singleShiftQR!)
+//   Hessenberg    hessenberg.jl:3-17, householder.jl:56-172
+//   Q formation   hessenberg.jl:150-166
+//   scaling       util.jl:14-80
+//
+// One workgroup owns one matrix.  The working matrix H is either resident
+// in LDS (batched kernel, n <= zschur_max_lds_n()) or in HBM (single-matrix
+// kernel, any n); the same core serves both through a plain pointer whose
+// address space the compiler infers after inlining.  Z always lives in HBM:
+// the rotations of a sweep are logged in LDS and applied afterwards as one
+// chain per row of Z (load Z(r,k+1), rotate with the carried Z(r,k), store),
+// which keeps the sequential bulge chase free of HBM traffic.
+//
+// Every thread runs the scalar control logic (deflation tests, shifts,
+// reflectors) redundantly on values read from H; reductions are butterflies,
+// so all threads take the same branches.
+#include "dev_common.h"
+#include "internal.h"
+
+namespace gs {
+
+#define HH(i, j) H[((i) - 1) + (int64_t)((j) - 1) * ld]
+#define ZZ(i, j) Zg[((i) - 1) + (int64_t)((j) - 1) * ldz]
+
+template <int NT> GS_DEV void bsync() {
+  if (NT > 64) __syncthreads();
+  else __builtin_amdgcn_wave_barrier();
+}
+
+// util.jl:41-80 (xLASCL-style ratio multiply), the multipliers in order.
+// Returns the number of multipliers written to mul[0..3].
+GS_DEV int safescale_muls(double cfrom, double cto, double* mul) {
+  const double smlnum = GS_SAFEMIN;
+  const double bignum = 1.0 / smlnum;
+  double cfromc = cfrom, ctoc = cto;
+  bool done = false;
+  int k = 0;
+  while (!done && k < 64) {
+    double m;
+    const double cfrom1 = cfromc * smlnum;
+    if (cfrom1 == cfromc) {
+      m = ctoc / cfromc;
+      done = true;
+    } else {
+      const double cto1 = ctoc / bignum;
+      if (cto1 == ctoc) {
+        m = cto;
+        done = true;
+        cfromc = 1.0;
+      } else if (fabs(cfrom1) > fabs(ctoc) && ctoc != 0.0) {
+        m = smlnum;
+        cfromc = 1.0;
+      } else if (fabs(cto1) > fabs(cfromc)) {
+        m = bignum;
+        cfromc = cfrom1;
+      } else {
+        m = ctoc / cfromc;
+        done = true;
+      }
+    }
+    if (k < 4) mul[k] = m;
+    ++k;
+  }
+  return k < 4 ? k : 4;
+}
+
+// ------------------------------------------------------------------------
+// Hessenberg reduction in place (hessenberg.jl:3-17).  Reflector p (0-based
+// column) is stored below the subdiagonal of column p, tau[p] its scalar.
+template <int NT> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n, zd* tau, double* red) {
+  const int tid = threadIdx.x;
+  const double sfmin = GS_FMIN / GS_EPS;
+  for (int p = 0; p <= n - 2; ++p) {
+    // reflector on x = H[p+1 : n-1, p]   (householder.jl:56-102)
+    const int r0 = p + 2;  // first tail row
+    double amax = 0.0;
+    for (int r = r0 + tid; r < n; r += NT) {
+      const zd v = H[r + p * ld];
+      amax = fmax(amax, fmax(fabs(v.x), fabs(v.y)));
+    }
+    amax = block_max<NT>(amax, red);
+    double xnorm = 0.0;
+    if (amax > 0.0) {
+      const double ra = 1.0 / amax;
+      double s = 0.0;
+      for (int r = r0 + tid; r < n; r += NT) {
+        const zd v = H[r + p * ld];
+        const double a = v.x * ra, b = v.y * ra;
+        s += a * a + b * b;
+      }
+      s = block_sum<NT>(s, red);
+      xnorm = amax * sqrt(s);
+    }
+    zd alpha = H[(p + 1) + p * ld];
+    zd t_ = zr(0.0);
+    if (!(xnorm == 0.0 && alpha.y == 0.0)) {
+      double ar = alpha.x, ai = alpha.y;
+      double beta = -copysign(hypot3(ar, ai, xnorm), ar);
+      int kount = 0;
+      if (fabs(beta) < sfmin) {
+        const double rsfmin = 1.0 / sfmin;
+        bool smallb = true;
+        while (smallb) {
+          kount += 1;
+          bsync<NT>();
+          for (int r = r0 + tid; r < n; r += NT) H[r + p * ld] = H[r + p * ld] * rsfmin;
+          beta *= rsfmin;
+          ar *= rsfmin;
+          ai *= rsfmin;
+          smallb = (fabs(beta) < sfmin) && (kount < 20);
+        }
+        __syncthreads();
+        double am2 = 0.0;
+        for (int r = r0 + tid; r < n; r += NT) {
+          const zd v = H[r + p * ld];
+          am2 = fmax(am2, fmax(fabs(v.x), fabs(v.y)));
+        }
+        am2 = block_max<NT>(am2, red);
+        xnorm = 0.0;
+        if (am2 > 0.0) {
+          const double ra = 1.0 / am2;
+          double s = 0.0;
+          for (int r = r0 + tid; r < n; r += NT) {
+            const zd v = H[r + p * ld];
+            const double a = v.x * ra, b = v.y * ra;
+            s += a * a + b * b;
+          }
+          s = block_sum<NT>(s, red);
+          xnorm = am2 * sqrt(s);
+        }
+        alpha = zmk(ar, ai);
+        beta = -copysign(hypot3(ar, ai, xnorm), ar);
+      }
+      t_ = zmk((beta - ar) / beta, -ai / beta);
+      const zd sc = zdiv(zr(1.0), alpha - zr(beta));
+      __syncthreads();
+      for (int r = r0 + tid; r < n; r += NT) H[r + p * ld] = H[r + p * ld] * sc;
+      for (int j = 0; j < kount; ++j) beta *= sfmin;
+      if (tid == 0) H[(p + 1) + p * ld] = zr(beta);
+    }
+    if (tid == 0) tau[p] = t_;
+    __syncthreads();
+    const zd ct = zconj(t_);
+    // lmul!(H', A[p+1:n, p+1:n])   (householder.jl:157-172)
+    for (int j = p + 1 + tid; j < n; j += NT) {
+      zd va = H[(p + 1) + (int64_t)j * ld];
+      for (int r = r0; r < n; ++r) va = va + zconj(H[r + p * ld]) * H[r + (int64_t)j * ld];
+      va = ct * va;
+      H[(p + 1) + (int64_t)j * ld] = H[(p + 1) + (int64_t)j * ld] - va;
+      for (int r = r0; r < n; ++r) H[r + (int64_t)j * ld] = H[r + (int64_t)j * ld] - va * H[r + p * ld];
+    }
+    __syncthreads();
+    // rmul!(A[:, p+1:n], H, x)     (householder.jl:140-155, util.jl:103-114)
+    const zd mt = -t_;
+    for (int r = tid; r < n; r += NT) {
+      zd x = H[r + (int64_t)(p + 1) * ld];
+      for (int c = r0; c < n; ++c) x = x + H[r + (int64_t)c * ld] * H[c + p * ld];
+      H[r + (int64_t)(p + 1) * ld] = H[r + (int64_t)(p + 1) * ld] - t_ * x;
+      const zd xm = x * mt;
+      for (int c = r0; c < n; ++c) H[r + (int64_t)c * ld] = H[r + (int64_t)c * ld] + xm * zconj(H[c + p * ld]);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------
+// Q = diag(1, H_0 H_1 ... H_{n-2}) formed in place over the reflector
+// storage (hessenberg.jl:150-166; the unblocked backward accumulation of
+// xUNG2R on the reflectors shifted one column right).
+template <int NT> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, int n, const zd* tau) {
+  const int tid = threadIdx.x;
+  for (int k = n - 2; k >= 0; --k) {
+    const int c = k + 1;  // column of Q produced by reflector k
+    const zd t = tau[k];
+    // apply H_k to Q(c:n-1, c+1:n-1); Q(c, j>c) is zero at this point
+    for (int j = c + 1 + tid; j < n; j += NT) {
+      zd s = H[c + (int64_t)j * ld];
+      for (int r = c + 1; r < n; ++r) s = s + zconj(H[r + (int64_t)k * ld]) * H[r + (int64_t)j * ld];
+      s = t * s;
+      H[c + (int64_t)j * ld] = H[c + (int64_t)j * ld] - s;
+      for (int r = c + 1; r < n; ++r) H[r + (int64_t)j * ld] = H[r + (int64_t)j * ld] - s * H[r + (int64_t)k * ld];
+    }
+    __syncthreads();
+    // column c: -tau * v below, 1 - tau on the diagonal, zero above
+    for (int r = tid; r < n; r += NT) {
+      zd q;
+      if (r < c) q = zr(0.0);
+      else if (r == c) q = zr(1.0) - t;
+      else q = -(t * H[r + (int64_t)k * ld]);
+      H[r + (int64_t)c * ld] = q;
+    }
+    __syncthreads();
+  }
+  for (int r = tid; r < n; r += NT) H[r] = (r == 0) ? zr(1.0) : zr(0.0);
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------
+// Deferred application of one sweep's rotations to Z (columns k0..k1+1),
+// as a chain per row; `ph` is the phase of the shifted-start scaling
+// (applied to columns k0 and k0+2..k1+1), `tl` the tail phase of column k1+1.
+template <int NT> GS_DEV void z_apply_chain(zd* Zg, int64_t ldz, int n, const zd* rot, int k0, int k1,
+                                            bool has_ph, zd ph, bool has_tl, zd tl) {
+  const zd cph = zconj(ph);
+  for (int r = 1 + threadIdx.x; r <= n; r += NT) {
+    zd carry = ZZ(r, k0);
+    for (int k = k0; k <= k1; ++k) {
+      const zd tau1 = rot[2 * (k - k0)];
+      const zd v2 = rot[2 * (k - k0) + 1];
+      const double tau2 = (tau1 * v2).x;
+      zd b = ZZ(r, k + 1);
+      if (has_ph && k + 1 >= k0 + 2) b = b * cph;
+      const zd ss = tau1 * carry + tau2 * b;
+      zd znew = carry - ss;
+      if (has_ph && k == k0) znew = znew * cph;
+      ZZ(r, k) = znew;
+      carry = b - ss * zconj(v2);
+    }
+    if (has_tl) carry = carry * tl;
+    ZZ(r, k1 + 1) = carry;
+  }
+}
+
+// ------------------------------------------------------------------------
+// singleShiftQR! (GenericSchur.jl:374-504) on H with deferred Z.
+template <int NT> GS_DEV void z_sweep(zd* H, int64_t ld, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot, zd shift,
+                                      int istart, int iend, int64_t& nstep) {
+  const int tid = threadIdx.x;
+  const double ulp = GS_EPS;
+  int istart1 = -1;
+  zd v0 = zr(0.0), v1 = zr(0.0);
+  bool flag = false;
+  for (int mm = iend - 1; mm >= istart + 1; --mm) {
+    const zd h11 = HH(mm, mm);
+    const zd h22 = HH(mm + 1, mm + 1);
+    zd h11s = h11 - shift;
+    double h21 = HH(mm + 1, mm).x;
+    const double s = zabs1(h11s) + fabs(h21);
+    h11s = h11s / s;
+    h21 = h21 / s;
+    v0 = h11s;
+    v1 = zr(h21);
+    const double h10 = HH(mm, mm - 1).x;
+    if (fabs(h10) * fabs(h21) <= ulp * (zabs1(h11s) * (zabs1(h11) + zabs1(h22)))) {
+      istart1 = mm;
+      flag = true;
+      break;
+    }
+  }
+  if (!flag) {
+    istart1 = istart;
+    const zd h11 = HH(istart, istart);
+    zd h11s = h11 - shift;
+    double h21 = HH(istart + 1, istart).x;
+    const double s = zabs1(h11s) + fabs(h21);
+    h11s = h11s / s;
+    h21 = h21 / s;
+    v0 = h11s;
+    v1 = zr(h21);
+  }
+  bool has_ph = false;
+  zd ph = zr(1.0);
+  for (int k = istart1; k <= iend - 1; ++k) {
+    if (k > istart1) {
+      v0 = HH(k, k - 1);
+      v1 = HH(k + 1, k - 1);
+    }
+    const zd tau1 = zreflector2(v0, v1);
+    bsync<NT>();
+    if (k > istart1 && tid == 0) {
+      HH(k, k - 1) = v0;
+      HH(k + 1, k - 1) = zr(0.0);
+    }
+    const zd v2 = v1;
+    const double tau2 = (tau1 * v2).x;
+    const zd ct1 = zconj(tau1);
+    // left: rows k, k+1 over columns k..n
+    for (int j = k + tid; j <= n; j += NT) {
+      const zd a = HH(k, j), b = HH(k + 1, j);
+      const zd ss = ct1 * a + tau2 * b;
+      HH(k, j) = a - ss;
+      HH(k + 1, j) = b - ss * v2;
+    }
+    __syncthreads();
+    // right: columns k, k+1 over rows 1..min(k+2, iend)
+    const int jm = min(k + 2, iend);
+    const zd cv2 = zconj(v2);
+    for (int j = 1 + tid; j <= jm; j += NT) {
+      const zd a = HH(j, k), b = HH(j, k + 1);
+      const zd ss = tau1 * a + tau2 * b;
+      HH(j, k) = a - ss;
+      HH(j, k + 1) = b - ss * cv2;
+    }
+    if (wantz && tid == 0) {
+      rot[2 * (k - istart1)] = tau1;
+      rot[2 * (k - istart1) + 1] = v2;
+    }
+    __syncthreads();
+    if (k == istart1 && istart1 > istart) {
+      zd t = zr(1.0) - tau1;
+      t = t / zabs(t);
+      has_ph = true;
+      ph = t;
+      const zd ctt = zconj(t);
+      if (tid == 0) {
+        HH(istart1 + 1, istart1) = HH(istart1 + 1, istart1) * ctt;
+        if (istart1 + 2 <= iend) HH(istart1 + 2, istart1 + 1) = HH(istart1 + 2, istart1 + 1) * t;
+      }
+      __syncthreads();
+      // rows j in S right of the diagonal times t, then columns j in S
+      // above the diagonal times conj(t); S = istart1..iend minus istart1+1
+      for (int c = 1 + tid; c <= n; c += NT) {
+        const bool cin = (c >= istart1 && c <= iend && c != istart1 + 1);
+        const int rmax = min(c - 1, iend);
+        for (int r = 1; r <= rmax; ++r) {
+          const bool rin = (r >= istart1 && r != istart1 + 1);
+          if (!rin && !cin) continue;
+          zd x = HH(r, c);
+          if (rin) x = x * t;
+          if (cin) x = x * ctt;
+          HH(r, c) = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  nstep += iend - istart1;
+  // ensure reality of the tail subdiagonal
+  zd t = HH(iend, iend - 1);
+  bool has_tl = false;
+  if (t.y != 0.0) {
+    const double rtv = zabs(t);
+    t = t / rtv;
+    has_tl = true;
+    bsync<NT>();
+    if (tid == 0) HH(iend, iend - 1) = zr(rtv);
+    const zd ctt = zconj(t);
+    for (int c = iend + 1 + tid; c <= n; c += NT) HH(iend, c) = HH(iend, c) * ctt;
+    for (int r = 1 + tid; r <= iend - 1; r += NT) HH(r, iend) = HH(r, iend) * t;
+  }
+  if (wantz) z_apply_chain<NT>(Zg, ldz, n, rot, istart1, iend - 1, has_ph, ph, has_tl, t);
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------
+// gschur!(H::Hessenberg{Complex}, Z) (GenericSchur.jl:194-335).
+// Returns 0, or the positive iteration count when maxiter was exceeded.
+template <int NT> GS_DEV int64_t z_qr(zd* H, int64_t ld, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot,
+                                      int64_t maxiter, int64_t maxinner, int64_t& nsweep, int64_t& nstep) {
+  const int tid = threadIdx.x;
+  const double ulp = GS_EPS;
+  const double smallnum = GS_SAFEMIN * ((double)n / ulp);
+  const double half = 0.5;
+  const double threeq = 0.75;
+  int64_t it = 0;
+  int iend = n;
+  while (iend >= 1) {
+    int istart = 1;
+    for (int64_t its = 0; its <= maxinner; ++its) {
+      it += 1;
+      if (it > maxiter) return it;
+      for (int is = iend - 1; is >= istart; --is) {
+        if (zabs1(HH(is + 1, is)) <= smallnum) {
+          istart = is + 1;
+          break;
+        }
+        double tst = zabs1(HH(is, is)) + zabs1(HH(is + 1, is + 1));
+        if (tst == 0.0) {
+          if (is - 1 >= 1) tst += fabs(HH(is, is - 1).x);
+          if (is + 2 <= n) tst += fabs(HH(is + 2, is + 1).x);
+        }
+        if (fabs(HH(is + 1, is).x) <= ulp * tst) {
+          const double ab = fmax(zabs1(HH(is + 1, is)), zabs1(HH(is, is + 1)));
+          const double ba = fmin(zabs1(HH(is + 1, is)), zabs1(HH(is, is + 1)));
+          const double aa = fmax(zabs1(HH(is + 1, is + 1)), zabs1(HH(is, is) - HH(is + 1, is + 1)));
+          const double bb = fmin(zabs1(HH(is + 1, is + 1)), zabs1(HH(is, is) - HH(is + 1, is + 1)));
+          const double s = aa + ab;
+          if (ba * (ab / s) <= fmax(smallnum, ulp * (bb * (aa / s)))) {
+            istart = is + 1;
+            break;
+          }
+        }
+      }
+      if (istart > 1) {
+        bsync<NT>();
+        if (tid == 0) HH(istart, istart - 1) = zr(0.0);
+        __syncthreads();
+      }
+      if (istart >= iend) {
+        iend -= 1;
+        break;
+      }
+      zd t;
+      if (its % 30 == 10) {
+        const double s = threeq * fabs(HH(istart + 1, istart).x);
+        t = zr(s) + HH(istart, istart);
+      } else if (its % 30 == 20) {
+        const double s = threeq * fabs(HH(iend, iend - 1).x);
+        t = zr(s) + HH(iend, iend);
+      } else {
+        t = HH(iend, iend);
+        const zd u = zsqrt(HH(iend - 1, iend)) * zsqrt(HH(iend, iend - 1));
+        double s = zabs1(u);
+        if (s != 0.0) {
+          const zd x = half * (HH(iend - 1, iend - 1) - t);
+          const double sx = zabs1(x);
+          s = fmax(s, zabs1(x));
+          const zd xs = x / s, us = u / s;
+          zd y = s * zsqrt(xs * xs + us * us);
+          if (sx > 0.0) {
+            const zd xx = x / sx;
+            if (xx.x * y.x + xx.y * y.y < 0.0) y = -y;
+          }
+          t = t - u * (u / (x + y));
+        }
+      }
+      nsweep += 1;
+      z_sweep<NT>(H, ld, n, Zg, ldz, wantz, rot, t, istart, iend, nstep);
+    }
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------------
+// Full driver for one matrix.  H: working array (LDS or HBM) with leading
+// dimension ld; Tg/Zg/Wg: outputs in HBM; tmp: HBM array used to hold H
+// while Q is formed in place (T for the LDS kernel, Z for the HBM kernel).
+template <int NT> GS_DEV void z_driver(const zd* Ag, int64_t lda, zd* H, int64_t ld, int n, zd* Tg, int64_t ldt,
+                                       zd* Zg, int64_t ldz, zd* Wg, zd* tmp, int64_t ldtmp, zd* tau, zd* rot,
+                                       double* red, int flags, int64_t maxiter, int64_t maxinner, int32_t* info,
+                                       int64_t* stats) {
+  const int tid = threadIdx.x;
+  const bool wantz = (flags & 0x01) != 0;
+  const bool scale = (flags & 0x02) != 0;
+  const bool hessin = (flags & 0x04) != 0;
+  const bool checksd = (flags & 0x08) != 0;
+  if (maxiter <= 0) maxiter = 100 * (int64_t)n;
+  if (maxinner <= 0) maxinner = 30 * (int64_t)n;
+  // load A
+  if ((const void*)Ag != (const void*)H) {
+    for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+      H[r + c * ld] = Ag[r + c * lda];
+    }
+  }
+  __syncthreads();
+  if (hessin && checksd) {
+    bool bad = false;
+    for (int j = 1; j <= n - 1; ++j) bad |= (H[j + (int64_t)(j - 1) * ld].y != 0.0);
+    if (bad) {
+      if (tid == 0) *info = -2;
+      return;
+    }
+  }
+  // _scale! (util.jl:14-29): norm(A, Inf) is the largest entry modulus
+  bool scaled = false;
+  double cscale = 1.0, anrm = 0.0;
+  if (scale) {
+    double m = 0.0;
+    for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+      m = fmax(m, zabs(H[r + c * ld]));
+    }
+    anrm = block_max<NT>(m, red);
+    const double smlnum = sqrt(GS_SAFEMIN) / GS_EPS;
+    const double bignum = 1.0 / smlnum;
+    if (anrm > 0.0 && anrm < smlnum) { scaled = true; cscale = smlnum; }
+    else if (anrm > bignum) { scaled = true; cscale = bignum; }
+    if (scaled) {
+      double mul[4];
+      const int nm = safescale_muls(anrm, cscale, mul);
+      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        zd v = H[r + c * ld];
+        for (int q = 0; q < nm; ++q) v = v * mul[q];
+        H[r + c * ld] = v;
+      }
+      __syncthreads();
+    }
+  }
+  if (!hessin) {
+    z_hessenberg<NT>(H, ld, n, tau, red);
+    if (wantz) {
+      // hold H in tmp, form Q in place, then Q -> Z and H <- tmp
+      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        tmp[r + c * ldtmp] = H[r + c * ld];
+      }
+      __syncthreads();
+      z_form_q_inplace<NT>(H, ld, n, tau);
+      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        const zd q = H[r + c * ld];
+        const zd h = tmp[r + c * ldtmp];
+        Zg[r + c * ldz] = q;
+        H[r + c * ld] = h;
+      }
+      __syncthreads();
+    }
+  }
+  // zero below the subdiagonal (GenericSchur.jl:219-221)
+  for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+    const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+    if (r > c + 1) H[r + c * ld] = zr(0.0);
+  }
+  __syncthreads();
+  int64_t nsweep = 0, nstep = 0;
+  const int64_t st = z_qr<NT>(H, ld, n, Zg, ldz, wantz, rot, maxiter, maxinner, nsweep, nstep);
+  __syncthreads();
+  if (stats && tid == 0) {
+    stats[0] = nsweep;
+    stats[1] = nstep;
+  }
+  if (st != 0) {
+    if (tid == 0) *info = (int32_t)(st > 0x7fffffff ? 0x7fffffff : st);
+    return;
+  }
+  // T = triu(H) (GenericSchur.jl:334), values = diag
+  double umul[4];
+  int nu = 0;
+  if (scaled) nu = safescale_muls(cscale, anrm, umul);
+  for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+    const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+    zd v = (r > c) ? zr(0.0) : H[r + c * ld];
+    for (int q = 0; q < nu; ++q) v = v * umul[q];
+    Tg[r + c * ldt] = v;
+    if (r == c) Wg[r] = v;
+  }
+  if (tid == 0) *info = 0;
+}
+
+// ------------------------------------------------------------------------
+// Batched kernel: matrix per workgroup, H resident in LDS.
+template <int NT> __global__ void __launch_bounds__(NT) zschur_lds_kernel(BatchArgs a) {
+  extern __shared__ double smem[];
+  const int b = blockIdx.x;
+  if (b >= a.count) return;
+  const int m = a.order[a.first + b];
+  const int n = a.n[m];
+  const int64_t ld = (int64_t)(n | 1);
+  zd* H = reinterpret_cast<zd*>(smem);
+  zd* tau = H + ld * n;
+  zd* rot = tau + (n > 0 ? n : 1);
+  double* red = reinterpret_cast<double*>(rot + 2 * (n > 0 ? n : 1));
+  const int64_t o = a.off[m];
+  const zd* A = reinterpret_cast<const zd*>(a.A) + o;
+  zd* T = reinterpret_cast<zd*>(a.T) + o;
+  zd* Z = reinterpret_cast<zd*>(a.Z) + o;
+  zd* W = reinterpret_cast<zd*>(a.W) + a.woff[m];
+  if (n <= 0) {
+    if (threadIdx.x == 0) a.info[m] = (n == 0) ? 0 : -1;
+    return;
+  }
+  z_driver<NT>(A, n, H, ld, n, T, n, Z, n, W, T, n, tau, rot, red, a.flags, a.maxiter, a.maxinner, a.info + m,
+               a.stats ? a.stats + 2 * m : nullptr);
+}
+
+// Single-matrix kernel: H in HBM (the T output buffer), any n.
+template <int NT> __global__ void __launch_bounds__(NT) zschur_global_kernel(SingleArgs a) {
+  __shared__ double red[NT / 64 + 1];
+  const int n = (int)a.n;
+  zd* A = reinterpret_cast<zd*>(a.A);
+  zd* T = reinterpret_cast<zd*>(a.T);
+  zd* Z = reinterpret_cast<zd*>(a.Z);
+  zd* W = reinterpret_cast<zd*>(a.W);
+  zd* tau = reinterpret_cast<zd*>(a.ws);
+  zd* rot = tau + n;
+  // copy A into T, which then serves as the working array
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+    const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+    T[r + c * a.ldt] = A[r + c * a.lda];
+  }
+  __syncthreads();
+  z_driver<NT>(T, a.ldt, T, a.ldt, n, T, a.ldt, Z, a.ldz, W, Z, a.ldz, tau, rot, red, a.flags, a.maxiter,
+               a.maxinner, a.info, a.stats);
+}
+
+size_t zschur_lds_bytes(int n, int nt) {
+  const size_t ld = (size_t)(n | 1);
+  return 16 * (ld * n + 3 * (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
+}
+
+int zschur_max_lds_n() {
+  // largest n whose footprint fits the 160 KiB of LDS of a gfx950 CU
+  int n = 1;
+  while (zschur_lds_bytes(n + 1, 256) <= 160 * 1024) ++n;
+  return n;
+}
+
+hipError_t launch_zschur_lds(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
+  const size_t lds = zschur_lds_bytes(nmax, nt);
+  if (a.count <= 0) return hipSuccess;
+  switch (nt) {
+    case 64:
+      (void)hipFuncSetAttribute((const void*)zschur_lds_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(zschur_lds_kernel<64>, dim3(a.count), dim3(64), lds, s, a);
+      break;
+    case 128:
+      (void)hipFuncSetAttribute((const void*)zschur_lds_kernel<128>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(zschur_lds_kernel<128>, dim3(a.count), dim3(128), lds, s, a);
+      break;
+    case 256:
+      (void)hipFuncSetAttribute((const void*)zschur_lds_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(zschur_lds_kernel<256>, dim3(a.count), dim3(256), lds, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_zschur_global(const SingleArgs& a, int nt, hipStream_t s) {
+  switch (nt) {
+    case 64:
+      hipLaunchKernelGGL(zschur_global_kernel<64>, dim3(1), dim3(64), 0, s, a);
+      break;
+    case 256:
+      hipLaunchKernelGGL(zschur_global_kernel<256>, dim3(1), dim3(256), 0, s, a);
+      break;
+    case 1024:
+      hipLaunchKernelGGL(zschur_global_kernel<1024>, dim3(1), dim3(1024), 0, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace gs

@@ -1,0 +1,18 @@
+"""genericschur — GenericSchur.jl's Schur hot path on AMD Instinct MI355X.
+
+Complex and real Schur decompositions (Householder Hessenberg reduction,
+Francis QR with deflation and Schur-vector accumulation), eigenvectors of a
+Schur factorization and an eigenvalue vector-Jacobian product, computed by
+hand-written HIP kernels for gfx950 behind the C ABI in
+include/gschur_mi355x.h.
+"""
+from .api import (ArgumentError, DimensionMismatch, Schur, UnconvergedException, eigvals_, eigvecs, geigvecs,
+                  gschur, gschur_, gschur_batched, gschur_batched_device, gschur_hessenberg_, pack, schur_,
+                  synchronize, unpack, eig_offsets)
+from ._lib import GSError, last_stats
+
+__all__ = [
+    "ArgumentError", "DimensionMismatch", "Schur", "UnconvergedException", "GSError",
+    "gschur", "gschur_", "schur_", "gschur_hessenberg_", "eigvals_", "eigvecs", "geigvecs",
+    "gschur_batched", "gschur_batched_device", "pack", "unpack", "eig_offsets", "synchronize", "last_stats",
+]

@@ -1,0 +1,114 @@
+"""ctypes binding of libgschur_mi355x.so (the C ABI in include/gschur_mi355x.h).
+
+The product path has no CPU fallback: if the library is missing, or no GPU is
+visible when a compute entry point needs one, the call raises.
+"""
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libgschur_mi355x.so"))
+
+GS_WANTZ = 0x01
+GS_SCALE = 0x02
+GS_HESSENBERG = 0x04
+GS_CHECK_SUBDIAG = 0x08
+GS_DEVICE_PTRS = 0x10
+GS_ASYNC = 0x20
+GS_PLAIN = 0x40
+
+GS_OK = 0
+GS_E_DIM = -1
+GS_E_ARG = -3
+GS_E_UNSUPPORTED = -4
+GS_E_HIP = -100
+
+# every symbol the header declares
+EXPORTS = (
+    "gs_version", "gs_create", "gs_destroy", "gs_last_error", "gs_synchronize",
+    "gs_schur_z", "gs_schur_d", "gs_schur_batched_z", "gs_schur_batched_d", "gs_batched_max_n",
+    "gs_trevc_z", "gs_trevc_d", "gs_trevc_batched_z", "gs_trevc_batched_d",
+    "gs_eig_vjp_batched_d", "gs_last_stats",
+)
+
+_lib = None
+_lock = threading.RLock()
+_handles = {}
+
+c_i64 = ctypes.c_int64
+c_int = ctypes.c_int
+c_vp = ctypes.c_void_p
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class GSError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the library (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIBPATH):
+            raise GSError(f"HIP library not built: {LIBPATH} is missing (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIBPATH)
+        L.gs_version.restype = c_int
+        L.gs_create.argtypes = [ctypes.POINTER(c_vp), c_int]
+        L.gs_destroy.argtypes = [c_vp]
+        L.gs_last_error.argtypes = [c_vp]
+        L.gs_last_error.restype = ctypes.c_char_p
+        L.gs_synchronize.argtypes = [c_vp]
+        sig_single = [c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_int, c_i64, c_i64, c_i32p]
+        L.gs_schur_z.argtypes = sig_single
+        L.gs_schur_d.argtypes = sig_single
+        sig_b = [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int]
+        L.gs_schur_batched_z.argtypes = sig_b
+        L.gs_schur_batched_d.argtypes = sig_b
+        L.gs_batched_max_n.argtypes = [c_int]
+        sig_tv = [c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_int, c_int, c_int]
+        L.gs_trevc_z.argtypes = sig_tv
+        L.gs_trevc_d.argtypes = sig_tv
+        sig_tvb = [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int]
+        L.gs_trevc_batched_z.argtypes = sig_tvb
+        L.gs_trevc_batched_d.argtypes = sig_tvb
+        L.gs_eig_vjp_batched_d.argtypes = [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int]
+        L.gs_last_stats.argtypes = [c_vp, c_i64p, c_int]
+        _lib = L
+        return _lib
+
+
+def handle(device=0):
+    """The process-wide handle for `device` (created on first use)."""
+    h = _handles.get(device)
+    if h is not None:
+        return h
+    with _lock:
+        h = _handles.get(device)
+        if h is None:
+            hp = c_vp()
+            rc = lib().gs_create(ctypes.byref(hp), int(device))
+            if rc != GS_OK:
+                raise GSError(f"gs_create(device={device}) failed with status {rc}: no usable GPU")
+            h = hp
+            _handles[device] = h
+    return h
+
+
+def check(rc, h, what):
+    if rc != GS_OK:
+        msg = lib().gs_last_error(h)
+        msg = msg.decode() if msg else ""
+        raise GSError(f"{what} failed with status {rc}: {msg}")
+
+
+def last_stats(device=0):
+    """(sweeps, bulge steps, kernel microseconds) of the last Schur call."""
+    out = (ctypes.c_int64 * 3)()
+    k = lib().gs_last_stats(handle(device), out, 3)
+    return tuple(int(out[i]) for i in range(k))

@@ -1,0 +1,295 @@
+"""GenericSchur-style API over the MI355X library.
+
+Mirrors the entry points of GenericSchur.jl that the Schur hot path serves:
+
+==========================  ====================================================
+this module                 reference
+==========================  ====================================================
+``gschur_(A, ...)``         ``gschur!(A; wantZ, scale, maxiter, maxinner)``
+                            (GenericSchur.jl:350-372 complex, :805-835 real)
+``gschur(A, ...)``          ``gschur(A)`` (GenericSchur.jl:343)
+``schur_(A, ...)``          ``LinearAlgebra.schur!(A)`` pirate (pirates.jl:8-10)
+``gschur_hessenberg_(H,Z)`` ``gschur!(H::Hessenberg, Z)`` (GenericSchur.jl:194,513)
+``eigvals_(A, sortby)``     ``LinearAlgebra.eigvals!(A)`` pirate (pirates.jl:17-27)
+``geigvecs(S, left)``       ``geigvecs(S; left)`` (vectors.jl:12-20)
+``eigvecs(S, left)``        ``eigvecs(S::Schur; left)`` pirate (pirates.jl:38-48)
+``Schur``                   ``LinearAlgebra.Schur`` (fields T, Z, values)
+``UnconvergedException``    ``GenericSchur.UnconvergedException``
+``DimensionMismatch``       ``Base.DimensionMismatch``
+``ArgumentError``           ``Base.ArgumentError``
+==========================  ====================================================
+
+Matrices are numpy arrays indexed ``A[i, j]`` as in Julia; Float64 input
+takes the real double-shift path, ComplexF64 input the complex single-shift
+path.  Batched entry points take lists of matrices (host) or packed device
+tensors.  Every computation runs in the HIP library; nothing here computes a
+decomposition on the CPU.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (GS_CHECK_SUBDIAG, GS_DEVICE_PTRS, GS_HESSENBERG, GS_PLAIN, GS_SCALE, GS_WANTZ, GSError)
+
+
+class UnconvergedException(Exception):
+    """Iteration limit reached (GenericSchur.jl:39-45)."""
+
+    def __init__(self, msg):
+        super().__init__("Convergence failure; " + msg)
+        self.msg = msg
+
+
+class DimensionMismatch(ValueError):
+    pass
+
+
+class ArgumentError(ValueError):
+    pass
+
+
+class Schur:
+    """Schur factorization A = Z T Z^H (LinearAlgebra.Schur: T, Z, values)."""
+
+    __slots__ = ("T", "Z", "values")
+
+    def __init__(self, T, Z, values):
+        self.T = T
+        self.Z = Z
+        self.values = values
+
+    @property
+    def vectors(self):
+        return self.Z
+
+    @property
+    def Schur(self):
+        return self.T
+
+    def __iter__(self):
+        return iter((self.T, self.Z, self.values))
+
+    def __repr__(self):
+        return f"Schur(T={self.T.shape} {self.T.dtype}, Z={self.Z.shape}, values={self.values.shape})"
+
+
+def _checksquare(A):
+    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+        raise DimensionMismatch(f"matrix is not square: dimensions are {A.shape}")
+    return A.shape[0]
+
+
+def _kind(A):
+    if A.dtype == np.complex128:
+        return True
+    if A.dtype == np.float64:
+        return False
+    raise TypeError(f"element type {A.dtype} is not served by this library (Float64 / ComplexF64 only)")
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _raise_info(info, maxiter, n):
+    if info == 0:
+        return
+    if info > 0:
+        mi = 100 * n if maxiter is None else maxiter
+        raise UnconvergedException(f"iteration limit {mi} reached")
+    if info == -1:
+        raise DimensionMismatch("bad dimension")
+    if info == -2:
+        raise ArgumentError("algorithm assumes real subdiagonal")
+    raise GSError(f"info = {info}")
+
+
+def _single(A, wantZ, scale, maxiter, maxinner, hess=False, Z0=None, checksd=False, plain=False, device=0):
+    cplx = _kind(A)
+    n = A.shape[0]
+    dt = A.dtype
+    a = np.array(A, dtype=dt, order="F", copy=True)
+    T = np.zeros((n, n), dtype=dt, order="F")
+    Z = np.zeros((n, n), dtype=dt, order="F") if Z0 is None else np.array(Z0, dtype=dt, order="F", copy=True)
+    W = np.zeros(n, dtype=np.complex128)
+    info = ctypes.c_int32(0)
+    flags = (GS_WANTZ if wantZ else 0) | (GS_SCALE if scale else 0)
+    if hess:
+        flags |= GS_HESSENBERG
+        if checksd:
+            flags |= GS_CHECK_SUBDIAG
+    if plain:
+        flags |= GS_PLAIN
+    h = _lib.handle(device)
+    f = _lib.lib().gs_schur_z if cplx else _lib.lib().gs_schur_d
+    ld = max(n, 1)
+    rc = f(h, n, _ptr(a), ld, _ptr(T), ld, _ptr(Z), ld, _ptr(W), flags,
+           0 if maxiter is None else int(maxiter), 0 if maxinner is None else int(maxinner), ctypes.byref(info))
+    if rc == _lib.GS_E_DIM:
+        raise DimensionMismatch("bad dimension")
+    _lib.check(rc, h, "gs_schur")
+    _raise_info(info.value, maxiter, n)
+    T = np.ascontiguousarray(T)
+    Zo = np.ascontiguousarray(Z) if wantZ else np.zeros((0, 0), dtype=dt)
+    return Schur(T, Zo, W)
+
+
+def gschur_(A, wantZ=True, scale=True, maxiter=None, maxinner=None, plain=False, device=0):
+    """gschur!(A) -> Schur (GenericSchur.jl:350-372 / :805-835)."""
+    if _is_torch(A):
+        raise TypeError("use gschur_batched_device for device tensors")
+    A = np.asarray(A)
+    _checksquare(A)
+    return _single(A, wantZ, scale, maxiter, maxinner, plain=plain, device=device)
+
+
+def gschur(A, **kw):
+    """gschur(A) = gschur!(copy(A)) (GenericSchur.jl:343)."""
+    return gschur_(np.array(A, copy=True), **kw)
+
+
+schur_ = gschur_
+
+
+def gschur_hessenberg_(H, Z=None, maxiter=None, maxinner=None, checksd=True, device=0):
+    """gschur!(H::Hessenberg, Z) (GenericSchur.jl:194-335 / :513-699).
+
+    H is upper Hessenberg (entries below the subdiagonal are ignored); Z, if
+    given, is the n x n transformation to update.  Returns Schur.
+    """
+    H = np.asarray(H)
+    n = _checksquare(H)
+    if Z is not None:
+        Z = np.asarray(Z)
+        if Z.ndim != 2 or Z.shape[1] != n or Z.shape[0] != n:
+            raise DimensionMismatch("second dimension of Z must match H")
+        if Z.dtype != H.dtype:
+            Z = Z.astype(H.dtype)
+    cplx = _kind(H)
+    return _single(H, Z is not None, False, maxiter, maxinner, hess=True, Z0=Z, checksd=(checksd and cplx),
+                   device=device)
+
+
+def eigvals_(A, sortby="eigsortby", **kw):
+    """eigvals!(A) (pirates.jl:17-27): gschur!(A; wantZ=false), sorted by (re, im)."""
+    S = gschur_(A, wantZ=False, **kw)
+    v = S.values
+    if sortby is None:
+        return v
+    if sortby == "eigsortby":
+        return v[np.lexsort((v.imag, v.real))]
+    return np.array(sorted(v, key=sortby))
+
+
+def geigvecs(S, left=False, normalize=True, device=0):
+    """geigvecs(S; left) (vectors.jl:12-20): eigenvectors as columns, ordered
+    like S.values, unit 2-norm with the largest component real."""
+    T = np.asarray(S.T)
+    n = _checksquare(T)
+    cplx = _kind(T)
+    t = np.array(T, dtype=T.dtype, order="F", copy=True)
+    Z = np.asarray(S.Z)
+    z = None if Z.size == 0 else np.array(Z, dtype=T.dtype, order="F", copy=True)
+    V = np.zeros((n, n), dtype=np.complex128, order="F")
+    h = _lib.handle(device)
+    f = _lib.lib().gs_trevc_z if cplx else _lib.lib().gs_trevc_d
+    ld = max(n, 1)
+    rc = f(h, n, _ptr(t), ld, None if z is None else _ptr(z), ld, _ptr(V), ld, int(bool(left)), int(bool(normalize)), 0)
+    _lib.check(rc, h, "gs_trevc")
+    return np.ascontiguousarray(V)
+
+
+def eigvecs(S, left=False, device=0):
+    """eigvecs(S::Schur; left) (pirates.jl:38-48)."""
+    return geigvecs(S, left=left, device=device)
+
+
+# ------------------------------------------------------------------ batched
+def pack(mats, dtype):
+    """Pack square matrices column-major, back to back.  Returns (n, off, flat)."""
+    ns = np.array([m.shape[0] for m in mats], dtype=np.int32)
+    off = np.zeros(len(mats), dtype=np.int64)
+    if len(mats) > 1:
+        off[1:] = np.cumsum(ns[:-1].astype(np.int64) ** 2)
+    total = int(off[-1] + int(ns[-1]) ** 2) if len(mats) else 0
+    flat = np.zeros(total, dtype=dtype)
+    for m, o, k in zip(mats, off, ns):
+        flat[o:o + k * k] = np.asarray(m, dtype=dtype).reshape(-1, order="F")
+    return ns, off, flat
+
+
+def unpack(ns, off, flat):
+    return [flat[o:o + k * k].reshape((k, k), order="F") for k, o in zip(ns, off)]
+
+
+def eig_offsets(ns):
+    w = np.zeros(len(ns), dtype=np.int64)
+    if len(ns) > 1:
+        w[1:] = np.cumsum(np.asarray(ns[:-1], dtype=np.int64))
+    return w
+
+
+def gschur_batched(mats, wantZ=True, scale=True, device=0, raise_on_info=True):
+    """Batched gschur! of host matrices (all Float64 or all ComplexF64) in one
+    graph launch.  Returns a list of Schur (and the info array)."""
+    mats = [np.asarray(m) for m in mats]
+    if not mats:
+        return [], np.zeros(0, dtype=np.int32)
+    for m in mats:
+        _checksquare(m)
+    cplx = _kind(mats[0])
+    if any(_kind(m) != cplx for m in mats):
+        raise TypeError("mixed element types in one batch")
+    dt = np.complex128 if cplx else np.float64
+    ns, off, flat = pack(mats, dt)
+    T = np.zeros_like(flat)
+    Z = np.zeros_like(flat)
+    W = np.zeros(int(ns.sum()), dtype=np.complex128)
+    info = np.zeros(len(mats), dtype=np.int32)
+    flags = (GS_WANTZ if wantZ else 0) | (GS_SCALE if scale else 0)
+    h = _lib.handle(device)
+    f = _lib.lib().gs_schur_batched_z if cplx else _lib.lib().gs_schur_batched_d
+    rc = f(h, len(mats), _ptr(ns), _ptr(off), _ptr(flat), _ptr(T), _ptr(Z), _ptr(W), _ptr(info), flags)
+    if rc == _lib.GS_E_DIM:
+        raise DimensionMismatch("matrix order outside the batched range")
+    _lib.check(rc, h, "gs_schur_batched")
+    woff = eig_offsets(ns)
+    out = []
+    for i, (k, o) in enumerate(zip(ns, off)):
+        if raise_on_info:
+            _raise_info(int(info[i]), None, int(k))
+        Ti = T[o:o + k * k].reshape((k, k), order="F").copy()
+        Zi = Z[o:o + k * k].reshape((k, k), order="F").copy() if wantZ else np.zeros((0, 0), dtype=dt)
+        out.append(Schur(Ti, Zi, W[woff[i]:woff[i] + k].copy()))
+    return out, info
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def gschur_batched_device(ns, off, A, T, Z, W, info, complex_input, wantZ=True, scale=True, asynchronous=False,
+                          device=0):
+    """Batched gschur! on device-resident packed data (torch CUDA tensors or
+    raw device addresses).  ns/off are host int32/int64 arrays."""
+    ns = np.ascontiguousarray(ns, dtype=np.int32)
+    off = np.ascontiguousarray(off, dtype=np.int64)
+
+    def addr(x):
+        if x is None:
+            return None
+        return ctypes.c_void_p(x.data_ptr() if _is_torch(x) else int(x))
+
+    flags = (GS_WANTZ if wantZ else 0) | (GS_SCALE if scale else 0) | GS_DEVICE_PTRS
+    if asynchronous:
+        flags |= _lib.GS_ASYNC
+    h = _lib.handle(device)
+    f = _lib.lib().gs_schur_batched_z if complex_input else _lib.lib().gs_schur_batched_d
+    rc = f(h, len(ns), _ptr(ns), _ptr(off), addr(A), addr(T), addr(Z), addr(W), addr(info), flags)
+    _lib.check(rc, h, "gs_schur_batched")
+
+
+def synchronize(device=0):
+    h = _lib.handle(device)
+    _lib.check(_lib.lib().gs_synchronize(h), h, "gs_synchronize")

@@ -1,0 +1,144 @@
+/*
+ * gschur_mi355x.h — C ABI of the MI355X (gfx950) Schur library.
+ *
+ * The library computes the Schur decomposition A = Z T Z^H of square
+ * Float64 / ComplexF64 matrices the way GenericSchur.jl's `gschur!` does —
+ * Householder Hessenberg reduction, Francis QR sweeps with deflation and
+ * accumulation of Z (single shift for complex input, double shift to the
+ * standard-form quasi-triangular T for real input) — plus the eigenvector
+ * back-substitution behind `eigvecs`/`geigvecs` and an eigenvalue
+ * vector-Jacobian product.  All work runs in hand-written HIP kernels.
+ *
+ * Conventions
+ *   - Matrices are column-major; complex data are interleaved (re, im)
+ *     doubles, i.e. Julia ComplexF64 / C `double _Complex`.  `double*`
+ *     arguments of complex routines point at the first real part.
+ *   - Eigenvalues are always complex (interleaved pairs), in the diagonal
+ *     order of T, as `Schur.values` in the reference.
+ *   - Without GS_DEVICE_PTRS every data pointer is host memory and the call
+ *     stages it through device buffers; with it every data pointer is device
+ *     memory owned by the caller.  Size/offset arrays are always host memory.
+ *   - Calls are synchronous unless GS_ASYNC is set (device pointers only).
+ *   - One handle serialises its calls on its own HIP stream; distinct
+ *     handles are independent and may be used from different threads.
+ *
+ * Return value of every call: GS_OK or a negative status (GS_E_*).
+ * Per-matrix `info`: 0 = converged, > 0 = iteration limit reached
+ * (UnconvergedException), -1 = bad dimension (DimensionMismatch),
+ * -2 = non-real subdiagonal in Hessenberg input (ArgumentError).
+ */
+#ifndef GSCHUR_MI355X_H
+#define GSCHUR_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gs_handle gs_handle;
+
+/* flags */
+#define GS_WANTZ 0x01        /* accumulate Schur vectors (wantZ = true)        */
+#define GS_SCALE 0x02        /* over/underflow scaling of A (scale = true)     */
+#define GS_HESSENBERG 0x04   /* A is upper Hessenberg; with GS_WANTZ, Z holds  */
+                             /* the transformation to update (gschur!(H, Z))   */
+#define GS_CHECK_SUBDIAG 0x08 /* with GS_HESSENBERG, complex: reject non-real  */
+                              /* subdiagonals (checksd = true)                 */
+#define GS_DEVICE_PTRS 0x10  /* data pointers are device memory                */
+#define GS_ASYNC 0x20        /* do not synchronise before returning            */
+#define GS_PLAIN 0x40        /* single matrix: force the unblocked global-     */
+                             /* memory kernel (the plain GPU form)             */
+
+/* status codes */
+#define GS_OK 0
+#define GS_E_ARG (-3)
+#define GS_E_DIM (-1)
+#define GS_E_UNSUPPORTED (-4)
+#define GS_E_HIP (-100) /* -100 - hipError_t */
+
+/* info codes */
+#define GS_INFO_DIM (-1)
+#define GS_INFO_NONREAL_SUBDIAG (-2)
+
+int gs_version(void);
+
+/* Context: owns the device, stream, staging buffers, size-bin tables and
+ * captured graphs.  No counterpart in the reference (stateless Julia). */
+int gs_create(gs_handle** h, int device);
+int gs_destroy(gs_handle* h);
+const char* gs_last_error(const gs_handle* h);
+int gs_synchronize(gs_handle* h);
+
+/* Single matrix, ComplexF64.
+ * Replaces gschur!(A::StridedMatrix{Complex{T}}; wantZ, scale, maxiter,
+ * maxinner) — GenericSchur.jl:350-372, with the QR driver of :194-335 and the
+ * sweep of :374-504; with GS_HESSENBERG, gschur!(H::Hessenberg, Z) of
+ * :194-335.  A is n x n (lda >= n) and is overwritten (destroyed), as in the
+ * reference.  T (ldt), Z (ldz, referenced with GS_WANTZ) receive the factors,
+ * W (2n doubles) the eigenvalues diag(T).  maxiter <= 0 selects 100 n,
+ * maxinner <= 0 selects 30 n. */
+int gs_schur_z(gs_handle* h, int64_t n, double* A, int64_t lda, double* T, int64_t ldt, double* Z, int64_t ldz,
+               double* W, int flags, int64_t maxiter, int64_t maxinner, int32_t* info);
+
+/* Single matrix, Float64.
+ * Replaces gschur!(A::StridedMatrix{<:AbstractFloat}; wantZ, scale, maxiter)
+ * — GenericSchur.jl:805-835 with the QR driver :513-699, the double-shift
+ * sweep :837-952 and the 2x2 standardisation _gs2x2! :716-803.  T is the
+ * standard-form quasi-triangular factor; W (2n doubles) holds the
+ * eigenvalues by position with the +imag member of a pair first.
+ * maxinner is ignored (the reference's real driver has none). */
+int gs_schur_d(gs_handle* h, int64_t n, double* A, int64_t lda, double* T, int64_t ldt, double* Z, int64_t ldz,
+               double* W, int flags, int64_t maxiter, int64_t maxinner, int32_t* info);
+
+/* Batched: `count` matrices of orders n[i], packed column-major (ld = n[i])
+ * at element offsets off[i] in A, T and Z (complex elements for _z).  W holds
+ * the eigenvalues of matrix i at complex offset sum_{j<i} n[j].  info[i] per
+ * matrix.  A is read only.  One call is one graph launch of the size-binned
+ * LDS-resident kernels; n[i] must not exceed gs_batched_max_n(). */
+int gs_schur_batched_z(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off, const double* A,
+                       double* T, double* Z, double* W, int32_t* info, int flags);
+int gs_schur_batched_d(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off, const double* A,
+                       double* T, double* Z, double* W, int32_t* info, int flags);
+int gs_batched_max_n(int is_complex);
+
+/* Eigenvectors of a Schur factorization (T, Z), columns in the order of the
+ * eigenvalues, normalised to unit 2-norm with the largest component real
+ * (geigvecs(S; left) — vectors.jl:12-20 with _enormalize! util.jl:572-592;
+ * complex right/left: vectors.jl:45-131 / :372-460 with _usolve!/_cusolve!
+ * util.jl:128-461; real right/left: vectors.jl:133-362 / :462-698 with
+ * _xsolve/_safe_lu_solve! util.jl:603-751).  Z may be NULL (vectors of T).
+ * V is n x n complex (ldv).  normalize = 0 returns the max-abs1-scaled
+ * vectors before _enormalize!. */
+int gs_trevc_z(gs_handle* h, int64_t n, const double* T, int64_t ldt, const double* Z, int64_t ldz, double* V,
+               int64_t ldv, int left, int normalize, int flags);
+int gs_trevc_d(gs_handle* h, int64_t n, const double* T, int64_t ldt, const double* Z, int64_t ldz, double* V,
+               int64_t ldv, int left, int normalize, int flags);
+/* Batched eigenvectors: T, Z at off[i] (complex elements for _z, real for
+ * _d); V (complex) at complex offset off[i]. */
+int gs_trevc_batched_z(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off, const double* T,
+                       const double* Z, double* V, int left, int normalize, int flags);
+int gs_trevc_batched_d(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off, const double* T,
+                       const double* Z, double* V, int left, int normalize, int flags);
+
+/* Eigenvalue vector-Jacobian product (no counterpart in the reference; the
+ * backward of the north-star rrule).  For the Schur factorizations (T, Z)
+ * of real matrices A_i and cotangents gbar_i = dL/dRe(lambda) + i dL/dIm(lambda)
+ * per eigenvalue (complex, at the eigenvalue offsets of gs_schur_batched_d),
+ * writes Abar_i = Re( X^{-H} diag(gbar) X^H ) with X the right eigenvectors,
+ * computed from right and left eigenvectors recomputed from T and Z:
+ * Abar = sum_k conj(gbar_k) ... see the library documentation.  Abar is real
+ * at off[i].  info[i] = 0, or 1 when some y_k^H x_k vanished. */
+int gs_eig_vjp_batched_d(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off, const double* T,
+                         const double* Z, const double* gbar, double* Abar, int32_t* info, int flags);
+
+/* Statistics of the last Schur call on this handle: out[0] = QR sweeps,
+ * out[1] = bulge steps, out[2] = kernel time in microseconds (events on the
+ * handle's stream, 0 when unavailable).  Returns the number written. */
+int gs_last_stats(const gs_handle* h, int64_t* out, int len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSCHUR_MI355X_H */

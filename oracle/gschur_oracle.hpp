@@ -1,0 +1,1720 @@
+// TEST INFRASTRUCTURE ONLY — the CPU oracle for the Schur hot path.
+//
+// A restatement, in C++ templated on the real type, of GenericSchur.jl's
+// complex and real Schur path and its eigenvector back-substitution.  Only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may call
+// it, and only as the checker.  Each function cites the reference lines it
+// follows.  Indices inside the restated routines are 1-based, column-major,
+// as in the reference, through the accessor `Mat` below.
+#pragma once
+#include <algorithm>
+#include <vector>
+
+#include "scalar.hpp"
+
+namespace orc {
+
+enum {
+  ORC_OK = 0,
+  ORC_UNCONVERGED = 1,   // UnconvergedException (GenericSchur.jl:234-236,554-556)
+  ORC_DIM = -1,          // DimensionMismatch
+  ORC_NONREAL_SUB = -2,  // ArgumentError("algorithm assumes real subdiagonal")
+};
+
+template <class T> struct Mat {
+  T* p;
+  long ld;
+  T& operator()(long i, long j) const { return p[(i - 1) + (j - 1) * ld]; }
+};
+
+// ------------------------------------------------------------ util.jl:5-12
+template <class R> R safemin() {
+  R sfmin = rt<R>::fmin();
+  R small = R(1.0) / rt<R>::fmax();
+  if (small >= sfmin) sfmin = small * (R(1.0) + rt<R>::eps());
+  return sfmin;
+}
+
+// ------------------------------------------------- util.jl:506-525 (real)
+template <class R> R norm2(long n, const R* x, long inc) {
+  if (n < 1) return R(0.0);
+  if (n == 1) return rt<R>::abs(x[0]);
+  R scale(0.0), ssq(0.0);
+  for (long i = 0; i < n; ++i) {
+    const R xi = x[i * inc];
+    if (xi != R(0.0)) {
+      R a = rt<R>::abs(xi);
+      if (scale < a) {
+        R q = scale / a;
+        ssq = R(1.0) + ssq * (q * q);
+        scale = a;
+      } else {
+        R q = a / scale;
+        ssq = ssq + q * q;
+      }
+    }
+  }
+  return scale * rt<R>::sqrt(ssq);
+}
+
+// ---------------------------------------------- util.jl:527-557 (complex)
+template <class R> R norm2(long n, const cx<R>* x, long inc) {
+  if (n < 1) return R(0.0);
+  if (n == 1) return absv(x[0]);
+  R scale(0.0), ssq(0.0);
+  for (long i = 0; i < n; ++i) {
+    const R parts[2] = {x[i * inc].re, x[i * inc].im};
+    for (int p = 0; p < 2; ++p) {
+      if (parts[p] != R(0.0)) {
+        R a = rt<R>::abs(parts[p]);
+        if (scale < a) {
+          R q = scale / a;
+          ssq = R(1.0) + ssq * (q * q);
+          scale = a;
+        } else {
+          R q = a / scale;
+          ssq = ssq + q * q;
+        }
+      }
+    }
+  }
+  return scale * rt<R>::sqrt(ssq);
+}
+
+// ---------------------------------------------------------- util.jl:562-570
+template <class R> R hypot3(R x, R y, R z) {
+  R xa = rt<R>::abs(x), ya = rt<R>::abs(y), za = rt<R>::abs(z);
+  R w = rmax3(xa, ya, za);
+  R rw = R(1.0) / w;
+  R a = rw * xa, b = rw * ya, c = rw * za;
+  return w * rt<R>::sqrt(a * a + b * b + c * c);
+}
+
+// --------------------------------------- householder.jl:12-54 (real xLARFG)
+template <class R> R reflector(long n, R* x, long inc) {
+  if (n <= 1) return R(0.0);
+  const R sfmin = R(2.0) * rt<R>::fmin() / rt<R>::eps();
+  R alpha = x[0];
+  R xnorm = norm2(n - 1, x + inc, inc);
+  if (xnorm == R(0.0)) return R(0.0);
+  R beta = -rt<R>::copysign(rt<R>::hypot(alpha, xnorm), alpha);
+  int kount = 0;
+  bool smallb = rt<R>::abs(beta) < sfmin;
+  if (smallb) {
+    const R rsfmin = R(1.0) / sfmin;
+    while (smallb) {
+      kount += 1;
+      for (long j = 1; j < n; ++j) x[j * inc] = x[j * inc] * rsfmin;
+      beta = beta * rsfmin;
+      alpha = alpha * rsfmin;
+      smallb = (rt<R>::abs(beta) < sfmin) && (kount < 20);
+    }
+    xnorm = norm2(n - 1, x + inc, inc);
+    beta = -rt<R>::copysign(rt<R>::hypot(alpha, xnorm), alpha);
+  }
+  R tau = (beta - alpha) / beta;
+  R t = R(1.0) / (alpha - beta);
+  for (long j = 1; j < n; ++j) x[j * inc] = x[j * inc] * t;
+  for (int j = 0; j < kount; ++j) beta = beta * sfmin;
+  x[0] = beta;
+  return tau;
+}
+
+// ------------------------------------- householder.jl:56-102 (complex xLARFG)
+template <class R> cx<R> reflector(long n, cx<R>* x, long inc) {
+  if (n < 1) return cx<R>();
+  const R sfmin = rt<R>::fmin() / rt<R>::eps();
+  cx<R> alpha = x[0];
+  R ar = alpha.re, ai = alpha.im;
+  R xnorm = norm2(n - 1, x + inc, inc);
+  if (xnorm == R(0.0) && ai == R(0.0)) return cx<R>();
+  R beta = -rt<R>::copysign(hypot3(ar, ai, xnorm), ar);
+  int kount = 0;
+  bool smallb = rt<R>::abs(beta) < sfmin;
+  if (smallb) {
+    const R rsfmin = R(1.0) / sfmin;
+    while (smallb) {
+      kount += 1;
+      for (long j = 1; j < n; ++j) x[j * inc] = x[j * inc] * rsfmin;
+      beta = beta * rsfmin;
+      ar = ar * rsfmin;
+      ai = ai * rsfmin;
+      smallb = (rt<R>::abs(beta) < sfmin) && (kount < 20);
+    }
+    xnorm = norm2(n - 1, x + inc, inc);
+    alpha = cx<R>(ar, ai);
+    beta = -rt<R>::copysign(hypot3(ar, ai, xnorm), ar);
+  }
+  cx<R> tau((beta - ar) / beta, -ai / beta);
+  cx<R> t = cx<R>(R(1.0)) / (alpha - cx<R>(beta));
+  for (long j = 1; j < n; ++j) x[j * inc] = x[j * inc] * t;
+  for (int j = 0; j < kount; ++j) beta = beta * sfmin;
+  x[0] = cx<R>(beta);
+  return tau;
+}
+
+// --------------------- hessenberg.jl:3-17 with householder.jl:140-172 and
+// util.jl:100-114: unblocked Householder reduction A <- Q^H A Q.
+// On exit A holds H on and above the subdiagonal and reflector i in
+// A[i+2:n, i]; tau[i-1] holds its scalar.
+template <class T> void hessenberg(long n, T* a, long lda, T* tau) {
+  Mat<T> A{a, lda};
+  std::vector<T> x(n > 0 ? n : 1);
+  for (long i = 1; i <= n - 1; ++i) {
+    T t = reflector(n - i, &A(i + 1, i), 1);
+    tau[i - 1] = t;
+    const long m = n - i;  // order of the reflector block
+    // lmul!(H', A[i+1:n, i+1:n])   (householder.jl:157-172)
+    for (long j = i + 1; j <= n; ++j) {
+      T va = A(i + 1, j);
+      for (long r = 2; r <= m; ++r) va = va + conj(A(i + r, i)) * A(i + r, j);
+      va = conj(t) * va;
+      A(i + 1, j) = A(i + 1, j) - va;
+      for (long r = 2; r <= m; ++r) A(i + r, j) = A(i + r, j) - va * A(i + r, i);
+    }
+    // rmul!(A[:, i+1:n], H, x)     (householder.jl:140-155)
+    for (long r = 1; r <= n; ++r) {
+      T s = T();
+      for (long c = 2; c <= m; ++c) s = s + A(r, i + c) * A(i + c, i);
+      x[r - 1] = s + A(r, i + 1);
+    }
+    for (long r = 1; r <= n; ++r) A(r, i + 1) = A(r, i + 1) - t * x[r - 1];
+    for (long c = 2; c <= m; ++c) {
+      const T yc = conj(A(i + c, i));
+      for (long r = 1; r <= n; ++r) A(r, i + c) = A(r, i + c) + x[r - 1] * (-t) * yc;
+    }
+  }
+}
+
+// ---------------- hessenberg.jl:150-166: Q = diag(1, H_1 H_2 ... H_{n-1}),
+// formed by backward accumulation of the reflectors onto the identity.
+template <class T> void materialize_q(long n, const T* a, long lda, const T* tau, T* q, long ldq) {
+  Mat<const T> A{a, lda};
+  Mat<T> Q{q, ldq};
+  for (long j = 1; j <= n; ++j)
+    for (long i = 1; i <= n; ++i) Q(i, j) = (i == j) ? T(1.0) : T();
+  for (long i = n - 1; i >= 1; --i) {
+    const T t = tau[i - 1];
+    for (long c = i + 1; c <= n; ++c) {
+      T s = Q(i + 1, c);
+      for (long r = i + 2; r <= n; ++r) s = s + conj(A(r, i)) * Q(r, c);
+      s = t * s;
+      Q(i + 1, c) = Q(i + 1, c) - s;
+      for (long r = i + 2; r <= n; ++r) Q(r, c) = Q(r, c) - s * A(r, i);
+    }
+  }
+}
+
+// ---------------------------------------------------------- util.jl:41-80
+template <class T> void safescale(long len, T* a, typename real_of<T>::type cfrom,
+                                  typename real_of<T>::type cto) {
+  typedef typename real_of<T>::type R;
+  const R smlnum = safemin<R>();
+  const R bignum = R(1.0) / smlnum;
+  R cfromc = cfrom, ctoc = cto, mul(1.0);
+  bool done = false;
+  while (!done) {
+    R cfrom1 = cfromc * smlnum;
+    if (cfrom1 == cfromc) {
+      mul = ctoc / cfromc;
+      done = true;
+    } else {
+      R cto1 = ctoc / bignum;
+      if (cto1 == ctoc) {
+        mul = cto;
+        done = true;
+        cfromc = R(1.0);
+      } else if (rt<R>::abs(cfrom1) > rt<R>::abs(ctoc) && ctoc != R(0.0)) {
+        mul = smlnum;
+        done = false;
+        cfromc = R(1.0);
+      } else if (rt<R>::abs(cto1) > rt<R>::abs(cfromc)) {
+        mul = bignum;
+        done = false;
+        cfromc = cfrom1;
+      } else {
+        mul = ctoc / cfromc;
+        done = true;
+      }
+    }
+    for (long i = 0; i < len; ++i) a[i] = a[i] * mul;
+  }
+}
+
+// ---------------------------------------------------------- util.jl:14-29
+// Returns whether A was scaled; cscale and anrm receive the factors.
+template <class T> bool scale_matrix(long n, T* a, long lda, typename real_of<T>::type& cscale,
+                                     typename real_of<T>::type& anrm) {
+  typedef typename real_of<T>::type R;
+  const R smlnum = rt<R>::sqrt(safemin<R>()) / rt<R>::eps();
+  const R bignum = R(1.0) / smlnum;
+  anrm = R(0.0);  // norm(A, Inf): largest entry modulus
+  for (long j = 0; j < n; ++j)
+    for (long i = 0; i < n; ++i) anrm = rmax(anrm, absv(a[i + j * lda]));
+  cscale = R(1.0);
+  bool scaled = false;
+  if (anrm > R(0.0) && anrm < smlnum) {
+    scaled = true;
+    cscale = smlnum;
+  } else if (anrm > bignum) {
+    scaled = true;
+    cscale = bignum;
+  }
+  if (scaled) {
+    for (long j = 0; j < n; ++j) safescale(n, a + j * lda, anrm, cscale);
+  }
+  return scaled;
+}
+
+// --------------------------------------------- GenericSchur.jl:374-504
+template <class R> void single_shift_qr(long n, Mat<cx<R>> HH, cx<R>* zp, long ldz, cx<R> shift,
+                                        long istart, long iend) {
+  typedef cx<R> T;
+  Mat<T> Z{zp, ldz};
+  const R ulp = rt<R>::eps();
+  long istart1 = -1;
+  T v[2];
+  T h11s;
+  R h21(0.0);
+  bool flag = false;
+  for (long mm = iend - 1; mm >= istart + 1; --mm) {
+    T h11 = HH(mm, mm);
+    T h22 = HH(mm + 1, mm + 1);
+    h11s = h11 - shift;
+    h21 = HH(mm + 1, mm).re;
+    R s = abs1(h11s) + rt<R>::abs(h21);
+    h11s = h11s / s;
+    h21 = h21 / s;
+    v[0] = h11s;
+    v[1] = T(h21);
+    R h10 = HH(mm, mm - 1).re;
+    if (rt<R>::abs(h10) * rt<R>::abs(h21) <= ulp * (abs1(h11s) * (abs1(h11) + abs1(h22)))) {
+      istart1 = mm;
+      flag = true;
+      break;
+    }
+  }
+  if (!flag) {
+    istart1 = istart;
+    T h11 = HH(istart, istart);
+    h11s = h11 - shift;
+    h21 = HH(istart + 1, istart).re;
+    R s = abs1(h11s) + rt<R>::abs(h21);
+    h11s = h11s / s;
+    h21 = h21 / s;
+    v[0] = h11s;
+    v[1] = T(h21);
+  }
+  for (long k = istart1; k <= iend - 1; ++k) {
+    if (k > istart1) {
+      v[0] = HH(k, k - 1);
+      v[1] = HH(k + 1, k - 1);
+    }
+    T tau1 = reflector(2, v, 1);
+    if (k > istart1) {
+      HH(k, k - 1) = v[0];
+      HH(k + 1, k - 1) = T();
+    }
+    T v2 = v[1];
+    R tau2 = (tau1 * v2).re;
+    for (long j = k; j <= n; ++j) {
+      T ss = conj(tau1) * HH(k, j) + tau2 * HH(k + 1, j);
+      HH(k, j) = HH(k, j) - ss;
+      HH(k + 1, j) = HH(k + 1, j) - ss * v2;
+    }
+    const long jmax = std::min(k + 2, iend);
+    for (long j = 1; j <= jmax; ++j) {
+      T ss = tau1 * HH(j, k) + tau2 * HH(j, k + 1);
+      HH(j, k) = HH(j, k) - ss;
+      HH(j, k + 1) = HH(j, k + 1) - ss * conj(v2);
+    }
+    if (zp) {
+      for (long j = 1; j <= n; ++j) {
+        T ss = tau1 * Z(j, k) + tau2 * Z(j, k + 1);
+        Z(j, k) = Z(j, k) - ss;
+        Z(j, k + 1) = Z(j, k + 1) - ss * conj(v2);
+      }
+    }
+    if (k == istart1 && istart1 > istart) {
+      T t = T(R(1.0)) - tau1;
+      t = t / absv(t);
+      HH(istart1 + 1, istart1) = HH(istart1 + 1, istart1) * conj(t);
+      if (istart1 + 2 <= iend) HH(istart1 + 2, istart1 + 1) = HH(istart1 + 2, istart1 + 1) * t;
+      for (long j = istart1; j <= iend; ++j) {
+        if (j != istart1 + 1) {
+          if (n > j)
+            for (long c = j + 1; c <= n; ++c) HH(j, c) = HH(j, c) * t;
+          for (long r = 1; r <= j - 1; ++r) HH(r, j) = HH(r, j) * conj(t);
+          if (zp)
+            for (long r = 1; r <= n; ++r) Z(r, j) = Z(r, j) * conj(t);
+        }
+      }
+    }
+  }
+  T t = HH(iend, iend - 1);
+  if (t.im != R(0.0)) {
+    R rtv = absv(t);
+    HH(iend, iend - 1) = T(rtv);
+    t = t / rtv;
+    if (n > iend)
+      for (long c = iend + 1; c <= n; ++c) HH(iend, c) = HH(iend, c) * conj(t);
+    for (long r = 1; r <= iend - 1; ++r) HH(r, iend) = HH(r, iend) * t;
+    if (zp)
+      for (long r = 1; r <= n; ++r) Z(r, iend) = Z(r, iend) * t;
+  }
+}
+
+// ----------------- GenericSchur.jl:194-335: complex Schur of a Hessenberg
+// matrix (zlahqr logic).  `stats` (optional) receives [iterations, sweeps,
+// bulge steps].
+template <class R> int schur_hess_complex(long n, cx<R>* h, long ldh, cx<R>* zp, long ldz,
+                                          long maxiter, long maxinner, bool checksd, cx<R>* w,
+                                          long* stats = nullptr) {
+  typedef cx<R> T;
+  Mat<T> HH{h, ldh};
+  if (checksd) {
+    for (long j = 1; j <= n - 1; ++j)
+      if (!isreal(HH(j + 1, j))) return ORC_NONREAL_SUB;
+  }
+  const R ulp = rt<R>::eps();
+  const R smallnum = safemin<R>() * (R(double(n)) / ulp);
+  const R half = R(1.0) / R(2.0);
+  const R threeq = R(3.0) / R(4.0);
+  for (long j = 1; j <= n - 1; ++j)
+    for (long i = j + 2; i <= n; ++i) HH(i, j) = T();
+  long it = 0, nsweep = 0, nstep = 0;
+  long iend = n;
+  long istart = 1;
+  while (iend >= 1) {
+    istart = 1;
+    for (long its = 0; its <= maxinner; ++its) {
+      it += 1;
+      if (it > maxiter) {
+        if (stats) { stats[0] = it; stats[1] = nsweep; stats[2] = nstep; }
+        return ORC_UNCONVERGED;
+      }
+      for (long is = iend - 1; is >= istart; --is) {
+        if (abs1(HH(is + 1, is)) <= smallnum) {
+          istart = is + 1;
+          break;
+        }
+        R tst = abs1(HH(is, is)) + abs1(HH(is + 1, is + 1));
+        if (tst == R(0.0)) {
+          if (is - 1 >= 1) tst = tst + rt<R>::abs(HH(is, is - 1).re);
+          if (is + 2 <= n) tst = tst + rt<R>::abs(HH(is + 2, is + 1).re);
+        }
+        if (rt<R>::abs(HH(is + 1, is).re) <= ulp * tst) {
+          R ab = rmax(abs1(HH(is + 1, is)), abs1(HH(is, is + 1)));
+          R ba = rmin(abs1(HH(is + 1, is)), abs1(HH(is, is + 1)));
+          R aa = rmax(abs1(HH(is + 1, is + 1)), abs1(HH(is, is) - HH(is + 1, is + 1)));
+          R bb = rmin(abs1(HH(is + 1, is + 1)), abs1(HH(is, is) - HH(is + 1, is + 1)));
+          R s = aa + ab;
+          if (ba * (ab / s) <= rmax(smallnum, ulp * (bb * (aa / s)))) {
+            istart = is + 1;
+            break;
+          }
+        }
+      }
+      if (istart > 1) HH(istart, istart - 1) = T();
+      if (istart >= iend) {
+        iend -= 1;
+        break;
+      }
+      T t;
+      if (its % 30 == 10) {
+        R s = threeq * rt<R>::abs(HH(istart + 1, istart).re);
+        t = T(s) + HH(istart, istart);
+      } else if (its % 30 == 20) {
+        R s = threeq * rt<R>::abs(HH(iend, iend - 1).re);
+        t = T(s) + HH(iend, iend);
+      } else {
+        t = HH(iend, iend);
+        T u = csqrt(HH(iend - 1, iend)) * csqrt(HH(iend, iend - 1));
+        R s = abs1(u);
+        if (s != R(0.0)) {
+          T x = half * (HH(iend - 1, iend - 1) - t);
+          R sx = abs1(x);
+          s = rmax(s, abs1(x));
+          T xs = x / s, us = u / s;
+          T y = s * csqrt(xs * xs + us * us);
+          if (sx > R(0.0)) {
+            T xx = x / sx;
+            if (xx.re * y.re + xx.im * y.im < R(0.0)) y = -y;
+          }
+          t = t - u * (u / (x + y));
+        }
+      }
+      nsweep += 1;
+      nstep += iend - istart;  // upper bound on the bulge steps of this sweep
+      single_shift_qr<R>(n, HH, zp, ldz, t, istart, iend);
+    }
+  }
+  for (long j = 1; j <= n; ++j) w[j - 1] = HH(j, j);
+  if (stats) { stats[0] = it; stats[1] = nsweep; stats[2] = nstep; }
+  return ORC_OK;
+}
+
+// --------------------------------------- GenericSchur.jl:716-803 (dlanv2)
+// In/out: a,b,c,d.  Out: rotation (cs, sn) and eigenvalues w1, w2.
+template <class R> void gs2x2(R& a, R& b, R& c, R& d, R& cs, R& sn, cx<R>& w1, cx<R>& w2) {
+  auto sgn = [](R x) { return (x < R(0.0)) ? R(-1.0) : R(1.0); };
+  const R half = R(1.0) / R(2.0);
+  const R small = R(4.0) * rt<R>::eps();
+  if (c == R(0.0)) {
+    cs = R(1.0);
+    sn = R(0.0);
+  } else if (b == R(0.0)) {
+    cs = R(0.0);
+    sn = R(1.0);
+    R a0 = a, c0 = c, d0 = d;
+    a = d0;
+    b = -c0;
+    c = R(0.0);
+    d = a0;
+  } else if ((a - d) == R(0.0) && (b * c < R(0.0))) {
+    cs = R(1.0);
+    sn = R(0.0);
+  } else {
+    R asubd = a - d;
+    R p = half * asubd;
+    R bcmax = rmax(rt<R>::abs(b), rt<R>::abs(c));
+    R bcmis = rmin(rt<R>::abs(b), rt<R>::abs(c)) * sgn(b) * sgn(c);
+    R scale = rmax(rt<R>::abs(p), bcmax);
+    R z = (p / scale) * p + (bcmax / scale) * bcmis;
+    if (z >= small) {
+      z = p + rt<R>::sqrt(scale) * rt<R>::sqrt(z) * sgn(p);
+      a = d + z;
+      d = d - (bcmax / z) * bcmis;
+      R tau = rt<R>::hypot(c, z);
+      cs = z / tau;
+      sn = c / tau;
+      b = b - c;
+      c = R(0.0);
+    } else {
+      R sigma = b + c;
+      R tau = rt<R>::hypot(sigma, asubd);
+      cs = rt<R>::sqrt(half * (R(1.0) + rt<R>::abs(sigma) / tau));
+      sn = -(p / (tau * cs)) * sgn(sigma);
+      R aa = a * cs + b * sn;
+      R bb = -a * sn + b * cs;
+      R cc = c * cs + d * sn;
+      R dd = -c * sn + d * cs;
+      a = aa * cs + cc * sn;
+      b = bb * cs + dd * sn;
+      c = -aa * sn + cc * cs;
+      d = -bb * sn + dd * cs;
+      R midad = half * (a + d);
+      a = midad;
+      d = a;
+      if (c != R(0.0)) {
+        if (b != R(0.0)) {
+          if (b * c >= R(0.0)) {
+            R sab = rt<R>::sqrt(rt<R>::abs(b));
+            R sac = rt<R>::sqrt(rt<R>::abs(c));
+            p = sab * sac * sgn(c);
+            tau = R(1.0) / rt<R>::sqrt(rt<R>::abs(b + c));
+            a = midad + p;
+            d = midad - p;
+            b = b - c;
+            c = R(0.0);
+            R cs1 = sab * tau;
+            R sn1 = sac * tau;
+            R csn = cs * cs1 - sn * sn1;
+            R snn = cs * sn1 + sn * cs1;
+            cs = csn;
+            sn = snn;
+          }
+        } else {
+          b = -c;
+          c = R(0.0);
+          R tmp = cs;
+          cs = -sn;
+          sn = tmp;
+        }
+      }
+    }
+  }
+  if (c == R(0.0)) {
+    w1 = cx<R>(a);
+    w2 = cx<R>(d);
+  } else {
+    R rti = rt<R>::sqrt(rt<R>::abs(b)) * rt<R>::sqrt(rt<R>::abs(c));
+    w1 = cx<R>(a, rti);
+    w2 = cx<R>(d, -rti);
+  }
+}
+
+// --------------------------------------------- GenericSchur.jl:837-952
+template <class R> void double_shift_qr(long n, Mat<R> H, R* zp, long ldz, cx<R> shift1, cx<R> shift2,
+                                        long istart, long iend) {
+  Mat<R> Z{zp, ldz};
+  const long i1 = 1, i2 = n;
+  const R r1r = shift1.re, r1i = shift1.im, r2r = shift2.re, r2i = shift2.im;
+  R v[3] = {R(0.0), R(0.0), R(0.0)};
+  const R eps = rt<R>::eps();
+  long mx = istart;
+  for (long m = iend - 2; m >= istart; --m) {
+    R H21s = H(m + 1, m);
+    R s = rt<R>::abs(H(m, m) - r2r) + rt<R>::abs(r2i) + rt<R>::abs(H21s);
+    H21s = H21s / s;
+    v[0] = H21s * H(m, m + 1) + (H(m, m) - r1r) * ((H(m, m) - r2r) / s) - r1i * (r2i / s);
+    v[1] = H21s * (H(m, m) + H(m + 1, m + 1) - r1r - r2r);
+    v[2] = H21s * H(m + 2, m + 1);
+    s = rt<R>::abs(v[0]) + rt<R>::abs(v[1]) + rt<R>::abs(v[2]);
+    v[0] = v[0] / s;
+    v[1] = v[1] / s;
+    v[2] = v[2] / s;
+    if (m > istart &&
+        rt<R>::abs(H(m, m - 1)) * (rt<R>::abs(v[1]) + rt<R>::abs(v[2])) <=
+            eps * rt<R>::abs(v[0]) * (rt<R>::abs(H(m - 1, m - 1)) + rt<R>::abs(H(m, m)) + rt<R>::abs(H(m + 1, m + 1)))) {
+      mx = m;
+      break;
+    }
+  }
+  for (long k = mx; k <= iend - 1; ++k) {
+    const long nr = std::min(3L, iend - k + 1);
+    if (k > mx)
+      for (long ii = 0; ii < nr; ++ii) v[ii] = H(k + ii, k - 1);
+    R tau1 = reflector(nr, v, 1);
+    if (k > mx) {
+      H(k, k - 1) = v[0];
+      H(k + 1, k - 1) = R(0.0);
+      if (k < iend - 1) H(k + 2, k - 1) = R(0.0);
+    } else if (mx > istart) {
+      H(k, k - 1) = H(k, k - 1) * (R(1.0) - tau1);
+    }
+    const R v2 = v[1];
+    const R tau2 = tau1 * v2;
+    if (nr == 3) {
+      const R v3 = v[2];
+      const R tau3 = tau1 * v3;
+      for (long j = k; j <= i2; ++j) {
+        R ss = H(k, j) + v2 * H(k + 1, j) + v3 * H(k + 2, j);
+        H(k, j) = H(k, j) - ss * tau1;
+        H(k + 1, j) = H(k + 1, j) - ss * tau2;
+        H(k + 2, j) = H(k + 2, j) - ss * tau3;
+      }
+      for (long j = i1; j <= std::min(k + 3, iend); ++j) {
+        R ss = H(j, k) + v2 * H(j, k + 1) + v3 * H(j, k + 2);
+        H(j, k) = H(j, k) - ss * tau1;
+        H(j, k + 1) = H(j, k + 1) - ss * tau2;
+        H(j, k + 2) = H(j, k + 2) - ss * tau3;
+      }
+      if (zp) {
+        for (long j = 1; j <= n; ++j) {
+          R ss = Z(j, k) + v2 * Z(j, k + 1) + v3 * Z(j, k + 2);
+          Z(j, k) = Z(j, k) - ss * tau1;
+          Z(j, k + 1) = Z(j, k + 1) - ss * tau2;
+          Z(j, k + 2) = Z(j, k + 2) - ss * tau3;
+        }
+      }
+    } else if (nr == 2) {
+      for (long j = k; j <= i2; ++j) {
+        R ss = H(k, j) + v2 * H(k + 1, j);
+        H(k, j) = H(k, j) - ss * tau1;
+        H(k + 1, j) = H(k + 1, j) - ss * tau2;
+      }
+      for (long j = i1; j <= iend; ++j) {
+        R ss = H(j, k) + v2 * H(j, k + 1);
+        H(j, k) = H(j, k) - ss * tau1;
+        H(j, k + 1) = H(j, k + 1) - ss * tau2;
+      }
+      if (zp) {
+        for (long j = 1; j <= n; ++j) {
+          R ss = Z(j, k) + v2 * Z(j, k + 1);
+          Z(j, k) = Z(j, k) - ss * tau1;
+          Z(j, k + 1) = Z(j, k + 1) - ss * tau2;
+        }
+      }
+    }
+  }
+}
+
+// ------------ GenericSchur.jl:513-699: real Schur of a Hessenberg matrix
+// (dlahqr logic) to standard form.  w receives the eigenvalues by position.
+template <class R> int schur_hess_real(long n, R* h, long ldh, R* zp, long ldz, long maxiter, cx<R>* w,
+                                       long* stats = nullptr) {
+  Mat<R> HH{h, ldh};
+  Mat<R> Z{zp, ldz};
+  for (long j = 1; j <= n; ++j)
+    for (long i = j + 2; i <= n; ++i) HH(i, j) = R(0.0);
+  long iwcur = n;
+  const R eps = rt<R>::eps();
+  const R smallnum = rt<R>::fmin() * (R(double(n)) / eps);
+  const R threeq = R(3.0) / R(4.0);
+  const R m7_16 = R(-7.0) / R(16.0);
+  long iter = 0, nsweep = 0, nstep = 0;
+  long iend = n, istart = 1;
+  while (iend >= 1) {
+    istart = 1;
+    long iterqr = 0;
+    bool deflate = false;
+    while (true) {
+      iter += 1;
+      if (iter > maxiter) {
+        if (stats) { stats[0] = iter; stats[1] = nsweep; stats[2] = nstep; }
+        return ORC_UNCONVERGED;
+      }
+      bool split = false;
+      for (long k = iend; k >= istart + 1; --k) {
+        if (rt<R>::abs(HH(k, k - 1)) < smallnum) {
+          split = true;
+        } else {
+          R Hkk = HH(k, k);
+          R Hkm1km1 = HH(k - 1, k - 1);
+          R t = rt<R>::abs(Hkm1km1) + rt<R>::abs(Hkk);
+          if (t == R(0.0)) {
+            if (k > 2) t = t + rt<R>::abs(HH(k - 1, k - 2));
+            if (k + 1 <= n) t = t + rt<R>::abs(HH(k + 1, k));
+          }
+          R aHkkm1 = rt<R>::abs(HH(k, k - 1));
+          if (aHkkm1 <= t * eps) {
+            R aHkm1k = rt<R>::abs(HH(k - 1, k));
+            R ab = rmax(aHkkm1, aHkm1k);
+            R ba = rmin(aHkkm1, aHkm1k);
+            R aa = rmax(rt<R>::abs(Hkk), rt<R>::abs(Hkm1km1 - Hkk));
+            R bb = rmin(rt<R>::abs(Hkk), rt<R>::abs(Hkm1km1 - Hkk));
+            R s = aa + bb;
+            if (ba * (ab / s) <= rmax(smallnum, eps * (bb * (aa / s)))) split = true;
+          }
+        }
+        if (split) {
+          istart = k;
+          break;
+        }
+      }
+      if (!split) istart = 1;
+      if (istart > 1) HH(istart, istart - 1) = R(0.0);
+      if (istart >= iend - 1) {
+        deflate = true;
+        break;
+      }
+      iterqr += 1;
+      R H11, H12, H21, H22;
+      if (iterqr == 10) {
+        R s = rt<R>::abs(HH(istart + 1, istart)) + rt<R>::abs(HH(istart + 2, istart + 1));
+        H11 = threeq * s + HH(istart, istart);
+        H12 = m7_16 * s;
+        H21 = s;
+        H22 = H11;
+      } else if (iterqr == 20) {
+        R s = rt<R>::abs(HH(iend, iend - 1)) + rt<R>::abs(HH(iend - 1, iend - 2));
+        H11 = threeq * s + HH(iend, iend);
+        H12 = m7_16 * s;
+        H21 = s;
+        H22 = H11;
+      } else {
+        H11 = HH(iend - 1, iend - 1);
+        H21 = HH(iend, iend - 1);
+        H12 = HH(iend - 1, iend);
+        H22 = HH(iend, iend);
+      }
+      R s = rt<R>::abs(H11) + rt<R>::abs(H12) + rt<R>::abs(H21) + rt<R>::abs(H22);
+      R r1r, r2r, r1i, r2i;
+      if (s == R(0.0)) {
+        r1r = r2r = r1i = r2i = R(0.0);
+      } else {
+        H11 = H11 / s;
+        H12 = H12 / s;
+        H21 = H21 / s;
+        H22 = H22 / s;
+        R tr = (H11 + H22) / R(2.0);
+        R d = (H11 - tr) * (H22 - tr) - H12 * H21;
+        R rtd = rt<R>::sqrt(rt<R>::abs(d));
+        if (d >= R(0.0)) {
+          r1r = tr * s;
+          r2r = r1r;
+          r1i = rtd * s;
+          r2i = -r1i;
+        } else {
+          r1r = tr + rtd;
+          r2r = tr - rtd;
+          if (rt<R>::abs(r1r - H22) <= rt<R>::abs(r2r - H22)) {
+            r1r = r1r * s;
+            r2r = r1r;
+          } else {
+            r2r = r2r * s;
+            r1r = r2r;
+          }
+          r1i = r2i = R(0.0);
+        }
+      }
+      nsweep += 1;
+      nstep += iend - istart;
+      double_shift_qr<R>(n, HH, zp, ldz, cx<R>(r1r, r1i), cx<R>(r2r, r2i), istart, iend);
+    }
+    if (deflate && istart >= iend) {
+      w[iwcur - 1] = cx<R>(HH(iend, iend));
+      iwcur -= 1;
+      iend = istart - 1;
+    } else if (deflate && istart + 1 == iend) {
+      R a = HH(iend - 1, iend - 1), b = HH(iend - 1, iend), c = HH(iend, iend - 1), d = HH(iend, iend);
+      R cs, sn;
+      cx<R> w1, w2;
+      gs2x2(a, b, c, d, cs, sn, w1, w2);
+      w[iwcur - 1] = w2;
+      iwcur -= 1;
+      w[iwcur - 1] = w1;
+      iwcur -= 1;
+      // lmul!(G, HH[:, istart:n]) with G = Givens(iend-1, iend, cs, sn)
+      for (long j = istart; j <= n; ++j) {
+        R a1 = HH(iend - 1, j), a2 = HH(iend, j);
+        HH(iend - 1, j) = cs * a1 + sn * a2;
+        HH(iend, j) = -sn * a1 + cs * a2;
+      }
+      // rmul!(HH[1:iend, :], G')
+      for (long i = 1; i <= iend; ++i) {
+        R a1 = HH(i, iend - 1), a2 = HH(i, iend);
+        HH(i, iend - 1) = a1 * cs + a2 * sn;
+        HH(i, iend) = -a1 * sn + a2 * cs;
+      }
+      HH(iend - 1, iend - 1) = a;
+      HH(iend - 1, iend) = b;
+      HH(iend, iend - 1) = c;
+      HH(iend, iend) = d;
+      if (iend > 2) HH(iend - 1, iend - 2) = R(0.0);
+      if (zp) {
+        for (long i = 1; i <= n; ++i) {
+          R a1 = Z(i, iend - 1), a2 = Z(i, iend);
+          Z(i, iend - 1) = a1 * cs + a2 * sn;
+          Z(i, iend) = -a1 * sn + a2 * cs;
+        }
+      }
+    }
+    iend = istart - 1;
+  }
+  for (long j = 1; j <= n; ++j)
+    for (long i = j + 2; i <= n; ++i) HH(i, j) = R(0.0);
+  if (stats) { stats[0] = iter; stats[1] = nsweep; stats[2] = nstep; }
+  return ORC_OK;
+}
+
+// ------ GenericSchur.jl:350-372 (complex) and :805-835 (real) drivers.
+// On entry `a` holds A (overwritten).  On exit t holds T, z holds Z (when
+// wantz) and w the eigenvalues.
+template <class R> int gschur(long n, cx<R>* a, long lda, cx<R>* t, long ldt, cx<R>* z, long ldz, cx<R>* w,
+                              bool wantz, bool scale, long maxiter, long maxinner, long* stats = nullptr) {
+  typedef cx<R> T;
+  R cscale(1.0), anrm(1.0);
+  bool scaled = false;
+  if (scale) scaled = scale_matrix(n, a, lda, cscale, anrm);
+  std::vector<T> tau(n > 1 ? n - 1 : 1);
+  hessenberg(n, a, lda, tau.data());
+  if (wantz) materialize_q(n, a, lda, tau.data(), z, ldz);
+  for (long j = 0; j < n; ++j)
+    for (long i = 0; i < n; ++i) t[i + j * ldt] = a[i + j * lda];
+  int info = schur_hess_complex<R>(n, t, ldt, wantz ? z : nullptr, ldz, maxiter, maxinner, false, w, stats);
+  if (info != ORC_OK) return info;
+  Mat<T> TT{t, ldt};
+  for (long j = 1; j <= n; ++j)
+    for (long i = j + 1; i <= n; ++i) TT(i, j) = T();  // triu(HH)
+  if (scaled) {
+    for (long j = 0; j < n; ++j) safescale(n, t + j * ldt, cscale, anrm);
+    for (long j = 1; j <= n; ++j) w[j - 1] = TT(j, j);
+  }
+  return ORC_OK;
+}
+
+template <class R> int gschur(long n, R* a, long lda, R* t, long ldt, R* z, long ldz, cx<R>* w, bool wantz,
+                              bool scale, long maxiter, long /*maxinner*/, long* stats = nullptr) {
+  R cscale(1.0), anrm(1.0);
+  bool scaled = false;
+  if (scale) scaled = scale_matrix(n, a, lda, cscale, anrm);
+  std::vector<R> tau(n > 1 ? n - 1 : 1);
+  hessenberg(n, a, lda, tau.data());
+  if (wantz) materialize_q(n, a, lda, tau.data(), z, ldz);
+  for (long j = 0; j < n; ++j)
+    for (long i = 0; i < n; ++i) t[i + j * ldt] = a[i + j * lda];
+  int info = schur_hess_real<R>(n, t, ldt, wantz ? z : nullptr, ldz, maxiter, w, stats);
+  if (info != ORC_OK) return info;
+  if (scaled) {
+    for (long j = 0; j < n; ++j) safescale(n, t + j * ldt, cscale, anrm);
+    safescale(n, w, cscale, anrm);
+  }
+  return ORC_OK;
+}
+
+// ========================================================= eigenvectors
+
+// --------------------------------------------- util.jl:128-293 (zlatrs)
+template <class T> typename real_of<T>::type usolve(Mat<T> A, long n, T* x, typename real_of<T>::type* cnorm) {
+  typedef typename real_of<T>::type R;
+  auto cabs1half = [](const T& z) { return rt<R>::abs(real(z) / R(2.0)) + rt<R>::abs(imag(z) / R(2.0)); };
+  const R half = R(1.0) / R(2.0);
+  const R smallnum = safemin<R>() / rt<R>::eps();
+  const R bignum = R(1.0) / smallnum;
+  R xscale(1.0);
+  R tmax = rt<R>::abs(cnorm[0]);
+  for (long j = 2; j <= n; ++j) tmax = rmax(tmax, rt<R>::abs(cnorm[j - 1]));
+  R tscale;
+  if (tmax <= bignum * half) {
+    tscale = R(1.0);
+  } else {
+    tscale = half / (smallnum * tmax);
+    for (long j = 0; j < n; ++j) cnorm[j] = cnorm[j] * tscale;
+  }
+  R xmax = cabs1half(x[0]);
+  for (long j = 2; j <= n; ++j) xmax = rmax(xmax, cabs1half(x[j - 1]));
+  R xbound = xmax;
+  R grow;
+  if (tscale != R(1.0)) {
+    grow = R(0.0);
+  } else {
+    grow = half / rmax(xbound, smallnum);
+    xbound = grow;
+    bool toosmall = false;
+    for (long j = n; j >= 1; --j) {
+      toosmall = (grow <= smallnum);
+      if (toosmall) break;
+      T tjjs = A(j, j);
+      R tjj = abs1(tjjs);
+      if (tjj >= smallnum) {
+        xbound = rmin(xbound, rmin(R(1.0), tjj) * grow);
+      } else {
+        xbound = R(0.0);
+      }
+      if (tjj + cnorm[j - 1] >= smallnum) {
+        grow = grow * (tjj / (tjj + cnorm[j - 1]));
+      } else {
+        grow = R(0.0);
+      }
+    }
+    if (!toosmall) grow = xbound;
+  }
+  if (grow * tscale > smallnum) {
+    for (long j = n; j >= 1; --j) {
+      x[j - 1] = x[j - 1] / A(j, j);
+      T xj = x[j - 1];
+      for (long i = j - 1; i >= 1; --i) x[i - 1] = x[i - 1] - A(i, j) * xj;
+    }
+  } else {
+    if (xmax > bignum * half) {
+      xscale = (bignum * half) / xmax;
+      for (long i = 0; i < n; ++i) x[i] = x[i] * xscale;
+      xmax = bignum;
+    } else {
+      xmax = xmax * R(2.0);
+    }
+    for (long j = n; j >= 1; --j) {
+      R xj = abs1(x[j - 1]);
+      T tjjs = A(j, j) * tscale;
+      R tjj = abs1(tjjs);
+      if (tjj > smallnum) {
+        if (tjj < R(1.0)) {
+          if (xj > tjj * bignum) {
+            R rec = R(1.0) / xj;
+            for (long i = 0; i < n; ++i) x[i] = x[i] * rec;
+            xscale = xscale * rec;
+            xmax = xmax * rec;
+          }
+        }
+        x[j - 1] = x[j - 1] / tjjs;
+        xj = abs1(x[j - 1]);
+      } else if (tjj > R(0.0)) {
+        if (xj > tjj * bignum) {
+          R rec = (tjj * bignum) / xj;
+          if (cnorm[j - 1] > R(1.0)) rec = rec / cnorm[j - 1];
+          for (long i = 0; i < n; ++i) x[i] = x[i] * rec;
+          xscale = xscale * rec;
+          xmax = xmax * rec;
+        }
+        x[j - 1] = x[j - 1] / tjjs;
+        xj = abs1(x[j - 1]);
+      } else {
+        for (long i = 0; i < n; ++i) x[i] = T();
+        x[j - 1] = T(R(1.0));
+        xscale = R(0.0);
+        xmax = R(0.0);
+      }
+      if (xj > R(1.0)) {
+        R rec = R(1.0) / xj;
+        if (cnorm[j - 1] > (bignum - xmax) * rec) {
+          rec = rec * half;
+          for (long i = 0; i < n; ++i) x[i] = x[i] * rec;
+          xscale = xscale * rec;
+        }
+      } else if (xj * cnorm[j - 1] > bignum - xmax) {
+        for (long i = 0; i < n; ++i) x[i] = x[i] * half;
+        xscale = xscale * half;
+      }
+      if (j > 1) {
+        T xjt = x[j - 1] * tscale;
+        for (long i = 1; i <= j - 1; ++i) x[i - 1] = x[i - 1] - xjt * A(i, j);
+        xmax = abs1(x[0]);
+        for (long i = 2; i <= j - 1; ++i) xmax = rmax(xmax, abs1(x[i - 1]));
+      }
+    }
+    xscale = xscale / tscale;
+  }
+  if (tscale != R(1.0)) {
+    for (long i = 0; i < n; ++i) cnorm[i] = cnorm[i] * (R(1.0) / tscale);
+  }
+  return xscale;
+}
+
+// --------------------------------------------- util.jl:301-461 (zlatrs, ^H)
+template <class T> typename real_of<T>::type cusolve(Mat<T> A, long n, T* x, typename real_of<T>::type* cnorm) {
+  typedef typename real_of<T>::type R;
+  auto cabs1half = [](const T& z) { return rt<R>::abs(real(z) / R(2.0)) + rt<R>::abs(imag(z) / R(2.0)); };
+  const R half = R(1.0) / R(2.0);
+  const R smallnum = safemin<R>() / rt<R>::eps();
+  const R bignum = R(1.0) / smallnum;
+  R xscale(1.0);
+  R tmax = rt<R>::abs(cnorm[0]);
+  for (long j = 2; j <= n; ++j) tmax = rmax(tmax, rt<R>::abs(cnorm[j - 1]));
+  R tscale;
+  if (tmax <= bignum * half) {
+    tscale = R(1.0);
+  } else {
+    tscale = half / (smallnum * tmax);
+    for (long j = 0; j < n; ++j) cnorm[j] = cnorm[j] * tscale;
+  }
+  R xmax = cabs1half(x[0]);
+  for (long j = 2; j <= n; ++j) xmax = rmax(xmax, cabs1half(x[j - 1]));
+  R xbound = xmax;
+  R grow;
+  if (tscale != R(1.0)) {
+    grow = R(0.0);
+  } else {
+    grow = half / rmax(xbound, smallnum);
+    xbound = grow;
+    bool toosmall = false;
+    for (long j = 1; j <= n; ++j) {
+      toosmall = (grow <= smallnum);
+      if (toosmall) break;
+      R xj = R(1.0) + cnorm[j - 1];
+      grow = rmin(grow, xbound / xj);
+      T tjjs = A(j, j);
+      R tjj = abs1(tjjs);
+      if (tjj >= smallnum) {
+        if (xj > tjj) xbound = xbound * (tjj / xj);
+      } else {
+        xbound = R(0.0);
+      }
+    }
+    if (!toosmall) grow = rmin(grow, xbound);
+  }
+  if (grow * tscale > smallnum) {
+    for (long j = 1; j <= n; ++j) {
+      T zz = x[j - 1];
+      for (long i = 1; i <= j - 1; ++i) zz = zz - conj(A(i, j)) * x[i - 1];
+      x[j - 1] = zz / conj(A(j, j));
+    }
+  } else {
+    if (xmax > bignum * half) {
+      xscale = (bignum * half) / xmax;
+      for (long i = 0; i < n; ++i) x[i] = x[i] * xscale;
+      xmax = bignum;
+    } else {
+      xmax = xmax * R(2.0);
+    }
+    for (long j = 1; j <= n; ++j) {
+      R xj = abs1(x[j - 1]);
+      T uscale = T(tscale);
+      R rec = R(1.0) / rmax(xmax, R(1.0));
+      T tjjs = conj(A(j, j)) * tscale;
+      if (cnorm[j - 1] > (bignum - xj) * rec) {
+        rec = rec * half;
+        tjjs = conj(A(j, j)) * tscale;
+        R tjj = abs1(tjjs);
+        if (tjj > R(1.0)) {
+          rec = rmin(R(1.0), rec * tjj);
+          uscale = uscale / tjjs;
+        }
+        if (rec < R(1.0)) {
+          for (long i = 0; i < n; ++i) x[i] = x[i] * rec;
+          xscale = xscale * rec;
+          xmax = xmax * rec;
+        }
+      }
+      T csumj = T();
+      for (long i = 1; i <= j - 1; ++i) csumj = csumj + (conj(A(i, j)) * uscale) * x[i - 1];
+      bool unscaled = (uscale == T(tscale)) || rt<R>::isnan(uscale.re) || rt<R>::isnan(imag(uscale));
+      if (unscaled) {
+        x[j - 1] = x[j - 1] - csumj;
+        xj = abs1(x[j - 1]);
+        tjjs = conj(A(j, j)) * tscale;
+        R tjj = abs1(tjjs);
+        if (tjj > smallnum) {
+          if (tjj < R(1.0)) {
+            if (xj > tjj * bignum) {
+              R rec2 = R(1.0) / xj;
+              for (long i = 0; i < n; ++i) x[i] = x[i] * rec2;
+              xscale = xscale * rec2;
+              xmax = xmax * rec2;
+            }
+          }
+          x[j - 1] = x[j - 1] / tjjs;
+        } else if (tjj > R(0.0)) {
+          if (xj > tjj * bignum) {
+            R rec2 = (tjj * bignum) / xj;
+            for (long i = 0; i < n; ++i) x[i] = x[i] * rec2;
+            xscale = xscale * rec2;
+            xmax = xmax * rec2;
+          }
+          x[j - 1] = x[j - 1] / tjjs;
+        } else {
+          for (long i = 0; i < n; ++i) x[i] = T();
+          x[j - 1] = T(R(1.0));
+          xscale = R(0.0);
+          xmax = R(0.0);
+        }
+      } else {
+        x[j - 1] = x[j - 1] / tjjs - csumj;
+      }
+      xmax = rmax(xmax, abs1(x[j - 1]));
+    }
+    xscale = xscale / tscale;
+  }
+  if (tscale != R(1.0)) {
+    for (long i = 0; i < n; ++i) cnorm[i] = cnorm[i] * (R(1.0) / tscale);
+  }
+  return xscale;
+}
+
+// ------------------------------------------------ util.jl:603-687
+template <class R> R safe_lu_solve(long n, cx<R>* ap, cx<R>* b, bool& perturbed) {
+  typedef cx<R> T;
+  Mat<T> A{ap, n};
+  perturbed = false;
+  const R smallnum = rt<R>::fmin() / rt<R>::eps();
+  R smin(0.0);
+  for (long i = 0; i < n * n; ++i) smin = rmax(smin, absv(ap[i]));
+  smin = rmax(rt<R>::eps() * smin, smallnum);
+  long ipsv = 0, jpsv = 0;
+  std::vector<long> jpiv(n, 0);
+  for (long i = 1; i <= n - 1; ++i) {
+    R xmax(0.0);
+    for (long ip = i; ip <= n; ++ip)
+      for (long jp = i; jp <= n; ++jp)
+        if (absv(A(ip, jp)) >= xmax) {
+          xmax = absv(A(ip, jp));
+          ipsv = ip;
+          jpsv = jp;
+        }
+    if (ipsv != i) {
+      for (long j = 1; j <= n; ++j) std::swap(A(ipsv, j), A(i, j));
+      std::swap(b[i - 1], b[ipsv - 1]);
+    }
+    if (jpsv != i) {
+      for (long j = 1; j <= n; ++j) std::swap(A(j, jpsv), A(j, i));
+    }
+    jpiv[i - 1] = jpsv;
+    if (absv(A(i, i)) < smin) {
+      perturbed = true;
+      A(i, i) = T(smin);
+    }
+    for (long j = i + 1; j <= n; ++j) {
+      A(j, i) = A(j, i) / A(i, i);
+      b[j - 1] = b[j - 1] - A(j, i) * b[i - 1];
+      for (long k = i + 1; k <= n; ++k) A(j, k) = A(j, k) - A(j, i) * A(i, k);
+    }
+  }
+  if (absv(A(n, n)) < smin) {
+    perturbed = true;
+    A(n, n) = T(smin);
+  }
+  jpiv[n - 1] = n;
+  R scale(1.0);
+  const R a = R(8.0) * smallnum;
+  bool needscl = false;
+  for (long i = 1; i <= n; ++i)
+    if (a * absv(b[i - 1]) > absv(A(i, i))) {
+      needscl = true;
+      break;
+    }
+  if (needscl) {
+    R bm(0.0);
+    for (long i = 0; i < n; ++i) bm = rmax(bm, absv(b[i]));
+    scale = (R(1.0) / R(8.0)) / bm;
+    for (long i = 0; i < n; ++i) b[i] = b[i] * scale;
+  }
+  for (long i = 1; i <= n; ++i) {
+    long k = n + 1 - i;
+    T t = T(R(1.0)) / A(k, k);
+    b[k - 1] = b[k - 1] * t;
+    for (long j = k + 1; j <= n; ++j) b[k - 1] = b[k - 1] - (t * A(k, j)) * b[j - 1];
+  }
+  for (long i = 1; i <= n - 1; ++i) {
+    long jj = jpiv[n - i - 1];
+    if (jj != n - i) std::swap(b[n - i - 1], b[jj - 1]);
+  }
+  return scale;
+}
+
+// ------------------------------------------------ util.jl:698-751
+// Solves (a*A - bw*D) X = s*B for na = 1 or 2, D = I.  A is given by its
+// entries in column-major order (a11, a21, a12, a22).  Returns s; B is
+// overwritten by X and xnorm receives the reference's xnorm.
+template <class R> R xsolve(R a, long na, const R* Ae, cx<R> bw, cx<R>* B, R smin, R& xnorm) {
+  typedef cx<R> T;
+  R s(1.0);
+  const R small = R(2.0) * safemin<R>();
+  const R bignum = s / small;
+  smin = rmax(smin, small);
+  if (na == 1) {
+    R csr = a * Ae[0] - bw.re;
+    R csi = -bw.im;
+    R cnorm = rt<R>::abs(csr) + rt<R>::abs(csi);
+    if (cnorm < smin) {
+      csr = smin;
+      csi = R(0.0);
+      cnorm = smin;
+    }
+    R bnorm = abs1(B[0]);
+    if (cnorm < R(1.0) && bnorm > R(1.0) && bnorm > bignum * cnorm) s = R(1.0) / bnorm;
+    T x = (B[0] * s) / T(csr, csi);
+    xnorm = abs1(x);
+    B[0] = x;
+    return s;
+  }
+  R cmax(0.0);
+  T M[4];
+  for (long j = 0; j < 2; ++j)
+    for (long i = 0; i < 2; ++i) {
+      T z = T(a * Ae[i + 2 * j]);
+      if (i == j) z = z - bw;
+      M[i + 2 * j] = z;
+      cmax = rmax(cmax, abs1(z));
+    }
+  if (cmax < smin) {
+    R bnorm = rmax(abs1(B[0]), abs1(B[1]));
+    if (smin < R(1.0) && bnorm > R(1.0) && bnorm > bignum * smin) s = R(1.0) / bnorm;
+    R t1 = s / smin;
+    B[0] = B[0] * t1;
+    B[1] = B[1] * t1;
+  } else {
+    bool pert;
+    s = safe_lu_solve<R>(2, M, B, pert);
+  }
+  xnorm = rt<R>::sqrt(abs2(B[0]) + abs2(B[1]));
+  return s;
+}
+
+// ------------------------------------------------ util.jl:572-592
+template <class R> void enormalize(long n, cx<R>* v, long ldv) {
+  Mat<cx<R>> V{v, ldv};
+  for (long j = 1; j <= n; ++j) {
+    R nrm = norm2(n, &V(1, j), 1);
+    R s = R(1.0) / nrm;
+    R t = abs2(V(1, j));
+    long i0 = 1;
+    for (long i = 2; i <= n; ++i) {
+      R u = abs2(V(i, j));
+      if (u > t) {
+        i0 = i;
+        t = u;
+      }
+    }
+    cx<R> tt = (s * conj(V(i0, j))) / rt<R>::sqrt(t);
+    for (long i = 1; i <= n; ++i) V(i, j) = V(i, j) * tt;
+    V(i0, j) = cx<R>(V(i0, j).re);
+  }
+}
+
+// ------------------------------- vectors.jl:45-131 (ztrevc-style right)
+template <class R> void eigvecs_right_complex(long n, cx<R>* tp, long ldt, const cx<R>* zp, long ldz, cx<R>* vp,
+                                              long ldv) {
+  typedef cx<R> T;
+  Mat<T> TT{tp, ldt};
+  Mat<const T> Z{zp, ldz};
+  Mat<T> V{vp, ldv};
+  const R ulp = rt<R>::eps();
+  const R smallnum = safemin<R>() * R(double(n));
+  std::vector<T> v(n), tdiag(n);
+  std::vector<R> tnorms(n, R(0.0));
+  for (long j = 1; j <= n; ++j) tdiag[j - 1] = TT(j, j);
+  for (long j = 2; j <= n; ++j)
+    for (long i = 1; i <= j - 1; ++i) tnorms[j - 1] = tnorms[j - 1] + absv(TT(i, j));
+  for (long ki = n; ki >= 1; --ki) {
+    R smin = rmax(ulp * abs1(TT(ki, ki)), smallnum);
+    v[0] = T(R(1.0));
+    for (long k = 1; k <= ki - 1; ++k) v[k - 1] = -TT(k, ki);
+    for (long k = 1; k <= ki - 1; ++k) {
+      TT(k, k) = TT(k, k) - TT(ki, ki);
+      if (abs1(TT(k, k)) < smin) TT(k, k) = T(smin);
+    }
+    R vscale;
+    if (ki > 1) {
+      vscale = usolve<T>(TT, ki - 1, v.data(), tnorms.data());
+      v[ki - 1] = T(vscale);
+    } else {
+      vscale = R(1.0);
+    }
+    if (zp) {
+      for (long j = 1; j <= n; ++j) {
+        V(j, ki) = vscale * Z(j, ki);
+        for (long i = 1; i <= ki - 1; ++i) V(j, ki) = V(j, ki) + Z(j, i) * v[i - 1];
+      }
+    } else {
+      for (long j = 1; j <= ki; ++j) V(j, ki) = v[j - 1];
+      for (long j = ki + 1; j <= n; ++j) V(j, ki) = T();
+    }
+    R t0 = abs1(V(1, ki));
+    for (long i = 2; i <= n; ++i) t0 = rmax(t0, abs1(V(i, ki)));
+    R remax = R(1.0) / t0;
+    for (long i = 1; i <= n; ++i) V(i, ki) = V(i, ki) * remax;
+    for (long k = 1; k <= ki - 1; ++k) TT(k, k) = tdiag[k - 1];
+  }
+}
+
+// ------------------------------ vectors.jl:372-460 (ztrevc-style left)
+template <class R> void eigvecs_left_complex(long n, cx<R>* tp, long ldt, const cx<R>* zp, long ldz, cx<R>* vp,
+                                             long ldv) {
+  typedef cx<R> T;
+  Mat<T> TT{tp, ldt};
+  Mat<const T> Z{zp, ldz};
+  Mat<T> V{vp, ldv};
+  const R ulp = rt<R>::eps();
+  const R smallnum = safemin<R>() * R(double(n));
+  std::vector<T> v(n), tdiag(n);
+  std::vector<R> tnorms(n, R(0.0));
+  for (long j = 1; j <= n; ++j) tdiag[j - 1] = TT(j, j);
+  for (long j = 2; j <= n; ++j)
+    for (long i = 1; i <= j - 1; ++i) tnorms[j - 1] = tnorms[j - 1] + abs1(TT(i, j));
+  for (long ki = 1; ki <= n; ++ki) {
+    R smin = rmax(ulp * abs1(TT(ki, ki)), smallnum);
+    for (long k = ki + 1; k <= n; ++k) v[k - 1] = -conj(TT(ki, k));
+    for (long k = 1; k <= ki; ++k) v[k - 1] = T();
+    for (long k = ki + 1; k <= n; ++k) {
+      TT(k, k) = TT(k, k) - TT(ki, ki);
+      if (abs1(TT(k, k)) < smin) TT(k, k) = T(smin);
+    }
+    R vscale;
+    if (ki < n) {
+      Mat<T> sub{&TT(ki + 1, ki + 1), ldt};
+      // the reference passes the full tnorms here (vectors.jl:417-420)
+      vscale = cusolve<T>(sub, n - ki, &v[ki], tnorms.data());
+      v[ki - 1] = T(vscale);
+    } else {
+      v[n - 1] = T(R(1.0));
+      vscale = R(1.0);
+    }
+    if (zp) {
+      for (long j = 1; j <= n; ++j) {
+        V(j, ki) = vscale * Z(j, ki);
+        for (long i = ki + 1; i <= n; ++i) V(j, ki) = V(j, ki) + Z(j, i) * v[i - 1];
+      }
+    } else {
+      for (long j = ki; j <= n; ++j) V(j, ki) = v[j - 1];
+      for (long j = 1; j <= ki - 1; ++j) V(j, ki) = T();
+    }
+    R t0 = abs1(V(1, ki));
+    for (long i = 2; i <= n; ++i) t0 = rmax(t0, abs1(V(i, ki)));
+    R remax = R(1.0) / t0;
+    for (long i = 1; i <= n; ++i) V(i, ki) = V(i, ki) * remax;
+    for (long k = ki + 1; k <= n; ++k) TT(k, k) = tdiag[k - 1];
+  }
+}
+
+// ------------------------------- vectors.jl:133-362 (dtrevc-style right)
+template <class R> void eigvecs_right_real(long n, const R* tp, long ldt, const R* zp, long ldz, cx<R>* vp,
+                                           long ldv) {
+  typedef cx<R> C;
+  Mat<const R> T{tp, ldt};
+  Mat<const R> Z{zp, ldz};
+  Mat<C> V{vp, ldv};
+  const R ulp = rt<R>::eps();
+  const R smallnum = safemin<R>() * (R(double(n)) / ulp);
+  const R bignum = (R(1.0) - ulp) / smallnum;
+  for (long j = 1; j <= n; ++j)
+    for (long i = 1; i <= n; ++i) V(i, j) = C();
+  const bool have_z = (zp != nullptr);
+  std::vector<C> v(n);
+  std::vector<R> tnorms(n, R(0.0));
+  for (long j = 2; j <= n; ++j)
+    for (long i = 1; i <= j - 1; ++i) tnorms[j - 1] = tnorms[j - 1] + rt<R>::abs(T(i, j));
+  int ip = 0;
+  long isel = n;
+  const R one(1.0);
+  for (long ki = n; ki >= 1; --ki) {
+    if (ip == 1) {
+      ip = 0;
+      if (have_z) {
+        for (long i = 1; i <= n; ++i) V(i, ki) = conj(V(i, ki + 1));
+      } else {
+        for (long i = 1; i <= n; ++i) V(i, isel + 1) = conj(V(i, isel + 2));
+      }
+      continue;
+    }
+    if (ki > 1 && T(ki, ki - 1) != R(0.0)) ip = -1;
+    R wr = T(ki, ki);
+    R wi = (ip == 0) ? R(0.0) : rt<R>::sqrt(rt<R>::abs(T(ki, ki - 1))) * rt<R>::sqrt(rt<R>::abs(T(ki - 1, ki)));
+    R smin = rmax(ulp * (rt<R>::abs(wr) + rt<R>::abs(wi)), smallnum);
+    if (ip == 0) {
+      v[ki - 1] = C(one);
+      for (long i = 1; i <= ki - 1; ++i) v[i - 1] = C(-T(i, ki));
+      long jnxt = ki - 1;
+      for (long j = ki - 1; j >= 1; --j) {
+        if (j > jnxt) continue;
+        long j1 = j, j2 = j;
+        if (j > 1 && T(j, j - 1) != R(0.0)) {
+          j1 = j - 1;
+          jnxt = j - 2;
+        }
+        if (j1 == j2) {
+          C vt1[1] = {v[j - 1]};
+          R Ae[1] = {T(j, j)};
+          R t1;
+          R vscale = xsolve<R>(one, 1, Ae, C(wr), vt1, smin, t1);
+          if (t1 > one && tnorms[j - 1] > bignum / t1) {
+            vt1[0] = vt1[0] / t1;
+            vscale = vscale / t1;
+          }
+          if (vscale != one)
+            for (long i = 1; i <= ki; ++i) v[i - 1] = v[i - 1] * vscale;
+          v[j - 1] = vt1[0];
+          for (long i = 1; i <= j - 1; ++i) v[i - 1] = v[i - 1] - vt1[0] * T(i, j);
+        } else {
+          C vt2[2] = {v[j - 2], v[j - 1]};
+          R Ae[4] = {T(j - 1, j - 1), T(j, j - 1), T(j - 1, j), T(j, j)};
+          R t1;
+          R vscale = xsolve<R>(one, 2, Ae, C(wr), vt2, smin, t1);
+          if (t1 > one) {
+            R beta = rmax(tnorms[j - 2], tnorms[j - 1]);
+            if (beta > bignum / t1) {
+              vt2[0] = vt2[0] / t1;
+              vt2[1] = vt2[1] / t1;
+              vscale = vscale / t1;
+            }
+          }
+          if (vscale != one)
+            for (long i = 1; i <= ki; ++i) v[i - 1] = v[i - 1] * vscale;
+          v[j - 2] = vt2[0];
+          v[j - 1] = vt2[1];
+          for (long i = 1; i <= j - 2; ++i) v[i - 1] = (v[i - 1] - vt2[0] * T(i, j - 1) - vt2[1] * T(i, j));
+        }
+      }
+      if (!have_z) {
+        R m(0.0);
+        for (long i = 1; i <= ki; ++i) m = rmax(m, absv(v[i - 1]));
+        R vscale = one / m;
+        for (long i = 1; i <= ki; ++i) V(i, isel) = vscale * v[i - 1];
+      } else {
+        if (ki == 1) {
+          for (long i = 1; i <= n; ++i) V(i, ki) = C(Z(i, ki));
+        } else {
+          for (long i = 1; i <= n; ++i) {
+            C s;
+            for (long k = 1; k <= ki; ++k) s = s + Z(i, k) * v[k - 1];
+            V(i, ki) = s;
+          }
+          R m(0.0);
+          for (long i = 1; i <= n; ++i) m = rmax(m, absv(V(i, ki)));
+          R vscale = one / m;
+          for (long i = 1; i <= n; ++i) V(i, ki) = V(i, ki) * vscale;
+        }
+      }
+    } else {
+      if (rt<R>::abs(T(ki - 1, ki)) >= rt<R>::abs(T(ki, ki - 1))) {
+        v[ki - 2] = C(one);
+        v[ki - 1] = C(R(0.0), wi / T(ki - 1, ki));
+      } else {
+        v[ki - 2] = C(-wi / T(ki, ki - 1));
+        v[ki - 1] = C(R(0.0), one);
+      }
+      for (long i = 1; i <= ki - 2; ++i)
+        v[i - 1] = C(-v[ki - 2].re * T(i, ki - 1), -(v[ki - 1].im * T(i, ki)));
+      long jnxt = ki - 2;
+      for (long j = ki - 2; j >= 1; --j) {
+        if (j > jnxt) continue;
+        long j1 = j, j2 = j;
+        jnxt = j - 1;
+        if (j > 1 && T(j, j - 1) != R(0.0)) {
+          j1 = j - 1;
+          jnxt = j - 2;
+        }
+        if (j1 == j2) {
+          C vt1[1] = {v[j - 1]};
+          R Ae[1] = {T(j, j)};
+          R t1;
+          R vscale = xsolve<R>(one, 1, Ae, C(wr, wi), vt1, smin, t1);
+          if (t1 > one && tnorms[j - 1] > bignum / t1) {
+            vt1[0] = vt1[0] / t1;
+            vscale = vscale / t1;
+          }
+          if (vscale != one)
+            for (long i = 1; i <= ki; ++i) v[i - 1] = v[i - 1] * vscale;
+          v[j - 1] = vt1[0];
+          for (long i = 1; i <= j - 1; ++i) v[i - 1] = v[i - 1] - vt1[0] * T(i, j);
+        } else {
+          C vt2[2] = {v[j - 2], v[j - 1]};
+          R Ae[4] = {T(j - 1, j - 1), T(j, j - 1), T(j - 1, j), T(j, j)};
+          R t1;
+          R vscale = xsolve<R>(one, 2, Ae, C(wr, wi), vt2, smin, t1);
+          if (t1 > one) {
+            R beta = rmax(tnorms[j - 2], tnorms[j - 1]);
+            if (beta > bignum / t1) {
+              vt2[0] = vt2[0] / t1;
+              vt2[1] = vt2[1] / t1;
+              vscale = vscale / t1;
+            }
+          }
+          if (vscale != one)
+            for (long i = 1; i <= ki; ++i) v[i - 1] = v[i - 1] * vscale;
+          v[j - 2] = vt2[0];
+          v[j - 1] = vt2[1];
+          for (long i = 1; i <= j - 2; ++i) v[i - 1] = (v[i - 1] - vt2[0] * T(i, j - 1) - vt2[1] * T(i, j));
+        }
+      }
+      if (!have_z) {
+        for (long i = 1; i <= ki; ++i) V(i, isel) = conj(v[i - 1]);
+        R t1(0.0);
+        for (long k = 1; k <= ki; ++k) t1 = rmax(t1, abs1(V(k, isel)));
+        t1 = one / t1;
+        for (long i = 1; i <= ki; ++i) V(i, isel) = V(i, isel) * t1;
+      } else {
+        for (long i = 1; i <= n; ++i) {
+          C s;
+          for (long k = 1; k <= ki; ++k) s = s + Z(i, k) * v[k - 1];
+          V(i, ki) = s;
+        }
+        R m(0.0);
+        for (long i = 1; i <= n; ++i) m = rmax(m, absv(V(i, ki)));
+        R vscale = one / m;
+        for (long i = 1; i <= n; ++i) V(i, ki) = vscale * conj(V(i, ki));
+      }
+    }
+    isel -= (ip == 0) ? 1 : 2;
+    if (ip == -1) ip = 1;
+  }
+}
+
+// -------------------------------- vectors.jl:462-698 (dtrevc-style left)
+template <class R> void eigvecs_left_real(long n, const R* tp, long ldt, const R* zp, long ldz, cx<R>* vp,
+                                          long ldv) {
+  typedef cx<R> C;
+  Mat<const R> T{tp, ldt};
+  Mat<const R> Z{zp, ldz};
+  Mat<C> V{vp, ldv};
+  const R ulp = rt<R>::eps();
+  const R smallnum = safemin<R>() * (R(double(n)) / ulp);
+  const R bignum = (R(1.0) - ulp) / smallnum;
+  for (long j = 1; j <= n; ++j)
+    for (long i = 1; i <= n; ++i) V(i, j) = C();
+  const bool have_z = (zp != nullptr);
+  std::vector<C> v(n);
+  std::vector<R> tnorms(n, R(0.0));
+  for (long j = 2; j <= n; ++j)
+    for (long i = 1; i <= j - 1; ++i) tnorms[j - 1] = tnorms[j - 1] + rt<R>::abs(T(i, j));
+  int ip = 0;
+  long isel = 1;
+  const R one(1.0);
+  for (long ki = 1; ki <= n; ++ki) {
+    if (ip == -1) {
+      ip = 0;
+      if (have_z) {
+        for (long i = 1; i <= n; ++i) V(i, ki) = conj(V(i, ki - 1));
+      } else {
+        for (long i = 1; i <= n; ++i) V(i, isel - 1) = conj(V(i, isel - 2));
+      }
+      continue;
+    }
+    if (ki < n && T(ki + 1, ki) != R(0.0)) ip = 1;
+    R wr = T(ki, ki);
+    R wi = (ip == 0) ? R(0.0) : rt<R>::sqrt(rt<R>::abs(T(ki, ki + 1))) * rt<R>::sqrt(rt<R>::abs(T(ki + 1, ki)));
+    R smin = rmax(ulp * (rt<R>::abs(wr) + rt<R>::abs(wi)), smallnum);
+    if (ip == 0) {
+      v[ki - 1] = C(one);
+      for (long i = ki + 1; i <= n; ++i) v[i - 1] = C(-T(ki, i));
+      R vmax = one, vcrit = bignum;
+      long jnxt = ki + 1;
+      for (long j = ki + 1; j <= n; ++j) {
+        if (j < jnxt) continue;
+        long j1 = j, j2 = j;
+        jnxt = j + 1;
+        if (j < n && T(j + 1, j) != R(0.0)) {
+          j2 = j + 1;
+          jnxt = j + 2;
+        }
+        if (j1 == j2) {
+          if (tnorms[j - 1] > vcrit) {
+            R t1 = one / vmax;
+            for (long i = ki; i <= n; ++i) v[i - 1] = v[i - 1] * t1;
+            vmax = one;
+            vcrit = bignum;
+          }
+          C d;
+          for (long i = ki + 1; i <= j - 1; ++i) d = d + T(i, j) * v[i - 1];
+          C vt1[1] = {v[j - 1] - d};
+          R Ae[1] = {T(j, j)};
+          R xn;
+          R vscale = xsolve<R>(one, 1, Ae, C(wr), vt1, smin, xn);
+          if (vscale != one)
+            for (long i = ki; i <= n; ++i) v[i - 1] = v[i - 1] * vscale;
+          v[j - 1] = vt1[0];
+          vmax = rmax(absv(v[j - 1]), vmax);
+          vcrit = bignum / vmax;
+        } else {
+          R beta = rmax(tnorms[j], tnorms[j - 1]);
+          if (beta > vcrit) {
+            R t1 = one / vmax;
+            for (long i = ki; i <= n; ++i) v[i - 1] = v[i - 1] * t1;
+            vmax = one;
+            vcrit = bignum;
+          }
+          C d0, d1;
+          for (long i = ki + 1; i <= j - 1; ++i) {
+            d0 = d0 + T(i, j) * v[i - 1];
+            d1 = d1 + T(i, j + 1) * v[i - 1];
+          }
+          C vt2[2] = {v[j - 1] - d0, v[j] - d1};
+          // transpose of the 2x2 diagonal block
+          R Ae[4] = {T(j, j), T(j, j + 1), T(j + 1, j), T(j + 1, j + 1)};
+          R xn;
+          R vscale = xsolve<R>(one, 2, Ae, C(wr), vt2, smin, xn);
+          if (vscale != one)
+            for (long i = ki; i <= n; ++i) v[i - 1] = v[i - 1] * vscale;
+          v[j - 1] = vt2[0];
+          v[j] = vt2[1];
+          vmax = rmax3(absv(v[j - 1]), absv(v[j]), vmax);
+          vcrit = bignum / vmax;
+        }
+      }
+      if (!have_z) {
+        R m(0.0);
+        for (long i = ki; i <= n; ++i) m = rmax(m, absv(v[i - 1]));
+        R vscale = one / m;
+        for (long i = ki; i <= n; ++i) V(i, isel) = vscale * v[i - 1];
+      } else {
+        if (ki == n) {
+          for (long i = 1; i <= n; ++i) V(i, ki) = C(Z(i, ki));
+        } else {
+          for (long i = 1; i <= n; ++i) {
+            C s;
+            for (long k = ki; k <= n; ++k) s = s + Z(i, k) * v[k - 1];
+            V(i, ki) = s;
+          }
+          R m(0.0);
+          for (long i = 1; i <= n; ++i) m = rmax(m, absv(V(i, ki)));
+          R vscale = one / m;
+          for (long i = 1; i <= n; ++i) V(i, ki) = V(i, ki) * vscale;
+        }
+      }
+    } else {
+      if (rt<R>::abs(T(ki, ki + 1)) >= rt<R>::abs(T(ki + 1, ki))) {
+        v[ki - 1] = C(wi / T(ki, ki + 1));
+        v[ki] = C(R(0.0), one);
+      } else {
+        v[ki - 1] = C(one);
+        v[ki] = C(R(0.0), -wi / T(ki + 1, ki));
+      }
+      for (long i = ki + 2; i <= n; ++i) v[i - 1] = C(-v[ki - 1].re * T(ki, i), -(v[ki].im * T(ki + 1, i)));
+      R vmax = one, vcrit = bignum;
+      long jnxt = ki + 2;
+      for (long j = ki + 1; j <= n; ++j) {
+        if (j < jnxt) continue;
+        long j1 = j, j2 = j;
+        jnxt = j + 1;
+        if (j < n && T(j + 1, j) != R(0.0)) {
+          j1 = j + 1;  // vectors.jl:627 sets j1 (not j2); kept as in the reference
+          jnxt = j + 2;
+        }
+        if (j1 == j2) {
+          if (tnorms[j - 1] > vcrit) {
+            R t1 = one / vmax;
+            for (long i = ki; i <= n; ++i) v[i - 1] = v[i - 1] * t1;
+            vmax = one;
+            vcrit = bignum;
+          }
+          C d;
+          for (long i = ki + 2; i <= j - 1; ++i) d = d + T(i, j) * v[i - 1];
+          C vt1[1] = {v[j - 1] - d};
+          R Ae[1] = {T(j, j)};
+          R xn;
+          R vscale = xsolve<R>(one, 1, Ae, C(wr, -wi), vt1, smin, xn);
+          if (vscale != one)
+            for (long i = ki; i <= n; ++i) v[i - 1] = v[i - 1] * vscale;
+          v[j - 1] = vt1[0];
+          vmax = rmax(abs1(v[j - 1]), vmax);
+          vcrit = bignum / vmax;
+        } else {
+          // with j1 = j+1 the overflow guard reads tnorms[j-1] (vectors.jl:653)
+          R beta = rmax(tnorms[j - 2], tnorms[j - 1]);
+          if (beta > vcrit) {
+            R t1 = one / vmax;
+            for (long i = ki; i <= n; ++i) v[i - 1] = v[i - 1] * t1;
+            vmax = one;
+            vcrit = bignum;
+          }
+          C d0, d1;
+          for (long i = ki + 2; i <= j - 1; ++i) {
+            d0 = d0 + T(i, j) * v[i - 1];
+            d1 = d1 + T(i, j + 1) * v[i - 1];
+          }
+          C vt2[2] = {v[j - 1] - d0, v[j] - d1};
+          R Ae[4] = {T(j, j), T(j, j + 1), T(j + 1, j), T(j + 1, j + 1)};
+          R xn;
+          R vscale = xsolve<R>(one, 2, Ae, C(wr, -wi), vt2, smin, xn);
+          if (vscale != one)
+            for (long i = ki; i <= n; ++i) v[i - 1] = v[i - 1] * vscale;
+          v[j - 1] = vt2[0];
+          v[j] = vt2[1];
+          vmax = rmax3(abs1(vt2[0]), abs1(vt2[1]), vmax);
+          vcrit = bignum / vmax;
+        }
+      }
+      if (!have_z) {
+        for (long i = ki; i <= n; ++i) V(i, isel) = v[i - 1];
+        R t1(0.0);
+        for (long k = ki; k <= n; ++k) t1 = rmax(t1, abs1(V(k, isel)));
+        t1 = one / t1;
+        for (long i = ki; i <= n; ++i) V(i, isel) = V(i, isel) * t1;
+      } else {
+        for (long i = 1; i <= n; ++i) {
+          C s;
+          for (long k = ki; k <= n; ++k) s = s + Z(i, k) * v[k - 1];
+          V(i, ki) = s;
+        }
+        R m(0.0);
+        for (long i = 1; i <= n; ++i) m = rmax(m, absv(V(i, ki)));
+        R vscale = one / m;
+        for (long i = 1; i <= n; ++i) V(i, ki) = V(i, ki) * vscale;
+      }
+    }
+    isel += (ip == 0) ? 1 : 2;
+    if (ip == 1) ip = -1;
+  }
+}
+
+// ---------------------------------------------------- triang.jl:9-43
+// Real standard-form Schur (T, Z) -> complex triangular (Tc, Zc).
+template <class R> void triangularize(long n, const R* tr, long ldt, const R* zr, long ldz, cx<R>* tc, cx<R>* zc) {
+  typedef cx<R> C;
+  Mat<const R> Tr{tr, ldt};
+  Mat<C> T{tc, n};
+  Mat<C> Z{zc, n};
+  for (long j = 1; j <= n; ++j)
+    for (long i = 1; i <= n; ++i) {
+      T(i, j) = C(Tr(i, j));
+      Z(i, j) = C(zr[(i - 1) + (j - 1) * ldz]);
+    }
+  for (long j = n; j >= 2; --j) {
+    if (Tr(j, j - 1) != R(0.0)) {
+      R s = rt<R>::sqrt(rt<R>::abs(Tr(j, j - 1)));
+      R c = rt<R>::sqrt(rt<R>::abs(Tr(j - 1, j)));
+      R r = rt<R>::hypot(s, c);
+      C gc = C(c / r);            // Givens c (complex-typed, real valued)
+      C gs = C(R(0.0), -(s / r));  // Givens s = -im*(s/r)
+      // lmul!(G, T): rows j-1, j
+      for (long k = 1; k <= n; ++k) {
+        C a1 = T(j - 1, k), a2 = T(j, k);
+        T(j - 1, k) = gc * a1 + gs * a2;
+        T(j, k) = -conj(gs) * a1 + gc * a2;
+      }
+      // rmul!(T, G') and rmul!(Z, G'): columns j-1, j times G^H
+      for (long k = 1; k <= n; ++k) {
+        C a1 = T(k, j - 1), a2 = T(k, j);
+        T(k, j - 1) = a1 * conj(gc) + a2 * conj(gs);
+        T(k, j) = -a1 * gs + a2 * conj(gc);
+        C b1 = Z(k, j - 1), b2 = Z(k, j);
+        Z(k, j - 1) = b1 * conj(gc) + b2 * conj(gs);
+        Z(k, j) = -b1 * gs + b2 * conj(gc);
+      }
+    }
+  }
+  for (long j = 1; j <= n; ++j)
+    for (long i = j + 1; i <= n; ++i) T(i, j) = C();
+}
+
+}  // namespace orc

@@ -1,0 +1,213 @@
+// TEST INFRASTRUCTURE ONLY — C entry points of the CPU oracle (see
+// gschur_oracle.hpp).  Loaded by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py through ctypes; never by the product library.
+//
+// Complex arrays are interleaved (re, im) doubles, column-major, like Julia's
+// ComplexF64 and C `double _Complex`.
+#include <omp.h>
+
+#include <cstring>
+#include <vector>
+
+#include "gschur_oracle.hpp"
+
+using namespace orc;
+typedef cx<double> zc;
+
+extern "C" {
+
+int orc_abi_version() { return 3; }
+
+// gschur!(A::StridedMatrix{ComplexF64}) — GenericSchur.jl:350-372
+int orc_schur_z(long n, double* a, long lda, double* t, long ldt, double* z, long ldz, double* w, int wantz,
+                int scale, long maxiter, long maxinner, long* stats) {
+  return gschur<double>(n, (zc*)a, lda, (zc*)t, ldt, (zc*)z, ldz, (zc*)w, wantz != 0, scale != 0, maxiter,
+                        maxinner, stats);
+}
+
+// gschur!(A::StridedMatrix{Float64}) — GenericSchur.jl:805-835
+int orc_schur_d(long n, double* a, long lda, double* t, long ldt, double* z, long ldz, double* w, int wantz,
+                int scale, long maxiter, long* stats) {
+  return gschur<double>(n, a, lda, t, ldt, z, ldz, (zc*)w, wantz != 0, scale != 0, maxiter, 0, stats);
+}
+
+// gschur!(H::Hessenberg{ComplexF64}, Z) — GenericSchur.jl:194-335.  On exit h
+// holds triu(HH).  z may be NULL (no Schur vectors).
+int orc_schur_hess_z(long n, double* h, long ldh, double* z, long ldz, long maxiter, long maxinner, int checksd,
+                     double* w) {
+  int info = schur_hess_complex<double>(n, (zc*)h, ldh, (zc*)z, ldz, maxiter, maxinner, checksd != 0, (zc*)w);
+  if (info == ORC_OK) {
+    zc* H = (zc*)h;
+    for (long j = 0; j < n; ++j)
+      for (long i = j + 1; i < n; ++i) H[i + j * ldh] = zc();
+  }
+  return info;
+}
+
+// gschur!(H::Hessenberg{Float64}, Z) — GenericSchur.jl:513-699
+int orc_schur_hess_d(long n, double* h, long ldh, double* z, long ldz, long maxiter, double* w) {
+  return schur_hess_real<double>(n, h, ldh, z, ldz, maxiter, (zc*)w);
+}
+
+// _hessenberg! + _materializeQ — hessenberg.jl:3-17,150-166.  q may be NULL.
+void orc_hessenberg_z(long n, double* a, long lda, double* tau, double* q, long ldq) {
+  hessenberg<zc>(n, (zc*)a, lda, (zc*)tau);
+  if (q) materialize_q<zc>(n, (zc*)a, lda, (zc*)tau, (zc*)q, ldq);
+}
+void orc_hessenberg_d(long n, double* a, long lda, double* tau, double* q, long ldq) {
+  hessenberg<double>(n, a, lda, tau);
+  if (q) materialize_q<double>(n, a, lda, tau, q, ldq);
+}
+
+// _reflector! — householder.jl:12-102.  Returns tau through `tau`.
+void orc_reflector_z(long m, double* x, double* tau) {
+  zc t = reflector<double>(m, (zc*)x, 1);
+  tau[0] = t.re;
+  tau[1] = t.im;
+}
+void orc_reflector_d(long m, double* x, double* tau) { tau[0] = reflector<double>(m, x, 1); }
+
+// _gs2x2! — GenericSchur.jl:716-803.  abcd in/out; cssn = (cs, sn);
+// w = (re w1, im w1, re w2, im w2).
+void orc_gs2x2(double* abcd, double* cssn, double* w) {
+  zc w1, w2;
+  gs2x2<double>(abcd[0], abcd[1], abcd[2], abcd[3], cssn[0], cssn[1], w1, w2);
+  w[0] = w1.re;
+  w[1] = w1.im;
+  w[2] = w2.re;
+  w[3] = w2.im;
+}
+
+// geigvecs(S; left) — vectors.jl:12-20 (+ _enormalize! when normalize != 0).
+// t is modified temporarily and restored (complex case).  z may be NULL.
+void orc_eigvecs_z(long n, double* t, long ldt, const double* z, long ldz, double* v, long ldv, int left,
+                   int normalize) {
+  if (left)
+    eigvecs_left_complex<double>(n, (zc*)t, ldt, (const zc*)z, ldz, (zc*)v, ldv);
+  else
+    eigvecs_right_complex<double>(n, (zc*)t, ldt, (const zc*)z, ldz, (zc*)v, ldv);
+  if (normalize) enormalize<double>(n, (zc*)v, ldv);
+}
+void orc_eigvecs_d(long n, const double* t, long ldt, const double* z, long ldz, double* v, long ldv, int left,
+                   int normalize) {
+  if (left)
+    eigvecs_left_real<double>(n, t, ldt, z, ldz, (zc*)v, ldv);
+  else
+    eigvecs_right_real<double>(n, t, ldt, z, ldz, (zc*)v, ldv);
+  if (normalize) enormalize<double>(n, (zc*)v, ldv);
+}
+
+void orc_enormalize(long n, double* v, long ldv) { enormalize<double>(n, (zc*)v, ldv); }
+
+// triangularize — triang.jl:9-43 (tc, zc are n x n, ld = n)
+void orc_triangularize(long n, const double* t, long ldt, const double* z, long ldz, double* tc, double* zcm) {
+  triangularize<double>(n, t, ldt, z, ldz, (zc*)tc, (zc*)zcm);
+}
+
+// Batched drivers over packed column-major matrices at element offsets
+// (`off`), eigenvalues at the exclusive prefix sum of n.  OpenMP over
+// matrices with `nthreads` threads (<= 0: library default).
+int orc_schur_batched_z(long count, const int* n, const long* off, const double* A, double* T, double* Z, double* W,
+                        int* info, int nthreads, int wantz) {
+  std::vector<long> woff(count + 1, 0);
+  for (long i = 0; i < count; ++i) woff[i + 1] = woff[i] + n[i];
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : bad)
+  for (long i = 0; i < count; ++i) {
+    const long m = n[i];
+    std::vector<zc> a((size_t)m * m);
+    std::memcpy((void*)a.data(), A + 2 * off[i], sizeof(zc) * m * m);
+    info[i] = gschur<double>(m, a.data(), m, (zc*)(T + 2 * off[i]), m, (zc*)(Z + 2 * off[i]), m,
+                             (zc*)(W + 2 * woff[i]), wantz != 0, true, 100 * m, 30 * m);
+    bad += (info[i] != 0);
+  }
+  return bad;
+}
+
+int orc_schur_batched_d(long count, const int* n, const long* off, const double* A, double* T, double* Z, double* W,
+                        int* info, int nthreads, int wantz) {
+  std::vector<long> woff(count + 1, 0);
+  for (long i = 0; i < count; ++i) woff[i + 1] = woff[i] + n[i];
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : bad)
+  for (long i = 0; i < count; ++i) {
+    const long m = n[i];
+    std::vector<double> a((size_t)m * m);
+    std::memcpy((void*)a.data(), A + off[i], sizeof(double) * m * m);
+    info[i] = gschur<double>(m, a.data(), m, T + off[i], m, Z + off[i], m, (zc*)(W + 2 * woff[i]), wantz != 0, true,
+                             100 * m, 0);
+    bad += (info[i] != 0);
+  }
+  return bad;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ extended precision
+// Eigenvalues of the (exactly representable) double matrix A computed by
+// the same restated algorithm in quad-double (prec = 2: about 212 bits) or
+// __float128 (prec = 1: 113 bits).  w receives (re_hi, re_lo, im_hi, im_lo)
+// per eigenvalue.  Returns the info code.
+template <class R> static void put_hilo(const R& x, double* out) {
+  double hi = rt<R>::to_double(x);
+  R rem = x - R(rt<R>::from(hi));
+  out[0] = hi;
+  out[1] = rt<R>::to_double(rem);
+}
+
+template <class R> static int eig_hp_z(long n, const double* a, double* w) {
+  std::vector<cx<R>> A((size_t)n * n), T((size_t)n * n), W(n);
+  for (long i = 0; i < n * n; ++i) A[i] = cx<R>(rt<R>::from(a[2 * i]), rt<R>::from(a[2 * i + 1]));
+  int info = gschur<R>(n, A.data(), n, T.data(), n, nullptr, n, W.data(), false, true, 100 * n, 30 * n);
+  for (long i = 0; i < n; ++i) {
+    put_hilo(W[i].re, w + 4 * i);
+    put_hilo(W[i].im, w + 4 * i + 2);
+  }
+  return info;
+}
+template <class R> static int eig_hp_d(long n, const double* a, double* w) {
+  std::vector<R> A((size_t)n * n), T((size_t)n * n);
+  std::vector<cx<R>> W(n);
+  for (long i = 0; i < n * n; ++i) A[i] = rt<R>::from(a[i]);
+  int info = gschur<R>(n, A.data(), n, T.data(), n, nullptr, n, W.data(), false, true, 100 * n, 0);
+  for (long i = 0; i < n; ++i) {
+    put_hilo(W[i].re, w + 4 * i);
+    put_hilo(W[i].im, w + 4 * i + 2);
+  }
+  return info;
+}
+
+extern "C" {
+
+int orc_eig_hp_z(long n, const double* a, double* w, int prec) {
+  return prec >= 2 ? eig_hp_z<qd_real>(n, a, w) : eig_hp_z<__float128>(n, a, w);
+}
+int orc_eig_hp_d(long n, const double* a, double* w, int prec) {
+  return prec >= 2 ? eig_hp_d<qd_real>(n, a, w) : eig_hp_d<__float128>(n, a, w);
+}
+
+// quad-double primitives, exported so tests can pin the arithmetic itself
+void orc_qd_sqrt(const double* a, double* r) {
+  qd_real x(a[0], a[1], a[2], a[3]);
+  qd_real s = qd_sqrt(x);
+  for (int i = 0; i < 4; ++i) r[i] = s.x[i];
+}
+void orc_qd_div(const double* a, const double* b, double* r) {
+  qd_real x(a[0], a[1], a[2], a[3]), y(b[0], b[1], b[2], b[3]);
+  qd_real s = x / y;
+  for (int i = 0; i < 4; ++i) r[i] = s.x[i];
+}
+void orc_qd_mul(const double* a, const double* b, double* r) {
+  qd_real x(a[0], a[1], a[2], a[3]), y(b[0], b[1], b[2], b[3]);
+  qd_real s = x * y;
+  for (int i = 0; i < 4; ++i) r[i] = s.x[i];
+}
+void orc_qd_add(const double* a, const double* b, double* r) {
+  qd_real x(a[0], a[1], a[2], a[3]), y(b[0], b[1], b[2], b[3]);
+  qd_real s = x + y;
+  for (int i = 0; i < 4; ++i) r[i] = s.x[i];
+}
+
+}  // extern "C"

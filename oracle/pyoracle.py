@@ -1,0 +1,246 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes bindings of the CPU oracle.
+
+The oracle is a C++ restatement of GenericSchur.jl's Schur path (see
+``gschur_oracle.hpp``).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only to check
+or time against it.  Matrices are numpy arrays indexed ``A[i, j]`` like the
+reference; they are handed to C in column-major (Fortran) order.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "liboracle_gschur.so")
+_lib = None
+
+c_long = ctypes.c_long
+c_int = ctypes.c_int
+c_dp = ctypes.POINTER(ctypes.c_double)
+c_lp = ctypes.POINTER(ctypes.c_long)
+c_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIBPATH):
+            raise RuntimeError("oracle library not built: run `make -C oracle`")
+        _lib = ctypes.CDLL(_LIBPATH)
+        L = _lib
+        L.orc_schur_z.argtypes = [c_long, c_dp, c_long, c_dp, c_long, c_dp, c_long, c_dp, c_int, c_int,
+                                  c_long, c_long, c_lp]
+        L.orc_schur_d.argtypes = [c_long, c_dp, c_long, c_dp, c_long, c_dp, c_long, c_dp, c_int, c_int,
+                                  c_long, c_lp]
+        L.orc_schur_hess_z.argtypes = [c_long, c_dp, c_long, c_dp, c_long, c_long, c_long, c_int, c_dp]
+        L.orc_schur_hess_d.argtypes = [c_long, c_dp, c_long, c_dp, c_long, c_long, c_dp]
+        L.orc_hessenberg_z.argtypes = [c_long, c_dp, c_long, c_dp, c_dp, c_long]
+        L.orc_hessenberg_d.argtypes = [c_long, c_dp, c_long, c_dp, c_dp, c_long]
+        L.orc_hessenberg_z.restype = None
+        L.orc_hessenberg_d.restype = None
+        L.orc_reflector_z.argtypes = [c_long, c_dp, c_dp]
+        L.orc_reflector_d.argtypes = [c_long, c_dp, c_dp]
+        L.orc_reflector_z.restype = None
+        L.orc_reflector_d.restype = None
+        L.orc_gs2x2.argtypes = [c_dp, c_dp, c_dp]
+        L.orc_gs2x2.restype = None
+        L.orc_eigvecs_z.argtypes = [c_long, c_dp, c_long, c_dp, c_long, c_dp, c_long, c_int, c_int]
+        L.orc_eigvecs_d.argtypes = [c_long, c_dp, c_long, c_dp, c_long, c_dp, c_long, c_int, c_int]
+        L.orc_eigvecs_z.restype = None
+        L.orc_eigvecs_d.restype = None
+        L.orc_enormalize.argtypes = [c_long, c_dp, c_long]
+        L.orc_enormalize.restype = None
+        L.orc_triangularize.argtypes = [c_long, c_dp, c_long, c_dp, c_long, c_dp, c_dp]
+        L.orc_triangularize.restype = None
+        L.orc_schur_batched_z.argtypes = [c_long, c_ip, c_lp, c_dp, c_dp, c_dp, c_dp, c_ip, c_int, c_int]
+        L.orc_schur_batched_d.argtypes = [c_long, c_ip, c_lp, c_dp, c_dp, c_dp, c_dp, c_ip, c_int, c_int]
+        L.orc_eig_hp_z.argtypes = [c_long, c_dp, c_dp, c_int]
+        L.orc_eig_hp_d.argtypes = [c_long, c_dp, c_dp, c_int]
+        for f in ("orc_qd_sqrt",):
+            getattr(L, f).argtypes = [c_dp, c_dp]
+            getattr(L, f).restype = None
+        for f in ("orc_qd_div", "orc_qd_mul", "orc_qd_add"):
+            getattr(L, f).argtypes = [c_dp, c_dp, c_dp]
+            getattr(L, f).restype = None
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags.f_contiguous or a.ndim == 1
+    return a.ctypes.data_as(c_dp)
+
+
+def _fort(a, dtype):
+    return np.array(a, dtype=dtype, order="F", copy=True)
+
+
+UNCONVERGED = 1
+DIM = -1
+NONREAL_SUB = -2
+
+
+def gschur(A, wantz=True, scale=True, maxiter=None, maxinner=None):
+    """gschur!(A) (GenericSchur.jl:350-372 complex, :805-835 real).
+
+    Returns (T, Z, values, info, stats) with stats = (iterations, sweeps,
+    bulge-step bound).
+    """
+    A = np.asarray(A)
+    n = A.shape[0]
+    if A.ndim != 2 or A.shape[1] != n:
+        raise ValueError("DimensionMismatch: matrix is not square")
+    cplx = np.iscomplexobj(A)
+    dt = np.complex128 if cplx else np.float64
+    a = _fort(A, dt)
+    T = np.zeros((n, n), dtype=dt, order="F")
+    Z = np.zeros((n, n), dtype=dt, order="F")
+    w = np.zeros(n, dtype=np.complex128)
+    stats = np.zeros(3, dtype=np.int64)
+    mi = 100 * n if maxiter is None else maxiter
+    if cplx:
+        mn = 30 * n if maxinner is None else maxinner
+        info = lib().orc_schur_z(n, _p(a), n, _p(T), n, _p(Z), n, w.ctypes.data_as(c_dp), int(wantz),
+                                 int(scale), mi, mn, stats.ctypes.data_as(c_lp))
+    else:
+        info = lib().orc_schur_d(n, _p(a), n, _p(T), n, _p(Z), n, w.ctypes.data_as(c_dp), int(wantz),
+                                 int(scale), mi, stats.ctypes.data_as(c_lp))
+    return T, (Z if wantz else np.zeros((0, 0), dtype=dt)), w, info, tuple(int(s) for s in stats)
+
+
+def schur_hess(H, Z=None, maxiter=None, maxinner=None, checksd=True):
+    """gschur!(H::Hessenberg, Z) (GenericSchur.jl:194-335, :513-699)."""
+    H = np.asarray(H)
+    n = H.shape[0]
+    cplx = np.iscomplexobj(H)
+    dt = np.complex128 if cplx else np.float64
+    h = _fort(H, dt)
+    z = None if Z is None else _fort(Z, dt)
+    w = np.zeros(n, dtype=np.complex128)
+    mi = 100 * n if maxiter is None else maxiter
+    if cplx:
+        mn = 30 * n if maxinner is None else maxinner
+        info = lib().orc_schur_hess_z(n, _p(h), n, _p(z), n, mi, mn, int(checksd), w.ctypes.data_as(c_dp))
+    else:
+        info = lib().orc_schur_hess_d(n, _p(h), n, _p(z), n, mi, w.ctypes.data_as(c_dp))
+    return h, z, w, info
+
+
+def hessenberg(A):
+    """_hessenberg! + _materializeQ (hessenberg.jl:3-17,150-166).
+
+    Returns (factors, tau, Q, H) with H = triu(factors, -1).
+    """
+    A = np.asarray(A)
+    n = A.shape[0]
+    cplx = np.iscomplexobj(A)
+    dt = np.complex128 if cplx else np.float64
+    a = _fort(A, dt)
+    tau = np.zeros(max(n - 1, 1), dtype=dt)
+    Q = np.zeros((n, n), dtype=dt, order="F")
+    f = lib().orc_hessenberg_z if cplx else lib().orc_hessenberg_d
+    f(n, _p(a), n, tau.ctypes.data_as(c_dp), _p(Q), n)
+    return a, tau[: max(n - 1, 0)], Q, np.triu(a, -1)
+
+
+def reflector(x):
+    """_reflector! (householder.jl:12-102). Returns (x_out, tau)."""
+    x = np.array(x)
+    cplx = np.iscomplexobj(x)
+    dt = np.complex128 if cplx else np.float64
+    xx = np.array(x, dtype=dt)
+    tau = np.zeros(2)
+    if cplx:
+        lib().orc_reflector_z(len(xx), xx.ctypes.data_as(c_dp), tau.ctypes.data_as(c_dp))
+        return xx, complex(tau[0], tau[1])
+    lib().orc_reflector_d(len(xx), xx.ctypes.data_as(c_dp), tau.ctypes.data_as(c_dp))
+    return xx, float(tau[0])
+
+
+def gs2x2(a, b, c, d):
+    """_gs2x2! (GenericSchur.jl:716-803).  Returns ((a,b,c,d), (cs,sn), w1, w2)."""
+    abcd = np.array([a, b, c, d], dtype=np.float64)
+    cssn = np.zeros(2)
+    w = np.zeros(4)
+    lib().orc_gs2x2(abcd.ctypes.data_as(c_dp), cssn.ctypes.data_as(c_dp), w.ctypes.data_as(c_dp))
+    return tuple(abcd), tuple(cssn), complex(w[0], w[1]), complex(w[2], w[3])
+
+
+def eigvecs(T, Z=None, left=False, normalize=True):
+    """geigvecs(S; left) (vectors.jl:12-20); normalize=False skips _enormalize!."""
+    T = np.asarray(T)
+    n = T.shape[0]
+    cplx = np.iscomplexobj(T)
+    dt = np.complex128 if cplx else np.float64
+    t = _fort(T, dt)
+    z = None if Z is None else _fort(Z, dt)
+    V = np.zeros((n, n), dtype=np.complex128, order="F")
+    f = lib().orc_eigvecs_z if cplx else lib().orc_eigvecs_d
+    f(n, _p(t), n, _p(z), n, _p(V), n, int(left), int(normalize))
+    return V
+
+
+def enormalize(V):
+    v = _fort(V, np.complex128)
+    lib().orc_enormalize(v.shape[0], _p(v), v.shape[0])
+    return v
+
+
+def triangularize(T, Z):
+    """triangularize (triang.jl:9-43)."""
+    n = T.shape[0]
+    t = _fort(T, np.float64)
+    z = _fort(Z, np.float64)
+    tc = np.zeros((n, n), dtype=np.complex128, order="F")
+    zc = np.zeros((n, n), dtype=np.complex128, order="F")
+    lib().orc_triangularize(n, _p(t), n, _p(z), n, _p(tc), _p(zc))
+    return tc, zc
+
+
+def schur_batched(ns, offsets, Aflat, complex_input, nthreads=0, wantz=True):
+    """Batched gschur! over packed column-major matrices (OpenMP over matrices)."""
+    ns = np.ascontiguousarray(ns, dtype=np.int32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    count = len(ns)
+    dt = np.complex128 if complex_input else np.float64
+    A = np.ascontiguousarray(Aflat, dtype=dt)
+    T = np.zeros_like(A)
+    Z = np.zeros_like(A)
+    W = np.zeros(int(ns.sum()), dtype=np.complex128)
+    info = np.zeros(count, dtype=np.int32)
+    f = lib().orc_schur_batched_z if complex_input else lib().orc_schur_batched_d
+    f(count, ns.ctypes.data_as(c_ip), offsets.ctypes.data_as(c_lp), A.ctypes.data_as(c_dp),
+      T.ctypes.data_as(c_dp), Z.ctypes.data_as(c_dp), W.ctypes.data_as(c_dp), info.ctypes.data_as(c_ip),
+      int(nthreads), int(wantz))
+    return T, Z, W, info
+
+
+def eig_hp(A, prec=2):
+    """Eigenvalues of the double matrix A by the restated algorithm run in
+    quad-double (prec=2, ~212 bits) or binary128 (prec=1).
+
+    Returns (values as complex128 rounded, values as (re_hi, re_lo, im_hi,
+    im_lo) array, info).
+    """
+    A = np.asarray(A)
+    n = A.shape[0]
+    cplx = np.iscomplexobj(A)
+    a = _fort(A, np.complex128 if cplx else np.float64)
+    w = np.zeros((n, 4))
+    f = lib().orc_eig_hp_z if cplx else lib().orc_eig_hp_d
+    info = f(n, _p(a), w.ctypes.data_as(c_dp), int(prec))
+    vals = (w[:, 0] + w[:, 1]) + 1j * (w[:, 2] + w[:, 3])
+    return vals, w, info
+
+
+def qd_op(op, a, b=None):
+    a = np.asarray(a, dtype=np.float64)
+    r = np.zeros(4)
+    if op == "sqrt":
+        lib().orc_qd_sqrt(a.ctypes.data_as(c_dp), r.ctypes.data_as(c_dp))
+    else:
+        b = np.asarray(b, dtype=np.float64)
+        getattr(lib(), "orc_qd_" + op)(a.ctypes.data_as(c_dp), b.ctypes.data_as(c_dp), r.ctypes.data_as(c_dp))
+    return r
