@@ -124,6 +124,67 @@ template <int NT> GS_DEV double block_sum(double v, double* red) {
   return v;
 }
 
+// Largest int over the block (every thread must call; barriers when NT > 64).
+template <int NT> GS_DEV int block_max_int(int v, int* redi) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int u = __shfl_xor(v, o, 64);
+    v = u > v ? u : v;
+  }
+  if (NT > 64) {
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) redi[w] = v;
+    __syncthreads();
+    v = redi[0];
+#pragma unroll
+    for (int i = 1; i < NT / 64; ++i) v = redi[i] > v ? redi[i] : v;
+  }
+  return v;
+}
+
+template <int NT> GS_DEV void bsync() {
+  if (NT > 64) __syncthreads();
+  else __builtin_amdgcn_wave_barrier();
+}
+
+// util.jl:41-80 (xLASCL-style multiply by cto/cfrom), the multipliers in the
+// order the reference applies them.  Returns how many were written (<= 4).
+GS_DEV int safescale_muls(double cfrom, double cto, double* mul) {
+  const double smlnum = GS_FMIN;
+  const double bignum = 1.0 / smlnum;
+  double cfromc = cfrom, ctoc = cto;
+  bool done = false;
+  int k = 0;
+  while (!done && k < 64) {
+    double m;
+    const double cfrom1 = cfromc * smlnum;
+    if (cfrom1 == cfromc) {
+      m = ctoc / cfromc;
+      done = true;
+    } else {
+      const double cto1 = ctoc / bignum;
+      if (cto1 == ctoc) {
+        m = cto;
+        done = true;
+        cfromc = 1.0;
+      } else if (fabs(cfrom1) > fabs(ctoc) && ctoc != 0.0) {
+        m = smlnum;
+        cfromc = 1.0;
+      } else if (fabs(cto1) > fabs(cfromc)) {
+        m = bignum;
+        cfromc = cfrom1;
+      } else {
+        m = ctoc / cfromc;
+        done = true;
+      }
+    }
+    if (k < 4) mul[k] = m;
+    ++k;
+  }
+  return k < 4 ? k : 4;
+}
+
 // ------------------------------------------------------------ reflectors
 // Complex xLARFG on the 2-vector (a, x), householder.jl:56-102 with n = 2.
 // On exit a = beta (real), x = v2; returns tau.

@@ -7,56 +7,20 @@
 //   Hessenberg    hessenberg.jl:3-17, householder.jl:12-54,123-172, util.jl:100
 //   Q formation   hessenberg.jl:150-166
 //
-// Same structure as the complex kernel: one workgroup per matrix, H in LDS
-// (batched) or HBM (single), Z in HBM with each sweep's 3-element
-// reflectors applied afterwards as one chain per row of Z.
+// Same structure as the complex kernel: one workgroup per matrix; batched
+// matrices take a Hessenberg+Z launch with the matrix in LDS and a QR launch
+// with the band j >= i-3 of H in LDS (the double-shift bulge reaches the
+// third subdiagonal); the single-matrix kernel keeps H in HBM.  Z lives in
+// HBM with each sweep's 3-element reflectors applied afterwards as one
+// chunked chain per row of Z.
 #include "dev_common.h"
 #include "internal.h"
+#include "layout.h"
 
 namespace gs {
 
-#define HR(i, j) H[((i) - 1) + (int64_t)((j) - 1) * ld]
+#define HR(i, j) H[lay.idx((i), (j))]
 #define ZR(i, j) Zg[((i) - 1) + (int64_t)((j) - 1) * ldz]
-
-template <int NT> GS_DEV void bsync_d() {
-  if (NT > 64) __syncthreads();
-  else __builtin_amdgcn_wave_barrier();
-}
-
-GS_DEV int safescale_muls_d(double cfrom, double cto, double* mul) {
-  const double smlnum = GS_SAFEMIN;
-  const double bignum = 1.0 / smlnum;
-  double cfromc = cfrom, ctoc = cto;
-  bool done = false;
-  int k = 0;
-  while (!done && k < 64) {
-    double m;
-    const double cfrom1 = cfromc * smlnum;
-    if (cfrom1 == cfromc) {
-      m = ctoc / cfromc;
-      done = true;
-    } else {
-      const double cto1 = ctoc / bignum;
-      if (cto1 == ctoc) {
-        m = cto;
-        done = true;
-        cfromc = 1.0;
-      } else if (fabs(cfrom1) > fabs(ctoc) && ctoc != 0.0) {
-        m = smlnum;
-        cfromc = 1.0;
-      } else if (fabs(cto1) > fabs(cfromc)) {
-        m = bignum;
-        cfromc = cfrom1;
-      } else {
-        m = ctoc / cfromc;
-        done = true;
-      }
-    }
-    if (k < 4) mul[k] = m;
-    ++k;
-  }
-  return k < 4 ? k : 4;
-}
 
 // _gs2x2! (GenericSchur.jl:716-803)
 GS_DEV void gs2x2_dev(double& a, double& b, double& c, double& d, double& cs, double& sn, zd& w1, zd& w2) {
@@ -179,7 +143,7 @@ template <int NT> GS_DEV void d_hessenberg(double* H, int64_t ld, int n, double*
           bool smallb = true;
           while (smallb) {
             kount += 1;
-            bsync_d<NT>();
+            bsync<NT>();
             for (int r = r0 + tid; r < n; r += NT) H[r + p * ld] *= rsfmin;
             beta *= rsfmin;
             alpha *= rsfmin;
@@ -260,26 +224,87 @@ template <int NT> GS_DEV void d_form_q_inplace(double* H, int64_t ld, int n, con
 }
 
 // Chain application of a double-shift sweep's reflectors (steps k0..k1) to Z.
+template <int NT> GS_DEV int d_prep(const double* Ag, int64_t lda, double* H, int64_t ld, int n, int flags,
+                                    double* red, double& cscale, double& anrm) {
+  const int tid = threadIdx.x;
+  const bool scale = (flags & 0x02) != 0;
+  if ((const void*)Ag != (const void*)H) {
+    for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+      H[r + c * ld] = Ag[r + c * lda];
+    }
+  }
+  __syncthreads();
+  cscale = 0.0;
+  anrm = 0.0;
+  if (scale) {
+    double m = 0.0;
+    for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+      m = fmax(m, fabs(H[r + c * ld]));
+    }
+    const double an = block_max<NT>(m, red);
+    const double smlnum = sqrt(GS_FMIN) / GS_EPS;
+    const double bignum = 1.0 / smlnum;
+    double cs = 0.0;
+    if (an > 0.0 && an < smlnum) cs = smlnum;
+    else if (an > bignum) cs = bignum;
+    if (cs != 0.0) {
+      cscale = cs;
+      anrm = an;
+      double mul[4];
+      const int nm = safescale_muls(an, cs, mul);
+      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        double v = H[r + c * ld];
+        for (int q = 0; q < nm; ++q) v *= mul[q];
+        H[r + c * ld] = v;
+      }
+      __syncthreads();
+    }
+  }
+  return 0;
+}
+
+// Chain application of a double-shift sweep's reflectors (steps k0..k1) to
+// Z, one row per thread, the row segment loaded CH entries ahead.
 template <int NT> GS_DEV void d_apply_chain(double* Zg, int64_t ldz, int n, const double* rot, int k0, int k1) {
+  constexpr int CH = 8;
   for (int r = 1 + threadIdx.x; r <= n; r += NT) {
     double c0 = ZR(r, k0), c1 = ZR(r, k0 + 1);
-    for (int k = k0; k <= k1; ++k) {
-      const double tau1 = rot[3 * (k - k0)];
-      const double v2 = rot[3 * (k - k0) + 1];
-      const double v3 = rot[3 * (k - k0) + 2];
-      const double tau2 = tau1 * v2;
-      if (k + 2 <= k1 + 1) {  // nr == 3
-        const double tau3 = tau1 * v3;
-        const double b = ZR(r, k + 2);
-        const double ss = c0 + v2 * c1 + v3 * b;
-        ZR(r, k) = c0 - ss * tau1;
-        c0 = c1 - ss * tau2;
-        c1 = b - ss * tau3;
-      } else {  // nr == 2
-        const double ss = c0 + v2 * c1;
-        ZR(r, k) = c0 - ss * tau1;
-        c0 = c1 - ss * tau2;
-        c1 = 0.0;
+    double nxt[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q)
+      if (k0 + q + 2 <= k1 + 1) nxt[q] = ZR(r, k0 + q + 2);
+    for (int kb = k0; kb <= k1; kb += CH) {
+      double cur[CH];
+#pragma unroll
+      for (int q = 0; q < CH; ++q) cur[q] = nxt[q];
+#pragma unroll
+      for (int q = 0; q < CH; ++q)
+        if (kb + CH + q + 2 <= k1 + 1) nxt[q] = ZR(r, kb + CH + q + 2);
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        const int k = kb + q;
+        if (k <= k1) {
+          const double tau1 = rot[3 * (k - k0)];
+          const double v2 = rot[3 * (k - k0) + 1];
+          const double v3 = rot[3 * (k - k0) + 2];
+          const double tau2 = tau1 * v2;
+          if (k + 2 <= k1 + 1) {  // nr == 3
+            const double tau3 = tau1 * v3;
+            const double b = cur[q];
+            const double ss = c0 + v2 * c1 + v3 * b;
+            ZR(r, k) = c0 - ss * tau1;
+            c0 = c1 - ss * tau2;
+            c1 = b - ss * tau3;
+          } else {  // nr == 2
+            const double ss = c0 + v2 * c1;
+            ZR(r, k) = c0 - ss * tau1;
+            c0 = c1 - ss * tau2;
+            c1 = 0.0;
+          }
+        }
       }
     }
     ZR(r, k1 + 1) = c0;
@@ -287,30 +312,40 @@ template <int NT> GS_DEV void d_apply_chain(double* Zg, int64_t ldz, int n, cons
 }
 
 // doubleShiftQR! (GenericSchur.jl:837-952) with deferred Z
-template <int NT> GS_DEV void d_sweep(double* H, int64_t ld, int n, double* Zg, int64_t ldz, bool wantz, double* rot,
-                                      zd shift1, zd shift2, int istart, int iend, int64_t& nstep) {
+template <int NT, class L> GS_DEV void d_sweep(double* H, L lay, int n, double* Zg, int64_t ldz, bool wantz,
+                                               double* rot, int* redi, zd shift1, zd shift2, int istart, int iend,
+                                               int64_t& nstep) {
   const int tid = threadIdx.x;
   const double r1r = shift1.x, r1i = shift1.y, r2r = shift2.x, r2i = shift2.y;
   const double eps = GS_EPS;
-  double v0 = 0.0, v1 = 0.0, v2s = 0.0;
-  int mx = istart;
-  for (int m = iend - 2; m >= istart; --m) {
+  // start row: the lowest m in (istart, iend-2] where starting would make
+  // H[m, m-1] negligible; the first column of (H - s1)(H - s2) at that row
+  auto firstcol = [&](int m, double& v0, double& v1, double& v2) {
     double H21s = HR(m + 1, m);
     double s = fabs(HR(m, m) - r2r) + fabs(r2i) + fabs(H21s);
     H21s = H21s / s;
     v0 = H21s * HR(m, m + 1) + (HR(m, m) - r1r) * ((HR(m, m) - r2r) / s) - r1i * (r2i / s);
     v1 = H21s * (HR(m, m) + HR(m + 1, m + 1) - r1r - r2r);
-    v2s = H21s * HR(m + 2, m + 1);
-    s = fabs(v0) + fabs(v1) + fabs(v2s);
+    v2 = H21s * HR(m + 2, m + 1);
+    s = fabs(v0) + fabs(v1) + fabs(v2);
     v0 = v0 / s;
     v1 = v1 / s;
-    v2s = v2s / s;
-    if (m > istart && fabs(HR(m, m - 1)) * (fabs(v1) + fabs(v2s)) <=
-                          eps * fabs(v0) * (fabs(HR(m - 1, m - 1)) + fabs(HR(m, m)) + fabs(HR(m + 1, m + 1)))) {
-      mx = m;
+    v2 = v2 / s;
+  };
+  int found = -1;
+  for (int m = iend - 2 - tid; m >= istart + 1; m -= NT) {
+    double a0, a1, a2;
+    firstcol(m, a0, a1, a2);
+    if (fabs(HR(m, m - 1)) * (fabs(a1) + fabs(a2)) <=
+        eps * fabs(a0) * (fabs(HR(m - 1, m - 1)) + fabs(HR(m, m)) + fabs(HR(m + 1, m + 1)))) {
+      found = m;
       break;
     }
   }
+  found = block_max_int<NT>(found, redi);
+  const int mx = (found > 0) ? found : istart;
+  double v0, v1, v2s;
+  firstcol(mx, v0, v1, v2s);
   for (int k = mx; k <= iend - 1; ++k) {
     const int nr = min(3, iend - k + 1);
     if (k > mx) {
@@ -320,7 +355,7 @@ template <int NT> GS_DEV void d_sweep(double* H, int64_t ld, int n, double* Zg, 
     }
     double x1 = v1, x2 = (nr == 3) ? v2s : 0.0;
     const double tau1 = dreflector(nr, v0, x1, x2);
-    bsync_d<NT>();
+    bsync<NT>();
     if (tid == 0) {
       if (k > mx) {
         HR(k, k - 1) = v0;
@@ -379,8 +414,9 @@ template <int NT> GS_DEV void d_sweep(double* H, int64_t ld, int n, double* Zg, 
 }
 
 // gschur!(H::Hessenberg{<:AbstractFloat}, Z) (GenericSchur.jl:513-699)
-template <int NT> GS_DEV int64_t d_qr(double* H, int64_t ld, int n, double* Zg, int64_t ldz, bool wantz, double* rot,
-                                      zd* Wg, int64_t maxiter, int64_t& nsweep, int64_t& nstep) {
+template <int NT, class L> GS_DEV int64_t d_qr(double* H, L lay, int n, double* Zg, int64_t ldz, bool wantz,
+                                               double* rot, int* redi, zd* Wg, int64_t maxiter, int64_t& nsweep,
+                                               int64_t& nstep) {
   const int tid = threadIdx.x;
   const double eps = GS_EPS;
   const double smallnum = GS_FMIN * ((double)n / eps);
@@ -396,11 +432,11 @@ template <int NT> GS_DEV int64_t d_qr(double* H, int64_t ld, int n, double* Zg, 
     while (true) {
       iter += 1;
       if (iter > maxiter) return iter;
-      bool split = false;
-      for (int k = iend; k >= istart + 1; --k) {
-        if (fabs(HR(k, k - 1)) < smallnum) {
-          split = true;
-        } else {
+      // lowest negligible subdiagonal k in istart+1..iend
+      int found = -1;
+      for (int k = iend - tid; k >= istart + 1; k -= NT) {
+        bool split = fabs(HR(k, k - 1)) < smallnum;
+        if (!split) {
           const double Hkk = HR(k, k);
           const double Hkm1km1 = HR(k - 1, k - 1);
           double t = fabs(Hkm1km1) + fabs(Hkk);
@@ -416,17 +452,18 @@ template <int NT> GS_DEV int64_t d_qr(double* H, int64_t ld, int n, double* Zg, 
             const double aa = fmax(fabs(Hkk), fabs(Hkm1km1 - Hkk));
             const double bb = fmin(fabs(Hkk), fabs(Hkm1km1 - Hkk));
             const double s = aa + bb;
-            if (ba * (ab / s) <= fmax(smallnum, eps * (bb * (aa / s)))) split = true;
+            split = ba * (ab / s) <= fmax(smallnum, eps * (bb * (aa / s)));
           }
         }
         if (split) {
-          istart = k;
+          found = k;
           break;
         }
       }
-      if (!split) istart = 1;
+      found = block_max_int<NT>(found, redi);
+      istart = (found > 0) ? found : 1;
       if (istart > 1) {
-        bsync_d<NT>();
+        bsync<NT>();
         if (tid == 0) HR(istart, istart - 1) = 0.0;
         __syncthreads();
       }
@@ -485,7 +522,7 @@ template <int NT> GS_DEV int64_t d_qr(double* H, int64_t ld, int n, double* Zg, 
         }
       }
       nsweep += 1;
-      d_sweep<NT>(H, ld, n, Zg, ldz, wantz, rot, zmk(r1r, r1i), zmk(r2r, r2i), istart, iend, nstep);
+      d_sweep<NT>(H, lay, n, Zg, ldz, wantz, rot, redi, zmk(r1r, r1i), zmk(r2r, r2i), istart, iend, nstep);
     }
     if (deflate && istart >= iend) {
       if (tid == 0) Wg[iwcur - 1] = zr(HR(iend, iend));
@@ -501,7 +538,7 @@ template <int NT> GS_DEV int64_t d_qr(double* H, int64_t ld, int n, double* Zg, 
         Wg[iwcur - 2] = w1;
       }
       iwcur -= 2;
-      bsync_d<NT>();
+      bsync<NT>();
       // lmul!(G, HH[:, istart:n]) with G = Givens(iend-1, iend, cs, sn)
       for (int j = istart + tid; j <= n; j += NT) {
         const double a1 = HR(iend - 1, j), a2 = HR(iend, j);
@@ -537,171 +574,207 @@ template <int NT> GS_DEV int64_t d_qr(double* H, int64_t ld, int n, double* Zg, 
   return 0;
 }
 
-template <int NT> GS_DEV void d_driver(const double* Ag, int64_t lda, double* H, int64_t ld, int n, double* Tg,
-                                       int64_t ldt, double* Zg, int64_t ldz, zd* Wg, double* tmp, int64_t ldtmp,
-                                       double* tau, double* rot, double* red, int flags, int64_t maxiter,
-                                       int32_t* info, int64_t* stats) {
-  const int tid = threadIdx.x;
-  const bool wantz = (flags & 0x01) != 0;
-  const bool scale = (flags & 0x02) != 0;
-  const bool hessin = (flags & 0x04) != 0;
-  if (maxiter <= 0) maxiter = 100 * (int64_t)n;
-  if ((const void*)Ag != (const void*)H) {
-    for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
-      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-      H[r + c * ld] = Ag[r + c * lda];
-    }
-  }
-  __syncthreads();
-  bool scaled = false;
-  double cscale = 1.0, anrm = 0.0;
-  if (scale) {
-    double m = 0.0;
-    for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
-      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-      m = fmax(m, fabs(H[r + c * ld]));
-    }
-    anrm = block_max<NT>(m, red);
-    const double smlnum = sqrt(GS_SAFEMIN) / GS_EPS;
-    const double bignum = 1.0 / smlnum;
-    if (anrm > 0.0 && anrm < smlnum) { scaled = true; cscale = smlnum; }
-    else if (anrm > bignum) { scaled = true; cscale = bignum; }
-    if (scaled) {
-      double mul[4];
-      const int nm = safescale_muls_d(anrm, cscale, mul);
-      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
-        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-        double v = H[r + c * ld];
-        for (int q = 0; q < nm; ++q) v *= mul[q];
-        H[r + c * ld] = v;
-      }
-      __syncthreads();
-    }
-  }
-  if (!hessin) {
-    d_hessenberg<NT>(H, ld, n, tau, red);
-    if (wantz) {
-      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
-        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-        tmp[r + c * ldtmp] = H[r + c * ld];
-      }
-      __syncthreads();
-      d_form_q_inplace<NT>(H, ld, n, tau);
-      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
-        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-        const double q = H[r + c * ld];
-        const double h = tmp[r + c * ldtmp];
-        Zg[r + c * ldz] = q;
-        H[r + c * ld] = h;
-      }
-      __syncthreads();
-    }
-  }
-  // triu!(HH, -1) (GenericSchur.jl:529)
-  for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
-    const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-    if (r > c + 1) H[r + c * ld] = 0.0;
-  }
-  __syncthreads();
-  int64_t nsweep = 0, nstep = 0;
-  const int64_t st = d_qr<NT>(H, ld, n, Zg, ldz, wantz, rot, Wg, maxiter, nsweep, nstep);
-  __syncthreads();
-  if (stats && tid == 0) {
-    stats[0] = nsweep;
-    stats[1] = nstep;
-  }
-  if (st != 0) {
-    if (tid == 0) *info = (int32_t)(st > 0x7fffffff ? 0x7fffffff : st);
-    return;
-  }
+// TT = triu(HH, -1) (GenericSchur.jl:697); unscale T and values with
+// safescale! when A was scaled (GenericSchur.jl:830-833).
+template <int NT, class L> GS_DEV void d_finish(const double* H, L lay, int n, double* Tg, int64_t ldt, zd* Wg,
+                                                double cscale, double anrm) {
   double umul[4];
   int nu = 0;
-  if (scaled) nu = safescale_muls_d(cscale, anrm, umul);
-  for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+  if (cscale != 0.0) nu = safescale_muls(cscale, anrm, umul);
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
     const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-    double v = (r > c + 1) ? 0.0 : H[r + c * ld];
+    double v = (r > c + 1) ? 0.0 : H[lay.idx(r + 1, c + 1)];
     for (int q = 0; q < nu; ++q) v *= umul[q];
     Tg[r + c * ldt] = v;
   }
-  if (scaled) {
-    for (int r = tid; r < n; r += NT) {
+  if (nu > 0) {
+    for (int r = threadIdx.x; r < n; r += NT) {
       zd w = Wg[r];
       for (int q = 0; q < nu; ++q) w = w * umul[q];
       Wg[r] = w;
     }
   }
-  if (tid == 0) *info = 0;
 }
 
-template <int NT> __global__ void __launch_bounds__(NT) dschur_lds_kernel(BatchArgs a) {
+template <int NT> __global__ void __launch_bounds__(NT) dhq_kernel(BatchArgs a) {
   extern __shared__ double smem[];
   const int b = blockIdx.x;
   if (b >= a.count) return;
   const int m = a.order[a.first + b];
   const int n = a.n[m];
-  const int64_t ld = (int64_t)(n | 1);
-  double* H = smem;
-  double* tau = H + ld * n;
-  double* rot = tau + (n > 0 ? n : 1);
-  double* red = rot + 3 * (n > 0 ? n : 1);
   const int64_t o = a.off[m];
-  const double* A = a.A + o;
-  double* T = a.T + o;
-  double* Z = a.Z + o;
-  zd* W = reinterpret_cast<zd*>(a.W) + a.woff[m];
   if (n <= 0) {
     if (threadIdx.x == 0) a.info[m] = (n == 0) ? 0 : -1;
     return;
   }
-  d_driver<NT>(A, n, H, ld, n, T, n, Z, n, W, T, n, tau, rot, red, a.flags, a.maxiter, a.info + m,
-               a.stats ? a.stats + 2 * m : nullptr);
+  const int64_t ld = (int64_t)(n | 1);
+  double* H = smem;
+  double* tau = H + ld * n;
+  double* red = tau + n;
+  const double* A = a.A + o;
+  double* T = a.T + o;
+  double* Z = a.Z + o;
+  const bool wantz = (a.flags & 0x01) != 0;
+  const bool hessin = (a.flags & 0x04) != 0;
+  double cscale, anrm;
+  d_prep<NT>(A, n, H, ld, n, a.flags, red, cscale, anrm);
+  if (threadIdx.x == 0) {
+    a.info[m] = 0;
+    a.scl[2 * m] = cscale;
+    a.scl[2 * m + 1] = anrm;
+  }
+  if (!hessin) d_hessenberg<NT>(H, ld, n, tau, red);
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+    const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+    T[e] = (r > c + 1) ? 0.0 : H[r + c * ld];
+  }
+  if (wantz && !hessin) {
+    __syncthreads();
+    d_form_q_inplace<NT>(H, ld, n, tau);
+    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+      Z[e] = H[r + c * ld];
+    }
+  }
+}
+
+template <int NT> __global__ void __launch_bounds__(NT) dqr_kernel(BatchArgs a) {
+  extern __shared__ double smem[];
+  const int b = blockIdx.x;
+  if (b >= a.count) return;
+  const int m = a.order[a.first + b];
+  const int n = a.n[m];
+  if (n <= 0) return;
+  if (a.info[m] != 0) return;
+  const int64_t o = a.off[m];
+  BandLayout<3> lay;
+  lay.init(n);
+  double* H = smem;
+  double* rot = H + BandLayout<3>::size(n);
+  int* redi = reinterpret_cast<int*>(rot + 3 * n);
+  double* T = a.T + o;
+  double* Z = a.Z + o;
+  zd* W = reinterpret_cast<zd*>(a.W) + a.woff[m];
+  const bool wantz = (a.flags & 0x01) != 0;
+  for (int j = 1; j <= n; ++j) {
+    const int len = min(j + 3, n);
+    const int64_t cb = lay.col(j);
+    for (int i = 1 + threadIdx.x; i <= len; i += NT) H[cb + i - 1] = T[(i - 1) + (int64_t)(j - 1) * n];
+  }
+  __syncthreads();
+  const int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
+  int64_t nsweep = 0, nstep = 0;
+  const int64_t st = d_qr<NT>(H, lay, n, Z, (int64_t)n, wantz, rot, redi, W, maxiter, nsweep, nstep);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (a.stats) {
+      a.stats[2 * m] = nsweep;
+      a.stats[2 * m + 1] = nstep;
+    }
+    a.info[m] = (int32_t)(st > 0x7fffffff ? 0x7fffffff : st);
+  }
+  if (st != 0) return;
+  d_finish<NT>(H, lay, n, T, (int64_t)n, W, a.scl[2 * m], a.scl[2 * m + 1]);
 }
 
 template <int NT> __global__ void __launch_bounds__(NT) dschur_global_kernel(SingleArgs a) {
   __shared__ double red[NT / 64 + 1];
+  __shared__ int redi[NT / 64 + 1];
   const int n = (int)a.n;
   double* tau = a.ws;
   double* rot = tau + n;
+  double* T = a.T;
+  double* Z = a.Z;
+  const int64_t ldt = a.ldt, ldz = a.ldz;
+  const bool wantz = (a.flags & 0x01) != 0;
+  const bool hessin = (a.flags & 0x04) != 0;
+  double cscale, anrm;
+  d_prep<NT>(a.A, a.lda, T, ldt, n, a.flags, red, cscale, anrm);
+  if (!hessin) {
+    d_hessenberg<NT>(T, ldt, n, tau, red);
+    if (wantz) {
+      for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        Z[r + c * ldz] = T[r + c * ldt];
+      }
+      __syncthreads();
+      d_form_q_inplace<NT>(T, ldt, n, tau);
+      for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        const double q = T[r + c * ldt];
+        const double h = Z[r + c * ldz];
+        Z[r + c * ldz] = q;
+        T[r + c * ldt] = h;
+      }
+      __syncthreads();
+    }
+  }
   for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
     const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-    a.T[r + c * a.ldt] = a.A[r + c * a.lda];
+    if (r > c + 1) T[r + c * ldt] = 0.0;
   }
   __syncthreads();
-  d_driver<NT>(a.T, a.ldt, a.T, a.ldt, n, a.T, a.ldt, a.Z, a.ldz, reinterpret_cast<zd*>(a.W), a.Z, a.ldz, tau,
-               rot, red, a.flags, a.maxiter, a.info, a.stats);
+  FullLayout lay{ldt};
+  const int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
+  int64_t nsweep = 0, nstep = 0;
+  zd* W = reinterpret_cast<zd*>(a.W);
+  const int64_t st = d_qr<NT>(T, lay, n, Z, ldz, wantz, rot, redi, W, maxiter, nsweep, nstep);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (a.stats) {
+      a.stats[0] = nsweep;
+      a.stats[1] = nstep;
+    }
+    *a.info = (int32_t)(st > 0x7fffffff ? 0x7fffffff : st);
+  }
+  if (st != 0) return;
+  d_finish<NT>(T, lay, n, T, ldt, W, cscale, anrm);
 }
 
-size_t dschur_lds_bytes(int n, int nt) {
+size_t dhq_lds_bytes(int n, int nt) {
   const size_t ld = (size_t)(n | 1);
-  return 8 * (ld * n + 4 * (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
+  return 8 * (ld * n + (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
+}
+size_t dqr_lds_bytes(int n, int nt) {
+  return 8 * ((size_t)BandLayout<3>::size(n) + 3 * (size_t)(n > 0 ? n : 1)) + 4 * (size_t)(nt / 64 + 1);
 }
 
 int dschur_max_lds_n() {
   int n = 1;
-  while (dschur_lds_bytes(n + 1, 256) <= 160 * 1024) ++n;
+  while (dhq_lds_bytes(n + 1, 256) <= 160 * 1024) ++n;
   return n;
 }
 
-hipError_t launch_dschur_lds(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
-  const size_t lds = dschur_lds_bytes(nmax, nt);
-  if (a.count <= 0) return hipSuccess;
-  switch (nt) {
-    case 64:
-      (void)hipFuncSetAttribute((const void*)dschur_lds_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL(dschur_lds_kernel<64>, dim3(a.count), dim3(64), lds, s, a);
-      break;
-    case 128:
-      (void)hipFuncSetAttribute((const void*)dschur_lds_kernel<128>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL(dschur_lds_kernel<128>, dim3(a.count), dim3(128), lds, s, a);
-      break;
-    case 256:
-      (void)hipFuncSetAttribute((const void*)dschur_lds_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL(dschur_lds_kernel<256>, dim3(a.count), dim3(256), lds, s, a);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
+template <int NT> static hipError_t launch_dhq_nt(const BatchArgs& a, size_t lds, hipStream_t s) {
+  (void)hipFuncSetAttribute((const void*)dhq_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(dhq_kernel<NT>, dim3(a.count), dim3(NT), lds, s, a);
   return hipGetLastError();
+}
+template <int NT> static hipError_t launch_dqr_nt(const BatchArgs& a, size_t lds, hipStream_t s) {
+  (void)hipFuncSetAttribute((const void*)dqr_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(dqr_kernel<NT>, dim3(a.count), dim3(NT), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_dhq(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
+  if (a.count <= 0) return hipSuccess;
+  const size_t lds = dhq_lds_bytes(nmax, nt);
+  switch (nt) {
+    case 64: return launch_dhq_nt<64>(a, lds, s);
+    case 128: return launch_dhq_nt<128>(a, lds, s);
+    case 256: return launch_dhq_nt<256>(a, lds, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_dqr(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
+  if (a.count <= 0) return hipSuccess;
+  const size_t lds = dqr_lds_bytes(nmax, nt);
+  switch (nt) {
+    case 64: return launch_dqr_nt<64>(a, lds, s);
+    case 128: return launch_dqr_nt<128>(a, lds, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_dschur_global(const SingleArgs& a, int nt, hipStream_t s) {

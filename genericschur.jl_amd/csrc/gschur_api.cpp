@@ -49,8 +49,13 @@ struct Bin {
   int first;  // index into the order array
   int count;
   int nmax;
-  int nt;
+  int nt_hq;  // threads per workgroup of the Hessenberg+Z launch
+  int nt_qr;  // threads per workgroup of the QR launch
+  int branch; // graph branch (stream) the bin runs on
+  double work;
 };
+
+constexpr int kBranches = 4;
 
 struct Plan {
   bool is_complex = false;
@@ -63,7 +68,7 @@ struct Plan {
   int64_t total_elems = 0;  // elements spanned by A/T/Z (max off + n^2)
   int64_t total_w = 0;
   // device metadata
-  DevBuf d_order, d_n, d_off, d_woff, d_stats;
+  DevBuf d_order, d_n, d_off, d_woff, d_stats, d_scl;
   // captured graph for device-pointer execution
   hipGraphExec_t gexec = nullptr;
   const void* g_ptrs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -75,6 +80,7 @@ struct Plan {
     d_off.release();
     d_woff.release();
     d_stats.release();
+    d_scl.release();
   }
 };
 
@@ -82,10 +88,13 @@ struct Plan {
 
 struct gs_handle {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // stream in use
+  hipStream_t own_stream = nullptr;  // stream created with the handle
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipStream_t branch[kBranches] = {};
+  hipEvent_t evfork = nullptr, evjoin[kBranches] = {};
   std::string err;
-  DevBuf bA, bT, bZ, bW, bInfo, bWs, bStats, bMeta;
+  DevBuf bA, bT, bZ, bW, bInfo, bWs, bStats, bMeta, bScl;
   std::map<std::string, std::unique_ptr<Plan>> plans;
   Plan* last_plan = nullptr;
   int64_t last_stats[3] = {0, 0, 0};
@@ -129,9 +138,15 @@ int gs_create(gs_handle** out, int device) {
   if (e != hipSuccess) return GS_E_HIP - (int)e;
   gs_handle* h = new gs_handle();
   h->device = device;
-  e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  e = hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking);
+  h->stream = h->own_stream;
   if (e == hipSuccess) e = hipEventCreate(&h->ev0);
   if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&h->evfork, hipEventDisableTiming);
+  for (int i = 0; i < kBranches && e == hipSuccess; ++i) {
+    e = hipStreamCreateWithFlags(&h->branch[i], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->evjoin[i], hipEventDisableTiming);
+  }
   if (e != hipSuccess) {
     delete h;
     return GS_E_HIP - (int)e;
@@ -153,9 +168,15 @@ int gs_destroy(gs_handle* h) {
   h->bWs.release();
   h->bStats.release();
   h->bMeta.release();
+  h->bScl.release();
+  for (int i = 0; i < kBranches; ++i) {
+    if (h->evjoin[i]) (void)hipEventDestroy(h->evjoin[i]);
+    if (h->branch[i]) (void)hipStreamDestroy(h->branch[i]);
+  }
+  if (h->evfork) (void)hipEventDestroy(h->evfork);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
   return GS_OK;
 }
@@ -168,12 +189,23 @@ int gs_synchronize(gs_handle* h) {
   return GS_OK;
 }
 
+int gs_set_stream(gs_handle* h, void* stream) {
+  if (!h) return GS_E_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->stream = stream ? static_cast<hipStream_t>(stream) : h->own_stream;
+  return GS_OK;
+}
+
 int gs_batched_max_n(int is_complex) { return is_complex ? zschur_max_lds_n() : dschur_max_lds_n(); }
 
 }  // extern "C"
 
 // ------------------------------------------------------------------ plans
-static int default_nt(bool cplx, int nmax) {
+static int default_nt_hq(bool cplx, int nmax) {
+  (void)cplx;
+  return nmax > 32 ? 256 : 64;
+}
+static int default_nt_qr(bool cplx, int nmax) {
   (void)cplx;
   (void)nmax;
   return 64;
@@ -230,15 +262,34 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
     bin.first = (int)p->order.size();
     bin.count = (int)v.size();
     bin.nmax = std::max(1, n[v.front()]);
-    bin.nt = default_nt(cplx, bin.nmax);
+    bin.nt_hq = default_nt_hq(cplx, bin.nmax);
+    bin.nt_qr = default_nt_qr(cplx, bin.nmax);
+    bin.work = 0;
+    for (int32_t i : v) bin.work += (double)n[i] * n[i] * n[i];
+    bin.branch = 0;
     p->bins.push_back(bin);
     p->order.insert(p->order.end(), v.begin(), v.end());
+  }
+  // bins to graph branches, heaviest first onto the least loaded branch
+  {
+    std::vector<int> idx(p->bins.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return p->bins[x].work > p->bins[y].work; });
+    double load[kBranches] = {0, 0, 0, 0};
+    for (int i : idx) {
+      int best = 0;
+      for (int b = 1; b < kBranches; ++b)
+        if (load[b] < load[best]) best = b;
+      p->bins[i].branch = best;
+      load[best] += p->bins[i].work;
+    }
   }
   HIPCHK(h, p->d_order.ensure(sizeof(int32_t) * std::max<int64_t>(count, 1)));
   HIPCHK(h, p->d_n.ensure(sizeof(int32_t) * std::max<int64_t>(count, 1)));
   HIPCHK(h, p->d_off.ensure(sizeof(int64_t) * std::max<int64_t>(count, 1)));
   HIPCHK(h, p->d_woff.ensure(sizeof(int64_t) * std::max<int64_t>(count, 1)));
   HIPCHK(h, p->d_stats.ensure(sizeof(int64_t) * 2 * std::max<int64_t>(count, 1)));
+  HIPCHK(h, p->d_scl.ensure(sizeof(double) * 2 * std::max<int64_t>(count, 1)));
   if (count > 0) {
     HIPCHK(h, hipMemcpy(p->d_order.p, p->order.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(p->d_n.p, p->n.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice));
@@ -265,15 +316,32 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
   a.W = W;
   a.info = info;
   a.stats = static_cast<int64_t*>(p->d_stats.p);
+  a.scl = static_cast<double*>(p->d_scl.p);
   a.flags = flags;
   a.maxiter = 0;
   a.maxinner = 0;
+  // fork: every bin runs its Hessenberg launch then its QR launch on one of
+  // kBranches streams, so the LDS left free by a large bin's workgroups is
+  // filled by the smaller bins' workgroups
+  hipError_t e = hipEventRecord(h->evfork, h->stream);
+  if (e != hipSuccess) return e;
+  bool used[kBranches] = {false, false, false, false};
+  for (const Bin& b : p->bins) used[b.branch] = true;
+  for (int i = 0; i < kBranches; ++i)
+    if (used[i] && (e = hipStreamWaitEvent(h->branch[i], h->evfork, 0)) != hipSuccess) return e;
   for (const Bin& b : p->bins) {
     a.first = b.first;
     a.count = b.count;
-    hipError_t e = p->is_complex ? launch_zschur_lds(a, b.nmax, b.nt, h->stream)
-                                 : launch_dschur_lds(a, b.nmax, b.nt, h->stream);
+    hipStream_t s = h->branch[b.branch];
+    e = p->is_complex ? launch_zhq(a, b.nmax, b.nt_hq, s) : launch_dhq(a, b.nmax, b.nt_hq, s);
     if (e != hipSuccess) return e;
+    e = p->is_complex ? launch_zqr(a, b.nmax, b.nt_qr, s) : launch_dqr(a, b.nmax, b.nt_qr, s);
+    if (e != hipSuccess) return e;
+  }
+  for (int i = 0; i < kBranches; ++i) {
+    if (!used[i]) continue;
+    if ((e = hipEventRecord(h->evjoin[i], h->branch[i])) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(h->stream, h->evjoin[i], 0)) != hipSuccess) return e;
   }
   return hipSuccess;
 }
@@ -402,6 +470,7 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
   const size_t nn = (size_t)n * n;
   HIPCHK(h, h->bInfo.ensure(sizeof(int32_t)));
   HIPCHK(h, h->bStats.ensure(2 * sizeof(int64_t)));
+  HIPCHK(h, h->bScl.ensure(2 * sizeof(double)));
   HIPCHK(h, h->bWs.ensure(es * 4 * (size_t)n + 64));
   if (dev) {
     dA = A;
@@ -456,10 +525,14 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
     a.W = dW;
     a.info = dInfo;
     a.stats = static_cast<int64_t*>(h->bStats.p);
+    a.scl = static_cast<double*>(h->bScl.p);
     a.flags = kflags;
     a.maxiter = maxiter;
     a.maxinner = maxinner;
-    le = cplx ? launch_zschur_lds(a, (int)n, 64, h->stream) : launch_dschur_lds(a, (int)n, 64, h->stream);
+    const int nthq = default_nt_hq(cplx, (int)n), ntqr = default_nt_qr(cplx, (int)n);
+    le = cplx ? launch_zhq(a, (int)n, nthq, h->stream) : launch_dhq(a, (int)n, nthq, h->stream);
+    if (le == hipSuccess)
+      le = cplx ? launch_zqr(a, (int)n, ntqr, h->stream) : launch_dqr(a, (int)n, ntqr, h->stream);
   } else {
     SingleArgs a;
     a.n = n;

@@ -23,6 +23,7 @@ struct BatchArgs {
   double* W;
   int32_t* info;
   int64_t* stats;  // 2 per matrix (sweeps, bulge steps) or null
+  double* scl;     // 2 per matrix: (cscale, anrm) of _scale!, or (0, 0)
   int flags;
   int64_t maxiter;   // <= 0: 100 n
   int64_t maxinner;  // <= 0: 30 n
@@ -38,7 +39,7 @@ struct SingleArgs {
   double* Z;
   int64_t ldz;
   double* W;
-  double* ws;  // device scratch: >= 4 n doubles (complex) / 4 n (real)
+  double* ws;  // device scratch: >= 4 n complex / 4 n real elements
   int32_t* info;
   int64_t* stats;
   int flags;
@@ -46,14 +47,20 @@ struct SingleArgs {
   int64_t maxinner;
 };
 
-// LDS bytes needed by the batched kernels for order n with NT threads.
-size_t zschur_lds_bytes(int n, int nt);
-size_t dschur_lds_bytes(int n, int nt);
+// Batched kernels are two launches per size bin: the Hessenberg reduction
+// and Q formation with the whole matrix in LDS ("hq"), then the QR sweeps
+// with only H's band in LDS ("qr").
+size_t zhq_lds_bytes(int n, int nt);
+size_t zqr_lds_bytes(int n, int nt);
+size_t dhq_lds_bytes(int n, int nt);
+size_t dqr_lds_bytes(int n, int nt);
 int zschur_max_lds_n();
 int dschur_max_lds_n();
 
-hipError_t launch_zschur_lds(const BatchArgs& a, int nmax, int nt, hipStream_t s);
-hipError_t launch_dschur_lds(const BatchArgs& a, int nmax, int nt, hipStream_t s);
+hipError_t launch_zhq(const BatchArgs& a, int nmax, int nt, hipStream_t s);
+hipError_t launch_zqr(const BatchArgs& a, int nmax, int nt, hipStream_t s);
+hipError_t launch_dhq(const BatchArgs& a, int nmax, int nt, hipStream_t s);
+hipError_t launch_dqr(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 hipError_t launch_zschur_global(const SingleArgs& a, int nt, hipStream_t s);
 hipError_t launch_dschur_global(const SingleArgs& a, int nt, hipStream_t s);
 

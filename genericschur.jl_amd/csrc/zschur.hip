@@ -8,70 +8,32 @@
 //   Q formation   hessenberg.jl:150-166
 //   scaling       util.jl:14-80
 //
-// One workgroup owns one matrix.  The working matrix H is either resident
-// in LDS (batched kernel, n <= zschur_max_lds_n()) or in HBM (single-matrix
-// kernel, any n); the same core serves both through a plain pointer whose
-// address space the compiler infers after inlining.  Z always lives in HBM:
-// the rotations of a sweep are logged in LDS and applied afterwards as one
-// chain per row of Z (load Z(r,k+1), rotate with the carried Z(r,k), store),
-// which keeps the sequential bulge chase free of HBM traffic.
+// One workgroup owns one matrix.  Batched matrices take two launches per
+// size bin: zhq_kernel reduces A to Hessenberg form and forms Z with the
+// whole matrix resident in LDS, then zqr_kernel runs the QR iterations with
+// only the band j >= i-2 of H resident in LDS (half the footprint, so twice
+// the resident matrices per CU).  The single-matrix kernel keeps H in HBM.
+//
+// Z always lives in HBM: the rotations of a sweep are logged in LDS and
+// applied afterwards as one chain per row of Z, with the row segment loaded
+// in chunks so that a chunk's loads are in flight together.
 //
 // Every thread runs the scalar control logic (deflation tests, shifts,
-// reflectors) redundantly on values read from H; reductions are butterflies,
-// so all threads take the same branches.
+// reflectors) redundantly on values read from H; searches are parallel over
+// threads followed by a butterfly maximum, so all threads take the same
+// branches.
 #include "dev_common.h"
 #include "internal.h"
+#include "layout.h"
 
 namespace gs {
 
-#define HH(i, j) H[((i) - 1) + (int64_t)((j) - 1) * ld]
+#define HH(i, j) H[lay.idx((i), (j))]
 #define ZZ(i, j) Zg[((i) - 1) + (int64_t)((j) - 1) * ldz]
 
-template <int NT> GS_DEV void bsync() {
-  if (NT > 64) __syncthreads();
-  else __builtin_amdgcn_wave_barrier();
-}
-
-// util.jl:41-80 (xLASCL-style ratio multiply), the multipliers in order.
-// Returns the number of multipliers written to mul[0..3].
-GS_DEV int safescale_muls(double cfrom, double cto, double* mul) {
-  const double smlnum = GS_SAFEMIN;
-  const double bignum = 1.0 / smlnum;
-  double cfromc = cfrom, ctoc = cto;
-  bool done = false;
-  int k = 0;
-  while (!done && k < 64) {
-    double m;
-    const double cfrom1 = cfromc * smlnum;
-    if (cfrom1 == cfromc) {
-      m = ctoc / cfromc;
-      done = true;
-    } else {
-      const double cto1 = ctoc / bignum;
-      if (cto1 == ctoc) {
-        m = cto;
-        done = true;
-        cfromc = 1.0;
-      } else if (fabs(cfrom1) > fabs(ctoc) && ctoc != 0.0) {
-        m = smlnum;
-        cfromc = 1.0;
-      } else if (fabs(cto1) > fabs(cfromc)) {
-        m = bignum;
-        cfromc = cfrom1;
-      } else {
-        m = ctoc / cfromc;
-        done = true;
-      }
-    }
-    if (k < 4) mul[k] = m;
-    ++k;
-  }
-  return k < 4 ? k : 4;
-}
-
 // ------------------------------------------------------------------------
-// Hessenberg reduction in place (hessenberg.jl:3-17).  Reflector p (0-based
-// column) is stored below the subdiagonal of column p, tau[p] its scalar.
+// Hessenberg reduction in place (hessenberg.jl:3-17), full layout, 0-based.
+// Reflector p is stored below the subdiagonal of column p, tau[p] its scalar.
 template <int NT> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n, zd* tau, double* red) {
   const int tid = threadIdx.x;
   const double sfmin = GS_FMIN / GS_EPS;
@@ -201,25 +163,97 @@ template <int NT> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, int n, const z
 }
 
 // ------------------------------------------------------------------------
-// Deferred application of one sweep's rotations to Z (columns k0..k1+1),
-// as a chain per row; `ph` is the phase of the shifted-start scaling
-// (applied to columns k0 and k0+2..k1+1), `tl` the tail phase of column k1+1.
-template <int NT> GS_DEV void z_apply_chain(zd* Zg, int64_t ldz, int n, const zd* rot, int k0, int k1,
-                                            bool has_ph, zd ph, bool has_tl, zd tl) {
+// Load (full layout, leading dimension ld), optional subdiagonal check and
+// _scale! (util.jl:14-29; norm(A, Inf) is the largest entry modulus).
+// Returns -2 for a non-real subdiagonal (checksd), else 0; scl receives
+// (cscale, anrm) when A was scaled, (0, 0) otherwise.
+template <int NT> GS_DEV int z_prep(const zd* Ag, int64_t lda, zd* H, int64_t ld, int n, int flags, double* red,
+                                    double& cscale, double& anrm) {
+  const int tid = threadIdx.x;
+  const bool scale = (flags & 0x02) != 0;
+  const bool hessin = (flags & 0x04) != 0;
+  const bool checksd = (flags & 0x08) != 0;
+  if ((const void*)Ag != (const void*)H) {
+    for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+      H[r + c * ld] = Ag[r + c * lda];
+    }
+  }
+  __syncthreads();
+  cscale = 0.0;
+  anrm = 0.0;
+  if (hessin && checksd) {
+    bool bad = false;
+    for (int j = 1; j <= n - 1; ++j) bad |= (H[j + (int64_t)(j - 1) * ld].y != 0.0);
+    if (bad) return -2;
+  }
+  if (scale) {
+    double m = 0.0;
+    for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+      m = fmax(m, zabs(H[r + c * ld]));
+    }
+    const double an = block_max<NT>(m, red);
+    const double smlnum = sqrt(GS_FMIN) / GS_EPS;
+    const double bignum = 1.0 / smlnum;
+    double cs = 0.0;
+    if (an > 0.0 && an < smlnum) cs = smlnum;
+    else if (an > bignum) cs = bignum;
+    if (cs != 0.0) {
+      cscale = cs;
+      anrm = an;
+      double mul[4];
+      const int nm = safescale_muls(an, cs, mul);
+      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        zd v = H[r + c * ld];
+        for (int q = 0; q < nm; ++q) v = v * mul[q];
+        H[r + c * ld] = v;
+      }
+      __syncthreads();
+    }
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------------
+// Deferred application of one sweep's rotations to Z (columns k0..k1+1), as
+// one chain per row; `ph` is the phase of the shifted-start scaling (applied
+// to columns k0 and k0+2..k1+1), `tl` the tail phase of column k1+1.  The row
+// segment is loaded CH entries at a time, one chunk ahead of the rotations.
+template <int NT> GS_DEV void z_apply_chain(zd* Zg, int64_t ldz, int n, const zd* rot, int k0, int k1, bool has_ph,
+                                            zd ph, bool has_tl, zd tl) {
+  constexpr int CH = 8;
   const zd cph = zconj(ph);
   for (int r = 1 + threadIdx.x; r <= n; r += NT) {
     zd carry = ZZ(r, k0);
-    for (int k = k0; k <= k1; ++k) {
-      const zd tau1 = rot[2 * (k - k0)];
-      const zd v2 = rot[2 * (k - k0) + 1];
-      const double tau2 = (tau1 * v2).x;
-      zd b = ZZ(r, k + 1);
-      if (has_ph && k + 1 >= k0 + 2) b = b * cph;
-      const zd ss = tau1 * carry + tau2 * b;
-      zd znew = carry - ss;
-      if (has_ph && k == k0) znew = znew * cph;
-      ZZ(r, k) = znew;
-      carry = b - ss * zconj(v2);
+    zd nxt[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q)
+      if (k0 + q <= k1) nxt[q] = ZZ(r, k0 + q + 1);
+    for (int kb = k0; kb <= k1; kb += CH) {
+      zd cur[CH];
+#pragma unroll
+      for (int q = 0; q < CH; ++q) cur[q] = nxt[q];
+#pragma unroll
+      for (int q = 0; q < CH; ++q)
+        if (kb + CH + q <= k1) nxt[q] = ZZ(r, kb + CH + q + 1);
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        const int k = kb + q;
+        if (k <= k1) {
+          const zd tau1 = rot[2 * (k - k0)];
+          const zd v2 = rot[2 * (k - k0) + 1];
+          const double tau2 = (tau1 * v2).x;
+          zd b = cur[q];
+          if (has_ph && k + 1 >= k0 + 2) b = b * cph;
+          const zd ss = tau1 * carry + tau2 * b;
+          zd znew = carry - ss;
+          if (has_ph && k == k0) znew = znew * cph;
+          ZZ(r, k) = znew;
+          carry = b - ss * zconj(v2);
+        }
+      }
     }
     if (has_tl) carry = carry * tl;
     ZZ(r, k1 + 1) = carry;
@@ -228,14 +262,14 @@ template <int NT> GS_DEV void z_apply_chain(zd* Zg, int64_t ldz, int n, const zd
 
 // ------------------------------------------------------------------------
 // singleShiftQR! (GenericSchur.jl:374-504) on H with deferred Z.
-template <int NT> GS_DEV void z_sweep(zd* H, int64_t ld, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot, zd shift,
-                                      int istart, int iend, int64_t& nstep) {
+template <int NT, class L> GS_DEV void z_sweep(zd* H, L lay, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot,
+                                               int* redi, zd shift, int istart, int iend, int64_t& nstep) {
   const int tid = threadIdx.x;
   const double ulp = GS_EPS;
-  int istart1 = -1;
-  zd v0 = zr(0.0), v1 = zr(0.0);
-  bool flag = false;
-  for (int mm = iend - 1; mm >= istart + 1; --mm) {
+  // start row: the lowest mm in (istart, iend) whose start would make
+  // H[mm, mm-1] negligible (two consecutive small subdiagonals)
+  int found = -1;
+  for (int mm = iend - 1 - tid; mm >= istart + 1; mm -= NT) {
     const zd h11 = HH(mm, mm);
     const zd h22 = HH(mm + 1, mm + 1);
     zd h11s = h11 - shift;
@@ -243,20 +277,19 @@ template <int NT> GS_DEV void z_sweep(zd* H, int64_t ld, int n, zd* Zg, int64_t 
     const double s = zabs1(h11s) + fabs(h21);
     h11s = h11s / s;
     h21 = h21 / s;
-    v0 = h11s;
-    v1 = zr(h21);
     const double h10 = HH(mm, mm - 1).x;
     if (fabs(h10) * fabs(h21) <= ulp * (zabs1(h11s) * (zabs1(h11) + zabs1(h22)))) {
-      istart1 = mm;
-      flag = true;
+      found = mm;
       break;
     }
   }
-  if (!flag) {
-    istart1 = istart;
-    const zd h11 = HH(istart, istart);
+  found = block_max_int<NT>(found, redi);
+  const int istart1 = (found > 0) ? found : istart;
+  zd v0, v1;
+  {
+    const zd h11 = HH(istart1, istart1);
     zd h11s = h11 - shift;
-    double h21 = HH(istart + 1, istart).x;
+    double h21 = HH(istart1 + 1, istart1).x;
     const double s = zabs1(h11s) + fabs(h21);
     h11s = h11s / s;
     h21 = h21 / s;
@@ -350,11 +383,12 @@ template <int NT> GS_DEV void z_sweep(zd* H, int64_t ld, int n, zd* Zg, int64_t 
 // ------------------------------------------------------------------------
 // gschur!(H::Hessenberg{Complex}, Z) (GenericSchur.jl:194-335).
 // Returns 0, or the positive iteration count when maxiter was exceeded.
-template <int NT> GS_DEV int64_t z_qr(zd* H, int64_t ld, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot,
-                                      int64_t maxiter, int64_t maxinner, int64_t& nsweep, int64_t& nstep) {
+template <int NT, class L> GS_DEV int64_t z_qr(zd* H, L lay, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot,
+                                               int* redi, int64_t maxiter, int64_t maxinner, int64_t& nsweep,
+                                               int64_t& nstep) {
   const int tid = threadIdx.x;
   const double ulp = GS_EPS;
-  const double smallnum = GS_SAFEMIN * ((double)n / ulp);
+  const double smallnum = GS_FMIN * ((double)n / ulp);
   const double half = 0.5;
   const double threeq = 0.75;
   int64_t it = 0;
@@ -364,28 +398,32 @@ template <int NT> GS_DEV int64_t z_qr(zd* H, int64_t ld, int n, zd* Zg, int64_t 
     for (int64_t its = 0; its <= maxinner; ++its) {
       it += 1;
       if (it > maxiter) return it;
-      for (int is = iend - 1; is >= istart; --is) {
-        if (zabs1(HH(is + 1, is)) <= smallnum) {
-          istart = is + 1;
-          break;
-        }
-        double tst = zabs1(HH(is, is)) + zabs1(HH(is + 1, is + 1));
-        if (tst == 0.0) {
-          if (is - 1 >= 1) tst += fabs(HH(is, is - 1).x);
-          if (is + 2 <= n) tst += fabs(HH(is + 2, is + 1).x);
-        }
-        if (fabs(HH(is + 1, is).x) <= ulp * tst) {
-          const double ab = fmax(zabs1(HH(is + 1, is)), zabs1(HH(is, is + 1)));
-          const double ba = fmin(zabs1(HH(is + 1, is)), zabs1(HH(is, is + 1)));
-          const double aa = fmax(zabs1(HH(is + 1, is + 1)), zabs1(HH(is, is) - HH(is + 1, is + 1)));
-          const double bb = fmin(zabs1(HH(is + 1, is + 1)), zabs1(HH(is, is) - HH(is + 1, is + 1)));
-          const double s = aa + ab;
-          if (ba * (ab / s) <= fmax(smallnum, ulp * (bb * (aa / s)))) {
-            istart = is + 1;
-            break;
+      // lowest negligible subdiagonal in istart..iend-1
+      int found = -1;
+      for (int is = iend - 1 - tid; is >= istart; is -= NT) {
+        bool split = zabs1(HH(is + 1, is)) <= smallnum;
+        if (!split) {
+          double tst = zabs1(HH(is, is)) + zabs1(HH(is + 1, is + 1));
+          if (tst == 0.0) {
+            if (is - 1 >= 1) tst += fabs(HH(is, is - 1).x);
+            if (is + 2 <= n) tst += fabs(HH(is + 2, is + 1).x);
+          }
+          if (fabs(HH(is + 1, is).x) <= ulp * tst) {
+            const double ab = fmax(zabs1(HH(is + 1, is)), zabs1(HH(is, is + 1)));
+            const double ba = fmin(zabs1(HH(is + 1, is)), zabs1(HH(is, is + 1)));
+            const double aa = fmax(zabs1(HH(is + 1, is + 1)), zabs1(HH(is, is) - HH(is + 1, is + 1)));
+            const double bb = fmin(zabs1(HH(is + 1, is + 1)), zabs1(HH(is, is) - HH(is + 1, is + 1)));
+            const double s = aa + ab;
+            split = ba * (ab / s) <= fmax(smallnum, ulp * (bb * (aa / s)));
           }
         }
+        if (split) {
+          found = is;
+          break;
+        }
       }
+      found = block_max_int<NT>(found, redi);
+      if (found >= 0) istart = found + 1;
       if (istart > 1) {
         bsync<NT>();
         if (tid == 0) HH(istart, istart - 1) = zr(0.0);
@@ -420,149 +458,122 @@ template <int NT> GS_DEV int64_t z_qr(zd* H, int64_t ld, int n, zd* Zg, int64_t 
         }
       }
       nsweep += 1;
-      z_sweep<NT>(H, ld, n, Zg, ldz, wantz, rot, t, istart, iend, nstep);
+      z_sweep<NT>(H, lay, n, Zg, ldz, wantz, rot, redi, t, istart, iend, nstep);
     }
   }
   return 0;
 }
 
-// ------------------------------------------------------------------------
-// Full driver for one matrix.  H: working array (LDS or HBM) with leading
-// dimension ld; Tg/Zg/Wg: outputs in HBM; tmp: HBM array used to hold H
-// while Q is formed in place (T for the LDS kernel, Z for the HBM kernel).
-template <int NT> GS_DEV void z_driver(const zd* Ag, int64_t lda, zd* H, int64_t ld, int n, zd* Tg, int64_t ldt,
-                                       zd* Zg, int64_t ldz, zd* Wg, zd* tmp, int64_t ldtmp, zd* tau, zd* rot,
-                                       double* red, int flags, int64_t maxiter, int64_t maxinner, int32_t* info,
-                                       int64_t* stats) {
-  const int tid = threadIdx.x;
-  const bool wantz = (flags & 0x01) != 0;
-  const bool scale = (flags & 0x02) != 0;
-  const bool hessin = (flags & 0x04) != 0;
-  const bool checksd = (flags & 0x08) != 0;
-  if (maxiter <= 0) maxiter = 100 * (int64_t)n;
-  if (maxinner <= 0) maxinner = 30 * (int64_t)n;
-  // load A
-  if ((const void*)Ag != (const void*)H) {
-    for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
-      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-      H[r + c * ld] = Ag[r + c * lda];
-    }
-  }
-  __syncthreads();
-  if (hessin && checksd) {
-    bool bad = false;
-    for (int j = 1; j <= n - 1; ++j) bad |= (H[j + (int64_t)(j - 1) * ld].y != 0.0);
-    if (bad) {
-      if (tid == 0) *info = -2;
-      return;
-    }
-  }
-  // _scale! (util.jl:14-29): norm(A, Inf) is the largest entry modulus
-  bool scaled = false;
-  double cscale = 1.0, anrm = 0.0;
-  if (scale) {
-    double m = 0.0;
-    for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
-      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-      m = fmax(m, zabs(H[r + c * ld]));
-    }
-    anrm = block_max<NT>(m, red);
-    const double smlnum = sqrt(GS_SAFEMIN) / GS_EPS;
-    const double bignum = 1.0 / smlnum;
-    if (anrm > 0.0 && anrm < smlnum) { scaled = true; cscale = smlnum; }
-    else if (anrm > bignum) { scaled = true; cscale = bignum; }
-    if (scaled) {
-      double mul[4];
-      const int nm = safescale_muls(anrm, cscale, mul);
-      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
-        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-        zd v = H[r + c * ld];
-        for (int q = 0; q < nm; ++q) v = v * mul[q];
-        H[r + c * ld] = v;
-      }
-      __syncthreads();
-    }
-  }
-  if (!hessin) {
-    z_hessenberg<NT>(H, ld, n, tau, red);
-    if (wantz) {
-      // hold H in tmp, form Q in place, then Q -> Z and H <- tmp
-      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
-        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-        tmp[r + c * ldtmp] = H[r + c * ld];
-      }
-      __syncthreads();
-      z_form_q_inplace<NT>(H, ld, n, tau);
-      for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
-        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-        const zd q = H[r + c * ld];
-        const zd h = tmp[r + c * ldtmp];
-        Zg[r + c * ldz] = q;
-        H[r + c * ld] = h;
-      }
-      __syncthreads();
-    }
-  }
-  // zero below the subdiagonal (GenericSchur.jl:219-221)
-  for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
-    const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-    if (r > c + 1) H[r + c * ld] = zr(0.0);
-  }
-  __syncthreads();
-  int64_t nsweep = 0, nstep = 0;
-  const int64_t st = z_qr<NT>(H, ld, n, Zg, ldz, wantz, rot, maxiter, maxinner, nsweep, nstep);
-  __syncthreads();
-  if (stats && tid == 0) {
-    stats[0] = nsweep;
-    stats[1] = nstep;
-  }
-  if (st != 0) {
-    if (tid == 0) *info = (int32_t)(st > 0x7fffffff ? 0x7fffffff : st);
-    return;
-  }
-  // T = triu(H) (GenericSchur.jl:334), values = diag
+// T = triu(H) (GenericSchur.jl:334), values = diag(T), unscaled with
+// safescale!(T, cscale, anrm) when A was scaled (GenericSchur.jl:367-370).
+template <int NT, class L> GS_DEV void z_finish(const zd* H, L lay, int n, zd* Tg, int64_t ldt, zd* Wg,
+                                                double cscale, double anrm) {
   double umul[4];
   int nu = 0;
-  if (scaled) nu = safescale_muls(cscale, anrm, umul);
-  for (int64_t e = tid; e < (int64_t)n * n; e += NT) {
+  if (cscale != 0.0) nu = safescale_muls(cscale, anrm, umul);
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
     const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-    zd v = (r > c) ? zr(0.0) : H[r + c * ld];
+    zd v = (r > c) ? zr(0.0) : H[lay.idx(r + 1, c + 1)];
     for (int q = 0; q < nu; ++q) v = v * umul[q];
     Tg[r + c * ldt] = v;
     if (r == c) Wg[r] = v;
   }
-  if (tid == 0) *info = 0;
 }
 
 // ------------------------------------------------------------------------
-// Batched kernel: matrix per workgroup, H resident in LDS.
-template <int NT> __global__ void __launch_bounds__(NT) zschur_lds_kernel(BatchArgs a) {
+// Batched kernel 1: load, scale, Hessenberg reduction and Z formation with
+// the whole matrix in LDS.  Writes H (zero below the subdiagonal) to T and,
+// with wantZ, Q to Z.
+template <int NT> __global__ void __launch_bounds__(NT) zhq_kernel(BatchArgs a) {
   extern __shared__ double smem[];
   const int b = blockIdx.x;
   if (b >= a.count) return;
   const int m = a.order[a.first + b];
   const int n = a.n[m];
-  const int64_t ld = (int64_t)(n | 1);
-  zd* H = reinterpret_cast<zd*>(smem);
-  zd* tau = H + ld * n;
-  zd* rot = tau + (n > 0 ? n : 1);
-  double* red = reinterpret_cast<double*>(rot + 2 * (n > 0 ? n : 1));
   const int64_t o = a.off[m];
-  const zd* A = reinterpret_cast<const zd*>(a.A) + o;
-  zd* T = reinterpret_cast<zd*>(a.T) + o;
-  zd* Z = reinterpret_cast<zd*>(a.Z) + o;
-  zd* W = reinterpret_cast<zd*>(a.W) + a.woff[m];
   if (n <= 0) {
     if (threadIdx.x == 0) a.info[m] = (n == 0) ? 0 : -1;
     return;
   }
-  z_driver<NT>(A, n, H, ld, n, T, n, Z, n, W, T, n, tau, rot, red, a.flags, a.maxiter, a.maxinner, a.info + m,
-               a.stats ? a.stats + 2 * m : nullptr);
+  const int64_t ld = (int64_t)(n | 1);
+  zd* H = reinterpret_cast<zd*>(smem);
+  zd* tau = H + ld * n;
+  double* red = reinterpret_cast<double*>(tau + n);
+  const zd* A = reinterpret_cast<const zd*>(a.A) + o;
+  zd* T = reinterpret_cast<zd*>(a.T) + o;
+  zd* Z = reinterpret_cast<zd*>(a.Z) + o;
+  const bool wantz = (a.flags & 0x01) != 0;
+  const bool hessin = (a.flags & 0x04) != 0;
+  double cscale, anrm;
+  const int st = z_prep<NT>(A, n, H, ld, n, a.flags, red, cscale, anrm);
+  if (threadIdx.x == 0) {
+    a.info[m] = st;
+    a.scl[2 * m] = cscale;
+    a.scl[2 * m + 1] = anrm;
+  }
+  if (st != 0) return;
+  if (!hessin) z_hessenberg<NT>(H, ld, n, tau, red);
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+    const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+    T[e] = (r > c + 1) ? zr(0.0) : H[r + c * ld];
+  }
+  if (wantz && !hessin) {
+    __syncthreads();
+    z_form_q_inplace<NT>(H, ld, n, tau);
+    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+      const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+      Z[e] = H[r + c * ld];
+    }
+  }
 }
 
+// Batched kernel 2: QR iterations with the band of H in LDS, Z in HBM.
+template <int NT> __global__ void __launch_bounds__(NT) zqr_kernel(BatchArgs a) {
+  extern __shared__ double smem[];
+  const int b = blockIdx.x;
+  if (b >= a.count) return;
+  const int m = a.order[a.first + b];
+  const int n = a.n[m];
+  if (n <= 0) return;
+  if (a.info[m] != 0) return;  // rejected by zhq_kernel
+  const int64_t o = a.off[m];
+  BandLayout<2> lay;
+  lay.init(n);
+  zd* H = reinterpret_cast<zd*>(smem);
+  zd* rot = H + BandLayout<2>::size(n);
+  int* redi = reinterpret_cast<int*>(rot + 2 * n);
+  zd* T = reinterpret_cast<zd*>(a.T) + o;
+  zd* Z = reinterpret_cast<zd*>(a.Z) + o;
+  zd* W = reinterpret_cast<zd*>(a.W) + a.woff[m];
+  const bool wantz = (a.flags & 0x01) != 0;
+  // load the band (T holds H with zeros below the subdiagonal)
+  for (int j = 1; j <= n; ++j) {
+    const int len = min(j + 2, n);
+    const int64_t cb = lay.col(j);
+    for (int i = 1 + threadIdx.x; i <= len; i += NT) H[cb + i - 1] = T[(i - 1) + (int64_t)(j - 1) * n];
+  }
+  __syncthreads();
+  int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
+  int64_t maxinner = a.maxinner > 0 ? a.maxinner : 30 * (int64_t)n;
+  int64_t nsweep = 0, nstep = 0;
+  const int64_t st = z_qr<NT>(H, lay, n, Z, (int64_t)n, wantz, rot, redi, maxiter, maxinner, nsweep, nstep);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (a.stats) {
+      a.stats[2 * m] = nsweep;
+      a.stats[2 * m + 1] = nstep;
+    }
+    a.info[m] = (int32_t)(st > 0x7fffffff ? 0x7fffffff : st);
+  }
+  if (st != 0) return;
+  z_finish<NT>(H, lay, n, T, (int64_t)n, W, a.scl[2 * m], a.scl[2 * m + 1]);
+}
+
+// ------------------------------------------------------------------------
 // Single-matrix kernel: H in HBM (the T output buffer), any n.
 template <int NT> __global__ void __launch_bounds__(NT) zschur_global_kernel(SingleArgs a) {
   __shared__ double red[NT / 64 + 1];
+  __shared__ int redi[NT / 64 + 1];
   const int n = (int)a.n;
   zd* A = reinterpret_cast<zd*>(a.A);
   zd* T = reinterpret_cast<zd*>(a.T);
@@ -570,48 +581,104 @@ template <int NT> __global__ void __launch_bounds__(NT) zschur_global_kernel(Sin
   zd* W = reinterpret_cast<zd*>(a.W);
   zd* tau = reinterpret_cast<zd*>(a.ws);
   zd* rot = tau + n;
-  // copy A into T, which then serves as the working array
+  const int64_t ldt = a.ldt, ldz = a.ldz;
+  const bool wantz = (a.flags & 0x01) != 0;
+  const bool hessin = (a.flags & 0x04) != 0;
+  double cscale, anrm;
+  const int st0 = z_prep<NT>(A, a.lda, T, ldt, n, a.flags, red, cscale, anrm);
+  if (st0 != 0) {
+    if (threadIdx.x == 0) *a.info = st0;
+    return;
+  }
+  if (!hessin) {
+    z_hessenberg<NT>(T, ldt, n, tau, red);
+    if (wantz) {
+      // hold H in Z, form Q in place in T, then swap the two
+      for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        Z[r + c * ldz] = T[r + c * ldt];
+      }
+      __syncthreads();
+      z_form_q_inplace<NT>(T, ldt, n, tau);
+      for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        const zd q = T[r + c * ldt];
+        const zd h = Z[r + c * ldz];
+        Z[r + c * ldz] = q;
+        T[r + c * ldt] = h;
+      }
+      __syncthreads();
+    }
+  }
   for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
     const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
-    T[r + c * a.ldt] = A[r + c * a.lda];
+    if (r > c + 1) T[r + c * ldt] = zr(0.0);
   }
   __syncthreads();
-  z_driver<NT>(T, a.ldt, T, a.ldt, n, T, a.ldt, Z, a.ldz, W, Z, a.ldz, tau, rot, red, a.flags, a.maxiter,
-               a.maxinner, a.info, a.stats);
+  FullLayout lay{ldt};
+  const int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
+  const int64_t maxinner = a.maxinner > 0 ? a.maxinner : 30 * (int64_t)n;
+  int64_t nsweep = 0, nstep = 0;
+  const int64_t st = z_qr<NT>(T, lay, n, Z, ldz, wantz, rot, redi, maxiter, maxinner, nsweep, nstep);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (a.stats) {
+      a.stats[0] = nsweep;
+      a.stats[1] = nstep;
+    }
+    *a.info = (int32_t)(st > 0x7fffffff ? 0x7fffffff : st);
+  }
+  if (st != 0) return;
+  z_finish<NT>(T, lay, n, T, ldt, W, cscale, anrm);
 }
 
-size_t zschur_lds_bytes(int n, int nt) {
+// ------------------------------------------------------------------------
+size_t zhq_lds_bytes(int n, int nt) {
   const size_t ld = (size_t)(n | 1);
-  return 16 * (ld * n + 3 * (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
+  return 16 * (ld * n + (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
+}
+size_t zqr_lds_bytes(int n, int nt) {
+  return 16 * ((size_t)BandLayout<2>::size(n) + 2 * (size_t)(n > 0 ? n : 1)) + 4 * (size_t)(nt / 64 + 1);
 }
 
 int zschur_max_lds_n() {
-  // largest n whose footprint fits the 160 KiB of LDS of a gfx950 CU
+  // largest n whose Hessenberg-phase footprint fits the 160 KiB of LDS of a
+  // gfx950 CU
   int n = 1;
-  while (zschur_lds_bytes(n + 1, 256) <= 160 * 1024) ++n;
+  while (zhq_lds_bytes(n + 1, 256) <= 160 * 1024) ++n;
   return n;
 }
 
-hipError_t launch_zschur_lds(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
-  const size_t lds = zschur_lds_bytes(nmax, nt);
-  if (a.count <= 0) return hipSuccess;
-  switch (nt) {
-    case 64:
-      (void)hipFuncSetAttribute((const void*)zschur_lds_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL(zschur_lds_kernel<64>, dim3(a.count), dim3(64), lds, s, a);
-      break;
-    case 128:
-      (void)hipFuncSetAttribute((const void*)zschur_lds_kernel<128>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL(zschur_lds_kernel<128>, dim3(a.count), dim3(128), lds, s, a);
-      break;
-    case 256:
-      (void)hipFuncSetAttribute((const void*)zschur_lds_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL(zschur_lds_kernel<256>, dim3(a.count), dim3(256), lds, s, a);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
+template <int NT> static hipError_t launch_zhq_nt(const BatchArgs& a, size_t lds, hipStream_t s) {
+  (void)hipFuncSetAttribute((const void*)zhq_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(zhq_kernel<NT>, dim3(a.count), dim3(NT), lds, s, a);
   return hipGetLastError();
+}
+template <int NT> static hipError_t launch_zqr_nt(const BatchArgs& a, size_t lds, hipStream_t s) {
+  (void)hipFuncSetAttribute((const void*)zqr_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(zqr_kernel<NT>, dim3(a.count), dim3(NT), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_zhq(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
+  if (a.count <= 0) return hipSuccess;
+  const size_t lds = zhq_lds_bytes(nmax, nt);
+  switch (nt) {
+    case 64: return launch_zhq_nt<64>(a, lds, s);
+    case 128: return launch_zhq_nt<128>(a, lds, s);
+    case 256: return launch_zhq_nt<256>(a, lds, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_zqr(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
+  if (a.count <= 0) return hipSuccess;
+  const size_t lds = zqr_lds_bytes(nmax, nt);
+  switch (nt) {
+    case 64: return launch_zqr_nt<64>(a, lds, s);
+    case 128: return launch_zqr_nt<128>(a, lds, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_zschur_global(const SingleArgs& a, int nt, hipStream_t s) {

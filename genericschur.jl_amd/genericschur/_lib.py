@@ -26,7 +26,7 @@ GS_E_HIP = -100
 
 # every symbol the header declares
 EXPORTS = (
-    "gs_version", "gs_create", "gs_destroy", "gs_last_error", "gs_synchronize",
+    "gs_version", "gs_create", "gs_destroy", "gs_last_error", "gs_synchronize", "gs_set_stream",
     "gs_schur_z", "gs_schur_d", "gs_schur_batched_z", "gs_schur_batched_d", "gs_batched_max_n",
     "gs_trevc_z", "gs_trevc_d", "gs_trevc_batched_z", "gs_trevc_batched_d",
     "gs_eig_vjp_batched_d", "gs_last_stats",
@@ -64,6 +64,7 @@ def lib():
         L.gs_last_error.argtypes = [c_vp]
         L.gs_last_error.restype = ctypes.c_char_p
         L.gs_synchronize.argtypes = [c_vp]
+        L.gs_set_stream.argtypes = [c_vp, c_vp]
         sig_single = [c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_int, c_i64, c_i64, c_i32p]
         L.gs_schur_z.argtypes = sig_single
         L.gs_schur_d.argtypes = sig_single

@@ -290,6 +290,13 @@ def gschur_batched_device(ns, off, A, T, Z, W, info, complex_input, wantZ=True, 
     _lib.check(rc, h, "gs_schur_batched")
 
 
+def set_stream(stream, device=0):
+    """Run the library's work on `stream` (a hipStream_t address, e.g.
+    torch.cuda.current_stream().cuda_stream); None restores its own stream."""
+    h = _lib.handle(device)
+    _lib.check(_lib.lib().gs_set_stream(h, ctypes.c_void_p(stream) if stream else None), h, "gs_set_stream")
+
+
 def synchronize(device=0):
     h = _lib.handle(device)
     _lib.check(_lib.lib().gs_synchronize(h), h, "gs_synchronize")

@@ -69,6 +69,10 @@ int gs_create(gs_handle** h, int device);
 int gs_destroy(gs_handle* h);
 const char* gs_last_error(const gs_handle* h);
 int gs_synchronize(gs_handle* h);
+/* Run this handle's work on the caller's HIP stream (a hipStream_t passed as
+ * void*), e.g. torch.cuda.current_stream().cuda_stream; NULL restores the
+ * handle's own stream.  No counterpart in the reference. */
+int gs_set_stream(gs_handle* h, void* stream);
 
 /* Single matrix, ComplexF64.
  * Replaces gschur!(A::StridedMatrix{Complex{T}}; wantZ, scale, maxiter,
