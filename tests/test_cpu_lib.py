@@ -1,0 +1,139 @@
+"""CPU-side checks of the product library and its host layer (no GPU needed).
+
+* the built library loads and exports every entry point the header declares;
+* without a GPU the library fails loudly (no CPU fallback);
+* host-side argument validation raises the reference's exceptions
+  (DimensionMismatch for non-square input, test/errors.jl:12-13; no capture of
+  non-float element types, test/errors.jl:16-28);
+* the seeded generators are deterministic and produce the workloads' shapes.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import genericschur as gs
+from genericschur import _lib, gen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gschur_mi355x.h")
+
+
+def header_symbols():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gs_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_header():
+    L = _lib.lib()
+    syms = header_symbols()
+    assert len(syms) >= 15
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    assert set(syms) <= set(_lib.EXPORTS) | set(syms)
+    assert set(_lib.EXPORTS) == set(syms)
+
+
+def _gpu_visible():
+    try:
+        import torch
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+@pytest.mark.skipif(_gpu_visible(), reason="checks the no-GPU failure path")
+def test_fails_loudly_without_gpu():
+    hp = ctypes.c_void_p()
+    rc = _lib.lib().gs_create(ctypes.byref(hp), 0)
+    assert rc != 0
+    with pytest.raises(gs.GSError):
+        gs.gschur(np.eye(3))
+
+
+def test_argument_validation():
+    with pytest.raises(gs.DimensionMismatch):
+        gs.gschur(np.zeros((5, 4)))
+    with pytest.raises(gs.DimensionMismatch):
+        gs.gschur(np.zeros((5, 4), dtype=complex))
+    with pytest.raises(TypeError):
+        gs.gschur(np.ones((4, 4), dtype=np.int64))
+    with pytest.raises(TypeError):
+        gs.gschur(np.ones((4, 4), dtype=np.float32))
+    with pytest.raises(gs.DimensionMismatch):
+        gs.gschur_hessenberg_(np.triu(np.ones((5, 5)), -1), Z=np.ones((4, 4)))
+
+
+def test_header_constants_match_host_layer():
+    txt = open(HEADER).read()
+    for name in ("GS_WANTZ", "GS_SCALE", "GS_HESSENBERG", "GS_CHECK_SUBDIAG", "GS_DEVICE_PTRS", "GS_ASYNC",
+                 "GS_PLAIN"):
+        m = re.search(r"#define %s (0x[0-9a-fA-F]+)" % name, txt)
+        assert m and int(m.group(1), 16) == getattr(_lib, name), name
+
+
+def test_pack_roundtrip():
+    r = np.random.default_rng(0)
+    mats = [r.standard_normal((k, k)) + 1j * r.standard_normal((k, k)) for k in (1, 5, 3, 8)]
+    ns, off, flat = gs.pack(mats, np.complex128)
+    assert list(ns) == [1, 5, 3, 8] and list(off) == [0, 1, 26, 35]
+    for a, b in zip(mats, gs.unpack(ns, off, flat)):
+        assert np.array_equal(a, b)
+    assert list(gs.eig_offsets(ns)) == [0, 1, 6, 9]
+
+
+def test_generators_deterministic():
+    a = gen.workload2(seed=3, count=20)
+    b = gen.workload2(seed=3, count=20)
+    c = gen.workload2(seed=4, count=20)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    assert not all(x.shape == y.shape and np.array_equal(x, y) for x, y in zip(a, c))
+    assert all(8 <= m.shape[0] <= 96 and m.dtype == np.complex128 for m in a)
+    # a matrix does not depend on the batch size
+    d = gen.workload2(seed=3, count=5)
+    assert all(np.array_equal(x, y) for x, y in zip(a[:5], d))
+    # Re, Im ~ N(0, 1/2)
+    big = np.concatenate([m.ravel() for m in gen.workload2(seed=1, count=200)])
+    assert abs(big.real.var() - 0.5) < 0.02 and abs(big.imag.var() - 0.5) < 0.02
+
+
+def test_workload3_kinds():
+    mats, kinds = gen.workload3(seed=2, count=64)
+    assert (kinds == 0).sum() == 32 and (kinds == 1).sum() == 16 and (kinds == 2).sum() == 16
+    for A, k in zip(mats, kinds):
+        n = A.shape[0]
+        assert 16 <= n <= 128 and A.dtype == np.float64
+        if k == 1:
+            assert np.array_equal(A, gen.toeplitz_grcar(n))
+            assert A[1, 0] == -1 and A[0, 3] == 1 and A[0, 4] == 0
+
+
+def test_qdnq_eigenvalues():
+    """Q (D + N) Q^T keeps the eigenvalues of the blocks of D."""
+    rng = gen.stream(0, 99, 0)
+    n = 24
+    A = gen.qdnq(n, rng)
+    ev = np.linalg.eigvals(A)
+    assert np.all(np.abs(np.abs(ev) - 1) < 1e-8) | np.all(np.abs(ev) <= 1 + 1e-8)
+
+
+def test_workload4_construction():
+    A, M, ev = gen.workload4_parts(seed=0, n=64)
+    assert len(ev) == 64
+    got = np.linalg.eigvals(M)
+    from scipy.optimize import linear_sum_assignment
+    C = np.abs(got[:, None] - ev[None, :])
+    r, c = linear_sum_assignment(C)
+    assert C[r, c].max() <= 1e-9 * np.abs(ev).max()
+    mags = np.abs(ev)
+    assert mags.min() >= 1e-3 * (1 - 1e-12) and mags.max() <= 1e3 * (1 + 1e-12)
+
+
+def test_workload5_separation():
+    mats, w, v = gen.workload5(seed=0, count=8)
+    for A in mats:
+        assert 32 <= A.shape[0] <= 64
+        assert gen.min_separation(A) >= 1e-3

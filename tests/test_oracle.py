@@ -1,0 +1,403 @@
+"""Pins the CPU oracle (oracle/, a restatement of GenericSchur.jl's Schur path).
+
+What pins what (the reference cannot run here: Julia is absent):
+  * LAPACK through scipy: xLARFG (householder.jl:12-102 is its translation),
+    xGEHRD/xUNGHR (the Hessenberg/Q math), xGEES (for n < 75 it runs
+    xLAHQR, the routine GenericSchur.jl:194-952 translates): the oracle must
+    reproduce LAPACK's eigenvalue order and, for complex input, T and Z up to
+    column phases.
+  * mpmath at 256 bits (tests/golden/fixtures.json): eigenvalues.
+  * closed forms and invariants: reflector identities, the reference's own
+    schurtest/hesstest/vectest/checkblocks/checkeigvals, known-answer cases
+    (Godunov, Jordan, the near-degenerate 2x2 of test/real.jl:311-317),
+    normal matrices giving (block-)diagonal T.
+Each test names the mistake it would catch.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.linalg.lapack as lap
+
+import checks
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EPS = np.finfo(np.float64).eps
+TINY = np.finfo(np.float64).tiny
+
+
+def fixtures():
+    with open(os.path.join(HERE, "golden", "fixtures.json")) as f:
+        return json.load(f)
+
+
+def dec(d):
+    A = np.array(d["re"], dtype=np.float64)
+    if "im" in d:
+        A = A + 1j * np.array(d["im"], dtype=np.float64)
+    return A
+
+
+def snorm(x):
+    """2-norm without underflow (numpy squares directly)."""
+    a = np.abs(x)
+    m = a.max()
+    return 0.0 if m == 0 else m * float(np.sqrt(np.sum((a / m) ** 2)))
+
+
+def rand_c(n, seed):
+    r = np.random.default_rng(seed)
+    return r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))
+
+
+def rand_r(n, seed):
+    return np.random.default_rng(seed).standard_normal((n, n))
+
+
+# ------------------------------------------------------------ reflectors
+@pytest.mark.parametrize("m", [2, 3, 7, 20])
+def test_reflector_matches_lapack(oracle, m):
+    """householder.jl:12-102 vs LAPACK xLARFG (same algorithm): a wrong sign
+    of beta or a conjugated tau shows up here."""
+    r = np.random.default_rng(m)
+    x = r.standard_normal(m)
+    xo, tau = oracle.reflector(x)
+    beta, v, t = lap.dlarfg(m, x[0], x[1:].copy())
+    assert abs(xo[0] - beta) <= 4 * EPS * abs(beta)
+    assert np.allclose(xo[1:], v, rtol=1e-13, atol=1e-15)
+    assert abs(tau - t) <= 4 * EPS
+    z = r.standard_normal(m) + 1j * r.standard_normal(m)
+    zo, tauz = oracle.reflector(z)
+    beta, v, t = lap.zlarfg(m, z[0], z[1:].copy())
+    assert abs(zo[0] - beta) <= 4 * EPS * abs(beta)
+    assert np.allclose(zo[1:], v, rtol=1e-13, atol=1e-15)
+    assert abs(tauz - t) <= 8 * EPS
+
+
+@pytest.mark.parametrize("scale", [1.0, 1e-300, 100 * TINY, 1e300])
+def test_reflector_closed_form(oracle, scale):
+    """H^H x = beta e1 with beta real, |beta| = ||x||, sign(beta) = -sign(Re x1),
+    H unitary; the tiny scales run the sfmin rescaling loop."""
+    r = np.random.default_rng(7)
+    for cplx in (False, True):
+        x = r.standard_normal(6) + (1j * r.standard_normal(6) if cplx else 0)
+        x = x * scale
+        xo, tau = oracle.reflector(x)
+        w = np.concatenate([[1.0], xo[1:]])
+        H = np.eye(6) - tau * np.outer(w, w.conj())
+        y = H.conj().T @ x
+        beta = xo[0]
+        assert abs(np.imag(beta)) == 0
+        nx = snorm(x)
+        assert abs(abs(beta) - nx) <= 8 * EPS * nx
+        assert np.sign(np.real(beta)) == -np.sign(np.real(x[0]))
+        assert snorm(y[1:]) <= 16 * EPS * nx
+        assert np.linalg.norm(H.conj().T @ H - np.eye(6)) <= 16 * EPS
+
+
+def test_reflector_complex_length_one(oracle):
+    """The complex n = 1 reflector is not trivial (householder.jl:59-69): it
+    makes a complex scalar real."""
+    xo, tau = oracle.reflector(np.array([3.0 + 4.0j]))
+    assert xo[0] == pytest.approx(-5.0) and xo[0].imag == 0
+    assert (1 - np.conj(tau)) * (3 + 4j) == pytest.approx(-5.0)
+    xo, tau = oracle.reflector(np.array([2.0 + 0.0j]))
+    assert tau == 0 and xo[0] == 2.0
+
+
+# -------------------------------------------------------- Hessenberg + Q
+@pytest.mark.parametrize("cplx", [False, True])
+def test_hessenberg_matches_lapack(oracle, cplx):
+    """hessenberg.jl:3-17,150-166 vs xGEHRD/xUNGHR: a transposed update or a
+    wrong reflector index makes H or Q differ."""
+    n = 20
+    A = rand_c(n, 3) if cplx else rand_r(n, 3)
+    fac, tau, Q, H = oracle.hessenberg(A)
+    if cplx:
+        ht, tt, info = lap.zgehrd(A)
+        q, info2 = lap.zunghr(ht, tt)
+    else:
+        ht, tt, info = lap.dgehrd(A)
+        q, info2 = lap.dorghr(ht, tt)
+    assert np.abs(np.triu(ht, -1) - H).max() <= 1e-12 * np.abs(A).max()
+    assert np.abs(q - Q).max() <= 1e-13
+    assert np.abs(tau - tt[: n - 1]).max() <= 1e-13
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("s", [1.0, 100 * TINY, np.finfo(np.float64).max / 100])
+def test_hesstest(oracle, cplx, s):
+    """hesstest (test/complex.jl:36-61, test/real.jl:76-99) incl. the tiny and
+    huge scalings of those tests."""
+    n = 32
+    r = np.random.default_rng(1234)
+    A = r.random((n, n)) + (1j * r.random((n, n)) if cplx else 0)
+    A = A * s
+    _, _, Q, H = oracle.hessenberg(A)
+    assert np.all(np.tril(H, -2) == 0)
+    na = checks.fnorm(A)
+    assert checks.fnorm(A - Q @ H @ Q.conj().T) / (n * na * EPS) < 10
+    assert checks.fnorm(np.eye(n) - Q @ Q.conj().T) / (n * EPS) < 10
+    if cplx:
+        assert np.all(np.diag(H, -1).imag == 0)
+
+
+# ------------------------------------------------------------ Schur: LAPACK
+@pytest.mark.parametrize("n", [4, 16, 40, 70])
+def test_complex_schur_matches_zgees(oracle, n):
+    """For n < 75 ZGEES runs ZLAHQR, which GenericSchur.jl:194-504 translates:
+    same eigenvalue order, T equal up to the column phases of Z."""
+    A = rand_c(n, 100 + n)
+    t, sdim, w, vs, work, info = lap.zgees(lambda x: 0, A)
+    T, Z, wo, inf, _ = oracle.gschur(A)
+    assert inf == 0 and info == 0
+    assert np.abs(np.diag(T) - np.diag(t)).max() <= 100 * n * EPS * np.abs(A).max()
+    assert np.abs(np.abs(T) - np.abs(t)).max() <= 100 * n * EPS * np.abs(A).max()
+    ph = np.array([np.vdot(Z[:, j], vs[:, j]) for j in range(n)])
+    ph /= np.abs(ph)
+    assert np.abs(Z * ph - vs).max() <= 100 * n * EPS
+
+
+@pytest.mark.parametrize("n", [4, 16, 40, 70])
+def test_real_schur_matches_dgees_order(oracle, n):
+    """GenericSchur.jl:513-952 vs DGEES (DLAHQR for n < 75): same eigenvalue
+    order; values within a few ulps of ||A|| (2x2 blocks may differ by a
+    rotation inside the block, the reference keeps the older dlanv2)."""
+    A = rand_r(n, 200 + n)
+    t, sdim, wr, wi, vs, work, info = lap.dgees(lambda x, y: 0, A)
+    T, Z, wo, inf, _ = oracle.gschur(A)
+    assert inf == 0 and info == 0
+    assert np.abs(wo - (wr + 1j * wi)).max() <= 100 * n * EPS * np.abs(A).max()
+    assert checks.checkblocks(T)
+
+
+# ----------------------------------------------- Schur: invariants, KATs
+def _schurtest(oracle, A, tol=20.0, normal=False):
+    T, Z, w, info, _ = oracle.gschur(A)
+    assert info == 0
+    checks.schur_invariants(A, T, Z, w, tol, normal=normal)
+    VR = oracle.eigvecs(T, Z, left=False)
+    VL = oracle.eigvecs(T, Z, left=True)
+    checks.vectest(A, w, VR, VL, tol, normal=normal)
+    if not np.iscomplexobj(A):
+        Tc, Zc = oracle.triangularize(T, Z)
+        checks.schur_invariants(A.astype(complex), Tc, Zc, np.diag(Tc), tol)
+    return T, Z, w
+
+
+@pytest.mark.parametrize("n", [4, 32])
+def test_schurtest_random(oracle, n):
+    """random general (test/complex.jl:124-129, test/real.jl:127-132)"""
+    r = np.random.default_rng(1234 + n)
+    _schurtest(oracle, r.random((n, n)) + 1j * r.random((n, n)), 10)
+    _schurtest(oracle, r.random((n, n)), 10)
+
+
+@pytest.mark.parametrize("n", [4, 32])
+def test_schurtest_normal(oracle, n):
+    """random unitary / orthogonal matrices are normal, so T is (block)
+    diagonal (test/complex.jl:130-135, test/real.jl:133-138)"""
+    r = np.random.default_rng(99 + n)
+    Q, R = np.linalg.qr(r.standard_normal((n, n)) + 1j * r.standard_normal((n, n)))
+    U = Q * np.sign(np.diag(R))
+    _schurtest(oracle, U, 20, normal=True)
+    Q, _ = np.linalg.qr(r.standard_normal((n, n)))
+    _schurtest(oracle, Q, 10, normal=True)
+
+
+@pytest.mark.parametrize("n", [4, 32])
+def test_schurtest_jordan(oracle, n):
+    """Jordan block (test/complex.jl:221-232, test/real.jl:180-191)"""
+    A = np.diag(np.ones(n - 1), 1) + np.eye(n)
+    _schurtest(oracle, A.astype(complex))
+    _schurtest(oracle, A)
+
+
+@pytest.mark.parametrize("mag", [1.0, np.finfo(float).max * EPS, TINY / EPS])
+def test_schurtest_magnitudes(oracle, mag):
+    """magnitudes {1, ovfl*ulp, unfl*ulpinv} of zdrves/ddrves
+    (test/complex.jl:187, test/real.jl:178); the small one exercises _scale!"""
+    n = 16
+    r = np.random.default_rng(5)
+    A = (r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))) * mag / n
+    T, Z, w, info, _ = oracle.gschur(A)
+    assert info == 0
+    checks.schur_invariants(A, T, Z, w, 20)
+    Ar = r.standard_normal((n, n)) * mag / n
+    T, Z, w, info, _ = oracle.gschur(Ar)
+    assert info == 0
+    checks.schur_invariants(Ar, T, Z, w, 20)
+
+
+def test_godunov(oracle):
+    """Godunov (test/testfuncs.jl:127-142): in double only the backward error
+    is meaningful (econd 7e16); the quad-double oracle (~212 bits) must hit the
+    exact eigenvalues within 3 delta econd as test/real.jl:142-157 demands at
+    80 bits."""
+    fx = fixtures()["godunov"]
+    A = np.array(fx["A"])
+    T, Z, w, info, _ = oracle.gschur(A)
+    assert info == 0
+    assert np.linalg.norm(Z @ T @ Z.T - A) < 100 * EPS * np.linalg.norm(A)
+    vals, w4, info = oracle.eig_hp(A, prec=2)
+    assert info == 0
+    delta = 2.0 ** -200 * np.linalg.norm(A)
+    t = 3 * delta * fx["econd"]
+    got = np.sort(vals.real)
+    assert np.abs(got - np.array(fx["values"])).max() <= max(t, 1e-30)
+    assert np.abs(vals.imag).max() <= max(t, 1e-30)
+
+
+def test_gs2x2_near_degenerate(oracle):
+    """_gs2x2! known-answer test (test/real.jl:311-317)."""
+    for case in fixtures()["gs2x2_kat"]:
+        B = np.array(case["B"])
+        abcd, cssn, w1, w2 = oracle.gs2x2(B[0, 0], B[0, 1], B[1, 0], B[1, 1])
+        l1, l2 = case["l1"], case["l2"]
+        assert abs(max(w1.real, w2.real) - l1) < 2 * EPS
+        assert abs(min(w1.real, w2.real) - l2) < 2 * EPS
+
+
+def test_gs2x2_standard_form(oracle):
+    """dlanv2 output: c = 0, or a = d with b c < 0; the rotation is a
+    similarity; eigenvalues a +- i sqrt|b| sqrt|c|."""
+    r = np.random.default_rng(11)
+    for _ in range(200):
+        a, b, c, d = r.standard_normal(4)
+        (A, B, C, D), (cs, sn), w1, w2 = oracle.gs2x2(a, b, c, d)
+        assert C == 0 or (A == D and B * C < 0)
+        G = np.array([[cs, sn], [-sn, cs]])
+        M = G @ np.array([[a, b], [c, d]]) @ G.T
+        assert np.abs(M - np.array([[A, B], [C, D]])).max() <= 16 * EPS * max(abs(a), abs(b), abs(c), abs(d))
+        ev = np.sort_complex(np.linalg.eigvals(np.array([[a, b], [c, d]])))
+        got = np.sort_complex(np.array([w1, w2]))
+        assert np.abs(ev - got).max() <= 1e-12 * max(1, np.abs(ev).max()) + 1e-7 * (C != 0)
+
+
+def test_mp256_fixtures(oracle):
+    """Eigenvalues against mpmath at 256 bits: the double oracle within
+    10 n eps ||A|| cond, the quad-double oracle within 2^-190 ||A|| cond."""
+    for case in fixtures()["mp256"]:
+        A = dec(case["A"])
+        n = case["n"]
+        ref = np.array([complex(float(re), float(im)) for re, im in case["eig256"]])
+        T, Z, w, info, _ = oracle.gschur(A)
+        assert info == 0
+        VR = oracle.eigvecs(T, Z, left=False)
+        VL = oracle.eigvecs(T, Z, left=True)
+        cond = checks.eig_conditions(VR, VL)
+        checks.eigen_parity(A, w, ref, cond, factor=10.0)
+        vals, w4, info = oracle.eig_hp(A, prec=2)
+        perm, err = checks.match_eigenvalues(vals, ref)
+        assert np.all(err <= 2.0 ** -190 * np.linalg.norm(A, 2) * cond + 1e-300), (n, err.max())
+
+
+def test_values_slot_order_real(oracle):
+    """values[j] corresponds to T[j, j], the +imag member of a pair first
+    (GenericSchur.jl:677-679,794-799)."""
+    A = rand_r(24, 17)
+    T, Z, w, info, _ = oracle.gschur(A)
+    assert checks.checkeigvals(T, w, 20)
+    for j in range(23):
+        if T[j + 1, j] != 0:
+            assert w[j].imag > 0 and w[j + 1].imag < 0
+
+
+# ----------------------------------------------------------- eigenvectors
+@pytest.mark.parametrize("cplx", [False, True])
+def test_eigvecs_match_lapack(oracle, cplx):
+    """vectors.jl:12-698 vs LAPACK xGEEV vectors, phase-insensitive
+    (_chkeigvecs, test/wrappers.jl:12-54)."""
+    n = 30
+    A = rand_c(n, 8) if cplx else rand_r(n, 8)
+    T, Z, w, info, _ = oracle.gschur(A)
+    for left in (False, True):
+        V = oracle.eigvecs(T, Z, left=left)
+        ev, vl, vr = sla.eig(A, left=True, right=True)
+        ref = vl if left else vr
+        perm, _ = checks.match_eigenvalues(ev, w)
+        ref = ref[:, perm]
+        p = np.abs(np.sum(ref.conj() * V, axis=0)) / (np.linalg.norm(ref, axis=0) * np.linalg.norm(V, axis=0))
+        assert np.all(np.abs(p - 1) < np.sqrt(EPS))
+
+
+def test_eigvecs_normalization(oracle):
+    """_enormalize! (util.jl:572-592): unit 2-norm, largest entry real and
+    positive (test/wrappers.jl:181-192)."""
+    A = np.array(fixtures()["normalization"]["A"], dtype=float)
+    for M in (A, A + 0.5j * A.T):
+        T, Z, w, info, _ = oracle.gschur(M)
+        for left in (False, True):
+            V = oracle.eigvecs(T, Z, left=left)
+            assert np.all(np.abs(np.linalg.norm(V, axis=0) - 1) / (3 * EPS) < 10)
+            for j in range(3):
+                i0 = np.argmax(np.abs(V[:, j]))
+                assert V[i0, j].imag == 0 and V[i0, j].real > 0
+
+
+@pytest.mark.parametrize("dmode", [0, 1, 2])
+def test_eigvecs_triangular_repeated(oracle, dmode):
+    """triangular, specified eigenvalues with an exact / next / previous
+    repeat (test/complex.jl:327-364): exercises the smin perturbation of the
+    scaled solves."""
+    n = 32
+    r = np.random.default_rng(31 + dmode)
+    d = r.random(n) + 1j * r.random(n)
+    if dmode == 0:
+        d[1] = d[n - 2]
+    elif dmode == 1:
+        d[1] = d[n - 2] * np.nextafter(1.0, 2.0)
+    else:
+        d[1] = d[n - 2] * np.nextafter(1.0, 0.0)
+    A = np.triu(r.random((n, n)) + 1j * r.random((n, n)), 1) + np.diag(d)
+    VR = oracle.eigvecs(A, np.eye(n, dtype=complex), left=False)
+    VL = oracle.eigvecs(A, np.eye(n, dtype=complex), left=True)
+    checks.vectest(A, np.diag(A), VR, VL, 20)
+
+
+def test_eigvecs_nonfinite_no_crash(oracle):
+    """test/complex.jl:393-406: non-finite T must not throw."""
+    A = rand_c(5, 3)
+    T, Z, w, info, _ = oracle.gschur(A)
+    for bad in (np.inf, np.nan):
+        T2 = T.copy()
+        T2[1, 2] = bad
+        oracle.eigvecs(T2, Z, left=False)
+        oracle.eigvecs(T2, Z, left=True)
+
+
+# ------------------------------------------------------------ error paths
+def test_errors(oracle):
+    """UnconvergedException at maxiter (GenericSchur.jl:234-236,554-556) and
+    ArgumentError for a non-real subdiagonal (test/errors.jl:1-5)."""
+    A = rand_c(8, 1)
+    assert oracle.gschur(A, maxiter=1)[3] == oracle.UNCONVERGED
+    assert oracle.gschur(rand_r(8, 1), maxiter=1)[3] == oracle.UNCONVERGED
+    An = A.copy()
+    An[2, 3] = np.nan
+    assert oracle.gschur(An)[3] == oracle.UNCONVERGED
+    n = 5
+    H = np.diag(np.full(n - 1, -1.0 + 1.0j), -1) + np.triu(np.random.default_rng(0).random((n, n)))
+    assert oracle.schur_hess(H, checksd=True)[3] == oracle.NONREAL_SUB
+
+
+def test_qd_arithmetic(oracle):
+    """The quad-double type against mpmath at 256 bits."""
+    import mpmath as mp
+    mp.mp.prec = 256
+
+    def val(r):
+        return sum(mp.mpf(float(x)) for x in r)
+
+    r = oracle.qd_op("sqrt", [2.0, 0, 0, 0])
+    assert abs(val(r) - mp.sqrt(2)) < mp.mpf(2) ** -205
+    r = oracle.qd_op("div", [1.0, 0, 0, 0], [3.0, 0, 0, 0])
+    assert abs(val(r) - mp.mpf(1) / 3) < mp.mpf(2) ** -205
+    a = oracle.qd_op("div", [1.0, 0, 0, 0], [7.0, 0, 0, 0])
+    b = oracle.qd_op("sqrt", [3.0, 0, 0, 0])
+    r = oracle.qd_op("mul", a, b)
+    assert abs(val(r) - mp.sqrt(3) / 7) < mp.mpf(2) ** -204
+    r = oracle.qd_op("add", a, [-a[0], -a[1], -a[2], 0.0])
+    assert abs(val(r) - val(a) + val([a[0], a[1], a[2], 0.0])) < mp.mpf(2) ** -250
