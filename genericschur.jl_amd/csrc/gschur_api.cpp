@@ -19,31 +19,12 @@
 #include <vector>
 
 #include "../../include/gschur_mi355x.h"
+#include "handle.h"
 #include "internal.h"
 
 using namespace gs;
 
-namespace {
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t cap = 0;
-  hipError_t ensure(size_t bytes) {
-    if (bytes <= cap) return hipSuccess;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-    size_t want = std::max<size_t>(bytes, 256);
-    hipError_t e = hipMalloc(&p, want);
-    if (e == hipSuccess) cap = want;
-    return e;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-  }
-};
+namespace gs {
 
 struct Bin {
   int first;  // index into the order array
@@ -54,8 +35,6 @@ struct Bin {
   int branch; // graph branch (stream) the bin runs on
   double work;
 };
-
-constexpr int kBranches = 4;
 
 struct Plan {
   bool is_complex = false;
@@ -84,38 +63,10 @@ struct Plan {
   }
 };
 
-}  // namespace
+}  // namespace gs
 
-struct gs_handle {
-  int device = 0;
-  hipStream_t stream = nullptr;      // stream in use
-  hipStream_t own_stream = nullptr;  // stream created with the handle
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipStream_t branch[kBranches] = {};
-  hipEvent_t evfork = nullptr, evjoin[kBranches] = {};
-  std::string err;
-  DevBuf bA, bT, bZ, bW, bInfo, bWs, bStats, bMeta, bScl;
-  std::map<std::string, std::unique_ptr<Plan>> plans;
-  Plan* last_plan = nullptr;
-  int64_t last_stats[3] = {0, 0, 0};
-  bool last_stats_from_plan = false;
-  bool last_single_stats = false;
-  std::mutex mu;
-};
-
-static int fail_hip(gs_handle* h, hipError_t e, const char* where) {
-  if (h) {
-    char buf[256];
-    std::snprintf(buf, sizeof buf, "%s: %s", where, hipGetErrorString(e));
-    h->err = buf;
-  }
-  return GS_E_HIP - (int)e;
-}
-
-static int fail(gs_handle* h, int code, const char* msg) {
-  if (h) h->err = msg;
-  return code;
-}
+static int fail_hip(gs_handle* h, hipError_t e, const char* where) { return handle_fail_hip(h, e, where); }
+static int fail(gs_handle* h, int code, const char* msg) { return handle_fail(h, code, msg); }
 
 #define HIPCHK(h, call)                                  \
   do {                                                   \
@@ -169,6 +120,7 @@ int gs_destroy(gs_handle* h) {
   h->bStats.release();
   h->bMeta.release();
   h->bScl.release();
+  for (int i = 0; i < kAuxBufs; ++i) h->aux[i].release();
   for (int i = 0; i < kBranches; ++i) {
     if (h->evjoin[i]) (void)hipEventDestroy(h->evjoin[i]);
     if (h->branch[i]) (void)hipStreamDestroy(h->branch[i]);

@@ -1,0 +1,79 @@
+// The library handle (struct gs_handle) shared by the host translation
+// units, and small helpers around it.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "../../include/gschur_mi355x.h"
+
+namespace gs {
+
+// grow-only device buffer
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct Plan;
+constexpr int kBranches = 4;
+constexpr int kAuxBufs = 8;
+
+}  // namespace gs
+
+struct gs_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;      // stream in use
+  hipStream_t own_stream = nullptr;  // stream created with the handle
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipStream_t branch[gs::kBranches] = {};
+  hipEvent_t evfork = nullptr, evjoin[gs::kBranches] = {};
+  std::string err;
+  gs::DevBuf bA, bT, bZ, bW, bInfo, bWs, bStats, bMeta, bScl;
+  gs::DevBuf aux[gs::kAuxBufs];  // scratch of the eigenvector / VJP entry points
+  std::map<std::string, std::unique_ptr<gs::Plan>> plans;
+  gs::Plan* last_plan = nullptr;
+  bool last_stats_from_plan = false;
+  std::mutex mu;
+};
+
+namespace gs {
+
+inline int handle_fail_hip(gs_handle* h, hipError_t e, const char* where) {
+  if (h) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%s: %s", where, hipGetErrorString(e));
+    h->err = buf;
+  }
+  return GS_E_HIP - (int)e;
+}
+inline int handle_fail(gs_handle* h, int code, const char* msg) {
+  if (h) h->err = msg;
+  return code;
+}
+inline std::mutex& handle_mutex(gs_handle* h) { return h->mu; }
+inline int handle_device(gs_handle* h) { return h->device; }
+inline hipStream_t handle_stream(gs_handle* h) { return h->stream; }
+inline DevBuf& handle_buf(gs_handle* h, int i) { return h->aux[i]; }
+
+}  // namespace gs

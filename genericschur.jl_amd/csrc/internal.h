@@ -64,4 +64,44 @@ hipError_t launch_dqr(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 hipError_t launch_zschur_global(const SingleArgs& a, int nt, hipStream_t s);
 hipError_t launch_dschur_global(const SingleArgs& a, int nt, hipStream_t s);
 
+// Eigenvector kernels (one workgroup per matrix).  T, Z at element offset
+// off[m] (complex elements for the complex kernel), V (complex) at complex
+// offset off[m].  ld* > 0 gives leading dimensions for a single matrix;
+// 0 means ld = n.
+struct TrevcArgs {
+  const int32_t* order;
+  int count;
+  const int32_t* n;
+  const int64_t* off;
+  const double* T;
+  const double* Z;  // may be null
+  double* V;
+  int left;
+  int normalize;
+  int64_t ldt, ldz, ldv;
+};
+typedef TrevcArgs TrevcArgsD;
+
+size_t ztrevc_lds_bytes(int n);
+size_t dtrevc_lds_bytes(int n);
+hipError_t launch_ztrevc(const TrevcArgs& a, int nmax, hipStream_t s);
+hipError_t launch_dtrevc(const TrevcArgsD& a, int nmax, hipStream_t s);
+
+// Eigenvalue VJP assembly: VR, VL (complex) at complex offset off[m], gbar
+// (complex) at woff[m], Abar (real) at off[m].
+struct VjpArgs {
+  const int32_t* order;
+  int count;
+  const int32_t* n;
+  const int64_t* off;
+  const int64_t* woff;
+  const double* VR;
+  const double* VL;
+  const double* gbar;
+  double* Abar;
+  int32_t* info;
+};
+size_t vjp_lds_bytes(int n);
+hipError_t launch_vjp_assemble(const VjpArgs& a, int nmax, hipStream_t s);
+
 }  // namespace gs

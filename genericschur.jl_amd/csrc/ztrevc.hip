@@ -1,0 +1,475 @@
+// Eigenvectors of a complex Schur factorization (T upper triangular, Z) on
+// gfx950, following GenericSchur.jl:
+//   right   vectors.jl:45-131   with _usolve!  util.jl:128-293 (zlatrs)
+//   left    vectors.jl:372-460  with _cusolve! util.jl:301-461
+//   unit 2-norm, largest entry real: _enormalize! util.jl:572-592
+//
+// One workgroup per matrix, NT/64 wavefronts.  T is held in LDS in compact
+// upper-triangular storage.  Each wavefront solves whole eigenvectors: the
+// right-hand side / solution lives in registers across its 64 lanes
+// (WaveVecZ), the substitution walks T's columns (right) or rows (left) in
+// LDS, and the scalar control of the scaled solver runs uniformly in every
+// lane.  The solutions are parked in the output V, then each column is
+// back-transformed by Z (V[:, k] = Z x) and normalised.
+#include "dev_common.h"
+#include "internal.h"
+#include "wavevec.h"
+
+namespace gs {
+
+
+// compact upper triangle: column j (0-based) holds rows 0..j at j(j+1)/2
+GS_DEV int tri(int i, int j) { return (j * (j + 1)) / 2 + i; }
+
+// ------------------------------------------------------------------------
+// Shifted, clamped diagonal of the solve: T(j,j) - lam, replaced by smin
+// when its abs1 is below smin (vectors.jl:87-90 / :412-415).
+GS_DEV zd shifted_diag(const zd* Tc, int j, zd lam, double smin) {
+  zd d = Tc[tri(j, j)] - lam;
+  if (zabs1(d) < smin) d = zr(smin);
+  return d;
+}
+
+// _usolve! on entries 0..m-1 of x (A = T shifted by lam, leading m x m).
+template <int R> GS_DEV double usolve_w(const zd* Tc, const double* cnorm, int m, zd lam, double smin,
+                                        WaveVecZ<R>& x) {
+  const int lane = threadIdx.x & 63;
+  const double half = 0.5;
+  const double smallnum = GS_FMIN / GS_EPS;
+  const double bignum = 1.0 / smallnum;
+  double xscale = 1.0;
+  auto inrange = [&](int q) { return lane + 64 * q < m; };
+  auto cabs1half = [](zd z) { return fabs(z.x / 2) + fabs(z.y / 2); };
+  double tmax = 0.0;
+  for (int j = lane; j < m; j += 64) tmax = fmax(tmax, fabs(cnorm[j]));
+  tmax = wave_max(tmax);
+  double tscale;
+  if (tmax <= bignum * half) tscale = 1.0;
+  else tscale = half / (smallnum * tmax);
+  double xmax = 0.0;
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+    if (inrange(q)) xmax = fmax(xmax, cabs1half(x.v[q]));
+  xmax = wave_max(xmax);
+  double xbound = xmax;
+  double grow;
+  if (tscale != 1.0) {
+    grow = 0.0;
+  } else {
+    grow = half / fmax(xbound, smallnum);
+    xbound = grow;
+    bool toosmall = false;
+    for (int j = m - 1; j >= 0; --j) {
+      toosmall = (grow <= smallnum);
+      if (toosmall) break;
+      const double tjj = zabs1(shifted_diag(Tc, j, lam, smin));
+      if (tjj >= smallnum) xbound = fmin(xbound, fmin(1.0, tjj) * grow);
+      else xbound = 0.0;
+      const double cn = cnorm[j] * tscale;
+      if (tjj + cn >= smallnum) grow *= (tjj / (tjj + cn));
+      else grow = 0.0;
+    }
+    if (!toosmall) grow = xbound;
+  }
+  if (grow * tscale > smallnum) {
+    for (int j = m - 1; j >= 0; --j) {
+      const zd xj = x.get(j) / shifted_diag(Tc, j, lam, smin);
+      x.set(j, xj);
+      const int cb = (j * (j + 1)) / 2;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int i = lane + 64 * q;
+        if (i < j) x.v[q] = x.v[q] - Tc[cb + i] * xj;
+      }
+    }
+    return xscale;
+  }
+  // careful scaled solve
+  if (xmax > bignum * half) {
+    xscale = (bignum * half) / xmax;
+    x.scale(xscale);
+    xmax = bignum;
+  } else {
+    xmax *= 2.0;
+  }
+  for (int j = m - 1; j >= 0; --j) {
+    zd xjv = x.get(j);
+    double xj = zabs1(xjv);
+    const zd tjjs = shifted_diag(Tc, j, lam, smin) * tscale;
+    const double tjj = zabs1(tjjs);
+    const double cnj = cnorm[j] * tscale;
+    if (tjj > smallnum) {
+      if (tjj < 1.0 && xj > tjj * bignum) {
+        const double rec = 1.0 / xj;
+        x.scale(rec);
+        xscale *= rec;
+        xmax *= rec;
+        xjv = x.get(j);
+      }
+      xjv = xjv / tjjs;
+      x.set(j, xjv);
+      xj = zabs1(xjv);
+    } else if (tjj > 0.0) {
+      if (xj > tjj * bignum) {
+        double rec = (tjj * bignum) / xj;
+        if (cnj > 1.0) rec /= cnj;
+        x.scale(rec);
+        xscale *= rec;
+        xmax *= rec;
+        xjv = x.get(j);
+      }
+      xjv = xjv / tjjs;
+      x.set(j, xjv);
+      xj = zabs1(xjv);
+    } else {
+#pragma unroll
+      for (int q = 0; q < R; ++q) x.v[q] = zr(0.0);
+      x.set(j, zr(1.0));
+      xjv = zr(1.0);
+      xscale = 0.0;
+      xmax = 0.0;
+    }
+    if (xj > 1.0) {
+      double rec = 1.0 / xj;
+      if (cnj > (bignum - xmax) * rec) {
+        rec *= half;
+        x.scale(rec);
+        xscale *= rec;
+      }
+    } else if (xj * cnj > bignum - xmax) {
+      x.scale(half);
+      xscale *= half;
+    }
+    if (j > 0) {
+      const zd xjt = x.get(j) * tscale;
+      const int cb = (j * (j + 1)) / 2;
+      double xm = 0.0;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int i = lane + 64 * q;
+        if (i < j) {
+          x.v[q] = x.v[q] - xjt * Tc[cb + i];
+          xm = fmax(xm, zabs1(x.v[q]));
+        }
+      }
+      xmax = wave_max(xm);
+    }
+  }
+  xscale /= tscale;
+  return xscale;
+}
+
+// _cusolve! on entries o..o+m-1 of x: (A)^H y = s b with
+// A(i, j) = T(o+i, o+j) for i < j and the shifted, clamped diagonal; cnorm
+// is indexed from 0 (the reference passes the full tnorms, vectors.jl:417-420).
+template <int R> GS_DEV double cusolve_w(const zd* Tc, const double* cnorm, int o, int m, zd lam, double smin,
+                                         WaveVecZ<R>& x) {
+  const int lane = threadIdx.x & 63;
+  const double half = 0.5;
+  const double smallnum = GS_FMIN / GS_EPS;
+  const double bignum = 1.0 / smallnum;
+  double xscale = 1.0;
+  auto inrange = [&](int q) {
+    const int i = lane + 64 * q;
+    return i >= o && i < o + m;
+  };
+  auto cabs1half = [](zd z) { return fabs(z.x / 2) + fabs(z.y / 2); };
+  double tmax = 0.0;
+  for (int j = lane; j < m; j += 64) tmax = fmax(tmax, fabs(cnorm[j]));
+  tmax = wave_max(tmax);
+  double tscale;
+  if (tmax <= bignum * half) tscale = 1.0;
+  else tscale = half / (smallnum * tmax);
+  double xmax = 0.0;
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+    if (inrange(q)) xmax = fmax(xmax, cabs1half(x.v[q]));
+  xmax = wave_max(xmax);
+  double xbound = xmax;
+  double grow;
+  if (tscale != 1.0) {
+    grow = 0.0;
+  } else {
+    grow = half / fmax(xbound, smallnum);
+    xbound = grow;
+    bool toosmall = false;
+    for (int j = 0; j < m; ++j) {
+      toosmall = (grow <= smallnum);
+      if (toosmall) break;
+      const double xj = 1.0 + cnorm[j] * tscale;
+      grow = fmin(grow, xbound / xj);
+      const double tjj = zabs1(shifted_diag(Tc, o + j, lam, smin));
+      if (tjj >= smallnum) {
+        if (xj > tjj) xbound *= (tjj / xj);
+      } else {
+        xbound = 0.0;
+      }
+    }
+    if (!toosmall) grow = fmin(grow, xbound);
+  }
+  if (grow * tscale > smallnum) {
+    // forward substitution in axpy form: after x[j] is final, remove its
+    // contribution from the later right-hand sides
+    for (int j = 0; j < m; ++j) {
+      const zd xj = x.get(o + j) / zconj(shifted_diag(Tc, o + j, lam, smin));
+      x.set(o + j, xj);
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int i = lane + 64 * q;  // absolute entry index
+        if (i > o + j && i < o + m) x.v[q] = x.v[q] - zconj(Tc[tri(o + j, i)]) * xj;
+      }
+    }
+    return xscale;
+  }
+  if (xmax > bignum * half) {
+    xscale = (bignum * half) / xmax;
+    x.scale(xscale);
+    xmax = bignum;
+  } else {
+    xmax *= 2.0;
+  }
+  for (int j = 0; j < m; ++j) {
+    double xj = zabs1(x.get(o + j));
+    zd uscale = zr(tscale);
+    double rec = 1.0 / fmax(xmax, 1.0);
+    zd tjjs = zconj(shifted_diag(Tc, o + j, lam, smin)) * tscale;
+    const double cnj = cnorm[j] * tscale;
+    if (cnj > (bignum - xj) * rec) {
+      rec *= half;
+      const double tjj = zabs1(tjjs);
+      if (tjj > 1.0) {
+        rec = fmin(1.0, rec * tjj);
+        uscale = uscale / tjjs;
+      }
+      if (rec < 1.0) {
+        x.scale(rec);
+        xscale *= rec;
+        xmax *= rec;
+      }
+    }
+    // csumj = sum_{i<j} (conj(A(i,j)) * uscale) * x[i]
+    zd part = zr(0.0);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int i = lane + 64 * q;
+      if (i >= o && i < o + j) part = part + (zconj(Tc[tri(i, o + j)]) * uscale) * x.v[q];
+    }
+    const zd csumj = wave_zsum(part);
+    const bool unscaled = (uscale.x == tscale && uscale.y == 0.0) || isnan(uscale.x) || isnan(uscale.y);
+    if (unscaled) {
+      zd xjv = x.get(o + j) - csumj;
+      xj = zabs1(xjv);
+      tjjs = zconj(shifted_diag(Tc, o + j, lam, smin)) * tscale;
+      const double tjj = zabs1(tjjs);
+      if (tjj > smallnum) {
+        if (tjj < 1.0 && xj > tjj * bignum) {
+          const double rec2 = 1.0 / xj;
+          x.scale(rec2);
+          xscale *= rec2;
+          xmax *= rec2;
+          xjv = xjv * rec2;
+        }
+        xjv = xjv / tjjs;
+        x.set(o + j, xjv);
+      } else if (tjj > 0.0) {
+        if (xj > tjj * bignum) {
+          const double rec2 = (tjj * bignum) / xj;
+          x.scale(rec2);
+          xscale *= rec2;
+          xmax *= rec2;
+          xjv = xjv * rec2;
+        }
+        xjv = xjv / tjjs;
+        x.set(o + j, xjv);
+      } else {
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          if (inrange(q)) x.v[q] = zr(0.0);
+        x.set(o + j, zr(1.0));
+        xscale = 0.0;
+        xmax = 0.0;
+      }
+    } else {
+      x.set(o + j, x.get(o + j) / tjjs - csumj);
+    }
+    xmax = fmax(xmax, zabs1(x.get(o + j)));
+  }
+  xscale /= tscale;
+  return xscale;
+}
+
+// max abs1 scaling then _enormalize! of one column held in a WaveVecZ
+// (entries 0..n-1); normalize = 0 stops after the max-abs1 scaling.
+template <int R> GS_DEV void normalize_col(WaveVecZ<R>& v, int n, bool enorm) {
+  const int lane = threadIdx.x & 63;
+  double t0 = 0.0;
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+    if (lane + 64 * q < n) t0 = fmax(t0, zabs1(v.v[q]));
+  t0 = wave_max(t0);
+  const double remax = 1.0 / t0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) v.v[q] = v.v[q] * remax;
+  if (!enorm) return;
+  // 2-norm without over/underflow, and the first index of the largest |v|^2
+  double am = 0.0;
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+    if (lane + 64 * q < n) am = fmax(am, fmax(fabs(v.v[q].x), fabs(v.v[q].y)));
+  am = wave_max(am);
+  double ss = 0.0;
+  double best = -1.0;
+  int bi = 0x7fffffff;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = lane + 64 * q;
+    if (i < n) {
+      const double a = v.v[q].x / am, b = v.v[q].y / am;
+      ss += a * a + b * b;
+      const double u = zabs2(v.v[q]);
+      if (u > best) {
+        best = u;
+        bi = i;
+      }
+    }
+  }
+  ss = wave_sum(ss);
+  const double nrm = am * sqrt(ss);
+  // largest |v|^2, first index among ties (util.jl:578-584 uses strict >)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+  }
+  const zd vi0 = v.get(bi);
+  const double s = 1.0 / nrm;
+  const zd tt = (s * zconj(vi0)) / sqrt(zabs2(vi0));
+#pragma unroll
+  for (int q = 0; q < R; ++q) v.v[q] = v.v[q] * tt;
+  const zd r0 = v.get(bi);
+  v.set(bi, zr(r0.x));
+}
+
+template <int NT, int R> __global__ void __launch_bounds__(NT) ztrevc_kernel(TrevcArgs a) {
+  extern __shared__ double smem[];
+  const int b = blockIdx.x;
+  if (b >= a.count) return;
+  const int m = a.order ? a.order[b] : b;
+  const int n = a.n[m];
+  if (n <= 0) return;
+  const int64_t o = a.off[m];
+  const int64_t ldt = a.ldt > 0 ? a.ldt : n;
+  const int64_t ldz = a.ldz > 0 ? a.ldz : n;
+  const int64_t ldv = a.ldv > 0 ? a.ldv : n;
+  const zd* Tg = reinterpret_cast<const zd*>(a.T) + o;
+  const zd* Zg = a.Z ? reinterpret_cast<const zd*>(a.Z) + o : nullptr;
+  zd* Vg = reinterpret_cast<zd*>(a.V) + o;
+  zd* Tc = reinterpret_cast<zd*>(smem);
+  double* tnorms = reinterpret_cast<double*>(Tc + (n * (n + 1)) / 2);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  constexpr int NW = NT / 64;
+  const bool left = a.left != 0;
+  for (int j = 0; j < n; ++j)
+    for (int i = tid; i <= j; i += NT) Tc[tri(i, j)] = Tg[i + j * ldt];
+  __syncthreads();
+  // strictly-upper column norms: abs for right (vectors.jl:70-75), abs1 for
+  // left (vectors.jl:395-400)
+  for (int j = tid; j < n; j += NT) {
+    double s = 0.0;
+    for (int i = 0; i < j; ++i) s += left ? zabs1(Tc[tri(i, j)]) : zabs(Tc[tri(i, j)]);
+    tnorms[j] = s;
+  }
+  __syncthreads();
+  const double ulp = GS_EPS;
+  const double smallnum = GS_FMIN * (double)n;  // safemin * n (vectors.jl:61,386)
+  // solves, one eigenvector per wavefront at a time; the solution (with the
+  // scale at the diagonal slot) is parked in V's column
+  for (int kk = (left ? w : n - 1 - w); left ? (kk < n) : (kk >= 0); kk += (left ? NW : -NW)) {
+    const zd lam = Tc[tri(kk, kk)];
+    const double smin = fmax(ulp * zabs1(lam), smallnum);
+    WaveVecZ<R> x;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int i = lane + 64 * q;
+      zd v = zr(0.0);
+      if (!left) {
+        if (i < kk) v = -Tc[tri(i, kk)];
+      } else {
+        if (i > kk && i < n) v = -zconj(Tc[tri(kk, i)]);
+      }
+      x.v[q] = v;
+    }
+    double vscale;
+    if (!left) {
+      vscale = (kk > 0) ? usolve_w<R>(Tc, tnorms, kk, lam, smin, x) : 1.0;
+    } else {
+      vscale = (kk < n - 1) ? cusolve_w<R>(Tc, tnorms, kk + 1, n - 1 - kk, lam, smin, x) : 1.0;
+    }
+    x.set(kk, zr(vscale));
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int i = lane + 64 * q;
+      if (i < n) Vg[i + kk * ldv] = (left ? (i >= kk) : (i <= kk)) ? x.v[q] : zr(0.0);
+    }
+  }
+  __syncthreads();
+  // back-transform by Z and normalise, one column per wavefront
+  for (int kk = w; kk < n; kk += NW) {
+    WaveVecZ<R> x;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int i = lane + 64 * q;
+      x.v[q] = (i < n) ? Vg[i + kk * ldv] : zr(0.0);
+    }
+    WaveVecZ<R> y;
+    if (Zg) {
+      const zd vs = x.get(kk);
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int r = lane + 64 * q;
+        y.v[q] = (r < n) ? vs * Zg[r + kk * ldz] : zr(0.0);
+      }
+      const int i0 = left ? kk + 1 : 0, i1 = left ? n : kk;
+      for (int i = i0; i < i1; ++i) {
+        const zd xi = x.get(i);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int r = lane + 64 * q;
+          if (r < n) y.v[q] = y.v[q] + Zg[r + (int64_t)i * ldz] * xi;
+        }
+      }
+    } else {
+      y = x;
+    }
+    normalize_col<R>(y, n, a.normalize != 0);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int r = lane + 64 * q;
+      if (r < n) Vg[r + kk * ldv] = y.v[q];
+    }
+  }
+}
+
+size_t ztrevc_lds_bytes(int n) { return 16 * ((size_t)n * (n + 1) / 2) + 8 * (size_t)n + 64; }
+
+hipError_t launch_ztrevc(const TrevcArgs& a, int nmax, hipStream_t s) {
+  if (a.count <= 0) return hipSuccess;
+  const size_t lds = ztrevc_lds_bytes(nmax);
+  constexpr int NT = 256;
+  if (nmax <= 64) {
+    (void)hipFuncSetAttribute((const void*)ztrevc_kernel<NT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((ztrevc_kernel<NT, 1>), dim3(a.count), dim3(NT), lds, s, a);
+  } else if (nmax <= 128) {
+    (void)hipFuncSetAttribute((const void*)ztrevc_kernel<NT, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((ztrevc_kernel<NT, 2>), dim3(a.count), dim3(NT), lds, s, a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace gs

@@ -126,13 +126,15 @@ int gs_trevc_batched_d(gs_handle* h, int64_t count, const int32_t* n, const int6
                        const double* Z, double* V, int left, int normalize, int flags);
 
 /* Eigenvalue vector-Jacobian product (no counterpart in the reference; the
- * backward of the north-star rrule).  For the Schur factorizations (T, Z)
- * of real matrices A_i and cotangents gbar_i = dL/dRe(lambda) + i dL/dIm(lambda)
- * per eigenvalue (complex, at the eigenvalue offsets of gs_schur_batched_d),
- * writes Abar_i = Re( X^{-H} diag(gbar) X^H ) with X the right eigenvectors,
- * computed from right and left eigenvectors recomputed from T and Z:
- * Abar = sum_k conj(gbar_k) ... see the library documentation.  Abar is real
- * at off[i].  info[i] = 0, or 1 when some y_k^H x_k vanished. */
+ * backward of an eigenvalue rrule).  For the real Schur factorizations
+ * (T, Z) of real matrices A_i (as returned by gs_schur_batched_d) and
+ * cotangents gbar_k = dL/dRe(lambda_k) + i dL/dIm(lambda_k), one complex
+ * value per eigenvalue slot at the eigenvalue offsets of gs_schur_batched_d,
+ * writes Abar_i = Re( sum_k gbar_k / conj(y_k^H x_k) y_k x_k^H )
+ * = Re( X^{-H} diag(gbar) X^H ), with the right (x_k) and left (y_k)
+ * eigenvectors recomputed from T and Z on the device.  Abar is real at
+ * off[i].  info[i] = 0, or 1 when some y_k^H x_k vanished (defective).
+ * Orders up to 70. */
 int gs_eig_vjp_batched_d(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off, const double* T,
                          const double* Z, const double* gbar, double* Abar, int32_t* info, int flags);
 
