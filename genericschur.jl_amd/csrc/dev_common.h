@@ -276,3 +276,76 @@ GS_DEV double dreflector(int nr, double& a, double& x1, double& x2) {
 
 // safemin(Float64) (util.jl:5-12) is floatmin for IEEE double.
 #define GS_SAFEMIN GS_FMIN
+
+// ------------------------------------------------ low-latency reflectors
+// The same reflectors (householder.jl:12-102) for the bulge chase, with the
+// norms folded into one scaled square root and the divisions merged into a
+// single reciprocal.  Same result up to rounding.  Outside |beta| in
+// [2^-300, 2^300] (where the merged reciprocal could over/underflow) they
+// fall back to the reference formulas above.
+GS_DEV zd zreflector2_fast(zd& a, zd& x) {
+  const double ar = a.x, ai = a.y;
+  if (x.x == 0.0 && x.y == 0.0 && ai == 0.0) return zr(0.0);
+  const double w = fmax(fmax(fabs(ar), fabs(ai)), fmax(fabs(x.x), fabs(x.y)));
+  const double rw = 1.0 / w;
+  const double p = ar * rw, q = ai * rw, s = x.x * rw, t = x.y * rw;
+  const double beta = -copysign(w * sqrt(p * p + q * q + s * s + t * t), ar);
+  const double ab = fabs(beta);
+  if (!(ab >= 0x1p-300 && ab <= 0x1p300)) return zreflector2(a, x);
+  // tau = ((beta - ar) / beta, -ai / beta);  1 / (alpha - beta) = conj(d) / |d|^2
+  const double dr = ar - beta;
+  const double dd = dr * dr + ai * ai;
+  const double r = 1.0 / (beta * dd);
+  const double rb = dd * r;     // 1 / beta
+  const double rdd = beta * r;  // 1 / |d|^2
+  const zd tau = zmk((beta - ar) * rb, -ai * rb);
+  const zd inv = zmk(dr * rdd, -ai * rdd);
+  x = x * inv;
+  a = zr(beta);
+  return tau;
+}
+
+GS_DEV double dreflector_fast(int nr, double& a, double& x1, double& x2) {
+  if (nr <= 1) return 0.0;
+  const double alpha = a;
+  const double x2v = (nr == 3) ? x2 : 0.0;
+  if (x1 == 0.0 && x2v == 0.0) return 0.0;
+  const double w = fmax(fabs(alpha), fmax(fabs(x1), fabs(x2v)));
+  const double rw = 1.0 / w;
+  const double p = alpha * rw, q = x1 * rw, s = x2v * rw;
+  const double beta = -copysign(w * sqrt(p * p + q * q + s * s), alpha);
+  const double ab = fabs(beta);
+  if (!(ab >= 0x1p-300 && ab <= 0x1p300)) return dreflector(nr, a, x1, x2);
+  // tau = (beta - alpha) / beta, scale = 1 / (alpha - beta)
+  const double d = alpha - beta;
+  const double r = 1.0 / (beta * d);
+  const double tau = -(d * d) * r;
+  const double sc = beta * r;
+  x1 *= sc;
+  if (nr == 3) x2 *= sc;
+  a = beta;
+  return tau;
+}
+
+// uniform broadcast of lane l's double (l must be wave-uniform)
+GS_DEV double rdlane(double v, int l) {
+  const unsigned long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffu), l);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+  return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// ordering point for LDS traffic among the lanes of one wavefront: the
+// hardware processes a wave's LDS operations in order, so only compiler
+// reordering has to be prevented
+GS_DEV void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// block synchronisation that degrades to a wavefront ordering point when
+// the block is a single wavefront
+template <int NT> GS_DEV void bsync_full() {
+  if (NT > 64) __syncthreads();
+  else wsync();
+}

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -311,7 +312,13 @@ static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n,
   h->last_plan = p;
   h->last_stats_from_plan = true;
   if (count == 0) return GS_OK;
-  const int kflags = flags & (GS_WANTZ | GS_SCALE);
+  // GSCHUR_REFERENCE_SWEEP=1 selects the sweep written in the reference's
+  // form (LDS round trips, full divisions) for A/B measurements
+  static const bool ref_sweep = [] {
+    const char* v = std::getenv("GSCHUR_REFERENCE_SWEEP");
+    return v && v[0] == '1';
+  }();
+  const int kflags = (flags & (GS_WANTZ | GS_SCALE)) | (ref_sweep ? 0x1000 : 0);
   const size_t es = cplx ? 16 : 8;
   double* dZdummy = nullptr;
   if (flags & GS_DEVICE_PTRS) {

@@ -26,12 +26,13 @@ template <int B> struct BandLayout {
     if (nb < 0) nb = 0;
     cut = (int64_t)nb * (nb + 1 + 2 * B) / 2;
   }
-  __device__ __forceinline__ int64_t col(int j) const {
+  // 32-bit arithmetic: band matrices live in LDS, far below 2^31 elements
+  __device__ __forceinline__ int col(int j) const {
     const int jm = j - 1;
-    if (jm <= nb) return (int64_t)jm * (jm + 1 + 2 * B) / 2;
-    return cut + (int64_t)(jm - nb) * n;
+    if (jm <= nb) return (jm * (jm + 1 + 2 * B)) >> 1;
+    return (int)cut + (jm - nb) * n;
   }
-  __device__ __forceinline__ int64_t idx(int i, int j) const { return col(j) + (i - 1); }
+  __device__ __forceinline__ int idx(int i, int j) const { return col(j) + (i - 1); }
   // number of stored elements for order n
   static __host__ __device__ int64_t size(int n_) {
     int nb_ = n_ - B;

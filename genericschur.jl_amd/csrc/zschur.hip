@@ -262,7 +262,7 @@ template <int NT> GS_DEV void z_apply_chain(zd* Zg, int64_t ldz, int n, const zd
 
 // ------------------------------------------------------------------------
 // singleShiftQR! (GenericSchur.jl:374-504) on H with deferred Z.
-template <int NT, class L> GS_DEV void z_sweep(zd* H, L lay, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot,
+template <int NT, class L, bool FAST> GS_DEV void z_sweep(zd* H, L lay, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot,
                                                int* redi, zd shift, int istart, int iend, int64_t& nstep) {
   const int tid = threadIdx.x;
   const double ulp = GS_EPS;
@@ -298,13 +298,26 @@ template <int NT, class L> GS_DEV void z_sweep(zd* H, L lay, int n, zd* Zg, int6
   }
   bool has_ph = false;
   zd ph = zr(1.0);
+  // FAST: the next step's reflector input (H[k+1,k], H[k+2,k]) is taken
+  // from the registers of the lanes that computed it in the right update
+  // (single-wavefront blocks), the reflector uses fewer divisions, and the
+  // block barriers are wavefront ordering points
+  constexpr bool RL = FAST && NT == 64;
+  bool reload = true;
+  zd nv0 = zr(0.0), nv1 = zr(0.0);
   for (int k = istart1; k <= iend - 1; ++k) {
     if (k > istart1) {
-      v0 = HH(k, k - 1);
-      v1 = HH(k + 1, k - 1);
+      if (RL && !reload) {
+        v0 = nv0;
+        v1 = nv1;
+      } else {
+        v0 = HH(k, k - 1);
+        v1 = HH(k + 1, k - 1);
+      }
     }
-    const zd tau1 = zreflector2(v0, v1);
-    bsync<NT>();
+    const zd tau1 = FAST ? zreflector2_fast(v0, v1) : zreflector2(v0, v1);
+    if (FAST) bsync_full<NT>();
+    else bsync<NT>();
     if (k > istart1 && tid == 0) {
       HH(k, k - 1) = v0;
       HH(k + 1, k - 1) = zr(0.0);
@@ -319,22 +332,37 @@ template <int NT, class L> GS_DEV void z_sweep(zd* H, L lay, int n, zd* Zg, int6
       HH(k, j) = a - ss;
       HH(k + 1, j) = b - ss * v2;
     }
-    __syncthreads();
+    if (FAST) bsync_full<NT>();
+    else __syncthreads();
     // right: columns k, k+1 over rows 1..min(k+2, iend)
     const int jm = min(k + 2, iend);
     const zd cv2 = zconj(v2);
+    zd cap0 = zr(0.0), cap1 = zr(0.0);
     for (int j = 1 + tid; j <= jm; j += NT) {
       const zd a = HH(j, k), b = HH(j, k + 1);
       const zd ss = tau1 * a + tau2 * b;
-      HH(j, k) = a - ss;
+      const zd na = a - ss;
+      HH(j, k) = na;
       HH(j, k + 1) = b - ss * cv2;
+      if (RL) {
+        if (j == k + 1) cap0 = na;
+        if (j == k + 2) cap1 = na;
+      }
     }
     if (wantz && tid == 0) {
       rot[2 * (k - istart1)] = tau1;
       rot[2 * (k - istart1) + 1] = v2;
     }
-    __syncthreads();
+    if (RL) {
+      const int l0 = k & 63, l1 = (k + 1) & 63;
+      nv0 = zmk(rdlane(cap0.x, l0), rdlane(cap0.y, l0));
+      nv1 = zmk(rdlane(cap1.x, l1), rdlane(cap1.y, l1));
+      reload = false;
+    }
+    if (FAST) bsync_full<NT>();
+    else __syncthreads();
     if (k == istart1 && istart1 > istart) {
+      reload = true;  // H[istart1+1, istart1] is rescaled below
       zd t = zr(1.0) - tau1;
       t = t / zabs(t);
       has_ph = true;
@@ -383,7 +411,7 @@ template <int NT, class L> GS_DEV void z_sweep(zd* H, L lay, int n, zd* Zg, int6
 // ------------------------------------------------------------------------
 // gschur!(H::Hessenberg{Complex}, Z) (GenericSchur.jl:194-335).
 // Returns 0, or the positive iteration count when maxiter was exceeded.
-template <int NT, class L> GS_DEV int64_t z_qr(zd* H, L lay, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot,
+template <int NT, class L, bool FAST> GS_DEV int64_t z_qr(zd* H, L lay, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot,
                                                int* redi, int64_t maxiter, int64_t maxinner, int64_t& nsweep,
                                                int64_t& nstep) {
   const int tid = threadIdx.x;
@@ -458,7 +486,7 @@ template <int NT, class L> GS_DEV int64_t z_qr(zd* H, L lay, int n, zd* Zg, int6
         }
       }
       nsweep += 1;
-      z_sweep<NT>(H, lay, n, Zg, ldz, wantz, rot, redi, t, istart, iend, nstep);
+      z_sweep<NT, L, FAST>(H, lay, n, Zg, ldz, wantz, rot, redi, t, istart, iend, nstep);
     }
   }
   return 0;
@@ -528,7 +556,7 @@ template <int NT> __global__ void __launch_bounds__(NT) zhq_kernel(BatchArgs a) 
 }
 
 // Batched kernel 2: QR iterations with the band of H in LDS, Z in HBM.
-template <int NT> __global__ void __launch_bounds__(NT) zqr_kernel(BatchArgs a) {
+template <int NT, bool FAST> __global__ void __launch_bounds__(NT) zqr_kernel(BatchArgs a) {
   extern __shared__ double smem[];
   const int b = blockIdx.x;
   if (b >= a.count) return;
@@ -556,7 +584,7 @@ template <int NT> __global__ void __launch_bounds__(NT) zqr_kernel(BatchArgs a) 
   int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
   int64_t maxinner = a.maxinner > 0 ? a.maxinner : 30 * (int64_t)n;
   int64_t nsweep = 0, nstep = 0;
-  const int64_t st = z_qr<NT>(H, lay, n, Z, (int64_t)n, wantz, rot, redi, maxiter, maxinner, nsweep, nstep);
+  const int64_t st = z_qr<NT, BandLayout<2>, FAST>(H, lay, n, Z, (int64_t)n, wantz, rot, redi, maxiter, maxinner, nsweep, nstep);
   __syncthreads();
   if (threadIdx.x == 0) {
     if (a.stats) {
@@ -619,7 +647,7 @@ template <int NT> __global__ void __launch_bounds__(NT) zschur_global_kernel(Sin
   const int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
   const int64_t maxinner = a.maxinner > 0 ? a.maxinner : 30 * (int64_t)n;
   int64_t nsweep = 0, nstep = 0;
-  const int64_t st = z_qr<NT>(T, lay, n, Z, ldz, wantz, rot, redi, maxiter, maxinner, nsweep, nstep);
+  const int64_t st = z_qr<NT, FullLayout, true>(T, lay, n, Z, ldz, wantz, rot, redi, maxiter, maxinner, nsweep, nstep);
   __syncthreads();
   if (threadIdx.x == 0) {
     if (a.stats) {
@@ -655,8 +683,15 @@ template <int NT> static hipError_t launch_zhq_nt(const BatchArgs& a, size_t lds
   return hipGetLastError();
 }
 template <int NT> static hipError_t launch_zqr_nt(const BatchArgs& a, size_t lds, hipStream_t s) {
-  (void)hipFuncSetAttribute((const void*)zqr_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(zqr_kernel<NT>, dim3(a.count), dim3(NT), lds, s, a);
+  if (a.flags & 0x1000) {  // reference-form sweep (kept for A/B measurements)
+    (void)hipFuncSetAttribute((const void*)zqr_kernel<NT, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL((zqr_kernel<NT, false>), dim3(a.count), dim3(NT), lds, s, a);
+  } else {
+    (void)hipFuncSetAttribute((const void*)zqr_kernel<NT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL((zqr_kernel<NT, true>), dim3(a.count), dim3(NT), lds, s, a);
+  }
   return hipGetLastError();
 }
 

@@ -47,6 +47,37 @@ class GSError(RuntimeError):
     pass
 
 
+def _share_torch_hip_runtime():
+    """Make the library bind to the HIP runtime that torch ships, when torch is installed.
+
+    torch's wheel carries its own libamdhip64/libhsa-runtime64.  If this library
+    were loaded first it would pull in /opt/rocm's copies, the process would
+    hold two HIP runtimes, and the second one to start finds no GPU (torch
+    then reports "No HIP GPUs are available"); streams and device pointers
+    could not be shared either.  Loading torch's runtime first (globally,
+    without importing torch) makes the library's libamdhip64.so.7 dependency
+    resolve to that same copy whatever the import order.  Without torch the
+    system runtime is used.
+    """
+    if os.environ.get("GSCHUR_SYSTEM_HIP") == "1":
+        return None
+    try:
+        import importlib.util
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return None
+    if spec is None or not spec.submodule_search_locations:
+        return None
+    tlib = os.path.join(list(spec.submodule_search_locations)[0], "lib")
+    loaded = None
+    for name in ("libhsa-runtime64.so", "libamdhip64.so"):
+        p = os.path.join(tlib, name)
+        if os.path.exists(p):
+            ctypes.CDLL(p, mode=ctypes.RTLD_GLOBAL)
+            loaded = p
+    return loaded
+
+
 def lib():
     """Load the library (raises when it has not been built)."""
     global _lib
@@ -57,6 +88,7 @@ def lib():
             return _lib
         if not os.path.exists(LIBPATH):
             raise GSError(f"HIP library not built: {LIBPATH} is missing (run __graft_entry__.build())")
+        _share_torch_hip_runtime()
         L = ctypes.CDLL(LIBPATH)
         L.gs_version.restype = c_int
         L.gs_create.argtypes = [ctypes.POINTER(c_vp), c_int]

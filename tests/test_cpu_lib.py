@@ -137,3 +137,20 @@ def test_workload5_separation():
     for A in mats:
         assert 32 <= A.shape[0] <= 64
         assert gen.min_separation(A) >= 1e-3
+
+
+def test_single_hip_runtime_with_torch(gs):
+    """Loading the library before torch must not put a second HIP runtime in the process."""
+    import importlib.util
+    if importlib.util.find_spec("torch") is None:
+        pytest.skip("torch not installed")
+    from genericschur import _lib
+    _lib.lib()
+    import torch  # noqa: F401
+    hips = set()
+    with open("/proc/self/maps") as f:
+        for line in f:
+            path = line.split()[-1]
+            if "libamdhip64" in path:
+                hips.add(os.path.realpath(path))
+    assert len(hips) == 1, hips
