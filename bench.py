@@ -8,10 +8,16 @@ resident in HBM.  Prints one JSON line.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--count 8192]
 
 Alongside the throughput it reports:
-  roofline      model flops 100 n^3 per matrix (Golub & Van Loan, 4th ed.,
-                7.5.6) over the batch, divided by the batch's kernel time
-                measured with HIP events on the library's stream, against the
-                78.6 TFLOP/s FP64 peak;
+  roofline      for the dominant kernel (the QR-sweep kernel, one launch per
+                size bin): its model flops per launch (100 n^3 per complex
+                matrix for Hessenberg + Q + sweeps, Golub & Van Loan 4th ed.
+                7.5.6, minus the 56/3 n^3 of Hessenberg + Q done by the
+                other kernel) over its average launch duration, measured with
+                HIP events around every launch on its own stream in one
+                instrumented (eager, graph-free) pass after the timed steps,
+                against the 78.6 TFLOP/s FP64 peak; HBM traffic per launch
+                from rocprofv3 FETCH_SIZE/WRITE_SIZE runs when a summary of
+                them is present in profiles/;
   cpu_baseline  the CPU oracle (a C++ restatement of GenericSchur's gschur!)
                 on a bounded sample of the same workload, OpenMP over
                 matrices on the host cores;
@@ -39,6 +45,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def copy_probe(mib):
+    import torch
+    dev = torch.device("cuda:0")
+    nbytes = mib << 20
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes}), flush=True)
+
+
+def load_traffic():
+    """HBM bytes per QR-kernel launch from the committed rocprofv3 summary, if any."""
+    path = os.path.join(ROOT, "profiles", "traffic_zqr_workload2.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,7 +77,12 @@ def main():
     ap.add_argument("--dropin-sample", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (batches shard with no exchange; "
               "run one process per GPU on a slice of the batch)", file=sys.stderr)
@@ -96,6 +128,10 @@ def main():
     t2 = time.perf_counter()
     wall = t2 - t1
     kern_s = ev0.elapsed_time(ev1) * 1e-3 / args.steps
+    # one instrumented pass (not part of the timed steps): events around every
+    # launch on the stream it runs on
+    gs.gschur_batched_device(ns, off, A, T, Z, W, info, complex_input=True, wantZ=True, scale=True, timing=True)
+    recs = _lib.last_launches()
     gs.set_stream(None)
     ms_per_step = 1e3 * wall / args.steps
     value = args.count * args.steps / wall
@@ -104,7 +140,14 @@ def main():
     n64 = ns.astype(np.float64)
     flops = float((100.0 * n64 ** 3).sum())
     bytes_alg = float((3.0 * n64 ** 2 * 16 + 16 * n64).sum())
-    achieved = flops / kern_s
+    batch_rate = flops / kern_s
+    # dominant kernel: the QR-sweep launches
+    qr = [r for r in recs if r["kind"] == "qr"]
+    hq = [r for r in recs if r["kind"] == "hq"]
+    qr_flops = (100.0 - 56.0 / 3.0) * sum(r["sum_n3"] for r in qr)
+    qr_us = sum(r["us"] for r in qr)
+    achieved = qr_flops / (qr_us * 1e-6) if qr_us > 0 else 0.0
+    traffic = load_traffic()
 
     # acceptance residuals on a sample of the batch
     Th = T.cpu().numpy().view(np.complex128)
@@ -149,11 +192,20 @@ def main():
             "peak": PEAK_FP64 / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64,
-            "traffic": None,
-            "note": "FP64 vector-ALU work (bulge chase has no dense contraction); 100 n^3 model flops per matrix "
-                    "over one batch divided by the device time per batch (HIP events around the timed steps)",
-            "kernel_ms_per_batch": kern_s * 1e3,
-            "hbm_frac": bytes_alg / kern_s / PEAK_HBM,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "kernel": "gs::zqr_kernel<64, true> (QR sweeps + Z, one launch per size bin)",
+            "launches": len(qr),
+            "avg_launch_us": qr_us / max(len(qr), 1),
+            "flops_per_launch": qr_flops / max(len(qr), 1),
+            "note": "FP64 vector-ALU work (the bulge chase has no dense contraction; FP64 vector and FP64 MFMA "
+                    "peaks are both 78.6 TFLOP/s). achieved = model flops per launch (81.3 n^3 per complex "
+                    "matrix) / average launch duration (HIP events on the launch's stream, instrumented pass)",
+            "traffic_source": traffic["source"] if traffic else None,
+            "hq_kernel_avg_launch_us": sum(r["us"] for r in hq) / max(len(hq), 1),
+            "batch_ms": kern_s * 1e3,
+            "batch_tflops": batch_rate / 1e12,
+            "batch_frac": batch_rate / PEAK_FP64,
+            "batch_hbm_frac": bytes_alg / kern_s / PEAK_HBM,
         },
         "check": {
             "sample": int(len(samp)),

@@ -31,6 +31,7 @@ struct Bin {
   int first;  // index into the order array
   int count;
   int nmax;
+  int nmin;
   int nt_hq;  // threads per workgroup of the Hessenberg+Z launch
   int nt_qr;  // threads per workgroup of the QR launch
   int branch; // graph branch (stream) the bin runs on
@@ -129,6 +130,9 @@ int gs_destroy(gs_handle* h) {
   if (h->evfork) (void)hipEventDestroy(h->evfork);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  for (hipEvent_t ev : h->tev) (void)hipEventDestroy(ev);
+  h->lwork.release();
+  if (h->lhost) (void)hipHostFree(h->lhost);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
   return GS_OK;
@@ -215,6 +219,7 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
     bin.first = (int)p->order.size();
     bin.count = (int)v.size();
     bin.nmax = std::max(1, n[v.front()]);
+    bin.nmin = n[v.back()];
     bin.nt_hq = default_nt_hq(cplx, bin.nmax);
     bin.nt_qr = default_nt_qr(cplx, bin.nmax);
     bin.work = 0;
@@ -256,8 +261,10 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
   return GS_OK;
 }
 
+// With timed set, events bracket every launch on its stream (eager launches
+// only, never under capture): events 4b .. 4b+3 for bin b.
 static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T, double* Z, double* W,
-                              int32_t* info, int flags) {
+                              int32_t* info, int flags, bool timed = false) {
   BatchArgs a;
   a.order = static_cast<const int32_t*>(p->d_order.p);
   a.n = static_cast<const int32_t*>(p->d_n.p);
@@ -282,14 +289,19 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
   for (const Bin& b : p->bins) used[b.branch] = true;
   for (int i = 0; i < kBranches; ++i)
     if (used[i] && (e = hipStreamWaitEvent(h->branch[i], h->evfork, 0)) != hipSuccess) return e;
-  for (const Bin& b : p->bins) {
+  for (size_t bi = 0; bi < p->bins.size(); ++bi) {
+    const Bin& b = p->bins[bi];
     a.first = b.first;
     a.count = b.count;
     hipStream_t s = h->branch[b.branch];
+    if (timed && (e = hipEventRecord(h->tev[4 * bi], s)) != hipSuccess) return e;
     e = p->is_complex ? launch_zhq(a, b.nmax, b.nt_hq, s) : launch_dhq(a, b.nmax, b.nt_hq, s);
     if (e != hipSuccess) return e;
+    if (timed && (e = hipEventRecord(h->tev[4 * bi + 1], s)) != hipSuccess) return e;
+    if (timed && (e = hipEventRecord(h->tev[4 * bi + 2], s)) != hipSuccess) return e;
     e = p->is_complex ? launch_zqr(a, b.nmax, b.nt_qr, s) : launch_dqr(a, b.nmax, b.nt_qr, s);
     if (e != hipSuccess) return e;
+    if (timed && (e = hipEventRecord(h->tev[4 * bi + 3], s)) != hipSuccess) return e;
   }
   for (int i = 0; i < kBranches; ++i) {
     if (!used[i]) continue;
@@ -297,6 +309,35 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
     if ((e = hipStreamWaitEvent(h->stream, h->evjoin[i], 0)) != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+// One eager execution of the plan with an event pair around every launch,
+// then the per-launch records for gs_last_launches.
+static int run_timed(gs_handle* h, Plan* p, const double* A, double* T, double* Z, double* W, int32_t* info,
+                     int kflags) {
+  const size_t need = 4 * p->bins.size();
+  while (h->tev.size() < need) {
+    hipEvent_t ev = nullptr;
+    HIPCHK(h, hipEventCreate(&ev));
+    h->tev.push_back(ev);
+  }
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  hipError_t le = launch_plan(h, p, A, T, Z, W, info, kflags, true);
+  if (le != hipSuccess) return fail_hip(h, le, "kernel launch (timed)");
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->launch_recs.clear();
+  for (size_t bi = 0; bi < p->bins.size(); ++bi) {
+    const Bin& b = p->bins[bi];
+    for (int kind = 0; kind < 2; ++kind) {
+      float ms = 0.f;
+      HIPCHK(h, hipEventElapsedTime(&ms, h->tev[4 * bi + 2 * kind], h->tev[4 * bi + 2 * kind + 1]));
+      const double rec[GS_LAUNCH_REC] = {(double)kind, (double)b.nmin, (double)b.nmax, (double)b.count, b.work,
+                                         1000.0 * (double)ms};
+      h->launch_recs.insert(h->launch_recs.end(), rec, rec + GS_LAUNCH_REC);
+    }
+  }
+  return GS_OK;
 }
 
 static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n, const int64_t* off,
@@ -311,6 +352,7 @@ static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n,
   if (rc != GS_OK) return rc;
   h->last_plan = p;
   h->last_stats_from_plan = true;
+  h->launch_recs.clear();
   if (count == 0) return GS_OK;
   // GSCHUR_REFERENCE_SWEEP=1 selects the sweep written in the reference's
   // form (LDS round trips, full divisions) for A/B measurements
@@ -327,6 +369,7 @@ static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n,
       dZdummy = static_cast<double*>(h->bZ.p);
     }
     double* Zd = (flags & GS_WANTZ) ? Z : dZdummy;
+    if (flags & GS_TIMING) return run_timed(h, p, A, T, Zd, W, info, kflags);
     const void* ptrs[6] = {A, T, Zd, W, info, nullptr};
     if (!p->gexec || std::memcmp(ptrs, p->g_ptrs, sizeof ptrs) != 0 || p->g_flags != kflags) {
       if (p->gexec) {
@@ -402,6 +445,75 @@ extern "C" int gs_schur_batched_d(gs_handle* h, int64_t count, const int32_t* n,
 }
 
 // ------------------------------------------------------------ single matrix
+// The large real path (dlarge.hip) with its workspace; optional per-phase
+// and per-GEMM event records for gs_last_launches.
+static hipError_t run_large(gs_handle* h, int64_t n, double* dA, int64_t lda, double* dT, int64_t ldt, double* dZ,
+                            int64_t ldz, double* dW, int32_t* dInfo, int kflags, int64_t maxiter, bool timing) {
+  LargeWorkSizes sz;
+  const size_t nd = large_work_doubles(n, &sz);
+  hipError_t e = h->lwork.ensure(8 * nd + 256);
+  if (e != hipSuccess) return e;
+  if (!h->lhost) {
+    e = hipHostMalloc(&h->lhost, 256, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      h->lhost = nullptr;
+      return e;
+    }
+  }
+  double* base = static_cast<double*>(h->lwork.p);
+  LargeWork w;
+  auto carve = [&](int64_t cnt) {
+    double* q = base;
+    base += (cnt + 31) / 32 * 32;
+    return q;
+  };
+  w.Y = carve(sz.Y);
+  w.V = carve(sz.V);
+  w.Ytmp = carve(sz.Ytmp);
+  w.Tall = carve(sz.Tall);
+  w.Wb = carve(sz.Wb);
+  w.W2 = carve(sz.W2);
+  w.tau = carve(sz.tau);
+  w.tv = carve(sz.tv);
+  w.part = carve(sz.part);
+  w.scl = carve(sz.scl);
+  w.lg = carve(sz.lg);
+  w.st = carve(sz.st);
+  w.host_state = h->lhost;
+  LargeArgs a;
+  a.n = n;
+  a.A = dA;
+  a.lda = lda;
+  a.T = dT;
+  a.ldt = ldt;
+  a.Z = dZ;
+  a.ldz = ldz;
+  a.W = dW;
+  a.info = dInfo;
+  a.stats = static_cast<int64_t*>(h->bStats.p);
+  a.flags = kflags;
+  a.maxiter = maxiter;
+  LargeTimer tm;
+  tm.pool = &h->tev;
+  e = large_schur_d(a, w, h->stream, timing ? &tm : nullptr);
+  if (e != hipSuccess || !timing) return e;
+  if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return e;
+  for (const LargeTimer::Rec& r : tm.recs) {
+    float ms = 0.f;
+    if ((e = hipEventElapsedTime(&ms, h->tev[r.e0], h->tev[r.e1])) != hipSuccess) return e;
+    const double rec[GS_LAUNCH_REC] = {(double)r.kind, (double)r.a, (double)r.b, (double)r.c, r.flops,
+                                       1000.0 * (double)ms};
+    h->launch_recs.insert(h->launch_recs.end(), rec, rec + GS_LAUNCH_REC);
+  }
+  for (int ph = 0; ph < 5; ++ph) {
+    float ms = 0.f;
+    if ((e = hipEventElapsedTime(&ms, h->tev[tm.phase_ev[ph]], h->tev[tm.phase_ev[ph + 1]])) != hipSuccess) return e;
+    const double rec[GS_LAUNCH_REC] = {3.0, (double)ph, 0.0, 0.0, 0.0, 1000.0 * (double)ms};
+    h->launch_recs.insert(h->launch_recs.end(), rec, rec + GS_LAUNCH_REC);
+  }
+  return hipSuccess;
+}
+
 static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda, double* T, int64_t ldt, double* Z,
                       int64_t ldz, double* W, int flags, int64_t maxiter, int64_t maxinner, int32_t* info) {
   if (!h || !info) return fail(h, GS_E_ARG, "bad argument");
@@ -460,9 +572,13 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
   }
   const int nlds = cplx ? zschur_max_lds_n() : dschur_max_lds_n();
   const bool use_lds = !(flags & GS_PLAIN) && n <= nlds && dlda == n && dldt == n && dldz == n;
+  const bool use_large = !cplx && !use_lds && !(flags & (GS_PLAIN | GS_HESSENBERG)) && n > nlds;
+  h->launch_recs.clear();
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   hipError_t le;
-  if (use_lds) {
+  if (use_large) {
+    le = run_large(h, n, dA, dlda, dT, dldt, dZ, dldz, dW, dInfo, kflags, maxiter, (flags & GS_TIMING) != 0);
+  } else if (use_lds) {
     HIPCHK(h, h->bMeta.ensure(64));
     // metadata for a batch of one: order = 0, n, off = 0, woff = 0
     struct Meta {
@@ -533,6 +649,31 @@ extern "C" int gs_schur_z(gs_handle* h, int64_t n, double* A, int64_t lda, doubl
 extern "C" int gs_schur_d(gs_handle* h, int64_t n, double* A, int64_t lda, double* T, int64_t ldt, double* Z,
                           int64_t ldz, double* W, int flags, int64_t maxiter, int64_t maxinner, int32_t* info) {
   return run_single(h, false, n, A, lda, T, ldt, Z, ldz, W, flags, maxiter, maxinner, info);
+}
+
+extern "C" int gs_dgemm(gs_handle* h, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
+                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                        int64_t ldc) {
+  if (!h || m < 0 || n < 0 || k < 0 || m > INT32_MAX || n > INT32_MAX || k > INT32_MAX)
+    return fail(h, GS_E_ARG, "bad argument");
+  if (m == 0 || n == 0) return GS_OK;
+  if (!A || !B || !C || ldc < m || lda < (ta ? k : m) || ldb < (tb ? n : k))
+    return fail(h, GS_E_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(h->mu);
+  HIPCHK(h, hipSetDevice(h->device));
+  const hipError_t e = launch_dgemm(ta != 0, tb != 0, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc,
+                                    h->stream);
+  if (e != hipSuccess) return fail_hip(h, e, "gs_dgemm");
+  return GS_OK;
+}
+
+extern "C" int gs_last_launches(const gs_handle* hc, double* out, int max_records) {
+  if (!hc) return 0;
+  const int avail = (int)(hc->launch_recs.size() / GS_LAUNCH_REC);
+  if (out && max_records > 0)
+    std::copy(hc->launch_recs.begin(),
+              hc->launch_recs.begin() + (size_t)std::min(avail, max_records) * GS_LAUNCH_REC, out);
+  return avail;
 }
 
 extern "C" int gs_last_stats(const gs_handle* hc, int64_t* out, int len) {

@@ -9,6 +9,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/gschur_mi355x.h"
 
@@ -54,6 +55,13 @@ struct gs_handle {
   std::map<std::string, std::unique_ptr<gs::Plan>> plans;
   gs::Plan* last_plan = nullptr;
   bool last_stats_from_plan = false;
+  // GS_TIMING: events around each launch (two per launch) and the records
+  // read back by gs_last_launches
+  std::vector<hipEvent_t> tev;
+  std::vector<double> launch_recs;
+  // workspace of the large-matrix real path and its pinned state record
+  gs::DevBuf lwork;
+  void* lhost = nullptr;
   std::mutex mu;
 };
 

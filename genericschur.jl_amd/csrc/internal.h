@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace gs {
 
 // Arguments of the batched Schur kernels.  Matrix m has order n[m], lives at
@@ -103,5 +105,83 @@ struct VjpArgs {
 };
 size_t vjp_lds_bytes(int n);
 hipError_t launch_vjp_assemble(const VjpArgs& a, int nmax, hipStream_t s);
+
+
+// FP64 MFMA GEMM: C(m x n) = beta C + alpha op(A) op(B), column-major.
+hipError_t launch_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, int64_t lda,
+                        const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s);
+
+// ---------------------------------------------------------- large real path
+// One real matrix of any order on the whole GPU: blocked Hessenberg with
+// compact-WY GEMM updates, blocked Q formation, chunked double-shift QR.
+struct LargeArgs {
+  int64_t n;
+  const double* A;  // device, ld = lda (not modified)
+  int64_t lda;
+  double* T;  // working H, then T
+  int64_t ldt;
+  double* Z;  // n x n (required: Q accumulation target even without wantZ)
+  int64_t ldz;
+  double* W;  // n complex eigenvalues
+  int32_t* info;   // device
+  int64_t* stats;  // device, 2 values, or null
+  int flags;
+  int64_t maxiter;
+};
+
+struct LargeWorkSizes {
+  int64_t Y, V, Ytmp, Tall, Wb, W2, tau, tv, part, scl, lg, st;
+};
+
+// device workspace carved from one allocation, plus a pinned host record
+struct LargeWork {
+  double *Y, *V, *Ytmp, *Tall, *Wb, *W2, *tau, *tv, *part, *scl, *lg;
+  void* st;
+  void* host_state;
+};
+
+// events around phases and GEMM launches (only when timing is requested)
+struct LargeTimer {
+  std::vector<hipEvent_t>* pool = nullptr;
+  size_t used = 0;
+  struct Rec {
+    int kind;  // 2: GEMM (a, b, c = m, n, k), 3: phase (a = phase index)
+    int a, b, c;
+    double flops;
+    size_t e0, e1;
+  };
+  std::vector<Rec> recs;
+  size_t phase_ev[6] = {0, 0, 0, 0, 0, 0};
+  hipError_t ev(hipStream_t s, size_t& idx) {
+    if (used == pool->size()) {
+      hipEvent_t e = nullptr;
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      pool->push_back(e);
+    }
+    idx = used++;
+    return hipEventRecord((*pool)[idx], s);
+  }
+  hipError_t mark(hipStream_t s, int ph) { return ev(s, phase_ev[ph]); }
+};
+
+inline hipError_t tm_gemm(LargeTimer* tm, hipStream_t s, bool ta, bool tb, int m, int n, int k, double alpha,
+                          const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                          int64_t ldc) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  size_t e0 = 0, e1 = 0;
+  hipError_t r;
+  if (tm && (r = tm->ev(s, e0)) != hipSuccess) return r;
+  if ((r = launch_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s)) != hipSuccess) return r;
+  if (tm) {
+    if ((r = tm->ev(s, e1)) != hipSuccess) return r;
+    tm->recs.push_back({2, m, n, k, 2.0 * m * n * (double)k, e0, e1});
+  }
+  return hipSuccess;
+}
+
+size_t large_work_doubles(int64_t n, LargeWorkSizes* sz);
+size_t large_state_bytes();
+hipError_t large_schur_d(const LargeArgs& a, LargeWork& w, hipStream_t s, LargeTimer* tm);
 
 }  // namespace gs

@@ -17,6 +17,8 @@ GS_CHECK_SUBDIAG = 0x08
 GS_DEVICE_PTRS = 0x10
 GS_ASYNC = 0x20
 GS_PLAIN = 0x40
+GS_TIMING = 0x80
+GS_LAUNCH_REC = 6
 
 GS_OK = 0
 GS_E_DIM = -1
@@ -29,7 +31,7 @@ EXPORTS = (
     "gs_version", "gs_create", "gs_destroy", "gs_last_error", "gs_synchronize", "gs_set_stream",
     "gs_schur_z", "gs_schur_d", "gs_schur_batched_z", "gs_schur_batched_d", "gs_batched_max_n",
     "gs_trevc_z", "gs_trevc_d", "gs_trevc_batched_z", "gs_trevc_batched_d",
-    "gs_eig_vjp_batched_d", "gs_last_stats",
+    "gs_eig_vjp_batched_d", "gs_last_stats", "gs_last_launches", "gs_dgemm",
 )
 
 _lib = None
@@ -112,6 +114,9 @@ def lib():
         L.gs_trevc_batched_d.argtypes = sig_tvb
         L.gs_eig_vjp_batched_d.argtypes = [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int]
         L.gs_last_stats.argtypes = [c_vp, c_i64p, c_int]
+        L.gs_last_launches.argtypes = [c_vp, c_vp, c_int]
+        L.gs_dgemm.argtypes = [c_vp, c_int, c_int, c_i64, c_i64, c_i64, ctypes.c_double, c_vp, c_i64, c_vp, c_i64,
+                               ctypes.c_double, c_vp, c_i64]
         _lib = L
         return _lib
 
@@ -145,3 +150,29 @@ def last_stats(device=0):
     out = (ctypes.c_int64 * 3)()
     k = lib().gs_last_stats(handle(device), out, 3)
     return tuple(int(out[i]) for i in range(k))
+
+
+def last_launches(device=0):
+    """Records of the last call made with GS_TIMING: a list of dicts.  Batched
+    calls give kind 'hq' / 'qr' (nmin, nmax, count, sum_n3, us); the large
+    single-matrix path gives 'gemm' (m, n, k, flops, us) and 'phase' (phase:
+    0 copy+scale, 1 Hessenberg, 2 Q formation, 3 QR, 4 finish; us)."""
+    h = handle(device)
+    k = lib().gs_last_launches(h, None, 0)
+    if k <= 0:
+        return []
+    buf = (ctypes.c_double * (k * GS_LAUNCH_REC))()
+    lib().gs_last_launches(h, buf, k)
+    recs = []
+    for r in range(k):
+        v = buf[r * GS_LAUNCH_REC:(r + 1) * GS_LAUNCH_REC]
+        kind = int(v[0])
+        if kind in (0, 1):
+            recs.append({"kind": "qr" if kind == 1 else "hq", "nmin": int(v[1]), "nmax": int(v[2]),
+                         "count": int(v[3]), "sum_n3": v[4], "us": v[5]})
+        elif kind == 2:
+            recs.append({"kind": "gemm", "m": int(v[1]), "n": int(v[2]), "k": int(v[3]), "flops": v[4],
+                         "us": v[5]})
+        else:
+            recs.append({"kind": "phase", "phase": int(v[1]), "us": v[5]})
+    return recs

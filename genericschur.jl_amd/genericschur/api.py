@@ -105,7 +105,8 @@ def _raise_info(info, maxiter, n):
     raise GSError(f"info = {info}")
 
 
-def _single(A, wantZ, scale, maxiter, maxinner, hess=False, Z0=None, checksd=False, plain=False, device=0):
+def _single(A, wantZ, scale, maxiter, maxinner, hess=False, Z0=None, checksd=False, plain=False, device=0,
+            timing=False):
     cplx = _kind(A)
     n = A.shape[0]
     dt = A.dtype
@@ -121,6 +122,8 @@ def _single(A, wantZ, scale, maxiter, maxinner, hess=False, Z0=None, checksd=Fal
             flags |= GS_CHECK_SUBDIAG
     if plain:
         flags |= GS_PLAIN
+    if timing:
+        flags |= _lib.GS_TIMING
     h = _lib.handle(device)
     f = _lib.lib().gs_schur_z if cplx else _lib.lib().gs_schur_d
     ld = max(n, 1)
@@ -135,13 +138,53 @@ def _single(A, wantZ, scale, maxiter, maxinner, hess=False, Z0=None, checksd=Fal
     return Schur(T, Zo, W)
 
 
-def gschur_(A, wantZ=True, scale=True, maxiter=None, maxinner=None, plain=False, device=0):
-    """gschur!(A) -> Schur (GenericSchur.jl:350-372 / :805-835)."""
+def gschur_(A, wantZ=True, scale=True, maxiter=None, maxinner=None, plain=False, device=0, timing=False):
+    """gschur!(A) -> Schur (GenericSchur.jl:350-372 / :805-835).
+
+    Real matrices above the LDS-resident orders take the blocked large-matrix
+    path unless plain=True (the unblocked single-workgroup kernel).  With
+    timing=True the large path records per-phase and per-GEMM durations
+    (genericschur._lib.last_launches())."""
     if _is_torch(A):
-        raise TypeError("use gschur_batched_device for device tensors")
+        raise TypeError("use gschur_device / gschur_batched_device for device tensors")
     A = np.asarray(A)
     _checksquare(A)
-    return _single(A, wantZ, scale, maxiter, maxinner, plain=plain, device=device)
+    return _single(A, wantZ, scale, maxiter, maxinner, plain=plain, device=device, timing=timing)
+
+
+def gschur_device(n, A, T, Z, W, complex_input, wantZ=True, scale=True, plain=False, timing=False, maxiter=None,
+                  device=0):
+    """gschur! on one device-resident column-major matrix (torch CUDA tensors
+    or raw device addresses, ld = n): T, Z (n x n) and W (n complex) are
+    written on the device.  Returns info (0, or > 0 when unconverged)."""
+    def addr(x):
+        return ctypes.c_void_p(x.data_ptr() if _is_torch(x) else int(x))
+
+    flags = (GS_WANTZ if wantZ else 0) | (GS_SCALE if scale else 0) | GS_DEVICE_PTRS
+    if plain:
+        flags |= GS_PLAIN
+    if timing:
+        flags |= _lib.GS_TIMING
+    info = ctypes.c_int32(0)
+    h = _lib.handle(device)
+    f = _lib.lib().gs_schur_z if complex_input else _lib.lib().gs_schur_d
+    ld = max(int(n), 1)
+    rc = f(h, int(n), addr(A), ld, addr(T), ld, addr(Z) if Z is not None else None, ld, addr(W), flags,
+           0 if maxiter is None else int(maxiter), 0, ctypes.byref(info))
+    _lib.check(rc, h, "gs_schur (device)")
+    return info.value
+
+
+def dgemm_device(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, device=0):
+    """C = beta C + alpha op(A) op(B) with the library's FP64 MFMA GEMM
+    (device pointers, column-major)."""
+    def addr(x):
+        return ctypes.c_void_p(x.data_ptr() if _is_torch(x) else int(x))
+
+    h = _lib.handle(device)
+    rc = _lib.lib().gs_dgemm(h, int(bool(ta)), int(bool(tb)), m, n, k, float(alpha), addr(A), lda, addr(B), ldb,
+                             float(beta), addr(C), ldc)
+    _lib.check(rc, h, "gs_dgemm")
 
 
 def gschur(A, **kw):
@@ -270,9 +313,11 @@ def _is_torch(x):
 
 
 def gschur_batched_device(ns, off, A, T, Z, W, info, complex_input, wantZ=True, scale=True, asynchronous=False,
-                          device=0):
+                          device=0, timing=False):
     """Batched gschur! on device-resident packed data (torch CUDA tensors or
-    raw device addresses).  ns/off are host int32/int64 arrays."""
+    raw device addresses).  ns/off are host int32/int64 arrays.  With timing,
+    the batch runs eagerly with events around each launch (read them with
+    genericschur._lib.last_launches())."""
     ns = np.ascontiguousarray(ns, dtype=np.int32)
     off = np.ascontiguousarray(off, dtype=np.int64)
 
@@ -284,6 +329,8 @@ def gschur_batched_device(ns, off, A, T, Z, W, info, complex_input, wantZ=True, 
     flags = (GS_WANTZ if wantZ else 0) | (GS_SCALE if scale else 0) | GS_DEVICE_PTRS
     if asynchronous:
         flags |= _lib.GS_ASYNC
+    if timing:
+        flags |= _lib.GS_TIMING
     h = _lib.handle(device)
     f = _lib.lib().gs_schur_batched_z if complex_input else _lib.lib().gs_schur_batched_d
     rc = f(h, len(ns), _ptr(ns), _ptr(off), addr(A), addr(T), addr(Z), addr(W), addr(info), flags)

@@ -49,6 +49,11 @@ typedef struct gs_handle gs_handle;
 #define GS_ASYNC 0x20        /* do not synchronise before returning            */
 #define GS_PLAIN 0x40        /* single matrix: force the unblocked global-     */
                              /* memory kernel (the plain GPU form)             */
+#define GS_TIMING 0x80       /* record launch durations for gs_last_launches:  */
+                             /* batched with GS_DEVICE_PTRS (launched eagerly, */
+                             /* no graph, events around every kernel on its    */
+                             /* stream) and the large real single-matrix path  */
+                             /* (events around its phases and GEMMs)           */
 
 /* status codes */
 #define GS_OK 0
@@ -91,7 +96,14 @@ int gs_schur_z(gs_handle* h, int64_t n, double* A, int64_t lda, double* T, int64
  * sweep :837-952 and the 2x2 standardisation _gs2x2! :716-803.  T is the
  * standard-form quasi-triangular factor; W (2n doubles) holds the
  * eigenvalues by position with the +imag member of a pair first.
- * maxinner is ignored (the reference's real driver has none). */
+ * maxinner is ignored (the reference's real driver has none).
+ * Orders above gs_batched_max_n(0) take the large-matrix path unless
+ * GS_PLAIN or GS_HESSENBERG is set: blocked Hessenberg reduction and Q
+ * formation with FP64 MFMA GEMM updates, then the same double-shift QR
+ * iteration with each sweep chased in LDS-resident chunks whose reflectors
+ * are applied to the rest of H and to Z by the whole GPU (same per-element
+ * update order as the unblocked sweep).  GS_PLAIN runs the unblocked
+ * single-workgroup kernel instead. */
 int gs_schur_d(gs_handle* h, int64_t n, double* A, int64_t lda, double* T, int64_t ldt, double* Z, int64_t ldz,
                double* W, int flags, int64_t maxiter, int64_t maxinner, int32_t* info);
 
@@ -142,6 +154,26 @@ int gs_eig_vjp_batched_d(gs_handle* h, int64_t count, const int32_t* n, const in
  * out[1] = bulge steps, out[2] = kernel time in microseconds (events on the
  * handle's stream, 0 when unavailable).  Returns the number written. */
 int gs_last_stats(const gs_handle* h, int64_t* out, int len);
+
+/* Per-launch records of the last Schur call made with GS_TIMING
+ * (measurement support; no counterpart in the reference).  Each record is
+ * GS_LAUNCH_REC doubles {kind, a, b, c, d, microseconds} (HIP events on the
+ * launch's stream):
+ *   kind 0 / 1  batched Hessenberg + Q kernel / QR kernel of one size bin:
+ *               a, b = smallest and largest order, c = matrices, d = sum n^3
+ *   kind 2      GEMM of the large real path: a, b, c = m, n, k, d = 2 m n k
+ *   kind 3      phase of the large real path: a = 0 copy + scale,
+ *               1 Hessenberg, 2 Q formation, 3 QR iterations, 4 finish
+ * Writes at most max_records records and returns the number available. */
+#define GS_LAUNCH_REC 6
+int gs_last_launches(const gs_handle* h, double* out, int max_records);
+
+/* C(m x n) = beta C + alpha op(A) op(B) on the handle's stream with the
+ * FP64 MFMA GEMM kernel of the large real path (column-major device
+ * pointers; op = transpose when ta / tb is nonzero).  Support routine with
+ * no counterpart in the reference: it exposes the GEMM for measurement. */
+int gs_dgemm(gs_handle* h, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+             int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc);
 
 #ifdef __cplusplus
 }
