@@ -79,10 +79,25 @@ def main():
     ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--workload", type=int, default=2, choices=[2, 3, 4, 5],
+                    help="2: batched complex (the headline line); 3: batched real mix; 4: one large real matrix "
+                         "(blocked path vs plain kernel); 5: eigenvalue-loss forward + backward")
+    ap.add_argument("--large-n", type=int, default=4096)
+    ap.add_argument("--plain-n", type=int, default=1024,
+                    help="workload 4: order at which the plain kernel is timed (scaled by n^3 to --large-n)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == 3:
+        return bench_workload3(args)
+    if args.workload == 4:
+        return bench_workload4(args)
+    if args.workload == 5:
+        return bench_workload5(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (batches shard with no exchange; "
               "run one process per GPU on a slice of the batch)", file=sys.stderr)
@@ -242,6 +257,238 @@ def main():
         out["cpu_baseline"] = {"value": m / dt, "unit": "matrices/s", "cores": cores, "kind": "port",
                                "sample": f"first {m} matrices of the same workload; C++ restatement of gschur! "
                                          f"(oracle/), OpenMP over matrices, {cores} threads, {dt:.1f}s"}
+    print(json.dumps(out), flush=True)
+
+
+def _common(args, metric_value, unit, ms, workload, extra_cfg, dtype="f64"):
+    cfg = {"workload": workload}
+    cfg.update(extra_cfg)
+    return {"metric": "Schur decomps/s (batched n<=128) + n=4096 time vs plain GPU, bwd err <=50*n*eps",
+            "value": metric_value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic", "config": cfg}
+
+
+def _timed(stream, step, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t1 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    return wall, ev0.elapsed_time(ev1) * 1e-3 / steps
+
+
+def _residuals(mats, ns, off, Th, Zh, sample=128):
+    eps = np.finfo(np.float64).eps
+    idx = np.linspace(0, len(ns) - 1, min(sample, len(ns))).astype(int)
+    wb = wo = 0.0
+    for i in idx:
+        k, o = int(ns[i]), int(off[i])
+        Ai = mats[i]
+        Ti = Th[o:o + k * k].reshape((k, k), order="F")
+        Zi = Zh[o:o + k * k].reshape((k, k), order="F")
+        wb = max(wb, np.linalg.norm(Ai - Zi @ Ti @ Zi.conj().T) / np.linalg.norm(Ai) / (k * eps))
+        wo = max(wo, np.linalg.norm(Zi.conj().T @ Zi - np.eye(k)) / (k * eps))
+    return {"sample": int(len(idx)), "max_backward_over_n_eps": wb, "max_orth_over_n_eps": wo,
+            "bound_over_n_eps": 50.0}
+
+
+def bench_workload3(args):
+    """Batched real Schur of 4096 matrices n ~ U{16..128}: half real Ginibre,
+    a quarter Grcar-type Toeplitz, a quarter Q (D + N) Q^T."""
+    import genericschur as gs
+    from genericschur import gen, _lib
+    import torch
+    count = 4096 if args.count == 8192 else args.count
+    mats, _ = gen.workload3(seed=args.seed, count=count)
+    ns, off, flat = gs.pack(mats, np.float64)
+    dev = torch.device("cuda:0")
+    A = torch.from_numpy(flat).to(dev)
+    T = torch.zeros_like(A)
+    Z = torch.zeros_like(A)
+    W = torch.zeros(2 * int(ns.sum()), dtype=torch.float64, device=dev)
+    info = torch.zeros(len(ns), dtype=torch.int32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    gs.set_stream(stream.cuda_stream)
+    step = lambda: gs.gschur_batched_device(ns, off, A, T, Z, W, info, complex_input=False, asynchronous=True)
+    wall, kern_s = _timed(stream, step, args.steps, args.warmup)
+    gs.set_stream(None)
+    n64 = ns.astype(np.float64)
+    out = _common(args, count * args.steps / wall, "matrices/s", 1e3 * wall / args.steps,
+                  f"batched real Schur (T, Z, values): {count} Float64 matrices n ~ U{{16..128}}, 1/2 Ginibre, "
+                  "1/4 Toeplitz (-1 sub, +1 diag and 3 super), 1/4 Q(D+N)Q^T; seeded Philox",
+                  {"count": count, "n_range": [16, 128], "seed": args.seed})
+    flops = float((25.0 * n64 ** 3).sum())
+    out["roofline"] = {"bound": "mfma", "achieved": flops / kern_s / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                       "frac": flops / kern_s / PEAK_FP64, "traffic": None,
+                       "note": "25 n^3 model flops per matrix over the batch's device time"}
+    out["check"] = _residuals(mats, ns, off, T.cpu().numpy(), Z.cpu().numpy())
+    out["check"]["info_nonzero"] = int((info.cpu().numpy() != 0).sum())
+    if not args.no_cpu:
+        out["cpu_baseline"] = _cpu_batched(mats, False, min(args.cpu_sample, count))
+    print(json.dumps(out), flush=True)
+
+
+def _cpu_batched(mats, cplx, m):
+    """The oracle (C++ restatement of gschur!) over the first m matrices,
+    OpenMP over matrices on up to 16 host cores."""
+    import genericschur as gs
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    sn, so, sf = gs.pack(mats[:m], np.complex128 if cplx else np.float64)
+    cores = min(16, os.cpu_count() or 1)
+    tc = time.perf_counter()
+    pyoracle.schur_batched(sn, so, sf, cplx, nthreads=cores)
+    dt = time.perf_counter() - tc
+    return {"value": m / dt, "unit": "matrices/s", "cores": cores, "kind": "port",
+            "sample": f"first {m} matrices of the same workload; C++ restatement of gschur! (oracle/), OpenMP over "
+                      f"matrices, {cores} threads, {dt:.1f}s"}
+
+
+def bench_workload4(args):
+    """One large real matrix A = Q (Lambda + U) Q^T: the blocked large-matrix
+    path against the plain unblocked kernel."""
+    import genericschur as gs
+    from genericschur import gen, _lib
+    import torch
+    dev = torch.device("cuda:0")
+
+    def solve(n, plain, timing):
+        A = gen.workload4(seed=args.seed, n=n)
+        Ad = torch.from_numpy(np.asfortranarray(A).ravel(order="F")).to(dev)
+        T = torch.zeros_like(Ad)
+        Z = torch.zeros_like(Ad)
+        W = torch.zeros(2 * n, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = gs.gschur_device(n, Ad, T, Z, W, False, plain=plain, timing=timing)
+        gs.synchronize()
+        dt = time.perf_counter() - t0
+        Am, Tm, Zm = (x.view(n, n).t() for x in (Ad, T, Z))
+        eps = np.finfo(np.float64).eps
+        bwd = (torch.linalg.norm(Am - Zm @ Tm @ Zm.t()) / torch.linalg.norm(Am)).item() / (n * eps)
+        orth = torch.linalg.norm(Zm.t() @ Zm - torch.eye(n, dtype=torch.float64, device=dev)).item() / (n * eps)
+        return dt, info, bwd, orth
+
+    n = args.large_n
+    log(f"workload 4: large path n={n}")
+    dt, info, bwd, orth = solve(n, False, True)
+    recs = _lib.last_launches()
+    st = _lib.last_stats()
+    ph = {int(r["phase"]): r["us"] * 1e-6 for r in recs if r["kind"] == "phase"}
+    g = [r for r in recs if r["kind"] == "gemm"]
+    gf = sum(r["flops"] for r in g)
+    gt = sum(r["us"] for r in g) * 1e-6
+    log(f"workload 4: plain kernel n={args.plain_n}")
+    pdt, pinfo, pbwd, porth = solve(args.plain_n, True, False)
+    plain_est = pdt * (n / args.plain_n) ** 3
+    out = _common(args, 1.0 / dt, "solves/s", dt * 1e3,
+                  f"one {n}x{n} Float64 matrix Q(Lambda+U)Q^T (Haar Q, log-uniform real and complex-pair "
+                  "eigenvalues in [1e-3, 1e3], U strictly upper N(0,1)), T, Z and values",
+                  {"n": n, "seed": args.seed})
+    out["steps"] = 1
+    out["warmup"] = 0
+    out["large"] = {"seconds": dt, "info": info, "sweeps": st[0] if st else None,
+                    "bulge_steps": st[1] if len(st) > 1 else None,
+                    "phase_seconds": {"copy_scale": ph.get(0), "hessenberg": ph.get(1), "q_formation": ph.get(2),
+                                      "qr": ph.get(3), "finish": ph.get(4)},
+                    "gemm_phase": {"tflops": gf / gt / 1e12 if gt else None,
+                                   "frac_fp64_peak": gf / gt / PEAK_FP64 if gt else None,
+                                   "seconds": gt, "gemms": len(g), "flops": gf},
+                    "backward_over_n_eps": bwd, "orth_over_n_eps": orth}
+    out["plain"] = {"n": args.plain_n, "seconds": pdt, "info": pinfo, "backward_over_n_eps": pbwd,
+                    "orth_over_n_eps": porth,
+                    "seconds_scaled_to_n": plain_est,
+                    "note": f"plain unblocked kernel timed at n={args.plain_n}, scaled by (n/{args.plain_n})^3"}
+    out["large_over_plain_speedup"] = plain_est / dt
+    if not args.no_cpu:
+        # the oracle single-threaded (the reference's gschur! is scalar code)
+        # at n = 1024, scaled by n^3
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        nc = min(1024, n)
+        Ac = gen.workload4(seed=args.seed, n=nc)
+        tc = time.perf_counter()
+        pyoracle.gschur(Ac)
+        cdt = time.perf_counter() - tc
+        est = cdt * (n / nc) ** 3
+        out["cpu_baseline"] = {"value": 1.0 / est, "unit": "solves/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle (C++ restatement of gschur!) on one {nc}x{nc} matrix of the same "
+                                         f"construction, 1 thread, {cdt:.1f}s, scaled by ({n}/{nc})^3 = "
+                                         f"{est:.0f}s per solve"}
+    flops = 25.0 * n ** 3
+    out["roofline"] = {"bound": "mfma", "achieved": gf / gt / 1e12 if gt else None, "peak": 78.6,
+                       "unit": "TFLOP/s", "frac": gf / gt / PEAK_FP64 if gt else None, "traffic": None,
+                       "kernel": "gs::dgemm_kernel (GEMM phases: blocked Hessenberg and Q formation)",
+                       "whole_solve_tflops": flops / dt / 1e12}
+    print(json.dumps(out), flush=True)
+
+
+def bench_workload5(args):
+    """Eigenvalue loss L = sum w Re(lambda) + v Im(lambda)^2 over 2048 real
+    Ginibre matrices n ~ U{32..64}: batched Schur forward + VJP backward."""
+    import ctypes
+    import genericschur as gs
+    from genericschur import gen, _lib
+    import torch
+    count = 2048 if args.count == 8192 else args.count
+    mats, ws, vs = gen.workload5(seed=args.seed, count=count)
+    ns, off, flat = gs.pack(mats, np.float64)
+    dev = torch.device("cuda:0")
+    A = torch.from_numpy(flat).to(dev)
+    T = torch.zeros_like(A)
+    Z = torch.zeros_like(A)
+    Abar = torch.zeros_like(A)
+    nw = int(ns.sum())
+    W = torch.zeros(2 * nw, dtype=torch.float64, device=dev)
+    info = torch.zeros(len(ns), dtype=torch.int32, device=dev)
+    vinfo = torch.zeros(len(ns), dtype=torch.int32, device=dev)
+    wt = torch.from_numpy(np.concatenate(ws)).to(dev)
+    vt = torch.from_numpy(np.concatenate(vs)).to(dev)
+    g = torch.zeros(2 * nw, dtype=torch.float64, device=dev)
+    ns_c = np.ascontiguousarray(ns, dtype=np.int32)
+    off_c = np.ascontiguousarray(off, dtype=np.int64)
+    stream = torch.cuda.Stream(device=dev)
+    gs.set_stream(stream.cuda_stream)
+    h = _lib.handle()
+    a = lambda t: ctypes.c_void_p(t.data_ptr())
+
+    def step():
+        gs.gschur_batched_device(ns, off, A, T, Z, W, info, complex_input=False, asynchronous=True)
+        with torch.cuda.stream(stream):
+            # gbar = dL/dRe + i dL/dIm = w + 2 i v Im(lambda)
+            g[0::2] = wt
+            g[1::2] = 2.0 * vt * W[1::2]
+        rc = _lib.lib().gs_eig_vjp_batched_d(h, len(ns), ctypes.c_void_p(ns_c.ctypes.data),
+                                             ctypes.c_void_p(off_c.ctypes.data), a(T), a(Z), a(g), a(Abar), a(vinfo),
+                                             _lib.GS_DEVICE_PTRS | _lib.GS_ASYNC)
+        _lib.check(rc, h, "gs_eig_vjp_batched_d")
+
+    wall, kern_s = _timed(stream, step, args.steps, args.warmup)
+    gs.set_stream(None)
+    n64 = ns.astype(np.float64)
+    out = _common(args, count * args.steps / wall, "matrices/s (forward + backward)", 1e3 * wall / args.steps,
+                  f"eigenvalue loss sum w Re(lambda) + v Im(lambda)^2 over {count} Float64 Ginibre matrices "
+                  "n ~ U{32..64} (min eigenvalue separation >= 1e-3): batched Schur + VJP kernel",
+                  {"count": count, "n_range": [32, 64], "seed": args.seed})
+    flops = float(((25.0 + 2 * 8.0 / 3.0 + 8.0) * n64 ** 3).sum())
+    out["roofline"] = {"bound": "mfma", "achieved": flops / kern_s / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                       "frac": flops / kern_s / PEAK_FP64, "traffic": None,
+                       "note": "(25 + 16/3 + 8) n^3 model flops per matrix (Schur, two eigenvector sides, "
+                               "assembly) over the device time"}
+    out["check"] = {"info_nonzero": int((info.cpu().numpy() != 0).sum()),
+                    "vjp_info_nonzero": int((vinfo.cpu().numpy() != 0).sum())}
+    if not args.no_cpu:
+        out["cpu_baseline"] = _cpu_batched(mats, False, count)
+        out["cpu_baseline"]["note"] = "forward (Schur) only: the reference has no eigenvalue rrule"
     print(json.dumps(out), flush=True)
 
 

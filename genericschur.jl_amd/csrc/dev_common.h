@@ -148,6 +148,42 @@ template <int NT> GS_DEV void bsync() {
   else __builtin_amdgcn_wave_barrier();
 }
 
+// ------------------------------------------- split dot products (4 lanes)
+// out[i] = sum_{k0 <= k < k1} f(k, i) for i0 <= i < i1: four consecutive
+// lanes share an output, each summing a quarter of the k range, and the
+// partial sums are combined with lane shuffles.  Every thread must call
+// (the loop is uniform); out is written by one lane per output.
+template <int NT, class F> GS_DEV void zdots4(int i0, int i1, int k0, int k1, F f, zd* out) {
+  const int s = threadIdx.x & 3, il = threadIdx.x >> 2;
+  const int seg = (k1 - k0 + 3) >> 2;
+  const int ks = k0 + s * seg, ke = min(ks + seg, k1);
+  for (int ib = i0; ib < i1; ib += NT / 4) {
+    const int i = ib + il;
+    zd acc = zr(0.0);
+    if (i < i1)
+      for (int k = ks; k < ke; ++k) acc = acc + f(k, i);
+    acc.x += __shfl_xor(acc.x, 1, 64);
+    acc.y += __shfl_xor(acc.y, 1, 64);
+    acc.x += __shfl_xor(acc.x, 2, 64);
+    acc.y += __shfl_xor(acc.y, 2, 64);
+    if (s == 0 && i < i1) out[i] = acc;
+  }
+}
+template <int NT, class F> GS_DEV void ddots4(int i0, int i1, int k0, int k1, F f, double* out) {
+  const int s = threadIdx.x & 3, il = threadIdx.x >> 2;
+  const int seg = (k1 - k0 + 3) >> 2;
+  const int ks = k0 + s * seg, ke = min(ks + seg, k1);
+  for (int ib = i0; ib < i1; ib += NT / 4) {
+    const int i = ib + il;
+    double acc = 0.0;
+    if (i < i1)
+      for (int k = ks; k < ke; ++k) acc += f(k, i);
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    if (s == 0 && i < i1) out[i] = acc;
+  }
+}
+
 // util.jl:41-80 (xLASCL-style multiply by cto/cfrom), the multipliers in the
 // order the reference applies them.  Returns how many were written (<= 4).
 GS_DEV int safescale_muls(double cfrom, double cto, double* mul) {

@@ -13,6 +13,7 @@
 // third subdiagonal); the single-matrix kernel keeps H in HBM.  Z lives in
 // HBM with each sweep's 3-element reflectors applied afterwards as one
 // chunked chain per row of Z.
+#include "chains.h"
 #include "dev_common.h"
 #include "gs2x2.h"
 #include "internal.h"
@@ -26,8 +27,9 @@ namespace gs {
 
 // ------------------------------------------------------------------------
 // hessenberg.jl:3-17 for real input
-template <int NT> GS_DEV void d_hessenberg(double* H, int64_t ld, int n, double* tau, double* red) {
+template <int NT> GS_DEV void d_hessenberg(double* H, int64_t ld, int n, double* tau, double* red, double* wk) {
   const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const double sfmin = 2.0 * GS_FMIN / GS_EPS;
   for (int p = 0; p <= n - 2; ++p) {
     const int r0 = p + 2;
@@ -89,38 +91,56 @@ template <int NT> GS_DEV void d_hessenberg(double* H, int64_t ld, int n, double*
     }
     if (tid == 0) tau[p] = t_;
     __syncthreads();
-    // lmul!(H', A[p+1:n, p+1:n])
-    for (int j = p + 1 + tid; j < n; j += NT) {
-      double va = H[(p + 1) + (int64_t)j * ld];
-      for (int r = r0; r < n; ++r) va += H[r + p * ld] * H[r + (int64_t)j * ld];
-      va = t_ * va;
-      H[(p + 1) + (int64_t)j * ld] -= va;
-      for (int r = r0; r < n; ++r) H[r + (int64_t)j * ld] -= va * H[r + p * ld];
+    // lmul!(H', A[p+1:n, p+1:n]): va_j = tau (H[p+1, j] + sum_r v_r H[r, j]);
+    // column j -= va_j [1; v]
+    ddots4<NT>(p + 1, n, r0, n, [&](int r, int j) { return H[r + p * ld] * H[r + (int64_t)j * ld]; }, wk);
+    __syncthreads();
+    for (int j = p + 1 + tid; j < n; j += NT) wk[j] = t_ * (H[(p + 1) + (int64_t)j * ld] + wk[j]);
+    __syncthreads();
+    for (int j = p + 1 + wave; j < n; j += NT / 64) {
+      const double va = wk[j];
+      for (int r = p + 1 + lane; r < n; r += 64) {
+        const double h = H[r + (int64_t)j * ld];
+        H[r + (int64_t)j * ld] = (r == p + 1) ? h - va : h - va * H[r + p * ld];
+      }
     }
     __syncthreads();
-    // rmul!(A[:, p+1:n], H, x) with BLAS ger for the rank-1 part
+    // rmul!(A[:, p+1:n], H, x) with BLAS ger for the rank-1 part:
+    // x_r = H[r, p+1] + sum_c H[r, c] v_c; H[r, p+1] -= tau x_r;
+    // H[r, c] += (-tau x_r) v_c
+    ddots4<NT>(0, n, r0, n, [&](int c, int r) { return H[r + (int64_t)c * ld] * H[c + p * ld]; }, wk);
+    __syncthreads();
     for (int r = tid; r < n; r += NT) {
-      double x = H[r + (int64_t)(p + 1) * ld];
-      for (int c = r0; c < n; ++c) x += H[r + (int64_t)c * ld] * H[c + p * ld];
+      const double x = H[r + (int64_t)(p + 1) * ld] + wk[r];
       H[r + (int64_t)(p + 1) * ld] -= t_ * x;
-      const double xm = -t_ * x;
-      for (int c = r0; c < n; ++c) H[r + (int64_t)c * ld] += xm * H[c + p * ld];
+      wk[r] = -t_ * x;
+    }
+    __syncthreads();
+    for (int c = r0 + wave; c < n; c += NT / 64) {
+      const double vc = H[c + p * ld];
+      for (int r = lane; r < n; r += 64) H[r + (int64_t)c * ld] += wk[r] * vc;
     }
     __syncthreads();
   }
 }
 
-template <int NT> GS_DEV void d_form_q_inplace(double* H, int64_t ld, int n, const double* tau) {
+template <int NT> GS_DEV void d_form_q_inplace(double* H, int64_t ld, int n, const double* tau, double* wk) {
   const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   for (int k = n - 2; k >= 0; --k) {
     const int c = k + 1;
     const double t = tau[k];
-    for (int j = c + 1 + tid; j < n; j += NT) {
-      double s = H[c + (int64_t)j * ld];
-      for (int r = c + 1; r < n; ++r) s += H[r + (int64_t)k * ld] * H[r + (int64_t)j * ld];
-      s = t * s;
-      H[c + (int64_t)j * ld] -= s;
-      for (int r = c + 1; r < n; ++r) H[r + (int64_t)j * ld] -= s * H[r + (int64_t)k * ld];
+    // s_j = t (Q[c, j] + sum_{r>c} v_r Q[r, j]); column j -= s_j [1; v]
+    ddots4<NT>(c + 1, n, c + 1, n, [&](int r, int j) { return H[r + (int64_t)k * ld] * H[r + (int64_t)j * ld]; }, wk);
+    __syncthreads();
+    for (int j = c + 1 + tid; j < n; j += NT) wk[j] = t * (H[c + (int64_t)j * ld] + wk[j]);
+    __syncthreads();
+    for (int j = c + 1 + wave; j < n; j += NT / 64) {
+      const double sj = wk[j];
+      for (int r = c + lane; r < n; r += 64) {
+        const double h = H[r + (int64_t)j * ld];
+        H[r + (int64_t)j * ld] = (r == c) ? h - sj : h - sj * H[r + (int64_t)k * ld];
+      }
     }
     __syncthreads();
     for (int r = tid; r < n; r += NT) {
@@ -225,7 +245,7 @@ template <int NT> GS_DEV void d_apply_chain(double* Zg, int64_t ldz, int n, cons
 }
 
 // doubleShiftQR! (GenericSchur.jl:837-952) with deferred Z
-template <int NT, class L> GS_DEV void d_sweep(double* H, L lay, int n, double* Zg, int64_t ldz, bool wantz,
+template <int NT, class L, bool FAST> GS_DEV void d_sweep(double* H, L lay, int n, double* Zg, int64_t ldz, bool wantz,
                                                double* rot, int* redi, zd shift1, zd shift2, int istart, int iend,
                                                int64_t& nstep) {
   const int tid = threadIdx.x;
@@ -259,16 +279,29 @@ template <int NT, class L> GS_DEV void d_sweep(double* H, L lay, int n, double* 
   const int mx = (found > 0) ? found : istart;
   double v0, v1, v2s;
   firstcol(mx, v0, v1, v2s);
+  // FAST: the next step's reflector input (H[k+1..k+3, k]) is taken from the
+  // registers of the lanes that computed it in the right update
+  // (single-wavefront blocks), the reflector merges its divisions, and the
+  // block barriers are wavefront ordering points
+  constexpr bool RL = FAST && NT == 64;
+  double nv0 = 0.0, nv1 = 0.0, nv2 = 0.0;
   for (int k = mx; k <= iend - 1; ++k) {
     const int nr = min(3, iend - k + 1);
     if (k > mx) {
-      v0 = HR(k, k - 1);
-      v1 = HR(k + 1, k - 1);
-      v2s = (nr == 3) ? HR(k + 2, k - 1) : 0.0;
+      if (RL) {
+        v0 = nv0;
+        v1 = nv1;
+        v2s = (nr == 3) ? nv2 : 0.0;
+      } else {
+        v0 = HR(k, k - 1);
+        v1 = HR(k + 1, k - 1);
+        v2s = (nr == 3) ? HR(k + 2, k - 1) : 0.0;
+      }
     }
     double x1 = v1, x2 = (nr == 3) ? v2s : 0.0;
-    const double tau1 = dreflector(nr, v0, x1, x2);
-    bsync<NT>();
+    const double tau1 = FAST ? dreflector_fast(nr, v0, x1, x2) : dreflector(nr, v0, x1, x2);
+    if (FAST) bsync_full<NT>();
+    else bsync<NT>();
     if (tid == 0) {
       if (k > mx) {
         HR(k, k - 1) = v0;
@@ -290,14 +323,27 @@ template <int NT, class L> GS_DEV void d_sweep(double* H, L lay, int n, double* 
         HR(k + 1, j) = b - ss * tau2;
         HR(k + 2, j) = c - ss * tau3;
       }
-      __syncthreads();
+      if (FAST) bsync_full<NT>();
+      else __syncthreads();
       const int jm = min(k + 3, iend);
+      double cap0 = 0.0, cap1 = 0.0, cap2 = 0.0;
       for (int j = 1 + tid; j <= jm; j += NT) {
         const double a = HR(j, k), b = HR(j, k + 1), c = HR(j, k + 2);
         const double ss = a + vv2 * b + vv3 * c;
-        HR(j, k) = a - ss * tau1;
+        const double na = a - ss * tau1;
+        HR(j, k) = na;
         HR(j, k + 1) = b - ss * tau2;
         HR(j, k + 2) = c - ss * tau3;
+        if (RL) {
+          if (j == k + 1) cap0 = na;
+          if (j == k + 2) cap1 = na;
+          if (j == k + 3) cap2 = na;
+        }
+      }
+      if (RL) {
+        nv0 = rdlane(cap0, k & 63);
+        nv1 = rdlane(cap1, (k + 1) & 63);
+        nv2 = rdlane(cap2, (k + 2) & 63);
       }
     } else {
       for (int j = k + tid; j <= n; j += NT) {
@@ -306,7 +352,8 @@ template <int NT, class L> GS_DEV void d_sweep(double* H, L lay, int n, double* 
         HR(k, j) = a - ss * tau1;
         HR(k + 1, j) = b - ss * tau2;
       }
-      __syncthreads();
+      if (FAST) bsync_full<NT>();
+      else __syncthreads();
       for (int j = 1 + tid; j <= iend; j += NT) {
         const double a = HR(j, k), b = HR(j, k + 1);
         const double ss = a + vv2 * b;
@@ -319,15 +366,163 @@ template <int NT, class L> GS_DEV void d_sweep(double* H, L lay, int n, double* 
       rot[3 * (k - mx) + 1] = vv2;
       rot[3 * (k - mx) + 2] = (nr == 3) ? x2 : 0.0;
     }
-    __syncthreads();
+    if (FAST) bsync_full<NT>();
+    else __syncthreads();
   }
   nstep += iend - mx;
   if (wantz) d_apply_chain<NT>(Zg, ldz, n, rot, mx, iend - 1);
   __syncthreads();
 }
 
+// ------------------------------------------------------------------------
+// Windowed sweep for the batched QR kernel (one wavefront per matrix, H and
+// Z in HBM): the bulge chase runs in chunks of up to DKC steps whose window
+// of H (rows and columns k0-1 .. kend+3) is chased in LDS; the chunk's
+// reflectors, logged in LDS, then go to the rest of H (columns right of the
+// window, rows above it) and to Z as one chain per line (chain_line).  Each
+// element receives the same updates in the same order and with the same
+// expressions as in d_sweep.  The LDS footprint is the window only.
+constexpr int DW = 32;       // window rows and columns
+constexpr int DKC = DW - 4;  // bulge steps per chunk
+
+template <bool FAST> GS_DEV void d_sweep_win(double* H, int64_t ld, int n, double* Zg, int64_t ldz, bool wantz,
+                                            double* win, double* lg, int* redi, zd shift1, zd shift2, int istart,
+                                            int iend, int64_t& nstep) {
+  const int lane = threadIdx.x;
+  const FullLayout lay{ld};
+  const double r1r = shift1.x, r1i = shift1.y, r2r = shift2.x, r2i = shift2.y;
+  const double eps = GS_EPS;
+  auto firstcol = [&](int m, double& v0, double& v1, double& v2) {
+    double H21s = HR(m + 1, m);
+    double s = fabs(HR(m, m) - r2r) + fabs(r2i) + fabs(H21s);
+    H21s = H21s / s;
+    v0 = H21s * HR(m, m + 1) + (HR(m, m) - r1r) * ((HR(m, m) - r2r) / s) - r1i * (r2i / s);
+    v1 = H21s * (HR(m, m) + HR(m + 1, m + 1) - r1r - r2r);
+    v2 = H21s * HR(m + 2, m + 1);
+    s = fabs(v0) + fabs(v1) + fabs(v2);
+    v0 = v0 / s;
+    v1 = v1 / s;
+    v2 = v2 / s;
+  };
+  int found = -1;
+  for (int m = iend - 2 - lane; m >= istart + 1; m -= 64) {
+    double a0, a1, a2;
+    firstcol(m, a0, a1, a2);
+    if (fabs(HR(m, m - 1)) * (fabs(a1) + fabs(a2)) <=
+        eps * fabs(a0) * (fabs(HR(m - 1, m - 1)) + fabs(HR(m, m)) + fabs(HR(m + 1, m + 1)))) {
+      found = m;
+      break;
+    }
+  }
+  found = block_max_int<64>(found, redi);
+  const int mx = (found > 0) ? found : istart;
+  double v0, v1, v2s;
+  firstcol(mx, v0, v1, v2s);
+  for (int k0 = mx; k0 <= iend - 1;) {
+    const int kend = min(k0 + DKC - 1, iend - 1);
+    const int wb = max(k0 - 1, 1), we = min(kend + 3, iend);
+    const int W = we - wb + 1;
+    auto WIN = [&](int r, int c) -> double& { return win[(r - wb) + (c - wb) * DW]; };
+    for (int c = 0; c < W; ++c)
+      if (lane < W) win[lane + c * DW] = H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld];
+    __syncthreads();
+    double nv0 = 0.0, nv1 = 0.0, nv2 = 0.0;
+    for (int k = k0; k <= kend; ++k) {
+      const int nr = min(3, iend - k + 1);
+      if (k > mx) {
+        if (k > k0) {
+          v0 = nv0;
+          v1 = nv1;
+          v2s = (nr == 3) ? nv2 : 0.0;
+        } else {
+          v0 = WIN(k, k - 1);
+          v1 = WIN(k + 1, k - 1);
+          v2s = (nr == 3) ? WIN(k + 2, k - 1) : 0.0;
+        }
+      }
+      double x1 = v1, x2 = (nr == 3) ? v2s : 0.0;
+      const double tau1 = FAST ? dreflector_fast(nr, v0, x1, x2) : dreflector(nr, v0, x1, x2);
+      wsync();
+      if (lane == 0) {
+        if (k > mx) {
+          WIN(k, k - 1) = v0;
+          WIN(k + 1, k - 1) = 0.0;
+          if (k < iend - 1) WIN(k + 2, k - 1) = 0.0;
+        } else if (mx > istart) {
+          WIN(k, k - 1) = WIN(k, k - 1) * (1.0 - tau1);
+        }
+      }
+      wsync();
+      const double vv2 = x1;
+      const double tau2 = tau1 * vv2;
+      const double vv3 = (nr == 3) ? x2 : 0.0;
+      const double tau3 = tau1 * vv3;
+      {  // left: rows k.. over the window's columns k..we
+        const int j = k + lane;
+        if (j <= we) {
+          if (nr == 3) {
+            const double p0 = WIN(k, j), p1 = WIN(k + 1, j), p2 = WIN(k + 2, j);
+            const double ss = p0 + vv2 * p1 + vv3 * p2;
+            WIN(k, j) = p0 - ss * tau1;
+            WIN(k + 1, j) = p1 - ss * tau2;
+            WIN(k + 2, j) = p2 - ss * tau3;
+          } else {
+            const double p0 = WIN(k, j), p1 = WIN(k + 1, j);
+            const double ss = p0 + vv2 * p1;
+            WIN(k, j) = p0 - ss * tau1;
+            WIN(k + 1, j) = p1 - ss * tau2;
+          }
+        }
+      }
+      wsync();
+      {  // right: columns k.. over the window's rows wb..min(k+3, iend)
+        const int jm = (nr == 3) ? min(k + 3, iend) : iend;
+        const int j = wb + lane;
+        double na = 0.0;
+        if (j <= jm) {
+          if (nr == 3) {
+            const double p0 = WIN(j, k), p1 = WIN(j, k + 1), p2 = WIN(j, k + 2);
+            const double ss = p0 + vv2 * p1 + vv3 * p2;
+            na = p0 - ss * tau1;
+            WIN(j, k) = na;
+            WIN(j, k + 1) = p1 - ss * tau2;
+            WIN(j, k + 2) = p2 - ss * tau3;
+          } else {
+            const double p0 = WIN(j, k), p1 = WIN(j, k + 1);
+            const double ss = p0 + vv2 * p1;
+            na = p0 - ss * tau1;
+            WIN(j, k) = na;
+            WIN(j, k + 1) = p1 - ss * tau2;
+          }
+        }
+        nv0 = rdlane(na, (k + 1 - wb) & 63);
+        nv1 = rdlane(na, (k + 2 - wb) & 63);
+        nv2 = rdlane(na, (k + 3 - wb) & 63);
+      }
+      if (lane == 0) {
+        lg[3 * (k - k0)] = tau1;
+        lg[3 * (k - k0) + 1] = vv2;
+        lg[3 * (k - k0) + 2] = vv3;
+      }
+      wsync();
+    }
+    for (int c = 0; c < W; ++c)
+      if (lane < W) H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld] = win[lane + c * DW];
+    __syncthreads();
+    for (int col = we + 1 + lane; col <= n; col += 64)
+      chain_line(H + (int64_t)(col - 1) * ld, 1, lg, k0, k0, kend, iend);
+    for (int row = 1 + lane; row <= wb - 1; row += 64) chain_line(H + (row - 1), ld, lg, k0, k0, kend, iend);
+    if (wantz)
+      for (int row = 1 + lane; row <= n; row += 64) chain_line(Zg + (row - 1), ldz, lg, k0, k0, kend, iend);
+    __syncthreads();
+    k0 = kend + 1;
+  }
+  nstep += iend - mx;
+}
+
+// ------------------------------------------------------------------------
 // gschur!(H::Hessenberg{<:AbstractFloat}, Z) (GenericSchur.jl:513-699)
-template <int NT, class L> GS_DEV int64_t d_qr(double* H, L lay, int n, double* Zg, int64_t ldz, bool wantz,
+template <int NT, class L, bool FAST, bool WIN = false> GS_DEV int64_t d_qr(double* H, L lay, int n, double* Zg, int64_t ldz, bool wantz,
                                                double* rot, int* redi, zd* Wg, int64_t maxiter, int64_t& nsweep,
                                                int64_t& nstep) {
   const int tid = threadIdx.x;
@@ -435,7 +630,11 @@ template <int NT, class L> GS_DEV int64_t d_qr(double* H, L lay, int n, double* 
         }
       }
       nsweep += 1;
-      d_sweep<NT>(H, lay, n, Zg, ldz, wantz, rot, redi, zmk(r1r, r1i), zmk(r2r, r2i), istart, iend, nstep);
+      if constexpr (WIN)
+        d_sweep_win<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + DW * DW, redi, zmk(r1r, r1i), zmk(r2r, r2i),
+                          istart, iend, nstep);
+      else
+        d_sweep<NT, L, FAST>(H, lay, n, Zg, ldz, wantz, rot, redi, zmk(r1r, r1i), zmk(r2r, r2i), istart, iend, nstep);
     }
     if (deflate && istart >= iend) {
       if (tid == 0) Wg[iwcur - 1] = zr(HR(iend, iend));
@@ -523,7 +722,8 @@ template <int NT> __global__ void __launch_bounds__(NT) dhq_kernel(BatchArgs a) 
   const int64_t ld = (int64_t)(n | 1);
   double* H = smem;
   double* tau = H + ld * n;
-  double* red = tau + n;
+  double* wk = tau + n;
+  double* red = wk + n;
   const double* A = a.A + o;
   double* T = a.T + o;
   double* Z = a.Z + o;
@@ -536,14 +736,14 @@ template <int NT> __global__ void __launch_bounds__(NT) dhq_kernel(BatchArgs a) 
     a.scl[2 * m] = cscale;
     a.scl[2 * m + 1] = anrm;
   }
-  if (!hessin) d_hessenberg<NT>(H, ld, n, tau, red);
+  if (!hessin) d_hessenberg<NT>(H, ld, n, tau, red, wk);
   for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
     const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
     T[e] = (r > c + 1) ? 0.0 : H[r + c * ld];
   }
   if (wantz && !hessin) {
     __syncthreads();
-    d_form_q_inplace<NT>(H, ld, n, tau);
+    d_form_q_inplace<NT>(H, ld, n, tau, wk);
     for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
       const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
       Z[e] = H[r + c * ld];
@@ -551,7 +751,7 @@ template <int NT> __global__ void __launch_bounds__(NT) dhq_kernel(BatchArgs a) 
   }
 }
 
-template <int NT> __global__ void __launch_bounds__(NT) dqr_kernel(BatchArgs a) {
+template <int NT, bool FAST> __global__ void __launch_bounds__(NT) dqr_kernel(BatchArgs a) {
   extern __shared__ double smem[];
   const int b = blockIdx.x;
   if (b >= a.count) return;
@@ -577,7 +777,7 @@ template <int NT> __global__ void __launch_bounds__(NT) dqr_kernel(BatchArgs a) 
   __syncthreads();
   const int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
   int64_t nsweep = 0, nstep = 0;
-  const int64_t st = d_qr<NT>(H, lay, n, Z, (int64_t)n, wantz, rot, redi, W, maxiter, nsweep, nstep);
+  const int64_t st = d_qr<NT, BandLayout<3>, FAST>(H, lay, n, Z, (int64_t)n, wantz, rot, redi, W, maxiter, nsweep, nstep);
   __syncthreads();
   if (threadIdx.x == 0) {
     if (a.stats) {
@@ -604,14 +804,14 @@ template <int NT> __global__ void __launch_bounds__(NT) dschur_global_kernel(Sin
   double cscale, anrm;
   d_prep<NT>(a.A, a.lda, T, ldt, n, a.flags, red, cscale, anrm);
   if (!hessin) {
-    d_hessenberg<NT>(T, ldt, n, tau, red);
+    d_hessenberg<NT>(T, ldt, n, tau, red, tau + 4 * n);
     if (wantz) {
       for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
         const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
         Z[r + c * ldz] = T[r + c * ldt];
       }
       __syncthreads();
-      d_form_q_inplace<NT>(T, ldt, n, tau);
+      d_form_q_inplace<NT>(T, ldt, n, tau, tau + 4 * n);
       for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
         const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
         const double q = T[r + c * ldt];
@@ -631,7 +831,7 @@ template <int NT> __global__ void __launch_bounds__(NT) dschur_global_kernel(Sin
   const int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
   int64_t nsweep = 0, nstep = 0;
   zd* W = reinterpret_cast<zd*>(a.W);
-  const int64_t st = d_qr<NT>(T, lay, n, Z, ldz, wantz, rot, redi, W, maxiter, nsweep, nstep);
+  const int64_t st = d_qr<NT, FullLayout, true>(T, lay, n, Z, ldz, wantz, rot, redi, W, maxiter, nsweep, nstep);
   __syncthreads();
   if (threadIdx.x == 0) {
     if (a.stats) {
@@ -644,9 +844,50 @@ template <int NT> __global__ void __launch_bounds__(NT) dschur_global_kernel(Sin
   d_finish<NT>(T, lay, n, T, ldt, W, cscale, anrm);
 }
 
+// Batched kernel 2, windowed form: one wavefront per matrix, H (the T
+// buffer) and Z in HBM, LDS holds the chase window and the chunk's log.
+template <bool FAST> __global__ void __launch_bounds__(64) dqrw_kernel(BatchArgs a) {
+  __shared__ double win[DW * DW + 3 * DKC];
+  __shared__ int redi[2];
+  const int b = blockIdx.x;
+  if (b >= a.count) return;
+  const int m = a.order[a.first + b];
+  const int n = a.n[m];
+  if (n <= 0) return;
+  if (a.info[m] != 0) return;
+  const int64_t o = a.off[m];
+  double* T = a.T + o;
+  double* Z = a.Z + o;
+  zd* W = reinterpret_cast<zd*>(a.W) + a.woff[m];
+  const bool wantz = (a.flags & 0x01) != 0;
+  const FullLayout lay{(int64_t)n};
+  const int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
+  int64_t nsweep = 0, nstep = 0;
+  const int64_t st = d_qr<64, FullLayout, FAST, true>(T, lay, n, Z, (int64_t)n, wantz, win, redi, W, maxiter, nsweep,
+                                                      nstep);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (a.stats) {
+      a.stats[2 * m] = nsweep;
+      a.stats[2 * m + 1] = nstep;
+    }
+    a.info[m] = (int32_t)(st > 0x7fffffff ? 0x7fffffff : st);
+  }
+  if (st != 0) return;
+  d_finish<64>(T, lay, n, T, (int64_t)n, W, a.scl[2 * m], a.scl[2 * m + 1]);
+}
+
+hipError_t launch_dqrw(const BatchArgs& a, hipStream_t s) {
+  if (a.flags & 0x1000)
+    hipLaunchKernelGGL((dqrw_kernel<false>), dim3(a.count), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((dqrw_kernel<true>), dim3(a.count), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
 size_t dhq_lds_bytes(int n, int nt) {
   const size_t ld = (size_t)(n | 1);
-  return 8 * (ld * n + (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
+  return 8 * (ld * n + 2 * (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
 }
 size_t dqr_lds_bytes(int n, int nt) {
   return 8 * ((size_t)BandLayout<3>::size(n) + 3 * (size_t)(n > 0 ? n : 1)) + 4 * (size_t)(nt / 64 + 1);
@@ -664,8 +905,15 @@ template <int NT> static hipError_t launch_dhq_nt(const BatchArgs& a, size_t lds
   return hipGetLastError();
 }
 template <int NT> static hipError_t launch_dqr_nt(const BatchArgs& a, size_t lds, hipStream_t s) {
-  (void)hipFuncSetAttribute((const void*)dqr_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(dqr_kernel<NT>, dim3(a.count), dim3(NT), lds, s, a);
+  if (a.flags & 0x1000) {  // reference-form sweep (kept for A/B measurements)
+    (void)hipFuncSetAttribute((const void*)dqr_kernel<NT, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL((dqr_kernel<NT, false>), dim3(a.count), dim3(NT), lds, s, a);
+  } else {
+    (void)hipFuncSetAttribute((const void*)dqr_kernel<NT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL((dqr_kernel<NT, true>), dim3(a.count), dim3(NT), lds, s, a);
+  }
   return hipGetLastError();
 }
 

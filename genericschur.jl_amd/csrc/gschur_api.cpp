@@ -70,6 +70,16 @@ struct Plan {
 static int fail_hip(gs_handle* h, hipError_t e, const char* where) { return handle_fail_hip(h, e, where); }
 static int fail(gs_handle* h, int code, const char* msg) { return handle_fail(h, code, msg); }
 
+// GSCHUR_QR_BAND=1 selects the band-resident QR kernels everywhere (A/B
+// measurements), as GS_QR_BAND does per call
+static int env_qr_band() {
+  static const int v = [] {
+    const char* e = std::getenv("GSCHUR_QR_BAND");
+    return (e && e[0] == '1') ? GS_QR_BAND : 0;
+  }();
+  return v;
+}
+
 #define HIPCHK(h, call)                                  \
   do {                                                   \
     hipError_t e_ = (call);                              \
@@ -299,7 +309,10 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
     if (e != hipSuccess) return e;
     if (timed && (e = hipEventRecord(h->tev[4 * bi + 1], s)) != hipSuccess) return e;
     if (timed && (e = hipEventRecord(h->tev[4 * bi + 2], s)) != hipSuccess) return e;
-    e = p->is_complex ? launch_zqr(a, b.nmax, b.nt_qr, s) : launch_dqr(a, b.nmax, b.nt_qr, s);
+    if (flags & GS_QR_BAND)
+      e = p->is_complex ? launch_zqr(a, b.nmax, b.nt_qr, s) : launch_dqr(a, b.nmax, b.nt_qr, s);
+    else
+      e = p->is_complex ? launch_zqrw(a, s) : launch_dqrw(a, s);
     if (e != hipSuccess) return e;
     if (timed && (e = hipEventRecord(h->tev[4 * bi + 3], s)) != hipSuccess) return e;
   }
@@ -360,7 +373,7 @@ static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n,
     const char* v = std::getenv("GSCHUR_REFERENCE_SWEEP");
     return v && v[0] == '1';
   }();
-  const int kflags = (flags & (GS_WANTZ | GS_SCALE)) | (ref_sweep ? 0x1000 : 0);
+  const int kflags = (flags & (GS_WANTZ | GS_SCALE | GS_QR_BAND)) | (ref_sweep ? 0x1000 : 0) | env_qr_band();
   const size_t es = cplx ? 16 : 8;
   double* dZdummy = nullptr;
   if (flags & GS_DEVICE_PTRS) {
@@ -533,7 +546,7 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
   const size_t es = cplx ? 16 : 8;
   const size_t ed = es / 8;
   const bool dev = (flags & GS_DEVICE_PTRS) != 0;
-  const int kflags = flags & (GS_WANTZ | GS_SCALE | GS_HESSENBERG | GS_CHECK_SUBDIAG);
+  const int kflags = (flags & (GS_WANTZ | GS_SCALE | GS_HESSENBERG | GS_CHECK_SUBDIAG | GS_QR_BAND)) | env_qr_band();
   // device operands, packed with ld = n when staged
   double *dA, *dT, *dZ, *dW;
   int32_t* dInfo;
@@ -542,7 +555,7 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
   HIPCHK(h, h->bInfo.ensure(sizeof(int32_t)));
   HIPCHK(h, h->bStats.ensure(2 * sizeof(int64_t)));
   HIPCHK(h, h->bScl.ensure(2 * sizeof(double)));
-  HIPCHK(h, h->bWs.ensure(es * 4 * (size_t)n + 64));
+  HIPCHK(h, h->bWs.ensure(es * 5 * (size_t)n + 64));
   if (dev) {
     dA = A;
     dT = T;
@@ -606,8 +619,12 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
     a.maxinner = maxinner;
     const int nthq = default_nt_hq(cplx, (int)n), ntqr = default_nt_qr(cplx, (int)n);
     le = cplx ? launch_zhq(a, (int)n, nthq, h->stream) : launch_dhq(a, (int)n, nthq, h->stream);
-    if (le == hipSuccess)
-      le = cplx ? launch_zqr(a, (int)n, ntqr, h->stream) : launch_dqr(a, (int)n, ntqr, h->stream);
+    if (le == hipSuccess) {
+      if (kflags & GS_QR_BAND)
+        le = cplx ? launch_zqr(a, (int)n, ntqr, h->stream) : launch_dqr(a, (int)n, ntqr, h->stream);
+      else
+        le = cplx ? launch_zqrw(a, h->stream) : launch_dqrw(a, h->stream);
+    }
   } else {
     SingleArgs a;
     a.n = n;

@@ -41,7 +41,7 @@ struct SingleArgs {
   double* Z;
   int64_t ldz;
   double* W;
-  double* ws;  // device scratch: >= 4 n complex / 4 n real elements
+  double* ws;  // device scratch: >= 5 n complex / 5 n real elements
   int32_t* info;
   int64_t* stats;
   int flags;
@@ -61,6 +61,9 @@ int dschur_max_lds_n();
 
 hipError_t launch_zhq(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 hipError_t launch_zqr(const BatchArgs& a, int nmax, int nt, hipStream_t s);
+// windowed QR launch: one wavefront per matrix, H and Z in HBM
+hipError_t launch_zqrw(const BatchArgs& a, hipStream_t s);
+hipError_t launch_dqrw(const BatchArgs& a, hipStream_t s);
 hipError_t launch_dhq(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 hipError_t launch_dqr(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 hipError_t launch_zschur_global(const SingleArgs& a, int nt, hipStream_t s);

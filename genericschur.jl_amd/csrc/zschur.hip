@@ -34,8 +34,9 @@ namespace gs {
 // ------------------------------------------------------------------------
 // Hessenberg reduction in place (hessenberg.jl:3-17), full layout, 0-based.
 // Reflector p is stored below the subdiagonal of column p, tau[p] its scalar.
-template <int NT> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n, zd* tau, double* red) {
+template <int NT> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n, zd* tau, double* red, zd* wk) {
   const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const double sfmin = GS_FMIN / GS_EPS;
   for (int p = 0; p <= n - 2; ++p) {
     // reflector on x = H[p+1 : n-1, p]   (householder.jl:56-102)
@@ -108,23 +109,35 @@ template <int NT> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n, zd* tau, do
     if (tid == 0) tau[p] = t_;
     __syncthreads();
     const zd ct = zconj(t_);
-    // lmul!(H', A[p+1:n, p+1:n])   (householder.jl:157-172)
-    for (int j = p + 1 + tid; j < n; j += NT) {
-      zd va = H[(p + 1) + (int64_t)j * ld];
-      for (int r = r0; r < n; ++r) va = va + zconj(H[r + p * ld]) * H[r + (int64_t)j * ld];
-      va = ct * va;
-      H[(p + 1) + (int64_t)j * ld] = H[(p + 1) + (int64_t)j * ld] - va;
-      for (int r = r0; r < n; ++r) H[r + (int64_t)j * ld] = H[r + (int64_t)j * ld] - va * H[r + p * ld];
+    // lmul!(H', A[p+1:n, p+1:n])   (householder.jl:157-172):
+    // va_j = ct (H[p+1, j] + sum_r conj(v_r) H[r, j]); column j -= va_j [1; v]
+    zdots4<NT>(p + 1, n, r0, n, [&](int r, int j) { return zconj(H[r + p * ld]) * H[r + (int64_t)j * ld]; }, wk);
+    __syncthreads();
+    for (int j = p + 1 + tid; j < n; j += NT) wk[j] = ct * (H[(p + 1) + (int64_t)j * ld] + wk[j]);
+    __syncthreads();
+    for (int j = p + 1 + wave; j < n; j += NT / 64) {
+      const zd va = wk[j];
+      for (int r = p + 1 + lane; r < n; r += 64) {
+        const zd h = H[r + (int64_t)j * ld];
+        H[r + (int64_t)j * ld] = (r == p + 1) ? h - va : h - va * H[r + p * ld];
+      }
     }
     __syncthreads();
-    // rmul!(A[:, p+1:n], H, x)     (householder.jl:140-155, util.jl:103-114)
+    // rmul!(A[:, p+1:n], H, x)     (householder.jl:140-155, util.jl:103-114):
+    // x_r = H[r, p+1] + sum_c H[r, c] v_c; H[r, p+1] -= tau x_r;
+    // H[r, c] += (-tau x_r) conj(v_c)
     const zd mt = -t_;
+    zdots4<NT>(0, n, r0, n, [&](int c, int r) { return H[r + (int64_t)c * ld] * H[c + p * ld]; }, wk);
+    __syncthreads();
     for (int r = tid; r < n; r += NT) {
-      zd x = H[r + (int64_t)(p + 1) * ld];
-      for (int c = r0; c < n; ++c) x = x + H[r + (int64_t)c * ld] * H[c + p * ld];
+      const zd x = H[r + (int64_t)(p + 1) * ld] + wk[r];
       H[r + (int64_t)(p + 1) * ld] = H[r + (int64_t)(p + 1) * ld] - t_ * x;
-      const zd xm = x * mt;
-      for (int c = r0; c < n; ++c) H[r + (int64_t)c * ld] = H[r + (int64_t)c * ld] + xm * zconj(H[c + p * ld]);
+      wk[r] = x * mt;
+    }
+    __syncthreads();
+    for (int c = r0 + wave; c < n; c += NT / 64) {
+      const zd vc = zconj(H[c + p * ld]);
+      for (int r = lane; r < n; r += 64) H[r + (int64_t)c * ld] = H[r + (int64_t)c * ld] + wk[r] * vc;
     }
     __syncthreads();
   }
@@ -134,18 +147,25 @@ template <int NT> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n, zd* tau, do
 // Q = diag(1, H_0 H_1 ... H_{n-2}) formed in place over the reflector
 // storage (hessenberg.jl:150-166; the unblocked backward accumulation of
 // xUNG2R on the reflectors shifted one column right).
-template <int NT> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, int n, const zd* tau) {
+template <int NT> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, int n, const zd* tau, zd* wk) {
   const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   for (int k = n - 2; k >= 0; --k) {
     const int c = k + 1;  // column of Q produced by reflector k
     const zd t = tau[k];
-    // apply H_k to Q(c:n-1, c+1:n-1); Q(c, j>c) is zero at this point
-    for (int j = c + 1 + tid; j < n; j += NT) {
-      zd s = H[c + (int64_t)j * ld];
-      for (int r = c + 1; r < n; ++r) s = s + zconj(H[r + (int64_t)k * ld]) * H[r + (int64_t)j * ld];
-      s = t * s;
-      H[c + (int64_t)j * ld] = H[c + (int64_t)j * ld] - s;
-      for (int r = c + 1; r < n; ++r) H[r + (int64_t)j * ld] = H[r + (int64_t)j * ld] - s * H[r + (int64_t)k * ld];
+    // apply H_k to Q(c:n-1, c+1:n-1); Q(c, j>c) is zero at this point:
+    // s_j = t (Q[c, j] + sum_{r>c} conj(v_r) Q[r, j]); column j -= s_j [1; v]
+    zdots4<NT>(c + 1, n, c + 1, n, [&](int r, int j) { return zconj(H[r + (int64_t)k * ld]) * H[r + (int64_t)j * ld]; },
+               wk);
+    __syncthreads();
+    for (int j = c + 1 + tid; j < n; j += NT) wk[j] = t * (H[c + (int64_t)j * ld] + wk[j]);
+    __syncthreads();
+    for (int j = c + 1 + wave; j < n; j += NT / 64) {
+      const zd sj = wk[j];
+      for (int r = c + lane; r < n; r += 64) {
+        const zd h = H[r + (int64_t)j * ld];
+        H[r + (int64_t)j * ld] = (r == c) ? h - sj : h - sj * H[r + (int64_t)k * ld];
+      }
     }
     __syncthreads();
     // column c: -tau * v below, 1 - tau on the diagonal, zero above
@@ -409,9 +429,232 @@ template <int NT, class L, bool FAST> GS_DEV void z_sweep(zd* H, L lay, int n, z
 }
 
 // ------------------------------------------------------------------------
+// Windowed sweep for the batched QR kernel (one wavefront per matrix, H and
+// Z in HBM).  The bulge chase of a sweep runs in chunks of up to ZKC steps;
+// a chunk's window of H (rows and columns k0-1 .. kend+2) is chased in LDS
+// and the chunk's reflectors, logged in LDS, are then applied to the rest of
+// H (columns right of the window, rows above it) and to Z as one chain per
+// row or column.  Each element receives the reflectors in the same order
+// and with the same expressions as in z_sweep, and the phase and tail
+// scalings of singleShiftQR! (GenericSchur.jl:461-500) are applied to all
+// of H and Z at the same points of the sweep, so the result equals
+// z_sweep's.  The LDS footprint is the window only, so many matrices share
+// a CU and hide each other's latency.
+constexpr int ZW = 32;       // window rows and columns
+constexpr int ZKC = ZW - 3;  // bulge steps per chunk
+
+// Logged reflectors of steps ka..kb (lg[2(k-ka)] = tau1, lg[2(k-ka)+1] = v2)
+// applied to the line x(t) = p[(t - 1) st]: LEFT updates rows k, k+1 of a
+// column of H (GenericSchur.jl:437-446), otherwise columns k, k+1 of a row
+// of H or Z (:447-460).  x(k+1) is read PF steps ahead.
+template <bool LEFT> GS_DEV void zchain(zd* p, int64_t st, const zd* lg, int ka, int kb) {
+  constexpr int PF = 4;
+  zd c0 = p[(int64_t)(ka - 1) * st];
+  zd nb[PF];
+#pragma unroll
+  for (int q = 0; q < PF; ++q)
+    if (ka + q <= kb) nb[q] = p[(int64_t)(ka + q) * st];
+  int k = ka;
+  for (; k + PF - 1 <= kb; k += PF) {
+    zd cur[PF];
+#pragma unroll
+    for (int q = 0; q < PF; ++q) cur[q] = nb[q];
+#pragma unroll
+    for (int q = 0; q < PF; ++q)
+      if (k + PF + q <= kb) nb[q] = p[(int64_t)(k + PF + q) * st];
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      const zd tau1 = lg[2 * (k + q - ka)], v2 = lg[2 * (k + q - ka) + 1];
+      const double tau2 = (tau1 * v2).x;
+      const zd b = cur[q];
+      if (LEFT) {
+        const zd ss = zconj(tau1) * c0 + tau2 * b;
+        p[(int64_t)(k + q - 1) * st] = c0 - ss;
+        c0 = b - ss * v2;
+      } else {
+        const zd ss = tau1 * c0 + tau2 * b;
+        p[(int64_t)(k + q - 1) * st] = c0 - ss;
+        c0 = b - ss * zconj(v2);
+      }
+    }
+  }
+  for (; k <= kb; ++k) {
+    const zd tau1 = lg[2 * (k - ka)], v2 = lg[2 * (k - ka) + 1];
+    const double tau2 = (tau1 * v2).x;
+    const zd b = p[(int64_t)k * st];
+    if (LEFT) {
+      const zd ss = zconj(tau1) * c0 + tau2 * b;
+      p[(int64_t)(k - 1) * st] = c0 - ss;
+      c0 = b - ss * v2;
+    } else {
+      const zd ss = tau1 * c0 + tau2 * b;
+      p[(int64_t)(k - 1) * st] = c0 - ss;
+      c0 = b - ss * zconj(v2);
+    }
+  }
+  p[(int64_t)kb * st] = c0;
+}
+
+template <bool FAST> GS_DEV void z_sweep_win(zd* H, int64_t ld, int n, zd* Zg, int64_t ldz, bool wantz, zd* win,
+                                            zd* lg, int* redi, zd shift, int istart, int iend, int64_t& nstep) {
+  const int lane = threadIdx.x;
+  const double ulp = GS_EPS;
+  const FullLayout lay{ld};
+  // start row (as z_sweep)
+  int found = -1;
+  for (int mm = iend - 1 - lane; mm >= istart + 1; mm -= 64) {
+    const zd h11 = HH(mm, mm);
+    const zd h22 = HH(mm + 1, mm + 1);
+    zd h11s = h11 - shift;
+    double h21 = HH(mm + 1, mm).x;
+    const double s = zabs1(h11s) + fabs(h21);
+    h11s = h11s / s;
+    h21 = h21 / s;
+    const double h10 = HH(mm, mm - 1).x;
+    if (fabs(h10) * fabs(h21) <= ulp * (zabs1(h11s) * (zabs1(h11) + zabs1(h22)))) {
+      found = mm;
+      break;
+    }
+  }
+  found = block_max_int<64>(found, redi);
+  const int istart1 = (found > 0) ? found : istart;
+  zd v0, v1;
+  {
+    const zd h11 = HH(istart1, istart1);
+    zd h11s = h11 - shift;
+    double h21 = HH(istart1 + 1, istart1).x;
+    const double s = zabs1(h11s) + fabs(h21);
+    h11s = h11s / s;
+    h21 = h21 / s;
+    v0 = h11s;
+    v1 = zr(h21);
+  }
+  bool phase_pending = istart1 > istart;
+  int k0 = istart1;
+  while (k0 <= iend - 1) {
+    // the first step is a chunk of its own when the phase scaling follows it
+    const int kend = phase_pending ? k0 : min(k0 + ZKC - 1, iend - 1);
+    const int wb = max(k0 - 1, 1), we = min(kend + 2, iend);
+    const int W = we - wb + 1;
+    auto WIN = [&](int r, int c) -> zd& { return win[(r - wb) + (c - wb) * ZW]; };
+    for (int c = 0; c < W; ++c)
+      if (lane < W) win[lane + c * ZW] = H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld];
+    __syncthreads();
+    zd nv0 = zr(0.0), nv1 = zr(0.0);
+    for (int k = k0; k <= kend; ++k) {
+      if (k > istart1) {
+        if (k > k0) {
+          v0 = nv0;
+          v1 = nv1;
+        } else {
+          v0 = WIN(k, k - 1);
+          v1 = WIN(k + 1, k - 1);
+        }
+      }
+      const zd tau1 = FAST ? zreflector2_fast(v0, v1) : zreflector2(v0, v1);
+      wsync();
+      if (k > istart1 && lane == 0) {
+        WIN(k, k - 1) = v0;
+        WIN(k + 1, k - 1) = zr(0.0);
+      }
+      const zd v2 = v1;
+      const double tau2 = (tau1 * v2).x;
+      const zd ct1 = zconj(tau1);
+      {  // left: rows k, k+1 over the window's columns k..we
+        const int j = k + lane;
+        if (j <= we) {
+          const zd a = WIN(k, j), b = WIN(k + 1, j);
+          const zd ss = ct1 * a + tau2 * b;
+          WIN(k, j) = a - ss;
+          WIN(k + 1, j) = b - ss * v2;
+        }
+      }
+      wsync();
+      {  // right: columns k, k+1 over the window's rows wb..min(k+2, iend)
+        const int jm = min(k + 2, iend);
+        const zd cv2 = zconj(v2);
+        const int j = wb + lane;
+        zd na = zr(0.0);
+        if (j <= jm) {
+          const zd a = WIN(j, k), b = WIN(j, k + 1);
+          const zd ss = tau1 * a + tau2 * b;
+          na = a - ss;
+          WIN(j, k) = na;
+          WIN(j, k + 1) = b - ss * cv2;
+        }
+        const int l0 = (k + 1) - wb, l1 = (k + 2) - wb;
+        nv0 = zmk(rdlane(na.x, l0 & 63), rdlane(na.y, l0 & 63));
+        nv1 = zmk(rdlane(na.x, l1 & 63), rdlane(na.y, l1 & 63));
+      }
+      if (lane == 0) {
+        lg[2 * (k - k0)] = tau1;
+        lg[2 * (k - k0) + 1] = v2;
+      }
+      wsync();
+    }
+    // window back to H, then the chunk's reflectors outside the window
+    for (int c = 0; c < W; ++c)
+      if (lane < W) H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld] = win[lane + c * ZW];
+    __syncthreads();
+    for (int col = we + 1 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, lg, k0, kend);
+    for (int row = 1 + lane; row <= wb - 1; row += 64) zchain<false>(H + (row - 1), ld, lg, k0, kend);
+    if (wantz)
+      for (int row = 1 + lane; row <= n; row += 64) zchain<false>(Zg + (row - 1), ldz, lg, k0, kend);
+    __syncthreads();
+    if (phase_pending) {
+      // GenericSchur.jl:461-482 after the first step: rows of S right of the
+      // diagonal times t, columns of S above it times conj(t), S =
+      // istart1..iend minus istart1+1; Z's columns of S times conj(t)
+      phase_pending = false;
+      zd t = zr(1.0) - lg[0];
+      t = t / zabs(t);
+      const zd ctt = zconj(t);
+      if (lane == 0) {
+        HH(istart1 + 1, istart1) = HH(istart1 + 1, istart1) * ctt;
+        if (istart1 + 2 <= iend) HH(istart1 + 2, istart1 + 1) = HH(istart1 + 2, istart1 + 1) * t;
+      }
+      __syncthreads();
+      for (int c = 1; c <= n; ++c) {
+        const bool cin = (c >= istart1 && c <= iend && c != istart1 + 1);
+        const int rmax = min(c - 1, iend);
+        for (int r = 1 + lane; r <= rmax; r += 64) {
+          const bool rin = (r >= istart1 && r != istart1 + 1);
+          if (!rin && !cin) continue;
+          zd x = HH(r, c);
+          if (rin) x = x * t;
+          if (cin) x = x * ctt;
+          HH(r, c) = x;
+        }
+      }
+      if (wantz)
+        for (int r = 1 + lane; r <= n; r += 64)
+          for (int c = istart1; c <= iend; ++c)
+            if (c != istart1 + 1) ZZ(r, c) = ZZ(r, c) * ctt;
+      __syncthreads();
+    }
+    k0 = kend + 1;
+  }
+  nstep += iend - istart1;
+  // ensure reality of the tail subdiagonal (GenericSchur.jl:486-500)
+  zd t = HH(iend, iend - 1);
+  if (t.y != 0.0) {
+    const double rtv = zabs(t);
+    t = t / rtv;
+    __syncthreads();
+    if (lane == 0) HH(iend, iend - 1) = zr(rtv);
+    const zd ctt = zconj(t);
+    for (int c = iend + 1 + lane; c <= n; c += 64) HH(iend, c) = HH(iend, c) * ctt;
+    for (int r = 1 + lane; r <= iend - 1; r += 64) HH(r, iend) = HH(r, iend) * t;
+    if (wantz)
+      for (int r = 1 + lane; r <= n; r += 64) ZZ(r, iend) = ZZ(r, iend) * t;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------
 // gschur!(H::Hessenberg{Complex}, Z) (GenericSchur.jl:194-335).
 // Returns 0, or the positive iteration count when maxiter was exceeded.
-template <int NT, class L, bool FAST> GS_DEV int64_t z_qr(zd* H, L lay, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot,
+template <int NT, class L, bool FAST, bool WIN = false> GS_DEV int64_t z_qr(zd* H, L lay, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot,
                                                int* redi, int64_t maxiter, int64_t maxinner, int64_t& nsweep,
                                                int64_t& nstep) {
   const int tid = threadIdx.x;
@@ -486,7 +729,10 @@ template <int NT, class L, bool FAST> GS_DEV int64_t z_qr(zd* H, L lay, int n, z
         }
       }
       nsweep += 1;
-      z_sweep<NT, L, FAST>(H, lay, n, Zg, ldz, wantz, rot, redi, t, istart, iend, nstep);
+      if constexpr (WIN)
+        z_sweep_win<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + ZW * ZW, redi, t, istart, iend, nstep);
+      else
+        z_sweep<NT, L, FAST>(H, lay, n, Zg, ldz, wantz, rot, redi, t, istart, iend, nstep);
     }
   }
   return 0;
@@ -526,7 +772,8 @@ template <int NT> __global__ void __launch_bounds__(NT) zhq_kernel(BatchArgs a) 
   const int64_t ld = (int64_t)(n | 1);
   zd* H = reinterpret_cast<zd*>(smem);
   zd* tau = H + ld * n;
-  double* red = reinterpret_cast<double*>(tau + n);
+  zd* wk = tau + n;
+  double* red = reinterpret_cast<double*>(wk + n);
   const zd* A = reinterpret_cast<const zd*>(a.A) + o;
   zd* T = reinterpret_cast<zd*>(a.T) + o;
   zd* Z = reinterpret_cast<zd*>(a.Z) + o;
@@ -540,14 +787,14 @@ template <int NT> __global__ void __launch_bounds__(NT) zhq_kernel(BatchArgs a) 
     a.scl[2 * m + 1] = anrm;
   }
   if (st != 0) return;
-  if (!hessin) z_hessenberg<NT>(H, ld, n, tau, red);
+  if (!hessin) z_hessenberg<NT>(H, ld, n, tau, red, wk);
   for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
     const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
     T[e] = (r > c + 1) ? zr(0.0) : H[r + c * ld];
   }
   if (wantz && !hessin) {
     __syncthreads();
-    z_form_q_inplace<NT>(H, ld, n, tau);
+    z_form_q_inplace<NT>(H, ld, n, tau, wk);
     for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
       const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
       Z[e] = H[r + c * ld];
@@ -597,6 +844,41 @@ template <int NT, bool FAST> __global__ void __launch_bounds__(NT) zqr_kernel(Ba
   z_finish<NT>(H, lay, n, T, (int64_t)n, W, a.scl[2 * m], a.scl[2 * m + 1]);
 }
 
+// Batched kernel 2, windowed form: one wavefront per matrix, H in HBM (the
+// T buffer, written by zhq_kernel) and Z in HBM; LDS holds the chase window
+// and the chunk's reflector log only.
+template <bool FAST> __global__ void __launch_bounds__(64) zqrw_kernel(BatchArgs a) {
+  __shared__ zd win[ZW * ZW + 2 * ZKC];
+  __shared__ int redi[2];
+  const int b = blockIdx.x;
+  if (b >= a.count) return;
+  const int m = a.order[a.first + b];
+  const int n = a.n[m];
+  if (n <= 0) return;
+  if (a.info[m] != 0) return;  // rejected by zhq_kernel
+  const int64_t o = a.off[m];
+  zd* T = reinterpret_cast<zd*>(a.T) + o;
+  zd* Z = reinterpret_cast<zd*>(a.Z) + o;
+  zd* W = reinterpret_cast<zd*>(a.W) + a.woff[m];
+  const bool wantz = (a.flags & 0x01) != 0;
+  const FullLayout lay{(int64_t)n};
+  int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
+  int64_t maxinner = a.maxinner > 0 ? a.maxinner : 30 * (int64_t)n;
+  int64_t nsweep = 0, nstep = 0;
+  const int64_t st =
+      z_qr<64, FullLayout, FAST, true>(T, lay, n, Z, (int64_t)n, wantz, win, redi, maxiter, maxinner, nsweep, nstep);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (a.stats) {
+      a.stats[2 * m] = nsweep;
+      a.stats[2 * m + 1] = nstep;
+    }
+    a.info[m] = (int32_t)(st > 0x7fffffff ? 0x7fffffff : st);
+  }
+  if (st != 0) return;
+  z_finish<64>(T, lay, n, T, (int64_t)n, W, a.scl[2 * m], a.scl[2 * m + 1]);
+}
+
 // ------------------------------------------------------------------------
 // Single-matrix kernel: H in HBM (the T output buffer), any n.
 template <int NT> __global__ void __launch_bounds__(NT) zschur_global_kernel(SingleArgs a) {
@@ -619,7 +901,7 @@ template <int NT> __global__ void __launch_bounds__(NT) zschur_global_kernel(Sin
     return;
   }
   if (!hessin) {
-    z_hessenberg<NT>(T, ldt, n, tau, red);
+    z_hessenberg<NT>(T, ldt, n, tau, red, tau + 4 * n);
     if (wantz) {
       // hold H in Z, form Q in place in T, then swap the two
       for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
@@ -627,7 +909,7 @@ template <int NT> __global__ void __launch_bounds__(NT) zschur_global_kernel(Sin
         Z[r + c * ldz] = T[r + c * ldt];
       }
       __syncthreads();
-      z_form_q_inplace<NT>(T, ldt, n, tau);
+      z_form_q_inplace<NT>(T, ldt, n, tau, tau + 4 * n);
       for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
         const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
         const zd q = T[r + c * ldt];
@@ -663,7 +945,7 @@ template <int NT> __global__ void __launch_bounds__(NT) zschur_global_kernel(Sin
 // ------------------------------------------------------------------------
 size_t zhq_lds_bytes(int n, int nt) {
   const size_t ld = (size_t)(n | 1);
-  return 16 * (ld * n + (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
+  return 16 * (ld * n + 2 * (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
 }
 size_t zqr_lds_bytes(int n, int nt) {
   return 16 * ((size_t)BandLayout<2>::size(n) + 2 * (size_t)(n > 0 ? n : 1)) + 4 * (size_t)(nt / 64 + 1);
@@ -714,6 +996,14 @@ hipError_t launch_zqr(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
     case 128: return launch_zqr_nt<128>(a, lds, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_zqrw(const BatchArgs& a, hipStream_t s) {
+  if (a.flags & 0x1000)
+    hipLaunchKernelGGL((zqrw_kernel<false>), dim3(a.count), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((zqrw_kernel<true>), dim3(a.count), dim3(64), 0, s, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_zschur_global(const SingleArgs& a, int nt, hipStream_t s) {

@@ -8,11 +8,12 @@ include/gschur_mi355x.h.
 """
 from .api import (ArgumentError, DimensionMismatch, Schur, UnconvergedException, eigvals_, eigvecs, geigvecs,
                   gschur, gschur_, gschur_batched, gschur_batched_device, gschur_hessenberg_, pack, schur_,
-                  synchronize, unpack, eig_offsets, set_stream)
-from ._lib import GSError, last_stats
+                  synchronize, unpack, eig_offsets, set_stream, gschur_device, dgemm_device, batched_max_n)
+from ._lib import GSError, last_stats, last_launches
 
 __all__ = [
     "ArgumentError", "DimensionMismatch", "Schur", "UnconvergedException", "GSError",
     "gschur", "gschur_", "schur_", "gschur_hessenberg_", "eigvals_", "eigvecs", "geigvecs",
     "gschur_batched", "gschur_batched_device", "pack", "unpack", "eig_offsets", "synchronize", "set_stream", "last_stats",
+    "gschur_device", "dgemm_device", "batched_max_n", "last_launches",
 ]

@@ -18,6 +18,7 @@ GS_DEVICE_PTRS = 0x10
 GS_ASYNC = 0x20
 GS_PLAIN = 0x40
 GS_TIMING = 0x80
+GS_QR_BAND = 0x100
 GS_LAUNCH_REC = 6
 
 GS_OK = 0

@@ -273,9 +273,11 @@ def eig_offsets(ns):
     return w
 
 
-def gschur_batched(mats, wantZ=True, scale=True, device=0, raise_on_info=True):
+def gschur_batched(mats, wantZ=True, scale=True, device=0, raise_on_info=True, qr_band=False):
     """Batched gschur! of host matrices (all Float64 or all ComplexF64) in one
-    graph launch.  Returns a list of Schur (and the info array)."""
+    graph launch.  Returns a list of Schur (and the info array).  qr_band
+    selects the band-resident complex QR kernel instead of the windowed one
+    (same arithmetic; for comparisons)."""
     mats = [np.asarray(m) for m in mats]
     if not mats:
         return [], np.zeros(0, dtype=np.int32)
@@ -290,7 +292,7 @@ def gschur_batched(mats, wantZ=True, scale=True, device=0, raise_on_info=True):
     Z = np.zeros_like(flat)
     W = np.zeros(int(ns.sum()), dtype=np.complex128)
     info = np.zeros(len(mats), dtype=np.int32)
-    flags = (GS_WANTZ if wantZ else 0) | (GS_SCALE if scale else 0)
+    flags = (GS_WANTZ if wantZ else 0) | (GS_SCALE if scale else 0) | (_lib.GS_QR_BAND if qr_band else 0)
     h = _lib.handle(device)
     f = _lib.lib().gs_schur_batched_z if cplx else _lib.lib().gs_schur_batched_d
     rc = f(h, len(mats), _ptr(ns), _ptr(off), _ptr(flat), _ptr(T), _ptr(Z), _ptr(W), _ptr(info), flags)
@@ -342,6 +344,11 @@ def set_stream(stream, device=0):
     torch.cuda.current_stream().cuda_stream); None restores its own stream."""
     h = _lib.handle(device)
     _lib.check(_lib.lib().gs_set_stream(h, ctypes.c_void_p(stream) if stream else None), h, "gs_set_stream")
+
+
+def batched_max_n(complex_input):
+    """Largest order the LDS-resident batched kernels take (gs_batched_max_n)."""
+    return int(_lib.lib().gs_batched_max_n(1 if complex_input else 0))
 
 
 def synchronize(device=0):

@@ -49,6 +49,10 @@ typedef struct gs_handle gs_handle;
 #define GS_ASYNC 0x20        /* do not synchronise before returning            */
 #define GS_PLAIN 0x40        /* single matrix: force the unblocked global-     */
                              /* memory kernel (the plain GPU form)             */
+#define GS_QR_BAND 0x100     /* LDS-resident QR phase: keep H's band in LDS    */
+                             /* (up to 80 KB per matrix) instead of the        */
+                             /* default windowed kernel (H in HBM, a 32 x 32   */
+                             /* chase window in LDS); same arithmetic           */
 #define GS_TIMING 0x80       /* record launch durations for gs_last_launches:  */
                              /* batched with GS_DEVICE_PTRS (launched eagerly, */
                              /* no graph, events around every kernel on its    */
