@@ -3,18 +3,27 @@
 //
 //   C(m x n) = beta * C + alpha * op(A) * op(B),   op(X) = X or X^T,
 //
-// all column-major.  A workgroup of 256 threads (4 wavefronts) owns a
-// 128 x 128 tile of C; each wavefront owns a 64 x 64 quarter as 4 x 4 MFMA
-// blocks of 16 x 16 (16 accumulators of 4 doubles per lane).  K advances in
-// slabs of 16 staged through LDS (k-major, padded so the 16-lane row groups
-// of one MFMA operand read fall into different banks).
+// all column-major.  A workgroup of 256 threads (4 wavefronts) owns a BM x BN
+// tile of C, the wavefronts arranged WR x (4 / WR); each wavefront owns
+// (BM / WR) x (BN * WR / 4) as 16 x 16 MFMA blocks.  K advances in slabs of
+// 16 staged through LDS (k-major, padded); the next slab's global loads are
+// issued before the current slab's MFMAs.
+//
+//   128 x 128 tiles (WR = 2) for the trailing updates (large m and n, k = 32);
+//   32 x 128 tiles (WR = 1), split over K, for the W = V^T C products
+//   (m = 32, long k) of the left update and of the Q formation.
 //
 // Operand layout of v_mfma_f64_16x16x4_f64 (lane l, 0 <= l < 64):
 //   A (16 x 4):  lane l supplies A[l % 16][l / 16]
 //   B (4 x 16):  lane l supplies B[l / 16][l % 16]
-//   D (16 x 16): lane l holds D[4 * (l / 16) + r][l % 16] in register r.
+//   D (16 x 16): lane l holds D[l / 16 + 4 r][l % 16] in register r
+//                (established on gfx950 by tests/test_gpu_large.py::test_dgemm)
+// Each block is computed transposed, (A B)^T = B^T A^T, so that the 16 lanes
+// that share an accumulator register hold 16 consecutive rows of C.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 
 #include "internal.h"
 
@@ -22,105 +31,122 @@ namespace gs {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int GBM = 128, GBN = 128, GBK = 16, GPAD = 16;
-constexpr int GLD = GBM + GPAD;  // LDS pitch (doubles) of one k row
+constexpr int GBK = 16, GPAD = 16;
 
-template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void dgemm_kernel(int m, int n, int k, double alpha, const double* __restrict__ A,
-                                                   int64_t lda, const double* __restrict__ B, int64_t ldb,
-                                                   double beta, double* __restrict__ C, int64_t ldc) {
-  __shared__ double As[GBK * GLD];
-  __shared__ double Bs[GBK * GLD];
+template <bool TA, bool TB, int BM, int BN, int WR>
+__global__ __launch_bounds__(256) void dgemm_kernel(int m, int n, int k, int kchunk, double alpha,
+                                                   const double* __restrict__ A, int64_t lda,
+                                                   const double* __restrict__ B, int64_t ldb, double beta,
+                                                   double* __restrict__ C, int64_t ldc, int64_t split_stride) {
+  constexpr int WC = 4 / WR;
+  constexpr int WM = BM / WR, WN = BN / WC;  // wavefront tile
+  constexpr int IM = WM / 16, JN = WN / 16;  // MFMA blocks per wavefront
+  constexpr int LDA_ = BM + GPAD, LDB_ = BN + GPAD;
+  constexpr int NA = BM * GBK / 256, NB = BN * GBK / 256;  // staged elements per thread
+  static_assert(IM >= 1 && JN >= 1 && NA >= 1 && NB >= 1, "tile shape");
+  __shared__ double As[GBK * LDA_];
+  __shared__ double Bs[GBK * LDB_];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int m0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN;
-  const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;
-  v4d acc[4][4];
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int wr = (wave % WR) * WM, wc = (wave / WR) * WN;
+  // split over K: blockIdx.z takes the k range [kb, ke) and writes its
+  // partial product to C + z * split_stride
+  const int kb = blockIdx.z * kchunk, ke = min(k, kb + kchunk);
+  C += (int64_t)blockIdx.z * split_stride;
+  v4d acc[IM][JN];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < IM; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < JN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-  // the next slab's global loads are issued before the current slab's MFMAs
-  double ra[(GBM * GBK) / 256], rb[(GBN * GBK) / 256];
+  double ra[NA], rb[NB];
+  // op(A)[m0 + i, k0 + kk] and op(B)[k0 + kk, n0 + j]; the thread-to-element
+  // maps walk the contiguous direction of the stored operand
+  auto a_idx = [&](int r, int& i, int& kk) {
+    if (!TA) {
+      i = tid % BM;
+      kk = tid / BM + (256 / BM) * r;
+    } else {
+      kk = tid & (GBK - 1);
+      i = (tid >> 4) + 16 * r;
+    }
+  };
+  auto b_idx = [&](int r, int& j, int& kk) {
+    if (TB) {
+      j = tid % BN;
+      kk = tid / BN + (256 / BN) * r;
+    } else {
+      kk = tid & (GBK - 1);
+      j = (tid >> 4) + 16 * r;
+    }
+  };
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < (GBM * GBK) / 256; ++r) {
+    for (int r = 0; r < NA; ++r) {
       int i, kk;
-      if (!TA) {
-        i = tid & (GBM - 1);
-        kk = (tid >> 7) + 2 * r;
-      } else {
-        kk = tid & (GBK - 1);
-        i = (tid >> 4) + 16 * r;
-      }
+      a_idx(r, i, kk);
       const int gi = m0 + i, gk = k0 + kk;
       double v = 0.0;
-      if (gi < m && gk < k) v = TA ? A[(int64_t)gk + (int64_t)gi * lda] : A[(int64_t)gi + (int64_t)gk * lda];
+      if (gi < m && gk < ke) v = TA ? A[(int64_t)gk + (int64_t)gi * lda] : A[(int64_t)gi + (int64_t)gk * lda];
       ra[r] = v;
     }
 #pragma unroll
-    for (int r = 0; r < (GBN * GBK) / 256; ++r) {
+    for (int r = 0; r < NB; ++r) {
       int j, kk;
-      if (TB) {
-        j = tid & (GBN - 1);
-        kk = (tid >> 7) + 2 * r;
-      } else {
-        kk = tid & (GBK - 1);
-        j = (tid >> 4) + 16 * r;
-      }
+      b_idx(r, j, kk);
       const int gj = n0 + j, gk = k0 + kk;
       double v = 0.0;
-      if (gj < n && gk < k) v = TB ? B[(int64_t)gj + (int64_t)gk * ldb] : B[(int64_t)gk + (int64_t)gj * ldb];
+      if (gj < n && gk < ke) v = TB ? B[(int64_t)gj + (int64_t)gk * ldb] : B[(int64_t)gk + (int64_t)gj * ldb];
       rb[r] = v;
     }
   };
-  // op(A)[m0:m0+128, k0:k0+16] goes to As[kk][i], op(B)[k0:k0+16, n0:n0+128] to Bs[kk][j]
   auto stage = [&]() {
 #pragma unroll
-    for (int r = 0; r < (GBM * GBK) / 256; ++r) {
-      const int i = TA ? (tid >> 4) + 16 * r : tid & (GBM - 1);
-      const int kk = TA ? tid & (GBK - 1) : (tid >> 7) + 2 * r;
-      As[kk * GLD + i] = ra[r];
+    for (int r = 0; r < NA; ++r) {
+      int i, kk;
+      a_idx(r, i, kk);
+      As[kk * LDA_ + i] = ra[r];
     }
 #pragma unroll
-    for (int r = 0; r < (GBN * GBK) / 256; ++r) {
-      const int j = TB ? tid & (GBN - 1) : (tid >> 4) + 16 * r;
-      const int kk = TB ? (tid >> 7) + 2 * r : tid & (GBK - 1);
-      Bs[kk * GLD + j] = rb[r];
+    for (int r = 0; r < NB; ++r) {
+      int j, kk;
+      b_idx(r, j, kk);
+      Bs[kk * LDB_ + j] = rb[r];
     }
   };
-  if (k > 0) fetch(0);
-  for (int k0 = 0; k0 < k; k0 += GBK) {
+  if (kb < ke) fetch(kb);
+  for (int k0 = kb; k0 < ke; k0 += GBK) {
     stage();
     __syncthreads();
-    if (k0 + GBK < k) fetch(k0 + GBK);
+    if (k0 + GBK < ke) fetch(k0 + GBK);
 #pragma unroll
     for (int k4 = 0; k4 < GBK / 4; ++k4) {
       const int kk = 4 * k4 + (lane >> 4);
-      double a[4], b[4];
+      double a[IM], b[JN];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kk * GLD + wr + 16 * i + (lane & 15)];
+      for (int i = 0; i < IM; ++i) a[i] = As[kk * LDA_ + wr + 16 * i + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[kk * GLD + wc + 16 * j + (lane & 15)];
+      for (int j = 0; j < JN; ++j) b[j] = Bs[kk * LDB_ + wc + 16 * j + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < IM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < JN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
-  // write back: lane holds rows 4 * (lane / 16) + r, column lane % 16 of each block
+  // register r of lane l holds C row 16 i + l % 16, column 16 j + l / 16 + 4 r
+  // of the wavefront's block (i, j)
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < IM; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int gj = n0 + wc + 16 * j + (lane & 15);
-      if (gj >= n) continue;
+    for (int j = 0; j < JN; ++j) {
+      const int gi = m0 + wr + 16 * i + (lane & 15);
+      if (gi >= m) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gi = m0 + wr + 16 * i + 4 * (lane >> 4) + r;
-        if (gi >= m) continue;
+        const int gj = n0 + wc + 16 * j + (lane >> 4) + 4 * r;
+        if (gj >= n) continue;
         double* c = C + (int64_t)gi + (int64_t)gj * ldc;
         const double v = alpha * acc[i][j][r];
         *c = (beta == 0.0) ? v : beta * (*c) + v;
@@ -128,22 +154,71 @@ __global__ __launch_bounds__(256) void dgemm_kernel(int m, int n, int k, double 
     }
 }
 
+// C = beta C + the sum of the nsplit partial products (each m x n, ld = m)
+__global__ __launch_bounds__(256) void dgemm_reduce_kernel(int m, int n, int nsplit, const double* __restrict__ P,
+                                                           double beta, double* __restrict__ C, int64_t ldc) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)m * n) return;
+  const int j = (int)(e / m), i = (int)(e - (int64_t)j * m);
+  double s = 0.0;
+  for (int q = 0; q < nsplit; ++q) s += P[(int64_t)q * m * n + e];
+  double* c = C + (int64_t)i + (int64_t)j * ldc;
+  *c = (beta == 0.0) ? s : beta * (*c) + s;
+}
+
+template <bool TA, bool TB, int BM, int BN, int WR>
+static hipError_t launch_tile(int m, int n, int k, int nsplit, int kchunk, double alpha, const double* A, int64_t lda,
+                              const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int64_t split_stride,
+                              hipStream_t s) {
+  dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN, nsplit);
+  hipLaunchKernelGGL((dgemm_kernel<TA, TB, BM, BN, WR>), grid, dim3(256), 0, s, m, n, k, kchunk, alpha, A, lda, B,
+                     ldb, beta, C, ldc, split_stride);
+  return hipGetLastError();
+}
+
+template <int BM, int BN, int WR>
+static hipError_t launch_shape(bool ta, bool tb, int m, int n, int k, int nsplit, int kchunk, double alpha,
+                               const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                               int64_t ldc, int64_t split_stride, hipStream_t s) {
+  if (!ta && !tb)
+    return launch_tile<false, false, BM, BN, WR>(m, n, k, nsplit, kchunk, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                 split_stride, s);
+  if (!ta && tb)
+    return launch_tile<false, true, BM, BN, WR>(m, n, k, nsplit, kchunk, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                split_stride, s);
+  if (ta && !tb)
+    return launch_tile<true, false, BM, BN, WR>(m, n, k, nsplit, kchunk, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                split_stride, s);
+  return launch_tile<true, true, BM, BN, WR>(m, n, k, nsplit, kchunk, alpha, A, lda, B, ldb, beta, C, ldc,
+                                             split_stride, s);
+}
+
 hipError_t launch_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, int64_t lda,
                         const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s) {
   if (m <= 0 || n <= 0) return hipSuccess;
-  dim3 grid((m + GBM - 1) / GBM, (n + GBN - 1) / GBN);
-  if (!ta && !tb)
-    hipLaunchKernelGGL((dgemm_kernel<false, false>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
-                       ldc);
-  else if (!ta && tb)
-    hipLaunchKernelGGL((dgemm_kernel<false, true>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
-                       ldc);
-  else if (ta && !tb)
-    hipLaunchKernelGGL((dgemm_kernel<true, false>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
-                       ldc);
-  else
-    hipLaunchKernelGGL((dgemm_kernel<true, true>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
-                       ldc);
+  if (m <= 32) return launch_shape<32, 128, 1>(ta, tb, m, n, k, 1, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, s);
+  return launch_shape<128, 128, 2>(ta, tb, m, n, k, 1, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, s);
+}
+
+static int splitk_parts(int k) { return std::min(16, std::max(1, k / 256)); }
+
+size_t dgemm_splitk_doubles(int m, int n, int k) { return (size_t)splitk_parts(k) * m * n; }
+
+hipError_t launch_dgemm_splitk(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, int64_t lda,
+                               const double* B, int64_t ldb, double beta, double* C, int64_t ldc, double* ws,
+                               hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  const int nsplit = splitk_parts(k);
+  if (nsplit == 1 || m > 32) return launch_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  // K chunks are whole slabs; every launched split gets at least one
+  const int kchunk = ((k + nsplit - 1) / nsplit + GBK - 1) / GBK * GBK;
+  const int used = (k + kchunk - 1) / kchunk;
+  hipError_t e = launch_shape<32, 128, 1>(ta, tb, m, n, k, used, kchunk, alpha, A, lda, B, ldb, 0.0, ws, m,
+                                          (int64_t)m * n, s);
+  if (e != hipSuccess) return e;
+  const int64_t tot = (int64_t)m * n;
+  hipLaunchKernelGGL(dgemm_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, n, used, ws, beta,
+                     C, ldc);
   return hipGetLastError();
 }
 

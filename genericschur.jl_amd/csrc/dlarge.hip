@@ -617,6 +617,10 @@ __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
     __syncthreads();
     auto HW = [&](int r, int c) -> double& { return win[(r - wb) + (c - wb) * LW]; };
     const LState* st = a.st;
+    // the next step's reflector input comes from the lanes that computed it
+    // in the right update (v_readlane), the first step of the chunk reads
+    // the window
+    double nv0 = 0.0, nv1 = 0.0, nv2 = 0.0;
     for (int k = ck.k0; k <= ck.kend; ++k) {
       const int nr = min(3, iend - k + 1);
       double v0, v1, v2s;
@@ -624,13 +628,17 @@ __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
         v0 = st->v0;
         v1 = st->v1;
         v2s = st->v2;
+      } else if (k > ck.k0) {
+        v0 = nv0;
+        v1 = nv1;
+        v2s = (nr == 3) ? nv2 : 0.0;
       } else {
         v0 = HW(k, k - 1);
         v1 = HW(k + 1, k - 1);
         v2s = (nr == 3) ? HW(k + 2, k - 1) : 0.0;
       }
       double x1 = v1, x2 = (nr == 3) ? v2s : 0.0;
-      const double tau1 = dreflector(nr, v0, x1, x2);
+      const double tau1 = dreflector_fast(nr, v0, x1, x2);
       wsync();
       if (lane == 0) {
         if (k > mx) {
@@ -656,13 +664,18 @@ __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
         }
         wsync();
         const int jm = min(k + 3, iend);
+        double na = 0.0;
         for (int j = wb + lane; j <= jm; j += 64) {
           const double p0 = HW(j, k), p1 = HW(j, k + 1), p2 = HW(j, k + 2);
           const double ss = p0 + vv2 * p1 + vv3 * p2;
-          HW(j, k) = p0 - ss * tau1;
+          na = p0 - ss * tau1;
+          HW(j, k) = na;
           HW(j, k + 1) = p1 - ss * tau2;
           HW(j, k + 2) = p2 - ss * tau3;
         }
+        nv0 = rdlane(na, (k + 1 - wb) & 63);
+        nv1 = rdlane(na, (k + 2 - wb) & 63);
+        nv2 = rdlane(na, (k + 3 - wb) & 63);
       } else {
         for (int j = k + lane; j <= we; j += 64) {
           const double p0 = HW(k, j), p1 = HW(k + 1, j);
@@ -745,14 +758,20 @@ hipError_t large_schur_d(const LargeArgs& a, LargeWork& w, hipStream_t s, LargeT
   const int64_t ldt = a.ldt;
   double* H = a.T;
   hipError_t e;
-#define LCHK(x)                                \
-  do {                                         \
-    if ((e = (x)) != hipSuccess) return e;     \
+#define LCHK(x)                     \
+  do {                              \
+    if ((e = (x)) != hipSuccess) {  \
+      w.where = #x;                 \
+      return e;                     \
+    }                               \
   } while (0)
-#define LLAUNCH(...)                           \
-  do {                                         \
-    hipLaunchKernelGGL(__VA_ARGS__);           \
-    LCHK(hipGetLastError());                   \
+#define LLAUNCH(k, ...)                           \
+  do {                                            \
+    hipLaunchKernelGGL(k, __VA_ARGS__);           \
+    if ((e = hipGetLastError()) != hipSuccess) {  \
+      w.where = #k;                               \
+      return e;                                   \
+    }                                             \
   } while (0)
   const bool wantz = (a.flags & 0x01) != 0;
   const bool scale = (a.flags & 0x02) != 0;
@@ -807,7 +826,7 @@ hipError_t large_schur_d(const LargeArgs& a, LargeWork& w, hipStream_t s, LargeT
                    H + (int64_t)(i + ib) * ldt, ldt));
       // left update: H(i+1:n, i+ib:n) -= V T^T (V^T H(i+1:n, i+ib:n))
       LCHK(tm_gemm(tm, s, true, false, ib, ntr, n - i - 1, 1.0, w.V + (i + 1), n,
-                   H + (i + 1) + (int64_t)(i + ib) * ldt, ldt, 0.0, w.Wb, LNB));
+                   H + (i + 1) + (int64_t)(i + ib) * ldt, ldt, 0.0, w.Wb, LNB, w.gk));
       LLAUNCH((ltrmm_left_upper_kernel<256, true>), dim3((ntr + 255) / 256), dim3(256), 0, s, w.Wb, w.W2, p.Tm, ib,
               ntr);
       LCHK(tm_gemm(tm, s, false, false, n - i - 1, ntr, ib, -1.0, w.V + (i + 1), n, w.W2, LNB, 1.0,
@@ -825,7 +844,7 @@ hipError_t large_schur_d(const LargeArgs& a, LargeWork& w, hipStream_t s, LargeT
       const int m = n - i - 1;
       LLAUNCH(lbuild_v_kernel<256>, dim3((n + 255) / 256, ib), dim3(256), 0, s, H, ldt, n, i, ib, w.V);
       double* Zs = a.Z + (i + 1) + (int64_t)(i + 1) * a.ldz;
-      LCHK(tm_gemm(tm, s, true, false, ib, m, m, 1.0, w.V + (i + 1), n, Zs, a.ldz, 0.0, w.Wb, LNB));
+      LCHK(tm_gemm(tm, s, true, false, ib, m, m, 1.0, w.V + (i + 1), n, Zs, a.ldz, 0.0, w.Wb, LNB, w.gk));
       LLAUNCH((ltrmm_left_upper_kernel<256, false>), dim3((m + 255) / 256), dim3(256), 0, s, w.Wb, w.W2, Tm, ib, m);
       LCHK(tm_gemm(tm, s, false, false, m, m, ib, -1.0, w.V + (i + 1), n, w.W2, LNB, 1.0, Zs, a.ldz));
     }
@@ -905,8 +924,16 @@ size_t large_work_doubles(int64_t n, LargeWorkSizes* sz) {
   sz->scl = 2;
   sz->lg = 3 * n + 3;
   sz->st = (sizeof(LState) + 7) / 8;
-  return (size_t)(sz->Y + sz->V + sz->Ytmp + sz->Tall + sz->Wb + sz->W2 + sz->tau + sz->tv + sz->part + sz->scl +
-                  sz->lg + sz->st);
+  sz->gk = (int64_t)dgemm_splitk_doubles(LNB, (int)n, (int)n);
+  // every segment is carved at a multiple of 32 doubles (256 bytes)
+  int64_t* segs[] = {&sz->Y, &sz->V, &sz->Ytmp, &sz->Tall, &sz->Wb, &sz->W2, &sz->tau,
+                     &sz->tv, &sz->part, &sz->scl, &sz->lg, &sz->st, &sz->gk};
+  int64_t total = 0;
+  for (int64_t* q : segs) {
+    *q = (*q + 31) / 32 * 32;
+    total += *q;
+  }
+  return (size_t)total;
 }
 
 size_t large_state_bytes() { return sizeof(LState); }

@@ -27,7 +27,10 @@ namespace gs {
 
 // ------------------------------------------------------------------------
 // hessenberg.jl:3-17 for real input
-template <int NT> GS_DEV void d_hessenberg(double* H, int64_t ld, int n, double* tau, double* red, double* wk) {
+// RM > 0: n <= 64 RM (LDS-resident matrices), so each lane keeps the
+// reflector and row factors of its RM rows in registers in the rank-1
+// updates; RM = 0: any n.
+template <int NT, int RM = 0> GS_DEV void d_hessenberg(double* H, int64_t ld, int n, double* tau, double* red, double* wk) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const double sfmin = 2.0 * GS_FMIN / GS_EPS;
@@ -97,11 +100,28 @@ template <int NT> GS_DEV void d_hessenberg(double* H, int64_t ld, int n, double*
     __syncthreads();
     for (int j = p + 1 + tid; j < n; j += NT) wk[j] = t_ * (H[(p + 1) + (int64_t)j * ld] + wk[j]);
     __syncthreads();
-    for (int j = p + 1 + wave; j < n; j += NT / 64) {
-      const double va = wk[j];
-      for (int r = p + 1 + lane; r < n; r += 64) {
-        const double h = H[r + (int64_t)j * ld];
-        H[r + (int64_t)j * ld] = (r == p + 1) ? h - va : h - va * H[r + p * ld];
+    if constexpr (RM > 0) {
+      double vr[RM];
+#pragma unroll
+      for (int q = 0; q < RM; ++q) {
+        const int r = p + 1 + lane + 64 * q;
+        vr[q] = (r < n) ? ((r == p + 1) ? 1.0 : H[r + p * ld]) : 0.0;
+      }
+      for (int j = p + 1 + wave; j < n; j += NT / 64) {
+        const double va = wk[j];
+#pragma unroll
+        for (int q = 0; q < RM; ++q) {
+          const int r = p + 1 + lane + 64 * q;
+          if (r < n) H[r + (int64_t)j * ld] -= va * vr[q];
+        }
+      }
+    } else {
+      for (int j = p + 1 + wave; j < n; j += NT / 64) {
+        const double va = wk[j];
+        for (int r = p + 1 + lane; r < n; r += 64) {
+          const double h = H[r + (int64_t)j * ld];
+          H[r + (int64_t)j * ld] = (r == p + 1) ? h - va : h - va * H[r + p * ld];
+        }
       }
     }
     __syncthreads();
@@ -116,15 +136,29 @@ template <int NT> GS_DEV void d_hessenberg(double* H, int64_t ld, int n, double*
       wk[r] = -t_ * x;
     }
     __syncthreads();
-    for (int c = r0 + wave; c < n; c += NT / 64) {
-      const double vc = H[c + p * ld];
-      for (int r = lane; r < n; r += 64) H[r + (int64_t)c * ld] += wk[r] * vc;
+    if constexpr (RM > 0) {
+      double xr[RM];
+#pragma unroll
+      for (int q = 0; q < RM; ++q) xr[q] = (lane + 64 * q < n) ? wk[lane + 64 * q] : 0.0;
+      for (int c = r0 + wave; c < n; c += NT / 64) {
+        const double vc = H[c + p * ld];
+#pragma unroll
+        for (int q = 0; q < RM; ++q) {
+          const int r = lane + 64 * q;
+          if (r < n) H[r + (int64_t)c * ld] += xr[q] * vc;
+        }
+      }
+    } else {
+      for (int c = r0 + wave; c < n; c += NT / 64) {
+        const double vc = H[c + p * ld];
+        for (int r = lane; r < n; r += 64) H[r + (int64_t)c * ld] += wk[r] * vc;
+      }
     }
     __syncthreads();
   }
 }
 
-template <int NT> GS_DEV void d_form_q_inplace(double* H, int64_t ld, int n, const double* tau, double* wk) {
+template <int NT, int RM = 0> GS_DEV void d_form_q_inplace(double* H, int64_t ld, int n, const double* tau, double* wk) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   for (int k = n - 2; k >= 0; --k) {
@@ -135,11 +169,28 @@ template <int NT> GS_DEV void d_form_q_inplace(double* H, int64_t ld, int n, con
     __syncthreads();
     for (int j = c + 1 + tid; j < n; j += NT) wk[j] = t * (H[c + (int64_t)j * ld] + wk[j]);
     __syncthreads();
-    for (int j = c + 1 + wave; j < n; j += NT / 64) {
-      const double sj = wk[j];
-      for (int r = c + lane; r < n; r += 64) {
-        const double h = H[r + (int64_t)j * ld];
-        H[r + (int64_t)j * ld] = (r == c) ? h - sj : h - sj * H[r + (int64_t)k * ld];
+    if constexpr (RM > 0) {
+      double vr[RM];
+#pragma unroll
+      for (int q = 0; q < RM; ++q) {
+        const int r = c + lane + 64 * q;
+        vr[q] = (r < n) ? ((r == c) ? 1.0 : H[r + (int64_t)k * ld]) : 0.0;
+      }
+      for (int j = c + 1 + wave; j < n; j += NT / 64) {
+        const double sj = wk[j];
+#pragma unroll
+        for (int q = 0; q < RM; ++q) {
+          const int r = c + lane + 64 * q;
+          if (r < n) H[r + (int64_t)j * ld] -= sj * vr[q];
+        }
+      }
+    } else {
+      for (int j = c + 1 + wave; j < n; j += NT / 64) {
+        const double sj = wk[j];
+        for (int r = c + lane; r < n; r += 64) {
+          const double h = H[r + (int64_t)j * ld];
+          H[r + (int64_t)j * ld] = (r == c) ? h - sj : h - sj * H[r + (int64_t)k * ld];
+        }
       }
     }
     __syncthreads();
@@ -736,14 +787,14 @@ template <int NT> __global__ void __launch_bounds__(NT) dhq_kernel(BatchArgs a) 
     a.scl[2 * m] = cscale;
     a.scl[2 * m + 1] = anrm;
   }
-  if (!hessin) d_hessenberg<NT>(H, ld, n, tau, red, wk);
+  if (!hessin) d_hessenberg<NT, 3>(H, ld, n, tau, red, wk);
   for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
     const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
     T[e] = (r > c + 1) ? 0.0 : H[r + c * ld];
   }
   if (wantz && !hessin) {
     __syncthreads();
-    d_form_q_inplace<NT>(H, ld, n, tau, wk);
+    d_form_q_inplace<NT, 3>(H, ld, n, tau, wk);
     for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
       const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
       Z[e] = H[r + c * ld];

@@ -464,7 +464,7 @@ static hipError_t run_large(gs_handle* h, int64_t n, double* dA, int64_t lda, do
                             int64_t ldz, double* dW, int32_t* dInfo, int kflags, int64_t maxiter, bool timing) {
   LargeWorkSizes sz;
   const size_t nd = large_work_doubles(n, &sz);
-  hipError_t e = h->lwork.ensure(8 * nd + 256);
+  hipError_t e = h->lwork.ensure(8 * nd);
   if (e != hipSuccess) return e;
   if (!h->lhost) {
     e = hipHostMalloc(&h->lhost, 256, hipHostMallocDefault);
@@ -475,9 +475,11 @@ static hipError_t run_large(gs_handle* h, int64_t n, double* dA, int64_t lda, do
   }
   double* base = static_cast<double*>(h->lwork.p);
   LargeWork w;
+  // segment sizes are multiples of 32 doubles and sum to nd
+  double* const end = base + nd;
   auto carve = [&](int64_t cnt) {
     double* q = base;
-    base += (cnt + 31) / 32 * 32;
+    base += cnt;
     return q;
   };
   w.Y = carve(sz.Y);
@@ -492,6 +494,8 @@ static hipError_t run_large(gs_handle* h, int64_t n, double* dA, int64_t lda, do
   w.scl = carve(sz.scl);
   w.lg = carve(sz.lg);
   w.st = carve(sz.st);
+  w.gk = carve(sz.gk);
+  if (base != end) return hipErrorInvalidValue;
   w.host_state = h->lhost;
   LargeArgs a;
   a.n = n;
@@ -509,7 +513,11 @@ static hipError_t run_large(gs_handle* h, int64_t n, double* dA, int64_t lda, do
   LargeTimer tm;
   tm.pool = &h->tev;
   e = large_schur_d(a, w, h->stream, timing ? &tm : nullptr);
-  if (e != hipSuccess || !timing) return e;
+  if (e != hipSuccess) {
+    h->large_where = w.where;
+    return e;
+  }
+  if (!timing) return e;
   if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return e;
   for (const LargeTimer::Rec& r : tm.recs) {
     float ms = 0.f;
@@ -590,7 +598,10 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   hipError_t le;
   if (use_large) {
+    h->large_where.clear();
     le = run_large(h, n, dA, dlda, dT, dldt, dZ, dldz, dW, dInfo, kflags, maxiter, (flags & GS_TIMING) != 0);
+    if (le != hipSuccess)
+      return fail_hip(h, le, h->large_where.empty() ? "large path" : ("large path: " + h->large_where).c_str());
   } else if (use_lds) {
     HIPCHK(h, h->bMeta.ensure(64));
     // metadata for a batch of one: order = 0, n, off = 0, woff = 0

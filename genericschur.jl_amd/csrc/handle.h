@@ -62,6 +62,7 @@ struct gs_handle {
   // workspace of the large-matrix real path and its pinned state record
   gs::DevBuf lwork;
   void* lhost = nullptr;
+  std::string large_where;
   std::mutex mu;
 };
 

@@ -113,6 +113,12 @@ hipError_t launch_vjp_assemble(const VjpArgs& a, int nmax, hipStream_t s);
 // FP64 MFMA GEMM: C(m x n) = beta C + alpha op(A) op(B), column-major.
 hipError_t launch_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, int64_t lda,
                         const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s);
+// the same product split over K into partial products in ws (for m <= 32
+// and long k), then summed into C; ws holds dgemm_splitk_doubles(m, n, k)
+size_t dgemm_splitk_doubles(int m, int n, int k);
+hipError_t launch_dgemm_splitk(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, int64_t lda,
+                               const double* B, int64_t ldb, double beta, double* C, int64_t ldc, double* ws,
+                               hipStream_t s);
 
 // ---------------------------------------------------------- large real path
 // One real matrix of any order on the whole GPU: blocked Hessenberg with
@@ -133,14 +139,16 @@ struct LargeArgs {
 };
 
 struct LargeWorkSizes {
-  int64_t Y, V, Ytmp, Tall, Wb, W2, tau, tv, part, scl, lg, st;
+  int64_t Y, V, Ytmp, Tall, Wb, W2, tau, tv, part, scl, lg, st, gk;
 };
 
 // device workspace carved from one allocation, plus a pinned host record
 struct LargeWork {
   double *Y, *V, *Ytmp, *Tall, *Wb, *W2, *tau, *tv, *part, *scl, *lg;
   void* st;
+  double* gk;  // split-K partial products
   void* host_state;
+  const char* where = "";  // the call that failed, for the error message
 };
 
 // events around phases and GEMM launches (only when timing is requested)
@@ -170,12 +178,14 @@ struct LargeTimer {
 
 inline hipError_t tm_gemm(LargeTimer* tm, hipStream_t s, bool ta, bool tb, int m, int n, int k, double alpha,
                           const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
-                          int64_t ldc) {
+                          int64_t ldc, double* splitk_ws = nullptr) {
   if (m <= 0 || n <= 0) return hipSuccess;
   size_t e0 = 0, e1 = 0;
   hipError_t r;
   if (tm && (r = tm->ev(s, e0)) != hipSuccess) return r;
-  if ((r = launch_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s)) != hipSuccess) return r;
+  r = splitk_ws ? launch_dgemm_splitk(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, splitk_ws, s)
+                : launch_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  if (r != hipSuccess) return r;
   if (tm) {
     if ((r = tm->ev(s, e1)) != hipSuccess) return r;
     tm->recs.push_back({2, m, n, k, 2.0 * m * n * (double)k, e0, e1});

@@ -34,7 +34,10 @@ namespace gs {
 // ------------------------------------------------------------------------
 // Hessenberg reduction in place (hessenberg.jl:3-17), full layout, 0-based.
 // Reflector p is stored below the subdiagonal of column p, tau[p] its scalar.
-template <int NT> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n, zd* tau, double* red, zd* wk) {
+// RM > 0: n <= 64 RM (LDS-resident matrices), so each lane keeps the
+// reflector and row factors of its RM rows in registers in the rank-1
+// updates; RM = 0: any n.
+template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n, zd* tau, double* red, zd* wk) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const double sfmin = GS_FMIN / GS_EPS;
@@ -115,11 +118,28 @@ template <int NT> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n, zd* tau, do
     __syncthreads();
     for (int j = p + 1 + tid; j < n; j += NT) wk[j] = ct * (H[(p + 1) + (int64_t)j * ld] + wk[j]);
     __syncthreads();
-    for (int j = p + 1 + wave; j < n; j += NT / 64) {
-      const zd va = wk[j];
-      for (int r = p + 1 + lane; r < n; r += 64) {
-        const zd h = H[r + (int64_t)j * ld];
-        H[r + (int64_t)j * ld] = (r == p + 1) ? h - va : h - va * H[r + p * ld];
+    if constexpr (RM > 0) {
+      zd vr[RM];
+#pragma unroll
+      for (int q = 0; q < RM; ++q) {
+        const int r = p + 1 + lane + 64 * q;
+        vr[q] = (r < n) ? ((r == p + 1) ? zr(1.0) : H[r + p * ld]) : zr(0.0);
+      }
+      for (int j = p + 1 + wave; j < n; j += NT / 64) {
+        const zd va = wk[j];
+#pragma unroll
+        for (int q = 0; q < RM; ++q) {
+          const int r = p + 1 + lane + 64 * q;
+          if (r < n) H[r + (int64_t)j * ld] = H[r + (int64_t)j * ld] - va * vr[q];
+        }
+      }
+    } else {
+      for (int j = p + 1 + wave; j < n; j += NT / 64) {
+        const zd va = wk[j];
+        for (int r = p + 1 + lane; r < n; r += 64) {
+          const zd h = H[r + (int64_t)j * ld];
+          H[r + (int64_t)j * ld] = (r == p + 1) ? h - va : h - va * H[r + p * ld];
+        }
       }
     }
     __syncthreads();
@@ -135,9 +155,23 @@ template <int NT> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n, zd* tau, do
       wk[r] = x * mt;
     }
     __syncthreads();
-    for (int c = r0 + wave; c < n; c += NT / 64) {
-      const zd vc = zconj(H[c + p * ld]);
-      for (int r = lane; r < n; r += 64) H[r + (int64_t)c * ld] = H[r + (int64_t)c * ld] + wk[r] * vc;
+    if constexpr (RM > 0) {
+      zd xr[RM];
+#pragma unroll
+      for (int q = 0; q < RM; ++q) xr[q] = (lane + 64 * q < n) ? wk[lane + 64 * q] : zr(0.0);
+      for (int c = r0 + wave; c < n; c += NT / 64) {
+        const zd vc = zconj(H[c + p * ld]);
+#pragma unroll
+        for (int q = 0; q < RM; ++q) {
+          const int r = lane + 64 * q;
+          if (r < n) H[r + (int64_t)c * ld] = H[r + (int64_t)c * ld] + xr[q] * vc;
+        }
+      }
+    } else {
+      for (int c = r0 + wave; c < n; c += NT / 64) {
+        const zd vc = zconj(H[c + p * ld]);
+        for (int r = lane; r < n; r += 64) H[r + (int64_t)c * ld] = H[r + (int64_t)c * ld] + wk[r] * vc;
+      }
     }
     __syncthreads();
   }
@@ -147,7 +181,7 @@ template <int NT> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n, zd* tau, do
 // Q = diag(1, H_0 H_1 ... H_{n-2}) formed in place over the reflector
 // storage (hessenberg.jl:150-166; the unblocked backward accumulation of
 // xUNG2R on the reflectors shifted one column right).
-template <int NT> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, int n, const zd* tau, zd* wk) {
+template <int NT, int RM = 0> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, int n, const zd* tau, zd* wk) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   for (int k = n - 2; k >= 0; --k) {
@@ -160,11 +194,28 @@ template <int NT> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, int n, const z
     __syncthreads();
     for (int j = c + 1 + tid; j < n; j += NT) wk[j] = t * (H[c + (int64_t)j * ld] + wk[j]);
     __syncthreads();
-    for (int j = c + 1 + wave; j < n; j += NT / 64) {
-      const zd sj = wk[j];
-      for (int r = c + lane; r < n; r += 64) {
-        const zd h = H[r + (int64_t)j * ld];
-        H[r + (int64_t)j * ld] = (r == c) ? h - sj : h - sj * H[r + (int64_t)k * ld];
+    if constexpr (RM > 0) {
+      zd vr[RM];
+#pragma unroll
+      for (int q = 0; q < RM; ++q) {
+        const int r = c + lane + 64 * q;
+        vr[q] = (r < n) ? ((r == c) ? zr(1.0) : H[r + (int64_t)k * ld]) : zr(0.0);
+      }
+      for (int j = c + 1 + wave; j < n; j += NT / 64) {
+        const zd sj = wk[j];
+#pragma unroll
+        for (int q = 0; q < RM; ++q) {
+          const int r = c + lane + 64 * q;
+          if (r < n) H[r + (int64_t)j * ld] = H[r + (int64_t)j * ld] - sj * vr[q];
+        }
+      }
+    } else {
+      for (int j = c + 1 + wave; j < n; j += NT / 64) {
+        const zd sj = wk[j];
+        for (int r = c + lane; r < n; r += 64) {
+          const zd h = H[r + (int64_t)j * ld];
+          H[r + (int64_t)j * ld] = (r == c) ? h - sj : h - sj * H[r + (int64_t)k * ld];
+        }
       }
     }
     __syncthreads();
@@ -787,14 +838,14 @@ template <int NT> __global__ void __launch_bounds__(NT) zhq_kernel(BatchArgs a) 
     a.scl[2 * m + 1] = anrm;
   }
   if (st != 0) return;
-  if (!hessin) z_hessenberg<NT>(H, ld, n, tau, red, wk);
+  if (!hessin) z_hessenberg<NT, 2>(H, ld, n, tau, red, wk);
   for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
     const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
     T[e] = (r > c + 1) ? zr(0.0) : H[r + c * ld];
   }
   if (wantz && !hessin) {
     __syncthreads();
-    z_form_q_inplace<NT>(H, ld, n, tau, wk);
+    z_form_q_inplace<NT, 2>(H, ld, n, tau, wk);
     for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
       const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
       Z[e] = H[r + c * ld];
