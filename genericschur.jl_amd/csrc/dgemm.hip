@@ -9,9 +9,12 @@
 // 16 staged through LDS (k-major, padded); the next slab's global loads are
 // issued before the current slab's MFMAs.
 //
-//   128 x 128 tiles (WR = 2) for the trailing updates (large m and n, k = 32);
+//   64 x 64 tiles (WR = 2) for the trailing updates (large m and n, k = 32);
+//   128 x 128 tiles (WR = 2) for large m, n and k;
 //   32 x 128 tiles (WR = 1), split over K, for the W = V^T C products
-//   (m = 32, long k) of the left update and of the Q formation.
+//   (m = 32, long k) of the left update and of the Q formation;
+//   128 x 32 tiles (WR = 4), split over K, for Y's top rows A V (n = 32,
+//   long k).
 //
 // Operand layout of v_mfma_f64_16x16x4_f64 (lane l, 0 <= l < 64):
 //   A (16 x 4):  lane l supplies A[l % 16][l / 16]
@@ -193,11 +196,30 @@ static hipError_t launch_shape(bool ta, bool tb, int m, int n, int k, int nsplit
                                              split_stride, s);
 }
 
+// the tile shape for an m x n product: 32 x 128 for m <= 32, 128 x 32 for
+// n <= 32, else 128 x 128
+static hipError_t launch_any(bool ta, bool tb, int m, int n, int k, int nsplit, int kchunk, double alpha,
+                             const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                             int64_t ldc, int64_t split_stride, hipStream_t s) {
+  if (m <= 32)
+    return launch_shape<32, 128, 1>(ta, tb, m, n, k, nsplit, kchunk, alpha, A, lda, B, ldb, beta, C, ldc,
+                                    split_stride, s);
+  if (n <= 32)
+    return launch_shape<128, 32, 4>(ta, tb, m, n, k, nsplit, kchunk, alpha, A, lda, B, ldb, beta, C, ldc,
+                                    split_stride, s);
+  // short K (the rank-32 trailing updates): the C read-modify-write is the
+  // cost, so smaller tiles and more resident workgroups per CU
+  if (k <= 64)
+    return launch_shape<64, 64, 2>(ta, tb, m, n, k, nsplit, kchunk, alpha, A, lda, B, ldb, beta, C, ldc,
+                                   split_stride, s);
+  return launch_shape<128, 128, 2>(ta, tb, m, n, k, nsplit, kchunk, alpha, A, lda, B, ldb, beta, C, ldc,
+                                   split_stride, s);
+}
+
 hipError_t launch_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, int64_t lda,
                         const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s) {
   if (m <= 0 || n <= 0) return hipSuccess;
-  if (m <= 32) return launch_shape<32, 128, 1>(ta, tb, m, n, k, 1, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, s);
-  return launch_shape<128, 128, 2>(ta, tb, m, n, k, 1, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, s);
+  return launch_any(ta, tb, m, n, k, 1, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, s);
 }
 
 static int splitk_parts(int k) { return std::min(16, std::max(1, k / 256)); }
@@ -209,12 +231,11 @@ hipError_t launch_dgemm_splitk(bool ta, bool tb, int m, int n, int k, double alp
                                hipStream_t s) {
   if (m <= 0 || n <= 0) return hipSuccess;
   const int nsplit = splitk_parts(k);
-  if (nsplit == 1 || m > 32) return launch_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  if (nsplit == 1 || (m > 32 && n > 32)) return launch_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
   // K chunks are whole slabs; every launched split gets at least one
   const int kchunk = ((k + nsplit - 1) / nsplit + GBK - 1) / GBK * GBK;
   const int used = (k + kchunk - 1) / kchunk;
-  hipError_t e = launch_shape<32, 128, 1>(ta, tb, m, n, k, used, kchunk, alpha, A, lda, B, ldb, 0.0, ws, m,
-                                          (int64_t)m * n, s);
+  hipError_t e = launch_any(ta, tb, m, n, k, used, kchunk, alpha, A, lda, B, ldb, 0.0, ws, m, (int64_t)m * n, s);
   if (e != hipSuccess) return e;
   const int64_t tot = (int64_t)m * n;
   hipLaunchKernelGGL(dgemm_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, n, used, ws, beta,

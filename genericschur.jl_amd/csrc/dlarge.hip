@@ -812,7 +812,7 @@ hipError_t large_schur_d(const LargeArgs& a, LargeWork& w, hipStream_t s, LargeT
     }
     // rows 0..i of Y: Y(0:i+1, :) = H(0:i+1, i+1:n) V(i+1:n, :) T
     LCHK(tm_gemm(tm, s, false, false, i + 1, ib, n - i - 1, 1.0, H + (int64_t)(i + 1) * ldt, ldt, w.V + (i + 1), n,
-                 0.0, w.Ytmp, n));
+                 0.0, w.Ytmp, n, w.gk));
     LLAUNCH(ltrmm_right_upper_kernel<256>, dim3((i + 1 + 255) / 256), dim3(256), 0, s, w.Ytmp, (int64_t)n, w.Y,
             (int64_t)n, p.Tm, i + 1, ib);
     // rows 0..i of the panel's columns i+1 .. i+ib-1

@@ -22,6 +22,8 @@
 // reflectors) redundantly on values read from H; searches are parallel over
 // threads followed by a butterfly maximum, so all threads take the same
 // branches.
+#include <cstdlib>
+
 #include "dev_common.h"
 #include "internal.h"
 #include "layout.h"
@@ -702,10 +704,200 @@ template <bool FAST> GS_DEV void z_sweep_win(zd* H, int64_t ld, int n, zd* Zg, i
   __syncthreads();
 }
 
+// Two-wavefront form of z_sweep_win: wavefront 0 chases chunk c (after
+// applying chunk c-1's left update to the columns its window adds) while
+// wavefront 1 applies chunk c-1's chains to the rest of H and to Z; the two
+// regions are disjoint and every element still receives its updates in the
+// order of the unblocked sweep.  The logs are double-buffered.  When the
+// phase scaling is due, the first step's chains are applied before it, as in
+// z_sweep_win.  Called by all 128 threads.
+template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, int64_t ldz, bool wantz, zd* win,
+                                             zd* lgbuf, int* redi, zd shift, int istart, int iend, int64_t& nstep) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double ulp = GS_EPS;
+  const FullLayout lay{ld};
+  int found = -1;
+  for (int mm = iend - 1 - tid; mm >= istart + 1; mm -= 128) {
+    const zd h11 = HH(mm, mm);
+    const zd h22 = HH(mm + 1, mm + 1);
+    zd h11s = h11 - shift;
+    double h21 = HH(mm + 1, mm).x;
+    const double s = zabs1(h11s) + fabs(h21);
+    h11s = h11s / s;
+    h21 = h21 / s;
+    const double h10 = HH(mm, mm - 1).x;
+    if (fabs(h10) * fabs(h21) <= ulp * (zabs1(h11s) * (zabs1(h11) + zabs1(h22)))) {
+      found = mm;
+      break;
+    }
+  }
+  found = block_max_int<128>(found, redi);
+  const int istart1 = (found > 0) ? found : istart;
+  zd v0, v1;
+  {
+    const zd h11 = HH(istart1, istart1);
+    zd h11s = h11 - shift;
+    double h21 = HH(istart1 + 1, istart1).x;
+    const double s = zabs1(h11s) + fabs(h21);
+    h11s = h11s / s;
+    h21 = h21 / s;
+    v0 = h11s;
+    v1 = zr(h21);
+  }
+  bool phase_pending = istart1 > istart;
+  int pk0 = 1, pkend = 0, pwe = 0, pwb = 1;  // chunk whose chains are pending (none)
+  int k0 = istart1;
+  for (int c = 0;; ++c) {
+    const bool chase = k0 <= iend - 1;
+    const bool prev = pk0 <= pkend;
+    if (!chase && !prev) break;
+    const int kend = chase ? (phase_pending ? k0 : min(k0 + ZKC - 1, iend - 1)) : 0;
+    const int wb = chase ? max(k0 - 1, 1) : 0, we = chase ? min(kend + 2, iend) : 0;
+    zd* lg = lgbuf + (c & 1) * 2 * ZKC;
+    const zd* plg = lgbuf + ((c + 1) & 1) * 2 * ZKC;
+    if (wave == 0) {
+      if (chase) {
+        if (prev) {
+          const int col = pwe + 1 + lane;
+          if (col <= we) zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend);
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+          __builtin_amdgcn_wave_barrier();
+        }
+        const int W = we - wb + 1;
+        auto WIN = [&](int r, int cc) -> zd& { return win[(r - wb) + (cc - wb) * ZW]; };
+        for (int cc = 0; cc < W; ++cc)
+          if (lane < W) win[lane + cc * ZW] = H[(wb - 1 + lane) + (int64_t)(wb - 1 + cc) * ld];
+        wsync();
+        zd nv0 = zr(0.0), nv1 = zr(0.0);
+        for (int k = k0; k <= kend; ++k) {
+          if (k > istart1) {
+            if (k > k0) {
+              v0 = nv0;
+              v1 = nv1;
+            } else {
+              v0 = WIN(k, k - 1);
+              v1 = WIN(k + 1, k - 1);
+            }
+          }
+          const zd tau1 = FAST ? zreflector2_fast(v0, v1) : zreflector2(v0, v1);
+          wsync();
+          if (k > istart1 && lane == 0) {
+            WIN(k, k - 1) = v0;
+            WIN(k + 1, k - 1) = zr(0.0);
+          }
+          const zd v2 = v1;
+          const double tau2 = (tau1 * v2).x;
+          const zd ct1 = zconj(tau1);
+          {
+            const int j = k + lane;
+            if (j <= we) {
+              const zd a = WIN(k, j), b = WIN(k + 1, j);
+              const zd ss = ct1 * a + tau2 * b;
+              WIN(k, j) = a - ss;
+              WIN(k + 1, j) = b - ss * v2;
+            }
+          }
+          wsync();
+          {
+            const int jm = min(k + 2, iend);
+            const zd cv2 = zconj(v2);
+            const int j = wb + lane;
+            zd na = zr(0.0);
+            if (j <= jm) {
+              const zd a = WIN(j, k), b = WIN(j, k + 1);
+              const zd ss = tau1 * a + tau2 * b;
+              na = a - ss;
+              WIN(j, k) = na;
+              WIN(j, k + 1) = b - ss * cv2;
+            }
+            const int l0 = (k + 1) - wb, l1 = (k + 2) - wb;
+            nv0 = zmk(rdlane(na.x, l0 & 63), rdlane(na.y, l0 & 63));
+            nv1 = zmk(rdlane(na.x, l1 & 63), rdlane(na.y, l1 & 63));
+          }
+          if (lane == 0) {
+            lg[2 * (k - k0)] = tau1;
+            lg[2 * (k - k0) + 1] = v2;
+          }
+          wsync();
+        }
+        for (int cc = 0; cc < W; ++cc)
+          if (lane < W) H[(wb - 1 + lane) + (int64_t)(wb - 1 + cc) * ld] = win[lane + cc * ZW];
+      }
+    } else if (prev) {
+      const int lcol0 = (chase ? we : pwe) + 1;
+      for (int col = lcol0 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend);
+      for (int row = 1 + lane; row <= pwb - 1; row += 64) zchain<false>(H + (row - 1), ld, plg, pk0, pkend);
+      if (wantz)
+        for (int row = 1 + lane; row <= n; row += 64) zchain<false>(Zg + (row - 1), ldz, plg, pk0, pkend);
+    }
+    __syncthreads();
+    if (!chase) break;
+    if (phase_pending) {
+      // the first step's chains, then the phase scaling (GenericSchur.jl:461-482)
+      phase_pending = false;
+      if (wave == 1) {
+        for (int col = we + 1 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, lg, k0, kend);
+        for (int row = 1 + lane; row <= wb - 1; row += 64) zchain<false>(H + (row - 1), ld, lg, k0, kend);
+        if (wantz)
+          for (int row = 1 + lane; row <= n; row += 64) zchain<false>(Zg + (row - 1), ldz, lg, k0, kend);
+      }
+      __syncthreads();
+      zd t = zr(1.0) - lg[0];
+      t = t / zabs(t);
+      const zd ctt = zconj(t);
+      if (tid == 0) {
+        HH(istart1 + 1, istart1) = HH(istart1 + 1, istart1) * ctt;
+        if (istart1 + 2 <= iend) HH(istart1 + 2, istart1 + 1) = HH(istart1 + 2, istart1 + 1) * t;
+      }
+      __syncthreads();
+      for (int cc = 1; cc <= n; ++cc) {
+        const bool cin = (cc >= istart1 && cc <= iend && cc != istart1 + 1);
+        const int rmax = min(cc - 1, iend);
+        for (int r = 1 + tid; r <= rmax; r += 128) {
+          const bool rin = (r >= istart1 && r != istart1 + 1);
+          if (!rin && !cin) continue;
+          zd x = HH(r, cc);
+          if (rin) x = x * t;
+          if (cin) x = x * ctt;
+          HH(r, cc) = x;
+        }
+      }
+      if (wantz)
+        for (int r = 1 + tid; r <= n; r += 128)
+          for (int cc = istart1; cc <= iend; ++cc)
+            if (cc != istart1 + 1) ZZ(r, cc) = ZZ(r, cc) * ctt;
+      __syncthreads();
+      pk0 = 1;
+      pkend = 0;
+    } else {
+      pk0 = k0;
+      pkend = kend;
+      pwe = we;
+      pwb = wb;
+    }
+    k0 = kend + 1;
+  }
+  nstep += iend - istart1;
+  // tail realness (GenericSchur.jl:486-500)
+  zd t = HH(iend, iend - 1);
+  if (t.y != 0.0) {
+    const double rtv = zabs(t);
+    t = t / rtv;
+    __syncthreads();
+    if (tid == 0) HH(iend, iend - 1) = zr(rtv);
+    const zd ctt = zconj(t);
+    for (int cc = iend + 1 + tid; cc <= n; cc += 128) HH(iend, cc) = HH(iend, cc) * ctt;
+    for (int r = 1 + tid; r <= iend - 1; r += 128) HH(r, iend) = HH(r, iend) * t;
+    if (wantz)
+      for (int r = 1 + tid; r <= n; r += 128) ZZ(r, iend) = ZZ(r, iend) * t;
+  }
+  __syncthreads();
+}
+
 // ------------------------------------------------------------------------
 // gschur!(H::Hessenberg{Complex}, Z) (GenericSchur.jl:194-335).
 // Returns 0, or the positive iteration count when maxiter was exceeded.
-template <int NT, class L, bool FAST, bool WIN = false> GS_DEV int64_t z_qr(zd* H, L lay, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot,
+template <int NT, class L, bool FAST, int WIN = 0> GS_DEV int64_t z_qr(zd* H, L lay, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot,
                                                int* redi, int64_t maxiter, int64_t maxinner, int64_t& nsweep,
                                                int64_t& nstep) {
   const int tid = threadIdx.x;
@@ -780,7 +972,9 @@ template <int NT, class L, bool FAST, bool WIN = false> GS_DEV int64_t z_qr(zd* 
         }
       }
       nsweep += 1;
-      if constexpr (WIN)
+      if constexpr (WIN == 2)
+        z_sweep_win2<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + ZW * ZW, redi, t, istart, iend, nstep);
+      else if constexpr (WIN == 1)
         z_sweep_win<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + ZW * ZW, redi, t, istart, iend, nstep);
       else
         z_sweep<NT, L, FAST>(H, lay, n, Zg, ldz, wantz, rot, redi, t, istart, iend, nstep);
@@ -917,7 +1111,7 @@ template <bool FAST> __global__ void __launch_bounds__(64) zqrw_kernel(BatchArgs
   int64_t maxinner = a.maxinner > 0 ? a.maxinner : 30 * (int64_t)n;
   int64_t nsweep = 0, nstep = 0;
   const int64_t st =
-      z_qr<64, FullLayout, FAST, true>(T, lay, n, Z, (int64_t)n, wantz, win, redi, maxiter, maxinner, nsweep, nstep);
+      z_qr<64, FullLayout, FAST, 1>(T, lay, n, Z, (int64_t)n, wantz, win, redi, maxiter, maxinner, nsweep, nstep);
   __syncthreads();
   if (threadIdx.x == 0) {
     if (a.stats) {
@@ -928,6 +1122,40 @@ template <bool FAST> __global__ void __launch_bounds__(64) zqrw_kernel(BatchArgs
   }
   if (st != 0) return;
   z_finish<64>(T, lay, n, T, (int64_t)n, W, a.scl[2 * m], a.scl[2 * m + 1]);
+}
+
+// Two-wavefront windowed QR kernel: the chase of chunk c overlaps the chains
+// of chunk c-1 (z_sweep_win2); the control logic runs on both wavefronts.
+template <bool FAST> __global__ void __launch_bounds__(128) zqrw2_kernel(BatchArgs a) {
+  __shared__ zd win[ZW * ZW + 4 * ZKC];
+  __shared__ int redi[4];
+  const int b = blockIdx.x;
+  if (b >= a.count) return;
+  const int m = a.order[a.first + b];
+  const int n = a.n[m];
+  if (n <= 0) return;
+  if (a.info[m] != 0) return;
+  const int64_t o = a.off[m];
+  zd* T = reinterpret_cast<zd*>(a.T) + o;
+  zd* Z = reinterpret_cast<zd*>(a.Z) + o;
+  zd* W = reinterpret_cast<zd*>(a.W) + a.woff[m];
+  const bool wantz = (a.flags & 0x01) != 0;
+  const FullLayout lay{(int64_t)n};
+  int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
+  int64_t maxinner = a.maxinner > 0 ? a.maxinner : 30 * (int64_t)n;
+  int64_t nsweep = 0, nstep = 0;
+  const int64_t st =
+      z_qr<128, FullLayout, FAST, 2>(T, lay, n, Z, (int64_t)n, wantz, win, redi, maxiter, maxinner, nsweep, nstep);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (a.stats) {
+      a.stats[2 * m] = nsweep;
+      a.stats[2 * m + 1] = nstep;
+    }
+    a.info[m] = (int32_t)(st > 0x7fffffff ? 0x7fffffff : st);
+  }
+  if (st != 0) return;
+  z_finish<128>(T, lay, n, T, (int64_t)n, W, a.scl[2 * m], a.scl[2 * m + 1]);
 }
 
 // ------------------------------------------------------------------------
@@ -1050,10 +1278,22 @@ hipError_t launch_zqr(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
 }
 
 hipError_t launch_zqrw(const BatchArgs& a, hipStream_t s) {
-  if (a.flags & 0x1000)
-    hipLaunchKernelGGL((zqrw_kernel<false>), dim3(a.count), dim3(64), 0, s, a);
-  else
-    hipLaunchKernelGGL((zqrw_kernel<true>), dim3(a.count), dim3(64), 0, s, a);
+  // GSCHUR_QR_ONEWAVE=1: the single-wavefront windowed kernel (A/B)
+  static const bool one = [] {
+    const char* e = std::getenv("GSCHUR_QR_ONEWAVE");
+    return e && e[0] == '1';
+  }();
+  if (one) {
+    if (a.flags & 0x1000)
+      hipLaunchKernelGGL((zqrw_kernel<false>), dim3(a.count), dim3(64), 0, s, a);
+    else
+      hipLaunchKernelGGL((zqrw_kernel<true>), dim3(a.count), dim3(64), 0, s, a);
+  } else {
+    if (a.flags & 0x1000)
+      hipLaunchKernelGGL((zqrw2_kernel<false>), dim3(a.count), dim3(128), 0, s, a);
+    else
+      hipLaunchKernelGGL((zqrw2_kernel<true>), dim3(a.count), dim3(128), 0, s, a);
+  }
   return hipGetLastError();
 }
 

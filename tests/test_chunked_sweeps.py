@@ -550,3 +550,142 @@ def _real_case(c, n):
     istart = c[1] if c[1] > 0 else n + c[1]
     iend = n - c[2] if c[2] >= 0 else n + c[2]
     return mx, istart, iend
+
+
+def sweep_win2(H, Z, istart1, istart, iend, v0, v1, ZW):
+    """z_sweep_win2: wavefront 0 chases chunk c (after the previous chunk's
+    left update of the columns its window adds) while wavefront 1 applies the
+    previous chunk's chains; the writes of the two must be disjoint."""
+    n = H.shape[0]
+    ZKC = ZW - 3
+    h = lambda r, c: (r - 1, c - 1)
+    phase_pending = istart1 > istart
+    pk0, pkend, pwe, pwb = 1, 0, 0, 1
+    plg = None
+    k0 = istart1
+    while True:
+        chase = k0 <= iend - 1
+        prev = pk0 <= pkend
+        if not chase and not prev:
+            break
+        w0, w1 = set(), set()
+        if chase:
+            kend = k0 if phase_pending else min(k0 + ZKC - 1, iend - 1)
+            wb, we = max(k0 - 1, 1), min(kend + 2, iend)
+            if prev:
+                for col in range(pwe + 1, we + 1):
+                    zchain(lambda t: H[t - 1, col - 1],
+                           lambda t, v: (H.__setitem__((t - 1, col - 1), v), w0.add((t, col))), plg, pk0, pkend, True)
+            W = we - wb + 1
+            assert W <= ZW
+            win = H[wb - 1:we, wb - 1:we].copy()
+            WI = lambda r, c: (r - wb, c - wb)
+            lg = []
+            nv0 = nv1 = 0
+            for k in range(k0, kend + 1):
+                if k > istart1:
+                    if k > k0:
+                        v0, v1 = nv0, nv1
+                    else:
+                        v0, v1 = win[WI(k, k - 1)], win[WI(k + 1, k - 1)]
+                tau1, v0, v1 = zrefl(v0, v1)
+                if k > istart1:
+                    win[WI(k, k - 1)] = v0
+                    win[WI(k + 1, k - 1)] = 0
+                v2 = v1
+                tau2 = (tau1 * v2).real
+                for j in range(k, we + 1):
+                    a, b = win[WI(k, j)], win[WI(k + 1, j)]
+                    ss = np.conj(tau1) * a + tau2 * b
+                    win[WI(k, j)] = a - ss
+                    win[WI(k + 1, j)] = b - ss * v2
+                na = {}
+                for j in range(wb, min(k + 2, iend) + 1):
+                    a, b = win[WI(j, k)], win[WI(j, k + 1)]
+                    ss = tau1 * a + tau2 * b
+                    win[WI(j, k)] = a - ss
+                    win[WI(j, k + 1)] = b - ss * np.conj(v2)
+                    na[j] = a - ss
+                nv0, nv1 = na.get(k + 1, 0), na.get(k + 2, 0)
+                lg += [tau1, v2]
+            H[wb - 1:we, wb - 1:we] = win
+            w0.update((r, c) for r in range(wb, we + 1) for c in range(wb, we + 1))
+        if prev:
+            lcol0 = (we if chase else pwe) + 1
+            for col in range(lcol0, n + 1):
+                zchain(lambda t: H[t - 1, col - 1],
+                       lambda t, v: (H.__setitem__((t - 1, col - 1), v), w1.add((t, col))), plg, pk0, pkend, True)
+            for row in range(1, pwb):
+                zchain(lambda t: H[row - 1, t - 1],
+                       lambda t, v: (H.__setitem__((row - 1, t - 1), v), w1.add((row, t))), plg, pk0, pkend, False)
+            for row in range(1, n + 1):
+                zchain(lambda t: Z[row - 1, t - 1], lambda t, v: Z.__setitem__((row - 1, t - 1), v), plg, pk0, pkend,
+                       False)
+        assert not (w0 & w1), sorted(w0 & w1)[:5]
+        if not chase:
+            break
+        if phase_pending:
+            phase_pending = False
+            for col in range(we + 1, n + 1):
+                zchain(lambda t: H[t - 1, col - 1], lambda t, v: H.__setitem__((t - 1, col - 1), v), lg, k0, kend, True)
+            for row in range(1, wb):
+                zchain(lambda t: H[row - 1, t - 1], lambda t, v: H.__setitem__((row - 1, t - 1), v), lg, k0, kend,
+                       False)
+            for row in range(1, n + 1):
+                zchain(lambda t: Z[row - 1, t - 1], lambda t, v: Z.__setitem__((row - 1, t - 1), v), lg, k0, kend,
+                       False)
+            t = 1 - lg[0]
+            t = t / abs(t)
+            ctt = np.conj(t)
+            H[h(istart1 + 1, istart1)] *= ctt
+            if istart1 + 2 <= iend:
+                H[h(istart1 + 2, istart1 + 1)] *= t
+            for c in range(1, n + 1):
+                cin = istart1 <= c <= iend and c != istart1 + 1
+                for r in range(1, min(c - 1, iend) + 1):
+                    rin = r >= istart1 and r != istart1 + 1
+                    if not rin and not cin:
+                        continue
+                    x = H[h(r, c)]
+                    if rin:
+                        x = x * t
+                    if cin:
+                        x = x * ctt
+                    H[h(r, c)] = x
+            for c in range(istart1, iend + 1):
+                if c != istart1 + 1:
+                    Z[:, c - 1] *= ctt
+            pk0, pkend = 1, 0
+        else:
+            pk0, pkend, pwe, pwb, plg = k0, kend, we, wb, lg
+        k0 = kend + 1
+    t = H[h(iend, iend - 1)]
+    if t.imag != 0:
+        rtv = abs(t)
+        t = t / rtv
+        H[h(iend, iend - 1)] = rtv
+        for c in range(iend + 1, n + 1):
+            H[h(iend, c)] *= np.conj(t)
+        for r in range(1, iend):
+            H[h(r, iend)] *= t
+        Z[:, iend - 1] *= t
+
+
+@pytest.mark.parametrize("ZW", [6, 8, 32])
+@pytest.mark.parametrize("case", range(len(COMPLEX_CASES)))
+def test_complex_two_wave_sweep(ZW, case):
+    """z_sweep_win2: chase of chunk c overlapped with the chains of chunk c-1."""
+    n = 50
+    rng = np.random.default_rng(400 + case)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H, Z = _hessenberg(A)
+    c = COMPLEX_CASES[case]
+    istart1 = c[0] if c[0] > 0 else n + c[0]
+    istart = c[1] if c[1] > 0 else n + c[1]
+    iend = n - c[2]
+    v0 = complex(rng.standard_normal(), rng.standard_normal())
+    v1 = complex(abs(rng.standard_normal()), 0.0)
+    H1, Z1, H2, Z2 = H.copy(), Z.copy(), H.copy(), Z.copy()
+    sweep_seq(H1, Z1, istart1, istart, iend, v0, v1)
+    sweep_win2(H2, Z2, istart1, istart, iend, v0, v1, ZW)
+    assert np.array_equal(H1, H2) and np.array_equal(Z1, Z2)
