@@ -59,7 +59,7 @@ def copy_probe(mib):
 
 def load_traffic():
     """HBM bytes per QR-kernel launch from the committed rocprofv3 summary, if any."""
-    path = os.path.join(ROOT, "profiles", "traffic_zqrw_workload2.json")
+    path = os.path.join(ROOT, "profiles", "traffic_zqrw2_workload2.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -208,7 +208,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64,
             "traffic": traffic["bytes_per_launch"] if traffic else None,
-            "kernel": "gs::zqrw_kernel<true> (windowed QR sweeps + Z, one launch per size bin)",
+            "kernel": "gs::zqrw2_kernel<true> (two-wavefront windowed QR sweeps + Z, one launch per size bin)",
             "launches": len(qr),
             "avg_launch_us": qr_us / max(len(qr), 1),
             "flops_per_launch": qr_flops / max(len(qr), 1),

@@ -10,11 +10,11 @@ namespace gs {
 // v3) applied in order to one line x(t) = p[(t - 1) * st] of H or Z: the
 // right update of rows (GenericSchur.jl:916-935, 943-948) or the left
 // update of columns (:900-915), which have the same form.  The line's
-// values x(k), x(k+1) ride in registers; x(k+2) is read 4 steps ahead.
+// values x(k), x(k+1) ride in registers; x(k+2) is read PF steps ahead.
 GS_DEV void chain_line(double* p, int64_t st, const double* lg, int mx, int ka, int kb, int iend) {
   double c0 = p[(int64_t)(ka - 1) * st], c1 = p[(int64_t)ka * st];
   const int k3 = min(kb, iend - 2);  // steps with three-element reflectors
-  constexpr int PF = 4;
+  constexpr int PF = 8;
   double nb[PF];
 #pragma unroll
   for (int q = 0; q < PF; ++q)

@@ -35,6 +35,7 @@ namespace gs {
 constexpr int LNB = 32;  // Hessenberg panel width
 constexpr int LW = 64;   // bulge-chase window (rows and columns)
 constexpr int LKC = LW - 4;  // bulge steps per chunk
+constexpr int LWL = LW + 1;  // odd LDS pitch of a window column (bank spread)
 
 // ---------------------------------------------------------------- scaling
 template <int NT> __global__ void __launch_bounds__(NT) labsmax_kernel(const double* A, int64_t lda, int n,
@@ -595,7 +596,7 @@ struct LChunk {
 // window: columns right of the current window, rows above the previous
 // window, and Z.  The regions are disjoint.
 __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
-  __shared__ double win[LW * LW];
+  __shared__ double win[LWL * LW];
   const int lane = threadIdx.x;
   const int n = a.n, iend = ck.iend, mx = ck.mx;
   const int64_t ld = a.ld;
@@ -613,9 +614,9 @@ __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
     const int wb = max(ck.k0 - 1, 1);
     const int W = we - wb + 1;
     for (int c = 0; c < W; ++c)
-      for (int r = lane; r < W; r += 64) win[r + c * LW] = a.H[(wb - 1 + r) + (int64_t)(wb - 1 + c) * ld];
+      for (int r = lane; r < W; r += 64) win[r + c * LWL] = a.H[(wb - 1 + r) + (int64_t)(wb - 1 + c) * ld];
     __syncthreads();
-    auto HW = [&](int r, int c) -> double& { return win[(r - wb) + (c - wb) * LW]; };
+    auto HW = [&](int r, int c) -> double& { return win[(r - wb) + (c - wb) * LWL]; };
     const LState* st = a.st;
     // the next step's reflector input comes from the lanes that computed it
     // in the right update (v_readlane), the first step of the chunk reads
@@ -701,7 +702,7 @@ __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
     }
     __syncthreads();
     for (int c = 0; c < W; ++c)
-      for (int r = lane; r < W; r += 64) a.H[(wb - 1 + r) + (int64_t)(wb - 1 + c) * ld] = win[r + c * LW];
+      for (int r = lane; r < W; r += 64) a.H[(wb - 1 + r) + (int64_t)(wb - 1 + c) * ld] = win[r + c * LWL];
     return;
   }
   if (!prev) return;

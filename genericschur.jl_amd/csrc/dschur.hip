@@ -435,6 +435,7 @@ template <int NT, class L, bool FAST> GS_DEV void d_sweep(double* H, L lay, int 
 // expressions as in d_sweep.  The LDS footprint is the window only.
 constexpr int DW = 32;       // window rows and columns
 constexpr int DKC = DW - 4;  // bulge steps per chunk
+constexpr int DWL = DW + 1;  // odd LDS pitch of a window column (bank spread)
 
 template <bool FAST> GS_DEV void d_sweep_win(double* H, int64_t ld, int n, double* Zg, int64_t ldz, bool wantz,
                                             double* win, double* lg, int* redi, zd shift1, zd shift2, int istart,
@@ -473,9 +474,9 @@ template <bool FAST> GS_DEV void d_sweep_win(double* H, int64_t ld, int n, doubl
     const int kend = min(k0 + DKC - 1, iend - 1);
     const int wb = max(k0 - 1, 1), we = min(kend + 3, iend);
     const int W = we - wb + 1;
-    auto WIN = [&](int r, int c) -> double& { return win[(r - wb) + (c - wb) * DW]; };
+    auto WIN = [&](int r, int c) -> double& { return win[(r - wb) + (c - wb) * DWL]; };
     for (int c = 0; c < W; ++c)
-      if (lane < W) win[lane + c * DW] = H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld];
+      if (lane < W) win[lane + c * DWL] = H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld];
     __syncthreads();
     double nv0 = 0.0, nv1 = 0.0, nv2 = 0.0;
     for (int k = k0; k <= kend; ++k) {
@@ -558,7 +559,7 @@ template <bool FAST> GS_DEV void d_sweep_win(double* H, int64_t ld, int n, doubl
       wsync();
     }
     for (int c = 0; c < W; ++c)
-      if (lane < W) H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld] = win[lane + c * DW];
+      if (lane < W) H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld] = win[lane + c * DWL];
     __syncthreads();
     for (int col = we + 1 + lane; col <= n; col += 64)
       chain_line(H + (int64_t)(col - 1) * ld, 1, lg, k0, k0, kend, iend);
@@ -682,7 +683,7 @@ template <int NT, class L, bool FAST, bool WIN = false> GS_DEV int64_t d_qr(doub
       }
       nsweep += 1;
       if constexpr (WIN)
-        d_sweep_win<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + DW * DW, redi, zmk(r1r, r1i), zmk(r2r, r2i),
+        d_sweep_win<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + DWL * DW, redi, zmk(r1r, r1i), zmk(r2r, r2i),
                           istart, iend, nstep);
       else
         d_sweep<NT, L, FAST>(H, lay, n, Zg, ldz, wantz, rot, redi, zmk(r1r, r1i), zmk(r2r, r2i), istart, iend, nstep);
@@ -898,7 +899,7 @@ template <int NT> __global__ void __launch_bounds__(NT) dschur_global_kernel(Sin
 // Batched kernel 2, windowed form: one wavefront per matrix, H (the T
 // buffer) and Z in HBM, LDS holds the chase window and the chunk's log.
 template <bool FAST> __global__ void __launch_bounds__(64) dqrw_kernel(BatchArgs a) {
-  __shared__ double win[DW * DW + 3 * DKC];
+  __shared__ double win[DWL * DW + 3 * DKC];
   __shared__ int redi[2];
   const int b = blockIdx.x;
   if (b >= a.count) return;
@@ -975,6 +976,7 @@ hipError_t launch_dhq(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
     case 64: return launch_dhq_nt<64>(a, lds, s);
     case 128: return launch_dhq_nt<128>(a, lds, s);
     case 256: return launch_dhq_nt<256>(a, lds, s);
+    case 512: return launch_dhq_nt<512>(a, lds, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -71,11 +71,13 @@ static int fail_hip(gs_handle* h, hipError_t e, const char* where) { return hand
 static int fail(gs_handle* h, int code, const char* msg) { return handle_fail(h, code, msg); }
 
 // GSCHUR_QR_BAND=1 selects the band-resident QR kernels everywhere (A/B
-// measurements), as GS_QR_BAND does per call
+// measurements), as GS_QR_BAND does per call; GSCHUR_QR_NOPRIO=1 keeps the
+// chasing wavefront of the windowed kernel at the default priority
 static int env_qr_band() {
   static const int v = [] {
     const char* e = std::getenv("GSCHUR_QR_BAND");
-    return (e && e[0] == '1') ? GS_QR_BAND : 0;
+    const char* p = std::getenv("GSCHUR_QR_NOPRIO");
+    return ((e && e[0] == '1') ? GS_QR_BAND : 0) | ((p && p[0] == '1') ? 0x4000 : 0);
   }();
   return v;
 }
@@ -168,9 +170,16 @@ int gs_batched_max_n(int is_complex) { return is_complex ? zschur_max_lds_n() : 
 }  // extern "C"
 
 // ------------------------------------------------------------------ plans
+// GSCHUR_HQ_NT overrides the Hessenberg+Q workgroup size of bins above
+// n = 32 (A/B measurements; 256 or 512)
 static int default_nt_hq(bool cplx, int nmax) {
   (void)cplx;
-  return nmax > 32 ? 256 : 64;
+  static const int big = [] {
+    const char* e = std::getenv("GSCHUR_HQ_NT");
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 256 || v == 512) ? v : 256;
+  }();
+  return nmax > 32 ? big : 64;
 }
 static int default_nt_qr(bool cplx, int nmax) {
   (void)cplx;

@@ -495,13 +495,16 @@ template <int NT, class L, bool FAST> GS_DEV void z_sweep(zd* H, L lay, int n, z
 // a CU and hide each other's latency.
 constexpr int ZW = 32;       // window rows and columns
 constexpr int ZKC = ZW - 3;  // bulge steps per chunk
+// LDS pitch of a window column: odd, so that the lanes of a row access
+// (left update, lane = column) fall into different banks
+constexpr int ZWL = ZW + 1;
 
 // Logged reflectors of steps ka..kb (lg[2(k-ka)] = tau1, lg[2(k-ka)+1] = v2)
 // applied to the line x(t) = p[(t - 1) st]: LEFT updates rows k, k+1 of a
 // column of H (GenericSchur.jl:437-446), otherwise columns k, k+1 of a row
 // of H or Z (:447-460).  x(k+1) is read PF steps ahead.
 template <bool LEFT> GS_DEV void zchain(zd* p, int64_t st, const zd* lg, int ka, int kb) {
-  constexpr int PF = 4;
+  constexpr int PF = 6;
   zd c0 = p[(int64_t)(ka - 1) * st];
   zd nb[PF];
 #pragma unroll
@@ -589,9 +592,9 @@ template <bool FAST> GS_DEV void z_sweep_win(zd* H, int64_t ld, int n, zd* Zg, i
     const int kend = phase_pending ? k0 : min(k0 + ZKC - 1, iend - 1);
     const int wb = max(k0 - 1, 1), we = min(kend + 2, iend);
     const int W = we - wb + 1;
-    auto WIN = [&](int r, int c) -> zd& { return win[(r - wb) + (c - wb) * ZW]; };
+    auto WIN = [&](int r, int c) -> zd& { return win[(r - wb) + (c - wb) * ZWL]; };
     for (int c = 0; c < W; ++c)
-      if (lane < W) win[lane + c * ZW] = H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld];
+      if (lane < W) win[lane + c * ZWL] = H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld];
     __syncthreads();
     zd nv0 = zr(0.0), nv1 = zr(0.0);
     for (int k = k0; k <= kend; ++k) {
@@ -647,7 +650,7 @@ template <bool FAST> GS_DEV void z_sweep_win(zd* H, int64_t ld, int n, zd* Zg, i
     }
     // window back to H, then the chunk's reflectors outside the window
     for (int c = 0; c < W; ++c)
-      if (lane < W) H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld] = win[lane + c * ZW];
+      if (lane < W) H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld] = win[lane + c * ZWL];
     __syncthreads();
     for (int col = we + 1 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, lg, k0, kend);
     for (int row = 1 + lane; row <= wb - 1; row += 64) zchain<false>(H + (row - 1), ld, lg, k0, kend);
@@ -764,9 +767,9 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
           __builtin_amdgcn_wave_barrier();
         }
         const int W = we - wb + 1;
-        auto WIN = [&](int r, int cc) -> zd& { return win[(r - wb) + (cc - wb) * ZW]; };
+        auto WIN = [&](int r, int cc) -> zd& { return win[(r - wb) + (cc - wb) * ZWL]; };
         for (int cc = 0; cc < W; ++cc)
-          if (lane < W) win[lane + cc * ZW] = H[(wb - 1 + lane) + (int64_t)(wb - 1 + cc) * ld];
+          if (lane < W) win[lane + cc * ZWL] = H[(wb - 1 + lane) + (int64_t)(wb - 1 + cc) * ld];
         wsync();
         zd nv0 = zr(0.0), nv1 = zr(0.0);
         for (int k = k0; k <= kend; ++k) {
@@ -821,7 +824,7 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
           wsync();
         }
         for (int cc = 0; cc < W; ++cc)
-          if (lane < W) H[(wb - 1 + lane) + (int64_t)(wb - 1 + cc) * ld] = win[lane + cc * ZW];
+          if (lane < W) H[(wb - 1 + lane) + (int64_t)(wb - 1 + cc) * ld] = win[lane + cc * ZWL];
       }
     } else if (prev) {
       const int lcol0 = (chase ? we : pwe) + 1;
@@ -973,9 +976,9 @@ template <int NT, class L, bool FAST, int WIN = 0> GS_DEV int64_t z_qr(zd* H, L 
       }
       nsweep += 1;
       if constexpr (WIN == 2)
-        z_sweep_win2<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + ZW * ZW, redi, t, istart, iend, nstep);
+        z_sweep_win2<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + ZWL * ZW, redi, t, istart, iend, nstep);
       else if constexpr (WIN == 1)
-        z_sweep_win<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + ZW * ZW, redi, t, istart, iend, nstep);
+        z_sweep_win<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + ZWL * ZW, redi, t, istart, iend, nstep);
       else
         z_sweep<NT, L, FAST>(H, lay, n, Zg, ldz, wantz, rot, redi, t, istart, iend, nstep);
     }
@@ -1093,7 +1096,7 @@ template <int NT, bool FAST> __global__ void __launch_bounds__(NT) zqr_kernel(Ba
 // T buffer, written by zhq_kernel) and Z in HBM; LDS holds the chase window
 // and the chunk's reflector log only.
 template <bool FAST> __global__ void __launch_bounds__(64) zqrw_kernel(BatchArgs a) {
-  __shared__ zd win[ZW * ZW + 2 * ZKC];
+  __shared__ zd win[ZWL * ZW + 2 * ZKC];
   __shared__ int redi[2];
   const int b = blockIdx.x;
   if (b >= a.count) return;
@@ -1127,10 +1130,13 @@ template <bool FAST> __global__ void __launch_bounds__(64) zqrw_kernel(BatchArgs
 // Two-wavefront windowed QR kernel: the chase of chunk c overlaps the chains
 // of chunk c-1 (z_sweep_win2); the control logic runs on both wavefronts.
 template <bool FAST> __global__ void __launch_bounds__(128) zqrw2_kernel(BatchArgs a) {
-  __shared__ zd win[ZW * ZW + 4 * ZKC];
+  __shared__ zd win[ZWL * ZW + 4 * ZKC];
   __shared__ int redi[4];
   const int b = blockIdx.x;
   if (b >= a.count) return;
+  // the chasing wavefront (0) issues ahead of the chain wavefronts sharing
+  // its SIMD (flag 0x4000 turns this off for comparisons)
+  if (!(a.flags & 0x4000) && threadIdx.x < 64) __builtin_amdgcn_s_setprio(2);
   const int m = a.order[a.first + b];
   const int n = a.n[m];
   if (n <= 0) return;
@@ -1263,6 +1269,7 @@ hipError_t launch_zhq(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
     case 64: return launch_zhq_nt<64>(a, lds, s);
     case 128: return launch_zhq_nt<128>(a, lds, s);
     case 256: return launch_zhq_nt<256>(a, lds, s);
+    case 512: return launch_zhq_nt<512>(a, lds, s);
     default: return hipErrorInvalidValue;
   }
 }
