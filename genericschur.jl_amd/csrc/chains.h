@@ -7,60 +7,69 @@
 namespace gs {
 
 // The logged reflectors of bulge steps ka .. kb (log entry k - mx: tau1, v2,
-// v3) applied in order to one line x(t) = p[(t - 1) * st] of H or Z: the
-// right update of rows (GenericSchur.jl:916-935, 943-948) or the left
-// update of columns (:900-915), which have the same form.  The line's
-// values x(k), x(k+1) ride in registers; x(k+2) is read PF steps ahead.
+// v3; kb - ka < MAXS) applied in order to one line x(t) = p[(t - 1) * st] of
+// H or Z: the right update of rows (GenericSchur.jl:916-935, 943-948) or the
+// left update of columns (:900-915), which have the same form.  A step
+// k = iend - 1 has a two-element reflector.  The line's segment moves
+// through registers in groups of G elements, two groups in flight: a group's
+// loads are issued once the group two places before it has been stored, so a
+// chain waits on memory about once per group, not once per element.
+template <int MAXS, int G>
 GS_DEV void chain_line(double* p, int64_t st, const double* lg, int mx, int ka, int kb, int iend) {
-  double c0 = p[(int64_t)(ka - 1) * st], c1 = p[(int64_t)ka * st];
-  const int k3 = min(kb, iend - 2);  // steps with three-element reflectors
-  constexpr int PF = 8;
-  double nb[PF];
+  constexpr int NE = MAXS + 2, NG = (NE + G - 1) / G;
+  // uniform counts (SCC branches)
+  const int s = __builtin_amdgcn_readfirstlane(kb - ka + 1);
+  const int k3 = __builtin_amdgcn_readfirstlane(min(kb, iend - 2));  // last step with a three-element reflector
+  ka = __builtin_amdgcn_readfirstlane(ka);
+  const int ne = (k3 == ka + s - 1) ? s + 2 : s + 1;                   // elements x(ka) .. x(ka + ne - 1)
+  // log offset through an opaque (volatile asm) SGPR: the log is re-read per
+  // line (LDS broadcasts, or cached loads) rather than every step's
+  // reflector being held in registers across the callers' loops over lines
+  int lb = 3 * (ka - __builtin_amdgcn_readfirstlane(mx));
+  asm volatile("" : "+s"(lb));
+  const double* lv = lg + lb;
+  double* q0 = p + (int64_t)(ka - 1) * st;
+  double x[NG * G];
 #pragma unroll
-  for (int q = 0; q < PF; ++q)
-    if (ka + q <= k3) nb[q] = p[(int64_t)(ka + q + 1) * st];
-  int k = ka;
-  for (; k + PF - 1 <= k3; k += PF) {
-    double cur[PF];
+  for (int g = 0; g < 2 && g < NG; ++g)
+    if (g * G < ne) {
 #pragma unroll
-    for (int q = 0; q < PF; ++q) cur[q] = nb[q];
+      for (int i = 0; i < G; ++i) x[g * G + i] = q0[(int64_t)min(g * G + i, ne - 1) * st];
+    }
 #pragma unroll
-    for (int q = 0; q < PF; ++q)
-      if (k + PF + q <= k3) nb[q] = p[(int64_t)(k + PF + q + 1) * st];
+  for (int g = 0; g < NG; ++g) {
+    if (g * G >= ne) continue;
 #pragma unroll
-    for (int q = 0; q < PF; ++q) {
-      const double* e = lg + 3 * (k + q - mx);
-      const double tau1 = e[0], v2 = e[1], v3 = e[2];
-      const double tau2 = tau1 * v2, tau3 = tau1 * v3;
-      const double b = cur[q];
-      const double ss = c0 + v2 * c1 + v3 * b;
-      p[(int64_t)(k + q - 1) * st] = c0 - ss * tau1;
-      c0 = c1 - ss * tau2;
-      c1 = b - ss * tau3;
+    for (int i = 0; i < G; ++i) {
+      const int q = g * G + i;
+      if (q < s) {
+        const double tau1 = lv[3 * q], v2 = lv[3 * q + 1];
+        const double tau2 = tau1 * v2;
+        if (q <= k3 - ka) {
+          const double v3 = lv[3 * q + 2];
+          const double tau3 = tau1 * v3;
+          const double ss = x[q] + v2 * x[q + 1] + v3 * x[q + 2];
+          x[q] = x[q] - ss * tau1;
+          x[q + 1] = x[q + 1] - ss * tau2;
+          x[q + 2] = x[q + 2] - ss * tau3;
+        } else {
+          const double ss = x[q] + v2 * x[q + 1];
+          x[q] = x[q] - ss * tau1;
+          x[q + 1] = x[q + 1] - ss * tau2;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int q = g * G + i;
+      if (q < ne) q0[(int64_t)q * st] = x[q];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (g + 2 < NG && (g + 2) * G < ne) {
+#pragma unroll
+      for (int i = 0; i < G; ++i) x[(g + 2) * G + i] = q0[(int64_t)min((g + 2) * G + i, ne - 1) * st];
     }
   }
-  for (; k <= k3; ++k) {
-    const double* e = lg + 3 * (k - mx);
-    const double tau1 = e[0], v2 = e[1], v3 = e[2];
-    const double tau2 = tau1 * v2, tau3 = tau1 * v3;
-    const double b = p[(int64_t)(k + 1) * st];
-    const double ss = c0 + v2 * c1 + v3 * b;
-    p[(int64_t)(k - 1) * st] = c0 - ss * tau1;
-    c0 = c1 - ss * tau2;
-    c1 = b - ss * tau3;
-  }
-  if (k <= kb) {  // k == iend - 1: two-element reflector
-    const double* e = lg + 3 * (k - mx);
-    const double tau1 = e[0], v2 = e[1];
-    const double tau2 = tau1 * v2;
-    const double ss = c0 + v2 * c1;
-    p[(int64_t)(k - 1) * st] = c0 - ss * tau1;
-    c0 = c1 - ss * tau2;
-    p[(int64_t)k * st] = c0;
-    return;
-  }
-  p[(int64_t)kb * st] = c0;
-  p[(int64_t)(kb + 1) * st] = c1;
 }
 
 }  // namespace gs

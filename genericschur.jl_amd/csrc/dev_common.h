@@ -385,3 +385,58 @@ template <int NT> GS_DEV void bsync_full() {
   if (NT > 64) __syncthreads();
   else wsync();
 }
+
+// Copy the W x W block at src (column-major, leading dimension ld, W <= WMAX)
+// into LDS with column pitch WL (win_load), or back (win_store), by one
+// wavefront.  A batch's global loads are all issued before the first of them
+// is used, so the block costs one memory latency per batch, not one per
+// column.  Lane l covers row l % R of columns l / R, l / R + 64 / R, ...;
+// loads clamp the row and column into the block and are not masked, stores
+// are.
+template <class T, int WMAX, int WL> GS_DEV void win_load(const T* src, int64_t ld, int W, T* win, int lane) {
+  constexpr int R = WMAX < 64 ? WMAX : 64;
+  constexpr int CPI = 64 / R;
+  constexpr int NI = WMAX / CPI;
+  constexpr int B0 = 256 / (int)sizeof(T);
+  constexpr int B = NI < B0 ? NI : B0;
+  static_assert(64 % R == 0 && WMAX % CPI == 0 && NI % B == 0 && WL >= WMAX, "window shape");
+  W = __builtin_amdgcn_readfirstlane(W);  // uniform: branches on SCC
+  const int r = lane % R, ch = lane / R;
+  const T* bp = src + min(r, W - 1);
+  for (int i0 = 0; i0 < NI && CPI * i0 < W; i0 += B) {
+    T t[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) t[i] = bp[(int64_t)min(CPI * (i0 + i) + ch, W - 1) * ld];
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      const int c = CPI * (i0 + i) + ch;
+      if (r < W && c < W) win[r + c * WL] = t[i];
+    }
+  }
+}
+template <class T, int WMAX, int WL> GS_DEV void win_store(T* dst, int64_t ld, int W, const T* win, int lane) {
+  constexpr int R = WMAX < 64 ? WMAX : 64;
+  constexpr int CPI = 64 / R;
+  constexpr int NI = WMAX / CPI;
+  constexpr int B0 = 128 / (int)sizeof(T);  // LDS reads only: short batches suffice
+  constexpr int B = NI < B0 ? NI : B0;
+  static_assert(64 % R == 0 && WMAX % CPI == 0 && NI % B == 0 && WL >= WMAX, "window shape");
+  W = __builtin_amdgcn_readfirstlane(W);  // uniform: branches on SCC
+  const int r = lane % R, ch = lane / R;
+  T* bp = dst + r + (int64_t)ch * ld;
+  // opaque offset: keeps the compiler from computing the LDS addresses once
+  // and holding them in registers for the whole kernel
+  int o = 0;
+  asm volatile("" : "+s"(o));
+  win += o;
+  for (int i0 = 0; i0 < NI && CPI * i0 < W; i0 += B) {
+    T t[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) t[i] = win[r + (CPI * (i0 + i) + ch) * WL];
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      const int c0 = CPI * (i0 + i);
+      if (r < W && c0 + ch < W) bp[(int64_t)c0 * ld] = t[i];
+    }
+  }
+}

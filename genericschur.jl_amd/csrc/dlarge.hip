@@ -608,13 +608,12 @@ __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
     if (!chase) return;
     if (prev) {
       const int col = pwe + 1 + lane;
-      if (col <= we) chain_line(a.H + (int64_t)(col - 1) * ld, 1, a.lg, mx, ck.pk0, ck.pkend, iend);
+      if (col <= we) chain_line<LKC, 16>(a.H + (int64_t)(col - 1) * ld, 1, a.lg, mx, ck.pk0, ck.pkend, iend);
       __syncthreads();
     }
     const int wb = max(ck.k0 - 1, 1);
     const int W = we - wb + 1;
-    for (int c = 0; c < W; ++c)
-      for (int r = lane; r < W; r += 64) win[r + c * LWL] = a.H[(wb - 1 + r) + (int64_t)(wb - 1 + c) * ld];
+    win_load<double, LW, LWL>(a.H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
     __syncthreads();
     auto HW = [&](int r, int c) -> double& { return win[(r - wb) + (c - wb) * LWL]; };
     const LState* st = a.st;
@@ -701,8 +700,7 @@ __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
       wsync();
     }
     __syncthreads();
-    for (int c = 0; c < W; ++c)
-      for (int r = lane; r < W; r += 64) a.H[(wb - 1 + r) + (int64_t)(wb - 1 + c) * ld] = win[r + c * LWL];
+    win_store<double, LW, LWL>(a.H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
     return;
   }
   if (!prev) return;
@@ -716,13 +714,13 @@ __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
   for (int t = blockIdx.x - 1; t < ntask; t += gridDim.x - 1) {
     if (t < nL) {
       const int col = lcol0 + 64 * t + lane;
-      if (col <= n) chain_line(a.H + (int64_t)(col - 1) * ld, 1, a.lg, mx, ck.pk0, ck.pkend, iend);
+      if (col <= n) chain_line<LKC, 16>(a.H + (int64_t)(col - 1) * ld, 1, a.lg, mx, ck.pk0, ck.pkend, iend);
     } else if (t < nL + nR) {
       const int row = 1 + 64 * (t - nL) + lane;
-      if (row <= pwb - 1) chain_line(a.H + (row - 1), ld, a.lg, mx, ck.pk0, ck.pkend, iend);
+      if (row <= pwb - 1) chain_line<LKC, 16>(a.H + (row - 1), ld, a.lg, mx, ck.pk0, ck.pkend, iend);
     } else {
       const int row = 1 + 64 * (t - nL - nR) + lane;
-      if (row <= n) chain_line(a.Z + (row - 1), a.ldz, a.lg, mx, ck.pk0, ck.pkend, iend);
+      if (row <= n) chain_line<LKC, 16>(a.Z + (row - 1), a.ldz, a.lg, mx, ck.pk0, ck.pkend, iend);
     }
   }
   (void)plast;

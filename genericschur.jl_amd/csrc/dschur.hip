@@ -475,8 +475,7 @@ template <bool FAST> GS_DEV void d_sweep_win(double* H, int64_t ld, int n, doubl
     const int wb = max(k0 - 1, 1), we = min(kend + 3, iend);
     const int W = we - wb + 1;
     auto WIN = [&](int r, int c) -> double& { return win[(r - wb) + (c - wb) * DWL]; };
-    for (int c = 0; c < W; ++c)
-      if (lane < W) win[lane + c * DWL] = H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld];
+    win_load<double, DW, DWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
     __syncthreads();
     double nv0 = 0.0, nv1 = 0.0, nv2 = 0.0;
     for (int k = k0; k <= kend; ++k) {
@@ -558,14 +557,13 @@ template <bool FAST> GS_DEV void d_sweep_win(double* H, int64_t ld, int n, doubl
       }
       wsync();
     }
-    for (int c = 0; c < W; ++c)
-      if (lane < W) H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld] = win[lane + c * DWL];
+    win_store<double, DW, DWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
     __syncthreads();
     for (int col = we + 1 + lane; col <= n; col += 64)
-      chain_line(H + (int64_t)(col - 1) * ld, 1, lg, k0, k0, kend, iend);
-    for (int row = 1 + lane; row <= wb - 1; row += 64) chain_line(H + (row - 1), ld, lg, k0, k0, kend, iend);
+      chain_line<DKC, 8>(H + (int64_t)(col - 1) * ld, 1, lg, k0, k0, kend, iend);
+    for (int row = 1 + lane; row <= wb - 1; row += 64) chain_line<DKC, 8>(H + (row - 1), ld, lg, k0, k0, kend, iend);
     if (wantz)
-      for (int row = 1 + lane; row <= n; row += 64) chain_line(Zg + (row - 1), ldz, lg, k0, k0, kend, iend);
+      for (int row = 1 + lane; row <= n; row += 64) chain_line<DKC, 8>(Zg + (row - 1), ldz, lg, k0, k0, kend, iend);
     __syncthreads();
     k0 = kend + 1;
   }
@@ -898,7 +896,7 @@ template <int NT> __global__ void __launch_bounds__(NT) dschur_global_kernel(Sin
 
 // Batched kernel 2, windowed form: one wavefront per matrix, H (the T
 // buffer) and Z in HBM, LDS holds the chase window and the chunk's log.
-template <bool FAST> __global__ void __launch_bounds__(64) dqrw_kernel(BatchArgs a) {
+template <bool FAST> __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dqrw_kernel(BatchArgs a) {
   __shared__ double win[DWL * DW + 3 * DKC];
   __shared__ int redi[2];
   const int b = blockIdx.x;

@@ -499,56 +499,64 @@ constexpr int ZKC = ZW - 3;  // bulge steps per chunk
 // (left update, lane = column) fall into different banks
 constexpr int ZWL = ZW + 1;
 
-// Logged reflectors of steps ka..kb (lg[2(k-ka)] = tau1, lg[2(k-ka)+1] = v2)
-// applied to the line x(t) = p[(t - 1) st]: LEFT updates rows k, k+1 of a
-// column of H (GenericSchur.jl:437-446), otherwise columns k, k+1 of a row
-// of H or Z (:447-460).  x(k+1) is read PF steps ahead.
+// Logged reflectors of steps ka..kb (lg[2(k-ka)] = tau1, lg[2(k-ka)+1] = v2,
+// kb - ka < ZKC) applied to the line x(t) = p[(t - 1) st]: LEFT updates rows
+// k, k+1 of a column of H (GenericSchur.jl:437-446), otherwise columns k, k+1
+// of a row of H or Z (:447-460).  The segment x(ka) .. x(kb+1) moves through
+// registers in groups of ZCG elements, two groups in flight: a group's loads
+// are issued as soon as the group two places before it has been stored, so a
+// chain of a full chunk waits on memory about twice, not once per element.
+constexpr int ZCG = 8;
 template <bool LEFT> GS_DEV void zchain(zd* p, int64_t st, const zd* lg, int ka, int kb) {
-  constexpr int PF = 6;
-  zd c0 = p[(int64_t)(ka - 1) * st];
-  zd nb[PF];
+  constexpr int NG = (ZKC + 1 + ZCG - 1) / ZCG;
+  const int s = __builtin_amdgcn_readfirstlane(kb - ka + 1);  // uniform: branches on SCC
+  ka = __builtin_amdgcn_readfirstlane(ka);
+  // log offset through an opaque (volatile asm) SGPR: the log is re-read per
+  // line (LDS broadcasts) rather than held in registers across the callers'
+  // loops over lines
+  int lb = 0;
+  asm volatile("" : "+s"(lb));
+  const zd* lgq = lg + lb;
+  zd* q0 = p + (int64_t)(ka - 1) * st;
+  zd x[NG * ZCG];
 #pragma unroll
-  for (int q = 0; q < PF; ++q)
-    if (ka + q <= kb) nb[q] = p[(int64_t)(ka + q) * st];
-  int k = ka;
-  for (; k + PF - 1 <= kb; k += PF) {
-    zd cur[PF];
+  for (int g = 0; g < 2 && g < NG; ++g)
+    if (g * ZCG <= s) {
 #pragma unroll
-    for (int q = 0; q < PF; ++q) cur[q] = nb[q];
+      for (int i = 0; i < ZCG; ++i) x[g * ZCG + i] = q0[(int64_t)min(g * ZCG + i, s) * st];
+    }
 #pragma unroll
-    for (int q = 0; q < PF; ++q)
-      if (k + PF + q <= kb) nb[q] = p[(int64_t)(k + PF + q) * st];
+  for (int g = 0; g < NG; ++g) {
+    if (g * ZCG > s) break;
 #pragma unroll
-    for (int q = 0; q < PF; ++q) {
-      const zd tau1 = lg[2 * (k + q - ka)], v2 = lg[2 * (k + q - ka) + 1];
-      const double tau2 = (tau1 * v2).x;
-      const zd b = cur[q];
-      if (LEFT) {
-        const zd ss = zconj(tau1) * c0 + tau2 * b;
-        p[(int64_t)(k + q - 1) * st] = c0 - ss;
-        c0 = b - ss * v2;
-      } else {
-        const zd ss = tau1 * c0 + tau2 * b;
-        p[(int64_t)(k + q - 1) * st] = c0 - ss;
-        c0 = b - ss * zconj(v2);
+    for (int i = 0; i < ZCG; ++i) {
+      const int q = g * ZCG + i;
+      if (q < s) {
+        const zd tau1 = lgq[2 * q], v2 = lgq[2 * q + 1];
+        const double tau2 = (tau1 * v2).x;
+        const zd a = x[q], b = x[q + 1];
+        if (LEFT) {
+          const zd ss = zconj(tau1) * a + tau2 * b;
+          x[q] = a - ss;
+          x[q + 1] = b - ss * v2;
+        } else {
+          const zd ss = tau1 * a + tau2 * b;
+          x[q] = a - ss;
+          x[q + 1] = b - ss * zconj(v2);
+        }
       }
     }
-  }
-  for (; k <= kb; ++k) {
-    const zd tau1 = lg[2 * (k - ka)], v2 = lg[2 * (k - ka) + 1];
-    const double tau2 = (tau1 * v2).x;
-    const zd b = p[(int64_t)k * st];
-    if (LEFT) {
-      const zd ss = zconj(tau1) * c0 + tau2 * b;
-      p[(int64_t)(k - 1) * st] = c0 - ss;
-      c0 = b - ss * v2;
-    } else {
-      const zd ss = tau1 * c0 + tau2 * b;
-      p[(int64_t)(k - 1) * st] = c0 - ss;
-      c0 = b - ss * zconj(v2);
+#pragma unroll
+    for (int i = 0; i < ZCG; ++i) {
+      const int q = g * ZCG + i;
+      if (q <= s) q0[(int64_t)q * st] = x[q];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (g + 2 < NG && (g + 2) * ZCG <= s) {
+#pragma unroll
+      for (int i = 0; i < ZCG; ++i) x[(g + 2) * ZCG + i] = q0[(int64_t)min((g + 2) * ZCG + i, s) * st];
     }
   }
-  p[(int64_t)kb * st] = c0;
 }
 
 template <bool FAST> GS_DEV void z_sweep_win(zd* H, int64_t ld, int n, zd* Zg, int64_t ldz, bool wantz, zd* win,
@@ -593,8 +601,7 @@ template <bool FAST> GS_DEV void z_sweep_win(zd* H, int64_t ld, int n, zd* Zg, i
     const int wb = max(k0 - 1, 1), we = min(kend + 2, iend);
     const int W = we - wb + 1;
     auto WIN = [&](int r, int c) -> zd& { return win[(r - wb) + (c - wb) * ZWL]; };
-    for (int c = 0; c < W; ++c)
-      if (lane < W) win[lane + c * ZWL] = H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld];
+    win_load<zd, ZW, ZWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
     __syncthreads();
     zd nv0 = zr(0.0), nv1 = zr(0.0);
     for (int k = k0; k <= kend; ++k) {
@@ -649,8 +656,7 @@ template <bool FAST> GS_DEV void z_sweep_win(zd* H, int64_t ld, int n, zd* Zg, i
       wsync();
     }
     // window back to H, then the chunk's reflectors outside the window
-    for (int c = 0; c < W; ++c)
-      if (lane < W) H[(wb - 1 + lane) + (int64_t)(wb - 1 + c) * ld] = win[lane + c * ZWL];
+    win_store<zd, ZW, ZWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
     __syncthreads();
     for (int col = we + 1 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, lg, k0, kend);
     for (int row = 1 + lane; row <= wb - 1; row += 64) zchain<false>(H + (row - 1), ld, lg, k0, kend);
@@ -767,10 +773,9 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
           __builtin_amdgcn_wave_barrier();
         }
         const int W = we - wb + 1;
-        auto WIN = [&](int r, int cc) -> zd& { return win[(r - wb) + (cc - wb) * ZWL]; };
-        for (int cc = 0; cc < W; ++cc)
-          if (lane < W) win[lane + cc * ZWL] = H[(wb - 1 + lane) + (int64_t)(wb - 1 + cc) * ld];
+        win_load<zd, ZW, ZWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
         wsync();
+        auto WIN = [&](int r, int cc) -> zd& { return win[(r - wb) + (cc - wb) * ZWL]; };
         zd nv0 = zr(0.0), nv1 = zr(0.0);
         for (int k = k0; k <= kend; ++k) {
           if (k > istart1) {
@@ -823,8 +828,7 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
           }
           wsync();
         }
-        for (int cc = 0; cc < W; ++cc)
-          if (lane < W) H[(wb - 1 + lane) + (int64_t)(wb - 1 + cc) * ld] = win[lane + cc * ZWL];
+        win_store<zd, ZW, ZWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
       }
     } else if (prev) {
       const int lcol0 = (chase ? we : pwe) + 1;
@@ -1129,7 +1133,7 @@ template <bool FAST> __global__ void __launch_bounds__(64) zqrw_kernel(BatchArgs
 
 // Two-wavefront windowed QR kernel: the chase of chunk c overlaps the chains
 // of chunk c-1 (z_sweep_win2); the control logic runs on both wavefronts.
-template <bool FAST> __global__ void __launch_bounds__(128) zqrw2_kernel(BatchArgs a) {
+template <bool FAST> __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) zqrw2_kernel(BatchArgs a) {
   __shared__ zd win[ZWL * ZW + 4 * ZKC];
   __shared__ int redi[4];
   const int b = blockIdx.x;
