@@ -160,8 +160,10 @@ template <int NT, class F> GS_DEV void zdots4(int i0, int i1, int k0, int k1, F 
   for (int ib = i0; ib < i1; ib += NT / 4) {
     const int i = ib + il;
     zd acc = zr(0.0);
-    if (i < i1)
+    if (i < i1) {
+#pragma unroll 4
       for (int k = ks; k < ke; ++k) acc = acc + f(k, i);
+    }
     acc.x += __shfl_xor(acc.x, 1, 64);
     acc.y += __shfl_xor(acc.y, 1, 64);
     acc.x += __shfl_xor(acc.x, 2, 64);
@@ -176,8 +178,10 @@ template <int NT, class F> GS_DEV void ddots4(int i0, int i1, int k0, int k1, F 
   for (int ib = i0; ib < i1; ib += NT / 4) {
     const int i = ib + il;
     double acc = 0.0;
-    if (i < i1)
+    if (i < i1) {
+#pragma unroll 4
       for (int k = ks; k < ke; ++k) acc += f(k, i);
+    }
     acc += __shfl_xor(acc, 1, 64);
     acc += __shfl_xor(acc, 2, 64);
     if (s == 0 && i < i1) out[i] = acc;
@@ -384,6 +388,25 @@ GS_DEV void wsync() {
 template <int NT> GS_DEV void bsync_full() {
   if (NT > 64) __syncthreads();
   else wsync();
+}
+
+// The right update of a pair (x0, x1) by a logged complex reflector,
+// ss = tau1 x0 + tau2 x1, x0 -= ss, x1 -= ss conj(v2) with tau2 = Re(tau1 v2)
+// (GenericSchur.jl:447-460), and a complex product, with a fixed rounding:
+// no contraction beyond the fma calls written here.  Every path that applies
+// a reflector or a scaling to Z (in place during the sweeps, or replayed
+// from the Z log) goes through these, so the paths give the same bits.
+GS_DEV void zrot_right_fixed(zd& x0, zd& x1, zd tau1, zd v2) {
+#pragma clang fp contract(off)
+  const double tau2 = fma(tau1.x, v2.x, -(tau1.y * v2.y));
+  const double sr = fma(tau1.x, x0.x, fma(-tau1.y, x0.y, tau2 * x1.x));
+  const double si = fma(tau1.x, x0.y, fma(tau1.y, x0.x, tau2 * x1.y));
+  x0 = zmk(x0.x - sr, x0.y - si);
+  x1 = zmk(x1.x - fma(sr, v2.x, si * v2.y), x1.y - fma(si, v2.x, -(sr * v2.y)));
+}
+GS_DEV zd zmul_fixed(zd a, zd b) {
+#pragma clang fp contract(off)
+  return zmk(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
 }
 
 // Copy the W x W block at src (column-major, leading dimension ld, W <= WMAX)

@@ -107,6 +107,7 @@ template <int NT, int RM = 0> GS_DEV void d_hessenberg(double* H, int64_t ld, in
         const int r = p + 1 + lane + 64 * q;
         vr[q] = (r < n) ? ((r == p + 1) ? 1.0 : H[r + p * ld]) : 0.0;
       }
+      #pragma unroll 4
       for (int j = p + 1 + wave; j < n; j += NT / 64) {
         const double va = wk[j];
 #pragma unroll
@@ -116,6 +117,7 @@ template <int NT, int RM = 0> GS_DEV void d_hessenberg(double* H, int64_t ld, in
         }
       }
     } else {
+      #pragma unroll 4
       for (int j = p + 1 + wave; j < n; j += NT / 64) {
         const double va = wk[j];
         for (int r = p + 1 + lane; r < n; r += 64) {
@@ -140,6 +142,7 @@ template <int NT, int RM = 0> GS_DEV void d_hessenberg(double* H, int64_t ld, in
       double xr[RM];
 #pragma unroll
       for (int q = 0; q < RM; ++q) xr[q] = (lane + 64 * q < n) ? wk[lane + 64 * q] : 0.0;
+      #pragma unroll 4
       for (int c = r0 + wave; c < n; c += NT / 64) {
         const double vc = H[c + p * ld];
 #pragma unroll
@@ -149,6 +152,7 @@ template <int NT, int RM = 0> GS_DEV void d_hessenberg(double* H, int64_t ld, in
         }
       }
     } else {
+      #pragma unroll 4
       for (int c = r0 + wave; c < n; c += NT / 64) {
         const double vc = H[c + p * ld];
         for (int r = lane; r < n; r += 64) H[r + (int64_t)c * ld] += wk[r] * vc;
@@ -176,6 +180,7 @@ template <int NT, int RM = 0> GS_DEV void d_form_q_inplace(double* H, int64_t ld
         const int r = c + lane + 64 * q;
         vr[q] = (r < n) ? ((r == c) ? 1.0 : H[r + (int64_t)k * ld]) : 0.0;
       }
+      #pragma unroll 4
       for (int j = c + 1 + wave; j < n; j += NT / 64) {
         const double sj = wk[j];
 #pragma unroll
@@ -185,6 +190,7 @@ template <int NT, int RM = 0> GS_DEV void d_form_q_inplace(double* H, int64_t ld
         }
       }
     } else {
+      #pragma unroll 4
       for (int j = c + 1 + wave; j < n; j += NT / 64) {
         const double sj = wk[j];
         for (int r = c + lane; r < n; r += 64) {

@@ -48,8 +48,11 @@ struct Plan {
   std::vector<Bin> bins;
   int64_t total_elems = 0;  // elements spanned by A/T/Z (max off + n^2)
   int64_t total_w = 0;
+  // Z-log offsets (complex entries, count + 1 each): normal and tiny
+  // capacities, and their totals
+  std::vector<int64_t> loff[2];
   // device metadata
-  DevBuf d_order, d_n, d_off, d_woff, d_stats, d_scl;
+  DevBuf d_order, d_n, d_off, d_woff, d_stats, d_scl, d_loff[2], d_llen;
   // captured graph for device-pointer execution
   hipGraphExec_t gexec = nullptr;
   const void* g_ptrs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -62,6 +65,9 @@ struct Plan {
     d_woff.release();
     d_stats.release();
     d_scl.release();
+    d_loff[0].release();
+    d_loff[1].release();
+    d_llen.release();
   }
 };
 
@@ -134,6 +140,7 @@ int gs_destroy(gs_handle* h) {
   h->bStats.release();
   h->bMeta.release();
   h->bScl.release();
+  h->bLog.release();
   for (int i = 0; i < kAuxBufs; ++i) h->aux[i].release();
   for (int i = 0; i < kBranches; ++i) {
     if (h->evjoin[i]) (void)hipEventDestroy(h->evjoin[i]);
@@ -261,6 +268,17 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
       load[best] += p->bins[i].work;
     }
   }
+  if (cplx) {
+    for (int t = 0; t < 2; ++t) {
+      p->loff[t].assign(count + 1, 0);
+      for (int64_t i = 0; i < count; ++i)
+        p->loff[t][i + 1] = p->loff[t][i] + (n[i] > 1 ? zlog_capacity(n[i], t == 1) : 0);
+      HIPCHK(h, p->d_loff[t].ensure(sizeof(int64_t) * (size_t)(count + 1)));
+      HIPCHK(h, hipMemcpy(p->d_loff[t].p, p->loff[t].data(), sizeof(int64_t) * (size_t)(count + 1),
+                          hipMemcpyHostToDevice));
+    }
+    HIPCHK(h, p->d_llen.ensure(sizeof(int64_t) * std::max<int64_t>(count, 1)));
+  }
   HIPCHK(h, p->d_order.ensure(sizeof(int32_t) * std::max<int64_t>(count, 1)));
   HIPCHK(h, p->d_n.ensure(sizeof(int32_t) * std::max<int64_t>(count, 1)));
   HIPCHK(h, p->d_off.ensure(sizeof(int64_t) * std::max<int64_t>(count, 1)));
@@ -280,10 +298,22 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
   return GS_OK;
 }
 
+// Smallest bin order whose QR launch logs its Z updates: measured on
+// MI355X, 1400 matrices of one order, QR + replay against in-place updates:
+// n = 96 36 ms vs 42 ms, n = 64 14.4 ms vs 13.2 ms, n = 32 equal.
+constexpr int kZlogMinN = 65;
+
+// Whether the batched QR launches of this call log their Z updates (complex,
+// windowed kernel, Z wanted, not GS_ZDIRECT).
+static bool uses_zlog(const Plan* p, int kflags) {
+  return p->is_complex && (kflags & GS_WANTZ) && !(kflags & (GS_QR_BAND | GS_ZDIRECT));
+}
+
 // With timed set, events bracket every launch on its stream (eager launches
-// only, never under capture): events 4b .. 4b+3 for bin b.
+// only, never under capture): events 6b .. 6b+5 for bin b (Hessenberg, QR,
+// Z log).  zlog: the Z-log buffer when uses_zlog.
 static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T, double* Z, double* W,
-                              int32_t* info, int flags, bool timed = false) {
+                              int32_t* info, int flags, double* zlog, bool timed = false) {
   BatchArgs a;
   a.order = static_cast<const int32_t*>(p->d_order.p);
   a.n = static_cast<const int32_t*>(p->d_n.p);
@@ -299,6 +329,14 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
   a.flags = flags;
   a.maxiter = 0;
   a.maxinner = 0;
+  a.zlog = nullptr;
+  a.loff = nullptr;
+  a.llen = nullptr;
+  const bool zl = zlog && uses_zlog(p, flags);
+  if (zl) {
+    a.loff = static_cast<const int64_t*>(p->d_loff[(flags & GS_ZLOG_TINY) ? 1 : 0].p);
+    a.llen = static_cast<int64_t*>(p->d_llen.p);
+  }
   // fork: every bin runs its Hessenberg launch then its QR launch on one of
   // kBranches streams, so the LDS left free by a large bin's workgroups is
   // filled by the smaller bins' workgroups
@@ -312,18 +350,24 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
     const Bin& b = p->bins[bi];
     a.first = b.first;
     a.count = b.count;
+    // the Z log pays off where Z's rows are long (its replay holds all of Z
+    // in LDS); GS_ZLOG_TINY exercises it on every bin
+    a.zlog = (zl && (b.nmax >= kZlogMinN || (flags & GS_ZLOG_TINY))) ? zlog : nullptr;
     hipStream_t s = h->branch[b.branch];
-    if (timed && (e = hipEventRecord(h->tev[4 * bi], s)) != hipSuccess) return e;
+    if (timed && (e = hipEventRecord(h->tev[6 * bi], s)) != hipSuccess) return e;
     e = p->is_complex ? launch_zhq(a, b.nmax, b.nt_hq, s) : launch_dhq(a, b.nmax, b.nt_hq, s);
     if (e != hipSuccess) return e;
-    if (timed && (e = hipEventRecord(h->tev[4 * bi + 1], s)) != hipSuccess) return e;
-    if (timed && (e = hipEventRecord(h->tev[4 * bi + 2], s)) != hipSuccess) return e;
+    if (timed && (e = hipEventRecord(h->tev[6 * bi + 1], s)) != hipSuccess) return e;
+    if (timed && (e = hipEventRecord(h->tev[6 * bi + 2], s)) != hipSuccess) return e;
     if (flags & GS_QR_BAND)
       e = p->is_complex ? launch_zqr(a, b.nmax, b.nt_qr, s) : launch_dqr(a, b.nmax, b.nt_qr, s);
     else
       e = p->is_complex ? launch_zqrw(a, s) : launch_dqrw(a, s);
     if (e != hipSuccess) return e;
-    if (timed && (e = hipEventRecord(h->tev[4 * bi + 3], s)) != hipSuccess) return e;
+    if (timed && (e = hipEventRecord(h->tev[6 * bi + 3], s)) != hipSuccess) return e;
+    if (timed && (e = hipEventRecord(h->tev[6 * bi + 4], s)) != hipSuccess) return e;
+    if (a.zlog && (e = launch_zlog(a, b.nmax, s)) != hipSuccess) return e;
+    if (timed && (e = hipEventRecord(h->tev[6 * bi + 5], s)) != hipSuccess) return e;
   }
   for (int i = 0; i < kBranches; ++i) {
     if (!used[i]) continue;
@@ -336,25 +380,28 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
 // One eager execution of the plan with an event pair around every launch,
 // then the per-launch records for gs_last_launches.
 static int run_timed(gs_handle* h, Plan* p, const double* A, double* T, double* Z, double* W, int32_t* info,
-                     int kflags) {
-  const size_t need = 4 * p->bins.size();
+                     int kflags, double* zlog) {
+  const size_t need = 6 * p->bins.size();
   while (h->tev.size() < need) {
     hipEvent_t ev = nullptr;
     HIPCHK(h, hipEventCreate(&ev));
     h->tev.push_back(ev);
   }
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-  hipError_t le = launch_plan(h, p, A, T, Z, W, info, kflags, true);
+  hipError_t le = launch_plan(h, p, A, T, Z, W, info, kflags, zlog, true);
   if (le != hipSuccess) return fail_hip(h, le, "kernel launch (timed)");
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->launch_recs.clear();
   for (size_t bi = 0; bi < p->bins.size(); ++bi) {
     const Bin& b = p->bins[bi];
-    for (int kind = 0; kind < 2; ++kind) {
+    // kinds 0 (Hessenberg + Q), 1 (QR sweeps), 4 (Z log applied)
+    const int kinds[3] = {0, 1, 4};
+    for (int j = 0; j < 3; ++j) {
+      if (j == 2 && !(zlog && uses_zlog(p, kflags) && (b.nmax >= kZlogMinN || (kflags & GS_ZLOG_TINY)))) continue;
       float ms = 0.f;
-      HIPCHK(h, hipEventElapsedTime(&ms, h->tev[4 * bi + 2 * kind], h->tev[4 * bi + 2 * kind + 1]));
-      const double rec[GS_LAUNCH_REC] = {(double)kind, (double)b.nmin, (double)b.nmax, (double)b.count, b.work,
+      HIPCHK(h, hipEventElapsedTime(&ms, h->tev[6 * bi + 2 * j], h->tev[6 * bi + 2 * j + 1]));
+      const double rec[GS_LAUNCH_REC] = {(double)kinds[j], (double)b.nmin, (double)b.nmax, (double)b.count, b.work,
                                          1000.0 * (double)ms};
       h->launch_recs.insert(h->launch_recs.end(), rec, rec + GS_LAUNCH_REC);
     }
@@ -382,8 +429,15 @@ static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n,
     const char* v = std::getenv("GSCHUR_REFERENCE_SWEEP");
     return v && v[0] == '1';
   }();
-  const int kflags = (flags & (GS_WANTZ | GS_SCALE | GS_QR_BAND)) | (ref_sweep ? 0x1000 : 0) | env_qr_band();
+  const int kflags = (flags & (GS_WANTZ | GS_SCALE | GS_QR_BAND | GS_ZDIRECT | GS_ZLOG_TINY)) |
+                     (ref_sweep ? 0x1000 : 0) | env_qr_band();
   const size_t es = cplx ? 16 : 8;
+  double* zlog = nullptr;
+  if (uses_zlog(p, kflags)) {
+    const int64_t entries = p->loff[(kflags & GS_ZLOG_TINY) ? 1 : 0][count];
+    HIPCHK(h, h->bLog.ensure(16 * (size_t)std::max<int64_t>(entries, 1)));
+    zlog = static_cast<double*>(h->bLog.p);
+  }
   double* dZdummy = nullptr;
   if (flags & GS_DEVICE_PTRS) {
     if (!(flags & GS_WANTZ)) {
@@ -391,8 +445,8 @@ static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n,
       dZdummy = static_cast<double*>(h->bZ.p);
     }
     double* Zd = (flags & GS_WANTZ) ? Z : dZdummy;
-    if (flags & GS_TIMING) return run_timed(h, p, A, T, Zd, W, info, kflags);
-    const void* ptrs[6] = {A, T, Zd, W, info, nullptr};
+    if (flags & GS_TIMING) return run_timed(h, p, A, T, Zd, W, info, kflags, zlog);
+    const void* ptrs[6] = {A, T, Zd, W, info, zlog};
     if (!p->gexec || std::memcmp(ptrs, p->g_ptrs, sizeof ptrs) != 0 || p->g_flags != kflags) {
       if (p->gexec) {
         (void)hipGraphExecDestroy(p->gexec);
@@ -400,7 +454,7 @@ static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n,
       }
       hipGraph_t g = nullptr;
       HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      hipError_t le = launch_plan(h, p, A, T, Zd, W, info, kflags);
+      hipError_t le = launch_plan(h, p, A, T, Zd, W, info, kflags, zlog);
       hipError_t ce = hipStreamEndCapture(h->stream, &g);
       if (le != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
@@ -433,7 +487,7 @@ static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n,
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   hipError_t le = launch_plan(h, p, static_cast<double*>(h->bA.p), static_cast<double*>(h->bT.p),
                               static_cast<double*>(h->bZ.p), static_cast<double*>(h->bW.p),
-                              static_cast<int32_t*>(h->bInfo.p), kflags);
+                              static_cast<int32_t*>(h->bInfo.p), kflags, zlog);
   if (le != hipSuccess) return fail_hip(h, le, "kernel launch");
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
   // copy back only the matrices' own elements (leave caller memory between
@@ -637,6 +691,9 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
     a.flags = kflags;
     a.maxiter = maxiter;
     a.maxinner = maxinner;
+    a.zlog = nullptr;  // one matrix: Z updated in place
+    a.loff = nullptr;
+    a.llen = nullptr;
     const int nthq = default_nt_hq(cplx, (int)n), ntqr = default_nt_qr(cplx, (int)n);
     le = cplx ? launch_zhq(a, (int)n, nthq, h->stream) : launch_dhq(a, (int)n, nthq, h->stream);
     if (le == hipSuccess) {

@@ -51,6 +51,7 @@ struct gs_handle {
   hipEvent_t evfork = nullptr, evjoin[gs::kBranches] = {};
   std::string err;
   gs::DevBuf bA, bT, bZ, bW, bInfo, bWs, bStats, bMeta, bScl;
+  gs::DevBuf bLog;  // Z log of the batched complex QR launches
   gs::DevBuf aux[gs::kAuxBufs];  // scratch of the eigenvector / VJP entry points
   std::map<std::string, std::unique_ptr<gs::Plan>> plans;
   gs::Plan* last_plan = nullptr;

@@ -29,6 +29,12 @@ struct BatchArgs {
   int flags;
   int64_t maxiter;   // <= 0: 100 n
   int64_t maxinner;  // <= 0: 30 n
+  // Z log of the complex windowed QR kernel (null: Z updated in place):
+  // matrix m's records occupy zlog[2 loff[m] .. 2 loff[m+1]) (complex
+  // entries as double pairs); the kernel stores the length used in llen[m]
+  double* zlog;
+  const int64_t* loff;
+  int64_t* llen;
 };
 
 // Arguments of the single-matrix global-memory kernels (any n).
@@ -63,6 +69,13 @@ hipError_t launch_zhq(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 hipError_t launch_zqr(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 // windowed QR launch: one wavefront per matrix, H and Z in HBM
 hipError_t launch_zqrw(const BatchArgs& a, hipStream_t s);
+hipError_t launch_zlog(const BatchArgs& a, int nmax, hipStream_t s);
+// Z-log capacity of one matrix of order n, in complex entries (tiny: one
+// sweep's record, the least the kernel accepts)
+inline int64_t zlog_capacity(int n, bool tiny) {
+  const int64_t one_sweep = 3 + 2 * (int64_t)n;
+  return tiny ? one_sweep : 6 * (int64_t)n * n + 16 * (int64_t)n + 64;
+}
 hipError_t launch_dqrw(const BatchArgs& a, hipStream_t s);
 hipError_t launch_dhq(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 hipError_t launch_dqr(const BatchArgs& a, int nmax, int nt, hipStream_t s);

@@ -127,6 +127,7 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
         const int r = p + 1 + lane + 64 * q;
         vr[q] = (r < n) ? ((r == p + 1) ? zr(1.0) : H[r + p * ld]) : zr(0.0);
       }
+      #pragma unroll 4
       for (int j = p + 1 + wave; j < n; j += NT / 64) {
         const zd va = wk[j];
 #pragma unroll
@@ -136,6 +137,7 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
         }
       }
     } else {
+      #pragma unroll 4
       for (int j = p + 1 + wave; j < n; j += NT / 64) {
         const zd va = wk[j];
         for (int r = p + 1 + lane; r < n; r += 64) {
@@ -161,6 +163,7 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
       zd xr[RM];
 #pragma unroll
       for (int q = 0; q < RM; ++q) xr[q] = (lane + 64 * q < n) ? wk[lane + 64 * q] : zr(0.0);
+      #pragma unroll 4
       for (int c = r0 + wave; c < n; c += NT / 64) {
         const zd vc = zconj(H[c + p * ld]);
 #pragma unroll
@@ -170,6 +173,7 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
         }
       }
     } else {
+      #pragma unroll 4
       for (int c = r0 + wave; c < n; c += NT / 64) {
         const zd vc = zconj(H[c + p * ld]);
         for (int r = lane; r < n; r += 64) H[r + (int64_t)c * ld] = H[r + (int64_t)c * ld] + wk[r] * vc;
@@ -203,6 +207,7 @@ template <int NT, int RM = 0> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, in
         const int r = c + lane + 64 * q;
         vr[q] = (r < n) ? ((r == c) ? zr(1.0) : H[r + (int64_t)k * ld]) : zr(0.0);
       }
+      #pragma unroll 4
       for (int j = c + 1 + wave; j < n; j += NT / 64) {
         const zd sj = wk[j];
 #pragma unroll
@@ -212,6 +217,7 @@ template <int NT, int RM = 0> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, in
         }
       }
     } else {
+      #pragma unroll 4
       for (int j = c + 1 + wave; j < n; j += NT / 64) {
         const zd sj = wk[j];
         for (int r = c + lane; r < n; r += 64) {
@@ -540,9 +546,10 @@ template <bool LEFT> GS_DEV void zchain(zd* p, int64_t st, const zd* lg, int ka,
           x[q] = a - ss;
           x[q + 1] = b - ss * v2;
         } else {
-          const zd ss = tau1 * a + tau2 * b;
-          x[q] = a - ss;
-          x[q + 1] = b - ss * zconj(v2);
+          zd x0 = a, x1 = b;
+          zrot_right_fixed(x0, x1, tau1, v2);
+          x[q] = x0;
+          x[q + 1] = x1;
         }
       }
     }
@@ -559,169 +566,52 @@ template <bool LEFT> GS_DEV void zchain(zd* p, int64_t st, const zd* lg, int ka,
   }
 }
 
-template <bool FAST> GS_DEV void z_sweep_win(zd* H, int64_t ld, int n, zd* Zg, int64_t ldz, bool wantz, zd* win,
-                                            zd* lg, int* redi, zd shift, int istart, int iend, int64_t& nstep) {
-  const int lane = threadIdx.x;
-  const double ulp = GS_EPS;
-  const FullLayout lay{ld};
-  // start row (as z_sweep)
-  int found = -1;
-  for (int mm = iend - 1 - lane; mm >= istart + 1; mm -= 64) {
-    const zd h11 = HH(mm, mm);
-    const zd h22 = HH(mm + 1, mm + 1);
-    zd h11s = h11 - shift;
-    double h21 = HH(mm + 1, mm).x;
-    const double s = zabs1(h11s) + fabs(h21);
-    h11s = h11s / s;
-    h21 = h21 / s;
-    const double h10 = HH(mm, mm - 1).x;
-    if (fabs(h10) * fabs(h21) <= ulp * (zabs1(h11s) * (zabs1(h11) + zabs1(h22)))) {
-      found = mm;
-      break;
+// Z log of the windowed QR kernel.  Z is not read while the sweeps run, so
+// instead of updating Z in HBM after every chunk the kernel records each
+// sweep and zlog_kernel applies the records to Z, held in LDS, afterwards:
+// the same operations on every element of Z in the same order.  Record of a
+// sweep, in complex entries: (istart1, iend); ctt of the phase scaling after
+// the first step, or 0; (tau1, v2) per step; t of the tail scaling, or 0.
+// When a sweep's record would not fit, the kernel first applies the records
+// so far to Z itself (zlog_row on Z in HBM) and starts the log again.
+struct ZLog {
+  zd* g;        // null: Z updated in place
+  int64_t cap;  // capacity in complex entries (at least one sweep's record)
+  int64_t pos;  // entries in use
+};
+
+// Applies log records [0, len) to one row of Z, x(c) = row[(c - 1) st]: per
+// sweep the steps as one chain (GenericSchur.jl:447-460 on Z), the phase
+// scaling after the first step (:461-482) and the tail scaling (:486-500).
+GS_DEV void zlog_row(zd* row, int64_t st, const zd* __restrict__ lg, int64_t len) {
+  int64_t pos = 0;
+  while (pos < len) {
+    const zd h0 = lg[pos];
+    const int i1 = (int)h0.x, ie = (int)h0.y;
+    const zd ctt = lg[pos + 1];
+    const zd* e = lg + pos + 2;
+    zd c0 = row[(int64_t)(i1 - 1) * st];
+    for (int k = i1; k < ie; ++k) {
+      zd x1 = row[(int64_t)k * st];
+      zrot_right_fixed(c0, x1, e[2 * (k - i1)], e[2 * (k - i1) + 1]);
+      row[(int64_t)(k - 1) * st] = c0;
+      c0 = x1;
+      if (k == i1 && !(ctt.x == 0.0 && ctt.y == 0.0)) {
+        // columns i1 .. ie except i1 + 1 (whose value rides in c0)
+        row[(int64_t)(i1 - 1) * st] = zmul_fixed(row[(int64_t)(i1 - 1) * st], ctt);
+        for (int c = i1 + 2; c <= ie; ++c) row[(int64_t)(c - 1) * st] = zmul_fixed(row[(int64_t)(c - 1) * st], ctt);
+      }
     }
+    row[(int64_t)(ie - 1) * st] = c0;
+    const zd t = e[2 * (ie - i1)];
+    if (!(t.x == 0.0 && t.y == 0.0)) row[(int64_t)(ie - 1) * st] = zmul_fixed(row[(int64_t)(ie - 1) * st], t);
+    pos += 3 + 2 * (ie - i1);
   }
-  found = block_max_int<64>(found, redi);
-  const int istart1 = (found > 0) ? found : istart;
-  zd v0, v1;
-  {
-    const zd h11 = HH(istart1, istart1);
-    zd h11s = h11 - shift;
-    double h21 = HH(istart1 + 1, istart1).x;
-    const double s = zabs1(h11s) + fabs(h21);
-    h11s = h11s / s;
-    h21 = h21 / s;
-    v0 = h11s;
-    v1 = zr(h21);
-  }
-  bool phase_pending = istart1 > istart;
-  int k0 = istart1;
-  while (k0 <= iend - 1) {
-    // the first step is a chunk of its own when the phase scaling follows it
-    const int kend = phase_pending ? k0 : min(k0 + ZKC - 1, iend - 1);
-    const int wb = max(k0 - 1, 1), we = min(kend + 2, iend);
-    const int W = we - wb + 1;
-    auto WIN = [&](int r, int c) -> zd& { return win[(r - wb) + (c - wb) * ZWL]; };
-    win_load<zd, ZW, ZWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
-    __syncthreads();
-    zd nv0 = zr(0.0), nv1 = zr(0.0);
-    for (int k = k0; k <= kend; ++k) {
-      if (k > istart1) {
-        if (k > k0) {
-          v0 = nv0;
-          v1 = nv1;
-        } else {
-          v0 = WIN(k, k - 1);
-          v1 = WIN(k + 1, k - 1);
-        }
-      }
-      const zd tau1 = FAST ? zreflector2_fast(v0, v1) : zreflector2(v0, v1);
-      wsync();
-      if (k > istart1 && lane == 0) {
-        WIN(k, k - 1) = v0;
-        WIN(k + 1, k - 1) = zr(0.0);
-      }
-      const zd v2 = v1;
-      const double tau2 = (tau1 * v2).x;
-      const zd ct1 = zconj(tau1);
-      {  // left: rows k, k+1 over the window's columns k..we
-        const int j = k + lane;
-        if (j <= we) {
-          const zd a = WIN(k, j), b = WIN(k + 1, j);
-          const zd ss = ct1 * a + tau2 * b;
-          WIN(k, j) = a - ss;
-          WIN(k + 1, j) = b - ss * v2;
-        }
-      }
-      wsync();
-      {  // right: columns k, k+1 over the window's rows wb..min(k+2, iend)
-        const int jm = min(k + 2, iend);
-        const zd cv2 = zconj(v2);
-        const int j = wb + lane;
-        zd na = zr(0.0);
-        if (j <= jm) {
-          const zd a = WIN(j, k), b = WIN(j, k + 1);
-          const zd ss = tau1 * a + tau2 * b;
-          na = a - ss;
-          WIN(j, k) = na;
-          WIN(j, k + 1) = b - ss * cv2;
-        }
-        const int l0 = (k + 1) - wb, l1 = (k + 2) - wb;
-        nv0 = zmk(rdlane(na.x, l0 & 63), rdlane(na.y, l0 & 63));
-        nv1 = zmk(rdlane(na.x, l1 & 63), rdlane(na.y, l1 & 63));
-      }
-      if (lane == 0) {
-        lg[2 * (k - k0)] = tau1;
-        lg[2 * (k - k0) + 1] = v2;
-      }
-      wsync();
-    }
-    // window back to H, then the chunk's reflectors outside the window
-    win_store<zd, ZW, ZWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
-    __syncthreads();
-    for (int col = we + 1 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, lg, k0, kend);
-    for (int row = 1 + lane; row <= wb - 1; row += 64) zchain<false>(H + (row - 1), ld, lg, k0, kend);
-    if (wantz)
-      for (int row = 1 + lane; row <= n; row += 64) zchain<false>(Zg + (row - 1), ldz, lg, k0, kend);
-    __syncthreads();
-    if (phase_pending) {
-      // GenericSchur.jl:461-482 after the first step: rows of S right of the
-      // diagonal times t, columns of S above it times conj(t), S =
-      // istart1..iend minus istart1+1; Z's columns of S times conj(t)
-      phase_pending = false;
-      zd t = zr(1.0) - lg[0];
-      t = t / zabs(t);
-      const zd ctt = zconj(t);
-      if (lane == 0) {
-        HH(istart1 + 1, istart1) = HH(istart1 + 1, istart1) * ctt;
-        if (istart1 + 2 <= iend) HH(istart1 + 2, istart1 + 1) = HH(istart1 + 2, istart1 + 1) * t;
-      }
-      __syncthreads();
-      for (int c = 1; c <= n; ++c) {
-        const bool cin = (c >= istart1 && c <= iend && c != istart1 + 1);
-        const int rmax = min(c - 1, iend);
-        for (int r = 1 + lane; r <= rmax; r += 64) {
-          const bool rin = (r >= istart1 && r != istart1 + 1);
-          if (!rin && !cin) continue;
-          zd x = HH(r, c);
-          if (rin) x = x * t;
-          if (cin) x = x * ctt;
-          HH(r, c) = x;
-        }
-      }
-      if (wantz)
-        for (int r = 1 + lane; r <= n; r += 64)
-          for (int c = istart1; c <= iend; ++c)
-            if (c != istart1 + 1) ZZ(r, c) = ZZ(r, c) * ctt;
-      __syncthreads();
-    }
-    k0 = kend + 1;
-  }
-  nstep += iend - istart1;
-  // ensure reality of the tail subdiagonal (GenericSchur.jl:486-500)
-  zd t = HH(iend, iend - 1);
-  if (t.y != 0.0) {
-    const double rtv = zabs(t);
-    t = t / rtv;
-    __syncthreads();
-    if (lane == 0) HH(iend, iend - 1) = zr(rtv);
-    const zd ctt = zconj(t);
-    for (int c = iend + 1 + lane; c <= n; c += 64) HH(iend, c) = HH(iend, c) * ctt;
-    for (int r = 1 + lane; r <= iend - 1; r += 64) HH(r, iend) = HH(r, iend) * t;
-    if (wantz)
-      for (int r = 1 + lane; r <= n; r += 64) ZZ(r, iend) = ZZ(r, iend) * t;
-  }
-  __syncthreads();
 }
 
-// Two-wavefront form of z_sweep_win: wavefront 0 chases chunk c (after
-// applying chunk c-1's left update to the columns its window adds) while
-// wavefront 1 applies chunk c-1's chains to the rest of H and to Z; the two
-// regions are disjoint and every element still receives its updates in the
-// order of the unblocked sweep.  The logs are double-buffered.  When the
-// phase scaling is due, the first step's chains are applied before it, as in
-// z_sweep_win.  Called by all 128 threads.
 template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, int64_t ldz, bool wantz, zd* win,
-                                             zd* lgbuf, int* redi, zd shift, int istart, int iend, int64_t& nstep) {
+                                             zd* lgbuf, int* redi, zd shift, int istart, int iend, int64_t& nstep,
+                                             ZLog& zl) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const double ulp = GS_EPS;
   const FullLayout lay{ld};
@@ -754,6 +644,20 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
     v1 = zr(h21);
   }
   bool phase_pending = istart1 > istart;
+  // Z log: room for this sweep's record, then its header
+  const bool zlg = wantz && zl.g;
+  const int64_t need = 3 + 2 * (int64_t)(iend - istart1);
+  if (zlg && zl.pos + need > zl.cap) {
+    __syncthreads();
+    for (int row = 1 + tid; row <= n; row += 128) zlog_row(Zg + (row - 1), ldz, zl.g, zl.pos);
+    __syncthreads();
+    zl.pos = 0;
+  }
+  zd* rec = zlg ? zl.g + zl.pos : nullptr;
+  if (zlg && tid == 0) {
+    rec[0] = zmk((double)istart1, (double)iend);
+    rec[1] = zr(0.0);
+  }
   int pk0 = 1, pkend = 0, pwe = 0, pwb = 1;  // chunk whose chains are pending (none)
   int k0 = istart1;
   for (int c = 0;; ++c) {
@@ -829,12 +733,13 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
           wsync();
         }
         win_store<zd, ZW, ZWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
+        if (zlg && lane < 2 * (kend - k0 + 1)) rec[2 + 2 * (k0 - istart1) + lane] = lg[lane];
       }
     } else if (prev) {
       const int lcol0 = (chase ? we : pwe) + 1;
       for (int col = lcol0 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend);
       for (int row = 1 + lane; row <= pwb - 1; row += 64) zchain<false>(H + (row - 1), ld, plg, pk0, pkend);
-      if (wantz)
+      if (wantz && !zlg)
         for (int row = 1 + lane; row <= n; row += 64) zchain<false>(Zg + (row - 1), ldz, plg, pk0, pkend);
     }
     __syncthreads();
@@ -845,7 +750,7 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
       if (wave == 1) {
         for (int col = we + 1 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, lg, k0, kend);
         for (int row = 1 + lane; row <= wb - 1; row += 64) zchain<false>(H + (row - 1), ld, lg, k0, kend);
-        if (wantz)
+        if (wantz && !zlg)
           for (int row = 1 + lane; row <= n; row += 64) zchain<false>(Zg + (row - 1), ldz, lg, k0, kend);
       }
       __syncthreads();
@@ -869,10 +774,13 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
           HH(r, cc) = x;
         }
       }
-      if (wantz)
+      if (zlg) {
+        if (tid == 0) rec[1] = ctt;
+      } else if (wantz) {
         for (int r = 1 + tid; r <= n; r += 128)
           for (int cc = istart1; cc <= iend; ++cc)
-            if (cc != istart1 + 1) ZZ(r, cc) = ZZ(r, cc) * ctt;
+            if (cc != istart1 + 1) ZZ(r, cc) = zmul_fixed(ZZ(r, cc), ctt);
+      }
       __syncthreads();
       pk0 = 1;
       pkend = 0;
@@ -887,7 +795,8 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
   nstep += iend - istart1;
   // tail realness (GenericSchur.jl:486-500)
   zd t = HH(iend, iend - 1);
-  if (t.y != 0.0) {
+  const bool tail = t.y != 0.0;
+  if (tail) {
     const double rtv = zabs(t);
     t = t / rtv;
     __syncthreads();
@@ -895,8 +804,12 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
     const zd ctt = zconj(t);
     for (int cc = iend + 1 + tid; cc <= n; cc += 128) HH(iend, cc) = HH(iend, cc) * ctt;
     for (int r = 1 + tid; r <= iend - 1; r += 128) HH(r, iend) = HH(r, iend) * t;
-    if (wantz)
-      for (int r = 1 + tid; r <= n; r += 128) ZZ(r, iend) = ZZ(r, iend) * t;
+    if (wantz && !zlg)
+      for (int r = 1 + tid; r <= n; r += 128) ZZ(r, iend) = zmul_fixed(ZZ(r, iend), t);
+  }
+  if (zlg) {
+    if (tid == 0) rec[need - 1] = tail ? t : zr(0.0);
+    zl.pos += need;
   }
   __syncthreads();
 }
@@ -906,7 +819,7 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
 // Returns 0, or the positive iteration count when maxiter was exceeded.
 template <int NT, class L, bool FAST, int WIN = 0> GS_DEV int64_t z_qr(zd* H, L lay, int n, zd* Zg, int64_t ldz, bool wantz, zd* rot,
                                                int* redi, int64_t maxiter, int64_t maxinner, int64_t& nsweep,
-                                               int64_t& nstep) {
+                                               int64_t& nstep, ZLog* zl = nullptr) {
   const int tid = threadIdx.x;
   const double ulp = GS_EPS;
   const double smallnum = GS_FMIN * ((double)n / ulp);
@@ -980,9 +893,7 @@ template <int NT, class L, bool FAST, int WIN = 0> GS_DEV int64_t z_qr(zd* H, L 
       }
       nsweep += 1;
       if constexpr (WIN == 2)
-        z_sweep_win2<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + ZWL * ZW, redi, t, istart, iend, nstep);
-      else if constexpr (WIN == 1)
-        z_sweep_win<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + ZWL * ZW, redi, t, istart, iend, nstep);
+        z_sweep_win2<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + ZWL * ZW, redi, t, istart, iend, nstep, *zl);
       else
         z_sweep<NT, L, FAST>(H, lay, n, Zg, ldz, wantz, rot, redi, t, istart, iend, nstep);
     }
@@ -1096,41 +1007,6 @@ template <int NT, bool FAST> __global__ void __launch_bounds__(NT) zqr_kernel(Ba
   z_finish<NT>(H, lay, n, T, (int64_t)n, W, a.scl[2 * m], a.scl[2 * m + 1]);
 }
 
-// Batched kernel 2, windowed form: one wavefront per matrix, H in HBM (the
-// T buffer, written by zhq_kernel) and Z in HBM; LDS holds the chase window
-// and the chunk's reflector log only.
-template <bool FAST> __global__ void __launch_bounds__(64) zqrw_kernel(BatchArgs a) {
-  __shared__ zd win[ZWL * ZW + 2 * ZKC];
-  __shared__ int redi[2];
-  const int b = blockIdx.x;
-  if (b >= a.count) return;
-  const int m = a.order[a.first + b];
-  const int n = a.n[m];
-  if (n <= 0) return;
-  if (a.info[m] != 0) return;  // rejected by zhq_kernel
-  const int64_t o = a.off[m];
-  zd* T = reinterpret_cast<zd*>(a.T) + o;
-  zd* Z = reinterpret_cast<zd*>(a.Z) + o;
-  zd* W = reinterpret_cast<zd*>(a.W) + a.woff[m];
-  const bool wantz = (a.flags & 0x01) != 0;
-  const FullLayout lay{(int64_t)n};
-  int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
-  int64_t maxinner = a.maxinner > 0 ? a.maxinner : 30 * (int64_t)n;
-  int64_t nsweep = 0, nstep = 0;
-  const int64_t st =
-      z_qr<64, FullLayout, FAST, 1>(T, lay, n, Z, (int64_t)n, wantz, win, redi, maxiter, maxinner, nsweep, nstep);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (a.stats) {
-      a.stats[2 * m] = nsweep;
-      a.stats[2 * m + 1] = nstep;
-    }
-    a.info[m] = (int32_t)(st > 0x7fffffff ? 0x7fffffff : st);
-  }
-  if (st != 0) return;
-  z_finish<64>(T, lay, n, T, (int64_t)n, W, a.scl[2 * m], a.scl[2 * m + 1]);
-}
-
 // Two-wavefront windowed QR kernel: the chase of chunk c overlaps the chains
 // of chunk c-1 (z_sweep_win2); the control logic runs on both wavefronts.
 template <bool FAST> __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) zqrw2_kernel(BatchArgs a) {
@@ -1143,6 +1019,7 @@ template <bool FAST> __global__ void __launch_bounds__(128) __attribute__((amdgp
   if (!(a.flags & 0x4000) && threadIdx.x < 64) __builtin_amdgcn_s_setprio(2);
   const int m = a.order[a.first + b];
   const int n = a.n[m];
+  if (a.zlog && threadIdx.x == 0) a.llen[m] = 0;
   if (n <= 0) return;
   if (a.info[m] != 0) return;
   const int64_t o = a.off[m];
@@ -1154,10 +1031,16 @@ template <bool FAST> __global__ void __launch_bounds__(128) __attribute__((amdgp
   int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
   int64_t maxinner = a.maxinner > 0 ? a.maxinner : 30 * (int64_t)n;
   int64_t nsweep = 0, nstep = 0;
-  const int64_t st =
-      z_qr<128, FullLayout, FAST, 2>(T, lay, n, Z, (int64_t)n, wantz, win, redi, maxiter, maxinner, nsweep, nstep);
+  ZLog zl{nullptr, 0, 0};
+  if (a.zlog && wantz) {
+    zl.g = reinterpret_cast<zd*>(a.zlog) + a.loff[m];
+    zl.cap = a.loff[m + 1] - a.loff[m];
+  }
+  const int64_t st = z_qr<128, FullLayout, FAST, 2>(T, lay, n, Z, (int64_t)n, wantz, win, redi, maxiter, maxinner,
+                                                    nsweep, nstep, &zl);
   __syncthreads();
   if (threadIdx.x == 0) {
+    if (zl.g) a.llen[m] = zl.pos;
     if (a.stats) {
       a.stats[2 * m] = nsweep;
       a.stats[2 * m + 1] = nstep;
@@ -1289,22 +1172,112 @@ hipError_t launch_zqr(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
 }
 
 hipError_t launch_zqrw(const BatchArgs& a, hipStream_t s) {
-  // GSCHUR_QR_ONEWAVE=1: the single-wavefront windowed kernel (A/B)
-  static const bool one = [] {
-    const char* e = std::getenv("GSCHUR_QR_ONEWAVE");
-    return e && e[0] == '1';
-  }();
-  if (one) {
-    if (a.flags & 0x1000)
-      hipLaunchKernelGGL((zqrw_kernel<false>), dim3(a.count), dim3(64), 0, s, a);
-    else
-      hipLaunchKernelGGL((zqrw_kernel<true>), dim3(a.count), dim3(64), 0, s, a);
-  } else {
-    if (a.flags & 0x1000)
-      hipLaunchKernelGGL((zqrw2_kernel<false>), dim3(a.count), dim3(128), 0, s, a);
-    else
-      hipLaunchKernelGGL((zqrw2_kernel<true>), dim3(a.count), dim3(128), 0, s, a);
+  if (a.count <= 0) return hipSuccess;
+  if (a.flags & 0x1000)
+    hipLaunchKernelGGL((zqrw2_kernel<false>), dim3(a.count), dim3(128), 0, s, a);
+  else
+    hipLaunchKernelGGL((zqrw2_kernel<true>), dim3(a.count), dim3(128), 0, s, a);
+  return hipGetLastError();
+}
+
+// After a bin's QR launch: each matrix's Z log applied to Z by one
+// workgroup of ceil(n / 64) wavefronts, all of Z held in LDS (row r, column
+// c at zs[(c - 1) n + r]), each thread replaying the log on one row with
+// zlog_row's operations.  Each wavefront stages a sweep's record in its own
+// LDS buffer and loads the next record into registers while it replays the
+// current one, so the wavefronts never wait for each other.
+constexpr int ZLREC = 200;  // >= 3 + 2 (n - 1) for every batched complex order
+__global__ void __launch_bounds__(128) zlog_kernel(BatchArgs a) {
+  extern __shared__ zd zs[];
+  __shared__ zd rbuf[2][ZLREC];  // a separate array: its reads do not wait on the row writes
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int b = blockIdx.x;
+  if (b >= a.count) return;
+  const int m = a.order[a.first + b];
+  const int n = a.n[m];
+  if (n <= 0 || wave * 64 >= n) return;
+  const int64_t len = a.llen[m];
+  if (len <= 0) return;
+  zd* Z = reinterpret_cast<zd*>(a.Z) + a.off[m];
+  const zd* lg = reinterpret_cast<const zd*>(a.zlog) + a.loff[m];
+  const int r = min(tid, n - 1);  // threads past the last row repeat it, unstored
+  const bool live = tid < n;
+  zd t[4];
+  auto fetch = [&](int64_t p) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) t[j] = lg[min(p + lane + 64 * j, len - 1)];
+  };
+  fetch(0);
+  for (int c0 = 0; c0 < n; c0 += 8) {
+    zd v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = Z[r + (int64_t)min(c0 + i, n - 1) * n];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (c0 + i < n && live) zs[(c0 + i) * n + r] = v[i];
   }
+  zd* row = zs + r;
+  const int st = n;
+  zd* rec = rbuf[wave];
+  for (int64_t pos = 0; pos < len;) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (lane + 64 * j < ZLREC) rec[lane + 64 * j] = t[j];
+    wsync();
+    const zd h0 = rec[0];
+    const int i1 = __builtin_amdgcn_readfirstlane((int)h0.x), ie = __builtin_amdgcn_readfirstlane((int)h0.y);
+    const int64_t rl = 3 + 2 * (int64_t)(ie - i1);
+    if (pos + rl < len) fetch(pos + rl);
+    if (live) {
+      const zd ctt = rec[1];
+      const zd* e = rec + 2;
+      zd c0 = row[(i1 - 1) * st];
+      {  // first step, then the phase scaling when logged
+        zd x1 = row[i1 * st];
+        zrot_right_fixed(c0, x1, e[0], e[1]);
+        row[(i1 - 1) * st] = c0;
+        c0 = x1;
+        if (!(ctt.x == 0.0 && ctt.y == 0.0)) {
+          row[(i1 - 1) * st] = zmul_fixed(row[(i1 - 1) * st], ctt);
+#pragma unroll 4
+          for (int c = i1 + 2; c <= ie; ++c) row[(c - 1) * st] = zmul_fixed(row[(c - 1) * st], ctt);
+        }
+      }
+#pragma unroll 4
+      for (int k = i1 + 1; k < ie; ++k) {
+        zd x1 = row[k * st];
+        zrot_right_fixed(c0, x1, e[2 * (k - i1)], e[2 * (k - i1) + 1]);
+        row[(k - 1) * st] = c0;
+        c0 = x1;
+      }
+      row[(ie - 1) * st] = c0;
+      const zd tt = e[2 * (ie - i1)];
+      if (!(tt.x == 0.0 && tt.y == 0.0)) row[(ie - 1) * st] = zmul_fixed(row[(ie - 1) * st], tt);
+    }
+    wsync();  // the record buffer is rewritten for the next sweep
+    pos += rl;
+  }
+  if (live) {
+    for (int c0 = 0; c0 < n; c0 += 8) {
+      zd v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = zs[min(c0 + i, n - 1) * n + r];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (c0 + i < n) Z[r + (int64_t)(c0 + i) * n] = v[i];
+    }
+  }
+}
+
+static size_t zlog_lds_bytes(int nmax) { return (size_t)nmax * nmax * sizeof(zd); }
+
+hipError_t launch_zlog(const BatchArgs& a, int nmax, hipStream_t s) {
+  if (a.count <= 0 || !a.zlog) return hipSuccess;
+  if (3 + 2 * (nmax - 1) > ZLREC || nmax > 128) return hipErrorInvalidValue;
+  const int waves = (nmax + 63) / 64;
+  const size_t lds = zlog_lds_bytes(nmax);
+  (void)hipFuncSetAttribute((const void*)zlog_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(zlog_kernel, dim3(a.count), dim3(64 * waves), lds, s, a);
   return hipGetLastError();
 }
 

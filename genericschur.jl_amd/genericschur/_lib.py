@@ -19,6 +19,8 @@ GS_ASYNC = 0x20
 GS_PLAIN = 0x40
 GS_TIMING = 0x80
 GS_QR_BAND = 0x100
+GS_ZDIRECT = 0x200
+GS_ZLOG_TINY = 0x400
 GS_LAUNCH_REC = 6
 
 GS_OK = 0
@@ -155,7 +157,8 @@ def last_stats(device=0):
 
 def last_launches(device=0):
     """Records of the last call made with GS_TIMING: a list of dicts.  Batched
-    calls give kind 'hq' / 'qr' (nmin, nmax, count, sum_n3, us); the large
+    calls give kind 'hq' / 'qr' / 'zlog' (Z log applied after the complex QR
+    sweeps) with (nmin, nmax, count, sum_n3, us); the large
     single-matrix path gives 'gemm' (m, n, k, flops, us) and 'phase' (phase:
     0 copy+scale, 1 Hessenberg, 2 Q formation, 3 QR, 4 finish; us)."""
     h = handle(device)
@@ -168,8 +171,8 @@ def last_launches(device=0):
     for r in range(k):
         v = buf[r * GS_LAUNCH_REC:(r + 1) * GS_LAUNCH_REC]
         kind = int(v[0])
-        if kind in (0, 1):
-            recs.append({"kind": "qr" if kind == 1 else "hq", "nmin": int(v[1]), "nmax": int(v[2]),
+        if kind in (0, 1, 4):
+            recs.append({"kind": {0: "hq", 1: "qr", 4: "zlog"}[kind], "nmin": int(v[1]), "nmax": int(v[2]),
                          "count": int(v[3]), "sum_n3": v[4], "us": v[5]})
         elif kind == 2:
             recs.append({"kind": "gemm", "m": int(v[1]), "n": int(v[2]), "k": int(v[3]), "flops": v[4],

@@ -273,11 +273,14 @@ def eig_offsets(ns):
     return w
 
 
-def gschur_batched(mats, wantZ=True, scale=True, device=0, raise_on_info=True, qr_band=False):
+def gschur_batched(mats, wantZ=True, scale=True, device=0, raise_on_info=True, qr_band=False, zdirect=False,
+                   zlog_tiny=False):
     """Batched gschur! of host matrices (all Float64 or all ComplexF64) in one
     graph launch.  Returns a list of Schur (and the info array).  qr_band
     selects the band-resident complex QR kernel instead of the windowed one
-    (same arithmetic; for comparisons)."""
+    (same arithmetic; for comparisons).  Complex, windowed: zdirect updates Z
+    during the sweeps instead of through the Z log; zlog_tiny gives each
+    matrix a log of one sweep, so the QR kernel applies it itself (tests)."""
     mats = [np.asarray(m) for m in mats]
     if not mats:
         return [], np.zeros(0, dtype=np.int32)
@@ -293,6 +296,7 @@ def gschur_batched(mats, wantZ=True, scale=True, device=0, raise_on_info=True, q
     W = np.zeros(int(ns.sum()), dtype=np.complex128)
     info = np.zeros(len(mats), dtype=np.int32)
     flags = (GS_WANTZ if wantZ else 0) | (GS_SCALE if scale else 0) | (_lib.GS_QR_BAND if qr_band else 0)
+    flags |= (_lib.GS_ZDIRECT if zdirect else 0) | (_lib.GS_ZLOG_TINY if zlog_tiny else 0)
     h = _lib.handle(device)
     f = _lib.lib().gs_schur_batched_z if cplx else _lib.lib().gs_schur_batched_d
     rc = f(h, len(mats), _ptr(ns), _ptr(off), _ptr(flat), _ptr(T), _ptr(Z), _ptr(W), _ptr(info), flags)
@@ -315,7 +319,7 @@ def _is_torch(x):
 
 
 def gschur_batched_device(ns, off, A, T, Z, W, info, complex_input, wantZ=True, scale=True, asynchronous=False,
-                          device=0, timing=False):
+                          device=0, timing=False, zdirect=False):
     """Batched gschur! on device-resident packed data (torch CUDA tensors or
     raw device addresses).  ns/off are host int32/int64 arrays.  With timing,
     the batch runs eagerly with events around each launch (read them with
@@ -333,6 +337,8 @@ def gschur_batched_device(ns, off, A, T, Z, W, info, complex_input, wantZ=True, 
         flags |= _lib.GS_ASYNC
     if timing:
         flags |= _lib.GS_TIMING
+    if zdirect:
+        flags |= _lib.GS_ZDIRECT
     h = _lib.handle(device)
     f = _lib.lib().gs_schur_batched_z if complex_input else _lib.lib().gs_schur_batched_d
     rc = f(h, len(ns), _ptr(ns), _ptr(off), addr(A), addr(T), addr(Z), addr(W), addr(info), flags)

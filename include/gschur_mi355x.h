@@ -53,6 +53,14 @@ typedef struct gs_handle gs_handle;
                              /* (up to 80 KB per matrix) instead of the        */
                              /* default windowed kernel (H in HBM, a 32 x 32   */
                              /* chase window in LDS); same arithmetic           */
+#define GS_ZDIRECT 0x200     /* batched complex: apply each chunk's reflectors */
+                             /* to Z in HBM during the QR sweeps instead of    */
+                             /* logging them and applying the log to Z, held   */
+                             /* in LDS, after the sweeps; same arithmetic and  */
+                             /* order (comparisons)                            */
+#define GS_ZLOG_TINY 0x400   /* batched complex, testing: a Z log of one sweep */
+                             /* per matrix, so that the QR kernel applies the  */
+                             /* log to Z itself before almost every sweep      */
 #define GS_TIMING 0x80       /* record launch durations for gs_last_launches:  */
                              /* batched with GS_DEVICE_PTRS (launched eagerly, */
                              /* no graph, events around every kernel on its    */
@@ -168,6 +176,8 @@ int gs_last_stats(const gs_handle* h, int64_t* out, int len);
  *   kind 2      GEMM of the large real path: a, b, c = m, n, k, d = 2 m n k
  *   kind 3      phase of the large real path: a = 0 copy + scale,
  *               1 Hessenberg, 2 Q formation, 3 QR iterations, 4 finish
+ *   kind 4      batched complex: the Z log applied after a bin's QR kernel
+ *               (fields as kind 0 / 1)
  * Writes at most max_records records and returns the number available. */
 #define GS_LAUNCH_REC 6
 int gs_last_launches(const gs_handle* h, double* out, int max_records);

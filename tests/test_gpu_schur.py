@@ -105,6 +105,29 @@ def test_batched_matches_single(gs):
         assert np.array_equal(S.T, S1.T) and np.array_equal(S.Z, S1.Z) and np.array_equal(S.values, S1.values)
 
 
+def test_zlog_matches_direct(gs, oracle):
+    """The Z log: Z updated after the QR sweeps from the logged reflectors
+    and scalings (default), during the sweeps (GS_ZDIRECT), and with a log of
+    one sweep, so that the QR kernel applies it to Z in HBM before nearly
+    every sweep (GS_ZLOG_TINY).  T does not depend on how Z is updated, and
+    the three forms apply the same operations to Z in the same order with
+    the same rounding: all bitwise equal.  The log form also passes the
+    Schur invariants."""
+    eps = np.finfo(np.float64).eps
+    mats = gen.workload2(seed=7, count=64)
+    mats += [gen.toeplitz_grcar(40).astype(np.complex128), np.eye(12, dtype=np.complex128)]
+    Sl, il = gs.gschur_batched(mats)
+    Sd, idd = gs.gschur_batched(mats, zdirect=True)
+    St, it = gs.gschur_batched(mats, zlog_tiny=True)
+    assert np.all(il == 0) and np.all(idd == 0) and np.all(it == 0)
+    for A, a, d, t in zip(mats, Sl, Sd, St):
+        n = A.shape[0]
+        assert np.array_equal(a.T, d.T) and np.array_equal(a.T, t.T)
+        assert np.array_equal(a.Z, t.Z) and np.array_equal(a.Z, d.Z)
+        checks.schur_invariants(A, a.T, a.Z, a.values, 20.0)
+        assert checks.rel_backward(A, a.T, a.Z) <= 50 * n * eps
+
+
 def test_windowed_matches_band_qr(gs):
     """The windowed QR kernels (H in HBM, chase window in LDS, chained updates
     outside the window) apply the same reflectors in the same order as the
