@@ -171,6 +171,41 @@ template <int NT, class F> GS_DEV void zdots4(int i0, int i1, int k0, int k1, F 
     if (s == 0 && i < i1) out[i] = acc;
   }
 }
+// As zdots4, with the output handed to epi(i, sum) on the lane that holds it.
+template <int NT, class F, class E> GS_DEV void zdots4e(int i0, int i1, int k0, int k1, F f, E epi) {
+  const int s = threadIdx.x & 3, il = threadIdx.x >> 2;
+  const int seg = (k1 - k0 + 3) >> 2;
+  const int ks = k0 + s * seg, ke = min(ks + seg, k1);
+  for (int ib = i0; ib < i1; ib += NT / 4) {
+    const int i = ib + il;
+    zd acc = zr(0.0);
+    if (i < i1) {
+#pragma unroll 4
+      for (int k = ks; k < ke; ++k) acc = acc + f(k, i);
+    }
+    acc.x += __shfl_xor(acc.x, 1, 64);
+    acc.y += __shfl_xor(acc.y, 1, 64);
+    acc.x += __shfl_xor(acc.x, 2, 64);
+    acc.y += __shfl_xor(acc.y, 2, 64);
+    if (s == 0 && i < i1) epi(i, acc);
+  }
+}
+template <int NT, class F, class E> GS_DEV void ddots4e(int i0, int i1, int k0, int k1, F f, E epi) {
+  const int s = threadIdx.x & 3, il = threadIdx.x >> 2;
+  const int seg = (k1 - k0 + 3) >> 2;
+  const int ks = k0 + s * seg, ke = min(ks + seg, k1);
+  for (int ib = i0; ib < i1; ib += NT / 4) {
+    const int i = ib + il;
+    double acc = 0.0;
+    if (i < i1) {
+#pragma unroll 4
+      for (int k = ks; k < ke; ++k) acc += f(k, i);
+    }
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    if (s == 0 && i < i1) epi(i, acc);
+  }
+}
 template <int NT, class F> GS_DEV void ddots4(int i0, int i1, int k0, int k1, F f, double* out) {
   const int s = threadIdx.x & 3, il = threadIdx.x >> 2;
   const int seg = (k1 - k0 + 3) >> 2;

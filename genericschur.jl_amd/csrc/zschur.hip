@@ -40,85 +40,90 @@ namespace gs {
 // reflector and row factors of its RM rows in registers in the rank-1
 // updates; RM = 0: any n.
 template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n, zd* tau, double* red, zd* wk) {
+  (void)red;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const double sfmin = GS_FMIN / GS_EPS;
   for (int p = 0; p <= n - 2; ++p) {
-    // reflector on x = H[p+1 : n-1, p]   (householder.jl:56-102)
     const int r0 = p + 2;  // first tail row
-    double amax = 0.0;
-    for (int r = r0 + tid; r < n; r += NT) {
-      const zd v = H[r + p * ld];
-      amax = fmax(amax, fmax(fabs(v.x), fabs(v.y)));
-    }
-    amax = block_max<NT>(amax, red);
-    double xnorm = 0.0;
-    if (amax > 0.0) {
-      const double ra = 1.0 / amax;
-      double s = 0.0;
-      for (int r = r0 + tid; r < n; r += NT) {
+    // reflector on x = H[p+1 : n-1, p] (householder.jl:56-102), computed by
+    // wavefront 0 alone: its reductions need no workgroup barrier
+    if (wave == 0) {
+      double amax = 0.0;
+      for (int r = r0 + lane; r < n; r += 64) {
         const zd v = H[r + p * ld];
-        const double a = v.x * ra, b = v.y * ra;
-        s += a * a + b * b;
+        amax = fmax(amax, fmax(fabs(v.x), fabs(v.y)));
       }
-      s = block_sum<NT>(s, red);
-      xnorm = amax * sqrt(s);
-    }
-    zd alpha = H[(p + 1) + p * ld];
-    zd t_ = zr(0.0);
-    if (!(xnorm == 0.0 && alpha.y == 0.0)) {
-      double ar = alpha.x, ai = alpha.y;
-      double beta = -copysign(hypot3(ar, ai, xnorm), ar);
-      int kount = 0;
-      if (fabs(beta) < sfmin) {
-        const double rsfmin = 1.0 / sfmin;
-        bool smallb = true;
-        while (smallb) {
-          kount += 1;
-          bsync<NT>();
-          for (int r = r0 + tid; r < n; r += NT) H[r + p * ld] = H[r + p * ld] * rsfmin;
-          beta *= rsfmin;
-          ar *= rsfmin;
-          ai *= rsfmin;
-          smallb = (fabs(beta) < sfmin) && (kount < 20);
-        }
-        __syncthreads();
-        double am2 = 0.0;
-        for (int r = r0 + tid; r < n; r += NT) {
+      amax = wave_max(amax);
+      double xnorm = 0.0;
+      if (amax > 0.0) {
+        const double ra = 1.0 / amax;
+        double s = 0.0;
+        for (int r = r0 + lane; r < n; r += 64) {
           const zd v = H[r + p * ld];
-          am2 = fmax(am2, fmax(fabs(v.x), fabs(v.y)));
+          const double a = v.x * ra, b = v.y * ra;
+          s += a * a + b * b;
         }
-        am2 = block_max<NT>(am2, red);
-        xnorm = 0.0;
-        if (am2 > 0.0) {
-          const double ra = 1.0 / am2;
-          double s = 0.0;
-          for (int r = r0 + tid; r < n; r += NT) {
-            const zd v = H[r + p * ld];
-            const double a = v.x * ra, b = v.y * ra;
-            s += a * a + b * b;
-          }
-          s = block_sum<NT>(s, red);
-          xnorm = am2 * sqrt(s);
-        }
-        alpha = zmk(ar, ai);
-        beta = -copysign(hypot3(ar, ai, xnorm), ar);
+        s = wave_sum(s);
+        xnorm = amax * sqrt(s);
       }
-      t_ = zmk((beta - ar) / beta, -ai / beta);
-      const zd sc = zdiv(zr(1.0), alpha - zr(beta));
-      __syncthreads();
-      for (int r = r0 + tid; r < n; r += NT) H[r + p * ld] = H[r + p * ld] * sc;
-      for (int j = 0; j < kount; ++j) beta *= sfmin;
-      if (tid == 0) H[(p + 1) + p * ld] = zr(beta);
+      zd alpha = H[(p + 1) + p * ld];
+      zd t_ = zr(0.0);
+      if (!(xnorm == 0.0 && alpha.y == 0.0)) {
+        double ar = alpha.x, ai = alpha.y;
+        double beta = -copysign(hypot3(ar, ai, xnorm), ar);
+        int kount = 0;
+        if (fabs(beta) < sfmin) {
+          const double rsfmin = 1.0 / sfmin;
+          bool smallb = true;
+          while (smallb) {
+            kount += 1;
+            wsync();
+            for (int r = r0 + lane; r < n; r += 64) H[r + p * ld] = H[r + p * ld] * rsfmin;
+            beta *= rsfmin;
+            ar *= rsfmin;
+            ai *= rsfmin;
+            smallb = (fabs(beta) < sfmin) && (kount < 20);
+          }
+          wsync();
+          double am2 = 0.0;
+          for (int r = r0 + lane; r < n; r += 64) {
+            const zd v = H[r + p * ld];
+            am2 = fmax(am2, fmax(fabs(v.x), fabs(v.y)));
+          }
+          am2 = wave_max(am2);
+          xnorm = 0.0;
+          if (am2 > 0.0) {
+            const double ra = 1.0 / am2;
+            double s = 0.0;
+            for (int r = r0 + lane; r < n; r += 64) {
+              const zd v = H[r + p * ld];
+              const double a = v.x * ra, b = v.y * ra;
+              s += a * a + b * b;
+            }
+            s = wave_sum(s);
+            xnorm = am2 * sqrt(s);
+          }
+          alpha = zmk(ar, ai);
+          beta = -copysign(hypot3(ar, ai, xnorm), ar);
+        }
+        t_ = zmk((beta - ar) / beta, -ai / beta);
+        const zd sc = zdiv(zr(1.0), alpha - zr(beta));
+        for (int r = r0 + lane; r < n; r += 64) H[r + p * ld] = H[r + p * ld] * sc;
+        for (int j = 0; j < kount; ++j) beta *= sfmin;
+        if (lane == 0) H[(p + 1) + p * ld] = zr(beta);
+      }
+      if (lane == 0) tau[p] = t_;
     }
-    if (tid == 0) tau[p] = t_;
     __syncthreads();
+    const zd t_ = tau[p];
     const zd ct = zconj(t_);
     // lmul!(H', A[p+1:n, p+1:n])   (householder.jl:157-172):
     // va_j = ct (H[p+1, j] + sum_r conj(v_r) H[r, j]); column j -= va_j [1; v]
-    zdots4<NT>(p + 1, n, r0, n, [&](int r, int j) { return zconj(H[r + p * ld]) * H[r + (int64_t)j * ld]; }, wk);
-    __syncthreads();
-    for (int j = p + 1 + tid; j < n; j += NT) wk[j] = ct * (H[(p + 1) + (int64_t)j * ld] + wk[j]);
+    // (the dot products leave row p+1 alone, so the lane holding va_j's sum
+    // finishes it at once)
+    zdots4e<NT>(p + 1, n, r0, n, [&](int r, int j) { return zconj(H[r + p * ld]) * H[r + (int64_t)j * ld]; },
+                [&](int j, zd acc) { wk[j] = ct * (H[(p + 1) + (int64_t)j * ld] + acc); });
     __syncthreads();
     if constexpr (RM > 0) {
       zd vr[RM];
@@ -127,7 +132,7 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
         const int r = p + 1 + lane + 64 * q;
         vr[q] = (r < n) ? ((r == p + 1) ? zr(1.0) : H[r + p * ld]) : zr(0.0);
       }
-      #pragma unroll 4
+#pragma unroll 4
       for (int j = p + 1 + wave; j < n; j += NT / 64) {
         const zd va = wk[j];
 #pragma unroll
@@ -137,7 +142,7 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
         }
       }
     } else {
-      #pragma unroll 4
+#pragma unroll 4
       for (int j = p + 1 + wave; j < n; j += NT / 64) {
         const zd va = wk[j];
         for (int r = p + 1 + lane; r < n; r += 64) {
@@ -149,21 +154,21 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
     __syncthreads();
     // rmul!(A[:, p+1:n], H, x)     (householder.jl:140-155, util.jl:103-114):
     // x_r = H[r, p+1] + sum_c H[r, c] v_c; H[r, p+1] -= tau x_r;
-    // H[r, c] += (-tau x_r) conj(v_c)
+    // H[r, c] += (-tau x_r) conj(v_c) (the dot products leave column p+1
+    // alone, so the lane holding x_r's sum finishes it at once)
     const zd mt = -t_;
-    zdots4<NT>(0, n, r0, n, [&](int c, int r) { return H[r + (int64_t)c * ld] * H[c + p * ld]; }, wk);
-    __syncthreads();
-    for (int r = tid; r < n; r += NT) {
-      const zd x = H[r + (int64_t)(p + 1) * ld] + wk[r];
-      H[r + (int64_t)(p + 1) * ld] = H[r + (int64_t)(p + 1) * ld] - t_ * x;
-      wk[r] = x * mt;
-    }
+    zdots4e<NT>(0, n, r0, n, [&](int c, int r) { return H[r + (int64_t)c * ld] * H[c + p * ld]; },
+                [&](int r, zd acc) {
+                  const zd x = H[r + (int64_t)(p + 1) * ld] + acc;
+                  H[r + (int64_t)(p + 1) * ld] = H[r + (int64_t)(p + 1) * ld] - t_ * x;
+                  wk[r] = x * mt;
+                });
     __syncthreads();
     if constexpr (RM > 0) {
       zd xr[RM];
 #pragma unroll
       for (int q = 0; q < RM; ++q) xr[q] = (lane + 64 * q < n) ? wk[lane + 64 * q] : zr(0.0);
-      #pragma unroll 4
+#pragma unroll 4
       for (int c = r0 + wave; c < n; c += NT / 64) {
         const zd vc = zconj(H[c + p * ld]);
 #pragma unroll
@@ -173,7 +178,7 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
         }
       }
     } else {
-      #pragma unroll 4
+#pragma unroll 4
       for (int c = r0 + wave; c < n; c += NT / 64) {
         const zd vc = zconj(H[c + p * ld]);
         for (int r = lane; r < n; r += 64) H[r + (int64_t)c * ld] = H[r + (int64_t)c * ld] + wk[r] * vc;
@@ -186,7 +191,9 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
 // ------------------------------------------------------------------------
 // Q = diag(1, H_0 H_1 ... H_{n-2}) formed in place over the reflector
 // storage (hessenberg.jl:150-166; the unblocked backward accumulation of
-// xUNG2R on the reflectors shifted one column right).
+// xUNG2R on the reflectors shifted one column right).  Per reflector: the
+// dot products (finished by the lanes holding them), then the rank-1
+// update of columns c+1.. together with column c itself (disjoint columns).
 template <int NT, int RM = 0> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, int n, const zd* tau, zd* wk) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -195,10 +202,9 @@ template <int NT, int RM = 0> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, in
     const zd t = tau[k];
     // apply H_k to Q(c:n-1, c+1:n-1); Q(c, j>c) is zero at this point:
     // s_j = t (Q[c, j] + sum_{r>c} conj(v_r) Q[r, j]); column j -= s_j [1; v]
-    zdots4<NT>(c + 1, n, c + 1, n, [&](int r, int j) { return zconj(H[r + (int64_t)k * ld]) * H[r + (int64_t)j * ld]; },
-               wk);
-    __syncthreads();
-    for (int j = c + 1 + tid; j < n; j += NT) wk[j] = t * (H[c + (int64_t)j * ld] + wk[j]);
+    zdots4e<NT>(c + 1, n, c + 1, n,
+                [&](int r, int j) { return zconj(H[r + (int64_t)k * ld]) * H[r + (int64_t)j * ld]; },
+                [&](int j, zd acc) { wk[j] = t * (H[c + (int64_t)j * ld] + acc); });
     __syncthreads();
     if constexpr (RM > 0) {
       zd vr[RM];
@@ -207,7 +213,7 @@ template <int NT, int RM = 0> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, in
         const int r = c + lane + 64 * q;
         vr[q] = (r < n) ? ((r == c) ? zr(1.0) : H[r + (int64_t)k * ld]) : zr(0.0);
       }
-      #pragma unroll 4
+#pragma unroll 4
       for (int j = c + 1 + wave; j < n; j += NT / 64) {
         const zd sj = wk[j];
 #pragma unroll
@@ -217,7 +223,7 @@ template <int NT, int RM = 0> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, in
         }
       }
     } else {
-      #pragma unroll 4
+#pragma unroll 4
       for (int j = c + 1 + wave; j < n; j += NT / 64) {
         const zd sj = wk[j];
         for (int r = c + lane; r < n; r += 64) {
@@ -226,8 +232,8 @@ template <int NT, int RM = 0> GS_DEV void z_form_q_inplace(zd* H, int64_t ld, in
         }
       }
     }
-    __syncthreads();
-    // column c: -tau * v below, 1 - tau on the diagonal, zero above
+    // column c: -tau * v below, 1 - tau on the diagonal, zero above (reads
+    // column k, which the rank-1 update only reads)
     for (int r = tid; r < n; r += NT) {
       zd q;
       if (r < c) q = zr(0.0);
