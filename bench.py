@@ -253,18 +253,7 @@ def main():
                          "gs_schur_z call each", "batched_over_dropin": value / rate}
 
     if not args.no_cpu and args.cpu_sample > 0:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle
-        m = min(args.cpu_sample, len(mats))
-        sub = mats[:m]
-        sn, so, sf = gs.pack(sub, np.complex128)
-        cores = min(16, os.cpu_count() or 1)
-        tc = time.perf_counter()
-        _, _, _, cinfo = pyoracle.schur_batched(sn, so, sf, True, nthreads=cores)
-        dt = time.perf_counter() - tc
-        out["cpu_baseline"] = {"value": m / dt, "unit": "matrices/s", "cores": cores, "kind": "port",
-                               "sample": f"first {m} matrices of the same workload; C++ restatement of gschur! "
-                                         f"(oracle/), OpenMP over matrices, {cores} threads, {dt:.1f}s"}
+        out["cpu_baseline"] = _cpu_batched(mats, True, min(args.cpu_sample, len(mats)))
     print(json.dumps(out), flush=True)
 
 
@@ -345,20 +334,40 @@ def bench_workload3(args):
     print(json.dumps(out), flush=True)
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def _cpu_batched(mats, cplx, m):
     """The oracle (C++ restatement of gschur!) over the first m matrices,
-    OpenMP over matrices on up to 16 host cores."""
+    OpenMP over matrices on up to 16 host cores, and single-threaded over
+    the first m/16 of them (the reference is a single-thread algorithm)."""
     import genericschur as gs
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    sn, so, sf = gs.pack(mats[:m], np.complex128 if cplx else np.float64)
+    dt_ = np.complex128 if cplx else np.float64
+    sn, so, sf = gs.pack(mats[:m], dt_)
     cores = min(16, os.cpu_count() or 1)
     tc = time.perf_counter()
     pyoracle.schur_batched(sn, so, sf, cplx, nthreads=cores)
     dt = time.perf_counter() - tc
+    m1 = max(1, m // 16)
+    sn1, so1, sf1 = gs.pack(mats[:m1], dt_)
+    tc = time.perf_counter()
+    pyoracle.schur_batched(sn1, so1, sf1, cplx, nthreads=1)
+    dt1 = time.perf_counter() - tc
     return {"value": m / dt, "unit": "matrices/s", "cores": cores, "kind": "port",
             "sample": f"first {m} matrices of the same workload; C++ restatement of gschur! (oracle/), OpenMP over "
-                      f"matrices, {cores} threads, {dt:.1f}s"}
+                      f"matrices, {cores} threads, {dt:.1f}s",
+            "one_thread_value": m1 / dt1, "one_thread_sample": f"first {m1} matrices, 1 thread, {dt1:.1f}s",
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def bench_workload4(args):
