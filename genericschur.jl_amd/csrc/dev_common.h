@@ -355,22 +355,31 @@ GS_DEV double dreflector(int nr, double& a, double& x1, double& x2) {
 // ------------------------------------------------ low-latency reflectors
 // The same reflectors (householder.jl:12-102) for the bulge chase, with the
 // norms folded into one scaled square root and the divisions merged into a
-// single reciprocal.  Same result up to rounding.  Outside |beta| in
-// [2^-300, 2^300] (where the merged reciprocal could over/underflow) they
-// fall back to the reference formulas above.
+// single reciprocal (the complex one: entries scaled by a power of two, the
+// reciprocal from the hardware estimate and two Newton steps).  Same result
+// up to rounding.  Outside |beta| in [2^-300, 2^300] (where the merged
+// reciprocal could over/underflow) they fall back to the reference formulas
+// above.
 GS_DEV zd zreflector2_fast(zd& a, zd& x) {
   const double ar = a.x, ai = a.y;
   if (x.x == 0.0 && x.y == 0.0 && ai == 0.0) return zr(0.0);
+  // the norm of (ar, ai, x) with the entries scaled by a power of two (exact)
   const double w = fmax(fmax(fabs(ar), fabs(ai)), fmax(fabs(x.x), fabs(x.y)));
-  const double rw = 1.0 / w;
-  const double p = ar * rw, q = ai * rw, s = x.x * rw, t = x.y * rw;
-  const double beta = -copysign(w * sqrt(p * p + q * q + s * s + t * t), ar);
+  int e;
+  (void)frexp(w, &e);
+  const double p = ldexp(ar, -e), q = ldexp(ai, -e), s = ldexp(x.x, -e), t = ldexp(x.y, -e);
+  const double beta = -copysign(ldexp(sqrt(p * p + q * q + s * s + t * t), e), ar);
   const double ab = fabs(beta);
   if (!(ab >= 0x1p-300 && ab <= 0x1p300)) return zreflector2(a, x);
   // tau = ((beta - ar) / beta, -ai / beta);  1 / (alpha - beta) = conj(d) / |d|^2
   const double dr = ar - beta;
   const double dd = dr * dr + ai * ai;
-  const double r = 1.0 / (beta * dd);
+  // 1 / (beta dd), in [2^-905, 2^900] here: reciprocal estimate and two
+  // Newton steps (no range fix-ups needed)
+  const double bd = beta * dd;
+  double r = __builtin_amdgcn_rcp(bd);
+  r = fma(r, fma(-bd, r, 1.0), r);
+  r = fma(r, fma(-bd, r, 1.0), r);
   const double rb = dd * r;     // 1 / beta
   const double rdd = beta * r;  // 1 / |d|^2
   const zd tau = zmk((beta - ar) * rb, -ai * rb);
