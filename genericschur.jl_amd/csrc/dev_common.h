@@ -355,8 +355,8 @@ GS_DEV double dreflector(int nr, double& a, double& x1, double& x2) {
 // ------------------------------------------------ low-latency reflectors
 // The same reflectors (householder.jl:12-102) for the bulge chase, with the
 // norms folded into one scaled square root and the divisions merged into a
-// single reciprocal (the complex one: entries scaled by a power of two, the
-// reciprocal from the hardware estimate and two Newton steps).  Same result
+// single reciprocal (entries scaled by a power of two, the reciprocal from
+// the hardware estimate and two Newton steps).  Same result
 // up to rounding.  Outside |beta| in [2^-300, 2^300] (where the merged
 // reciprocal could over/underflow) they fall back to the reference formulas
 // above.
@@ -395,14 +395,19 @@ GS_DEV double dreflector_fast(int nr, double& a, double& x1, double& x2) {
   const double x2v = (nr == 3) ? x2 : 0.0;
   if (x1 == 0.0 && x2v == 0.0) return 0.0;
   const double w = fmax(fabs(alpha), fmax(fabs(x1), fabs(x2v)));
-  const double rw = 1.0 / w;
-  const double p = alpha * rw, q = x1 * rw, s = x2v * rw;
-  const double beta = -copysign(w * sqrt(p * p + q * q + s * s), alpha);
+  int e;
+  (void)frexp(w, &e);
+  const double p = ldexp(alpha, -e), q = ldexp(x1, -e), s = ldexp(x2v, -e);
+  const double beta = -copysign(ldexp(sqrt(p * p + q * q + s * s), e), alpha);
   const double ab = fabs(beta);
   if (!(ab >= 0x1p-300 && ab <= 0x1p300)) return dreflector(nr, a, x1, x2);
-  // tau = (beta - alpha) / beta, scale = 1 / (alpha - beta)
+  // tau = (beta - alpha) / beta, scale = 1 / (alpha - beta); beta d lies in
+  // [2^-600, 2^602] here: reciprocal estimate and two Newton steps
   const double d = alpha - beta;
-  const double r = 1.0 / (beta * d);
+  const double bd = beta * d;
+  double r = __builtin_amdgcn_rcp(bd);
+  r = fma(r, fma(-bd, r, 1.0), r);
+  r = fma(r, fma(-bd, r, 1.0), r);
   const double tau = -(d * d) * r;
   const double sc = beta * r;
   x1 *= sc;
