@@ -6,6 +6,35 @@
 
 namespace gs {
 
+// A step of a logged real reflector on a line's entries, x -= (tau1 ss)
+// (1, v2, v3) with ss = x0 + v2 x1 + v3 x2 (GenericSchur.jl:900-948), its
+// two-element form, and a Givens rotation of two entries (:674-687), with a
+// fixed rounding: no contraction beyond the fma calls written here.  Every
+// path that applies them to Z (in place during the sweeps, or replayed from
+// the Z log) goes through these, so the paths give the same bits.
+GS_DEV void drefl3_fixed(double& x0, double& x1, double& x2, double tau1, double v2, double v3) {
+#pragma clang fp contract(off)
+  const double tau2 = tau1 * v2, tau3 = tau1 * v3;
+  const double ss = fma(v3, x2, fma(v2, x1, x0));
+  x0 = fma(-ss, tau1, x0);
+  x1 = fma(-ss, tau2, x1);
+  x2 = fma(-ss, tau3, x2);
+}
+GS_DEV void drefl2_fixed(double& x0, double& x1, double tau1, double v2) {
+#pragma clang fp contract(off)
+  const double tau2 = tau1 * v2;
+  const double ss = fma(v2, x1, x0);
+  x0 = fma(-ss, tau1, x0);
+  x1 = fma(-ss, tau2, x1);
+}
+GS_DEV void drot_fixed(double& a1, double& a2, double cs, double sn) {
+#pragma clang fp contract(off)
+  const double b1 = fma(a1, cs, a2 * sn);
+  const double b2 = fma(-a1, sn, a2 * cs);
+  a1 = b1;
+  a2 = b2;
+}
+
 // The logged reflectors of bulge steps ka .. kb (log entry k - mx: tau1, v2,
 // v3; kb - ka < MAXS) applied in order to one line x(t) = p[(t - 1) * st] of
 // H or Z: the right update of rows (GenericSchur.jl:916-935, 943-948) or the
@@ -44,19 +73,10 @@ GS_DEV void chain_line(double* p, int64_t st, const double* lg, int mx, int ka, 
       const int q = g * G + i;
       if (q < s) {
         const double tau1 = lv[3 * q], v2 = lv[3 * q + 1];
-        const double tau2 = tau1 * v2;
-        if (q <= k3 - ka) {
-          const double v3 = lv[3 * q + 2];
-          const double tau3 = tau1 * v3;
-          const double ss = x[q] + v2 * x[q + 1] + v3 * x[q + 2];
-          x[q] = x[q] - ss * tau1;
-          x[q + 1] = x[q + 1] - ss * tau2;
-          x[q + 2] = x[q + 2] - ss * tau3;
-        } else {
-          const double ss = x[q] + v2 * x[q + 1];
-          x[q] = x[q] - ss * tau1;
-          x[q + 1] = x[q + 1] - ss * tau2;
-        }
+        if (q <= k3 - ka)
+          drefl3_fixed(x[q], x[q + 1], x[q + 2], tau1, v2, lv[3 * q + 2]);
+        else
+          drefl2_fixed(x[q], x[q + 1], tau1, v2);
       }
     }
 #pragma unroll
@@ -68,6 +88,42 @@ GS_DEV void chain_line(double* p, int64_t st, const double* lg, int mx, int ka, 
     if (g + 2 < NG && (g + 2) * G < ne) {
 #pragma unroll
       for (int i = 0; i < G; ++i) x[(g + 2) * G + i] = q0[(int64_t)min((g + 2) * G + i, ne - 1) * st];
+    }
+  }
+}
+
+// Real Z log (the windowed batched QR kernel, bins n = 65 .. 128): records
+// of doubles, a sweep (mx, iend) then (tau1, v2, v3) per step k = mx ..
+// iend-1 (the last step's reflector has two elements), or a Givens rotation
+// (-j, cs, sn) of columns j, j+1 from a 2 x 2 block's standardization.
+// Applies records [0, len) to one row of Z, x(c) = row[(c - 1) st], with the
+// operations and order of the in-place updates.
+GS_DEV void dlog_row(double* row, int64_t st, const double* __restrict__ lg, int64_t len) {
+  int64_t pos = 0;
+  while (pos < len) {
+    const double h0 = lg[pos];
+    if (h0 < 0.0) {
+      const int j = (int)(-h0);
+      double a1 = row[(int64_t)(j - 1) * st], a2 = row[(int64_t)j * st];
+      drot_fixed(a1, a2, lg[pos + 1], lg[pos + 2]);
+      row[(int64_t)(j - 1) * st] = a1;
+      row[(int64_t)j * st] = a2;
+      pos += 3;
+    } else {
+      const int mx = (int)h0, ie = (int)lg[pos + 1];
+      const double* e = lg + pos + 2;
+      double c0 = row[(int64_t)(mx - 1) * st], c1 = row[(int64_t)mx * st];
+      for (int k = mx; k < ie - 1; ++k) {
+        double b = row[(int64_t)(k + 1) * st];
+        drefl3_fixed(c0, c1, b, e[3 * (k - mx)], e[3 * (k - mx) + 1], e[3 * (k - mx) + 2]);
+        row[(int64_t)(k - 1) * st] = c0;
+        c0 = c1;
+        c1 = b;
+      }
+      drefl2_fixed(c0, c1, e[3 * (ie - 1 - mx)], e[3 * (ie - 1 - mx) + 1]);
+      row[(int64_t)(ie - 2) * st] = c0;
+      row[(int64_t)(ie - 1) * st] = c1;
+      pos += 2 + 3 * (int64_t)(ie - mx);
     }
   }
 }

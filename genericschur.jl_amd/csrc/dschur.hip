@@ -8,11 +8,13 @@
 //   Q formation   hessenberg.jl:150-166
 //
 // Same structure as the complex kernel: one workgroup per matrix; batched
-// matrices take a Hessenberg+Z launch with the matrix in LDS and a QR launch
-// with the band j >= i-3 of H in LDS (the double-shift bulge reaches the
-// third subdiagonal); the single-matrix kernel keeps H in HBM.  Z lives in
-// HBM with each sweep's 3-element reflectors applied afterwards as one
-// chunked chain per row of Z.
+// matrices take a Hessenberg+Z launch with the matrix in LDS, then the QR
+// launch: by default dqrw_kernel, one wavefront per matrix with H and Z in
+// HBM, the bulge chase in a 32 x 32 LDS window and each chunk's 3-element
+// reflectors applied to the rest of H and to Z as one chain per row or
+// column; with GS_QR_BAND dqr_kernel, the band j >= i-3 of H in LDS (the
+// double-shift bulge reaches the third subdiagonal).  The single-matrix
+// kernel keeps H in HBM.
 #include "chains.h"
 #include "dev_common.h"
 #include "gs2x2.h"
@@ -443,9 +445,33 @@ constexpr int DW = 32;       // window rows and columns
 constexpr int DKC = DW - 4;  // bulge steps per chunk
 constexpr int DWL = DW + 1;  // odd LDS pitch of a window column (bank spread)
 
+// Z log of the windowed real QR kernel (records: chains.h::dlog_row).  Z is
+// not read while the sweeps run, so the kernel records them (and the 2 x 2
+// blocks' rotations) and dlog_kernel applies the records to Z, held in LDS,
+// afterwards.  A record that would not fit makes the kernel apply the log so
+// far to Z in HBM itself and start it again.
+struct DLog {
+  double* g;    // null: Z updated in place
+  int64_t cap;  // capacity in doubles (at least one sweep and one rotation)
+  int64_t pos;  // doubles in use
+};
+
+// Room for `need` more doubles in the log: apply the log to Z first when full.
+GS_DEV void dlog_drain(double* Zg, int64_t ldz, int n, const double* lg, int64_t len) {
+  for (int row = 1 + (int)threadIdx.x; row <= n; row += (int)blockDim.x) dlog_row(Zg + (row - 1), ldz, lg, len);
+}
+GS_DEV void dlog_reserve(DLog& zl, int64_t need, double* Zg, int64_t ldz, int n) {
+  if (zl.pos + need > zl.cap) {
+    __syncthreads();
+    dlog_drain(Zg, ldz, n, zl.g, zl.pos);
+    __syncthreads();
+    zl.pos = 0;
+  }
+}
+
 template <bool FAST> GS_DEV void d_sweep_win(double* H, int64_t ld, int n, double* Zg, int64_t ldz, bool wantz,
                                             double* win, double* lg, int* redi, zd shift1, zd shift2, int istart,
-                                            int iend, int64_t& nstep) {
+                                            int iend, int64_t& nstep, DLog* zl) {
   const int lane = threadIdx.x;
   const FullLayout lay{ld};
   const double r1r = shift1.x, r1i = shift1.y, r2r = shift2.x, r2i = shift2.y;
@@ -476,6 +502,18 @@ template <bool FAST> GS_DEV void d_sweep_win(double* H, int64_t ld, int n, doubl
   const int mx = (found > 0) ? found : istart;
   double v0, v1, v2s;
   firstcol(mx, v0, v1, v2s);
+  // Z log: room for this sweep's record, then its header
+  const bool zlg = wantz && zl && zl->g;
+  const int64_t need = 2 + 3 * (int64_t)(iend - mx);
+  double* rec = nullptr;
+  if (zlg) {
+    dlog_reserve(*zl, need, Zg, ldz, n);
+    rec = zl->g + zl->pos;
+    if (lane == 0) {
+      rec[0] = (double)mx;
+      rec[1] = (double)iend;
+    }
+  }
   for (int k0 = mx; k0 <= iend - 1;) {
     const int kend = min(k0 + DKC - 1, iend - 1);
     const int wb = max(k0 - 1, 1), we = min(kend + 3, iend);
@@ -566,13 +604,17 @@ template <bool FAST> GS_DEV void d_sweep_win(double* H, int64_t ld, int n, doubl
     win_store<double, DW, DWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
     __syncthreads();
     for (int col = we + 1 + lane; col <= n; col += 64)
-      chain_line<DKC, 8>(H + (int64_t)(col - 1) * ld, 1, lg, k0, k0, kend, iend);
-    for (int row = 1 + lane; row <= wb - 1; row += 64) chain_line<DKC, 8>(H + (row - 1), ld, lg, k0, k0, kend, iend);
-    if (wantz)
-      for (int row = 1 + lane; row <= n; row += 64) chain_line<DKC, 8>(Zg + (row - 1), ldz, lg, k0, k0, kend, iend);
+      chain_line<DKC, 6>(H + (int64_t)(col - 1) * ld, 1, lg, k0, k0, kend, iend);
+    for (int row = 1 + lane; row <= wb - 1; row += 64) chain_line<DKC, 6>(H + (row - 1), ld, lg, k0, k0, kend, iend);
+    if (zlg) {
+      for (int i = lane; i < 3 * (kend - k0 + 1); i += 64) rec[2 + 3 * (k0 - mx) + i] = lg[i];
+    } else if (wantz) {
+      for (int row = 1 + lane; row <= n; row += 64) chain_line<DKC, 6>(Zg + (row - 1), ldz, lg, k0, k0, kend, iend);
+    }
     __syncthreads();
     k0 = kend + 1;
   }
+  if (zlg) zl->pos += need;
   nstep += iend - mx;
 }
 
@@ -580,7 +622,7 @@ template <bool FAST> GS_DEV void d_sweep_win(double* H, int64_t ld, int n, doubl
 // gschur!(H::Hessenberg{<:AbstractFloat}, Z) (GenericSchur.jl:513-699)
 template <int NT, class L, bool FAST, bool WIN = false> GS_DEV int64_t d_qr(double* H, L lay, int n, double* Zg, int64_t ldz, bool wantz,
                                                double* rot, int* redi, zd* Wg, int64_t maxiter, int64_t& nsweep,
-                                               int64_t& nstep) {
+                                               int64_t& nstep, DLog* zl = nullptr) {
   const int tid = threadIdx.x;
   const double eps = GS_EPS;
   const double smallnum = GS_FMIN * ((double)n / eps);
@@ -688,7 +730,7 @@ template <int NT, class L, bool FAST, bool WIN = false> GS_DEV int64_t d_qr(doub
       nsweep += 1;
       if constexpr (WIN)
         d_sweep_win<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + DWL * DW, redi, zmk(r1r, r1i), zmk(r2r, r2i),
-                          istart, iend, nstep);
+                          istart, iend, nstep, zl);
       else
         d_sweep<NT, L, FAST>(H, lay, n, Zg, ldz, wantz, rot, redi, zmk(r1r, r1i), zmk(r2r, r2i), istart, iend, nstep);
     }
@@ -728,11 +770,20 @@ template <int NT, class L, bool FAST, bool WIN = false> GS_DEV int64_t d_qr(doub
         HR(iend, iend) = d;
         if (iend > 2) HR(iend - 1, iend - 2) = 0.0;
       }
-      if (wantz) {
+      if (wantz && zl && zl->g) {
+        dlog_reserve(*zl, 3, Zg, ldz, n);
+        if (tid == 0) {
+          zl->g[zl->pos] = -(double)(iend - 1);
+          zl->g[zl->pos + 1] = cs;
+          zl->g[zl->pos + 2] = sn;
+        }
+        zl->pos += 3;
+      } else if (wantz) {
         for (int i = 1 + tid; i <= n; i += NT) {
-          const double a1 = ZR(i, iend - 1), a2 = ZR(i, iend);
-          ZR(i, iend - 1) = a1 * cs + a2 * sn;
-          ZR(i, iend) = -a1 * sn + a2 * cs;
+          double a1 = ZR(i, iend - 1), a2 = ZR(i, iend);
+          drot_fixed(a1, a2, cs, sn);
+          ZR(i, iend - 1) = a1;
+          ZR(i, iend) = a2;
         }
       }
       __syncthreads();
@@ -909,6 +960,7 @@ template <bool FAST> __global__ void __launch_bounds__(64) __attribute__((amdgpu
   if (b >= a.count) return;
   const int m = a.order[a.first + b];
   const int n = a.n[m];
+  if (a.zlog && threadIdx.x == 0) a.llen[m] = 0;
   if (n <= 0) return;
   if (a.info[m] != 0) return;
   const int64_t o = a.off[m];
@@ -919,10 +971,16 @@ template <bool FAST> __global__ void __launch_bounds__(64) __attribute__((amdgpu
   const FullLayout lay{(int64_t)n};
   const int64_t maxiter = a.maxiter > 0 ? a.maxiter : 100 * (int64_t)n;
   int64_t nsweep = 0, nstep = 0;
+  DLog zl{nullptr, 0, 0};
+  if (a.zlog && wantz) {
+    zl.g = a.zlog + a.loff[m];
+    zl.cap = a.loff[m + 1] - a.loff[m];
+  }
   const int64_t st = d_qr<64, FullLayout, FAST, true>(T, lay, n, Z, (int64_t)n, wantz, win, redi, W, maxiter, nsweep,
-                                                      nstep);
+                                                      nstep, &zl);
   __syncthreads();
   if (threadIdx.x == 0) {
+    if (zl.g) a.llen[m] = zl.pos;
     if (a.stats) {
       a.stats[2 * m] = nsweep;
       a.stats[2 * m + 1] = nstep;
@@ -938,6 +996,102 @@ hipError_t launch_dqrw(const BatchArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((dqrw_kernel<false>), dim3(a.count), dim3(64), 0, s, a);
   else
     hipLaunchKernelGGL((dqrw_kernel<true>), dim3(a.count), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+// After a bin's QR launch (orders 65 .. 128): each matrix's Z log applied to
+// Z by one workgroup of ceil(n / 64) wavefronts, all of Z in LDS (row r,
+// column c at zs[(c - 1) n + r]), each thread replaying the log on one row
+// (dlog_row's operations).  Each wavefront stages a record in its own LDS
+// buffer and loads the next one into registers while it replays the
+// current one.
+constexpr int DLREC = 384;  // >= 2 + 3 (n - 1) for n <= 128
+__global__ void __launch_bounds__(128) dlog_kernel(BatchArgs a) {
+  extern __shared__ double dzs[];
+  __shared__ double rbuf[2][DLREC];  // a separate array: its reads do not wait on the row writes
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int b = blockIdx.x;
+  if (b >= a.count) return;
+  const int m = a.order[a.first + b];
+  const int n = a.n[m];
+  if (n <= 0 || wave * 64 >= n) return;
+  const int64_t len = a.llen[m];
+  if (len <= 0) return;
+  double* Z = a.Z + a.off[m];
+  const double* lg = a.zlog + a.loff[m];
+  const int r = min(tid, n - 1);  // threads past the last row repeat it, unstored
+  const bool live = tid < n;
+  double t[DLREC / 64];
+  auto fetch = [&](int64_t p) {
+#pragma unroll
+    for (int j = 0; j < DLREC / 64; ++j) t[j] = lg[min(p + lane + 64 * j, len - 1)];
+  };
+  fetch(0);
+  for (int c0 = 0; c0 < n; c0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = Z[r + (int64_t)min(c0 + i, n - 1) * n];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (c0 + i < n && live) dzs[(c0 + i) * n + r] = v[i];
+  }
+  double* row = dzs + r;
+  const int st = n;
+  double* rec = rbuf[wave];
+  for (int64_t pos = 0; pos < len;) {
+#pragma unroll
+    for (int j = 0; j < DLREC / 64; ++j) rec[lane + 64 * j] = t[j];
+    wsync();
+    const double h0 = rec[0];
+    const bool rotation = h0 < 0.0;
+    const int i1 = __builtin_amdgcn_readfirstlane(rotation ? (int)(-h0) : (int)h0);
+    const int ie = __builtin_amdgcn_readfirstlane(rotation ? 0 : (int)rec[1]);
+    const int64_t rl = rotation ? 3 : 2 + 3 * (int64_t)(ie - i1);
+    if (pos + rl < len) fetch(pos + rl);
+    if (live) {
+      if (rotation) {
+        double a1 = row[(i1 - 1) * st], a2 = row[i1 * st];
+        drot_fixed(a1, a2, rec[1], rec[2]);
+        row[(i1 - 1) * st] = a1;
+        row[i1 * st] = a2;
+      } else {
+        const double* e = rec + 2;
+        double c0 = row[(i1 - 1) * st], c1 = row[i1 * st];
+#pragma unroll 4
+        for (int k = i1; k < ie - 1; ++k) {
+          double bb = row[(k + 1) * st];
+          drefl3_fixed(c0, c1, bb, e[3 * (k - i1)], e[3 * (k - i1) + 1], e[3 * (k - i1) + 2]);
+          row[(k - 1) * st] = c0;
+          c0 = c1;
+          c1 = bb;
+        }
+        drefl2_fixed(c0, c1, e[3 * (ie - 1 - i1)], e[3 * (ie - 1 - i1) + 1]);
+        row[(ie - 2) * st] = c0;
+        row[(ie - 1) * st] = c1;
+      }
+    }
+    wsync();  // the record buffer is rewritten for the next record
+    pos += rl;
+  }
+  if (live) {
+    for (int c0 = 0; c0 < n; c0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = dzs[min(c0 + i, n - 1) * n + r];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (c0 + i < n) Z[r + (int64_t)(c0 + i) * n] = v[i];
+    }
+  }
+}
+
+hipError_t launch_dlog(const BatchArgs& a, int nmax, hipStream_t s) {
+  if (a.count <= 0 || !a.zlog) return hipSuccess;
+  if (2 + 3 * (nmax - 1) > DLREC || nmax > 128) return hipErrorInvalidValue;
+  const int waves = (nmax + 63) / 64;
+  const size_t lds = (size_t)nmax * nmax * sizeof(double);
+  (void)hipFuncSetAttribute((const void*)dlog_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(dlog_kernel, dim3(a.count), dim3(64 * waves), lds, s, a);
   return hipGetLastError();
 }
 

@@ -268,11 +268,13 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
       load[best] += p->bins[i].work;
     }
   }
-  if (cplx) {
+  {
+    // Z-log regions: complex entries for complex input, doubles for real
     for (int t = 0; t < 2; ++t) {
       p->loff[t].assign(count + 1, 0);
       for (int64_t i = 0; i < count; ++i)
-        p->loff[t][i + 1] = p->loff[t][i] + (n[i] > 1 ? zlog_capacity(n[i], t == 1) : 0);
+        p->loff[t][i + 1] =
+            p->loff[t][i] + (n[i] > 1 ? (cplx ? zlog_capacity(n[i], t == 1) : dlog_capacity(n[i], t == 1)) : 0);
       HIPCHK(h, p->d_loff[t].ensure(sizeof(int64_t) * (size_t)(count + 1)));
       HIPCHK(h, hipMemcpy(p->d_loff[t].p, p->loff[t].data(), sizeof(int64_t) * (size_t)(count + 1),
                           hipMemcpyHostToDevice));
@@ -298,15 +300,25 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
   return GS_OK;
 }
 
-// Smallest bin order whose QR launch logs its Z updates: measured on
-// MI355X, 1400 matrices of one order, QR + replay against in-place updates:
-// n = 96 36 ms vs 42 ms, n = 64 14.4 ms vs 13.2 ms, n = 32 equal.
+// Bin orders whose QR launch logs its Z updates.  The smallest: measured on
+// MI355X, 1400 complex matrices of one order, QR + replay against in-place
+// updates: n = 96 36 ms vs 42 ms, n = 64 14.4 ms vs 13.2 ms, n = 32 equal.
+// The largest real one: the replay holds all of Z in LDS with its record
+// buffers (128^2 doubles + 6 KB).
 constexpr int kZlogMinN = 65;
+constexpr int kDlogMaxN = 128;
 
-// Whether the batched QR launches of this call log their Z updates (complex,
-// windowed kernel, Z wanted, not GS_ZDIRECT).
+// Whether the batched QR launches of this call log their Z updates
+// (windowed kernels, Z wanted, not GS_ZDIRECT).
 static bool uses_zlog(const Plan* p, int kflags) {
-  return p->is_complex && (kflags & GS_WANTZ) && !(kflags & (GS_QR_BAND | GS_ZDIRECT));
+  (void)p;
+  return (kflags & GS_WANTZ) && !(kflags & (GS_QR_BAND | GS_ZDIRECT));
+}
+// Whether bin b's QR launch logs its Z updates.
+static bool bin_zlog(const Plan* p, const Bin& b, int kflags) {
+  if (!uses_zlog(p, kflags)) return false;
+  if (!p->is_complex && b.nmax > kDlogMaxN) return false;
+  return b.nmax >= kZlogMinN || (kflags & GS_ZLOG_TINY);
 }
 
 // With timed set, events bracket every launch on its stream (eager launches
@@ -352,7 +364,7 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
     a.count = b.count;
     // the Z log pays off where Z's rows are long (its replay holds all of Z
     // in LDS); GS_ZLOG_TINY exercises it on every bin
-    a.zlog = (zl && (b.nmax >= kZlogMinN || (flags & GS_ZLOG_TINY))) ? zlog : nullptr;
+    a.zlog = (zl && bin_zlog(p, b, flags)) ? zlog : nullptr;
     hipStream_t s = h->branch[b.branch];
     if (timed && (e = hipEventRecord(h->tev[6 * bi], s)) != hipSuccess) return e;
     e = p->is_complex ? launch_zhq(a, b.nmax, b.nt_hq, s) : launch_dhq(a, b.nmax, b.nt_hq, s);
@@ -366,7 +378,7 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
     if (e != hipSuccess) return e;
     if (timed && (e = hipEventRecord(h->tev[6 * bi + 3], s)) != hipSuccess) return e;
     if (timed && (e = hipEventRecord(h->tev[6 * bi + 4], s)) != hipSuccess) return e;
-    if (a.zlog && (e = launch_zlog(a, b.nmax, s)) != hipSuccess) return e;
+    if (a.zlog && (e = p->is_complex ? launch_zlog(a, b.nmax, s) : launch_dlog(a, b.nmax, s)) != hipSuccess) return e;
     if (timed && (e = hipEventRecord(h->tev[6 * bi + 5], s)) != hipSuccess) return e;
   }
   for (int i = 0; i < kBranches; ++i) {
@@ -398,7 +410,7 @@ static int run_timed(gs_handle* h, Plan* p, const double* A, double* T, double* 
     // kinds 0 (Hessenberg + Q), 1 (QR sweeps), 4 (Z log applied)
     const int kinds[3] = {0, 1, 4};
     for (int j = 0; j < 3; ++j) {
-      if (j == 2 && !(zlog && uses_zlog(p, kflags) && (b.nmax >= kZlogMinN || (kflags & GS_ZLOG_TINY)))) continue;
+      if (j == 2 && !(zlog && bin_zlog(p, b, kflags))) continue;
       float ms = 0.f;
       HIPCHK(h, hipEventElapsedTime(&ms, h->tev[6 * bi + 2 * j], h->tev[6 * bi + 2 * j + 1]));
       const double rec[GS_LAUNCH_REC] = {(double)kinds[j], (double)b.nmin, (double)b.nmax, (double)b.count, b.work,
@@ -435,7 +447,7 @@ static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n,
   double* zlog = nullptr;
   if (uses_zlog(p, kflags)) {
     const int64_t entries = p->loff[(kflags & GS_ZLOG_TINY) ? 1 : 0][count];
-    HIPCHK(h, h->bLog.ensure(16 * (size_t)std::max<int64_t>(entries, 1)));
+    HIPCHK(h, h->bLog.ensure(es * (size_t)std::max<int64_t>(entries, 1)));
     zlog = static_cast<double*>(h->bLog.p);
   }
   double* dZdummy = nullptr;

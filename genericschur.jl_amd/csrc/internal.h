@@ -77,6 +77,13 @@ inline int64_t zlog_capacity(int n, bool tiny) {
   return tiny ? one_sweep : 6 * (int64_t)n * n + 16 * (int64_t)n + 64;
 }
 hipError_t launch_dqrw(const BatchArgs& a, hipStream_t s);
+hipError_t launch_dlog(const BatchArgs& a, int nmax, hipStream_t s);
+// Real Z-log capacity of one matrix of order n, in doubles (tiny: one sweep
+// and one rotation, the least the kernel accepts)
+inline int64_t dlog_capacity(int n, bool tiny) {
+  const int64_t one = 2 + 3 * (int64_t)n + 3;
+  return tiny ? one : 5 * (int64_t)n * n + 16 * (int64_t)n + 64;
+}
 hipError_t launch_dhq(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 hipError_t launch_dqr(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 hipError_t launch_zschur_global(const SingleArgs& a, int nt, hipStream_t s);

@@ -8,15 +8,15 @@
 //   Q formation   hessenberg.jl:150-166
 //   scaling       util.jl:14-80
 //
-// One workgroup owns one matrix.  Batched matrices take two launches per
-// size bin: zhq_kernel reduces A to Hessenberg form and forms Z with the
-// whole matrix resident in LDS, then zqr_kernel runs the QR iterations with
-// only the band j >= i-2 of H resident in LDS (half the footprint, so twice
-// the resident matrices per CU).  The single-matrix kernel keeps H in HBM.
-//
-// Z always lives in HBM: the rotations of a sweep are logged in LDS and
-// applied afterwards as one chain per row of Z, with the row segment loaded
-// in chunks so that a chunk's loads are in flight together.
+// Batched matrices take two or three launches per size bin: zhq_kernel
+// reduces A to Hessenberg form and forms Z with the whole matrix resident
+// in LDS (one workgroup per matrix); zqrw2_kernel runs the QR iterations
+// with H and Z in HBM, two wavefronts per matrix (the bulge chase in a
+// 32 x 32 LDS window, and the chunk's reflectors applied to the rest of H
+// and Z as chains); for bins above n = 64 it logs the Z updates instead and
+// zlog_kernel applies the log with Z in LDS.  zqr_kernel (GS_QR_BAND) keeps
+// H's band j >= i-2 in LDS instead.  The single-matrix kernel keeps H in
+// HBM.
 //
 // Every thread runs the scalar control logic (deflation tests, shifts,
 // reflectors) redundantly on values read from H; searches are parallel over

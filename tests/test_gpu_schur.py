@@ -128,6 +128,30 @@ def test_zlog_matches_direct(gs, oracle):
         assert checks.rel_backward(A, a.T, a.Z) <= 50 * n * eps
 
 
+def test_dlog_matches_direct(gs):
+    """The real Z log (sweeps' 3-element reflectors and the 2 x 2 blocks'
+    rotations, applied to Z after the sweeps) against in-place updates
+    (GS_ZDIRECT) and a one-sweep log that the QR kernel applies itself
+    before nearly every sweep or rotation (GS_ZLOG_TINY): T, Z and the
+    values bitwise equal; the log form also passes the real Schur checks."""
+    eps = np.finfo(np.float64).eps
+    mats, _ = gen.workload3(seed=8, count=48)
+    rng = gen.stream(31, 100, 0)
+    mats += [gen.haar_orthogonal(70, rng), gen.haar_orthogonal(100, rng), gen.toeplitz_grcar(90),
+             gen.ginibre_real(128, rng)]
+    Sl, il = gs.gschur_batched(mats)
+    Sd, idd = gs.gschur_batched(mats, zdirect=True)
+    St, it = gs.gschur_batched(mats, zlog_tiny=True)
+    assert np.all(il == 0) and np.all(idd == 0) and np.all(it == 0)
+    for A, a, d, t in zip(mats, Sl, Sd, St):
+        n = A.shape[0]
+        assert np.array_equal(a.T, d.T) and np.array_equal(a.T, t.T)
+        assert np.array_equal(a.values, d.values) and np.array_equal(a.values, t.values)
+        assert np.array_equal(a.Z, t.Z) and np.array_equal(a.Z, d.Z)
+        checks.schur_invariants(A, a.T, a.Z, a.values, 20.0)
+        assert checks.rel_backward(A, a.T, a.Z) <= 50 * n * eps
+
+
 def test_windowed_matches_band_qr(gs):
     """The windowed QR kernels (H in HBM, chase window in LDS, chained updates
     outside the window) apply the same reflectors in the same order as the
