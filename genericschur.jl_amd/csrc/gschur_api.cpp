@@ -306,6 +306,7 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
 // The largest real one: the replay holds all of Z in LDS with its record
 // buffers (128^2 doubles + 6 KB).
 constexpr int kZlogMinN = 65;
+constexpr int kDlogMinN = 33;
 constexpr int kDlogMaxN = 128;
 
 // Whether the batched QR launches of this call log their Z updates
@@ -318,7 +319,7 @@ static bool uses_zlog(const Plan* p, int kflags) {
 static bool bin_zlog(const Plan* p, const Bin& b, int kflags) {
   if (!uses_zlog(p, kflags)) return false;
   if (!p->is_complex && b.nmax > kDlogMaxN) return false;
-  return b.nmax >= kZlogMinN || (kflags & GS_ZLOG_TINY);
+  return b.nmax >= (p->is_complex ? kZlogMinN : kDlogMinN) || (kflags & GS_ZLOG_TINY);
 }
 
 // With timed set, events bracket every launch on its stream (eager launches
