@@ -1057,8 +1057,22 @@ __global__ void __launch_bounds__(128) dlog_kernel(BatchArgs a) {
       } else {
         const double* e = rec + 2;
         double c0 = row[(i1 - 1) * st], c1 = row[i1 * st];
-#pragma unroll 4
-        for (int k = i1; k < ie - 1; ++k) {
+        // groups of four steps: the group's row entries (which its own steps
+        // do not write first) are read before any of its writes
+        int k = i1;
+        for (; k + 3 < ie - 1; k += 4) {
+          double bb[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) bb[j] = row[(k + j + 1) * st];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            drefl3_fixed(c0, c1, bb[j], e[3 * (k + j - i1)], e[3 * (k + j - i1) + 1], e[3 * (k + j - i1) + 2]);
+            row[(k + j - 1) * st] = c0;
+            c0 = c1;
+            c1 = bb[j];
+          }
+        }
+        for (; k < ie - 1; ++k) {
           double bb = row[(k + 1) * st];
           drefl3_fixed(c0, c1, bb, e[3 * (k - i1)], e[3 * (k - i1) + 1], e[3 * (k - i1) + 2]);
           row[(k - 1) * st] = c0;

@@ -1249,8 +1249,22 @@ __global__ void __launch_bounds__(128) zlog_kernel(BatchArgs a) {
           for (int c = i1 + 2; c <= ie; ++c) row[(c - 1) * st] = zmul_fixed(row[(c - 1) * st], ctt);
         }
       }
-#pragma unroll 4
-      for (int k = i1 + 1; k < ie; ++k) {
+      // groups of four steps: the group's row entries (which its own steps
+      // do not write first) are read before any of its writes, so a group
+      // waits for LDS once
+      int k = i1 + 1;
+      for (; k + 3 < ie; k += 4) {
+        zd xa[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xa[j] = row[(k + j) * st];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          zrot_right_fixed(c0, xa[j], e[2 * (k + j - i1)], e[2 * (k + j - i1) + 1]);
+          row[(k + j - 1) * st] = c0;
+          c0 = xa[j];
+        }
+      }
+      for (; k < ie; ++k) {
         zd x1 = row[k * st];
         zrot_right_fixed(c0, x1, e[2 * (k - i1)], e[2 * (k - i1) + 1]);
         row[(k - 1) * st] = c0;
