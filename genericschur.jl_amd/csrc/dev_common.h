@@ -352,6 +352,19 @@ GS_DEV double dreflector(int nr, double& a, double& x1, double& x2) {
 // safemin(Float64) (util.jl:5-12) is floatmin for IEEE double.
 #define GS_SAFEMIN GS_FMIN
 
+// Square root of x in [2^-4, 2^4] (the reflectors' scaled sums of squares):
+// the hardware reciprocal square root estimate, one coupled Newton step for
+// sqrt(x) and 1 / (2 sqrt(x)), and one residual correction; within an ulp
+// or so, no range or special-value handling.
+GS_DEV double sqrt_scaled(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  double s = x * r, h = 0.5 * r;
+  const double e = fma(-s, h, 0.5);
+  s = fma(s, e, s);
+  h = fma(h, e, h);
+  return fma(fma(-s, s, x), h, s);
+}
+
 // ------------------------------------------------ low-latency reflectors
 // The same reflectors (householder.jl:12-102) for the bulge chase, with the
 // norms folded into one scaled square root and the divisions merged into a
@@ -359,18 +372,22 @@ GS_DEV double dreflector(int nr, double& a, double& x1, double& x2) {
 // the hardware estimate and two Newton steps).  Same result
 // up to rounding.  Outside |beta| in [2^-300, 2^300] (where the merged
 // reciprocal could over/underflow) they fall back to the reference formulas
-// above.
-GS_DEV zd zreflector2_fast(zd& a, zd& x) {
+// above.  UNI: every lane of the wavefront holds the same a and x, so the
+// two special-case tests branch on the wavefront's scalar unit.
+GS_DEV bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+template <bool UNI = false> GS_DEV zd zreflector2_fast(zd& a, zd& x) {
   const double ar = a.x, ai = a.y;
-  if (x.x == 0.0 && x.y == 0.0 && ai == 0.0) return zr(0.0);
+  const bool triv = x.x == 0.0 && x.y == 0.0 && ai == 0.0;
+  if (UNI ? wave_any(triv) : triv) return zr(0.0);
   // the norm of (ar, ai, x) with the entries scaled by a power of two (exact)
   const double w = fmax(fmax(fabs(ar), fabs(ai)), fmax(fabs(x.x), fabs(x.y)));
   int e;
   (void)frexp(w, &e);
   const double p = ldexp(ar, -e), q = ldexp(ai, -e), s = ldexp(x.x, -e), t = ldexp(x.y, -e);
-  const double beta = -copysign(ldexp(sqrt(p * p + q * q + s * s + t * t), e), ar);
+  const double beta = -copysign(ldexp(sqrt_scaled(p * p + q * q + s * s + t * t), e), ar);
   const double ab = fabs(beta);
-  if (!(ab >= 0x1p-300 && ab <= 0x1p300)) return zreflector2(a, x);
+  const bool out = !(ab >= 0x1p-300 && ab <= 0x1p300);
+  if (UNI ? wave_any(out) : out) return zreflector2(a, x);
   // tau = ((beta - ar) / beta, -ai / beta);  1 / (alpha - beta) = conj(d) / |d|^2
   const double dr = ar - beta;
   const double dd = dr * dr + ai * ai;
@@ -398,7 +415,7 @@ GS_DEV double dreflector_fast(int nr, double& a, double& x1, double& x2) {
   int e;
   (void)frexp(w, &e);
   const double p = ldexp(alpha, -e), q = ldexp(x1, -e), s = ldexp(x2v, -e);
-  const double beta = -copysign(ldexp(sqrt(p * p + q * q + s * s), e), alpha);
+  const double beta = -copysign(ldexp(sqrt_scaled(p * p + q * q + s * s), e), alpha);
   const double ab = fabs(beta);
   if (!(ab >= 0x1p-300 && ab <= 0x1p300)) return dreflector(nr, a, x1, x2);
   // tau = (beta - alpha) / beta, scale = 1 / (alpha - beta); beta d lies in
@@ -427,6 +444,15 @@ GS_DEV double rdlane(double v, int l) {
 // ordering point for LDS traffic among the lanes of one wavefront: the
 // hardware processes a wave's LDS operations in order, so only compiler
 // reordering has to be prevented
+// Quad permutation of a double across lanes (DPP quad_perm, CTRL = s0 |
+// s1 << 2 | s2 << 4 | s3 << 6: lane 4q + i takes lane 4q + s_i's value).
+template <int CTRL> GS_DEV double qperm(double v) {
+  const unsigned long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(u & 0xffffffffu), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xf, 0xf, true);
+  return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 GS_DEV void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();

@@ -686,53 +686,75 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
         win_load<zd, ZW, ZWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
         wsync();
         auto WIN = [&](int r, int cc) -> zd& { return win[(r - wb) + (cc - wb) * ZWL]; };
+        // Step k in lanes: the row pass (rows k, k + 1) on column k + lane;
+        // the column pass (columns k, k + 1) on row k + lane for lanes 0..2
+        // and on row wb + lane - 3 (rows above k) for lanes 3 and up.  All
+        // operands but the column pass's rows k and k + 1 are final before
+        // the step's reflector, so they are loaded first (the wavefront's
+        // LDS accesses complete in order); rows k and k + 1 come from the
+        // row pass's lanes 0 and 1 by a quad permutation, and the next
+        // reflector's inputs from the column pass's lanes 1 and 2, so no LDS
+        // round trip lies between one reflector and the next.  Lanes with
+        // nothing to do load in-window entries and store to a dummy area.
+        zd* const dmy = lgbuf + 4 * ZKC;
+        const bool hi3 = lane >= 3;
+        const int cidx = hi3 ? lane - 3 : lane;
+        if (k0 > istart1) {
+          v0 = WIN(k0, k0 - 1);
+          v1 = WIN(k0 + 1, k0 - 1);
+        }
         zd nv0 = zr(0.0), nv1 = zr(0.0);
         for (int k = k0; k <= kend; ++k) {
-          if (k > istart1) {
-            if (k > k0) {
-              v0 = nv0;
-              v1 = nv1;
-            } else {
-              v0 = WIN(k, k - 1);
-              v1 = WIN(k + 1, k - 1);
-            }
+          const int kw = k - wb;
+          const bool rv = lane <= we - k;
+          zd* const rp = win + kw + (kw + (rv ? lane : 0)) * ZWL;
+          zd ra = rp[0], rb = rp[1];
+          // column pass row - wb: k + lane - wb (lanes 0..2), lane - 3 (3 and up)
+          const bool cv = cidx < (hi3 ? kw : (k + 2 <= iend ? 3 : 2));
+          zd* const cp = win + (cv ? (hi3 ? cidx : cidx + kw) : kw) + kw * ZWL;
+          zd ca = cp[0], cb = cp[ZWL];
+          if (k > k0) {
+            v0 = nv0;
+            v1 = nv1;
           }
-          const zd tau1 = FAST ? zreflector2_fast(v0, v1) : zreflector2(v0, v1);
-          wsync();
-          if (k > istart1 && lane == 0) {
-            WIN(k, k - 1) = v0;
-            WIN(k + 1, k - 1) = zr(0.0);
-          }
+          const zd tau1 = FAST ? zreflector2_fast<true>(v0, v1) : zreflector2(v0, v1);
           const zd v2 = v1;
           const double tau2 = (tau1 * v2).x;
-          const zd ct1 = zconj(tau1);
           {
-            const int j = k + lane;
-            if (j <= we) {
-              const zd a = WIN(k, j), b = WIN(k + 1, j);
-              const zd ss = ct1 * a + tau2 * b;
-              WIN(k, j) = a - ss;
-              WIN(k + 1, j) = b - ss * v2;
+            const zd ss = zconj(tau1) * ra + tau2 * rb;
+            ra = ra - ss;
+            rb = rb - ss * v2;
+          }
+          {
+            // lane 0 takes lane 1's (k, k + 1), lane 1 takes lane 0's (k + 1, k)
+            const zd x10 = zmk(qperm<0xF5>(ra.x), qperm<0xF5>(ra.y));
+            const zd x01 = zmk(qperm<0xA0>(rb.x), qperm<0xA0>(rb.y));
+            if (lane == 0) {
+              ca = ra;
+              cb = x10;
+            } else if (lane == 1) {
+              ca = x01;
+              cb = rb;
             }
           }
-          wsync();
           {
-            const int jm = min(k + 2, iend);
-            const zd cv2 = zconj(v2);
-            const int j = wb + lane;
-            zd na = zr(0.0);
-            if (j <= jm) {
-              const zd a = WIN(j, k), b = WIN(j, k + 1);
-              const zd ss = tau1 * a + tau2 * b;
-              na = a - ss;
-              WIN(j, k) = na;
-              WIN(j, k + 1) = b - ss * cv2;
-            }
-            const int l0 = (k + 1) - wb, l1 = (k + 2) - wb;
-            nv0 = zmk(rdlane(na.x, l0 & 63), rdlane(na.y, l0 & 63));
-            nv1 = zmk(rdlane(na.x, l1 & 63), rdlane(na.y, l1 & 63));
+            const zd ss = tau1 * ca + tau2 * cb;
+            ca = ca - ss;
+            cb = cb - ss * zconj(v2);
           }
+          nv0 = zmk(rdlane(ca.x, 1), rdlane(ca.y, 1));
+          nv1 = zmk(rdlane(ca.x, 2), rdlane(ca.y, 2));
+          zd* const rq = (rv && lane >= 2) ? rp : dmy;
+          rq[0] = ra;
+          rq[1] = rb;
+          zd* const cq = cv ? cp : dmy;
+          cq[0] = ca;
+          cq[ZWL] = cb;
           if (lane == 0) {
+            if (k > istart1) {
+              WIN(k, k - 1) = v0;
+              WIN(k + 1, k - 1) = zr(0.0);
+            }
             lg[2 * (k - k0)] = tau1;
             lg[2 * (k - k0) + 1] = v2;
           }
@@ -1016,7 +1038,7 @@ template <int NT, bool FAST> __global__ void __launch_bounds__(NT) zqr_kernel(Ba
 // Two-wavefront windowed QR kernel: the chase of chunk c overlaps the chains
 // of chunk c-1 (z_sweep_win2); the control logic runs on both wavefronts.
 template <bool FAST> __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) zqrw2_kernel(BatchArgs a) {
-  __shared__ zd win[ZWL * ZW + 4 * ZKC];
+  __shared__ zd win[ZWL * ZW + 4 * ZKC + ZWL + 1];
   __shared__ int redi[4];
   const int b = blockIdx.x;
   if (b >= a.count) return;
