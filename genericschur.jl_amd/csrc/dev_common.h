@@ -372,13 +372,10 @@ GS_DEV double sqrt_scaled(double x) {
 // the hardware estimate and two Newton steps).  Same result
 // up to rounding.  Outside |beta| in [2^-300, 2^300] (where the merged
 // reciprocal could over/underflow) they fall back to the reference formulas
-// above.  UNI: every lane of the wavefront holds the same a and x, so the
-// two special-case tests branch on the wavefront's scalar unit.
-GS_DEV bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
-template <bool UNI = false> GS_DEV zd zreflector2_fast(zd& a, zd& x) {
+// above.
+GS_DEV zd zreflector2_fast(zd& a, zd& x) {
   const double ar = a.x, ai = a.y;
-  const bool triv = x.x == 0.0 && x.y == 0.0 && ai == 0.0;
-  if (UNI ? wave_any(triv) : triv) return zr(0.0);
+  if (x.x == 0.0 && x.y == 0.0 && ai == 0.0) return zr(0.0);
   // the norm of (ar, ai, x) with the entries scaled by a power of two (exact)
   const double w = fmax(fmax(fabs(ar), fabs(ai)), fmax(fabs(x.x), fabs(x.y)));
   int e;
@@ -386,8 +383,7 @@ template <bool UNI = false> GS_DEV zd zreflector2_fast(zd& a, zd& x) {
   const double p = ldexp(ar, -e), q = ldexp(ai, -e), s = ldexp(x.x, -e), t = ldexp(x.y, -e);
   const double beta = -copysign(ldexp(sqrt_scaled(p * p + q * q + s * s + t * t), e), ar);
   const double ab = fabs(beta);
-  const bool out = !(ab >= 0x1p-300 && ab <= 0x1p300);
-  if (UNI ? wave_any(out) : out) return zreflector2(a, x);
+  if (!(ab >= 0x1p-300 && ab <= 0x1p300)) return zreflector2(a, x);
   // tau = ((beta - ar) / beta, -ai / beta);  1 / (alpha - beta) = conj(d) / |d|^2
   const double dr = ar - beta;
   const double dd = dr * dr + ai * ai;
@@ -403,6 +399,42 @@ template <bool UNI = false> GS_DEV zd zreflector2_fast(zd& a, zd& x) {
   const zd inv = zmk(dr * rdd, -ai * rdd);
   x = x * inv;
   a = zr(beta);
+  return tau;
+}
+
+// zreflector2_fast for a wavefront whose lanes all hold the same a and x:
+// the formulas run straight through and one branch on the scalar unit
+// replaces them by zreflector2's when x = 0 with a real (tau = 0) or |beta|
+// lies outside [2^-300, 2^300].  Bitwise the same results as
+// zreflector2_fast.
+GS_DEV bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+GS_DEV zd zreflector2_wave(zd& a, zd& x) {
+  const double ar = a.x, ai = a.y;
+  const double w = fmax(fmax(fabs(ar), fabs(ai)), fmax(fabs(x.x), fabs(x.y)));
+  int e;
+  (void)frexp(w, &e);
+  const double p = ldexp(ar, -e), q = ldexp(ai, -e), s = ldexp(x.x, -e), t = ldexp(x.y, -e);
+  const double beta = -copysign(ldexp(sqrt_scaled(p * p + q * q + s * s + t * t), e), ar);
+  const double ab = fabs(beta);
+  const bool rare = ((x.x == 0.0) & (x.y == 0.0) & (ai == 0.0)) | !((ab >= 0x1p-300) & (ab <= 0x1p300));
+  const double dr = ar - beta;
+  const double dd = dr * dr + ai * ai;
+  const double bd = beta * dd;
+  double r = __builtin_amdgcn_rcp(bd);
+  r = fma(r, fma(-bd, r, 1.0), r);
+  r = fma(r, fma(-bd, r, 1.0), r);
+  const double rb = dd * r;
+  const double rdd = beta * r;
+  zd tau = zmk((beta - ar) * rb, -ai * rb);
+  const zd inv = zmk(dr * rdd, -ai * rdd);
+  zd xo = x * inv, ao = zr(beta);
+  if (wave_any(rare)) {
+    ao = a;
+    xo = x;
+    tau = zreflector2(ao, xo);
+  }
+  a = ao;
+  x = xo;
   return tau;
 }
 
@@ -434,6 +466,13 @@ GS_DEV double dreflector_fast(int nr, double& a, double& x1, double& x2) {
 }
 
 // uniform broadcast of lane l's double (l must be wave-uniform)
+GS_DEV double rdfirst(double v) {
+  const unsigned long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(u & 0xffffffffu));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(u >> 32));
+  return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 GS_DEV double rdlane(double v, int l) {
   const unsigned long long u = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffu), l);

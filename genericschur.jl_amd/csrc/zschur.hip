@@ -699,48 +699,51 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
         zd* const dmy = lgbuf + 4 * ZKC;
         const bool hi3 = lane >= 3;
         const int cidx = hi3 ? lane - 3 : lane;
+        const int laneZ = lane * ZWL;
         if (k0 > istart1) {
           v0 = WIN(k0, k0 - 1);
           v1 = WIN(k0 + 1, k0 - 1);
         }
-        zd nv0 = zr(0.0), nv1 = zr(0.0);
-        for (int k = k0; k <= kend; ++k) {
+        // the reflector's inputs, the same in every lane: kept in scalar
+        // registers from one step to the next
+        zd nv0 = zmk(rdfirst(v0.x), rdfirst(v0.y)), nv1 = zmk(rdfirst(v1.x), rdfirst(v1.y));
+        zd* wk = &WIN(k0, k0);
+        zd* lgk = lg;
+        for (int k = k0; k <= kend; ++k, wk += ZWL + 1, lgk += 2) {
           const int kw = k - wb;
           const bool rv = lane <= we - k;
-          zd* const rp = win + kw + (kw + (rv ? lane : 0)) * ZWL;
+          zd* const rp = wk + (rv ? laneZ : 0);
           zd ra = rp[0], rb = rp[1];
-          // column pass row - wb: k + lane - wb (lanes 0..2), lane - 3 (3 and up)
+          // column pass rows relative to k: lane (lanes 0..2), lane - 3 - kw (3 and up)
           const bool cv = cidx < (hi3 ? kw : (k + 2 <= iend ? 3 : 2));
-          zd* const cp = win + (cv ? (hi3 ? cidx : cidx + kw) : kw) + kw * ZWL;
+          zd* const cp = wk + (cv ? (hi3 ? cidx - kw : lane) : 0);
           zd ca = cp[0], cb = cp[ZWL];
-          if (k > k0) {
-            v0 = nv0;
-            v1 = nv1;
-          }
-          const zd tau1 = FAST ? zreflector2_fast<true>(v0, v1) : zreflector2(v0, v1);
+          v0 = nv0;
+          v1 = nv1;
+          const zd tau1 = FAST ? zreflector2_wave(v0, v1) : zreflector2(v0, v1);
           const zd v2 = v1;
-          const double tau2 = (tau1 * v2).x;
+          const double tau2 = tau1.x * v2.x - tau1.y * v2.y;
           {
-            const zd ss = zconj(tau1) * ra + tau2 * rb;
-            ra = ra - ss;
-            rb = rb - ss * v2;
+            // ss = conj(tau1) ra + tau2 rb; ra -= ss; rb -= ss v2
+            const double sx = fma(tau1.x, ra.x, fma(tau1.y, ra.y, tau2 * rb.x));
+            const double sy = fma(tau1.x, ra.y, fma(-tau1.y, ra.x, tau2 * rb.y));
+            ra = zmk(ra.x - sx, ra.y - sy);
+            rb = zmk(fma(-sx, v2.x, fma(sy, v2.y, rb.x)), fma(-sx, v2.y, fma(-sy, v2.x, rb.y)));
           }
           {
             // lane 0 takes lane 1's (k, k + 1), lane 1 takes lane 0's (k + 1, k)
             const zd x10 = zmk(qperm<0xF5>(ra.x), qperm<0xF5>(ra.y));
             const zd x01 = zmk(qperm<0xA0>(rb.x), qperm<0xA0>(rb.y));
-            if (lane == 0) {
-              ca = ra;
-              cb = x10;
-            } else if (lane == 1) {
-              ca = x01;
-              cb = rb;
-            }
+            const bool l0 = lane == 0, l1 = lane == 1;
+            ca = zmk(l0 ? ra.x : (l1 ? x01.x : ca.x), l0 ? ra.y : (l1 ? x01.y : ca.y));
+            cb = zmk(l0 ? x10.x : (l1 ? rb.x : cb.x), l0 ? x10.y : (l1 ? rb.y : cb.y));
           }
           {
-            const zd ss = tau1 * ca + tau2 * cb;
-            ca = ca - ss;
-            cb = cb - ss * zconj(v2);
+            // ss = tau1 ca + tau2 cb; ca -= ss; cb -= ss conj(v2)
+            const double sx = fma(tau1.x, ca.x, fma(-tau1.y, ca.y, tau2 * cb.x));
+            const double sy = fma(tau1.x, ca.y, fma(tau1.y, ca.x, tau2 * cb.y));
+            ca = zmk(ca.x - sx, ca.y - sy);
+            cb = zmk(fma(-sx, v2.x, fma(-sy, v2.y, cb.x)), fma(-sy, v2.x, fma(sx, v2.y, cb.y)));
           }
           nv0 = zmk(rdlane(ca.x, 1), rdlane(ca.y, 1));
           nv1 = zmk(rdlane(ca.x, 2), rdlane(ca.y, 2));
@@ -750,14 +753,14 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
           zd* const cq = cv ? cp : dmy;
           cq[0] = ca;
           cq[ZWL] = cb;
-          if (lane == 0) {
-            if (k > istart1) {
-              WIN(k, k - 1) = v0;
-              WIN(k + 1, k - 1) = zr(0.0);
-            }
-            lg[2 * (k - k0)] = tau1;
-            lg[2 * (k - k0) + 1] = v2;
-          }
+          // lane 0: the annihilated column's (k, k - 1) = beta, (k + 1, k - 1) = 0
+          // and the step's log entry; the other lanes store to the dummy area
+          zd* const bq = (lane == 0 && k > istart1) ? wk - ZWL : dmy;
+          bq[0] = v0;
+          bq[1] = zr(0.0);
+          zd* const lq = lane == 0 ? lgk : dmy;
+          lq[0] = tau1;
+          lq[1] = v2;
           wsync();
         }
         win_store<zd, ZW, ZWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
