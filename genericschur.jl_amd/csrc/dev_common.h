@@ -405,16 +405,14 @@ GS_DEV zd zreflector2_fast(zd& a, zd& x) {
 // zreflector2_fast for a wavefront whose lanes all hold the same a and x:
 // the formulas run straight through and one branch on the scalar unit
 // replaces them by zreflector2's when x = 0 with a real (tau = 0) or |beta|
-// lies outside [2^-300, 2^300].  Bitwise the same results as
-// zreflector2_fast.
+// lies outside [2^-300, 2^300].  Inside that range the sum of squares
+// needs no scaling: it lies in [2^-600, 2^600], so no square overflows and
+// squares that underflow are below its rounding; the square root is the
+// same iteration as sqrt_scaled's.
 GS_DEV bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
 GS_DEV zd zreflector2_wave(zd& a, zd& x) {
   const double ar = a.x, ai = a.y;
-  const double w = fmax(fmax(fabs(ar), fabs(ai)), fmax(fabs(x.x), fabs(x.y)));
-  int e;
-  (void)frexp(w, &e);
-  const double p = ldexp(ar, -e), q = ldexp(ai, -e), s = ldexp(x.x, -e), t = ldexp(x.y, -e);
-  const double beta = -copysign(ldexp(sqrt_scaled(p * p + q * q + s * s + t * t), e), ar);
+  const double beta = -copysign(sqrt_scaled(ar * ar + ai * ai + x.x * x.x + x.y * x.y), ar);
   const double ab = fabs(beta);
   const bool rare = ((x.x == 0.0) & (x.y == 0.0) & (ai == 0.0)) | !((ab >= 0x1p-300) & (ab <= 0x1p300));
   const double dr = ar - beta;

@@ -709,6 +709,7 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
         zd nv0 = zmk(rdfirst(v0.x), rdfirst(v0.y)), nv1 = zmk(rdfirst(v1.x), rdfirst(v1.y));
         zd* wk = &WIN(k0, k0);
         zd* lgk = lg;
+        double betal = 0.0;  // lane s: beta of step k0 + s
         for (int k = k0; k <= kend; ++k, wk += ZWL + 1, lgk += 2) {
           const int kw = k - wb;
           const bool rv = lane <= we - k;
@@ -753,14 +754,21 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
           zd* const cq = cv ? cp : dmy;
           cq[0] = ca;
           cq[ZWL] = cb;
-          // lane 0: the annihilated column's (k, k - 1) = beta, (k + 1, k - 1) = 0
-          // and the step's log entry; the other lanes store to the dummy area
-          zd* const bq = (lane == 0 && k > istart1) ? wk - ZWL : dmy;
-          bq[0] = v0;
-          bq[1] = zr(0.0);
+          betal = lane == k - k0 ? v0.x : betal;
+          // lane 0: the step's log entry; the other lanes store to the dummy area
           zd* const lq = lane == 0 ? lgk : dmy;
           lq[0] = tau1;
           lq[1] = v2;
+          wsync();
+        }
+        // the annihilated columns: (k, k - 1) = beta, (k + 1, k - 1) = 0 (no
+        // later step of the chunk reads them)
+        {
+          const int k = k0 + lane;
+          if (k <= kend && k > istart1) {
+            WIN(k, k - 1) = zr(betal);
+            WIN(k + 1, k - 1) = zr(0.0);
+          }
           wsync();
         }
         win_store<zd, ZW, ZWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
