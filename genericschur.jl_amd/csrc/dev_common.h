@@ -490,6 +490,14 @@ template <int CTRL> GS_DEV double qperm(double v) {
   return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
+// Workgroup barrier ordering LDS accesses only: global memory operations
+// still in flight stay in flight.
+GS_DEV void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 GS_DEV void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -528,7 +536,10 @@ GS_DEV zd zmul_fixed(zd a, zd b) {
 // column.  Lane l covers row l % R of columns l / R, l / R + 64 / R, ...;
 // loads clamp the row and column into the block and are not masked, stores
 // are.
-template <class T, int WMAX, int WL> GS_DEV void win_load(const T* src, int64_t ld, int W, T* win, int lane) {
+// (Entries in rows < rskip of columns >= cskip are left as they are: the
+// caller fills them from elsewhere.)
+template <class T, int WMAX, int WL>
+GS_DEV void win_load(const T* src, int64_t ld, int W, T* win, int lane, int rskip = 0, int cskip = 0) {
   constexpr int R = WMAX < 64 ? WMAX : 64;
   constexpr int CPI = 64 / R;
   constexpr int NI = WMAX / CPI;
@@ -545,10 +556,38 @@ template <class T, int WMAX, int WL> GS_DEV void win_load(const T* src, int64_t 
 #pragma unroll
     for (int i = 0; i < B; ++i) {
       const int c = CPI * (i0 + i) + ch;
-      if (r < W && c < W) win[r + c * WL] = t[i];
+      if (r < W && c < W && !(r < rskip && c >= cskip)) win[r + c * WL] = t[i];
     }
   }
 }
+// win_load in two halves, so that other work can run while the loads are in
+// flight: win_fetch issues one batch of loads (B = WMAX / (64 / R) entries
+// per lane) into t, win_put writes them to the window.
+template <class T, int WMAX> struct WinRegs {
+  static constexpr int R = WMAX < 64 ? WMAX : 64;
+  static constexpr int NI = WMAX / (64 / R);
+  T t[NI];
+};
+template <class T, int WMAX> GS_DEV void win_fetch(const T* src, int64_t ld, int W, WinRegs<T, WMAX>& w, int lane) {
+  constexpr int R = WinRegs<T, WMAX>::R, CPI = 64 / R, NI = WinRegs<T, WMAX>::NI;
+  W = __builtin_amdgcn_readfirstlane(W);
+  const int r = lane % R, ch = lane / R;
+  const T* bp = src + min(r, W - 1);
+#pragma unroll
+  for (int i = 0; i < NI; ++i) w.t[i] = bp[(int64_t)min(CPI * i + ch, W - 1) * ld];
+}
+template <class T, int WMAX, int WL>
+GS_DEV void win_put(const WinRegs<T, WMAX>& w, int W, T* win, int lane, int rskip = 0, int cskip = 0) {
+  constexpr int R = WinRegs<T, WMAX>::R, CPI = 64 / R, NI = WinRegs<T, WMAX>::NI;
+  W = __builtin_amdgcn_readfirstlane(W);
+  const int r = lane % R, ch = lane / R;
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int c = CPI * i + ch;
+    if (r < W && c < W && !(r < rskip && c >= cskip)) win[r + c * WL] = w.t[i];
+  }
+}
+
 template <class T, int WMAX, int WL> GS_DEV void win_store(T* dst, int64_t ld, int W, const T* win, int lane) {
   constexpr int R = WMAX < 64 ? WMAX : 64;
   constexpr int CPI = 64 / R;

@@ -518,8 +518,10 @@ constexpr int ZWL = ZW + 1;
 // registers in groups of ZCG elements, two groups in flight: a group's loads
 // are issued as soon as the group two places before it has been stored, so a
 // chain of a full chunk waits on memory about twice, not once per element.
+// tail2 (optional): receives the last two elements x(kb), x(kb+1) in place
+// of the line.
 constexpr int ZCG = 8;
-template <bool LEFT> GS_DEV void zchain(zd* p, int64_t st, const zd* lg, int ka, int kb) {
+template <bool LEFT> GS_DEV void zchain(zd* p, int64_t st, const zd* lg, int ka, int kb, zd* tail2 = nullptr) {
   constexpr int NG = (ZKC + 1 + ZCG - 1) / ZCG;
   const int s = __builtin_amdgcn_readfirstlane(kb - ka + 1);  // uniform: branches on SCC
   ka = __builtin_amdgcn_readfirstlane(ka);
@@ -562,7 +564,10 @@ template <bool LEFT> GS_DEV void zchain(zd* p, int64_t st, const zd* lg, int ka,
 #pragma unroll
     for (int i = 0; i < ZCG; ++i) {
       const int q = g * ZCG + i;
-      if (q <= s) q0[(int64_t)q * st] = x[q];
+      if (tail2 && q >= s - 1 && q <= s)
+        tail2[q - (s - 1)] = x[q];
+      else if (q <= s)
+        q0[(int64_t)q * st] = x[q];
     }
     __builtin_amdgcn_sched_barrier(0);
     if (g + 2 < NG && (g + 2) * ZCG <= s) {
@@ -676,14 +681,32 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
     const zd* plg = lgbuf + ((c + 1) & 1) * 2 * ZKC;
     if (wave == 0) {
       if (chase) {
-        if (prev) {
-          const int col = pwe + 1 + lane;
-          if (col <= we) zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend);
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-          __builtin_amdgcn_wave_barrier();
-        }
+        // the window's loads are issued first; while they are in flight the
+        // previous chunk's reflectors are applied to the window's columns
+        // right of the previous window (rows pk0 .. pkend + 1 in HBM), whose
+        // last two rows (the window's rows wb, wb + 1) go to the window
         const int W = we - wb + 1;
-        win_load<zd, ZW, ZWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
+        {
+          // window columns straight into LDS (lane r: row wb + r), the rows
+          // that the chains below provide left out
+          const int Wu = __builtin_amdgcn_readfirstlane(W);
+          const int cs = prev ? min(max(pwe + 1 - wb, 0), Wu) : Wu;
+          const zd* gp = H + (wb - 1) + (int64_t)(wb - 1) * ld + lane;
+          if (lane < Wu)
+            for (int c = 0; c < cs; ++c)
+              __builtin_amdgcn_global_load_lds((const void*)(gp + (int64_t)c * ld),
+                                               (__attribute__((address_space(3))) void*)(win + c * ZWL), 16, 0, 0);
+          if (lane < Wu && lane >= 2)
+            for (int c = cs; c < Wu; ++c)
+              __builtin_amdgcn_global_load_lds((const void*)(gp + (int64_t)c * ld),
+                                               (__attribute__((address_space(3))) void*)(win + c * ZWL), 16, 0, 0);
+          if (prev) {
+            const int col = pwe + 1 + lane;
+            if (col <= we)
+              zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend, win + (pkend - wb) + (col - wb) * ZWL);
+          }
+          __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the window is in LDS
+        }
         wsync();
         auto WIN = [&](int r, int cc) -> zd& { return win[(r - wb) + (cc - wb) * ZWL]; };
         // Step k in lanes: the row pass (rows k, k + 1) on column k + lane;
