@@ -1071,7 +1071,7 @@ template <int NT, bool FAST> __global__ void __launch_bounds__(NT) zqr_kernel(Ba
 
 // Two-wavefront windowed QR kernel: the chase of chunk c overlaps the chains
 // of chunk c-1 (z_sweep_win2); the control logic runs on both wavefronts.
-template <bool FAST> __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) zqrw2_kernel(BatchArgs a) {
+template <bool FAST> __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) zqrw2_kernel(BatchArgs a) {
   __shared__ zd win[ZWL * ZW + 4 * ZKC + ZWL + 1];
   __shared__ int redi[4];
   const int b = blockIdx.x;
