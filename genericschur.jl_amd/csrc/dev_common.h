@@ -522,7 +522,7 @@ GS_DEV void zrot_right_fixed(zd& x0, zd& x1, zd tau1, zd v2) {
   const double sr = fma(tau1.x, x0.x, fma(-tau1.y, x0.y, tau2 * x1.x));
   const double si = fma(tau1.x, x0.y, fma(tau1.y, x0.x, tau2 * x1.y));
   x0 = zmk(x0.x - sr, x0.y - si);
-  x1 = zmk(x1.x - fma(sr, v2.x, si * v2.y), x1.y - fma(si, v2.x, -(sr * v2.y)));
+  x1 = zmk(fma(-sr, v2.x, fma(-si, v2.y, x1.x)), fma(-si, v2.x, fma(sr, v2.y, x1.y)));
 }
 GS_DEV zd zmul_fixed(zd a, zd b) {
 #pragma clang fp contract(off)

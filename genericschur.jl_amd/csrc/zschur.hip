@@ -1305,16 +1305,16 @@ __global__ void __launch_bounds__(128) zlog_kernel(BatchArgs a) {
           for (int c = i1 + 2; c <= ie; ++c) row[(c - 1) * st] = zmul_fixed(row[(c - 1) * st], ctt);
         }
       }
-      // groups of four steps: the group's row entries (which its own steps
+      // groups of eight steps: the group's row entries (which its own steps
       // do not write first) are read before any of its writes, so a group
       // waits for LDS once
       int k = i1 + 1;
-      for (; k + 3 < ie; k += 4) {
-        zd xa[4];
+      for (; k + 7 < ie; k += 8) {
+        zd xa[8];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) xa[j] = row[(k + j) * st];
+        for (int j = 0; j < 8; ++j) xa[j] = row[(k + j) * st];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 8; ++j) {
           zrot_right_fixed(c0, xa[j], e[2 * (k + j - i1)], e[2 * (k + j - i1) + 1]);
           row[(k + j - 1) * st] = c0;
           c0 = xa[j];
