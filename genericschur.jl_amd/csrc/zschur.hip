@@ -799,18 +799,22 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
       }
     } else if (prev) {
       const int lcol0 = (chase ? we : pwe) + 1;
-      for (int col = lcol0 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend);
+      if (chase)  // with no chase left (the sweep's last chains) wavefront 0 takes the columns
+        for (int col = lcol0 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend);
       for (int row = 1 + lane; row <= pwb - 1; row += 64) zchain<false>(H + (row - 1), ld, plg, pk0, pkend);
       if (wantz && !zlg)
         for (int row = 1 + lane; row <= n; row += 64) zchain<false>(Zg + (row - 1), ldz, plg, pk0, pkend);
     }
+    if (wave == 0 && !chase && prev)
+      for (int col = pwe + 1 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend);
     __syncthreads();
     if (!chase) break;
     if (phase_pending) {
       // the first step's chains, then the phase scaling (GenericSchur.jl:461-482)
       phase_pending = false;
-      if (wave == 1) {
+      if (wave == 0) {
         for (int col = we + 1 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, lg, k0, kend);
+      } else {
         for (int row = 1 + lane; row <= wb - 1; row += 64) zchain<false>(H + (row - 1), ld, lg, k0, kend);
         if (wantz && !zlg)
           for (int row = 1 + lane; row <= n; row += 64) zchain<false>(Zg + (row - 1), ldz, lg, k0, kend);
