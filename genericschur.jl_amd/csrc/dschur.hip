@@ -1057,15 +1057,15 @@ __global__ void __launch_bounds__(128) dlog_kernel(BatchArgs a) {
       } else {
         const double* e = rec + 2;
         double c0 = row[(i1 - 1) * st], c1 = row[i1 * st];
-        // groups of four steps: the group's row entries (which its own steps
+        // groups of eight steps: the group's row entries (which its own steps
         // do not write first) are read before any of its writes
         int k = i1;
-        for (; k + 3 < ie - 1; k += 4) {
-          double bb[4];
+        for (; k + 7 < ie - 1; k += 8) {
+          double bb[8];
 #pragma unroll
-          for (int j = 0; j < 4; ++j) bb[j] = row[(k + j + 1) * st];
+          for (int j = 0; j < 8; ++j) bb[j] = row[(k + j + 1) * st];
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
+          for (int j = 0; j < 8; ++j) {
             drefl3_fixed(c0, c1, bb[j], e[3 * (k + j - i1)], e[3 * (k + j - i1) + 1], e[3 * (k + j - i1) + 2]);
             row[(k + j - 1) * st] = c0;
             c0 = c1;
