@@ -79,6 +79,19 @@ static int fail(gs_handle* h, int code, const char* msg) { return handle_fail(h,
 // GSCHUR_QR_BAND=1 selects the band-resident QR kernels everywhere (A/B
 // measurements), as GS_QR_BAND does per call; GSCHUR_QR_NOPRIO=1 keeps the
 // chasing wavefront of the windowed kernel at the default priority
+// Number of leading bins whose Hessenberg launches precede every other
+// launch of a batch: two for complex, one for real batches (GSCHUR_HQ_LEAD
+// overrides, 0..3).
+static int hq_lead(bool cplx) {
+  static const int v = [] {
+    const char* e = std::getenv("GSCHUR_HQ_LEAD");
+    if (!e) return -1;
+    const int x = std::atoi(e);
+    return x < 0 ? 0 : (x > 3 ? 3 : x);
+  }();
+  return v >= 0 ? v : (cplx ? 2 : 1);
+}
+
 static int env_qr_band() {
   static const int v = [] {
     const char* e = std::getenv("GSCHUR_QR_BAND");
@@ -118,6 +131,7 @@ int gs_create(gs_handle** out, int device) {
     e = hipStreamCreateWithFlags(&h->branch[i], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->evjoin[i], hipEventDisableTiming);
   }
+  for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&h->evlead[i], hipEventDisableTiming);
   if (e != hipSuccess) {
     delete h;
     return GS_E_HIP - (int)e;
@@ -147,6 +161,8 @@ int gs_destroy(gs_handle* h) {
     if (h->branch[i]) (void)hipStreamDestroy(h->branch[i]);
   }
   if (h->evfork) (void)hipEventDestroy(h->evfork);
+  for (int i = 0; i < 3; ++i)
+    if (h->evlead[i]) (void)hipEventDestroy(h->evlead[i]);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   for (hipEvent_t ev : h->tev) (void)hipEventDestroy(ev);
@@ -359,18 +375,43 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
   for (const Bin& b : p->bins) used[b.branch] = true;
   for (int i = 0; i < kBranches; ++i)
     if (used[i] && (e = hipStreamWaitEvent(h->branch[i], h->evfork, 0)) != hipSuccess) return e;
+  // the leading (largest) bins' Hessenberg launches run first, alone: the
+  // largest bin's chain (Hessenberg, QR, Z log) is the batch's longest, and
+  // started later it would finish last with the GPU half idle
+  const int lead = (int)std::min<size_t>((size_t)hq_lead(p->is_complex), p->bins.size());
+  auto launch_hq = [&](size_t bi) -> hipError_t {
+    const Bin& b = p->bins[bi];
+    a.first = b.first;
+    a.count = b.count;
+    hipStream_t s = h->branch[b.branch];
+    hipError_t e2;
+    if (timed && (e2 = hipEventRecord(h->tev[6 * bi], s)) != hipSuccess) return e2;
+    e2 = p->is_complex ? launch_zhq(a, b.nmax, b.nt_hq, s) : launch_dhq(a, b.nmax, b.nt_hq, s);
+    if (e2 != hipSuccess) return e2;
+    if (timed && (e2 = hipEventRecord(h->tev[6 * bi + 1], s)) != hipSuccess) return e2;
+    return hipSuccess;
+  };
+  for (int bi = 0; bi < lead; ++bi) {
+    if ((e = launch_hq(bi)) != hipSuccess) return e;
+    if ((e = hipEventRecord(h->evlead[bi], h->branch[p->bins[bi].branch])) != hipSuccess) return e;
+  }
+  bool waited[kBranches] = {false, false, false, false};
   for (size_t bi = 0; bi < p->bins.size(); ++bi) {
     const Bin& b = p->bins[bi];
+    hipStream_t s = h->branch[b.branch];
+    if ((int)bi >= lead) {
+      if (!waited[b.branch]) {
+        for (int j = 0; j < lead; ++j)
+          if (p->bins[j].branch != b.branch && (e = hipStreamWaitEvent(s, h->evlead[j], 0)) != hipSuccess) return e;
+        waited[b.branch] = true;
+      }
+      if ((e = launch_hq(bi)) != hipSuccess) return e;
+    }
     a.first = b.first;
     a.count = b.count;
     // the Z log pays off where Z's rows are long (its replay holds all of Z
     // in LDS); GS_ZLOG_TINY exercises it on every bin
     a.zlog = (zl && bin_zlog(p, b, flags)) ? zlog : nullptr;
-    hipStream_t s = h->branch[b.branch];
-    if (timed && (e = hipEventRecord(h->tev[6 * bi], s)) != hipSuccess) return e;
-    e = p->is_complex ? launch_zhq(a, b.nmax, b.nt_hq, s) : launch_dhq(a, b.nmax, b.nt_hq, s);
-    if (e != hipSuccess) return e;
-    if (timed && (e = hipEventRecord(h->tev[6 * bi + 1], s)) != hipSuccess) return e;
     if (timed && (e = hipEventRecord(h->tev[6 * bi + 2], s)) != hipSuccess) return e;
     if (flags & GS_QR_BAND)
       e = p->is_complex ? launch_zqr(a, b.nmax, b.nt_qr, s) : launch_dqr(a, b.nmax, b.nt_qr, s);

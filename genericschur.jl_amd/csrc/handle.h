@@ -49,6 +49,7 @@ struct gs_handle {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipStream_t branch[gs::kBranches] = {};
   hipEvent_t evfork = nullptr, evjoin[gs::kBranches] = {};
+  hipEvent_t evlead[3] = {};  // after the leading bins' Hessenberg launches
   std::string err;
   gs::DevBuf bA, bT, bZ, bW, bInfo, bWs, bStats, bMeta, bScl;
   gs::DevBuf bLog;  // Z log of the batched complex QR launches
