@@ -577,6 +577,52 @@ template <bool LEFT> GS_DEV void zchain(zd* p, int64_t st, const zd* lg, int ka,
   }
 }
 
+// zchain<true> for a full chunk (kb = ka + ZKC - 1) on a column of H whose
+// rows kb, kb + 1 go to tail2 (the next window) instead of HBM.  Every count
+// and index is a compile-time constant, so the compiler waits on memory once
+// per group rather than before every store.
+GS_DEV void zchain_full(zd* p, const zd* lg, int ka, zd* tail2) {
+  constexpr int S = ZKC, NG = (S + 1 + ZCG - 1) / ZCG;
+  ka = __builtin_amdgcn_readfirstlane(ka);
+  int lb = 0;
+  asm volatile("" : "+s"(lb));
+  const zd* lgq = lg + lb;
+  zd* q0 = p + (ka - 1);
+  zd x[NG * ZCG];
+#pragma unroll
+  for (int e = 0; e < 2 * ZCG; ++e)
+    if (e <= S) x[e] = q0[e];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+#pragma unroll
+    for (int i = 0; i < ZCG; ++i) {
+      const int q = g * ZCG + i;
+      if (q < S) {
+        const zd tau1 = lgq[2 * q], v2 = lgq[2 * q + 1];
+        const double tau2 = (tau1 * v2).x;
+        const zd a = x[q], b = x[q + 1];
+        const zd ss = zconj(tau1) * a + tau2 * b;
+        x[q] = a - ss;
+        x[q + 1] = b - ss * v2;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < ZCG; ++i) {
+      const int q = g * ZCG + i;
+      if (q < S - 1)
+        q0[q] = x[q];
+      else if (q <= S)
+        tail2[q - (S - 1)] = x[q];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < ZCG; ++i) {
+      const int e = (g + 2) * ZCG + i;
+      if (e <= S) x[e] = q0[e];
+    }
+  }
+}
+
 // Z log of the windowed QR kernel.  Z is not read while the sweeps run, so
 // instead of updating Z in HBM after every chunk the kernel records each
 // sweep and zlog_kernel applies the records to Z, held in LDS, afterwards:
@@ -692,6 +738,11 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
           const int Wu = __builtin_amdgcn_readfirstlane(W);
           const int cs = prev ? min(max(pwe + 1 - wb, 0), Wu) : Wu;
           const zd* gp = H + (wb - 1) + (int64_t)(wb - 1) * ld + lane;
+          if (prev) {
+            // the previous chunk is a full one (only a sweep's last chunk is shorter)
+            const int col = pwe + 1 + lane;
+            if (col <= we) zchain_full(H + (int64_t)(col - 1) * ld, plg, pk0, win + (pkend - wb) + (col - wb) * ZWL);
+          }
           if (lane < Wu)
             for (int c = 0; c < cs; ++c)
               __builtin_amdgcn_global_load_lds((const void*)(gp + (int64_t)c * ld),
@@ -700,11 +751,6 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
             for (int c = cs; c < Wu; ++c)
               __builtin_amdgcn_global_load_lds((const void*)(gp + (int64_t)c * ld),
                                                (__attribute__((address_space(3))) void*)(win + c * ZWL), 16, 0, 0);
-          if (prev) {
-            const int col = pwe + 1 + lane;
-            if (col <= we)
-              zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend, win + (pkend - wb) + (col - wb) * ZWL);
-          }
           __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the window is in LDS
         }
         wsync();
