@@ -36,10 +36,81 @@ template <int NT, int RM = 0> GS_DEV void d_hessenberg(double* H, int64_t ld, in
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const double sfmin = 2.0 * GS_FMIN / GS_EPS;
+  // RM > 0: wavefront 0 computes reflector p + 1 (column p + 1, final once
+  // the right update's dot products are done) while the other wavefronts
+  // apply reflector p's right update to columns p + 2 .. n - 1
+  constexpr bool OVL = RM > 0 && NT >= 128;
+  // the reflector by wavefront 0 alone (householder.jl:15-44)
+  auto reflect = [&](const int p) {
+    const int r0 = p + 2;
+    double t_ = 0.0;
+    if (n - 1 - p > 1) {
+      double amax = 0.0;
+      for (int r = r0 + lane; r < n; r += 64) amax = fmax(amax, fabs(H[r + p * ld]));
+      amax = wave_max(amax);
+      double xnorm = 0.0;
+      if (amax > 0.0) {
+        const double ra = 1.0 / amax;
+        double s = 0.0;
+        for (int r = r0 + lane; r < n; r += 64) {
+          const double a = H[r + p * ld] * ra;
+          s += a * a;
+        }
+        s = wave_sum(s);
+        xnorm = amax * sqrt(s);
+      }
+      if (xnorm != 0.0) {
+        double alpha = H[(p + 1) + p * ld];
+        double beta = -copysign(hypot(alpha, xnorm), alpha);
+        int kount = 0;
+        if (fabs(beta) < sfmin) {
+          const double rsfmin = 1.0 / sfmin;
+          bool smallb = true;
+          while (smallb) {
+            kount += 1;
+            wsync();
+            for (int r = r0 + lane; r < n; r += 64) H[r + p * ld] *= rsfmin;
+            beta *= rsfmin;
+            alpha *= rsfmin;
+            smallb = (fabs(beta) < sfmin) && (kount < 20);
+          }
+          wsync();
+          double am2 = 0.0;
+          for (int r = r0 + lane; r < n; r += 64) am2 = fmax(am2, fabs(H[r + p * ld]));
+          am2 = wave_max(am2);
+          xnorm = 0.0;
+          if (am2 > 0.0) {
+            const double ra = 1.0 / am2;
+            double s = 0.0;
+            for (int r = r0 + lane; r < n; r += 64) {
+              const double a = H[r + p * ld] * ra;
+              s += a * a;
+            }
+            s = wave_sum(s);
+            xnorm = am2 * sqrt(s);
+          }
+          beta = -copysign(hypot(alpha, xnorm), alpha);
+        }
+        t_ = (beta - alpha) / beta;
+        const double sc = 1.0 / (alpha - beta);
+        wsync();
+        for (int r = r0 + lane; r < n; r += 64) H[r + p * ld] *= sc;
+        for (int j = 0; j < kount; ++j) beta *= sfmin;
+        if (lane == 0) H[(p + 1) + p * ld] = beta;
+      }
+    }
+    if (lane == 0) tau[p] = t_;
+  };
+  if (OVL) {
+    if (wave == 0 && n >= 2) reflect(0);
+    __syncthreads();
+  }
   for (int p = 0; p <= n - 2; ++p) {
     const int r0 = p + 2;
     double t_ = 0.0;
-    if (n - 1 - p > 1) {  // reflector of length >= 2 (householder.jl:15)
+    if (OVL) {
+      t_ = tau[p];
+    } else if (n - 1 - p > 1) {  // reflector of length >= 2 (householder.jl:15)
       double amax = 0.0;
       for (int r = r0 + tid; r < n; r += NT) amax = fmax(amax, fabs(H[r + p * ld]));
       amax = block_max<NT>(amax, red);
@@ -94,8 +165,10 @@ template <int NT, int RM = 0> GS_DEV void d_hessenberg(double* H, int64_t ld, in
         if (tid == 0) H[(p + 1) + p * ld] = beta;
       }
     }
-    if (tid == 0) tau[p] = t_;
-    __syncthreads();
+    if (!OVL) {
+      if (tid == 0) tau[p] = t_;
+      __syncthreads();
+    }
     // lmul!(H', A[p+1:n, p+1:n]): va_j = tau (H[p+1, j] + sum_r v_r H[r, j]);
     // column j -= va_j [1; v]
     ddots4<NT>(p + 1, n, r0, n, [&](int r, int j) { return H[r + p * ld] * H[r + (int64_t)j * ld]; }, wk);
@@ -141,16 +214,21 @@ template <int NT, int RM = 0> GS_DEV void d_hessenberg(double* H, int64_t ld, in
     }
     __syncthreads();
     if constexpr (RM > 0) {
-      double xr[RM];
+      if (OVL && wave == 0) {
+        if (p + 1 <= n - 2) reflect(p + 1);
+      } else {
+        double xr[RM];
 #pragma unroll
-      for (int q = 0; q < RM; ++q) xr[q] = (lane + 64 * q < n) ? wk[lane + 64 * q] : 0.0;
-      #pragma unroll 4
-      for (int c = r0 + wave; c < n; c += NT / 64) {
-        const double vc = H[c + p * ld];
+        for (int q = 0; q < RM; ++q) xr[q] = (lane + 64 * q < n) ? wk[lane + 64 * q] : 0.0;
+        constexpr int NW = OVL ? NT / 64 - 1 : NT / 64;
+#pragma unroll 4
+        for (int c = r0 + wave - (OVL ? 1 : 0); c < n; c += NW) {
+          const double vc = H[c + p * ld];
 #pragma unroll
-        for (int q = 0; q < RM; ++q) {
-          const int r = lane + 64 * q;
-          if (r < n) H[r + (int64_t)c * ld] += xr[q] * vc;
+          for (int q = 0; q < RM; ++q) {
+            const int r = lane + 64 * q;
+            if (r < n) H[r + (int64_t)c * ld] += xr[q] * vc;
+          }
         }
       }
     } else {

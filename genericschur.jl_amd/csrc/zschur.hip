@@ -44,77 +44,86 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const double sfmin = GS_FMIN / GS_EPS;
-  for (int p = 0; p <= n - 2; ++p) {
-    const int r0 = p + 2;  // first tail row
-    // reflector on x = H[p+1 : n-1, p] (householder.jl:56-102), computed by
-    // wavefront 0 alone: its reductions need no workgroup barrier
-    if (wave == 0) {
-      double amax = 0.0;
+  // RM > 0: wavefront 0 computes reflector p + 1 (column p + 1, final once
+  // the right update's dot products are done) while the other wavefronts
+  // apply reflector p's right update to columns p + 2 .. n - 1
+  constexpr bool OVL = RM > 0 && NT >= 128;
+  // reflector on x = H[p+1 : n-1, p] (householder.jl:56-102), computed by
+  // wavefront 0 alone: its reductions need no workgroup barrier
+  auto reflect = [&](const int p) {
+    const int r0 = p + 2;
+    double amax = 0.0;
+    for (int r = r0 + lane; r < n; r += 64) {
+      const zd v = H[r + p * ld];
+      amax = fmax(amax, fmax(fabs(v.x), fabs(v.y)));
+    }
+    amax = wave_max(amax);
+    double xnorm = 0.0;
+    if (amax > 0.0) {
+      const double ra = 1.0 / amax;
+      double s = 0.0;
       for (int r = r0 + lane; r < n; r += 64) {
         const zd v = H[r + p * ld];
-        amax = fmax(amax, fmax(fabs(v.x), fabs(v.y)));
+        const double a = v.x * ra, b = v.y * ra;
+        s += a * a + b * b;
       }
-      amax = wave_max(amax);
-      double xnorm = 0.0;
-      if (amax > 0.0) {
-        const double ra = 1.0 / amax;
-        double s = 0.0;
+      s = wave_sum(s);
+      xnorm = amax * sqrt(s);
+    }
+    zd alpha = H[(p + 1) + p * ld];
+    zd t_ = zr(0.0);
+    if (!(xnorm == 0.0 && alpha.y == 0.0)) {
+      double ar = alpha.x, ai = alpha.y;
+      double beta = -copysign(hypot3(ar, ai, xnorm), ar);
+      int kount = 0;
+      if (fabs(beta) < sfmin) {
+        const double rsfmin = 1.0 / sfmin;
+        bool smallb = true;
+        while (smallb) {
+          kount += 1;
+          wsync();
+          for (int r = r0 + lane; r < n; r += 64) H[r + p * ld] = H[r + p * ld] * rsfmin;
+          beta *= rsfmin;
+          ar *= rsfmin;
+          ai *= rsfmin;
+          smallb = (fabs(beta) < sfmin) && (kount < 20);
+        }
+        wsync();
+        double am2 = 0.0;
         for (int r = r0 + lane; r < n; r += 64) {
           const zd v = H[r + p * ld];
-          const double a = v.x * ra, b = v.y * ra;
-          s += a * a + b * b;
+          am2 = fmax(am2, fmax(fabs(v.x), fabs(v.y)));
         }
-        s = wave_sum(s);
-        xnorm = amax * sqrt(s);
-      }
-      zd alpha = H[(p + 1) + p * ld];
-      zd t_ = zr(0.0);
-      if (!(xnorm == 0.0 && alpha.y == 0.0)) {
-        double ar = alpha.x, ai = alpha.y;
-        double beta = -copysign(hypot3(ar, ai, xnorm), ar);
-        int kount = 0;
-        if (fabs(beta) < sfmin) {
-          const double rsfmin = 1.0 / sfmin;
-          bool smallb = true;
-          while (smallb) {
-            kount += 1;
-            wsync();
-            for (int r = r0 + lane; r < n; r += 64) H[r + p * ld] = H[r + p * ld] * rsfmin;
-            beta *= rsfmin;
-            ar *= rsfmin;
-            ai *= rsfmin;
-            smallb = (fabs(beta) < sfmin) && (kount < 20);
-          }
-          wsync();
-          double am2 = 0.0;
+        am2 = wave_max(am2);
+        xnorm = 0.0;
+        if (am2 > 0.0) {
+          const double ra = 1.0 / am2;
+          double s = 0.0;
           for (int r = r0 + lane; r < n; r += 64) {
             const zd v = H[r + p * ld];
-            am2 = fmax(am2, fmax(fabs(v.x), fabs(v.y)));
+            const double a = v.x * ra, b = v.y * ra;
+            s += a * a + b * b;
           }
-          am2 = wave_max(am2);
-          xnorm = 0.0;
-          if (am2 > 0.0) {
-            const double ra = 1.0 / am2;
-            double s = 0.0;
-            for (int r = r0 + lane; r < n; r += 64) {
-              const zd v = H[r + p * ld];
-              const double a = v.x * ra, b = v.y * ra;
-              s += a * a + b * b;
-            }
-            s = wave_sum(s);
-            xnorm = am2 * sqrt(s);
-          }
-          alpha = zmk(ar, ai);
-          beta = -copysign(hypot3(ar, ai, xnorm), ar);
+          s = wave_sum(s);
+          xnorm = am2 * sqrt(s);
         }
-        t_ = zmk((beta - ar) / beta, -ai / beta);
-        const zd sc = zdiv(zr(1.0), alpha - zr(beta));
-        for (int r = r0 + lane; r < n; r += 64) H[r + p * ld] = H[r + p * ld] * sc;
-        for (int j = 0; j < kount; ++j) beta *= sfmin;
-        if (lane == 0) H[(p + 1) + p * ld] = zr(beta);
+        alpha = zmk(ar, ai);
+        beta = -copysign(hypot3(ar, ai, xnorm), ar);
       }
-      if (lane == 0) tau[p] = t_;
+      t_ = zmk((beta - ar) / beta, -ai / beta);
+      const zd sc = zdiv(zr(1.0), alpha - zr(beta));
+      for (int r = r0 + lane; r < n; r += 64) H[r + p * ld] = H[r + p * ld] * sc;
+      for (int j = 0; j < kount; ++j) beta *= sfmin;
+      if (lane == 0) H[(p + 1) + p * ld] = zr(beta);
     }
+    if (lane == 0) tau[p] = t_;
+  };
+  if (OVL) {
+    if (wave == 0 && n >= 2) reflect(0);
+  }
+  for (int p = 0; p <= n - 2; ++p) {
+    const int r0 = p + 2;  // first tail row
+    if (!OVL && wave == 0) reflect(p);
     __syncthreads();
     const zd t_ = tau[p];
     const zd ct = zconj(t_);
@@ -165,17 +174,22 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
                 });
     __syncthreads();
     if constexpr (RM > 0) {
+      if (OVL && wave == 0) {
+        if (p + 1 <= n - 2) reflect(p + 1);
+      } else {
       zd xr[RM];
 #pragma unroll
       for (int q = 0; q < RM; ++q) xr[q] = (lane + 64 * q < n) ? wk[lane + 64 * q] : zr(0.0);
+      constexpr int NW = OVL ? NT / 64 - 1 : NT / 64;
 #pragma unroll 4
-      for (int c = r0 + wave; c < n; c += NT / 64) {
+      for (int c = r0 + wave - (OVL ? 1 : 0); c < n; c += NW) {
         const zd vc = zconj(H[c + p * ld]);
 #pragma unroll
         for (int q = 0; q < RM; ++q) {
           const int r = lane + 64 * q;
           if (r < n) H[r + (int64_t)c * ld] = H[r + (int64_t)c * ld] + xr[q] * vc;
         }
+      }
       }
     } else {
 #pragma unroll 4
