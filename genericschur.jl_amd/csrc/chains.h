@@ -92,6 +92,43 @@ GS_DEV void chain_line(double* p, int64_t st, const double* lg, int mx, int ka, 
   }
 }
 
+// chain_line for a full chunk of three-element reflectors (kb = ka + MAXS - 1,
+// kb <= iend - 2): every count and index is a compile-time constant, so the
+// compiler waits on memory once per group rather than before every store.
+// Same operations in the same order as chain_line.
+template <int MAXS, int G>
+GS_DEV void chain_line_full(double* p, int64_t st, const double* lg, int mx, int ka) {
+  constexpr int S = MAXS, NE = MAXS + 2, NG = (NE + G - 1) / G;
+  ka = __builtin_amdgcn_readfirstlane(ka);
+  int lb = 3 * (ka - __builtin_amdgcn_readfirstlane(mx));
+  asm volatile("" : "+s"(lb));
+  const double* lv = lg + lb;
+  double* q0 = p + (int64_t)(ka - 1) * st;
+  double x[NG * G];
+#pragma unroll
+  for (int e = 0; e < 2 * G; ++e)
+    if (e < NE) x[e] = q0[(int64_t)e * st];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int q = g * G + i;
+      if (q < S) drefl3_fixed(x[q], x[q + 1], x[q + 2], lv[3 * q], lv[3 * q + 1], lv[3 * q + 2]);
+    }
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int q = g * G + i;
+      if (q < NE) q0[(int64_t)q * st] = x[q];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int e = (g + 2) * G + i;
+      if (e < NE) x[e] = q0[(int64_t)e * st];
+    }
+  }
+}
+
 // Real Z log (the windowed batched QR kernel, bins n = 65 .. 128): records
 // of doubles, a sweep (mx, iend) then (tau1, v2, v3) per step k = mx ..
 // iend-1 (the last step's reflector has two elements), or a Givens rotation
