@@ -684,7 +684,7 @@ template <bool FAST> GS_DEV void d_sweep_win(double* H, int64_t ld, int n, doubl
     if (kend - k0 + 1 == DKC && kend <= iend - 2) {  // a full chunk of three-element reflectors
       for (int col = we + 1 + lane; col <= n; col += 64)
         chain_line_full<DKC, 6>(H + (int64_t)(col - 1) * ld, 1, lg, k0, k0);
-      for (int row = 1 + lane; row <= wb - 1; row += 64) chain_line<DKC, 6>(H + (row - 1), ld, lg, k0, k0, kend, iend);
+      for (int row = 1 + lane; row <= wb - 1; row += 64) chain_line_full<DKC, 3>(H + (row - 1), ld, lg, k0, k0);
     } else {
       for (int col = we + 1 + lane; col <= n; col += 64)
         chain_line<DKC, 6>(H + (int64_t)(col - 1) * ld, 1, lg, k0, k0, kend, iend);
@@ -693,7 +693,10 @@ template <bool FAST> GS_DEV void d_sweep_win(double* H, int64_t ld, int n, doubl
     if (zlg) {
       for (int i = lane; i < 3 * (kend - k0 + 1); i += 64) rec[2 + 3 * (k0 - mx) + i] = lg[i];
     } else if (wantz) {
-      for (int row = 1 + lane; row <= n; row += 64) chain_line<DKC, 6>(Zg + (row - 1), ldz, lg, k0, k0, kend, iend);
+      if (kend - k0 + 1 == DKC && kend <= iend - 2)
+        for (int row = 1 + lane; row <= n; row += 64) chain_line_full<DKC, 3>(Zg + (row - 1), ldz, lg, k0, k0);
+      else
+        for (int row = 1 + lane; row <= n; row += 64) chain_line<DKC, 6>(Zg + (row - 1), ldz, lg, k0, k0, kend, iend);
     }
     __syncthreads();
     k0 = kend + 1;
