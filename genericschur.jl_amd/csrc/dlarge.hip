@@ -604,11 +604,18 @@ __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
   const int we = chase ? min(ck.kend + 3, iend) : 0;
   const int pwe = prev ? min(ck.pkend + 3, iend) : 0;
   const int plast = prev ? min(ck.pkend + 2, iend) : 0;
+  // a full chunk of three-element reflectors: fixed-shape chains
+  const bool pfull = prev && ck.pkend - ck.pk0 + 1 == LKC && ck.pkend <= iend - 2;
   if (blockIdx.x == 0) {
     if (!chase) return;
     if (prev) {
       const int col = pwe + 1 + lane;
-      if (col <= we) chain_line<LKC, 16>(a.H + (int64_t)(col - 1) * ld, 1, a.lg, mx, ck.pk0, ck.pkend, iend);
+      if (col <= we) {
+        if (pfull)
+          chain_line_full<LKC, 16>(a.H + (int64_t)(col - 1) * ld, 1, a.lg, mx, ck.pk0);
+        else
+          chain_line<LKC, 16>(a.H + (int64_t)(col - 1) * ld, 1, a.lg, mx, ck.pk0, ck.pkend, iend);
+      }
       __syncthreads();
     }
     const int wb = max(ck.k0 - 1, 1);
@@ -714,13 +721,28 @@ __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
   for (int t = blockIdx.x - 1; t < ntask; t += gridDim.x - 1) {
     if (t < nL) {
       const int col = lcol0 + 64 * t + lane;
-      if (col <= n) chain_line<LKC, 16>(a.H + (int64_t)(col - 1) * ld, 1, a.lg, mx, ck.pk0, ck.pkend, iend);
+      if (col <= n) {
+        if (pfull)
+          chain_line_full<LKC, 16>(a.H + (int64_t)(col - 1) * ld, 1, a.lg, mx, ck.pk0);
+        else
+          chain_line<LKC, 16>(a.H + (int64_t)(col - 1) * ld, 1, a.lg, mx, ck.pk0, ck.pkend, iend);
+      }
     } else if (t < nL + nR) {
       const int row = 1 + 64 * (t - nL) + lane;
-      if (row <= pwb - 1) chain_line<LKC, 16>(a.H + (row - 1), ld, a.lg, mx, ck.pk0, ck.pkend, iend);
+      if (row <= pwb - 1) {
+        if (pfull)
+          chain_line_full<LKC, 16>(a.H + (row - 1), ld, a.lg, mx, ck.pk0);
+        else
+          chain_line<LKC, 16>(a.H + (row - 1), ld, a.lg, mx, ck.pk0, ck.pkend, iend);
+      }
     } else {
       const int row = 1 + 64 * (t - nL - nR) + lane;
-      if (row <= n) chain_line<LKC, 16>(a.Z + (row - 1), a.ldz, a.lg, mx, ck.pk0, ck.pkend, iend);
+      if (row <= n) {
+        if (pfull)
+          chain_line_full<LKC, 16>(a.Z + (row - 1), a.ldz, a.lg, mx, ck.pk0);
+        else
+          chain_line<LKC, 16>(a.Z + (row - 1), a.ldz, a.lg, mx, ck.pk0, ck.pkend, iend);
+      }
     }
   }
   (void)plast;
