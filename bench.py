@@ -13,8 +13,9 @@ Alongside the throughput it reports:
                 matrix for Hessenberg + Q + sweeps, Golub & Van Loan 4th ed.
                 7.5.6, minus the 56/3 n^3 of Hessenberg + Q done by the
                 other kernel) over its average launch duration, measured with
-                HIP events around every launch on its own stream in one
-                instrumented (eager, graph-free) pass after the timed steps,
+                HIP events around every launch in one instrumented (eager,
+                graph-free) pass after the timed steps that runs each launch
+                alone, in order, on one stream,
                 against the 78.6 TFLOP/s FP64 peak; HBM traffic per launch
                 from rocprofv3 FETCH_SIZE/WRITE_SIZE runs when a summary of
                 them is present in profiles/;
@@ -144,8 +145,8 @@ def main():
     t2 = time.perf_counter()
     wall = t2 - t1
     kern_s = ev0.elapsed_time(ev1) * 1e-3 / args.steps
-    # one instrumented pass (not part of the timed steps): events around every
-    # launch on the stream it runs on
+    # one instrumented pass (not part of the timed steps): every launch alone,
+    # in order on one stream, between two events
     gs.gschur_batched_device(ns, off, A, T, Z, W, info, complex_input=True, wantZ=True, scale=True, timing=True)
     recs = _lib.last_launches()
     gs.set_stream(None)
@@ -221,8 +222,8 @@ def main():
             "flops_per_launch": qr_flops / max(len(qr), 1),
             "note": "FP64 vector-ALU work (the bulge chase has no dense contraction; FP64 vector and FP64 MFMA "
                     "peaks are both 78.6 TFLOP/s). achieved = model flops of a bin's QR phase (81.3 n^3 per "
-                    "complex matrix: H and Z updates) / its average duration (HIP events on the launches' "
-                    "stream around the QR-sweep and Z-log launches, instrumented pass)",
+                    "complex matrix: H and Z updates) / its average duration (HIP events around the QR-sweep "
+                    "and Z-log launches, instrumented pass with every launch alone)",
             "traffic_source": traffic["source"] if traffic else None,
             "hq_kernel_avg_launch_us": sum(r["us"] for r in hq) / max(len(hq), 1),
             "batch_ms": kern_s * 1e3,

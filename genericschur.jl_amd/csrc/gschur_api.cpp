@@ -338,9 +338,10 @@ static bool bin_zlog(const Plan* p, const Bin& b, int kflags) {
   return b.nmax >= (p->is_complex ? kZlogMinN : kDlogMinN) || (kflags & GS_ZLOG_TINY);
 }
 
-// With timed set, events bracket every launch on its stream (eager launches
-// only, never under capture): events 6b .. 6b+5 for bin b (Hessenberg, QR,
-// Z log).  zlog: the Z-log buffer when uses_zlog.
+// With timed set, every launch runs alone, in order on the handle's stream,
+// between two events (eager launches only, never under capture): events
+// 6b .. 6b+5 for bin b (Hessenberg, QR, Z log), so each record is the
+// launch's own duration.  zlog: the Z-log buffer when uses_zlog.
 static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T, double* Z, double* W,
                               int32_t* info, int flags, double* zlog, bool timed = false) {
   BatchArgs a;
@@ -379,11 +380,12 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
   // largest bin's chain (Hessenberg, QR, Z log) is the batch's longest, and
   // started later it would finish last with the GPU half idle
   const int lead = (int)std::min<size_t>((size_t)hq_lead(p->is_complex), p->bins.size());
+  auto bstream = [&](const Bin& b) { return timed ? h->stream : h->branch[b.branch]; };
   auto launch_hq = [&](size_t bi) -> hipError_t {
     const Bin& b = p->bins[bi];
     a.first = b.first;
     a.count = b.count;
-    hipStream_t s = h->branch[b.branch];
+    hipStream_t s = bstream(b);
     hipError_t e2;
     if (timed && (e2 = hipEventRecord(h->tev[6 * bi], s)) != hipSuccess) return e2;
     e2 = p->is_complex ? launch_zhq(a, b.nmax, b.nt_hq, s) : launch_dhq(a, b.nmax, b.nt_hq, s);
@@ -393,12 +395,12 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
   };
   for (int bi = 0; bi < lead; ++bi) {
     if ((e = launch_hq(bi)) != hipSuccess) return e;
-    if ((e = hipEventRecord(h->evlead[bi], h->branch[p->bins[bi].branch])) != hipSuccess) return e;
+    if ((e = hipEventRecord(h->evlead[bi], bstream(p->bins[bi]))) != hipSuccess) return e;
   }
   bool waited[kBranches] = {false, false, false, false};
   for (size_t bi = 0; bi < p->bins.size(); ++bi) {
     const Bin& b = p->bins[bi];
-    hipStream_t s = h->branch[b.branch];
+    hipStream_t s = bstream(b);
     if ((int)bi >= lead) {
       if (!waited[b.branch]) {
         for (int j = 0; j < lead; ++j)
