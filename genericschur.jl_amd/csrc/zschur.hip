@@ -595,7 +595,7 @@ template <bool LEFT> GS_DEV void zchain(zd* p, int64_t st, const zd* lg, int ka,
 // rows kb, kb + 1 go to tail2 (the next window) instead of HBM.  Every count
 // and index is a compile-time constant, so the compiler waits on memory once
 // per group rather than before every store.
-GS_DEV void zchain_full(zd* p, const zd* lg, int ka, zd* tail2) {
+template <bool TAIL = true> GS_DEV void zchain_full(zd* p, const zd* lg, int ka, zd* tail2) {
   constexpr int S = ZKC, NG = (S + 1 + ZCG - 1) / ZCG;
   ka = __builtin_amdgcn_readfirstlane(ka);
   int lb = 0;
@@ -623,7 +623,7 @@ GS_DEV void zchain_full(zd* p, const zd* lg, int ka, zd* tail2) {
 #pragma unroll
     for (int i = 0; i < ZCG; ++i) {
       const int q = g * ZCG + i;
-      if (q < S - 1)
+      if (q < S - 1 || (!TAIL && q <= S))
         q0[q] = x[q];
       else if (q <= S)
         tail2[q - (S - 1)] = x[q];
@@ -633,6 +633,40 @@ GS_DEV void zchain_full(zd* p, const zd* lg, int ka, zd* tail2) {
     for (int i = 0; i < ZCG; ++i) {
       const int e = (g + 2) * ZCG + i;
       if (e <= S) x[e] = q0[e];
+    }
+  }
+}
+
+// zchain<false> (a row of H or Z, x(t) = p[(t - 1) st]) for a full chunk:
+// fixed shape, as zchain_full.
+template <int G> GS_DEV void zchain_full_row(zd* p, int64_t st, const zd* lg, int ka) {
+  constexpr int ZCG = G, S = ZKC, NG = (S + 1 + ZCG - 1) / ZCG;
+  ka = __builtin_amdgcn_readfirstlane(ka);
+  int lb = 0;
+  asm volatile("" : "+s"(lb));
+  const zd* lgq = lg + lb;
+  zd* q0 = p + (int64_t)(ka - 1) * st;
+  zd x[NG * ZCG];
+#pragma unroll
+  for (int e = 0; e < 2 * ZCG; ++e)
+    if (e <= S) x[e] = q0[(int64_t)e * st];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+#pragma unroll
+    for (int i = 0; i < ZCG; ++i) {
+      const int q = g * ZCG + i;
+      if (q < S) zrot_right_fixed(x[q], x[q + 1], lgq[2 * q], lgq[2 * q + 1]);
+    }
+#pragma unroll
+    for (int i = 0; i < ZCG; ++i) {
+      const int q = g * ZCG + i;
+      if (q <= S) q0[(int64_t)q * st] = x[q];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < ZCG; ++i) {
+      const int e = (g + 2) * ZCG + i;
+      if (e <= S) x[e] = q0[(int64_t)e * st];
     }
   }
 }
@@ -859,11 +893,21 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
       }
     } else if (prev) {
       const int lcol0 = (chase ? we : pwe) + 1;
-      if (chase)  // with no chase left (the sweep's last chains) wavefront 0 takes the columns
-        for (int col = lcol0 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend);
-      for (int row = 1 + lane; row <= pwb - 1; row += 64) zchain<false>(H + (row - 1), ld, plg, pk0, pkend);
-      if (wantz && !zlg)
-        for (int row = 1 + lane; row <= n; row += 64) zchain<false>(Zg + (row - 1), ldz, plg, pk0, pkend);
+      if (chase) {  // with no chase left (the sweep's last chains) wavefront 0 takes the columns
+        if (pkend - pk0 + 1 == ZKC)
+          for (int col = lcol0 + lane; col <= n; col += 64) zchain_full<false>(H + (int64_t)(col - 1) * ld, plg, pk0, nullptr);
+        else
+          for (int col = lcol0 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend);
+      }
+      if (chase && pkend - pk0 + 1 == ZKC) {
+        for (int row = 1 + lane; row <= pwb - 1; row += 64) zchain_full_row<4>(H + (row - 1), ld, plg, pk0);
+        if (wantz && !zlg)
+          for (int row = 1 + lane; row <= n; row += 64) zchain_full_row<4>(Zg + (row - 1), ldz, plg, pk0);
+      } else {
+        for (int row = 1 + lane; row <= pwb - 1; row += 64) zchain<false>(H + (row - 1), ld, plg, pk0, pkend);
+        if (wantz && !zlg)
+          for (int row = 1 + lane; row <= n; row += 64) zchain<false>(Zg + (row - 1), ldz, plg, pk0, pkend);
+      }
     }
     if (wave == 0 && !chase && prev)
       for (int col = pwe + 1 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend);
