@@ -1236,6 +1236,7 @@ hipError_t launch_dhq(const BatchArgs& a, int nmax, int nt, hipStream_t s) {
     case 128: return launch_dhq_nt<128>(a, lds, s);
     case 256: return launch_dhq_nt<256>(a, lds, s);
     case 512: return launch_dhq_nt<512>(a, lds, s);
+    case 1024: return launch_dhq_nt<1024>(a, lds, s);
     default: return hipErrorInvalidValue;
   }
 }

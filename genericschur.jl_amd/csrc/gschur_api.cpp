@@ -194,13 +194,13 @@ int gs_batched_max_n(int is_complex) { return is_complex ? zschur_max_lds_n() : 
 
 // ------------------------------------------------------------------ plans
 // GSCHUR_HQ_NT overrides the Hessenberg+Q workgroup size of bins above
-// n = 32 (A/B measurements; 256 or 512)
+// n = 32 (A/B measurements; 256, 512 or 1024)
 static int default_nt_hq(bool cplx, int nmax) {
   (void)cplx;
   static const int big = [] {
     const char* e = std::getenv("GSCHUR_HQ_NT");
     const int v = e ? std::atoi(e) : 0;
-    return (v == 256 || v == 512) ? v : 256;
+    return (v == 256 || v == 512 || v == 1024) ? v : 512;
   }();
   return nmax > 32 ? big : 64;
 }
