@@ -317,11 +317,11 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
 }
 
 // Bin orders whose QR launch logs its Z updates.  The smallest: measured on
-// MI355X, 1400 complex matrices of one order, QR + replay against in-place
-// updates: n = 96 36 ms vs 42 ms, n = 64 14.4 ms vs 13.2 ms, n = 32 equal.
-// The largest real one: the replay holds all of Z in LDS with its record
-// buffers (128^2 doubles + 6 KB).
-constexpr int kZlogMinN = 65;
+// MI355X with each launch alone, a bin's QR + replay against in-place
+// updates: n = 81..96 (1470 matrices) 25.1 ms vs 32.4 ms, n = 65..80 (1479)
+// 16.7 ms vs 15.0 ms.  The largest real one: the replay holds all of Z in
+// LDS with its record buffers (128^2 doubles + 6 KB).
+constexpr int kZlogMinN = 81;
 constexpr int kDlogMinN = 33;
 constexpr int kDlogMaxN = 128;
 
