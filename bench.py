@@ -159,7 +159,7 @@ def main():
     bytes_alg = float((3.0 * n64 ** 2 * 16 + 16 * n64).sum())
     batch_rate = flops / kern_s
     # dominant kernel: the QR phase of each size bin, the QR-sweep launch
-    # plus (bins above n = 64) the launch applying its Z log
+    # plus (the bin above n = 80) the launch applying its Z log
     qr = [r for r in recs if r["kind"] == "qr"]
     zl = [r for r in recs if r["kind"] == "zlog"]
     hq = [r for r in recs if r["kind"] == "hq"]
@@ -213,7 +213,7 @@ def main():
             "frac": achieved / PEAK_FP64,
             "traffic": traffic["bytes_per_launch"] if traffic else None,
             "kernel": "QR phase of a size bin: gs::zqrw2_kernel<true> (two-wavefront windowed QR sweeps) "
-                      "then, for bins above n = 64, gs::zlog_kernel (the sweeps' Z log applied with Z in LDS)",
+                      "then, for the bin above n = 80, gs::zlog_kernel (the sweeps' Z log applied with Z in LDS)",
             "launches": len(qr),
             "avg_launch_us": qr_us / max(len(qr), 1),
             "qr_sweep_avg_launch_us": sum(r["us"] for r in qr) / max(len(qr), 1),

@@ -193,16 +193,20 @@ int gs_batched_max_n(int is_complex) { return is_complex ? zschur_max_lds_n() : 
 }  // extern "C"
 
 // ------------------------------------------------------------------ plans
-// GSCHUR_HQ_NT overrides the Hessenberg+Q workgroup size of bins above
-// n = 32 (A/B measurements; 256, 512 or 1024)
+// Hessenberg+Q workgroup size: 512 threads above n = 64, 256 above n = 32
+// (measured, each launch alone: n = 81..96 5.7 -> 4.6 ms with 512, n = 33..48
+// 0.64 -> 0.77 ms).  GSCHUR_HQ_NT overrides it above n = 32 (A/B
+// measurements; 256, 512 or 1024).
 static int default_nt_hq(bool cplx, int nmax) {
   (void)cplx;
-  static const int big = [] {
+  static const int force = [] {
     const char* e = std::getenv("GSCHUR_HQ_NT");
     const int v = e ? std::atoi(e) : 0;
-    return (v == 256 || v == 512 || v == 1024) ? v : 512;
+    return (v == 256 || v == 512 || v == 1024) ? v : 0;
   }();
-  return nmax > 32 ? big : 64;
+  if (nmax <= 32) return 64;
+  if (force) return force;
+  return nmax > 64 ? 512 : 256;
 }
 static int default_nt_qr(bool cplx, int nmax) {
   (void)cplx;

@@ -13,7 +13,7 @@
 // in LDS (one workgroup per matrix); zqrw2_kernel runs the QR iterations
 // with H and Z in HBM, two wavefronts per matrix (the bulge chase in a
 // 32 x 32 LDS window, and the chunk's reflectors applied to the rest of H
-// and Z as chains); for bins above n = 64 it logs the Z updates instead and
+// and Z as chains); for bins above n = 80 it logs the Z updates instead and
 // zlog_kernel applies the log with Z in LDS.  zqr_kernel (GS_QR_BAND) keeps
 // H's band j >= i-2 in LDS instead.  The single-matrix kernel keeps H in
 // HBM.
