@@ -214,7 +214,18 @@ static int default_nt_qr(bool cplx, int nmax) {
   return 64;
 }
 
-static int bin_width(bool cplx) { return cplx ? 16 : 16; }
+// Size-bin width: 16 orders for complex batches, 32 for real ones (measured
+// on n ~ U{16..128} real: 136.4k matrices/s with 16, 144.0k with 32; complex
+// n ~ U{8..96}: 16 and 24 within noise, 8, 12 and 32 slower).
+// GSCHUR_BIN_W overrides it (A/B measurements, 4 .. 64).
+static int bin_width(bool cplx) {
+  static const int w = [] {
+    const char* e = std::getenv("GSCHUR_BIN_W");
+    const int v = e ? std::atoi(e) : 0;
+    return (v >= 4 && v <= 64) ? v : 0;
+  }();
+  return w ? w : (cplx ? 16 : 32);
+}
 
 static std::string plan_key(bool cplx, int64_t count, const int32_t* n, const int64_t* off) {
   std::string k;
