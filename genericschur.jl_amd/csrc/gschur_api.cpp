@@ -193,12 +193,13 @@ int gs_batched_max_n(int is_complex) { return is_complex ? zschur_max_lds_n() : 
 }  // extern "C"
 
 // ------------------------------------------------------------------ plans
-// Hessenberg+Q workgroup size: 512 threads above n = 64, 256 above n = 32
-// (measured, each launch alone: n = 81..96 5.7 -> 4.6 ms with 512, n = 33..48
-// 0.64 -> 0.77 ms).  GSCHUR_HQ_NT overrides it above n = 32 (A/B
-// measurements; 256, 512 or 1024).
+// Hessenberg+Q workgroup size above n = 32: 512 threads, except 256 for
+// complex bins up to n = 64 (measured, each launch alone: complex n = 81..96
+// 5.7 -> 4.6 ms with 512, n = 33..48 0.64 -> 0.77 ms; real n ~ U{16..128}
+// in 32-wide bins 143.9k -> 146.2k matrices/s with 512 for the n = 33..64
+// bin).  GSCHUR_HQ_NT overrides it above n = 32 (A/B measurements; 256, 512
+// or 1024).
 static int default_nt_hq(bool cplx, int nmax) {
-  (void)cplx;
   static const int force = [] {
     const char* e = std::getenv("GSCHUR_HQ_NT");
     const int v = e ? std::atoi(e) : 0;
@@ -206,7 +207,7 @@ static int default_nt_hq(bool cplx, int nmax) {
   }();
   if (nmax <= 32) return 64;
   if (force) return force;
-  return nmax > 64 ? 512 : 256;
+  return (!cplx || nmax > 64) ? 512 : 256;
 }
 static int default_nt_qr(bool cplx, int nmax) {
   (void)cplx;
