@@ -813,9 +813,7 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
         // reflector's inputs from the column pass's lanes 1 and 2, so no LDS
         // round trip lies between one reflector and the next.  Lanes with
         // nothing to do load in-window entries and store to a dummy area.
-        // (lanes of a group of eight store to eight different entries: stores
-        // of many lanes to one LDS address are serialized as bank conflicts)
-        zd* const dmy = lgbuf + 4 * ZKC + (lane & 7);
+        zd* const dmy = lgbuf + 4 * ZKC;
         const bool hi3 = lane >= 3;
         const int cidx = hi3 ? lane - 3 : lane;
         const int laneZ = lane * ZWL;
@@ -1182,7 +1180,7 @@ template <int NT, bool FAST> __global__ void __launch_bounds__(NT) zqr_kernel(Ba
 // Two-wavefront windowed QR kernel: the chase of chunk c overlaps the chains
 // of chunk c-1 (z_sweep_win2); the control logic runs on both wavefronts.
 template <bool FAST> __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) zqrw2_kernel(BatchArgs a) {
-  __shared__ zd win[ZWL * ZW + 4 * ZKC + ZWL + 8];
+  __shared__ zd win[ZWL * ZW + 4 * ZKC + ZWL + 1];
   __shared__ int redi[4];
   const int b = blockIdx.x;
   if (b >= a.count) return;
