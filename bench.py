@@ -24,7 +24,15 @@ Alongside the throughput it reports:
                 matrices on the host cores;
   dropin        per-matrix host-memory gs_schur_z calls on the same GPU (the
                 drop-in schur! path) on a sample, and the batched/drop-in ratio;
-  check         acceptance residuals on a sample of the batch.
+  check         acceptance residuals on a sample of the batch;
+  single        the metric's single-matrix case: one 256 x 256 ComplexF64
+                Ginibre matrix, seconds per solve and its CPU oracle time;
+  large         the metric's n = 4096 half: one 4096 x 4096 Float64 solve on
+                the large path against the plain unblocked kernel (timed at
+                n = 1024, scaled by n^3), its GEMM-phase fraction of the FP64
+                peak, residuals;
+  ok            false (and exit status 1) when a residual exceeds 50 n eps or
+                an info is nonzero.
 """
 import argparse
 import ctypes
@@ -86,7 +94,14 @@ def main():
                          "(blocked path vs plain kernel); 5: eigenvalue-loss forward + backward")
     ap.add_argument("--large-n", type=int, default=4096)
     ap.add_argument("--plain-n", type=int, default=1024,
-                    help="workload 4: order at which the plain kernel is timed (scaled by n^3 to --large-n)")
+                    help="order at which the plain kernel is timed (scaled by n^3 to --large-n)")
+    ap.add_argument("--large-cpu-n", type=int, default=512,
+                    help="order at which the oracle is timed for the large matrix (scaled by n^3)")
+    ap.add_argument("--no-large", action="store_true", help="leave the large-matrix object out of the default line")
+    ap.add_argument("--no-single", action="store_true",
+                    help="leave the single 256 x 256 complex object out of the default line")
+    ap.add_argument("--single-n", type=int, default=256)
+    ap.add_argument("--single-reps", type=int, default=5)
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -207,6 +222,7 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
+            "engine": "valu-fp64",
             "achieved": achieved / 1e12,
             "peak": PEAK_FP64 / 1e12,
             "unit": "TFLOP/s",
@@ -220,10 +236,12 @@ def main():
             "zlog_launches": len(zl),
             "zlog_avg_launch_us": sum(r["us"] for r in zl) / max(len(zl), 1),
             "flops_per_launch": qr_flops / max(len(qr), 1),
-            "note": "FP64 vector-ALU work (the bulge chase has no dense contraction; FP64 vector and FP64 MFMA "
-                    "peaks are both 78.6 TFLOP/s). achieved = model flops of a bin's QR phase (81.3 n^3 per "
+            "note": "FP64 vector-ALU work (engine valu-fp64: the bulge chase has no dense contraction); the compute "
+                    "roof is the FP64 peak, 78.6 TFLOP/s for the vector ALUs and for MFMA alike. achieved = model flops of a bin's QR phase (81.3 n^3 per "
                     "complex matrix: H and Z updates) / its average duration (HIP events around the QR-sweep "
                     "and Z-log launches, instrumented pass with every launch alone)",
+            "traffic_kind": "calibrated (committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, not measured in this "
+                            "run)" if traffic else None,
             "traffic_source": traffic["source"] if traffic else None,
             "hq_kernel_avg_launch_us": sum(r["us"] for r in hq) / max(len(hq), 1),
             "batch_ms": kern_s * 1e3,
@@ -255,7 +273,18 @@ def main():
 
     if not args.no_cpu and args.cpu_sample > 0:
         out["cpu_baseline"] = _cpu_batched(mats, True, min(args.cpu_sample, len(mats)))
+    # the metric's other configurations on the same GPU, after the timed steps
+    if not args.no_single:
+        out["single"] = single_object(args, with_cpu=not args.no_cpu)
+    if not args.no_large:
+        out["large"] = large_object(args, with_cpu=not args.no_cpu)[0]
+    ok = (out["check"]["info_nonzero"] == 0 and worst_b <= 50.0 and worst_o <= 50.0)
+    ok = ok and all(out[k]["ok"] for k in ("single", "large") if k in out)
+    out["ok"] = bool(ok)
     print(json.dumps(out), flush=True)
+    if not ok:
+        log("acceptance check failed: residual above 50 n eps or info != 0 (see check / single / large)")
+        sys.exit(1)
 
 
 def _common(args, metric_value, unit, ms, workload, extra_cfg, dtype="f64"):
@@ -325,14 +354,19 @@ def bench_workload3(args):
                   "1/4 Toeplitz (-1 sub, +1 diag and 3 super), 1/4 Q(D+N)Q^T; seeded Philox",
                   {"count": count, "n_range": [16, 128], "seed": args.seed})
     flops = float((25.0 * n64 ** 3).sum())
-    out["roofline"] = {"bound": "mfma", "achieved": flops / kern_s / 1e12, "peak": 78.6, "unit": "TFLOP/s",
-                       "frac": flops / kern_s / PEAK_FP64, "traffic": None,
+    out["roofline"] = {"bound": "mfma", "engine": "valu-fp64", "achieved": flops / kern_s / 1e12, "peak": 78.6,
+                       "unit": "TFLOP/s", "frac": flops / kern_s / PEAK_FP64, "traffic": None,
                        "note": "25 n^3 model flops per matrix over the batch's device time"}
     out["check"] = _residuals(mats, ns, off, T.cpu().numpy(), Z.cpu().numpy())
     out["check"]["info_nonzero"] = int((info.cpu().numpy() != 0).sum())
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_batched(mats, False, min(args.cpu_sample, count))
+    c = out["check"]
+    out["ok"] = bool(c["info_nonzero"] == 0 and c["max_backward_over_n_eps"] <= 50.0
+                     and c["max_orth_over_n_eps"] <= 50.0)
     print(json.dumps(out), flush=True)
+    if not out["ok"]:
+        sys.exit(1)
 
 
 def _cpu_model():
@@ -371,9 +405,11 @@ def _cpu_batched(mats, cplx, m):
             "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
-def bench_workload4(args):
-    """One large real matrix A = Q (Lambda + U) Q^T: the blocked large-matrix
-    path against the plain unblocked kernel."""
+def large_object(args, with_cpu=True):
+    """One large real matrix A = Q (Lambda + U) Q^T of order args.large_n on
+    the large path (blocked Hessenberg, chunked QR), against the plain
+    unblocked kernel timed at args.plain_n and scaled by n^3; backward and
+    orthogonality residuals computed on the device."""
     import genericschur as gs
     from genericschur import gen, _lib
     import torch
@@ -397,7 +433,7 @@ def bench_workload4(args):
         return dt, info, bwd, orth
 
     n = args.large_n
-    log(f"workload 4: large path n={n}")
+    log(f"large path n={n}")
     dt, info, bwd, orth = solve(n, False, True)
     recs = _lib.last_launches()
     st = _lib.last_stats()
@@ -405,34 +441,32 @@ def bench_workload4(args):
     g = [r for r in recs if r["kind"] == "gemm"]
     gf = sum(r["flops"] for r in g)
     gt = sum(r["us"] for r in g) * 1e-6
-    log(f"workload 4: plain kernel n={args.plain_n}")
+    log(f"plain kernel n={args.plain_n}")
     pdt, pinfo, pbwd, porth = solve(args.plain_n, True, False)
     plain_est = pdt * (n / args.plain_n) ** 3
-    out = _common(args, 1.0 / dt, "solves/s", dt * 1e3,
-                  f"one {n}x{n} Float64 matrix Q(Lambda+U)Q^T (Haar Q, log-uniform real and complex-pair "
-                  "eigenvalues in [1e-3, 1e3], U strictly upper N(0,1)), T, Z and values",
-                  {"n": n, "seed": args.seed})
-    out["steps"] = 1
-    out["warmup"] = 0
-    out["large"] = {"seconds": dt, "info": info, "sweeps": st[0] if st else None,
-                    "bulge_steps": st[1] if len(st) > 1 else None,
-                    "phase_seconds": {"copy_scale": ph.get(0), "hessenberg": ph.get(1), "q_formation": ph.get(2),
-                                      "qr": ph.get(3), "finish": ph.get(4)},
-                    "gemm_phase": {"tflops": gf / gt / 1e12 if gt else None,
-                                   "frac_fp64_peak": gf / gt / PEAK_FP64 if gt else None,
-                                   "seconds": gt, "gemms": len(g), "flops": gf},
-                    "backward_over_n_eps": bwd, "orth_over_n_eps": orth}
-    out["plain"] = {"n": args.plain_n, "seconds": pdt, "info": pinfo, "backward_over_n_eps": pbwd,
-                    "orth_over_n_eps": porth,
-                    "seconds_scaled_to_n": plain_est,
-                    "note": f"plain unblocked kernel timed at n={args.plain_n}, scaled by (n/{args.plain_n})^3"}
-    out["large_over_plain_speedup"] = plain_est / dt
-    if not args.no_cpu:
+    flops = 25.0 * n ** 3
+    out = {"workload": f"one {n}x{n} Float64 matrix Q(Lambda+U)Q^T (Haar Q, log-uniform real and complex-pair "
+                       "eigenvalues in [1e-3, 1e3], U strictly upper N(0,1)), T, Z and values",
+           "n": n, "seconds": dt, "info": info, "sweeps": st[0] if st else None,
+           "bulge_steps": st[1] if len(st) > 1 else None,
+           "phase_seconds": {"copy_scale": ph.get(0), "hessenberg": ph.get(1), "q_formation": ph.get(2),
+                             "qr": ph.get(3), "finish": ph.get(4)},
+           "gemm_phase": {"tflops": gf / gt / 1e12 if gt else None,
+                          "frac_fp64_peak": gf / gt / PEAK_FP64 if gt else None,
+                          "seconds": gt, "gemms": len(g), "flops": gf},
+           "whole_solve_tflops": flops / dt / 1e12, "whole_solve_frac": flops / dt / PEAK_FP64,
+           "backward_over_n_eps": bwd, "orth_over_n_eps": orth,
+           "plain": {"n": args.plain_n, "seconds": pdt, "info": pinfo, "backward_over_n_eps": pbwd,
+                     "orth_over_n_eps": porth, "seconds_scaled_to_n": plain_est,
+                     "note": f"plain unblocked kernel timed at n={args.plain_n}, scaled by (n/{args.plain_n})^3"},
+           "speedup_over_plain": plain_est / dt,
+           "ok": bool(info == 0 and bwd <= 50.0 and orth <= 50.0)}
+    if with_cpu:
         # the oracle single-threaded (the reference's gschur! is scalar code)
-        # at n = 1024, scaled by n^3
+        # at a smaller order of the same construction, scaled by n^3
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
-        nc = min(1024, n)
+        nc = min(args.large_cpu_n, n)
         Ac = gen.workload4(seed=args.seed, n=nc)
         tc = time.perf_counter()
         pyoracle.gschur(Ac)
@@ -442,12 +476,81 @@ def bench_workload4(args):
                                "sample": f"oracle (C++ restatement of gschur!) on one {nc}x{nc} matrix of the same "
                                          f"construction, 1 thread, {cdt:.1f}s, scaled by ({n}/{nc})^3 = "
                                          f"{est:.0f}s per solve"}
-    flops = 25.0 * n ** 3
-    out["roofline"] = {"bound": "mfma", "achieved": gf / gt / 1e12 if gt else None, "peak": 78.6,
-                       "unit": "TFLOP/s", "frac": gf / gt / PEAK_FP64 if gt else None, "traffic": None,
+    return out, gf, gt
+
+
+def bench_workload4(args):
+    """One large real matrix A = Q (Lambda + U) Q^T: the blocked large-matrix
+    path against the plain unblocked kernel."""
+    lg, gf, gt = large_object(args, with_cpu=not args.no_cpu)
+    n = lg["n"]
+    out = _common(args, 1.0 / lg["seconds"], "solves/s", lg["seconds"] * 1e3, lg["workload"],
+                  {"n": n, "seed": args.seed})
+    out["steps"] = 1
+    out["warmup"] = 0
+    out["large"] = lg
+    if "cpu_baseline" in lg:
+        out["cpu_baseline"] = lg.pop("cpu_baseline")
+    out["roofline"] = {"bound": "mfma", "engine": "mfma-fp64", "achieved": gf / gt / 1e12 if gt else None,
+                       "peak": 78.6, "unit": "TFLOP/s", "frac": gf / gt / PEAK_FP64 if gt else None,
+                       "traffic": None,
                        "kernel": "gs::dgemm_kernel (GEMM phases: blocked Hessenberg and Q formation)",
-                       "whole_solve_tflops": flops / dt / 1e12}
+                       "whole_solve_tflops": lg["whole_solve_tflops"]}
+    out["ok"] = lg["ok"]
     print(json.dumps(out), flush=True)
+    if not out["ok"]:
+        sys.exit(1)
+
+
+def single_object(args, with_cpu=True):
+    """One n x n ComplexF64 Ginibre matrix (the reference's own CPU-runnable
+    case, n = 256): gschur! on device-resident data, median of repeated
+    solves, against the oracle single-threaded on the same matrix."""
+    import genericschur as gs
+    from genericschur import gen, _lib
+    import torch
+    dev = torch.device("cuda:0")
+    n = args.single_n
+    A = gen.workload1(seed=args.seed, n=n)
+    a = np.asfortranarray(A).ravel(order="F").view(np.float64)
+    Ad = torch.from_numpy(a).to(dev)
+    T = torch.zeros_like(Ad)
+    Z = torch.zeros_like(Ad)
+    W = torch.zeros(2 * n, dtype=torch.float64, device=dev)
+    A0 = Ad.clone()
+    times = []
+    for r in range(1 + args.single_reps):
+        Ad.copy_(A0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = gs.gschur_device(n, Ad, T, Z, W, True)
+        gs.synchronize()
+        if r:
+            times.append(time.perf_counter() - t0)
+    st = _lib.last_stats()
+    dt = float(np.median(times))
+    Th = T.cpu().numpy().view(np.complex128).reshape((n, n), order="F")
+    Zh = Z.cpu().numpy().view(np.complex128).reshape((n, n), order="F")
+    eps = np.finfo(np.float64).eps
+    bwd = np.linalg.norm(A - Zh @ Th @ Zh.conj().T) / np.linalg.norm(A) / (n * eps)
+    orth = np.linalg.norm(Zh.conj().T @ Zh - np.eye(n)) / (n * eps)
+    flops = 100.0 * n ** 3
+    out = {"workload": f"one {n}x{n} ComplexF64 Ginibre matrix, Re, Im ~ N(0, 1/2), seeded Philox; T, Z, values",
+           "n": n, "seconds": dt, "solves_per_s": 1.0 / dt, "reps": args.single_reps, "info": info,
+           "sweeps": st[0] if st else None, "bulge_steps": st[1] if len(st) > 1 else None,
+           "tflops": flops / dt / 1e12, "frac_fp64_peak": flops / dt / PEAK_FP64,
+           "backward_over_n_eps": float(bwd), "orth_over_n_eps": float(orth),
+           "ok": bool(info == 0 and bwd <= 50.0 and orth <= 50.0)}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        tc = time.perf_counter()
+        pyoracle.gschur(A)
+        cdt = time.perf_counter() - tc
+        out["cpu_baseline"] = {"value": 1.0 / cdt, "unit": "solves/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle (C++ restatement of gschur!) on the same matrix, 1 thread, "
+                                         f"{cdt:.2f}s", "speedup_gpu_over_cpu": cdt / dt}
+    return out
 
 
 def bench_workload5(args):
@@ -498,16 +601,22 @@ def bench_workload5(args):
                   "n ~ U{32..64} (min eigenvalue separation >= 1e-3): batched Schur + VJP kernel",
                   {"count": count, "n_range": [32, 64], "seed": args.seed})
     flops = float(((25.0 + 2 * 8.0 / 3.0 + 8.0) * n64 ** 3).sum())
-    out["roofline"] = {"bound": "mfma", "achieved": flops / kern_s / 1e12, "peak": 78.6, "unit": "TFLOP/s",
-                       "frac": flops / kern_s / PEAK_FP64, "traffic": None,
+    out["roofline"] = {"bound": "mfma", "engine": "valu-fp64", "achieved": flops / kern_s / 1e12, "peak": 78.6,
+                       "unit": "TFLOP/s", "frac": flops / kern_s / PEAK_FP64, "traffic": None,
                        "note": "(25 + 16/3 + 8) n^3 model flops per matrix (Schur, two eigenvector sides, "
                                "assembly) over the device time"}
-    out["check"] = {"info_nonzero": int((info.cpu().numpy() != 0).sum()),
-                    "vjp_info_nonzero": int((vinfo.cpu().numpy() != 0).sum())}
+    out["check"] = _residuals(mats, ns, off, T.cpu().numpy(), Z.cpu().numpy())
+    out["check"]["info_nonzero"] = int((info.cpu().numpy() != 0).sum())
+    out["check"]["vjp_info_nonzero"] = int((vinfo.cpu().numpy() != 0).sum())
+    c = out["check"]
+    out["ok"] = bool(c["info_nonzero"] == 0 and c["vjp_info_nonzero"] == 0 and c["max_backward_over_n_eps"] <= 50.0
+                     and c["max_orth_over_n_eps"] <= 50.0)
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_batched(mats, False, count)
         out["cpu_baseline"]["note"] = "forward (Schur) only: the reference has no eigenvalue rrule"
     print(json.dumps(out), flush=True)
+    if not out["ok"]:
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -246,6 +246,10 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
     return GS_OK;
   }
   const int nmax_lds = cplx ? zschur_max_lds_n() : dschur_max_lds_n();
+  for (int64_t i = 0; i < count; ++i)
+    if (n[i] < 0 || n[i] > nmax_lds) return fail(h, GS_E_DIM, "matrix order outside the batched range");
+  int rc = check_spans(h, count, n, off, nullptr);
+  if (rc != GS_OK) return rc;
   std::unique_ptr<Plan> p(new Plan());
   p->is_complex = cplx;
   p->count = count;
@@ -254,7 +258,6 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
   p->woff.resize(count);
   int64_t w = 0, span = 0;
   for (int64_t i = 0; i < count; ++i) {
-    if (n[i] < 0 || n[i] > nmax_lds) return fail(h, GS_E_DIM, "matrix order outside the batched range");
     p->woff[i] = w;
     w += n[i];
     span = std::max<int64_t>(span, off[i] + (int64_t)n[i] * n[i]);

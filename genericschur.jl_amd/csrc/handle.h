@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <map>
 #include <memory>
@@ -82,6 +83,31 @@ inline int handle_fail(gs_handle* h, int code, const char* msg) {
   if (h) h->err = msg;
   return code;
 }
+// Batched operand layout: matrix i occupies elements [off[i], off[i] + n[i]^2)
+// of its arrays (ld = n[i]).  Every n[i] must be >= 0 (GS_E_DIM otherwise),
+// every off[i] >= 0, and the nonempty spans of different matrices must be
+// disjoint (GS_E_ARG otherwise), so that no kernel writes outside the
+// caller's arrays or into another matrix.  *span receives max(off[i] + n[i]^2).
+inline int check_spans(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off, int64_t* span) {
+  std::vector<std::pair<int64_t, int64_t>> iv;
+  iv.reserve((size_t)count);
+  int64_t s = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    if (n[i] < 0) return handle_fail(h, GS_E_DIM, "negative matrix order");
+    if (off[i] < 0) return handle_fail(h, GS_E_ARG, "negative matrix offset");
+    const int64_t len = (int64_t)n[i] * n[i];
+    if (off[i] > INT64_MAX - len) return handle_fail(h, GS_E_ARG, "matrix offset out of range");
+    if (len == 0) continue;
+    iv.emplace_back(off[i], off[i] + len);
+    s = std::max(s, off[i] + len);
+  }
+  std::sort(iv.begin(), iv.end());
+  for (size_t k = 1; k < iv.size(); ++k)
+    if (iv[k].first < iv[k - 1].second) return handle_fail(h, GS_E_ARG, "overlapping matrix spans");
+  if (span) *span = s;
+  return GS_OK;
+}
+
 inline std::mutex& handle_mutex(gs_handle* h) { return h->mu; }
 inline int handle_device(gs_handle* h) { return h->device; }
 inline hipStream_t handle_stream(gs_handle* h) { return h->stream; }

@@ -21,11 +21,11 @@ int run_trevc(gs_handle* h, bool cplx, int64_t count, const int32_t* n, const in
   if (!h || count < 0) return GS_E_ARG;
   if (count == 0) return GS_OK;
   if (!n || !off || !T || !V) return handle_fail(h, GS_E_ARG, "null pointer");
+  const int src = check_spans(h, count, n, off, nullptr);
+  if (src != GS_OK) return src;
   int nmax = 0;
-  for (int64_t i = 0; i < count; ++i) {
-    if (n[i] < 0) return handle_fail(h, GS_E_DIM, "negative order");
-    nmax = std::max(nmax, (int)n[i]);
-  }
+  for (int64_t i = 0; i < count; ++i) nmax = std::max(nmax, (int)n[i]);
+  if (nmax == 0) return GS_OK;
   const int lim = cplx ? 128 : 192;
   if (nmax > lim) return handle_fail(h, GS_E_UNSUPPORTED, "order above the eigenvector kernel limit");
   std::lock_guard<std::mutex> lk(handle_mutex(h));
@@ -67,13 +67,21 @@ int run_trevc(gs_handle* h, bool cplx, int64_t count, const int32_t* n, const in
     DevBuf& bV = handle_buf(h, 3);
     if ((e = bT.ensure(es * spanT)) != hipSuccess) return handle_fail_hip(h, e, "hipMalloc");
     if ((e = bV.ensure(16 * spanV)) != hipSuccess) return handle_fail_hip(h, e, "hipMalloc");
-    if ((e = hipMemcpyAsync(bT.p, T, es * spanT, hipMemcpyHostToDevice, s)) != hipSuccess)
-      return handle_fail_hip(h, e, "hipMemcpyAsync");
+    // a single strided operand occupies ld (n - 1) + n elements of the
+    // caller's buffer: copy column by column, never past its last column
+    if (single)
+      e = hipMemcpy2DAsync(bT.p, es * ldt, T, es * ldt, es * n[0], n[0], hipMemcpyHostToDevice, s);
+    else
+      e = hipMemcpyAsync(bT.p, T, es * spanT, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return handle_fail_hip(h, e, "hipMemcpyAsync");
     dT = static_cast<const double*>(bT.p);
     if (Z) {
       if ((e = bZ.ensure(es * spanZ)) != hipSuccess) return handle_fail_hip(h, e, "hipMalloc");
-      if ((e = hipMemcpyAsync(bZ.p, Z, es * spanZ, hipMemcpyHostToDevice, s)) != hipSuccess)
-        return handle_fail_hip(h, e, "hipMemcpyAsync");
+      if (single)
+        e = hipMemcpy2DAsync(bZ.p, es * ldz, Z, es * ldz, es * n[0], n[0], hipMemcpyHostToDevice, s);
+      else
+        e = hipMemcpyAsync(bZ.p, Z, es * spanZ, hipMemcpyHostToDevice, s);
+      if (e != hipSuccess) return handle_fail_hip(h, e, "hipMemcpyAsync");
       dZ = static_cast<const double*>(bZ.p);
     }
     dV = static_cast<double*>(bV.p);

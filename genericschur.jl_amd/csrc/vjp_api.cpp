@@ -21,11 +21,11 @@ extern "C" int gs_eig_vjp_batched_d(gs_handle* h, int64_t count, const int32_t* 
   if (!n || !off || !T || !Z || !gbar || !Abar || !info) return handle_fail(h, GS_E_ARG, "null pointer");
   int nmax = 0;
   int64_t span = 0, wtot = 0;
+  const int src = check_spans(h, count, n, off, &span);
+  if (src != GS_OK) return src;
   std::vector<int64_t> woff(count);
   for (int64_t i = 0; i < count; ++i) {
-    if (n[i] < 0) return handle_fail(h, GS_E_DIM, "negative order");
     nmax = std::max(nmax, (int)n[i]);
-    span = std::max<int64_t>(span, off[i] + (int64_t)n[i] * n[i]);
     woff[i] = wtot;
     wtot += n[i];
   }

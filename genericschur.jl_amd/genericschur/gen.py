@@ -201,3 +201,157 @@ def godunov():
         [-2176, -287, -1565, -512, -541, -1152, -289],
     ], dtype=np.float64)
     return A, np.array([-4.0, -2.0, -1.0, 0.0, 1.0, 2.0, 4.0]), 7.0e16
+
+
+# ------------------------------------------------- test-matrix families
+# Re-creations of the families the reference's tests draw from LAPACK's
+# matrix generators (test/complex.jl:162-380, test/real.jl:160-300 through
+# test/TMGlib.jl's latme!/latmr!).  The reference's TMG binaries and Julia's
+# RNG stream are not available, so the families (eigenvalue modes,
+# magnitudes, eigenvector conditioning) are rebuilt here with the seeded
+# streams above; the matrices themselves differ from the reference's.
+
+ULP = float(np.finfo(np.float64).eps)
+UNFL = float(np.finfo(np.float64).tiny)
+OVFL = 1.0 / UNFL
+MAGNS = (1.0, OVFL * ULP, UNFL / ULP)  # test/complex.jl:183, test/real.jl:174
+RTULPI = 1.0 / np.sqrt(ULP)
+
+
+def mode_values(n, mode, cond, rng):
+    """Magnitudes of LAPACK xLATM1's eigenvalue modes: 1 one large, rest
+    1/cond; 2 one small; 3 geometric 1 .. 1/cond; 4 arithmetic; 5 random
+    log-uniform in [1/cond, 1]; 6 uniform in [-1, 1] (sign included)."""
+    if n == 0:
+        return np.zeros(0)
+    i = np.arange(n, dtype=np.float64)
+    if mode == 1:
+        d = np.full(n, 1.0 / cond)
+        d[0] = 1.0
+    elif mode == 2:
+        d = np.ones(n)
+        d[-1] = 1.0 / cond
+    elif mode == 3:
+        d = cond ** (-i / max(n - 1, 1))
+    elif mode == 4:
+        d = 1.0 - i / max(n - 1, 1) * (1.0 - 1.0 / cond)
+    elif mode == 5:
+        d = np.exp(-rng.random(n) * np.log(cond))
+    elif mode == 6:
+        d = rng.uniform(-1.0, 1.0, n)
+    else:
+        raise ValueError(mode)
+    return d
+
+
+def _random_phases(n, rng, cplx):
+    if cplx:
+        return np.exp(2j * np.pi * rng.random(n))
+    return np.where(rng.random(n) < 0.5, -1.0, 1.0)
+
+
+def _haar(n, rng, cplx):
+    if not cplx:
+        return haar_orthogonal(n, rng)
+    G = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * np.sqrt(0.5)
+    Q, R = np.linalg.qr(G)
+    d = np.diag(R)
+    ph = np.where(d == 0, 1.0, d / np.abs(np.where(d == 0, 1.0, d)))
+    return Q * ph
+
+
+def _scale_max(A, anorm):
+    m = np.abs(A).max() if A.size else 0.0
+    if m == 0 or anorm == 1.0 and m == 1.0:
+        return A
+    # two steps so that neither the intermediate nor the result over/underflows
+    return (A / m) * anorm
+
+
+def latme_like(n, mode, cond, simcond, anorm, rng, cplx):
+    """General matrix with specified eigenvalues (xLATME's role in
+    zdrves/ddrves, test/complex.jl:235-255): X (D + U) X^-1 with D's
+    magnitudes from `mode` and `cond` (random signs / phases), U strictly
+    upper uniform in [-1, 1] (for real input, paired 2x2 rotation blocks turn
+    half of the eigenvalues complex), X = Q1 S Q2 with singular values S
+    arithmetic from 1 to 1/simcond (simcond = 1: X unitary), then scaled to
+    max |a_ij| = anorm."""
+    d = mode_values(n, mode, cond, rng)
+    M = np.zeros((n, n), dtype=np.complex128 if cplx else np.float64)
+    if cplx:
+        M[np.diag_indices(n)] = d * _random_phases(n, rng, True)
+        M += np.triu(rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n)), 1)
+    else:
+        dd = d * _random_phases(n, rng, False)
+        i = 0
+        mask = np.triu(np.ones((n, n)), 1)
+        while i < n:
+            if i + 1 < n and rng.random() < 0.5:
+                r = abs(dd[i])
+                th = rng.uniform(0.0, np.pi)
+                M[i:i + 2, i:i + 2] = [[r * np.cos(th), -r * np.sin(th)], [r * np.sin(th), r * np.cos(th)]]
+                mask[i, i + 1] = 0.0
+                i += 2
+            else:
+                M[i, i] = dd[i]
+                i += 1
+        M += rng.uniform(-1, 1, (n, n)) * mask
+    if n > 0:
+        s = mode_values(n, 4, simcond, rng) if simcond != 1.0 else np.ones(n)
+        X = (_haar(n, rng, cplx) * s) @ _haar(n, rng, cplx)
+        M = X @ M @ np.linalg.inv(X)
+    return _scale_max(M, anorm)
+
+
+def latmr_like(n, anorm, rng, cplx, kind="general"):
+    """Random-entry families of xLATMR (test/complex.jl:258-380): entries
+    uniform in [-1, 1] (complex: both parts), shaped as `kind` = "general",
+    "diagonal", "hermitian" (symmetric for real input) or "triangular",
+    scaled to max |a_ij| = anorm."""
+    A = rng.uniform(-1, 1, (n, n))
+    if cplx:
+        A = A + 1j * rng.uniform(-1, 1, (n, n))
+    if kind == "diagonal":
+        A = np.diag(np.diag(A))
+    elif kind == "hermitian":
+        A = np.triu(A)
+        A = A + np.triu(A, 1).conj().T
+        if cplx:
+            A[np.diag_indices(n)] = A.diagonal().real
+    elif kind == "triangular":
+        A = np.triu(A)
+    elif kind != "general":
+        raise ValueError(kind)
+    return _scale_max(A, anorm)
+
+
+def jordan(n, anorm=1.0, cplx=False):
+    """diagm(1 => ones(n-1)) + anorm I (test/complex.jl:221-232,
+    test/real.jl:180-191)."""
+    A = np.diag(np.ones(max(n - 1, 0)), 1) + anorm * np.eye(n)
+    return A.astype(np.complex128) if cplx else A
+
+
+def cyclic_shift(n, cplx=False):
+    """The cyclic down-shift permutation, already Hessenberg with a zero
+    diagonal: the standard shifts of its trailing 2 x 2 leave it unchanged,
+    so the iteration reaches the exceptional shifts (GenericSchur.jl:302-307,
+    :614-627) before it deflates."""
+    A = np.diag(np.ones(max(n - 1, 0)), -1)
+    if n > 0:
+        A[0, n - 1] = 1.0
+    return A.astype(np.complex128) if cplx else A
+
+
+def triangular_repeated(n, dmode, rng):
+    """Upper triangular complex matrix with a repeated diagonal entry:
+    d[1] = d[n-2] exactly (dmode 0), times nextfloat(1) (1) or prevfloat(1)
+    (2) (test/complex.jl:327-364)."""
+    d = rng.random(n) + 1j * rng.random(n)
+    if dmode == 0:
+        d[1] = d[n - 2]
+    elif dmode == 1:
+        d[1] = d[n - 2] * np.nextafter(1.0, 2.0)
+    else:
+        d[1] = d[n - 2] * np.nextafter(1.0, 0.0)
+    return np.triu(rng.random((n, n)) + 1j * rng.random((n, n)), 1) + np.diag(d)

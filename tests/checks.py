@@ -109,7 +109,8 @@ def schur_invariants(A, T, Z, w, tol, normal=False):
     assert res["orth"] < tol, f"orthogonality error {res['orth']} >= {tol}"
     if normal:
         k = 1 if cplx else 2
-        d = fnorm(np.triu(T, k)) / (n * fnorm(A) * ULP)
+        na = fnorm(A)
+        d = 0.0 if na == 0 and fnorm(np.triu(T, k)) == 0 else fnorm(np.triu(T, k)) / (n * na * ULP)
         assert d < tol, f"normal matrix not diagonalised: {d}"
     return res
 
@@ -148,13 +149,118 @@ def eig_conditions(VR, VL):
         return nr / d
 
 
+def eigen_clusters(ref_w, bound):
+    """Groups of reference eigenvalues whose error discs |z - ref_w[i]| <=
+    bound[i] overlap (transitive closure).  Inside such a cluster the
+    individual condition numbers do not bound the error of each member (a
+    nearly multiple eigenvalue splits arbitrarily under perturbation); only
+    the cluster as a whole is determined."""
+    n = len(ref_w)
+    parent = list(range(n))
+
+    def find(i):
+        while parent[i] != i:
+            parent[i] = parent[parent[i]]
+            i = parent[i]
+        return i
+
+    d = np.abs(ref_w[:, None] - ref_w[None, :])
+    ov = d <= bound[:, None] + bound[None, :]
+    for i in range(n):
+        for j in np.nonzero(ov[i, i + 1:])[0] + i + 1:
+            parent[find(i)] = find(j)
+    return np.array([find(i) for i in range(n)])
+
+
 def eigen_parity(A, w, ref_w, cond, factor=10.0):
-    """|lambda - lambda_ref| <= factor * n * eps * ||A||_2 * cond(lambda)."""
+    """|lambda - lambda_ref| <= factor * n * eps * ||A||_2 * cond(lambda).
+
+    The computed values are matched to the reference values by an optimal
+    assignment.  A value whose reference belongs to a cluster of overlapping
+    error discs (eigen_clusters) passes if it lies in the disc of any member
+    of that cluster; the number of such values is returned with the worst
+    ratio err/bound over the isolated ones."""
     n = A.shape[0]
     if n == 0:
         return 0.0
+    ref_w = np.asarray(ref_w)
     perm, err = match_eigenvalues(w, ref_w)
-    bound = factor * n * ULP * np.linalg.norm(A, 2) * cond
-    worst = float(np.max(err / np.maximum(bound, TINY)))
-    assert worst <= 1.0, f"eigenvalue mismatch: worst err/bound = {worst}"
-    return worst
+    bound = factor * n * ULP * np.linalg.norm(A, 2) * np.asarray(cond, dtype=np.float64)
+    bound = np.where(np.isnan(bound), np.inf, bound)
+    ratio = err / np.maximum(bound, TINY)
+    bad = np.nonzero(ratio > 1.0)[0]
+    if len(bad):
+        cl = eigen_clusters(ref_w, bound)
+        wp = np.asarray(w)[perm]
+        for i in bad:
+            members = np.nonzero(cl == cl[i])[0]
+            ok = len(members) > 1 and np.any(np.abs(wp[i] - ref_w[members]) <= bound[members])
+            assert ok, (f"eigenvalue mismatch: err/bound = {ratio[i]} at {ref_w[i]} "
+                        f"(computed {wp[i]}, cluster of {len(members)})")
+    iso = np.setdiff1d(np.arange(n), bad)
+    return float(np.max(ratio[iso])) if len(iso) else 0.0
+
+
+def well_conditioned(A, ref_w, cond, margin=1e6):
+    """Whether every eigenvalue is simple and separated from the others by
+    more than margin times its error bound n eps ||A||_2 cond: the Schur form
+    is then determined up to phases to about 1/margin."""
+    n = A.shape[0]
+    if n < 2:
+        return True
+    b = n * ULP * np.linalg.norm(A, 2) * np.asarray(cond)
+    if not np.all(np.isfinite(b)):
+        return False
+    d = np.abs(ref_w[:, None] - ref_w[None, :])
+    np.fill_diagonal(d, np.inf)
+    return bool(np.all(d > margin * (b[:, None] + b[None, :])))
+
+
+def diag_blocks(T):
+    """Diagonal blocks of a (quasi-)triangular T as (start, size) pairs."""
+    n = T.shape[0]
+    out = []
+    j = 0
+    while j < n:
+        if not np.iscomplexobj(T) and j + 1 < n and T[j + 1, j] != 0:
+            out.append((j, 2))
+            j += 2
+        else:
+            out.append((j, 1))
+            j += 1
+    return out
+
+
+def schur_form_match(A, T, Z, w, To, Zo, wo, cond, ztol=1e-6):
+    """Compares a Schur factorization (T, Z, w) of A with the oracle's
+    (To, Zo, wo) up to the factorization's own freedom.
+
+    Returns one of
+      "bitwise"  T, Z and the values bit for bit equal;
+      "phases"   the same eigenvalue in every diagonal slot (within the
+                 eigenvalue bound 10 n eps ||A||_2 cond) and T = D^H To D,
+                 Z = Zo D for a block-diagonal unitary D over To's diagonal
+                 blocks (complex: unit phases; real: signs of 1 x 1 blocks,
+                 orthogonal 2 x 2 blocks), within ztol relative;
+      "order"    the same slot order, T or Z outside ztol (e.g. a multiple
+                 eigenvalue, whose invariant subspace has no unique basis);
+      "differs"  another eigenvalue order.
+    """
+    n = A.shape[0]
+    if np.array_equal(T, To) and np.array_equal(Z, Zo) and np.array_equal(w, wo):
+        return "bitwise"
+    if n == 0:
+        return "bitwise"
+    bound = 10.0 * n * ULP * np.linalg.norm(A, 2) * np.asarray(cond) + TINY
+    if np.any(np.abs(np.asarray(w) - np.asarray(wo)) > bound):
+        return "differs"
+    M = Zo.conj().T @ Z
+    D = np.zeros_like(M)
+    for j, k in diag_blocks(To):
+        D[j:j + k, j:j + k] = M[j:j + k, j:j + k]
+    na = fnorm(A)
+    zerr = fnorm(Z - Zo @ D)
+    terr = fnorm(T - D.conj().T @ To @ D)
+    if zerr <= ztol * np.sqrt(n) and terr <= ztol * max(na, TINY):
+        return "phases"
+    return "order"

@@ -401,3 +401,24 @@ def test_qd_arithmetic(oracle):
     assert abs(val(r) - mp.sqrt(3) / 7) < mp.mpf(2) ** -204
     r = oracle.qd_op("add", a, [-a[0], -a[1], -a[2], 0.0])
     assert abs(val(r) - val(a) + val([a[0], a[1], a[2], 0.0])) < mp.mpf(2) ** -250
+
+
+# -------------------------------------------------------- test families
+@pytest.mark.parametrize("cplx", [False, True])
+def test_families_vs_lapack(oracle, cplx):
+    """Every re-created family of the reference's test suite (tests/
+    families.py, at the reference's orders 4 and 32 and at 96): the oracle's
+    eigenvalues against LAPACK's xGEEV on a copy scaled exactly (by a power of
+    2) to norm 1, within 10 n eps ||A||_2 cond(lambda) (clusters of
+    overlapping error discs as a whole), and the reference's schurtest.  A
+    slip in the oracle's scaling (_scale!, util.jl:14-29), deflation or
+    shift logic shows up on one of these families."""
+    import families
+    for n in (4, 32, 96):
+        for name, A, normal in families.families(n, cplx, seed=5):
+            T, Z, w, cond, _ = families.oracle_result(oracle, A)
+            checks.schur_invariants(A, T, Z, w, 20.0, normal=normal)
+            m = np.abs(A).max()
+            s = 1.0 if m == 0 else 2.0 ** -np.round(np.log2(m))
+            ev = sla.eigvals(A * s) / s
+            checks.eigen_parity(A, ev, w, cond)
