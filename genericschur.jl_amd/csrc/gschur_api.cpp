@@ -71,6 +71,37 @@ struct Plan {
   }
 };
 
+// Device view of a plan: the metadata arrays a batch's launches read, from
+// a cached Plan's buffers or from the no-allocation scratch array.
+struct PlanDev {
+  bool is_complex = false;
+  const std::vector<Bin>* bins = nullptr;
+  const int32_t* order = nullptr;
+  const int32_t* n = nullptr;
+  const int64_t* off = nullptr;
+  const int64_t* woff = nullptr;
+  int64_t* stats = nullptr;
+  double* scl = nullptr;
+  const int64_t* loff[2] = {nullptr, nullptr};
+  int64_t* llen = nullptr;
+};
+
+static PlanDev plan_dev(const Plan* p) {
+  PlanDev d;
+  d.is_complex = p->is_complex;
+  d.bins = &p->bins;
+  d.order = static_cast<const int32_t*>(p->d_order.p);
+  d.n = static_cast<const int32_t*>(p->d_n.p);
+  d.off = static_cast<const int64_t*>(p->d_off.p);
+  d.woff = static_cast<const int64_t*>(p->d_woff.p);
+  d.stats = static_cast<int64_t*>(p->d_stats.p);
+  d.scl = static_cast<double*>(p->d_scl.p);
+  d.loff[0] = static_cast<const int64_t*>(p->d_loff[0].p);
+  d.loff[1] = static_cast<const int64_t*>(p->d_loff[1].p);
+  d.llen = static_cast<int64_t*>(p->d_llen.p);
+  return d;
+}
+
 }  // namespace gs
 
 static int fail_hip(gs_handle* h, hipError_t e, const char* where) { return handle_fail_hip(h, e, where); }
@@ -132,6 +163,10 @@ int gs_create(gs_handle** out, int device) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->evjoin[i], hipEventDisableTiming);
   }
   for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&h->evlead[i], hipEventDisableTiming);
+  // this GPU's scratch array of the no-allocation entry point: its device
+  // address is looked up once per device, here, with the device current
+  void* scratch_ptr = nullptr;
+  if (e == hipSuccess) e = noalloc_scratch(device, &scratch_ptr);
   if (e != hipSuccess) {
     delete h;
     return GS_E_HIP - (int)e;
@@ -238,33 +273,21 @@ static std::string plan_key(bool cplx, int64_t count, const int32_t* n, const in
   return k;
 }
 
-static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, const int64_t* off, Plan** out) {
-  const std::string key = plan_key(cplx, count, n, off);
-  auto it = h->plans.find(key);
-  if (it != h->plans.end()) {
-    *out = it->second.get();
-    return GS_OK;
-  }
+// Host part of a plan: size bins of width bin_width, the largest bin first
+// and the largest matrix first inside a bin (order[]), each bin's launch
+// shape and graph branch, and the eigenvalue offsets woff[].  Orders must
+// already be validated.
+static void plan_bins(bool cplx, int64_t count, const int32_t* n, std::vector<int32_t>& order, std::vector<Bin>& bins,
+                      std::vector<int64_t>& woff) {
   const int nmax_lds = cplx ? zschur_max_lds_n() : dschur_max_lds_n();
-  for (int64_t i = 0; i < count; ++i)
-    if (n[i] < 0 || n[i] > nmax_lds) return fail(h, GS_E_DIM, "matrix order outside the batched range");
-  int rc = check_spans(h, count, n, off, nullptr);
-  if (rc != GS_OK) return rc;
-  std::unique_ptr<Plan> p(new Plan());
-  p->is_complex = cplx;
-  p->count = count;
-  p->n.assign(n, n + count);
-  p->off.assign(off, off + count);
-  p->woff.resize(count);
-  int64_t w = 0, span = 0;
+  order.clear();
+  bins.clear();
+  woff.resize(count);
+  int64_t w = 0;
   for (int64_t i = 0; i < count; ++i) {
-    p->woff[i] = w;
+    woff[i] = w;
     w += n[i];
-    span = std::max<int64_t>(span, off[i] + (int64_t)n[i] * n[i]);
   }
-  p->total_w = w;
-  p->total_elems = span;
-  // bins of width bin_width, largest bin first; inside a bin largest n first
   const int bw = bin_width(cplx);
   const int nb = (nmax_lds + bw - 1) / bw + 1;
   std::vector<std::vector<int32_t>> byb(nb);
@@ -277,7 +300,7 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
     if (v.empty()) continue;
     std::stable_sort(v.begin(), v.end(), [&](int32_t x, int32_t y) { return n[x] > n[y]; });
     Bin bin;
-    bin.first = (int)p->order.size();
+    bin.first = (int)order.size();
     bin.count = (int)v.size();
     bin.nmax = std::max(1, n[v.front()]);
     bin.nmin = n[v.back()];
@@ -286,23 +309,50 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
     bin.work = 0;
     for (int32_t i : v) bin.work += (double)n[i] * n[i] * n[i];
     bin.branch = 0;
-    p->bins.push_back(bin);
-    p->order.insert(p->order.end(), v.begin(), v.end());
+    bins.push_back(bin);
+    order.insert(order.end(), v.begin(), v.end());
   }
   // bins to graph branches, heaviest first onto the least loaded branch
-  {
-    std::vector<int> idx(p->bins.size());
-    for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
-    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return p->bins[x].work > p->bins[y].work; });
-    double load[kBranches] = {0, 0, 0, 0};
-    for (int i : idx) {
-      int best = 0;
-      for (int b = 1; b < kBranches; ++b)
-        if (load[b] < load[best]) best = b;
-      p->bins[i].branch = best;
-      load[best] += p->bins[i].work;
-    }
+  std::vector<int> idx(bins.size());
+  for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return bins[x].work > bins[y].work; });
+  double load[kBranches] = {0, 0, 0, 0};
+  for (int i : idx) {
+    int best = 0;
+    for (int b = 1; b < kBranches; ++b)
+      if (load[b] < load[best]) best = b;
+    bins[i].branch = best;
+    load[best] += bins[i].work;
   }
+}
+
+static int check_orders(gs_handle* h, bool cplx, int64_t count, const int32_t* n) {
+  const int nmax_lds = cplx ? zschur_max_lds_n() : dschur_max_lds_n();
+  for (int64_t i = 0; i < count; ++i)
+    if (n[i] < 0 || n[i] > nmax_lds) return fail(h, GS_E_DIM, "matrix order outside the batched range");
+  return GS_OK;
+}
+
+static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, const int64_t* off, Plan** out) {
+  const std::string key = plan_key(cplx, count, n, off);
+  auto it = h->plans.find(key);
+  if (it != h->plans.end()) {
+    *out = it->second.get();
+    return GS_OK;
+  }
+  int rc = check_orders(h, cplx, count, n);
+  if (rc != GS_OK) return rc;
+  int64_t span = 0;
+  rc = check_spans(h, count, n, off, &span);
+  if (rc != GS_OK) return rc;
+  std::unique_ptr<Plan> p(new Plan());
+  p->is_complex = cplx;
+  p->count = count;
+  p->n.assign(n, n + count);
+  p->off.assign(off, off + count);
+  plan_bins(cplx, count, n, p->order, p->bins, p->woff);
+  p->total_w = count ? p->woff[count - 1] + n[count - 1] : 0;
+  p->total_elems = span;
   {
     // Z-log regions: complex entries for complex input, doubles for real
     for (int t = 0; t < 2; ++t) {
@@ -346,45 +396,44 @@ constexpr int kDlogMaxN = 128;
 
 // Whether the batched QR launches of this call log their Z updates
 // (windowed kernels, Z wanted, not GS_ZDIRECT).
-static bool uses_zlog(const Plan* p, int kflags) {
-  (void)p;
-  return (kflags & GS_WANTZ) && !(kflags & (GS_QR_BAND | GS_ZDIRECT));
-}
+static bool uses_zlog(int kflags) { return (kflags & GS_WANTZ) && !(kflags & (GS_QR_BAND | GS_ZDIRECT)); }
 // Whether bin b's QR launch logs its Z updates.
-static bool bin_zlog(const Plan* p, const Bin& b, int kflags) {
-  if (!uses_zlog(p, kflags)) return false;
-  if (!p->is_complex && b.nmax > kDlogMaxN) return false;
-  return b.nmax >= (p->is_complex ? kZlogMinN : kDlogMinN) || (kflags & GS_ZLOG_TINY);
+static bool bin_zlog(bool cplx, const Bin& b, int kflags) {
+  if (!uses_zlog(kflags)) return false;
+  if (!cplx && b.nmax > kDlogMaxN) return false;
+  return b.nmax >= (cplx ? kZlogMinN : kDlogMinN) || (kflags & GS_ZLOG_TINY);
 }
 
 // With timed set, every launch runs alone, in order on the handle's stream,
 // between two events (eager launches only, never under capture): events
 // 6b .. 6b+5 for bin b (Hessenberg, QR, Z log), so each record is the
 // launch's own duration.  zlog: the Z-log buffer when uses_zlog.
-static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T, double* Z, double* W,
+static hipError_t launch_plan(gs_handle* h, const PlanDev& pd, const double* A, double* T, double* Z, double* W,
                               int32_t* info, int flags, double* zlog, bool timed = false) {
+  const PlanDev* p = &pd;
+  const std::vector<Bin>& bins = *pd.bins;
   BatchArgs a;
-  a.order = static_cast<const int32_t*>(p->d_order.p);
-  a.n = static_cast<const int32_t*>(p->d_n.p);
-  a.off = static_cast<const int64_t*>(p->d_off.p);
-  a.woff = static_cast<const int64_t*>(p->d_woff.p);
+  a.order = pd.order;
+  a.n = pd.n;
+  a.off = pd.off;
+  a.woff = pd.woff;
   a.A = A;
   a.T = T;
   a.Z = Z;
   a.W = W;
   a.info = info;
-  a.stats = static_cast<int64_t*>(p->d_stats.p);
-  a.scl = static_cast<double*>(p->d_scl.p);
+  a.stats = pd.stats;
+  a.scl = pd.scl;
   a.flags = flags;
   a.maxiter = 0;
   a.maxinner = 0;
   a.zlog = nullptr;
   a.loff = nullptr;
   a.llen = nullptr;
-  const bool zl = zlog && uses_zlog(p, flags);
+  const bool zl = zlog && uses_zlog(flags);
   if (zl) {
-    a.loff = static_cast<const int64_t*>(p->d_loff[(flags & GS_ZLOG_TINY) ? 1 : 0].p);
-    a.llen = static_cast<int64_t*>(p->d_llen.p);
+    a.loff = pd.loff[(flags & GS_ZLOG_TINY) ? 1 : 0];
+    a.llen = pd.llen;
   }
   // fork: every bin runs its Hessenberg launch then its QR launch on one of
   // kBranches streams, so the LDS left free by a large bin's workgroups is
@@ -392,16 +441,16 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
   hipError_t e = hipEventRecord(h->evfork, h->stream);
   if (e != hipSuccess) return e;
   bool used[kBranches] = {false, false, false, false};
-  for (const Bin& b : p->bins) used[b.branch] = true;
+  for (const Bin& b : bins) used[b.branch] = true;
   for (int i = 0; i < kBranches; ++i)
     if (used[i] && (e = hipStreamWaitEvent(h->branch[i], h->evfork, 0)) != hipSuccess) return e;
   // the leading (largest) bins' Hessenberg launches run first, alone: the
   // largest bin's chain (Hessenberg, QR, Z log) is the batch's longest, and
   // started later it would finish last with the GPU half idle
-  const int lead = (int)std::min<size_t>((size_t)hq_lead(p->is_complex), p->bins.size());
+  const int lead = (int)std::min<size_t>((size_t)hq_lead(p->is_complex), bins.size());
   auto bstream = [&](const Bin& b) { return timed ? h->stream : h->branch[b.branch]; };
   auto launch_hq = [&](size_t bi) -> hipError_t {
-    const Bin& b = p->bins[bi];
+    const Bin& b = bins[bi];
     a.first = b.first;
     a.count = b.count;
     hipStream_t s = bstream(b);
@@ -414,16 +463,16 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
   };
   for (int bi = 0; bi < lead; ++bi) {
     if ((e = launch_hq(bi)) != hipSuccess) return e;
-    if ((e = hipEventRecord(h->evlead[bi], bstream(p->bins[bi]))) != hipSuccess) return e;
+    if ((e = hipEventRecord(h->evlead[bi], bstream(bins[bi]))) != hipSuccess) return e;
   }
   bool waited[kBranches] = {false, false, false, false};
-  for (size_t bi = 0; bi < p->bins.size(); ++bi) {
-    const Bin& b = p->bins[bi];
+  for (size_t bi = 0; bi < bins.size(); ++bi) {
+    const Bin& b = bins[bi];
     hipStream_t s = bstream(b);
     if ((int)bi >= lead) {
       if (!waited[b.branch]) {
         for (int j = 0; j < lead; ++j)
-          if (p->bins[j].branch != b.branch && (e = hipStreamWaitEvent(s, h->evlead[j], 0)) != hipSuccess) return e;
+          if (bins[j].branch != b.branch && (e = hipStreamWaitEvent(s, h->evlead[j], 0)) != hipSuccess) return e;
         waited[b.branch] = true;
       }
       if ((e = launch_hq(bi)) != hipSuccess) return e;
@@ -432,7 +481,7 @@ static hipError_t launch_plan(gs_handle* h, Plan* p, const double* A, double* T,
     a.count = b.count;
     // the Z log pays off where Z's rows are long (its replay holds all of Z
     // in LDS); GS_ZLOG_TINY exercises it on every bin
-    a.zlog = (zl && bin_zlog(p, b, flags)) ? zlog : nullptr;
+    a.zlog = (zl && bin_zlog(p->is_complex, b, flags)) ? zlog : nullptr;
     if (timed && (e = hipEventRecord(h->tev[6 * bi + 2], s)) != hipSuccess) return e;
     if (flags & GS_QR_BAND)
       e = p->is_complex ? launch_zqr(a, b.nmax, b.nt_qr, s) : launch_dqr(a, b.nmax, b.nt_qr, s);
@@ -463,7 +512,7 @@ static int run_timed(gs_handle* h, Plan* p, const double* A, double* T, double* 
     h->tev.push_back(ev);
   }
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-  hipError_t le = launch_plan(h, p, A, T, Z, W, info, kflags, zlog, true);
+  hipError_t le = launch_plan(h, plan_dev(p), A, T, Z, W, info, kflags, zlog, true);
   if (le != hipSuccess) return fail_hip(h, le, "kernel launch (timed)");
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -473,7 +522,7 @@ static int run_timed(gs_handle* h, Plan* p, const double* A, double* T, double* 
     // kinds 0 (Hessenberg + Q), 1 (QR sweeps), 4 (Z log applied)
     const int kinds[3] = {0, 1, 4};
     for (int j = 0; j < 3; ++j) {
-      if (j == 2 && !(zlog && bin_zlog(p, b, kflags))) continue;
+      if (j == 2 && !(zlog && bin_zlog(p->is_complex, b, kflags))) continue;
       float ms = 0.f;
       HIPCHK(h, hipEventElapsedTime(&ms, h->tev[6 * bi + 2 * j], h->tev[6 * bi + 2 * j + 1]));
       const double rec[GS_LAUNCH_REC] = {(double)kinds[j], (double)b.nmin, (double)b.nmax, (double)b.count, b.work,
@@ -508,7 +557,7 @@ static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n,
                      (ref_sweep ? 0x1000 : 0) | env_qr_band();
   const size_t es = cplx ? 16 : 8;
   double* zlog = nullptr;
-  if (uses_zlog(p, kflags)) {
+  if (uses_zlog(kflags)) {
     const int64_t entries = p->loff[(kflags & GS_ZLOG_TINY) ? 1 : 0][count];
     HIPCHK(h, h->bLog.ensure(es * (size_t)std::max<int64_t>(entries, 1)));
     zlog = static_cast<double*>(h->bLog.p);
@@ -529,7 +578,7 @@ static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n,
       }
       hipGraph_t g = nullptr;
       HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      hipError_t le = launch_plan(h, p, A, T, Zd, W, info, kflags, zlog);
+      hipError_t le = launch_plan(h, plan_dev(p), A, T, Zd, W, info, kflags, zlog);
       hipError_t ce = hipStreamEndCapture(h->stream, &g);
       if (le != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
@@ -560,7 +609,7 @@ static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n,
   HIPCHK(h, h->bInfo.ensure(sizeof(int32_t) * (size_t)count));
   HIPCHK(h, hipMemcpyAsync(h->bA.p, A, es * nel, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-  hipError_t le = launch_plan(h, p, static_cast<double*>(h->bA.p), static_cast<double*>(h->bT.p),
+  hipError_t le = launch_plan(h, plan_dev(p), static_cast<double*>(h->bA.p), static_cast<double*>(h->bT.p),
                               static_cast<double*>(h->bZ.p), static_cast<double*>(h->bW.p),
                               static_cast<int32_t*>(h->bInfo.p), kflags, zlog);
   if (le != hipSuccess) return fail_hip(h, le, "kernel launch");
@@ -593,6 +642,65 @@ extern "C" int gs_schur_batched_z(gs_handle* h, int64_t count, const int32_t* n,
 extern "C" int gs_schur_batched_d(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off, const double* A,
                                   double* T, double* Z, double* W, int32_t* info, int flags) {
   return run_batched(h, false, count, n, off, A, T, Z, W, info, flags);
+}
+
+// The batched complex path without device allocation: the per-matrix
+// metadata and the scale factors live in the GPU's fixed scratch array, Z is
+// updated in place during the sweeps (no Z log), and the launches run
+// eagerly on the handle's stream.  Synchronous: the scratch array and the
+// handle's host staging block are reused by the next call.
+extern "C" int gs_schur_batched_z_noalloc(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off,
+                                          const double* A, double* T, double* Z, double* W, int32_t* info,
+                                          int flags) {
+  if (!h || count < 0 || (count > 0 && (!n || !off || !A || !T || !Z || !W || !info)))
+    return fail(h, GS_E_ARG, "bad argument");
+  if (count > kNoAllocMaxCount)
+    return fail(h, GS_E_ARG, "gs_schur_batched_z_noalloc: more than 16384 matrices in one call");
+  std::lock_guard<std::mutex> lk(h->mu);
+  HIPCHK(h, hipSetDevice(h->device));
+  int rc = check_orders(h, true, count, n);
+  if (rc != GS_OK) return rc;
+  rc = check_spans(h, count, n, off, nullptr);
+  if (rc != GS_OK) return rc;
+  h->last_plan = nullptr;
+  h->last_stats_from_plan = false;
+  h->launch_recs.clear();
+  if (count == 0) return GS_OK;
+  void* base = nullptr;
+  HIPCHK(h, noalloc_scratch(h->device, &base));
+  std::vector<int32_t> order;
+  std::vector<Bin> bins;
+  std::vector<int64_t> woff;
+  plan_bins(true, count, n, order, bins, woff);
+  // scratch layout: order, n, off, woff (uploaded), stats, scl (written by
+  // the kernels before they are read), each segment 256-byte aligned
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t c = (size_t)count;
+  const size_t o_n = al(4 * c), o_off = o_n + al(4 * c), o_woff = o_off + al(8 * c), o_stats = o_woff + al(8 * c);
+  const size_t o_scl = o_stats + al(16 * c), end = o_scl + al(16 * c);
+  if (end > kNoAllocScratchBytes) return fail(h, GS_E_ARG, "gs_schur_batched_z_noalloc: scratch too small");
+  std::vector<unsigned char>& hs = h->noalloc_host;
+  hs.resize(o_stats);
+  std::memcpy(hs.data(), order.data(), 4 * c);
+  std::memcpy(hs.data() + o_n, n, 4 * c);
+  std::memcpy(hs.data() + o_off, off, 8 * c);
+  std::memcpy(hs.data() + o_woff, woff.data(), 8 * c);
+  unsigned char* b = static_cast<unsigned char*>(base);
+  HIPCHK(h, hipMemcpyAsync(b, hs.data(), o_stats, hipMemcpyHostToDevice, h->stream));
+  PlanDev pd;
+  pd.is_complex = true;
+  pd.bins = &bins;
+  pd.order = reinterpret_cast<const int32_t*>(b);
+  pd.n = reinterpret_cast<const int32_t*>(b + o_n);
+  pd.off = reinterpret_cast<const int64_t*>(b + o_off);
+  pd.woff = reinterpret_cast<const int64_t*>(b + o_woff);
+  pd.stats = reinterpret_cast<int64_t*>(b + o_stats);
+  pd.scl = reinterpret_cast<double*>(b + o_scl);
+  const int kflags = GS_WANTZ | (flags & GS_SCALE) | GS_ZDIRECT;
+  const hipError_t le = launch_plan(h, pd, A, T, Z, W, info, kflags, nullptr);
+  if (le != hipSuccess) return fail_hip(h, le, "kernel launch");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return GS_OK;
 }
 
 // ------------------------------------------------------------ single matrix

@@ -66,6 +66,8 @@ struct gs_handle {
   gs::DevBuf lwork;
   void* lhost = nullptr;
   std::string large_where;
+  // host staging block of the no-allocation batched entry point
+  std::vector<unsigned char> noalloc_host;
   std::mutex mu;
 };
 

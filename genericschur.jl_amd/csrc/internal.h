@@ -130,6 +130,16 @@ size_t vjp_lds_bytes(int n);
 hipError_t launch_vjp_assemble(const VjpArgs& a, int nmax, hipStream_t s);
 
 
+// No-allocation batched entry point (gs_schur_batched_z_noalloc): the size
+// of its fixed device scratch array (one per GPU, noalloc.hip) and the most
+// matrices one call takes (56 bytes of metadata each, 256-byte aligned
+// segments).
+constexpr size_t kNoAllocScratchBytes = size_t(1) << 20;
+constexpr int64_t kNoAllocMaxCount = 16384;
+// Device address of this GPU's scratch array (fetched with
+// hipGetSymbolAddress once per device, then cached).
+hipError_t noalloc_scratch(int device, void** ptr);
+
 // FP64 MFMA GEMM: C(m x n) = beta C + alpha op(A) op(B), column-major.
 hipError_t launch_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, int64_t lda,
                         const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s);

@@ -34,7 +34,7 @@ EXPORTS = (
     "gs_version", "gs_create", "gs_destroy", "gs_last_error", "gs_synchronize", "gs_set_stream",
     "gs_schur_z", "gs_schur_d", "gs_schur_batched_z", "gs_schur_batched_d", "gs_batched_max_n",
     "gs_trevc_z", "gs_trevc_d", "gs_trevc_batched_z", "gs_trevc_batched_d",
-    "gs_eig_vjp_batched_d", "gs_last_stats", "gs_last_launches", "gs_dgemm",
+    "gs_eig_vjp_batched_d", "gs_last_stats", "gs_last_launches", "gs_dgemm", "gs_schur_batched_z_noalloc",
 )
 
 _lib = None
@@ -108,6 +108,7 @@ def lib():
         sig_b = [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int]
         L.gs_schur_batched_z.argtypes = sig_b
         L.gs_schur_batched_d.argtypes = sig_b
+        L.gs_schur_batched_z_noalloc.argtypes = sig_b
         L.gs_batched_max_n.argtypes = [c_int]
         sig_tv = [c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_int, c_int, c_int]
         L.gs_trevc_z.argtypes = sig_tv

@@ -130,6 +130,25 @@ int gs_schur_batched_d(gs_handle* h, int64_t count, const int32_t* n, const int6
                        double* T, double* Z, double* W, int32_t* info, int flags);
 int gs_batched_max_n(int is_complex);
 
+/* Batched ComplexF64 Schur without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and info codes as
+ * gs_schur_batched_z, with these differences:
+ *   - every data pointer is device memory (GS_DEVICE_PTRS is implied) and
+ *     Z is always written (GS_WANTZ is implied; GS_SCALE is honoured, other
+ *     flags are ignored);
+ *   - count <= 16384 (GS_E_ARG above): the per-matrix metadata lives in a
+ *     fixed 1 MiB device array that each GPU holds until the process ends;
+ *   - Z is updated in place during the sweeps (the GS_ZDIRECT form; same
+ *     results bit for bit as the Z-log form), the launches run eagerly on
+ *     the handle's stream, and the call returns when they are done;
+ *   - the scratch array is shared by every call on one GPU: calls that
+ *     overlap on one GPU (two handles, two threads) race.
+ * It calls no hipMalloc, hipFree or other allocating routine. */
+int gs_schur_batched_z_noalloc(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off, const double* A,
+                               double* T, double* Z, double* W, int32_t* info, int flags);
+
 /* Eigenvectors of a Schur factorization (T, Z), columns in the order of the
  * eigenvalues, normalised to unit 2-norm with the largest component real
  * (geigvecs(S; left) — vectors.jl:12-20 with _enormalize! util.jl:572-592;
