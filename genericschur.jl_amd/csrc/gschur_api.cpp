@@ -781,6 +781,18 @@ static hipError_t run_large(gs_handle* h, int64_t n, double* dA, int64_t lda, do
   return hipSuccess;
 }
 
+// Threads of the windowed QR kernel for one matrix above the LDS-resident
+// orders.
+static int win_nt(int n) {
+  static const int force = [] {
+    const char* e = std::getenv("GSCHUR_WIN_NT");
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 128 || v == 256 || v == 512 || v == 1024) ? v : 0;
+  }();
+  if (force) return force;
+  return n <= 128 ? 256 : (n <= 256 ? 512 : 1024);
+}
+
 static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda, double* T, int64_t ldt, double* Z,
                       int64_t ldz, double* W, int flags, int64_t maxiter, int64_t maxinner, int32_t* info) {
   if (!h || !info) return fail(h, GS_E_ARG, "bad argument");
@@ -840,6 +852,10 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
   const int nlds = cplx ? zschur_max_lds_n() : dschur_max_lds_n();
   const bool use_lds = !(flags & GS_PLAIN) && n <= nlds && dlda == n && dldt == n && dldz == n;
   const bool use_large = !cplx && !use_lds && !(flags & (GS_PLAIN | GS_HESSENBERG)) && n > nlds;
+  // complex orders above the LDS-resident range: Hessenberg + Q in global
+  // memory, then the windowed QR kernel with H and Z in HBM (ld = n)
+  const bool use_win = cplx && !use_lds && !(flags & GS_PLAIN) && n > nlds && n <= INT32_MAX && dlda == n &&
+                       dldt == n && dldz == n;
   h->launch_recs.clear();
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   hipError_t le;
@@ -884,6 +900,55 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
         le = cplx ? launch_zqr(a, (int)n, ntqr, h->stream) : launch_dqr(a, (int)n, ntqr, h->stream);
       else
         le = cplx ? launch_zqrw(a, h->stream) : launch_dqrw(a, h->stream);
+    }
+  } else if (use_win) {
+    HIPCHK(h, h->bMeta.ensure(64));
+    struct Meta {
+      int32_t order, n;
+      int64_t off, woff;
+    } meta = {0, (int32_t)n, 0, 0};
+    HIPCHK(h, hipMemcpyAsync(h->bMeta.p, &meta, sizeof meta, hipMemcpyHostToDevice, h->stream));
+    SingleArgs g;
+    g.n = n;
+    g.A = dA;
+    g.lda = dlda;
+    g.T = dT;
+    g.ldt = dldt;
+    g.Z = dZ;
+    g.ldz = dldz;
+    g.W = dW;
+    g.ws = static_cast<double*>(h->bWs.p);
+    g.info = dInfo;
+    g.stats = static_cast<int64_t*>(h->bStats.p);
+    g.flags = kflags;
+    g.maxiter = maxiter;
+    g.maxinner = maxinner;
+    le = launch_zhq_global(g, static_cast<double*>(h->bScl.p), h->stream);
+    if (le == hipSuccess) {
+      char* mp = static_cast<char*>(h->bMeta.p);
+      BatchArgs a;
+      a.order = reinterpret_cast<const int32_t*>(mp + offsetof(Meta, order));
+      a.n = reinterpret_cast<const int32_t*>(mp + offsetof(Meta, n));
+      a.off = reinterpret_cast<const int64_t*>(mp + offsetof(Meta, off));
+      a.woff = reinterpret_cast<const int64_t*>(mp + offsetof(Meta, woff));
+      a.first = 0;
+      a.count = 1;
+      a.A = dA;
+      a.T = dT;
+      a.Z = dZ;
+      a.W = dW;
+      a.info = dInfo;
+      a.stats = static_cast<int64_t*>(h->bStats.p);
+      a.scl = static_cast<double*>(h->bScl.p);
+      a.flags = kflags;
+      a.maxiter = maxiter;
+      a.maxinner = maxinner;
+      a.zlog = nullptr;
+      a.loff = nullptr;
+      a.llen = nullptr;
+      // chain wavefronts in proportion to the lines a chunk updates (about
+      // 2n rows and columns): GSCHUR_WIN_NT overrides (128 .. 1024)
+      le = launch_zqrw_single(a, win_nt((int)n), h->stream);
     }
   } else {
     SingleArgs a;

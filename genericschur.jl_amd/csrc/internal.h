@@ -87,6 +87,12 @@ inline int64_t dlog_capacity(int n, bool tiny) {
 hipError_t launch_dhq(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 hipError_t launch_dqr(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 hipError_t launch_zschur_global(const SingleArgs& a, int nt, hipStream_t s);
+// one matrix above the LDS-resident orders: Hessenberg + Q in global memory
+// (H in a.T, Q in a.Z, ld = n), then launch_zqrw on a batch of one
+hipError_t launch_zhq_global(const SingleArgs& a, double* scl, hipStream_t s);
+// the windowed QR kernel with nt threads (128 .. 1024): one chasing
+// wavefront, nt / 64 - 1 chain wavefronts
+hipError_t launch_zqrw_single(const BatchArgs& a, int nt, hipStream_t s);
 hipError_t launch_dschur_global(const SingleArgs& a, int nt, hipStream_t s);
 
 // Eigenvector kernels (one workgroup per matrix).  T, Z at element offset

@@ -714,14 +714,16 @@ GS_DEV void zlog_row(zd* row, int64_t st, const zd* __restrict__ lg, int64_t len
   }
 }
 
-template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, int64_t ldz, bool wantz, zd* win,
+template <bool FAST, int NT = 128> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, int64_t ldz, bool wantz, zd* win,
                                              zd* lgbuf, int* redi, zd shift, int istart, int iend, int64_t& nstep,
                                              ZLog& zl) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int CS = NT - 64;  // threads of the chain wavefronts
+  const int cw = tid - 64;
   const double ulp = GS_EPS;
   const FullLayout lay{ld};
   int found = -1;
-  for (int mm = iend - 1 - tid; mm >= istart + 1; mm -= 128) {
+  for (int mm = iend - 1 - tid; mm >= istart + 1; mm -= NT) {
     const zd h11 = HH(mm, mm);
     const zd h22 = HH(mm + 1, mm + 1);
     zd h11s = h11 - shift;
@@ -735,7 +737,7 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
       break;
     }
   }
-  found = block_max_int<128>(found, redi);
+  found = block_max_int<NT>(found, redi);
   const int istart1 = (found > 0) ? found : istart;
   zd v0, v1;
   {
@@ -754,7 +756,7 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
   const int64_t need = 3 + 2 * (int64_t)(iend - istart1);
   if (zlg && zl.pos + need > zl.cap) {
     __syncthreads();
-    for (int row = 1 + tid; row <= n; row += 128) zlog_row(Zg + (row - 1), ldz, zl.g, zl.pos);
+    for (int row = 1 + tid; row <= n; row += NT) zlog_row(Zg + (row - 1), ldz, zl.g, zl.pos);
     __syncthreads();
     zl.pos = 0;
   }
@@ -892,21 +894,23 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
         if (zlg && lane < 2 * (kend - k0 + 1)) rec[2 + 2 * (k0 - istart1) + lane] = lg[lane];
       }
     } else if (prev) {
+      // the chain wavefronts (1 .. NT / 64 - 1) share the lines: thread cw
+      // of CS takes every CS-th column, row above the window and row of Z
       const int lcol0 = (chase ? we : pwe) + 1;
       if (chase) {  // with no chase left (the sweep's last chains) wavefront 0 takes the columns
         if (pkend - pk0 + 1 == ZKC)
-          for (int col = lcol0 + lane; col <= n; col += 64) zchain_full<false>(H + (int64_t)(col - 1) * ld, plg, pk0, nullptr);
+          for (int col = lcol0 + cw; col <= n; col += CS) zchain_full<false>(H + (int64_t)(col - 1) * ld, plg, pk0, nullptr);
         else
-          for (int col = lcol0 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend);
+          for (int col = lcol0 + cw; col <= n; col += CS) zchain<true>(H + (int64_t)(col - 1) * ld, 1, plg, pk0, pkend);
       }
       if (chase && pkend - pk0 + 1 == ZKC) {
-        for (int row = 1 + lane; row <= pwb - 1; row += 64) zchain_full_row<4>(H + (row - 1), ld, plg, pk0);
+        for (int row = 1 + cw; row <= pwb - 1; row += CS) zchain_full_row<4>(H + (row - 1), ld, plg, pk0);
         if (wantz && !zlg)
-          for (int row = 1 + lane; row <= n; row += 64) zchain_full_row<4>(Zg + (row - 1), ldz, plg, pk0);
+          for (int row = 1 + cw; row <= n; row += CS) zchain_full_row<4>(Zg + (row - 1), ldz, plg, pk0);
       } else {
-        for (int row = 1 + lane; row <= pwb - 1; row += 64) zchain<false>(H + (row - 1), ld, plg, pk0, pkend);
+        for (int row = 1 + cw; row <= pwb - 1; row += CS) zchain<false>(H + (row - 1), ld, plg, pk0, pkend);
         if (wantz && !zlg)
-          for (int row = 1 + lane; row <= n; row += 64) zchain<false>(Zg + (row - 1), ldz, plg, pk0, pkend);
+          for (int row = 1 + cw; row <= n; row += CS) zchain<false>(Zg + (row - 1), ldz, plg, pk0, pkend);
       }
     }
     if (wave == 0 && !chase && prev)
@@ -919,9 +923,9 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
       if (wave == 0) {
         for (int col = we + 1 + lane; col <= n; col += 64) zchain<true>(H + (int64_t)(col - 1) * ld, 1, lg, k0, kend);
       } else {
-        for (int row = 1 + lane; row <= wb - 1; row += 64) zchain<false>(H + (row - 1), ld, lg, k0, kend);
+        for (int row = 1 + cw; row <= wb - 1; row += CS) zchain<false>(H + (row - 1), ld, lg, k0, kend);
         if (wantz && !zlg)
-          for (int row = 1 + lane; row <= n; row += 64) zchain<false>(Zg + (row - 1), ldz, lg, k0, kend);
+          for (int row = 1 + cw; row <= n; row += CS) zchain<false>(Zg + (row - 1), ldz, lg, k0, kend);
       }
       __syncthreads();
       zd t = zr(1.0) - lg[0];
@@ -935,7 +939,7 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
       for (int cc = 1; cc <= n; ++cc) {
         const bool cin = (cc >= istart1 && cc <= iend && cc != istart1 + 1);
         const int rmax = min(cc - 1, iend);
-        for (int r = 1 + tid; r <= rmax; r += 128) {
+        for (int r = 1 + tid; r <= rmax; r += NT) {
           const bool rin = (r >= istart1 && r != istart1 + 1);
           if (!rin && !cin) continue;
           zd x = HH(r, cc);
@@ -947,7 +951,7 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
       if (zlg) {
         if (tid == 0) rec[1] = ctt;
       } else if (wantz) {
-        for (int r = 1 + tid; r <= n; r += 128)
+        for (int r = 1 + tid; r <= n; r += NT)
           for (int cc = istart1; cc <= iend; ++cc)
             if (cc != istart1 + 1) ZZ(r, cc) = zmul_fixed(ZZ(r, cc), ctt);
       }
@@ -972,10 +976,10 @@ template <bool FAST> GS_DEV void z_sweep_win2(zd* H, int64_t ld, int n, zd* Zg, 
     __syncthreads();
     if (tid == 0) HH(iend, iend - 1) = zr(rtv);
     const zd ctt = zconj(t);
-    for (int cc = iend + 1 + tid; cc <= n; cc += 128) HH(iend, cc) = HH(iend, cc) * ctt;
-    for (int r = 1 + tid; r <= iend - 1; r += 128) HH(r, iend) = HH(r, iend) * t;
+    for (int cc = iend + 1 + tid; cc <= n; cc += NT) HH(iend, cc) = HH(iend, cc) * ctt;
+    for (int r = 1 + tid; r <= iend - 1; r += NT) HH(r, iend) = HH(r, iend) * t;
     if (wantz && !zlg)
-      for (int r = 1 + tid; r <= n; r += 128) ZZ(r, iend) = zmul_fixed(ZZ(r, iend), t);
+      for (int r = 1 + tid; r <= n; r += NT) ZZ(r, iend) = zmul_fixed(ZZ(r, iend), t);
   }
   if (zlg) {
     if (tid == 0) rec[need - 1] = tail ? t : zr(0.0);
@@ -1063,7 +1067,7 @@ template <int NT, class L, bool FAST, int WIN = 0> GS_DEV int64_t z_qr(zd* H, L 
       }
       nsweep += 1;
       if constexpr (WIN == 2)
-        z_sweep_win2<FAST>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + ZWL * ZW, redi, t, istart, iend, nstep, *zl);
+        z_sweep_win2<FAST, NT>(H, lay.ld, n, Zg, ldz, wantz, rot, rot + ZWL * ZW, redi, t, istart, iend, nstep, *zl);
       else
         z_sweep<NT, L, FAST>(H, lay, n, Zg, ldz, wantz, rot, redi, t, istart, iend, nstep);
     }
@@ -1179,9 +1183,10 @@ template <int NT, bool FAST> __global__ void __launch_bounds__(NT) zqr_kernel(Ba
 
 // Two-wavefront windowed QR kernel: the chase of chunk c overlaps the chains
 // of chunk c-1 (z_sweep_win2); the control logic runs on both wavefronts.
-template <bool FAST> __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) zqrw2_kernel(BatchArgs a) {
+template <bool FAST, int NT = 128>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) zqrw2_kernel(BatchArgs a) {
   __shared__ zd win[ZWL * ZW + 4 * ZKC + ZWL + 1];
-  __shared__ int redi[4];
+  __shared__ int redi[NT / 64 > 4 ? NT / 64 : 4];
   const int b = blockIdx.x;
   if (b >= a.count) return;
   // the chasing wavefront (0) issues ahead of the chain wavefronts sharing
@@ -1206,7 +1211,7 @@ template <bool FAST> __global__ void __launch_bounds__(128) __attribute__((amdgp
     zl.g = reinterpret_cast<zd*>(a.zlog) + a.loff[m];
     zl.cap = a.loff[m + 1] - a.loff[m];
   }
-  const int64_t st = z_qr<128, FullLayout, FAST, 2>(T, lay, n, Z, (int64_t)n, wantz, win, redi, maxiter, maxinner,
+  const int64_t st = z_qr<NT, FullLayout, FAST, 2>(T, lay, n, Z, (int64_t)n, wantz, win, redi, maxiter, maxinner,
                                                     nsweep, nstep, &zl);
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1218,7 +1223,7 @@ template <bool FAST> __global__ void __launch_bounds__(128) __attribute__((amdgp
     a.info[m] = (int32_t)(st > 0x7fffffff ? 0x7fffffff : st);
   }
   if (st != 0) return;
-  z_finish<128>(T, lay, n, T, (int64_t)n, W, a.scl[2 * m], a.scl[2 * m + 1]);
+  z_finish<NT>(T, lay, n, T, (int64_t)n, W, a.scl[2 * m], a.scl[2 * m + 1]);
 }
 
 // ------------------------------------------------------------------------
@@ -1282,6 +1287,60 @@ template <int NT> __global__ void __launch_bounds__(NT) zschur_global_kernel(Sin
   }
   if (st != 0) return;
   z_finish<NT>(T, lay, n, T, ldt, W, cscale, anrm);
+}
+
+// Hessenberg reduction and Q formation of one matrix in global memory, for
+// the windowed QR kernel above the LDS-resident orders: the scaled A (or
+// the Hessenberg input) as H in T, zero below the subdiagonal, Q (or the
+// caller's Z) in Z, (cscale, anrm) of _scale! in scl[0], scl[1] and the
+// status in info.  The same steps as zschur_global_kernel before its QR
+// phase; zqrw2_kernel then runs the sweeps with T and Z in HBM.
+template <int NT> __global__ void __launch_bounds__(NT) zhq_global_kernel(SingleArgs a, double* scl) {
+  __shared__ double red[NT / 64 + 1];
+  const int n = (int)a.n;
+  zd* A = reinterpret_cast<zd*>(a.A);
+  zd* T = reinterpret_cast<zd*>(a.T);
+  zd* Z = reinterpret_cast<zd*>(a.Z);
+  zd* tau = reinterpret_cast<zd*>(a.ws);
+  const int64_t ldt = a.ldt, ldz = a.ldz;
+  const bool wantz = (a.flags & 0x01) != 0;
+  const bool hessin = (a.flags & 0x04) != 0;
+  double cscale, anrm;
+  const int st0 = z_prep<NT>(A, a.lda, T, ldt, n, a.flags, red, cscale, anrm);
+  if (threadIdx.x == 0) {
+    *a.info = st0;
+    scl[0] = cscale;
+    scl[1] = anrm;
+  }
+  if (st0 != 0) return;
+  if (!hessin) {
+    z_hessenberg<NT>(T, ldt, n, tau, red, tau + 4 * n);
+    if (wantz) {
+      for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        Z[r + c * ldz] = T[r + c * ldt];
+      }
+      __syncthreads();
+      z_form_q_inplace<NT>(T, ldt, n, tau, tau + 4 * n);
+      for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+        const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+        const zd q = T[r + c * ldt];
+        const zd h = Z[r + c * ldz];
+        Z[r + c * ldz] = q;
+        T[r + c * ldt] = h;
+      }
+      __syncthreads();
+    }
+  }
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) {
+    const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+    if (r > c + 1) T[r + c * ldt] = zr(0.0);
+  }
+}
+
+hipError_t launch_zhq_global(const SingleArgs& a, double* scl, hipStream_t s) {
+  hipLaunchKernelGGL(zhq_global_kernel<1024>, dim3(1), dim3(1024), 0, s, a, scl);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------
@@ -1348,6 +1407,21 @@ hipError_t launch_zqrw(const BatchArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((zqrw2_kernel<false>), dim3(a.count), dim3(128), 0, s, a);
   else
     hipLaunchKernelGGL((zqrw2_kernel<true>), dim3(a.count), dim3(128), 0, s, a);
+  return hipGetLastError();
+}
+
+// One matrix above the LDS-resident orders: wavefront 0 chases, the other
+// nt / 64 - 1 wavefronts of the workgroup share the chains (H's rows above
+// and columns right of the window, and Z), whose count grows with n.
+hipError_t launch_zqrw_single(const BatchArgs& a, int nt, hipStream_t s) {
+  if (a.count <= 0) return hipSuccess;
+  switch (nt) {
+    case 128: hipLaunchKernelGGL((zqrw2_kernel<true, 128>), dim3(a.count), dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL((zqrw2_kernel<true, 256>), dim3(a.count), dim3(256), 0, s, a); break;
+    case 512: hipLaunchKernelGGL((zqrw2_kernel<true, 512>), dim3(a.count), dim3(512), 0, s, a); break;
+    case 1024: hipLaunchKernelGGL((zqrw2_kernel<true, 1024>), dim3(a.count), dim3(1024), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -82,6 +82,18 @@ def test_families_single(gs, oracle, cplx, sizes):
     report(f"single {'complex' if cplx else 'real'}", tally)
 
 
+@pytest.mark.parametrize("n", [100, 130, 200])
+def test_families_single_complex_windowed(gs, oracle, n):
+    """Complex orders above the LDS-resident range: Hessenberg + Q in global
+    memory, then the windowed QR kernel with H and Z in HBM."""
+    assert n > gs.batched_max_n(True)
+    tally = {}
+    for name, A, normal in families(n, True, seed=4):
+        S = gs.gschur(A)
+        check_case(oracle, name, A, S, normal, tally)
+    report(f"windowed complex n={n}", tally)
+
+
 @pytest.mark.parametrize("cplx", [True, False])
 def test_families_batched(gs, oracle, cplx):
     """All families and orders as one batch (size bins, graph, Z log)."""
@@ -222,7 +234,7 @@ def test_checksd_argument_error(gs, oracle, n):
 
 
 @pytest.mark.parametrize("cplx,n,plain", [(True, 32, False), (False, 32, False), (True, 120, True),
-                                          (False, 160, True), (False, 200, False)])
+                                          (True, 120, False), (False, 160, True), (False, 200, False)])
 def test_maxiter_unconverged(gs, oracle, cplx, n, plain):
     """maxiter = 1 raises UnconvergedException (GenericSchur.jl:234-236,
     :554-556) on every kernel, as in the oracle."""
