@@ -136,6 +136,19 @@ size_t vjp_lds_bytes(int n);
 hipError_t launch_vjp_assemble(const VjpArgs& a, int nmax, hipStream_t s);
 
 
+// eigen! pieces (eigen.hip): balancing of one matrix (out = ilo, ihi,
+// trivial, status), the balancer's back-transform of complex columns,
+// triangularize, _enormalize!, reciprocal eigenvalue condition numbers
+hipError_t launch_balance(bool cplx, double* A, int64_t lda, int n, int scale, int permute, int32_t* out, int32_t* sp,
+                          double* D, hipStream_t s);
+hipError_t launch_balback(int n, int m, const int32_t* iit, const int32_t* sp, const double* D, double* V,
+                          int64_t ldv, int inverse, hipStream_t s);
+hipError_t launch_triangularize(int n, const double* T, int64_t ldt, const double* Z, int64_t ldz, double* Tc,
+                                int64_t ldtc, double* Zc, int64_t ldzc, double* W, hipStream_t s);
+hipError_t launch_enormalize(int n, int m, double* V, int64_t ldv, hipStream_t s);
+hipError_t launch_eig_rcond(int n, int m, const double* VL, int64_t ldl, const double* VR, int64_t ldr, double* rc,
+                            hipStream_t s);
+
 // No-allocation batched entry point (gs_schur_batched_z_noalloc): the size
 // of its fixed device scratch array (one per GPU, noalloc.hip) and the most
 // matrices one call takes (56 bytes of metadata each, 256-byte aligned

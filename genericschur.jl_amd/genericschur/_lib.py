@@ -35,6 +35,7 @@ EXPORTS = (
     "gs_schur_z", "gs_schur_d", "gs_schur_batched_z", "gs_schur_batched_d", "gs_batched_max_n",
     "gs_trevc_z", "gs_trevc_d", "gs_trevc_batched_z", "gs_trevc_batched_d",
     "gs_eig_vjp_batched_d", "gs_last_stats", "gs_last_launches", "gs_dgemm", "gs_schur_batched_z_noalloc",
+    "gs_balance_z", "gs_balance_d", "gs_balance_apply", "gs_triangularize_d", "gs_enormalize", "gs_eig_rcond",
 )
 
 _lib = None
@@ -117,6 +118,13 @@ def lib():
         L.gs_trevc_batched_z.argtypes = sig_tvb
         L.gs_trevc_batched_d.argtypes = sig_tvb
         L.gs_eig_vjp_batched_d.argtypes = [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int]
+        sig_bal = [c_vp, c_i64, c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_int]
+        L.gs_balance_z.argtypes = sig_bal
+        L.gs_balance_d.argtypes = sig_bal
+        L.gs_balance_apply.argtypes = [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int]
+        L.gs_triangularize_d.argtypes = [c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_int]
+        L.gs_enormalize.argtypes = [c_vp, c_i64, c_i64, c_vp, c_i64, c_int]
+        L.gs_eig_rcond.argtypes = [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_int]
         L.gs_last_stats.argtypes = [c_vp, c_i64p, c_int]
         L.gs_last_launches.argtypes = [c_vp, c_vp, c_int]
         L.gs_dgemm.argtypes = [c_vp, c_int, c_int, c_i64, c_i64, c_i64, ctypes.c_double, c_vp, c_i64, c_vp, c_i64,

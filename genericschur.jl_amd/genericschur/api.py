@@ -13,6 +13,12 @@ this module                 reference
 ``eigvals_(A, sortby)``     ``LinearAlgebra.eigvals!(A)`` pirate (pirates.jl:17-27)
 ``geigvecs(S, left)``       ``geigvecs(S; left)`` (vectors.jl:12-20)
 ``eigvecs(S, left)``        ``eigvecs(S::Schur; left)`` pirate (pirates.jl:38-48)
+``eigen_(A, ...)``          ``LinearAlgebra.eigen!(A; permute, scale, sortby,
+                            jvl, jvr, jce)`` pirate (pirates.jl:63-160)
+``balance_(A, ...)``        ``balance!(A; scale, permute)`` (balance.jl:33-199)
+``lmul_(B, V)``, ``ldiv_``  ``lmul!/ldiv!(B::Balancer, V)`` (balance.jl:201-261)
+``triangularize(S)``        ``triangularize(S::Schur{<:Real})`` (triang.jl:9-43)
+``enormalize_(V)``          ``_enormalize!(V)`` (util.jl:572-592)
 ``Schur``                   ``LinearAlgebra.Schur`` (fields T, Z, values)
 ``UnconvergedException``    ``GenericSchur.UnconvergedException``
 ``DimensionMismatch``       ``Base.DimensionMismatch``
@@ -360,3 +366,184 @@ def batched_max_n(complex_input):
 def synchronize(device=0):
     h = _lib.handle(device)
     _lib.check(_lib.lib().gs_synchronize(h), h, "gs_synchronize")
+
+
+# ------------------------------------------------------------------ eigen!
+class Balancer:
+    """Balancer (balance.jl:3-10): ilo, ihi (1-based), prow, pcol (1-based
+    permutation records), D (scale factors), trivial."""
+
+    __slots__ = ("ilo", "ihi", "sp", "D", "trivial")
+
+    def __init__(self, ilo, ihi, sp, D, trivial):
+        self.ilo, self.ihi, self.sp, self.D, self.trivial = ilo, ihi, sp, D, trivial
+
+    @property
+    def prow(self):
+        return self.sp[: self.ilo - 1]
+
+    @property
+    def pcol(self):
+        return self.sp[self.ihi:]
+
+    def __repr__(self):
+        return f"Balancer(ilo={self.ilo}, ihi={self.ihi}, trivial={self.trivial})"
+
+
+class Eigen:
+    """LinearAlgebra.Eigen: values, vectors, and (eigen!'s jvl / jce options)
+    vectorsl, rconde."""
+
+    __slots__ = ("values", "vectors", "vectorsl", "rconde", "rcondv")
+
+    def __init__(self, values, vectors, vectorsl=None, rconde=None, rcondv=None):
+        self.values, self.vectors, self.vectorsl, self.rconde, self.rcondv = values, vectors, vectorsl, rconde, rcondv
+
+    def __iter__(self):
+        return iter((self.values, self.vectors))
+
+
+def balance_(A, scale=True, permute=True, device=0):
+    """balance!(A; scale, permute) -> (Abal, Balancer) (balance.jl:33-199).
+    A (numpy, Float64 or ComplexF64) is balanced on the GPU; the input array
+    is not modified (a balanced copy is returned)."""
+    A = np.asarray(A)
+    n = _checksquare(A)
+    cplx = _kind(A)
+    a = np.array(A, dtype=A.dtype, order="F", copy=True)
+    out = np.zeros(4, dtype=np.int32)
+    sp = np.zeros(max(n, 1), dtype=np.int32)
+    D = np.ones(max(n, 1))
+    h = _lib.handle(device)
+    f = _lib.lib().gs_balance_z if cplx else _lib.lib().gs_balance_d
+    rc = f(h, n, _ptr(a), max(n, 1), int(bool(scale)), int(bool(permute)), _ptr(out), _ptr(sp), _ptr(D), 0)
+    _lib.check(rc, h, "gs_balance")
+    if out[3] == -3:
+        raise RuntimeError("NaN encountered while balancing")
+    if out[3] != 0:
+        raise GSError(f"balancing did not settle (status {out[3]})")
+    return np.ascontiguousarray(a), Balancer(int(out[0]), int(out[1]), sp[:n].copy(), D[:n].copy(), bool(out[2]))
+
+
+def _bal_apply(B, V, inverse, device):
+    V = np.asarray(V)
+    if V.ndim != 2:
+        raise DimensionMismatch("eigenvector matrix expected")
+    n, m = V.shape
+    v = np.array(V, dtype=np.complex128, order="F", copy=True)
+    b3 = np.array([B.ilo, B.ihi, int(B.trivial)], dtype=np.int32)
+    sp = np.ascontiguousarray(B.sp, dtype=np.int32)
+    D = np.ascontiguousarray(B.D, dtype=np.float64)
+    if len(sp) != n or len(D) != n:
+        raise DimensionMismatch("balancer and vectors differ in size")
+    h = _lib.handle(device)
+    rc = _lib.lib().gs_balance_apply(h, n, m, _ptr(b3), _ptr(sp), _ptr(D), _ptr(v), max(n, 1), int(inverse), 0)
+    _lib.check(rc, h, "gs_balance_apply")
+    return np.ascontiguousarray(v)
+
+
+def lmul_(B, V, device=0):
+    """lmul!(B::Balancer, V): back-transform of right eigenvectors
+    (balance.jl:201-230).  Returns the transformed (complex) copy."""
+    return _bal_apply(B, V, False, device)
+
+
+def ldiv_(B, V, device=0):
+    """ldiv!(B::Balancer, V): back-transform of left eigenvectors
+    (balance.jl:233-261).  Returns the transformed (complex) copy."""
+    return _bal_apply(B, V, True, device)
+
+
+def triangularize(S, device=0):
+    """triangularize(S::Schur{<:Real}) -> complex Schur (triang.jl:9-43)."""
+    T = np.asarray(S.T)
+    n = _checksquare(T)
+    if T.dtype != np.float64:
+        raise TypeError("triangularize takes a real Schur factorization")
+    t = np.array(T, order="F", copy=True)
+    Z = np.asarray(S.Z)
+    z = None if Z.size == 0 else np.array(Z, dtype=np.float64, order="F", copy=True)
+    Tc = np.zeros((n, n), dtype=np.complex128, order="F")
+    Zc = np.zeros((n, n), dtype=np.complex128, order="F") if z is not None else np.zeros((0, 0), dtype=np.complex128)
+    W = np.zeros(n, dtype=np.complex128)
+    h = _lib.handle(device)
+    ld = max(n, 1)
+    rc = _lib.lib().gs_triangularize_d(h, n, _ptr(t), ld, None if z is None else _ptr(z), ld, _ptr(Tc), ld,
+                                       None if z is None else _ptr(Zc), ld, _ptr(W), 0)
+    _lib.check(rc, h, "gs_triangularize_d")
+    return Schur(np.ascontiguousarray(Tc), np.ascontiguousarray(Zc), W)
+
+
+def enormalize_(V, device=0):
+    """_enormalize!(V) (util.jl:572-592): unit 2-norm columns, the first
+    largest entry of each real and positive.  Returns the normalised copy."""
+    V = np.asarray(V)
+    n, m = V.shape
+    v = np.array(V, dtype=np.complex128, order="F", copy=True)
+    h = _lib.handle(device)
+    rc = _lib.lib().gs_enormalize(h, n, m, _ptr(v), max(n, 1), 0)
+    _lib.check(rc, h, "gs_enormalize")
+    return np.ascontiguousarray(v)
+
+
+def _eig_rcond(VL, VR, device):
+    n, m = VR.shape
+    vl = np.array(VL, dtype=np.complex128, order="F", copy=True)
+    vr = np.array(VR, dtype=np.complex128, order="F", copy=True)
+    rc = np.zeros(m)
+    h = _lib.handle(device)
+    st = _lib.lib().gs_eig_rcond(h, n, m, _ptr(vl), max(n, 1), _ptr(vr), max(n, 1), _ptr(rc), 0)
+    _lib.check(st, h, "gs_eig_rcond")
+    return rc
+
+
+def eigen_(A, permute=True, scale=True, sortby="eigsortby", jvl=False, jvr=True, jce=False, device=0):
+    """eigen!(A; permute, scale, sortby, jvl, jvr, jce) (pirates.jl:63-160).
+
+    Balancing (balance!), gschur!, triangularize for real input, the complex
+    right (and left) eigenvector solves of geigvecs without normalisation,
+    the balancer's back-transform, _enormalize!, and sorteig! by
+    eigsortby = (real, imag).  With jce the reciprocal eigenvalue condition
+    numbers of the balanced form are returned in rconde.  Every numerical
+    step runs on the GPU; only the final sort permutation is host logic."""
+    A = np.asarray(A)
+    n = _checksquare(A)
+    cplx = _kind(A)
+    if permute or scale:
+        Ab, B = balance_(A, scale=scale, permute=permute, device=device)
+    else:
+        Ab, B = np.array(A, copy=True), None
+    S = gschur_(Ab, device=device)
+    if not cplx:
+        S = triangularize(S, device=device)
+    vals = S.values
+    do_vr = jvr or jce
+    do_vl = jvl or jce
+    v = geigvecs(S, left=False, normalize=False, device=device) if do_vr else np.zeros((0, 0), dtype=np.complex128)
+    vl = geigvecs(S, left=True, normalize=False, device=device) if do_vl else np.zeros((0, 0), dtype=np.complex128)
+    rconde = _eig_rcond(vl, v, device) if jce else None
+    if jvr:
+        if B is not None:
+            v = lmul_(B, v, device=device)
+        v = enormalize_(v, device=device)
+    else:
+        v = np.zeros((0, 0), dtype=np.complex128)
+    if jvl:
+        if B is not None:
+            vl = ldiv_(B, vl, device=device)
+        vl = enormalize_(vl, device=device)
+    else:
+        vl = None
+    if sortby is not None:
+        if sortby == "eigsortby":
+            p = np.lexsort((vals.imag, vals.real))
+        else:
+            p = np.array(sorted(range(n), key=lambda i: sortby(vals[i])), dtype=np.int64)
+        vals = vals[p]
+        if v.size:
+            v = v[:, p]
+        if vl is not None:
+            vl = vl[:, p]
+        if rconde is not None:
+            rconde = rconde[p]
+    return Eigen(vals, v, vl, rconde)

@@ -181,6 +181,41 @@ int gs_trevc_batched_d(gs_handle* h, int64_t count, const int32_t* n, const int6
 int gs_eig_vjp_batched_d(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off, const double* T,
                          const double* Z, const double* gbar, double* Abar, int32_t* info, int flags);
 
+/* ---- eigen! around the Schur factorization (pirates.jl:63-160) ---- */
+
+/* Balancing: balance!(A; scale, permute) — balance.jl:33-199 (xGEBAL's
+ * permutation search and power-of-2 scaling, p = 1, algo = :pr).  A (n x n,
+ * lda) is balanced in place.  out4 receives {ilo, ihi, trivial, status}
+ * (1-based ilo/ihi as Balancer.ilo/ihi; status 0, -3 when a NaN was met
+ * while scaling — the reference's error("NaN encountered while balancing")
+ * — or -4 when the scaling sweeps did not settle); sp (n int32) the
+ * permutation record, 1-based (Balancer.prow = sp[0 .. ilo-2],
+ * Balancer.pcol = sp[ihi .. n-1]); D (n doubles) the scale factors.
+ * With GS_DEVICE_PTRS every array is device memory. */
+int gs_balance_z(gs_handle* h, int64_t n, double* A, int64_t lda, int scale, int permute, int32_t* out4, int32_t* sp,
+                 double* D, int flags);
+int gs_balance_d(gs_handle* h, int64_t n, double* A, int64_t lda, int scale, int permute, int32_t* out4, int32_t* sp,
+                 double* D, int flags);
+/* The balancer's back-transform of the m complex columns of V (n x m, ldv):
+ * lmul!(B, V) for right eigenvectors (inverse = 0) or ldiv!(B, V) for left
+ * eigenvectors (inverse = 1) — balance.jl:201-261.  bal3 = {ilo, ihi,
+ * trivial}, sp and D as returned by gs_balance_*. */
+int gs_balance_apply(gs_handle* h, int64_t n, int64_t m, const int32_t* bal3, const int32_t* sp, const double* D,
+                     double* V, int64_t ldv, int inverse, int flags);
+/* triangularize(S) — triang.jl:9-43: the standard-form real Schur
+ * factorization (T, Z) to the complex triangular one (Tc, Zc), W = diag(Tc)
+ * (n complex).  Z may be NULL (then Zc is not referenced). */
+int gs_triangularize_d(gs_handle* h, int64_t n, const double* T, int64_t ldt, const double* Z, int64_t ldz,
+                       double* Tc, int64_t ldtc, double* Zc, int64_t ldzc, double* W, int flags);
+/* _enormalize!(V) — util.jl:572-592: each of the m complex columns of V
+ * scaled to unit 2-norm with its first largest entry real and positive. */
+int gs_enormalize(gs_handle* h, int64_t n, int64_t m, double* V, int64_t ldv, int flags);
+/* Reciprocal eigenvalue condition numbers of eigen!(A; jce = true) —
+ * pirates.jl:112-120: rconde[j] = |vl_j^H vr_j| / ||vl_j|| / ||vr_j|| over
+ * the m complex columns of VL and VR. */
+int gs_eig_rcond(gs_handle* h, int64_t n, int64_t m, const double* VL, int64_t ldvl, const double* VR, int64_t ldvr,
+                 double* rconde, int flags);
+
 /* Statistics of the last Schur call on this handle: out[0] = QR sweeps,
  * out[1] = bulge steps, out[2] = kernel time in microseconds (events on the
  * handle's stream, 0 when unavailable).  Returns the number written. */

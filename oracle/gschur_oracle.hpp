@@ -1717,4 +1717,182 @@ template <class R> void triangularize(long n, const R* tr, long ldt, const R* zr
     for (long i = j + 1; i <= n; ++i) T(i, j) = C();
 }
 
+// ------------------------------------------------- balance.jl:33-199
+// balance!(A; scale, permute) with p = 1, algo = :pr (xGEBAL's scheme).
+// On exit a holds the balanced matrix, ilo/ihi (1-based) the bounds of the
+// unreduced block, sp[0..n-1] the permutation record (sp[ms-1] = js), d the
+// diagonal scale factors.  Returns 0, or -3 when a NaN was met while
+// scaling (the reference's error("NaN encountered while balancing")).
+template <class R> R bal_abs2max(long m, const R* x, long inc) {
+  R ax0 = rt<R>::abs(x[0]);  // norm1est.jl:76-87
+  for (long i = 1; i < m; ++i) {
+    R ax = rt<R>::abs(x[i * inc]);
+    if (ax > ax0) ax0 = ax;
+  }
+  return ax0;
+}
+template <class R> R bal_abs2max(long m, const cx<R>* x, long inc) {
+  R ax0 = abs2(x[0]);  // norm1est.jl:89-100
+  for (long i = 1; i < m; ++i) {
+    R ax = abs2(x[i * inc]);
+    if (ax > ax0) ax0 = ax;
+  }
+  return rt<R>::sqrt(ax0);
+}
+// Julia's max / min propagate a NaN operand
+template <class R> R jmax(const R& a, const R& b) { return (rt<R>::isnan(a) || rt<R>::isnan(b)) ? a + b : rmax(a, b); }
+template <class R> R jmin(const R& a, const R& b) { return (rt<R>::isnan(a) || rt<R>::isnan(b)) ? a + b : rmin(a, b); }
+template <class T> int balance(long n, T* a, long lda, bool scale, bool permute, long& ilo, long& ihi, long* sp,
+                               typename real_of<T>::type* d, bool& trivial) {
+  typedef typename real_of<T>::type R;
+  Mat<T> A{a, lda};
+  const R beta = R(2.0);
+  const R factor = R(0.95);
+  const R sfmin1 = rt<R>::fmin() / rt<R>::eps();
+  const R sfmin2 = sfmin1 * beta;
+  const R sfmax2 = R(1.0) / sfmin2;
+  for (long i = 0; i < n; ++i) {
+    d[i] = R(1.0);
+    sp[i] = 0;
+  }
+  ilo = 1;
+  ihi = n;
+  trivial = true;
+  auto swap_cols_rows = [&](long js, long ms, long rlast, long cfirst) {
+    for (long i = 1; i <= rlast; ++i) std::swap(A(i, js), A(i, ms));
+    for (long i = cfirst; i <= n; ++i) std::swap(A(js, i), A(ms, i));
+  };
+  if (permute) {
+    ihi = n + 1;
+    long js = 0;
+    while (ihi > 1) {
+      ihi -= 1;
+      bool exch = false;
+      long ms = 0;
+      for (long j = ihi; j >= 1; --j) {
+        exch = true;
+        js = j;
+        for (long i = 1; i <= ihi; ++i) {
+          if (i == j) continue;
+          if (!(A(j, i) == T(R(0.0)))) exch = false;
+        }
+        if (exch) {
+          ms = ihi;
+          break;
+        }
+      }
+      if (exch) {
+        sp[ms - 1] = js;
+        if (js != ms) {
+          trivial = false;
+          swap_cols_rows(js, ms, ihi, ilo);
+        }
+      } else {
+        break;
+      }
+    }
+    if (ihi > 1) {
+      ilo = 0;
+      while (ilo < n) {
+        ilo += 1;
+        bool exch = false;
+        long ms = 0;
+        js = 0;
+        for (long j = ilo; j <= ihi; ++j) {
+          js = j;
+          exch = true;
+          for (long i = ilo; i <= ihi; ++i) {
+            if (i == j) continue;
+            if (!(A(i, j) == T(R(0.0)))) exch = false;
+          }
+          if (exch) {
+            ms = ilo;
+            break;
+          }
+        }
+        if (exch) {
+          sp[ms - 1] = js;
+          if (ms != js) {
+            trivial = false;
+            swap_cols_rows(js, ms, ihi, ilo);
+          }
+        } else {
+          break;
+        }
+      }
+    }
+  }
+  if (scale) {
+    bool converged = false;
+    while (!converged) {
+      converged = true;
+      for (long i = ilo; i <= ihi; ++i) {
+        const long m = ihi - ilo + 1;
+        R c = norm2(m, &A(ilo, i), 1);
+        R r = norm2(m, &A(i, ilo), lda);
+        R ca = bal_abs2max(m, &A(ilo, i), 1);
+        R ra = bal_abs2max(m, &A(i, ilo), lda);
+        // a NaN norm would make the reference's sweep loop run forever; it
+        // is reported as the reference's NaN error instead
+        if (rt<R>::isnan(c) || rt<R>::isnan(r)) return -3;
+        if (c == R(0.0) || r == R(0.0)) continue;
+        R g = r / beta;
+        R sv = c + r;
+        R f = R(1.0);
+        while (c < r / beta) {
+          if (c >= g || jmax(f, jmax(c, ca)) >= sfmax2 || jmin(r, jmin(g, ra)) <= sfmin2)
+            break;
+          if (rt<R>::isnan(c + f + ca + r + g + ra)) return -3;
+          f *= beta;
+          c *= beta;
+          ca *= beta;
+          r /= beta;
+          g /= beta;
+          ra /= beta;
+        }
+        g = c / beta;
+        while (r <= c / beta) {
+          if (g < r || jmax(r, ra) >= sfmax2 || jmin(jmin(f, c), jmin(g, ca)) <= sfmin2) break;
+          f /= beta;
+          c /= beta;
+          g /= beta;
+          ca /= beta;
+          r *= beta;
+          ra *= beta;
+        }
+        if (f != R(1.0)) trivial = false;
+        if (c + r >= factor * sv) continue;
+        converged = false;
+        d[i - 1] *= f;
+        const R fi = R(1.0) / f;
+        for (long k = ilo; k <= n; ++k) A(i, k) = A(i, k) * fi;
+        for (long k = 1; k <= ihi; ++k) A(k, i) = A(k, i) * f;
+      }
+    }
+  }
+  return 0;
+}
+
+// ----------------------------------------------- balance.jl:201-261
+// lmul!(B, V) (right eigenvectors, inverse = false) or ldiv!(B, V) (left
+// eigenvectors, inverse = true) on the n x m columns of V.
+template <class R> void balance_apply(long n, long m, long ilo, long ihi, const long* sp, const R* d, bool trivial,
+                                      cx<R>* v, long ldv, bool inverse) {
+  if (trivial) return;
+  Mat<cx<R>> V{v, ldv};
+  if (ilo != ihi)
+    for (long j = 1; j <= m; ++j)
+      for (long i = 1; i <= n; ++i) V(i, j) = V(i, j) * (inverse ? R(1.0) / d[i - 1] : d[i - 1]);
+  for (long j = ilo - 1; j >= 1; --j) {
+    const long mm = sp[j - 1];
+    if (mm == j) continue;
+    for (long i = 1; i <= m; ++i) std::swap(V(j, i), V(mm, i));
+  }
+  for (long j = ihi + 1; j <= n; ++j) {
+    const long mm = sp[j - 1];
+    if (mm == j) continue;
+    for (long i = 1; i <= m; ++i) std::swap(V(j, i), V(mm, i));
+  }
+}
+
 }  // namespace orc

@@ -210,4 +210,32 @@ void orc_qd_add(const double* a, const double* b, double* r) {
   for (int i = 0; i < 4; ++i) r[i] = s.x[i];
 }
 
+
+// balance!(A; scale, permute) — balance.jl:33-199.  ilo_ihi_triv receives
+// {ilo, ihi, trivial}; sp (n) the permutation record; d (n) the scale
+// factors.  Returns 0, or -3 for a NaN met while scaling.
+int orc_balance_z(long n, double* a, long lda, int scale, int permute, long* ilo_ihi_triv, long* sp, double* d) {
+  long ilo = 1, ihi = n;
+  bool triv = true;
+  int r = balance<zc>(n, (zc*)a, lda, scale != 0, permute != 0, ilo, ihi, sp, d, triv);
+  ilo_ihi_triv[0] = ilo;
+  ilo_ihi_triv[1] = ihi;
+  ilo_ihi_triv[2] = triv ? 1 : 0;
+  return r;
+}
+int orc_balance_d(long n, double* a, long lda, int scale, int permute, long* ilo_ihi_triv, long* sp, double* d) {
+  long ilo = 1, ihi = n;
+  bool triv = true;
+  int r = balance<double>(n, a, lda, scale != 0, permute != 0, ilo, ihi, sp, d, triv);
+  ilo_ihi_triv[0] = ilo;
+  ilo_ihi_triv[1] = ihi;
+  ilo_ihi_triv[2] = triv ? 1 : 0;
+  return r;
+}
+// lmul!(B, V) / ldiv!(B, V) — balance.jl:201-261, V complex n x m
+void orc_balance_apply(long n, long m, const long* ilo_ihi_triv, const long* sp, const double* d, double* v, long ldv,
+                       int inverse) {
+  balance_apply<double>(n, m, ilo_ihi_triv[0], ilo_ihi_triv[1], sp, d, ilo_ihi_triv[2] != 0, (zc*)v, ldv,
+                        inverse != 0);
+}
 }  // extern "C"

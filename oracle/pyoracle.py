@@ -55,6 +55,10 @@ def lib():
         L.orc_triangularize.restype = None
         L.orc_schur_batched_z.argtypes = [c_long, c_ip, c_lp, c_dp, c_dp, c_dp, c_dp, c_ip, c_int, c_int]
         L.orc_schur_batched_d.argtypes = [c_long, c_ip, c_lp, c_dp, c_dp, c_dp, c_dp, c_ip, c_int, c_int]
+        L.orc_balance_z.argtypes = [c_long, c_dp, c_long, c_int, c_int, c_lp, c_lp, c_dp]
+        L.orc_balance_d.argtypes = [c_long, c_dp, c_long, c_int, c_int, c_lp, c_lp, c_dp]
+        L.orc_balance_apply.argtypes = [c_long, c_long, c_lp, c_lp, c_dp, c_dp, c_long, c_int]
+        L.orc_balance_apply.restype = None
         L.orc_eig_hp_z.argtypes = [c_long, c_dp, c_dp, c_int]
         L.orc_eig_hp_d.argtypes = [c_long, c_dp, c_dp, c_int]
         for f in ("orc_qd_sqrt",):
@@ -183,6 +187,8 @@ def eigvecs(T, Z=None, left=False, normalize=True):
 
 
 def enormalize(V):
+    """_enormalize!(V) (util.jl:572-592); V is square, as in the reference."""
+    assert V.ndim == 2 and V.shape[0] == V.shape[1], "enormalize takes a square matrix"
     v = _fort(V, np.complex128)
     lib().orc_enormalize(v.shape[0], _p(v), v.shape[0])
     return v
@@ -244,3 +250,75 @@ def qd_op(op, a, b=None):
         b = np.asarray(b, dtype=np.float64)
         getattr(lib(), "orc_qd_" + op)(a.ctypes.data_as(c_dp), b.ctypes.data_as(c_dp), r.ctypes.data_as(c_dp))
     return r
+
+
+def balance(A, scale=True, permute=True):
+    """balance!(A; scale, permute) (balance.jl:33-199).
+
+    Returns (Abal, (ilo, ihi, trivial), sp, d, info); ilo/ihi are 1-based,
+    sp is the permutation record (1-based), d the scale factors."""
+    A = np.asarray(A)
+    n = A.shape[0]
+    cplx = np.iscomplexobj(A)
+    a = _fort(A, np.complex128 if cplx else np.float64)
+    iit = np.zeros(3, dtype=np.int64)
+    sp = np.zeros(max(n, 1), dtype=np.int64)
+    d = np.ones(max(n, 1))
+    f = lib().orc_balance_z if cplx else lib().orc_balance_d
+    info = f(n, _p(a), max(n, 1), int(scale), int(permute), iit.ctypes.data_as(c_lp), sp.ctypes.data_as(c_lp),
+             d.ctypes.data_as(c_dp))
+    return a, (int(iit[0]), int(iit[1]), bool(iit[2])), sp[:n], d[:n], info
+
+
+def balance_apply(V, bal, sp, d, inverse=False):
+    """lmul!(B, V) (inverse=False) / ldiv!(B, V) (inverse=True) (balance.jl:201-261)."""
+    v = _fort(V, np.complex128)
+    n, m = v.shape
+    iit = np.array([bal[0], bal[1], int(bal[2])], dtype=np.int64)
+    spa = np.ascontiguousarray(sp, dtype=np.int64)
+    da = np.ascontiguousarray(d, dtype=np.float64)
+    lib().orc_balance_apply(n, m, iit.ctypes.data_as(c_lp), spa.ctypes.data_as(c_lp), da.ctypes.data_as(c_dp),
+                            _p(v), max(n, 1), int(inverse))
+    return v
+
+
+def eigen(A, permute=True, scale=True, sort=True, jvl=False, jce=False):
+    """eigen!(A; permute, scale, sortby=eigsortby, jvl, jce) (pirates.jl:63-160),
+    composed from the restated pieces: balance!, gschur!, triangularize
+    (real input), the complex right / left eigenvector solves without
+    normalisation, lmul!/ldiv!(B, .), _enormalize!, sorteig!.
+
+    Returns (values, vectors, vectors_left or None, rconde or None)."""
+    A = np.asarray(A)
+    n = A.shape[0]
+    if permute or scale:
+        Ab, bal, sp, d, info = balance(A, scale=scale, permute=permute)
+        assert info == 0
+    else:
+        Ab = np.array(A)
+    T, Z, w, info, _ = gschur(Ab)
+    assert info == 0
+    if not np.iscomplexobj(A):
+        T, Z = triangularize(T, Z)
+        w = np.diag(T).copy()
+    v = eigvecs(T, Z, left=False, normalize=False)
+    vl = eigvecs(T, Z, left=True, normalize=False) if (jvl or jce) else None
+    rc = None
+    if jce:
+        rc = np.array([abs(np.vdot(vl[:, j], v[:, j])) / np.linalg.norm(vl[:, j]) / np.linalg.norm(v[:, j])
+                       for j in range(n)])
+    if permute or scale:
+        v = balance_apply(v, bal, sp, d, inverse=False)
+        if vl is not None:
+            vl = balance_apply(vl, bal, sp, d, inverse=True)
+    v = enormalize(v)
+    if vl is not None:
+        vl = enormalize(vl)
+    if sort:
+        p = np.lexsort((w.imag, w.real))
+        w, v = w[p], v[:, p]
+        if vl is not None:
+            vl = vl[:, p]
+        if rc is not None:
+            rc = rc[p]
+    return w, v, (vl if jvl else None), rc

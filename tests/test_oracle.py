@@ -422,3 +422,120 @@ def test_families_vs_lapack(oracle, cplx):
             s = 1.0 if m == 0 else 2.0 ** -np.round(np.log2(m))
             ev = sla.eigvals(A * s) / s
             checks.eigen_parity(A, ev, w, cond)
+
+
+# ------------------------------------------------------------ balancing
+def _bal_cases():
+    r = np.random.default_rng(17)
+    out = []
+    for cplx in (False, True):
+        for trial in range(6):
+            n = (5, 12, 30)[trial % 3]
+            A = r.standard_normal((n, n)) * np.exp(3 * r.standard_normal((n, 1))) * np.exp(3 * r.standard_normal((1, n)))
+            if trial >= 3:
+                A[np.abs(A) < 1.0] = 0.0  # zeros: row / column permutations
+            if cplx:
+                A = A + 1j * A[::-1]
+            out.append(A)
+    return out
+
+
+def test_balance_matches_lapack(oracle):
+    """balance.jl:33-199 vs LAPACK xGEBAL (the routine it translates): the
+    balanced matrix, ilo/ihi, the scale factors and the permutation record
+    bit for bit.  A wrong loop bound of the permutation search, a swapped
+    row/column exchange or a wrong scaling test changes one of them."""
+    for A in _bal_cases():
+        ab, bal, sp, d, info = oracle.balance(A)
+        assert info == 0
+        f = lap.zgebal if np.iscomplexobj(A) else lap.dgebal
+        ba, lo, hi, ps, inf = f(A, scale=1, permute=1)
+        assert (bal[0], bal[1]) == (lo + 1, hi + 1)
+        assert np.array_equal(ab, ba)
+        assert np.array_equal(d[bal[0] - 1:bal[1]], ps[lo:hi + 1])
+        outside = np.r_[0:lo, hi + 1:A.shape[0]]
+        assert np.array_equal(sp[outside], ps[outside].astype(np.int64))
+
+
+def test_balance_sanity_block_structure(oracle):
+    """test/balance.jl:74-106: a permuted block upper-triangular matrix
+    [T1 0 0; 0 A2 0; 0 0 T3] balances to upper triangular outside the
+    ilo:ihi block."""
+    r = np.random.default_rng(3)
+    for n1 in (0, 1, 2):
+        for n2 in (0, 1, 2):
+            for n3 in (0, 1, 2):
+                n = n1 + n2 + n3
+                if n == 0:
+                    continue
+                A = np.zeros((n, n))
+                A[:n1, :n1] = np.triu(r.random((n1, n1)))
+                A[n1:n1 + n2, n1:n1 + n2] = r.random((n2, n2))
+                A[n1 + n2:, n1 + n2:] = np.triu(r.random((n3, n3)))
+                P = np.eye(n)[r.permutation(n)]
+                Ax = P.T @ A @ P
+                C, bal, sp, d, info = oracle.balance(Ax)
+                ilo, ihi = bal[0], bal[1]
+                assert ilo <= ihi
+                for i in range(1, n + 1):
+                    for j in range(1, i):
+                        if i < ilo or j > ihi:
+                            assert C[i - 1, j - 1] == 0
+
+
+def unbal_classic(cplx):
+    """test/balance.jl:28-42: a badly balanced 3 x 3 matrix with known
+    asymptotic eigenvalues."""
+    m = np.round(-2 * np.log2(EPS) / 5)
+    x = 2.0 ** m
+    A = np.array([[1, 0, x ** -2], [1, 1, x ** -1], [x ** 2, x, 1]], dtype=np.complex128 if cplx else np.float64)
+    rt2 = np.sqrt(2.0)
+    lam = np.array([1 - rt2 + 1 / (4 * x), 1 - 1 / (2 * x), 1 + rt2 + 1 / (4 * x)])
+    return A, lam
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_balance_unbal_classic(oracle, cplx):
+    """test/balance.jl:58-72: eigenvalues of the balanced matrix are far more
+    accurate (ru > 100 rb), and the left eigenvectors of the balanced
+    matrix, back-transformed by ldiv!(B, Vl), are left eigenvectors of A."""
+    A, lam = unbal_classic(cplx)
+    wu = np.sort(oracle.gschur(A)[2].real)
+    Ab, bal, sp, d, info = oracle.balance(A)
+    T, Z, wb, info2, _ = oracle.gschur(Ab)
+    ru = np.abs(wu - lam).max()
+    rb = np.abs(np.sort(wb.real) - lam).max()
+    assert ru > 100 * rb
+    if cplx:
+        Vl = oracle.eigvecs(T, Z, left=True)
+        Vl = oracle.balance_apply(Vl, bal, sp, d, inverse=True)
+        assert np.allclose(A.conj().T @ Vl, Vl @ np.diag(wb.conj()), rtol=1e-8, atol=1e-8)
+
+
+def test_eigen_composition_matches_lapack(oracle):
+    """eigen!(A) (pirates.jl:63-160) composed from the restated pieces:
+    sorted eigenvalues and unit eigenvectors (largest entry real) against
+    scipy's xGEEV, phase-insensitive, for balanced and unbalanced input."""
+    r = np.random.default_rng(5)
+    for cplx in (False, True):
+        for n in (6, 25):
+            A = r.standard_normal((n, n)) * np.exp(2 * r.standard_normal((n, 1))) * np.exp(-2 * r.standard_normal((1, n)))
+            if cplx:
+                A = A + 1j * r.standard_normal((n, n))
+            for bal in (True, False):
+                w, V, VL, rc = oracle.eigen(A, permute=bal, scale=bal, jvl=True, jce=True)
+                ev, vl, vr = sla.eig(A, left=True, right=True)
+                perm, err = checks.match_eigenvalues(ev, w)
+                assert err.max() <= 1e-9 * np.abs(ev).max()
+                for ref, got in ((vr[:, perm], V), (vl[:, perm], VL)):
+                    p = np.abs(np.sum(ref.conj() * got, axis=0)) / np.linalg.norm(ref, axis=0)
+                    assert np.all(np.abs(p - 1) < 1e-7)
+                assert np.all(np.abs(np.linalg.norm(V, axis=0) - 1) < 1e-13)
+                assert np.all((rc > 0) & (rc <= 1 + 1e-12))
+
+
+def test_balance_nan_error(oracle):
+    """A NaN entry is the reference's "NaN encountered while balancing"
+    (balance.jl:160-162) rather than a sweep loop that never settles."""
+    A = np.array([[1.0, 2.0, 0.5], [np.nan, 3.0, 1.0], [4.0, 1.0, 2.0]])
+    assert oracle.balance(A)[4] == -3
