@@ -11,6 +11,8 @@
 // parked in V, then back-transform by Z and normalisation.  The conjugate
 // partner of a complex pair is written last as the conjugate of the
 // finished column, which equals the reference's copy-then-normalise.
+#include <type_traits>
+
 #include "dev_common.h"
 #include "internal.h"
 #include "wavevec.h"
@@ -20,9 +22,21 @@ namespace gs {
 
 GS_DEV int trd(int i, int j) { return (j * (j + 1)) / 2 + i; }
 
+// Entry (i, j), i <= j, of T: from the compact copy in LDS, or from T in
+// global memory for orders whose triangle does not fit a CU's LDS.
+struct TriLdsD {
+  const double* p;
+  GS_DEV double operator()(int i, int j) const { return p[trd(i, j)]; }
+};
+struct GlobD {
+  const double* p;
+  int64_t ld;
+  GS_DEV double operator()(int i, int j) const { return p[i + (int64_t)j * ld]; }
+};
+
 // T entry (0-based) from compact upper storage + subdiagonal
-GS_DEV double Tat(const double* Tc, const double* sub, int i, int j) {
-  if (i <= j) return Tc[trd(i, j)];
+template <class TA> GS_DEV double Tat(const TA& Tc, const double* sub, int i, int j) {
+  if (i <= j) return Tc(i, j);
   if (i == j + 1) return sub[j];
   return 0.0;
 }
@@ -139,12 +153,12 @@ GS_DEV double xsolve_dev(int na, const double* Ae, zd bw, zd* B, double smin, do
 // Solution for column ki (0-based) with ip = 0 (real wr) or -1 (pair,
 // lambda = wr + i wi, ki the second row of the block).  Entries 0..ki hold
 // the vector on exit.
-template <int R> GS_DEV void dsolve_right(const double* Tc, const double* sub, const double* tnorms, int ki, int ip,
+template <int R, class TA> GS_DEV void dsolve_right(const TA& Tc, const double* sub, const double* tnorms, int ki, int ip,
                                           double smallnum, double bignum, WaveVecZ<R>& v) {
   const int lane = threadIdx.x & 63;
   const double ulp = GS_EPS;
-  const double wr = Tc[trd(ki, ki)];
-  const double wi = (ip == 0) ? 0.0 : sqrt(fabs(sub[ki - 1])) * sqrt(fabs(Tc[trd(ki - 1, ki)]));
+  const double wr = Tc(ki, ki);
+  const double wi = (ip == 0) ? 0.0 : sqrt(fabs(sub[ki - 1])) * sqrt(fabs(Tc(ki - 1, ki)));
   const double smin = fmax(ulp * (fabs(wr) + fabs(wi)), smallnum);
   const zd lam = zmk(wr, wi);
   int jtop;
@@ -152,14 +166,14 @@ template <int R> GS_DEV void dsolve_right(const double* Tc, const double* sub, c
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int i = lane + 64 * q;
-      v.v[q] = (i < ki) ? zr(-Tc[trd(i, ki)]) : (i == ki ? zr(1.0) : zr(0.0));
+      v.v[q] = (i < ki) ? zr(-Tc(i, ki)) : (i == ki ? zr(1.0) : zr(0.0));
     }
     jtop = ki - 1;
   } else {
     zd a, c;
-    if (fabs(Tc[trd(ki - 1, ki)]) >= fabs(sub[ki - 1])) {
+    if (fabs(Tc(ki - 1, ki)) >= fabs(sub[ki - 1])) {
       a = zr(1.0);
-      c = zmk(0.0, wi / Tc[trd(ki - 1, ki)]);
+      c = zmk(0.0, wi / Tc(ki - 1, ki));
     } else {
       a = zr(-wi / sub[ki - 1]);
       c = zmk(0.0, 1.0);
@@ -168,7 +182,7 @@ template <int R> GS_DEV void dsolve_right(const double* Tc, const double* sub, c
     for (int q = 0; q < R; ++q) {
       const int i = lane + 64 * q;
       zd val = zr(0.0);
-      if (i < ki - 1) val = zmk(-a.x * Tc[trd(i, ki - 1)], -(c.y * Tc[trd(i, ki)]));
+      if (i < ki - 1) val = zmk(-a.x * Tc(i, ki - 1), -(c.y * Tc(i, ki)));
       else if (i == ki - 1) val = a;
       else if (i == ki) val = c;
       v.v[q] = val;
@@ -180,7 +194,7 @@ template <int R> GS_DEV void dsolve_right(const double* Tc, const double* sub, c
     const bool two = (j > 0) && (sub[j - 1] != 0.0);
     if (!two) {
       zd B[1] = {v.get(j)};
-      const double Ae[1] = {Tc[trd(j, j)]};
+      const double Ae[1] = {Tc(j, j)};
       double t1;
       double vscale = xsolve_dev(1, Ae, lam, B, smin, t1);
       if (t1 > 1.0 && tnorms[j] > bignum / t1) {
@@ -196,12 +210,12 @@ template <int R> GS_DEV void dsolve_right(const double* Tc, const double* sub, c
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int i = lane + 64 * q;
-        if (i < j) v.v[q] = v.v[q] - B[0] * Tc[trd(i, j)];
+        if (i < j) v.v[q] = v.v[q] - B[0] * Tc(i, j);
       }
       j -= 1;
     } else {
       zd B[2] = {v.get(j - 1), v.get(j)};
-      const double Ae[4] = {Tc[trd(j - 1, j - 1)], sub[j - 1], Tc[trd(j - 1, j)], Tc[trd(j, j)]};
+      const double Ae[4] = {Tc(j - 1, j - 1), sub[j - 1], Tc(j - 1, j), Tc(j, j)};
       double t1;
       double vscale = xsolve_dev(2, Ae, lam, B, smin, t1);
       if (t1 > 1.0) {
@@ -222,7 +236,7 @@ template <int R> GS_DEV void dsolve_right(const double* Tc, const double* sub, c
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int i = lane + 64 * q;
-        if (i < j - 1) v.v[q] = (v.v[q] - B[0] * Tc[trd(i, j - 1)] - B[1] * Tc[trd(i, j)]);
+        if (i < j - 1) v.v[q] = (v.v[q] - B[0] * Tc(i, j - 1) - B[1] * Tc(i, j));
       }
       j -= 2;
     }
@@ -234,12 +248,12 @@ template <int R> GS_DEV void dsolve_right(const double* Tc, const double* sub, c
 // block, lambda = wr - i wi in the solves).  Entries ki..n-1 hold it.
 // Forward substitution in axpy form; the reference's overflow guards
 // (vmax, vcrit) and the j1 = j + 1 quirk of vectors.jl:627 are kept.
-template <int R> GS_DEV void dsolve_left(const double* Tc, const double* sub, const double* tnorms, int n, int ki,
+template <int R, class TA> GS_DEV void dsolve_left(const TA& Tc, const double* sub, const double* tnorms, int n, int ki,
                                          int ip, double smallnum, double bignum, WaveVecZ<R>& v) {
   const int lane = threadIdx.x & 63;
   const double ulp = GS_EPS;
-  const double wr = Tc[trd(ki, ki)];
-  const double wi = (ip == 0) ? 0.0 : sqrt(fabs(Tc[trd(ki, ki + 1)])) * sqrt(fabs(sub[ki]));
+  const double wr = Tc(ki, ki);
+  const double wi = (ip == 0) ? 0.0 : sqrt(fabs(Tc(ki, ki + 1))) * sqrt(fabs(sub[ki]));
   const double smin = fmax(ulp * (fabs(wr) + fabs(wi)), smallnum);
   int jstart;
   zd lam;
@@ -247,14 +261,14 @@ template <int R> GS_DEV void dsolve_left(const double* Tc, const double* sub, co
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int i = lane + 64 * q;
-      v.v[q] = (i > ki && i < n) ? zr(-Tc[trd(ki, i)]) : (i == ki ? zr(1.0) : zr(0.0));
+      v.v[q] = (i > ki && i < n) ? zr(-Tc(ki, i)) : (i == ki ? zr(1.0) : zr(0.0));
     }
     jstart = ki + 1;
     lam = zr(wr);
   } else {
     zd a, c;
-    if (fabs(Tc[trd(ki, ki + 1)]) >= fabs(sub[ki])) {
-      a = zr(wi / Tc[trd(ki, ki + 1)]);
+    if (fabs(Tc(ki, ki + 1)) >= fabs(sub[ki])) {
+      a = zr(wi / Tc(ki, ki + 1));
       c = zmk(0.0, 1.0);
     } else {
       a = zr(1.0);
@@ -264,7 +278,7 @@ template <int R> GS_DEV void dsolve_left(const double* Tc, const double* sub, co
     for (int q = 0; q < R; ++q) {
       const int i = lane + 64 * q;
       zd val = zr(0.0);
-      if (i > ki + 1 && i < n) val = zmk(-a.x * Tc[trd(ki, i)], -(c.y * Tc[trd(ki + 1, i)]));
+      if (i > ki + 1 && i < n) val = zmk(-a.x * Tc(ki, i), -(c.y * Tc(ki + 1, i)));
       else if (i == ki) val = a;
       else if (i == ki + 1) val = c;
       v.v[q] = val;
@@ -287,7 +301,7 @@ template <int R> GS_DEV void dsolve_left(const double* Tc, const double* sub, co
         vcrit = bignum;
       }
       zd B[1] = {v.get(j)};
-      const double Ae[1] = {Tc[trd(j, j)]};
+      const double Ae[1] = {Tc(j, j)};
       double xn;
       const double vscale = xsolve_dev(1, Ae, lam, B, smin, xn);
       if (vscale != 1.0) {
@@ -302,7 +316,7 @@ template <int R> GS_DEV void dsolve_left(const double* Tc, const double* sub, co
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int i = lane + 64 * q;
-        if (i > j && i < n) v.v[q] = v.v[q] - B[0] * Tc[trd(j, i)];
+        if (i > j && i < n) v.v[q] = v.v[q] - B[0] * Tc(j, i);
       }
       j += 1;
     } else {
@@ -317,7 +331,7 @@ template <int R> GS_DEV void dsolve_left(const double* Tc, const double* sub, co
         vcrit = bignum;
       }
       zd B[2] = {v.get(j), v.get(j + 1)};
-      const double Ae[4] = {Tc[trd(j, j)], Tc[trd(j, j + 1)], sub[j], Tc[trd(j + 1, j + 1)]};
+      const double Ae[4] = {Tc(j, j), Tc(j, j + 1), sub[j], Tc(j + 1, j + 1)};
       double xn;
       const double vscale = xsolve_dev(2, Ae, lam, B, smin, xn);
       if (vscale != 1.0) {
@@ -332,7 +346,7 @@ template <int R> GS_DEV void dsolve_left(const double* Tc, const double* sub, co
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int i = lane + 64 * q;
-        if (i > j + 1 && i < n) v.v[q] = v.v[q] - B[0] * Tc[trd(j, i)] - B[1] * Tc[trd(j + 1, i)];
+        if (i > j + 1 && i < n) v.v[q] = v.v[q] - B[0] * Tc(j, i) - B[1] * Tc(j + 1, i);
       }
       j += 2;
     }
@@ -397,7 +411,7 @@ template <int R> GS_DEV void enormalize_w(WaveVecZ<R>& v, int n) {
   v.set(bi, zr(r0.x));
 }
 
-template <int NT, int R> __global__ void __launch_bounds__(NT) dtrevc_kernel(TrevcArgsD a) {
+template <int NT, int R, bool GLOB> __global__ void __launch_bounds__(NT) dtrevc_kernel(TrevcArgsD a) {
   extern __shared__ double smem[];
   const int b = blockIdx.x;
   if (b >= a.count) return;
@@ -411,20 +425,25 @@ template <int NT, int R> __global__ void __launch_bounds__(NT) dtrevc_kernel(Tre
   const double* Tg = a.T + o;
   const double* Zg = a.Z ? a.Z + o : nullptr;
   zd* Vg = reinterpret_cast<zd*>(a.V) + o;
-  double* Tc = smem;
-  double* sub = Tc + (n * (n + 1)) / 2;
+  double* Tl = smem;
+  double* sub = GLOB ? smem : Tl + (n * (n + 1)) / 2;
   double* tnorms = sub + n;
   int* kind = reinterpret_cast<int*>(tnorms + n);  // per column: 0 real, -1/+1 pair rows
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   constexpr int NW = NT / 64;
   const bool left = a.left != 0;
-  for (int j = 0; j < n; ++j)
-    for (int i = tid; i <= j; i += NT) Tc[trd(i, j)] = Tg[i + j * ldt];
+  if (!GLOB)
+    for (int j = 0; j < n; ++j)
+      for (int i = tid; i <= j; i += NT) Tl[trd(i, j)] = Tg[i + j * ldt];
   for (int j = tid; j < n; j += NT) sub[j] = (j + 1 < n) ? Tg[(j + 1) + j * ldt] : 0.0;
   __syncthreads();
+  typedef typename std::conditional<GLOB, GlobD, TriLdsD>::type TA;
+  TA Tc;
+  if constexpr (GLOB) Tc = GlobD{Tg, ldt};
+  else Tc = TriLdsD{Tl};
   for (int j = tid; j < n; j += NT) {
     double s = 0.0;
-    for (int i = 0; i < j; ++i) s += fabs(Tc[trd(i, j)]);
+    for (int i = 0; i < j; ++i) s += fabs(Tc(i, j));
     tnorms[j] = s;
     // right: the second row of a pair (T[j, j-1] != 0) solves; left: the first
     const bool pr = (j > 0) && (sub[j - 1] != 0.0);
@@ -539,24 +558,28 @@ template <int NT, int R> __global__ void __launch_bounds__(NT) dtrevc_kernel(Tre
 }
 
 size_t dtrevc_lds_bytes(int n) { return 8 * ((size_t)n * (n + 1) / 2 + 2 * (size_t)n) + 4 * (size_t)n + 64; }
+static size_t dtrevc_glob_lds_bytes(int n) { return 8 * 2 * (size_t)n + 4 * (size_t)n + 64; }
 
+template <int NT, int R, bool GLOB> static void launch_dtrevc_t(const TrevcArgsD& a, size_t lds, hipStream_t s) {
+  (void)hipFuncSetAttribute((const void*)dtrevc_kernel<NT, R, GLOB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+  hipLaunchKernelGGL((dtrevc_kernel<NT, R, GLOB>), dim3(a.count), dim3(NT), lds, s, a);
+}
+
+// Orders up to 192 keep T's triangle in LDS; above, T is read from global
+// memory (subdiagonal, column norms and block kinds in LDS), up to
+// dtrevc_max_n().
 hipError_t launch_dtrevc(const TrevcArgsD& a, int nmax, hipStream_t s) {
   if (a.count <= 0) return hipSuccess;
-  const size_t lds = dtrevc_lds_bytes(nmax);
-  constexpr int NT = 256;
-  if (nmax <= 64) {
-    (void)hipFuncSetAttribute((const void*)dtrevc_kernel<NT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((dtrevc_kernel<NT, 1>), dim3(a.count), dim3(NT), lds, s, a);
-  } else if (nmax <= 128) {
-    (void)hipFuncSetAttribute((const void*)dtrevc_kernel<NT, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((dtrevc_kernel<NT, 2>), dim3(a.count), dim3(NT), lds, s, a);
-  } else if (nmax <= 192) {
-    (void)hipFuncSetAttribute((const void*)dtrevc_kernel<NT, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((dtrevc_kernel<NT, 3>), dim3(a.count), dim3(NT), lds, s, a);
-  } else {
-    return hipErrorInvalidValue;
-  }
+  if (nmax <= 64) launch_dtrevc_t<256, 1, false>(a, dtrevc_lds_bytes(nmax), s);
+  else if (nmax <= 128) launch_dtrevc_t<256, 2, false>(a, dtrevc_lds_bytes(nmax), s);
+  else if (nmax <= 192) launch_dtrevc_t<256, 3, false>(a, dtrevc_lds_bytes(nmax), s);
+  else if (nmax <= 256) launch_dtrevc_t<512, 4, true>(a, dtrevc_glob_lds_bytes(nmax), s);
+  else if (nmax <= 512) launch_dtrevc_t<512, 8, true>(a, dtrevc_glob_lds_bytes(nmax), s);
+  else if (nmax <= 1024) launch_dtrevc_t<256, 16, true>(a, dtrevc_glob_lds_bytes(nmax), s);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
+int dtrevc_max_n() { return 1024; }
 
 }  // namespace gs

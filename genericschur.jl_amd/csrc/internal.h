@@ -115,6 +115,10 @@ typedef TrevcArgs TrevcArgsD;
 
 size_t ztrevc_lds_bytes(int n);
 size_t dtrevc_lds_bytes(int n);
+// largest order the eigenvector kernels take (T in global memory above the
+// LDS-resident orders 128 complex / 192 real)
+int ztrevc_max_n();
+int dtrevc_max_n();
 hipError_t launch_ztrevc(const TrevcArgs& a, int nmax, hipStream_t s);
 hipError_t launch_dtrevc(const TrevcArgsD& a, int nmax, hipStream_t s);
 

@@ -26,7 +26,7 @@ int run_trevc(gs_handle* h, bool cplx, int64_t count, const int32_t* n, const in
   int nmax = 0;
   for (int64_t i = 0; i < count; ++i) nmax = std::max(nmax, (int)n[i]);
   if (nmax == 0) return GS_OK;
-  const int lim = cplx ? 128 : 192;
+  const int lim = cplx ? ztrevc_max_n() : dtrevc_max_n();
   if (nmax > lim) return handle_fail(h, GS_E_UNSUPPORTED, "order above the eigenvector kernel limit");
   std::lock_guard<std::mutex> lk(handle_mutex(h));
   hipError_t e = hipSetDevice(handle_device(h));

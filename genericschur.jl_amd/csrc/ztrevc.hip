@@ -11,6 +11,8 @@
 // LDS, and the scalar control of the scaled solver runs uniformly in every
 // lane.  The solutions are parked in the output V, then each column is
 // back-transformed by Z (V[:, k] = Z x) and normalised.
+#include <type_traits>
+
 #include "dev_common.h"
 #include "internal.h"
 #include "wavevec.h"
@@ -21,18 +23,31 @@ namespace gs {
 // compact upper triangle: column j (0-based) holds rows 0..j at j(j+1)/2
 GS_DEV int tri(int i, int j) { return (j * (j + 1)) / 2 + i; }
 
+// Entry (i, j), i <= j, of T: from the compact copy in LDS (orders whose
+// triangle fits a CU's LDS) or from T itself in global memory (larger
+// orders; the reads stay in L2 across the substitutions).
+struct TriLdsZ {
+  const zd* p;
+  GS_DEV zd operator()(int i, int j) const { return p[tri(i, j)]; }
+};
+struct GlobZ {
+  const zd* p;
+  int64_t ld;
+  GS_DEV zd operator()(int i, int j) const { return p[i + (int64_t)j * ld]; }
+};
+
 // ------------------------------------------------------------------------
 // Shifted, clamped diagonal of the solve: T(j,j) - lam, replaced by smin
 // when its abs1 is below smin (vectors.jl:87-90 / :412-415).
-GS_DEV zd shifted_diag(const zd* Tc, int j, zd lam, double smin) {
-  zd d = Tc[tri(j, j)] - lam;
+template <class TA> GS_DEV zd shifted_diag(const TA& Tc, int j, zd lam, double smin) {
+  zd d = Tc(j, j) - lam;
   if (zabs1(d) < smin) d = zr(smin);
   return d;
 }
 
 // _usolve! on entries 0..m-1 of x (A = T shifted by lam, leading m x m).
-template <int R> GS_DEV double usolve_w(const zd* Tc, const double* cnorm, int m, zd lam, double smin,
-                                        WaveVecZ<R>& x) {
+template <int R, class TA> GS_DEV double usolve_w(const TA& Tc, const double* cnorm, int m, zd lam, double smin,
+                                                  WaveVecZ<R>& x) {
   const int lane = threadIdx.x & 63;
   const double half = 0.5;
   const double smallnum = GS_FMIN / GS_EPS;
@@ -75,11 +90,10 @@ template <int R> GS_DEV double usolve_w(const zd* Tc, const double* cnorm, int m
     for (int j = m - 1; j >= 0; --j) {
       const zd xj = x.get(j) / shifted_diag(Tc, j, lam, smin);
       x.set(j, xj);
-      const int cb = (j * (j + 1)) / 2;
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int i = lane + 64 * q;
-        if (i < j) x.v[q] = x.v[q] - Tc[cb + i] * xj;
+        if (i < j) x.v[q] = x.v[q] - Tc(i, j) * xj;
       }
     }
     return xscale;
@@ -142,13 +156,12 @@ template <int R> GS_DEV double usolve_w(const zd* Tc, const double* cnorm, int m
     }
     if (j > 0) {
       const zd xjt = x.get(j) * tscale;
-      const int cb = (j * (j + 1)) / 2;
       double xm = 0.0;
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int i = lane + 64 * q;
         if (i < j) {
-          x.v[q] = x.v[q] - xjt * Tc[cb + i];
+          x.v[q] = x.v[q] - xjt * Tc(i, j);
           xm = fmax(xm, zabs1(x.v[q]));
         }
       }
@@ -162,8 +175,8 @@ template <int R> GS_DEV double usolve_w(const zd* Tc, const double* cnorm, int m
 // _cusolve! on entries o..o+m-1 of x: (A)^H y = s b with
 // A(i, j) = T(o+i, o+j) for i < j and the shifted, clamped diagonal; cnorm
 // is indexed from 0 (the reference passes the full tnorms, vectors.jl:417-420).
-template <int R> GS_DEV double cusolve_w(const zd* Tc, const double* cnorm, int o, int m, zd lam, double smin,
-                                         WaveVecZ<R>& x) {
+template <int R, class TA> GS_DEV double cusolve_w(const TA& Tc, const double* cnorm, int o, int m, zd lam,
+                                                   double smin, WaveVecZ<R>& x) {
   const int lane = threadIdx.x & 63;
   const double half = 0.5;
   const double smallnum = GS_FMIN / GS_EPS;
@@ -216,7 +229,7 @@ template <int R> GS_DEV double cusolve_w(const zd* Tc, const double* cnorm, int 
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int i = lane + 64 * q;  // absolute entry index
-        if (i > o + j && i < o + m) x.v[q] = x.v[q] - zconj(Tc[tri(o + j, i)]) * xj;
+        if (i > o + j && i < o + m) x.v[q] = x.v[q] - zconj(Tc(o + j, i)) * xj;
       }
     }
     return xscale;
@@ -252,7 +265,7 @@ template <int R> GS_DEV double cusolve_w(const zd* Tc, const double* cnorm, int 
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int i = lane + 64 * q;
-      if (i >= o && i < o + j) part = part + (zconj(Tc[tri(i, o + j)]) * uscale) * x.v[q];
+      if (i >= o && i < o + j) part = part + (zconj(Tc(i, o + j)) * uscale) * x.v[q];
     }
     const zd csumj = wave_zsum(part);
     const bool unscaled = (uscale.x == tscale && uscale.y == 0.0) || isnan(uscale.x) || isnan(uscale.y);
@@ -354,7 +367,7 @@ template <int R> GS_DEV void normalize_col(WaveVecZ<R>& v, int n, bool enorm) {
   v.set(bi, zr(r0.x));
 }
 
-template <int NT, int R> __global__ void __launch_bounds__(NT) ztrevc_kernel(TrevcArgs a) {
+template <int NT, int R, bool GLOB> __global__ void __launch_bounds__(NT) ztrevc_kernel(TrevcArgs a) {
   extern __shared__ double smem[];
   const int b = blockIdx.x;
   if (b >= a.count) return;
@@ -368,19 +381,25 @@ template <int NT, int R> __global__ void __launch_bounds__(NT) ztrevc_kernel(Tre
   const zd* Tg = reinterpret_cast<const zd*>(a.T) + o;
   const zd* Zg = a.Z ? reinterpret_cast<const zd*>(a.Z) + o : nullptr;
   zd* Vg = reinterpret_cast<zd*>(a.V) + o;
-  zd* Tc = reinterpret_cast<zd*>(smem);
-  double* tnorms = reinterpret_cast<double*>(Tc + (n * (n + 1)) / 2);
+  zd* Tl = reinterpret_cast<zd*>(smem);
+  double* tnorms = GLOB ? smem : reinterpret_cast<double*>(Tl + (n * (n + 1)) / 2);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   constexpr int NW = NT / 64;
   const bool left = a.left != 0;
-  for (int j = 0; j < n; ++j)
-    for (int i = tid; i <= j; i += NT) Tc[tri(i, j)] = Tg[i + j * ldt];
-  __syncthreads();
+  if (!GLOB) {
+    for (int j = 0; j < n; ++j)
+      for (int i = tid; i <= j; i += NT) Tl[tri(i, j)] = Tg[i + j * ldt];
+    __syncthreads();
+  }
+  typedef typename std::conditional<GLOB, GlobZ, TriLdsZ>::type TA;
+  TA Tc;
+  if constexpr (GLOB) Tc = GlobZ{Tg, ldt};
+  else Tc = TriLdsZ{Tl};
   // strictly-upper column norms: abs for right (vectors.jl:70-75), abs1 for
   // left (vectors.jl:395-400)
   for (int j = tid; j < n; j += NT) {
     double s = 0.0;
-    for (int i = 0; i < j; ++i) s += left ? zabs1(Tc[tri(i, j)]) : zabs(Tc[tri(i, j)]);
+    for (int i = 0; i < j; ++i) s += left ? zabs1(Tc(i, j)) : zabs(Tc(i, j));
     tnorms[j] = s;
   }
   __syncthreads();
@@ -389,7 +408,7 @@ template <int NT, int R> __global__ void __launch_bounds__(NT) ztrevc_kernel(Tre
   // solves, one eigenvector per wavefront at a time; the solution (with the
   // scale at the diagonal slot) is parked in V's column
   for (int kk = (left ? w : n - 1 - w); left ? (kk < n) : (kk >= 0); kk += (left ? NW : -NW)) {
-    const zd lam = Tc[tri(kk, kk)];
+    const zd lam = Tc(kk, kk);
     const double smin = fmax(ulp * zabs1(lam), smallnum);
     WaveVecZ<R> x;
 #pragma unroll
@@ -397,9 +416,9 @@ template <int NT, int R> __global__ void __launch_bounds__(NT) ztrevc_kernel(Tre
       const int i = lane + 64 * q;
       zd v = zr(0.0);
       if (!left) {
-        if (i < kk) v = -Tc[tri(i, kk)];
+        if (i < kk) v = -Tc(i, kk);
       } else {
-        if (i > kk && i < n) v = -zconj(Tc[tri(kk, i)]);
+        if (i > kk && i < n) v = -zconj(Tc(kk, i));
       }
       x.v[q] = v;
     }
@@ -456,20 +475,24 @@ template <int NT, int R> __global__ void __launch_bounds__(NT) ztrevc_kernel(Tre
 
 size_t ztrevc_lds_bytes(int n) { return 16 * ((size_t)n * (n + 1) / 2) + 8 * (size_t)n + 64; }
 
+template <int NT, int R, bool GLOB> static void launch_ztrevc_t(const TrevcArgs& a, size_t lds, hipStream_t s) {
+  (void)hipFuncSetAttribute((const void*)ztrevc_kernel<NT, R, GLOB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+  hipLaunchKernelGGL((ztrevc_kernel<NT, R, GLOB>), dim3(a.count), dim3(NT), lds, s, a);
+}
+
+// Orders up to 128 keep T's triangle in LDS; above, T is read from global
+// memory (only the column norms in LDS), up to ztrevc_max_n().
 hipError_t launch_ztrevc(const TrevcArgs& a, int nmax, hipStream_t s) {
   if (a.count <= 0) return hipSuccess;
-  const size_t lds = ztrevc_lds_bytes(nmax);
-  constexpr int NT = 256;
-  if (nmax <= 64) {
-    (void)hipFuncSetAttribute((const void*)ztrevc_kernel<NT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((ztrevc_kernel<NT, 1>), dim3(a.count), dim3(NT), lds, s, a);
-  } else if (nmax <= 128) {
-    (void)hipFuncSetAttribute((const void*)ztrevc_kernel<NT, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((ztrevc_kernel<NT, 2>), dim3(a.count), dim3(NT), lds, s, a);
-  } else {
-    return hipErrorInvalidValue;
-  }
+  if (nmax <= 64) launch_ztrevc_t<256, 1, false>(a, ztrevc_lds_bytes(nmax), s);
+  else if (nmax <= 128) launch_ztrevc_t<256, 2, false>(a, ztrevc_lds_bytes(nmax), s);
+  else if (nmax <= 256) launch_ztrevc_t<512, 4, true>(a, 8 * (size_t)nmax + 64, s);
+  else if (nmax <= 512) launch_ztrevc_t<256, 8, true>(a, 8 * (size_t)nmax + 64, s);
+  else if (nmax <= 1024) launch_ztrevc_t<256, 16, true>(a, 8 * (size_t)nmax + 64, s);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
+int ztrevc_max_n() { return 1024; }
 
 }  // namespace gs

@@ -142,7 +142,7 @@ def test_enormalize_matches_oracle(gs, oracle):
 
 
 @pytest.mark.parametrize("cplx", [False, True])
-@pytest.mark.parametrize("n", [5, 33, 100])
+@pytest.mark.parametrize("n", [5, 33, 100, 200])
 def test_eigen_matches_oracle(gs, oracle, cplx, n):
     """eigen!(A; jvl, jce) on the GPU against the oracle's composition of the
     same steps: sorted eigenvalues within 10 n eps ||A|| cond, right and left
