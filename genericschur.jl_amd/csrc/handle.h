@@ -39,7 +39,7 @@ struct DevBuf {
 
 struct Plan;
 constexpr int kBranches = 4;
-constexpr int kAuxBufs = 12;
+constexpr int kAuxBufs = 13;
 
 }  // namespace gs
 

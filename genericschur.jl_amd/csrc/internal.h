@@ -138,6 +138,13 @@ struct VjpArgs {
 };
 size_t vjp_lds_bytes(int n);
 hipError_t launch_vjp_assemble(const VjpArgs& a, int nmax, hipStream_t s);
+// one matrix above the LDS-resident VJP orders: c_k and the planes
+// P = [Re(Y C) | Im(Y C)], Q = [Re X | Im X] (n x 2n, ld n) of the GEMM
+// Abar = P Q^T (info set to 1 when some y_k^H x_k vanished); and a fill
+// kernel that clears info[0 .. count)
+hipError_t launch_vjp_planes(int n, const double* VR, const double* VL, const double* g, double* P, double* Q,
+                             int32_t* info, hipStream_t s);
+hipError_t launch_vjp_info_clear(int32_t* info, int count, hipStream_t s);
 
 
 // eigen! pieces (eigen.hip): balancing of one matrix (out = ilo, ihi,

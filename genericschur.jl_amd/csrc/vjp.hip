@@ -75,6 +75,54 @@ template <int NT> __global__ void __launch_bounds__(NT) vjp_assemble_kernel(VjpA
   if (tid == 0) a.info[m] = *bad;
 }
 
+// Orders whose two factors do not fit LDS: c_k for one matrix, one
+// wavefront per column, with the factors split into planes for a real GEMM:
+// P = [Re(Y C) | Im(Y C)] and Q = [Re X | Im X] (n x 2n each, column-major,
+// ld = n), so that Abar = Re(Y C X^H) = P Q^T.  info = 1 when some
+// y_k^H x_k vanished.
+__global__ void __launch_bounds__(256) vjp_planes_kernel(int n, const zd* VR, const zd* VL, const zd* g,
+                                                         double* P, double* Q, int32_t* info) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= n) return;
+  const zd* x = VR + (int64_t)k * n;
+  const zd* y = VL + (int64_t)k * n;
+  zd s = zr(0.0);
+  for (int i = lane; i < n; i += 64) s = s + zconj(y[i]) * x[i];
+  s = wave_zsum(s);
+  const zd d = zconj(s);
+  const bool zero = d.x == 0.0 && d.y == 0.0;
+  const zd c = zero ? zr(0.0) : g[k] / d;
+  if (zero && lane == 0) *info = 1;
+  const int64_t nn = (int64_t)n * n;
+  for (int i = lane; i < n; i += 64) {
+    const zd t = c * y[i];
+    P[i + (int64_t)k * n] = t.x;
+    P[nn + i + (int64_t)k * n] = t.y;
+    Q[i + (int64_t)k * n] = x[i].x;
+    Q[nn + i + (int64_t)k * n] = x[i].y;
+  }
+}
+
+hipError_t launch_vjp_planes(int n, const double* VR, const double* VL, const double* g, double* P, double* Q,
+                             int32_t* info, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(vjp_planes_kernel, dim3((n + 3) / 4), dim3(256), 0, s, n, reinterpret_cast<const zd*>(VR),
+                     reinterpret_cast<const zd*>(VL), reinterpret_cast<const zd*>(g), P, Q, info);
+  return hipGetLastError();
+}
+
+__global__ void vjp_info_clear_kernel(int32_t* info, int count) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < count) info[i] = 0;
+}
+
+hipError_t launch_vjp_info_clear(int32_t* info, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(vjp_info_clear_kernel, dim3((count + 255) / 256), dim3(256), 0, s, info, count);
+  return hipGetLastError();
+}
+
 size_t vjp_lds_bytes(int n) { return 16 * (2 * (size_t)(n + 1) * n + (size_t)n) + 16; }
 
 hipError_t launch_vjp_assemble(const VjpArgs& a, int nmax, hipStream_t s) {

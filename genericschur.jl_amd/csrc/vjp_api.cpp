@@ -29,8 +29,7 @@ extern "C" int gs_eig_vjp_batched_d(gs_handle* h, int64_t count, const int32_t* 
     woff[i] = wtot;
     wtot += n[i];
   }
-  if (vjp_lds_bytes(nmax) > 160 * 1024 || nmax > 192)
-    return handle_fail(h, GS_E_UNSUPPORTED, "order above the VJP kernel limit");
+  if (nmax > dtrevc_max_n()) return handle_fail(h, GS_E_UNSUPPORTED, "order above the VJP kernel limit");
   std::lock_guard<std::mutex> lk(handle_mutex(h));
   hipError_t e = hipSetDevice(handle_device(h));
   if (e != hipSuccess) return handle_fail_hip(h, e, "hipSetDevice");
@@ -98,18 +97,43 @@ extern "C" int gs_eig_vjp_batched_d(gs_handle* h, int64_t count, const int32_t* 
   t.V = static_cast<double*>(bVL.p);
   t.left = 1;
   if ((e = launch_dtrevc(t, nmax, s)) != hipSuccess) return handle_fail_hip(h, e, "eigenvector launch");
-  VjpArgs v;
-  v.order = d_order;
-  v.count = (int)count;
-  v.n = d_n;
-  v.off = d_off;
-  v.woff = d_woff;
-  v.VR = static_cast<const double*>(bVR.p);
-  v.VL = static_cast<const double*>(bVL.p);
-  v.gbar = dG;
-  v.Abar = dA;
-  v.info = dI;
-  if ((e = launch_vjp_assemble(v, nmax, s)) != hipSuccess) return handle_fail_hip(h, e, "VJP launch");
+  // orders whose two factors fit LDS: one workgroup per matrix; larger
+  // ones (first in the descending order): c_k and the planes of each
+  // matrix, then Abar = P Q^T as one FP64 MFMA GEMM of n x n x 2n
+  int64_t nlarge = 0;
+  while (nlarge < count && vjp_lds_bytes(n[order[nlarge]]) > 160 * 1024) ++nlarge;
+  if (nlarge < count) {
+    VjpArgs v;
+    v.order = d_order + nlarge;
+    v.count = (int)(count - nlarge);
+    v.n = d_n;
+    v.off = d_off;
+    v.woff = d_woff;
+    v.VR = static_cast<const double*>(bVR.p);
+    v.VL = static_cast<const double*>(bVL.p);
+    v.gbar = dG;
+    v.Abar = dA;
+    v.info = dI;
+    if ((e = launch_vjp_assemble(v, n[order[nlarge]], s)) != hipSuccess) return handle_fail_hip(h, e, "VJP launch");
+  }
+  if (nlarge > 0) {
+    const int nl = n[order[0]];
+    DevBuf& planes = handle_buf(h, 12);
+    if ((e = planes.ensure(8 * 4 * (size_t)nl * nl)) != hipSuccess) return handle_fail_hip(h, e, "hipMalloc");
+    double* P = static_cast<double*>(planes.p);
+    double* Q = P + 2 * (size_t)nl * nl;
+    const double* vr = static_cast<const double*>(bVR.p);
+    const double* vl = static_cast<const double*>(bVL.p);
+    for (int64_t q = 0; q < nlarge; ++q) {
+      const int32_t mi = order[q];
+      const int nm = n[mi];
+      if ((e = launch_vjp_info_clear(dI + mi, 1, s)) != hipSuccess ||
+          (e = launch_vjp_planes(nm, vr + 2 * off[mi], vl + 2 * off[mi], dG + 2 * woff[mi], P, Q, dI + mi, s)) !=
+              hipSuccess ||
+          (e = launch_dgemm(false, true, nm, nm, 2 * nm, 1.0, P, nm, Q, nm, 0.0, dA + off[mi], nm, s)) != hipSuccess)
+        return handle_fail_hip(h, e, "VJP launch");
+    }
+  }
   if (!dev) {
     std::vector<double> ha((size_t)span);
     if ((e = hipMemcpyAsync(ha.data(), dA, 8 * span, hipMemcpyDeviceToHost, s)) != hipSuccess ||

@@ -72,7 +72,7 @@ def test_central_differences(gs):
     for A, S, w, v, Ab in zip(mats, Ss, ws, vs, Abs):
         n = A.shape[0]
         base = S.values
-        dirs = [rng.standard_normal((n, n)) for _ in range(16)]
+        dirs = [rng.standard_normal((n, n)) for _ in range(64)]  # the workload's 64 directions per matrix
         pert = []
         for E in dirs:
             pert += [A + h * E, A - h * E]
@@ -139,3 +139,57 @@ def test_autograd_function(gs):
     gbars = [_gbar(S.values, w, v) for S, w, v in zip(Ss, ws, vs)]
     ref = np.concatenate([a.reshape(-1, order="F") for a in _vjp(gs, mats, Ss, gbars)])
     assert np.abs(A.grad.cpu().numpy() - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def _sep_ginibre(n, seed):
+    k = 0
+    while True:
+        A = gen.ginibre_real(n, gen.stream(seed, 95, n + (k << 16)))
+        if gen.min_separation(A) >= 1e-3:
+            return A
+        k += 1
+
+
+def test_large_orders_gemm_assembly(gs, oracle):
+    """Orders above the LDS-resident assembly (n > 70) form Abar as one FP64
+    MFMA GEMM of the split factors: a mixed batch against the trace invariant
+    and the oracle-eigenvector formula, and n = 100 against central finite
+    differences."""
+    mats = [_sep_ginibre(n, 9) for n in (40, 100, 250, 71, 64)]
+    Ss = [gs.gschur(A) for A in mats]  # n = 250 takes the large single-matrix path
+    Abs = _vjp(gs, mats, Ss, [np.ones(A.shape[0], dtype=complex) for A in mats])
+    for A, Ab in zip(mats, Abs):
+        print(f"case vjp trace n={A.shape[0]}", flush=True)
+        assert np.abs(Ab - np.eye(A.shape[0])).max() <= 1e-9
+    rng = np.random.default_rng(11)
+    gbars = [rng.standard_normal(A.shape[0]) + 1j * rng.standard_normal(A.shape[0]) for A in mats]
+    Abs = _vjp(gs, mats, Ss, gbars)
+    for A, S, g, Ab in zip(mats, Ss, gbars, Abs):
+        X = oracle.eigvecs(S.T, S.Z, left=False)
+        Y = oracle.eigvecs(S.T, S.Z, left=True)
+        d = np.sum(Y.conj() * X, axis=0)
+        ref = np.real((Y * (g / np.conj(d))) @ X.conj().T)
+        print(f"case vjp formula n={A.shape[0]}", flush=True)
+        assert np.abs(Ab - ref).max() <= 1e-9 * max(1.0, np.abs(ref).max())
+    A = mats[1]
+    n = A.shape[0]
+    w, v = rng.standard_normal(n), rng.standard_normal(n)
+    S = Ss[1]
+    Ab = _vjp(gs, [A], [S], [_gbar(S.values, w, v)])[0]
+    h = 1e-6
+    dirs = [rng.standard_normal((n, n)) for _ in range(8)]
+    pert = []
+    for E in dirs:
+        pert += [A + h * E, A - h * E]
+    Sp = _forward(gs, pert)
+    worst = 0.0
+    for k, E in enumerate(dirs):
+        def L(vals):
+            idx = [int(np.argmin(np.abs(vals - b))) for b in S.values]
+            assert len(set(idx)) == n
+            return _loss(vals[idx], w, v)
+        fd = (L(Sp[2 * k].values) - L(Sp[2 * k + 1].values)) / (2 * h)
+        an = float(np.sum(Ab * E))
+        worst = max(worst, abs(fd - an) / max(1.0, abs(an)))
+    print("n=100 worst FD relative error", worst, flush=True)
+    assert worst <= 1e-6
