@@ -27,6 +27,7 @@
 
 #include "dev_common.h"
 #include "chains.h"
+#include "dchase.h"
 #include "gs2x2.h"
 #include "internal.h"
 
@@ -597,6 +598,10 @@ struct LChunk {
 // window, and Z.  The regions are disjoint.
 __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
   __shared__ double win[LWL * LW];
+  // the chunk's reflectors, logged in LDS during the chase and written to
+  // the global log after it: a global store per step would make every
+  // step's LDS ordering point wait for the store to complete
+  __shared__ double lgs[3 * LKC];
   const int lane = threadIdx.x;
   const int n = a.n, iend = ck.iend, mx = ck.mx;
   const int64_t ld = a.ld;
@@ -622,92 +627,11 @@ __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
     const int W = we - wb + 1;
     win_load<double, LW, LWL>(a.H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
     __syncthreads();
-    auto HW = [&](int r, int c) -> double& { return win[(r - wb) + (c - wb) * LWL]; };
     const LState* st = a.st;
-    // the next step's reflector input comes from the lanes that computed it
-    // in the right update (v_readlane), the first step of the chunk reads
-    // the window
-    double nv0 = 0.0, nv1 = 0.0, nv2 = 0.0;
-    for (int k = ck.k0; k <= ck.kend; ++k) {
-      const int nr = min(3, iend - k + 1);
-      double v0, v1, v2s;
-      if (k == mx) {
-        v0 = st->v0;
-        v1 = st->v1;
-        v2s = st->v2;
-      } else if (k > ck.k0) {
-        v0 = nv0;
-        v1 = nv1;
-        v2s = (nr == 3) ? nv2 : 0.0;
-      } else {
-        v0 = HW(k, k - 1);
-        v1 = HW(k + 1, k - 1);
-        v2s = (nr == 3) ? HW(k + 2, k - 1) : 0.0;
-      }
-      double x1 = v1, x2 = (nr == 3) ? v2s : 0.0;
-      const double tau1 = dreflector_fast(nr, v0, x1, x2);
-      wsync();
-      if (lane == 0) {
-        if (k > mx) {
-          HW(k, k - 1) = v0;
-          HW(k + 1, k - 1) = 0.0;
-          if (k < iend - 1) HW(k + 2, k - 1) = 0.0;
-        } else if (mx > ck.istart) {
-          HW(k, k - 1) = HW(k, k - 1) * (1.0 - tau1);
-        }
-      }
-      wsync();
-      const double vv2 = x1;
-      const double tau2 = tau1 * vv2;
-      if (nr == 3) {
-        const double vv3 = x2;
-        const double tau3 = tau1 * vv3;
-        for (int j = k + lane; j <= we; j += 64) {
-          const double p0 = HW(k, j), p1 = HW(k + 1, j), p2 = HW(k + 2, j);
-          const double ss = p0 + vv2 * p1 + vv3 * p2;
-          HW(k, j) = p0 - ss * tau1;
-          HW(k + 1, j) = p1 - ss * tau2;
-          HW(k + 2, j) = p2 - ss * tau3;
-        }
-        wsync();
-        const int jm = min(k + 3, iend);
-        double na = 0.0;
-        for (int j = wb + lane; j <= jm; j += 64) {
-          const double p0 = HW(j, k), p1 = HW(j, k + 1), p2 = HW(j, k + 2);
-          const double ss = p0 + vv2 * p1 + vv3 * p2;
-          na = p0 - ss * tau1;
-          HW(j, k) = na;
-          HW(j, k + 1) = p1 - ss * tau2;
-          HW(j, k + 2) = p2 - ss * tau3;
-        }
-        nv0 = rdlane(na, (k + 1 - wb) & 63);
-        nv1 = rdlane(na, (k + 2 - wb) & 63);
-        nv2 = rdlane(na, (k + 3 - wb) & 63);
-      } else {
-        for (int j = k + lane; j <= we; j += 64) {
-          const double p0 = HW(k, j), p1 = HW(k + 1, j);
-          const double ss = p0 + vv2 * p1;
-          HW(k, j) = p0 - ss * tau1;
-          HW(k + 1, j) = p1 - ss * tau2;
-        }
-        wsync();
-        for (int j = wb + lane; j <= iend; j += 64) {
-          const double p0 = HW(j, k), p1 = HW(j, k + 1);
-          const double ss = p0 + vv2 * p1;
-          HW(j, k) = p0 - ss * tau1;
-          HW(j, k + 1) = p1 - ss * tau2;
-        }
-      }
-      if (lane == 0) {
-        double* e = a.lg + 3 * (k - mx);
-        e[0] = tau1;
-        e[1] = vv2;
-        e[2] = (nr == 3) ? x2 : 0.0;
-      }
-      wsync();
-    }
+    dchase_chunk<LWL, true>(win, wb, we, ck.k0, ck.kend, mx, ck.istart, iend, st->v0, st->v1, st->v2, lgs);
     __syncthreads();
     win_store<double, LW, LWL>(a.H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
+    for (int e = lane; e < 3 * (ck.kend - ck.k0 + 1); e += 64) a.lg[3 * (ck.k0 - mx) + e] = lgs[e];
     return;
   }
   if (!prev) return;

@@ -16,6 +16,7 @@
 // double-shift bulge reaches the third subdiagonal).  The single-matrix
 // kernel keeps H in HBM.
 #include "chains.h"
+#include "dchase.h"
 #include "dev_common.h"
 #include "gs2x2.h"
 #include "internal.h"
@@ -596,89 +597,9 @@ template <bool FAST> GS_DEV void d_sweep_win(double* H, int64_t ld, int n, doubl
     const int kend = min(k0 + DKC - 1, iend - 1);
     const int wb = max(k0 - 1, 1), we = min(kend + 3, iend);
     const int W = we - wb + 1;
-    auto WIN = [&](int r, int c) -> double& { return win[(r - wb) + (c - wb) * DWL]; };
     win_load<double, DW, DWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
     __syncthreads();
-    double nv0 = 0.0, nv1 = 0.0, nv2 = 0.0;
-    for (int k = k0; k <= kend; ++k) {
-      const int nr = min(3, iend - k + 1);
-      if (k > mx) {
-        if (k > k0) {
-          v0 = nv0;
-          v1 = nv1;
-          v2s = (nr == 3) ? nv2 : 0.0;
-        } else {
-          v0 = WIN(k, k - 1);
-          v1 = WIN(k + 1, k - 1);
-          v2s = (nr == 3) ? WIN(k + 2, k - 1) : 0.0;
-        }
-      }
-      double x1 = v1, x2 = (nr == 3) ? v2s : 0.0;
-      const double tau1 = FAST ? dreflector_fast(nr, v0, x1, x2) : dreflector(nr, v0, x1, x2);
-      wsync();
-      if (lane == 0) {
-        if (k > mx) {
-          WIN(k, k - 1) = v0;
-          WIN(k + 1, k - 1) = 0.0;
-          if (k < iend - 1) WIN(k + 2, k - 1) = 0.0;
-        } else if (mx > istart) {
-          WIN(k, k - 1) = WIN(k, k - 1) * (1.0 - tau1);
-        }
-      }
-      wsync();
-      const double vv2 = x1;
-      const double tau2 = tau1 * vv2;
-      const double vv3 = (nr == 3) ? x2 : 0.0;
-      const double tau3 = tau1 * vv3;
-      {  // left: rows k.. over the window's columns k..we
-        const int j = k + lane;
-        if (j <= we) {
-          if (nr == 3) {
-            const double p0 = WIN(k, j), p1 = WIN(k + 1, j), p2 = WIN(k + 2, j);
-            const double ss = p0 + vv2 * p1 + vv3 * p2;
-            WIN(k, j) = p0 - ss * tau1;
-            WIN(k + 1, j) = p1 - ss * tau2;
-            WIN(k + 2, j) = p2 - ss * tau3;
-          } else {
-            const double p0 = WIN(k, j), p1 = WIN(k + 1, j);
-            const double ss = p0 + vv2 * p1;
-            WIN(k, j) = p0 - ss * tau1;
-            WIN(k + 1, j) = p1 - ss * tau2;
-          }
-        }
-      }
-      wsync();
-      {  // right: columns k.. over the window's rows wb..min(k+3, iend)
-        const int jm = (nr == 3) ? min(k + 3, iend) : iend;
-        const int j = wb + lane;
-        double na = 0.0;
-        if (j <= jm) {
-          if (nr == 3) {
-            const double p0 = WIN(j, k), p1 = WIN(j, k + 1), p2 = WIN(j, k + 2);
-            const double ss = p0 + vv2 * p1 + vv3 * p2;
-            na = p0 - ss * tau1;
-            WIN(j, k) = na;
-            WIN(j, k + 1) = p1 - ss * tau2;
-            WIN(j, k + 2) = p2 - ss * tau3;
-          } else {
-            const double p0 = WIN(j, k), p1 = WIN(j, k + 1);
-            const double ss = p0 + vv2 * p1;
-            na = p0 - ss * tau1;
-            WIN(j, k) = na;
-            WIN(j, k + 1) = p1 - ss * tau2;
-          }
-        }
-        nv0 = rdlane(na, (k + 1 - wb) & 63);
-        nv1 = rdlane(na, (k + 2 - wb) & 63);
-        nv2 = rdlane(na, (k + 3 - wb) & 63);
-      }
-      if (lane == 0) {
-        lg[3 * (k - k0)] = tau1;
-        lg[3 * (k - k0) + 1] = vv2;
-        lg[3 * (k - k0) + 2] = vv3;
-      }
-      wsync();
-    }
+    dchase_chunk<DWL, FAST>(win, wb, we, k0, kend, mx, istart, iend, v0, v1, v2s, lg);
     win_store<double, DW, DWL>(H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
     __syncthreads();
     if (kend - k0 + 1 == DKC && kend <= iend - 2) {  // a full chunk of three-element reflectors
