@@ -39,6 +39,9 @@ GS_DEV void dchase_chunk(double* win, int wb, int we, int k0, int kend, int mx, 
     nv1 = *W(k0 + 1, k0 - 1);
     nv2 = (iend - k0 + 1 >= 3) ? *W(k0 + 2, k0 - 1) : 0.0;
   }
+  // every memory operation issued before the loop has completed, so the
+  // loop's waits follow its own loads only
+  __builtin_amdgcn_s_waitcnt(0);
   double betal = 0.0;  // lane s: beta of step k0 + s
   for (int k = k0; k <= kend; ++k) {
     const bool three = iend - k + 1 >= 3;
@@ -46,12 +49,14 @@ GS_DEV void dchase_chunk(double* win, int wb, int we, int k0, int kend, int mx, 
     // operands that are final before this step's reflector
     const bool rv = k + lane <= we;
     double* const rp = W(k, rv ? k + lane : k);
-    double p0 = rp[0], p1 = rp[1], p2 = three ? rp[2] : 0.0;
+    // the third entries are loaded in any case (from row k when nr = 2, and
+    // then unused): no load sits under a branch
+    double p0 = rp[0], p1 = rp[1], p2 = rp[three ? 2 : 0];
     const int jm = three ? min(k + 3, iend) : iend;
     const int crow = lane < 4 ? k + lane : wb + lane - 4;
     const bool cv = lane < 4 ? (crow <= jm) : (crow <= k - 1);
     double* const cp = W(cv ? crow : k, k);
-    double c0 = cp[0], c1 = cp[WP], c2 = three ? cp[2 * WP] : 0.0;
+    double c0 = cp[0], c1 = cp[WP], c2 = cp[three ? 2 * WP : 0];
     if (k > mx) {
       v0 = nv0;
       v1 = nv1;
@@ -81,19 +86,10 @@ GS_DEV void dchase_chunk(double* win, int wb, int we, int k0, int kend, int mx, 
       const double b01 = qperm<0x00>(p1), b02 = qperm<0x00>(p2);
       const double b10 = qperm<0x55>(p0), b12 = qperm<0x55>(p2);
       const double b20 = qperm<0xAA>(p0), b21 = qperm<0xAA>(p1);
-      if (lane == 0) {
-        c0 = p0;
-        c1 = b10;
-        c2 = b20;
-      } else if (lane == 1) {
-        c0 = b01;
-        c1 = p1;
-        c2 = b21;
-      } else if (lane == 2 && three) {
-        c0 = b02;
-        c1 = b12;
-        c2 = p2;
-      }
+      const bool l0 = lane == 0, l1 = lane == 1, l2 = lane == 2 && three;
+      c0 = l0 ? p0 : l1 ? b01 : l2 ? b02 : c0;
+      c1 = l0 ? b10 : l1 ? p1 : l2 ? b12 : c1;
+      c2 = l0 ? b20 : l1 ? b21 : l2 ? p2 : c2;
     }
     // right update of columns k .. k + nr - 1
     if (three) {

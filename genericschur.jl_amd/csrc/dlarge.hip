@@ -625,10 +625,12 @@ __global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
     }
     const int wb = max(ck.k0 - 1, 1);
     const int W = we - wb + 1;
+    // the sweep's first reflector input, read before the barrier so that it
+    // has landed when the chase starts
+    const double sv0 = a.st->v0, sv1 = a.st->v1, sv2 = a.st->v2;
     win_load<double, LW, LWL>(a.H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
     __syncthreads();
-    const LState* st = a.st;
-    dchase_chunk<LWL, true>(win, wb, we, ck.k0, ck.kend, mx, ck.istart, iend, st->v0, st->v1, st->v2, lgs);
+    dchase_chunk<LWL, true>(win, wb, we, ck.k0, ck.kend, mx, ck.istart, iend, sv0, sv1, sv2, lgs);
     __syncthreads();
     win_store<double, LW, LWL>(a.H + (wb - 1) + (int64_t)(wb - 1) * ld, ld, W, win, lane);
     for (int e = lane; e < 3 * (ck.kend - ck.k0 + 1); e += 64) a.lg[3 * (ck.k0 - mx) + e] = lgs[e];
