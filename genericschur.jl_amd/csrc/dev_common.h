@@ -436,6 +436,37 @@ GS_DEV zd zreflector2_wave(zd& a, zd& x) {
   return tau;
 }
 
+// dreflector_fast for three-element reflectors whose input every lane holds
+// alike (the chase's main steps): the unscaled sum of squares (within the
+// range checked below it neither overflows nor loses anything to underflow,
+// as for zreflector2_wave), the formulas straight through and one scalar
+// branch to the reference formulas when x = 0 or |beta| lies outside
+// [2^-300, 2^300].
+GS_DEV double dreflector3_wave(double& a, double& x1, double& x2) {
+  const double alpha = a;
+  const double beta = -copysign(sqrt_scaled(alpha * alpha + x1 * x1 + x2 * x2), alpha);
+  const double ab = fabs(beta);
+  const bool rare = ((x1 == 0.0) & (x2 == 0.0)) | !((ab >= 0x1p-300) & (ab <= 0x1p300));
+  const double d = alpha - beta;
+  const double bd = beta * d;
+  double r = __builtin_amdgcn_rcp(bd);
+  r = fma(r, fma(-bd, r, 1.0), r);
+  r = fma(r, fma(-bd, r, 1.0), r);
+  double tau = -(d * d) * r;
+  const double sc = beta * r;
+  double ao = beta, y1 = x1 * sc, y2 = x2 * sc;
+  if (wave_any(rare)) {
+    ao = a;
+    y1 = x1;
+    y2 = x2;
+    tau = dreflector(3, ao, y1, y2);
+  }
+  a = ao;
+  x1 = y1;
+  x2 = y2;
+  return tau;
+}
+
 GS_DEV double dreflector_fast(int nr, double& a, double& x1, double& x2) {
   if (nr <= 1) return 0.0;
   const double alpha = a;
