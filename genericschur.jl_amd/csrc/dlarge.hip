@@ -33,7 +33,7 @@
 
 namespace gs {
 
-constexpr int LNB = 32;  // Hessenberg panel width
+constexpr int LNB = 64;  // Hessenberg panel width
 constexpr int LW = 64;   // bulge-chase window (rows and columns)
 constexpr int LKC = LW - 4;  // bulge steps per chunk
 constexpr int LWL = LW + 1;  // odd LDS pitch of a window column (bank spread)
@@ -105,21 +105,25 @@ struct PanelArgs {
 // out[q] = sum_{r0 <= r < r1} V(r, q) x[r] for q < jq (block reduction)
 template <int NT> GS_DEV void vt_times(const double* V, int n, const double* x, int r0, int r1, int jq, double* out,
                                        double* red) {
-  double acc[LNB];
-#pragma unroll
-  for (int q = 0; q < LNB; ++q) acc[q] = 0.0;
-  for (int r = r0 + (int)threadIdx.x; r < r1; r += NT) {
-    const double xv = x[r];
-#pragma unroll
-    for (int q = 0; q < LNB; ++q)
-      if (q < jq) acc[q] += V[r + (int64_t)q * n] * xv;
-  }
+  // sixteen columns of V per pass (the accumulators stay in registers)
+  constexpr int QB = 16;
   const int w = threadIdx.x >> 6;
+  for (int qb = 0; qb < jq; qb += QB) {
+    double acc[QB];
 #pragma unroll
-  for (int q = 0; q < LNB; ++q) {
-    if (q < jq) {
-      const double s = wave_sum(acc[q]);
-      if ((threadIdx.x & 63) == 0) red[w * LNB + q] = s;
+    for (int q = 0; q < QB; ++q) acc[q] = 0.0;
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += NT) {
+      const double xv = x[r];
+#pragma unroll
+      for (int q = 0; q < QB; ++q)
+        if (qb + q < jq) acc[q] += V[r + (int64_t)(qb + q) * n] * xv;
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      if (qb + q < jq) {
+        const double s = wave_sum(acc[q]);
+        if ((threadIdx.x & 63) == 0) red[w * LNB + qb + q] = s;
+      }
     }
   }
   __syncthreads();
@@ -149,9 +153,8 @@ template <int NT> __global__ void __launch_bounds__(NT) lpanel_col_kernel(PanelA
     __syncthreads();
     for (int r = i + 1 + tid; r < n; r += NT) {
       double s = col[r];
-#pragma unroll
-      for (int q = 0; q < LNB; ++q)
-        if (q < j) s -= p.Y[r + (int64_t)q * n] * vc[q];
+#pragma unroll 8
+      for (int q = 0; q < j; ++q) s -= p.Y[r + (int64_t)q * n] * vc[q];
       col[r] = s;
     }
     __syncthreads();
@@ -164,9 +167,8 @@ template <int NT> __global__ void __launch_bounds__(NT) lpanel_col_kernel(PanelA
     __syncthreads();
     for (int r = i + 1 + tid; r < n; r += NT) {
       double s = col[r];
-#pragma unroll
-      for (int q = 0; q < LNB; ++q)
-        if (q < j) s -= p.V[r + (int64_t)q * n] * w2[q];
+#pragma unroll 8
+      for (int q = 0; q < j; ++q) s -= p.V[r + (int64_t)q * n] * w2[q];
       col[r] = s;
     }
     __syncthreads();

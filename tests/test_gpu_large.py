@@ -33,7 +33,8 @@ def _check(oracle, A, S):
 
 
 @pytest.mark.parametrize("ta,tb", [(False, False), (False, True), (True, False), (True, True)])
-@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (130, 70, 33), (257, 129, 64), (64, 300, 5), (200, 3, 129), (150, 140, 100)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (130, 70, 33), (257, 129, 64), (64, 300, 5), (200, 3, 129), (150, 140, 100),
+                                   (300, 60, 100), (40, 500, 70)])
 def test_dgemm(gs, ta, tb, m, n, k):
     """The FP64 MFMA GEMM (operand and accumulator lane layouts, edges, transposes)."""
     import torch
