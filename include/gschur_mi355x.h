@@ -98,7 +98,12 @@ int gs_set_stream(gs_handle* h, void* stream);
  * :194-335.  A is n x n (lda >= n) and is overwritten (destroyed), as in the
  * reference.  T (ldt), Z (ldz, referenced with GS_WANTZ) receive the factors,
  * W (2n doubles) the eigenvalues diag(T).  maxiter <= 0 selects 100 n,
- * maxinner <= 0 selects 30 n. */
+ * maxinner <= 0 selects 30 n.
+ * Orders up to gs_batched_max_n(1) run the LDS-resident kernels; larger ones
+ * (unless GS_PLAIN) a global-memory Hessenberg reduction and Q formation
+ * followed by the windowed QR kernel (H and Z in HBM, the chase in an LDS
+ * window, more chain wavefronts for larger n).  GS_PLAIN runs the unblocked
+ * single-workgroup kernel. */
 int gs_schur_z(gs_handle* h, int64_t n, double* A, int64_t lda, double* T, int64_t ldt, double* Z, int64_t ldz,
                double* W, int flags, int64_t maxiter, int64_t maxinner, int32_t* info);
 
@@ -156,7 +161,8 @@ int gs_schur_batched_z_noalloc(gs_handle* h, int64_t count, const int32_t* n, co
  * util.jl:128-461; real right/left: vectors.jl:133-362 / :462-698 with
  * _xsolve/_safe_lu_solve! util.jl:603-751).  Z may be NULL (vectors of T).
  * V is n x n complex (ldv).  normalize = 0 returns the max-abs1-scaled
- * vectors before _enormalize!. */
+ * vectors before _enormalize!.  Orders up to 1024 (GS_E_UNSUPPORTED above):
+ * T in LDS up to 128 (complex) / 192 (real), read from global memory above. */
 int gs_trevc_z(gs_handle* h, int64_t n, const double* T, int64_t ldt, const double* Z, int64_t ldz, double* V,
                int64_t ldv, int left, int normalize, int flags);
 int gs_trevc_d(gs_handle* h, int64_t n, const double* T, int64_t ldt, const double* Z, int64_t ldz, double* V,

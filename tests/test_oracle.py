@@ -532,6 +532,15 @@ def test_eigen_composition_matches_lapack(oracle):
                     assert np.all(np.abs(p - 1) < 1e-7)
                 assert np.all(np.abs(np.linalg.norm(V, axis=0) - 1) < 1e-13)
                 assert np.all((rc > 0) & (rc <= 1 + 1e-12))
+                # rconde (pirates.jl:112-120) is |y^H x| / (|y| |x|) of the
+                # matrix the eigenvectors belong to, the balanced one with
+                # balancing (as xGEEVX's RCONDE): the closed form from
+                # LAPACK's eigenvectors of that matrix
+                Ab = oracle.balance(A, scale=bal, permute=bal)[0] if bal else A
+                eb, lb, rb = sla.eig(Ab, left=True, right=True)
+                pb, _ = checks.match_eigenvalues(eb, w)
+                ref_rc = np.abs(np.sum(lb.conj() * rb, axis=0)) / np.linalg.norm(lb, axis=0) / np.linalg.norm(rb, axis=0)
+                assert np.allclose(rc, ref_rc[pb], rtol=1e-8, atol=1e-12)
 
 
 def test_balance_nan_error(oracle):
