@@ -66,6 +66,17 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes}), flush=True)
 
 
+def _tuning_note():
+    """Which launch parameters the batched calls ran with."""
+    from genericschur import tuning
+    t = tuning.load()
+    if not t:
+        return "built-in defaults"
+    return (f"autotuned table {os.path.relpath(os.environ.get('GSCHUR_TUNING') or tuning.DEFAULT_PATH, ROOT)} "
+            f"({t.get('date', '?')}; complex {json.dumps(t.get('complex', {}), sort_keys=True)}; "
+            f"real {json.dumps(t.get('real', {}), sort_keys=True)})")
+
+
 def load_traffic():
     """HBM bytes per size bin's QR phase (QR-sweep launch + Z-log launch) from
     the committed rocprofv3 summary, if any."""
@@ -178,6 +189,9 @@ def main():
     qr = [r for r in recs if r["kind"] == "qr"]
     zl = [r for r in recs if r["kind"] == "zlog"]
     hq = [r for r in recs if r["kind"] == "hq"]
+    # one QR record over the whole batch: the work-queue launch of the tuned
+    # configuration; else one per size bin
+    queue = len(qr) == 1 and int(qr[0]["count"]) == len(ns) and len(hq) > 1
     qr_flops = (100.0 - 56.0 / 3.0) * sum(r["sum_n3"] for r in qr)
     qr_us = sum(r["us"] for r in qr) + sum(r["us"] for r in zl)
     achieved = qr_flops / (qr_us * 1e-6) if qr_us > 0 else 0.0
@@ -219,6 +233,7 @@ def main():
             "seed": args.seed,
             "wantZ": True,
             "parallelism": "single GPU, one workgroup per matrix",
+            "tuning": _tuning_note(),
         },
         "roofline": {
             "bound": "mfma",
@@ -228,8 +243,12 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64,
             "traffic": traffic["bytes_per_launch"] if traffic else None,
-            "kernel": "QR phase of a size bin: gs::zqrw2_kernel<true> (two-wavefront windowed QR sweeps) "
-                      "then, for the bin above n = 80, gs::zlog_kernel (the sweeps' Z log applied with Z in LDS)",
+            "kernel": ("QR phase of the batch: gs::zqrw2_queue_kernel<true> (two-wavefront windowed QR sweeps, "
+                       "one work-queue launch over all size bins, largest matrix first) then gs::zlog_kernel "
+                       "(the Z log of the matrices above n = 80 applied with Z in LDS, one launch per bin)"
+                       if queue else
+                       "QR phase of a size bin: gs::zqrw2_kernel<true> (two-wavefront windowed QR sweeps) "
+                       "then, for the bin above n = 80, gs::zlog_kernel (the sweeps' Z log applied with Z in LDS)"),
             "launches": len(qr),
             "avg_launch_us": qr_us / max(len(qr), 1),
             "qr_sweep_avg_launch_us": sum(r["us"] for r in qr) / max(len(qr), 1),
@@ -237,8 +256,9 @@ def main():
             "zlog_avg_launch_us": sum(r["us"] for r in zl) / max(len(zl), 1),
             "flops_per_launch": qr_flops / max(len(qr), 1),
             "note": "FP64 vector-ALU work (engine valu-fp64: the bulge chase has no dense contraction); the compute "
-                    "roof is the FP64 peak, 78.6 TFLOP/s for the vector ALUs and for MFMA alike. achieved = model flops of a bin's QR phase (81.3 n^3 per "
-                    "complex matrix: H and Z updates) / its average duration (HIP events around the QR-sweep "
+                    "roof is the FP64 peak, 78.6 TFLOP/s for the vector ALUs and for MFMA alike. achieved = model "
+                    "flops of the QR phase (81.3 n^3 per complex matrix: H and Z updates; of the whole batch for "
+                    "the work-queue launch, of a bin otherwise) / its duration (HIP events around the QR-sweep "
                     "and Z-log launches, instrumented pass with every launch alone)",
             "traffic_kind": "calibrated (committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, not measured in this "
                             "run)" if traffic else None,

@@ -35,6 +35,7 @@ struct Bin {
   int nt_hq;  // threads per workgroup of the Hessenberg+Z launch
   int nt_qr;  // threads per workgroup of the QR launch
   int branch; // graph branch (stream) the bin runs on
+  int zlog;   // tuned Z log of the QR launch: 1 on, 0 off, -1 the default rule
   double work;
 };
 
@@ -52,7 +53,7 @@ struct Plan {
   // capacities, and their totals
   std::vector<int64_t> loff[2];
   // device metadata
-  DevBuf d_order, d_n, d_off, d_woff, d_stats, d_scl, d_loff[2], d_llen;
+  DevBuf d_order, d_n, d_off, d_woff, d_stats, d_scl, d_loff[2], d_llen, d_qctr;
   // captured graph for device-pointer execution
   hipGraphExec_t gexec = nullptr;
   const void* g_ptrs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -68,6 +69,7 @@ struct Plan {
     d_loff[0].release();
     d_loff[1].release();
     d_llen.release();
+    d_qctr.release();
   }
 };
 
@@ -84,6 +86,7 @@ struct PlanDev {
   double* scl = nullptr;
   const int64_t* loff[2] = {nullptr, nullptr};
   int64_t* llen = nullptr;
+  int* qctr = nullptr;  // work-queue counter of the persistent QR launch, or null
 };
 
 static PlanDev plan_dev(const Plan* p) {
@@ -99,6 +102,7 @@ static PlanDev plan_dev(const Plan* p) {
   d.loff[0] = static_cast<const int64_t*>(p->d_loff[0].p);
   d.loff[1] = static_cast<const int64_t*>(p->d_loff[1].p);
   d.llen = static_cast<int64_t*>(p->d_llen.p);
+  d.qctr = static_cast<int*>(p->d_qctr.p);
   return d;
 }
 
@@ -113,14 +117,49 @@ static int fail(gs_handle* h, int code, const char* msg) { return handle_fail(h,
 // Number of leading bins whose Hessenberg launches precede every other
 // launch of a batch: two for complex, one for real batches (GSCHUR_HQ_LEAD
 // overrides, 0..3).
-static int hq_lead(bool cplx) {
+static int hq_lead(const Tuning& t, bool cplx) {
   static const int v = [] {
     const char* e = std::getenv("GSCHUR_HQ_LEAD");
     if (!e) return -1;
     const int x = std::atoi(e);
     return x < 0 ? 0 : (x > 3 ? 3 : x);
   }();
-  return v >= 0 ? v : (cplx ? 2 : 1);
+  if (v >= 0) return v;
+  if (t.lead >= 0) return t.lead;
+  return cplx ? 2 : 1;
+}
+
+// Complex batches can run their QR iterations as one persistent work-queue
+// launch over all size bins (largest matrix first) after the bins'
+// Hessenberg launches, instead of one QR launch per bin on the graph's
+// branches (measured: 139.0k against 139.4k-139.8k matrices/s per bin on
+// the default workload, so the default is per bin).  The tuning table or
+// GSCHUR_QR_QUEUE=0/1 (A/B runs, first) choose.
+static bool qr_queue(const Tuning& t, bool cplx) {
+  static const int v = [] {
+    const char* e = std::getenv("GSCHUR_QR_QUEUE");
+    return e ? (e[0] == '1' ? 1 : 0) : -1;
+  }();
+  if (!cplx) return false;
+  if (v >= 0) return v == 1;
+  return t.queue == 1;
+}
+// workgroups of the persistent QR launch: eight two-wavefront workgroups
+// per CU by default (the occupancy of the per-bin launches)
+static int qr_queue_slots(const Tuning& t) {
+  static const int cu = [] {
+    int dev = 0, x = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&x, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && x > 0)
+      return x;
+    return 256;
+  }();
+  static const int env = [] {
+    const char* e = std::getenv("GSCHUR_QR_QUEUE_SLOTS");
+    return e ? std::max(1, std::min(16, std::atoi(e))) : 0;
+  }();
+  const int per = env ? env : (t.slots > 0 ? t.slots : 8);
+  return cu * per;
 }
 
 static int env_qr_band() {
@@ -225,6 +264,46 @@ int gs_set_stream(gs_handle* h, void* stream) {
 
 int gs_batched_max_n(int is_complex) { return is_complex ? zschur_max_lds_n() : dschur_max_lds_n(); }
 
+int gs_set_tuning(gs_handle* h, int is_complex, const int32_t* rows, int nrows) {
+  if (!h || nrows < 0 || (nrows > 0 && !rows)) return GS_E_ARG;
+  Tuning t;
+  for (int i = 0; i < nrows; ++i) {
+    const int32_t kind = rows[4 * i], a = rows[4 * i + 1], b = rows[4 * i + 2], v = rows[4 * i + 3];
+    if (kind == 0) {
+      if (a == 1 && v >= 4 && v <= 64) t.bin_w = v;
+      else if (a == 2 && v >= 0 && v <= 3) t.lead = v;
+      else if (a == 3 && (v == 0 || v == 1)) t.queue = v;
+      else if (a == 4 && v >= 1 && v <= 16) t.slots = v;
+      else return handle_fail(h, GS_E_ARG, "bad tuning entry");
+    } else if (kind == 1 || kind == 2) {
+      if (a < 1 || b < a) return handle_fail(h, GS_E_ARG, "bad tuning range");
+      if (kind == 1 && !(v == 0 || v == 64 || v == 256 || v == 512 || v == 1024))
+        return handle_fail(h, GS_E_ARG, "bad tuning workgroup size");
+      if (kind == 1 && v == 64 && b > 32) return handle_fail(h, GS_E_ARG, "64 threads only up to n = 32");
+      if (kind == 2 && !(v == -1 || v == 0 || v == 1)) return handle_fail(h, GS_E_ARG, "bad tuning Z-log entry");
+      Tuning::Row* r = nullptr;
+      for (Tuning::Row& x : t.rows)
+        if (x.nlo == a && x.nhi == b) r = &x;
+      if (!r) {
+        t.rows.push_back(Tuning::Row{a, b, 0, -1});
+        r = &t.rows.back();
+      }
+      if (kind == 1) r->nt_hq = v;
+      else r->zlog = v;
+    } else {
+      return handle_fail(h, GS_E_ARG, "bad tuning kind");
+    }
+  }
+  std::lock_guard<std::mutex> lk(h->mu);
+  (void)hipSetDevice(h->device);
+  // cached plans (and their graphs) were built with the old parameters
+  if (h->stream) HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->plans.clear();
+  h->last_plan = nullptr;
+  h->tune[is_complex ? 1 : 0] = std::move(t);
+  return GS_OK;
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------------ plans
@@ -244,6 +323,24 @@ static int default_nt_hq(bool cplx, int nmax) {
   if (force) return force;
   return (!cplx || nmax > 64) ? 512 : 256;
 }
+
+// The tuning table's row for a bin whose largest order is nmax, or null.
+static const Tuning::Row* tune_row(const Tuning& t, int nmax) {
+  for (const Tuning::Row& r : t.rows)
+    if (nmax >= r.nlo && nmax <= r.nhi) return &r;
+  return nullptr;
+}
+
+// Hessenberg + Q workgroup size of a bin (or a single matrix) of largest
+// order nmax: GSCHUR_HQ_NT, then the tuning table, then the default.  The
+// single-matrix path uses the same choice, so a matrix gives the same bits
+// alone and in a batch.
+static int bin_nt_hq(const Tuning& t, bool cplx, int nmax) {
+  int nt = default_nt_hq(cplx, nmax);
+  const Tuning::Row* r = tune_row(t, nmax);
+  if (r && r->nt_hq && !std::getenv("GSCHUR_HQ_NT")) nt = r->nt_hq;
+  return nt;
+}
 static int default_nt_qr(bool cplx, int nmax) {
   (void)cplx;
   (void)nmax;
@@ -254,13 +351,15 @@ static int default_nt_qr(bool cplx, int nmax) {
 // on n ~ U{16..128} real: 136.4k matrices/s with 16, 144.0k with 32; complex
 // n ~ U{8..96}: 16 and 24 within noise, 8, 12 and 32 slower).
 // GSCHUR_BIN_W overrides it (A/B measurements, 4 .. 64).
-static int bin_width(bool cplx) {
+static int bin_width(const Tuning& t, bool cplx) {
   static const int w = [] {
     const char* e = std::getenv("GSCHUR_BIN_W");
     const int v = e ? std::atoi(e) : 0;
     return (v >= 4 && v <= 64) ? v : 0;
   }();
-  return w ? w : (cplx ? 16 : 32);
+  if (w) return w;
+  if (t.bin_w) return t.bin_w;
+  return cplx ? 16 : 32;
 }
 
 static std::string plan_key(bool cplx, int64_t count, const int32_t* n, const int64_t* off) {
@@ -277,7 +376,8 @@ static std::string plan_key(bool cplx, int64_t count, const int32_t* n, const in
 // and the largest matrix first inside a bin (order[]), each bin's launch
 // shape and graph branch, and the eigenvalue offsets woff[].  Orders must
 // already be validated.
-static void plan_bins(bool cplx, int64_t count, const int32_t* n, std::vector<int32_t>& order, std::vector<Bin>& bins,
+static void plan_bins(const Tuning& tn, bool cplx, int64_t count, const int32_t* n, std::vector<int32_t>& order,
+                      std::vector<Bin>& bins,
                       std::vector<int64_t>& woff) {
   const int nmax_lds = cplx ? zschur_max_lds_n() : dschur_max_lds_n();
   order.clear();
@@ -288,7 +388,7 @@ static void plan_bins(bool cplx, int64_t count, const int32_t* n, std::vector<in
     woff[i] = w;
     w += n[i];
   }
-  const int bw = bin_width(cplx);
+  const int bw = bin_width(tn, cplx);
   const int nb = (nmax_lds + bw - 1) / bw + 1;
   std::vector<std::vector<int32_t>> byb(nb);
   for (int64_t i = 0; i < count; ++i) {
@@ -304,7 +404,9 @@ static void plan_bins(bool cplx, int64_t count, const int32_t* n, std::vector<in
     bin.count = (int)v.size();
     bin.nmax = std::max(1, n[v.front()]);
     bin.nmin = n[v.back()];
-    bin.nt_hq = default_nt_hq(cplx, bin.nmax);
+    bin.nt_hq = bin_nt_hq(tn, cplx, bin.nmax);
+    const Tuning::Row* r = tune_row(tn, bin.nmax);
+    bin.zlog = r ? r->zlog : -1;
     bin.nt_qr = default_nt_qr(cplx, bin.nmax);
     bin.work = 0;
     for (int32_t i : v) bin.work += (double)n[i] * n[i] * n[i];
@@ -350,7 +452,7 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
   p->count = count;
   p->n.assign(n, n + count);
   p->off.assign(off, off + count);
-  plan_bins(cplx, count, n, p->order, p->bins, p->woff);
+  plan_bins(h->tune[cplx ? 1 : 0], cplx, count, n, p->order, p->bins, p->woff);
   p->total_w = count ? p->woff[count - 1] + n[count - 1] : 0;
   p->total_elems = span;
   {
@@ -366,6 +468,7 @@ static int build_plan(gs_handle* h, bool cplx, int64_t count, const int32_t* n, 
     }
     HIPCHK(h, p->d_llen.ensure(sizeof(int64_t) * std::max<int64_t>(count, 1)));
   }
+  HIPCHK(h, p->d_qctr.ensure(256));
   HIPCHK(h, p->d_order.ensure(sizeof(int32_t) * std::max<int64_t>(count, 1)));
   HIPCHK(h, p->d_n.ensure(sizeof(int32_t) * std::max<int64_t>(count, 1)));
   HIPCHK(h, p->d_off.ensure(sizeof(int64_t) * std::max<int64_t>(count, 1)));
@@ -401,7 +504,9 @@ static bool uses_zlog(int kflags) { return (kflags & GS_WANTZ) && !(kflags & (GS
 static bool bin_zlog(bool cplx, const Bin& b, int kflags) {
   if (!uses_zlog(kflags)) return false;
   if (!cplx && b.nmax > kDlogMaxN) return false;
-  return b.nmax >= (cplx ? kZlogMinN : kDlogMinN) || (kflags & GS_ZLOG_TINY);
+  if (kflags & GS_ZLOG_TINY) return true;
+  if (b.zlog >= 0) return b.zlog == 1;
+  return b.nmax >= (cplx ? kZlogMinN : kDlogMinN);
 }
 
 // With timed set, every launch runs alone, in order on the handle's stream,
@@ -444,10 +549,12 @@ static hipError_t launch_plan(gs_handle* h, const PlanDev& pd, const double* A, 
   for (const Bin& b : bins) used[b.branch] = true;
   for (int i = 0; i < kBranches; ++i)
     if (used[i] && (e = hipStreamWaitEvent(h->branch[i], h->evfork, 0)) != hipSuccess) return e;
+  const Tuning& tn = h->tune[p->is_complex ? 1 : 0];
+  const bool queue = pd.qctr && !(flags & GS_QR_BAND) && qr_queue(tn, p->is_complex);
   // the leading (largest) bins' Hessenberg launches run first, alone: the
   // largest bin's chain (Hessenberg, QR, Z log) is the batch's longest, and
   // started later it would finish last with the GPU half idle
-  const int lead = (int)std::min<size_t>((size_t)hq_lead(p->is_complex), bins.size());
+  const int lead = (int)std::min<size_t>((size_t)hq_lead(tn, p->is_complex), bins.size());
   auto bstream = [&](const Bin& b) { return timed ? h->stream : h->branch[b.branch]; };
   auto launch_hq = [&](size_t bi) -> hipError_t {
     const Bin& b = bins[bi];
@@ -477,6 +584,7 @@ static hipError_t launch_plan(gs_handle* h, const PlanDev& pd, const double* A, 
       }
       if ((e = launch_hq(bi)) != hipSuccess) return e;
     }
+    if (queue) continue;
     a.first = b.first;
     a.count = b.count;
     // the Z log pays off where Z's rows are long (its replay holds all of Z
@@ -498,6 +606,42 @@ static hipError_t launch_plan(gs_handle* h, const PlanDev& pd, const double* A, 
     if ((e = hipEventRecord(h->evjoin[i], h->branch[i])) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(h->stream, h->evjoin[i], 0)) != hipSuccess) return e;
   }
+  if (queue) {
+    // every bin's Hessenberg launch is done: one persistent QR launch over
+    // the whole batch (order[] is largest bin first, largest matrix first
+    // within a bin), then the Z-log replays of the logging bins
+    int64_t tot = 0;
+    int zmin = 1 << 30;
+    for (const Bin& b : bins) {
+      tot += b.count;
+      if (zl && bin_zlog(p->is_complex, b, flags)) zmin = std::min(zmin, b.nmin);
+    }
+    a.first = 0;
+    a.count = (int)tot;
+    a.zlog = zmin < (1 << 30) ? zlog : nullptr;
+    a.zlog_min = (flags & GS_ZLOG_TINY) ? 0 : zmin;
+    a.qctr = pd.qctr;
+    if ((e = launch_fill_i32(pd.qctr, 1, h->stream)) != hipSuccess) return e;
+    const size_t qev = 6 * bins.size();
+    if (timed && (e = hipEventRecord(h->tev[qev], h->stream)) != hipSuccess) return e;
+    if ((e = launch_zqrw_queue(a, qr_queue_slots(tn), h->stream)) != hipSuccess) return e;
+    if (timed && (e = hipEventRecord(h->tev[qev + 1], h->stream)) != hipSuccess) return e;
+    a.qctr = nullptr;
+    // the queue logged every matrix of order >= zlog_min: replay the logs of
+    // exactly those bins (bins are ranges of orders)
+    const int zmin_used = a.zlog_min;
+    a.zlog_min = 0;
+    for (size_t bi = 0; bi < bins.size(); ++bi) {
+      const Bin& b = bins[bi];
+      if (!(a.zlog && b.nmin >= zmin_used)) continue;
+      a.first = b.first;
+      a.count = b.count;
+      a.zlog = zlog;
+      if (timed && (e = hipEventRecord(h->tev[6 * bi + 4], h->stream)) != hipSuccess) return e;
+      if ((e = launch_zlog(a, b.nmax, h->stream)) != hipSuccess) return e;
+      if (timed && (e = hipEventRecord(h->tev[6 * bi + 5], h->stream)) != hipSuccess) return e;
+    }
+  }
   return hipSuccess;
 }
 
@@ -505,7 +649,7 @@ static hipError_t launch_plan(gs_handle* h, const PlanDev& pd, const double* A, 
 // then the per-launch records for gs_last_launches.
 static int run_timed(gs_handle* h, Plan* p, const double* A, double* T, double* Z, double* W, int32_t* info,
                      int kflags, double* zlog) {
-  const size_t need = 6 * p->bins.size();
+  const size_t need = 6 * p->bins.size() + 2;
   while (h->tev.size() < need) {
     hipEvent_t ev = nullptr;
     HIPCHK(h, hipEventCreate(&ev));
@@ -517,20 +661,46 @@ static int run_timed(gs_handle* h, Plan* p, const double* A, double* T, double* 
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->launch_recs.clear();
-  for (size_t bi = 0; bi < p->bins.size(); ++bi) {
+  // the work-queue form: one QR record for the whole batch, and the Z-log
+  // replays of the bins at or above the smallest logging order
+  const bool queue = !(kflags & GS_QR_BAND) && qr_queue(h->tune[p->is_complex ? 1 : 0], p->is_complex);
+  int zmin = 1 << 30;
+  if (queue) {
+    for (const Bin& b : p->bins)
+      if (zlog && bin_zlog(p->is_complex, b, kflags)) zmin = std::min(zmin, b.nmin);
+    if (kflags & GS_ZLOG_TINY) zmin = zmin < (1 << 30) ? 0 : zmin;
+  }
+  auto put = [&](int kind, int nmin, int nmax, int64_t cnt, double work, hipEvent_t e0, hipEvent_t e1) -> int {
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+    const double rec[GS_LAUNCH_REC] = {(double)kind, (double)nmin, (double)nmax, (double)cnt, work,
+                                       1000.0 * (double)ms};
+    h->launch_recs.insert(h->launch_recs.end(), rec, rec + GS_LAUNCH_REC);
+    return GS_OK;
+  };
+  int rc = GS_OK;
+  for (size_t bi = 0; bi < p->bins.size() && rc == GS_OK; ++bi) {
     const Bin& b = p->bins[bi];
     // kinds 0 (Hessenberg + Q), 1 (QR sweeps), 4 (Z log applied)
-    const int kinds[3] = {0, 1, 4};
-    for (int j = 0; j < 3; ++j) {
-      if (j == 2 && !(zlog && bin_zlog(p->is_complex, b, kflags))) continue;
-      float ms = 0.f;
-      HIPCHK(h, hipEventElapsedTime(&ms, h->tev[6 * bi + 2 * j], h->tev[6 * bi + 2 * j + 1]));
-      const double rec[GS_LAUNCH_REC] = {(double)kinds[j], (double)b.nmin, (double)b.nmax, (double)b.count, b.work,
-                                         1000.0 * (double)ms};
-      h->launch_recs.insert(h->launch_recs.end(), rec, rec + GS_LAUNCH_REC);
-    }
+    rc = put(0, b.nmin, b.nmax, b.count, b.work, h->tev[6 * bi], h->tev[6 * bi + 1]);
+    if (rc == GS_OK && !queue) rc = put(1, b.nmin, b.nmax, b.count, b.work, h->tev[6 * bi + 2], h->tev[6 * bi + 3]);
+    const bool zb = queue ? (zlog && b.nmin >= zmin) : (zlog && bin_zlog(p->is_complex, b, kflags));
+    if (rc == GS_OK && zb) rc = put(4, b.nmin, b.nmax, b.count, b.work, h->tev[6 * bi + 4], h->tev[6 * bi + 5]);
   }
-  return GS_OK;
+  if (rc == GS_OK && queue && !p->bins.empty()) {
+    int nmin = 1 << 30, nmax = 0;
+    int64_t cnt = 0;
+    double work = 0;
+    for (const Bin& b : p->bins) {
+      nmin = std::min(nmin, b.nmin);
+      nmax = std::max(nmax, b.nmax);
+      cnt += b.count;
+      work += b.work;
+    }
+    const size_t qev = 6 * p->bins.size();
+    rc = put(1, nmin, nmax, cnt, work, h->tev[qev], h->tev[qev + 1]);
+  }
+  return rc;
 }
 
 static int run_batched(gs_handle* h, bool cplx, int64_t count, const int32_t* n, const int64_t* off,
@@ -671,7 +841,7 @@ extern "C" int gs_schur_batched_z_noalloc(gs_handle* h, int64_t count, const int
   std::vector<int32_t> order;
   std::vector<Bin> bins;
   std::vector<int64_t> woff;
-  plan_bins(true, count, n, order, bins, woff);
+  plan_bins(h->tune[1], true, count, n, order, bins, woff);
   // scratch layout: order, n, off, woff (uploaded), stats, scl (written by
   // the kernels before they are read), each segment 256-byte aligned
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -893,7 +1063,7 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
     a.zlog = nullptr;  // one matrix: Z updated in place
     a.loff = nullptr;
     a.llen = nullptr;
-    const int nthq = default_nt_hq(cplx, (int)n), ntqr = default_nt_qr(cplx, (int)n);
+    const int nthq = bin_nt_hq(h->tune[cplx ? 1 : 0], cplx, (int)n), ntqr = default_nt_qr(cplx, (int)n);
     le = cplx ? launch_zhq(a, (int)n, nthq, h->stream) : launch_dhq(a, (int)n, nthq, h->stream);
     if (le == hipSuccess) {
       if (kflags & GS_QR_BAND)

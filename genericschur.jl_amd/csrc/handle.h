@@ -41,6 +41,21 @@ struct Plan;
 constexpr int kBranches = 4;
 constexpr int kAuxBufs = 13;
 
+// Launch parameters of batched calls set by gs_set_tuning (a table written
+// by the autotuner); zero or -1 fields keep the built-in defaults.
+struct Tuning {
+  int bin_w = 0;   // size-bin width
+  int lead = -1;   // leading bins whose Hessenberg launches run first, alone
+  int queue = -1;  // complex: one work-queue QR launch over all bins (1) or one per bin (0)
+  int slots = 0;   // workgroups per CU of the work-queue launch
+  struct Row {
+    int nlo, nhi;  // bins whose largest order lies in [nlo, nhi]
+    int nt_hq;     // Hessenberg + Q workgroup size (0: default)
+    int zlog;      // Z log of the QR launch: 1 on, 0 off, -1 default
+  };
+  std::vector<Row> rows;
+};
+
 }  // namespace gs
 
 struct gs_handle {
@@ -56,6 +71,7 @@ struct gs_handle {
   gs::DevBuf bLog;  // Z log of the batched complex QR launches
   gs::DevBuf aux[gs::kAuxBufs];  // scratch of the eigenvector / VJP entry points
   std::map<std::string, std::unique_ptr<gs::Plan>> plans;
+  gs::Tuning tune[2];  // [0] real, [1] complex batches
   gs::Plan* last_plan = nullptr;
   bool last_stats_from_plan = false;
   // GS_TIMING: events around each launch (two per launch) and the records

@@ -35,6 +35,11 @@ struct BatchArgs {
   double* zlog;
   const int64_t* loff;
   int64_t* llen;
+  // with zlog set, only matrices of order >= zlog_min log their Z updates
+  int zlog_min = 0;
+  // work queue of the persistent QR launch: matrices order[first + q] are
+  // taken in turn, q from an atomic counter (zero before the launch)
+  int* qctr = nullptr;
 };
 
 // Arguments of the single-matrix global-memory kernels (any n).
@@ -69,6 +74,11 @@ hipError_t launch_zhq(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 hipError_t launch_zqr(const BatchArgs& a, int nmax, int nt, hipStream_t s);
 // windowed QR launch: one wavefront per matrix, H and Z in HBM
 hipError_t launch_zqrw(const BatchArgs& a, hipStream_t s);
+// The persistent form of launch_zqrw: `slots` workgroups take the matrices
+// order[first .. first + count) from the work queue a.qctr, in order.
+hipError_t launch_zqrw_queue(const BatchArgs& a, int slots, hipStream_t s);
+// Zeroes n ints (a kernel, so that it can sit in a captured graph).
+hipError_t launch_fill_i32(int* p, int n, hipStream_t s);
 hipError_t launch_zlog(const BatchArgs& a, int nmax, hipStream_t s);
 // Z-log capacity of one matrix of order n, in complex entries (tiny: one
 // sweep's record, the least the kernel accepts)

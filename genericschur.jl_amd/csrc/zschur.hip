@@ -22,6 +22,7 @@
 // reflectors) redundantly on values read from H; searches are parallel over
 // threads followed by a butterfly maximum, so all threads take the same
 // branches.
+#include <algorithm>
 #include <cstdlib>
 
 #include "dev_common.h"
@@ -1183,16 +1184,10 @@ template <int NT, bool FAST> __global__ void __launch_bounds__(NT) zqr_kernel(Ba
 
 // Two-wavefront windowed QR kernel: the chase of chunk c overlaps the chains
 // of chunk c-1 (z_sweep_win2); the control logic runs on both wavefronts.
-template <bool FAST, int NT = 128>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) zqrw2_kernel(BatchArgs a) {
-  __shared__ zd win[ZWL * ZW + 4 * ZKC + ZWL + 1];
-  __shared__ int redi[NT / 64 > 4 ? NT / 64 : 4];
-  const int b = blockIdx.x;
-  if (b >= a.count) return;
-  // the chasing wavefront (0) issues ahead of the chain wavefronts sharing
-  // its SIMD (flag 0x4000 turns this off for comparisons)
-  if (!(a.flags & 0x4000) && threadIdx.x < 64) __builtin_amdgcn_s_setprio(2);
-  const int m = a.order[a.first + b];
+// QR iterations of matrix m of a batch (the body of zqrw2_kernel and of its
+// work-queue form); every thread of the workgroup calls it.
+template <bool FAST, int NT>
+GS_DEV void zqrw2_matrix(const BatchArgs& a, int m, zd* win, int* redi) {
   const int n = a.n[m];
   if (a.zlog && threadIdx.x == 0) a.llen[m] = 0;
   if (n <= 0) return;
@@ -1207,7 +1202,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) zq
   int64_t maxinner = a.maxinner > 0 ? a.maxinner : 30 * (int64_t)n;
   int64_t nsweep = 0, nstep = 0;
   ZLog zl{nullptr, 0, 0};
-  if (a.zlog && wantz) {
+  if (a.zlog && wantz && n >= a.zlog_min) {
     zl.g = reinterpret_cast<zd*>(a.zlog) + a.loff[m];
     zl.cap = a.loff[m + 1] - a.loff[m];
   }
@@ -1224,6 +1219,40 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) zq
   }
   if (st != 0) return;
   z_finish<NT>(T, lay, n, T, (int64_t)n, W, a.scl[2 * m], a.scl[2 * m + 1]);
+}
+
+template <bool FAST, int NT = 128>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) zqrw2_kernel(BatchArgs a) {
+  __shared__ zd win[ZWL * ZW + 4 * ZKC + ZWL + 1];
+  __shared__ int redi[NT / 64 > 4 ? NT / 64 : 4];
+  const int b = blockIdx.x;
+  if (b >= a.count) return;
+  // the chasing wavefront (0) issues ahead of the chain wavefronts sharing
+  // its SIMD (flag 0x4000 turns this off for comparisons)
+  if (!(a.flags & 0x4000) && threadIdx.x < 64) __builtin_amdgcn_s_setprio(2);
+  zqrw2_matrix<FAST, NT>(a, a.order[a.first + b], win, redi);
+}
+
+// Work-queue form: a fixed number of workgroups, each taking the next
+// matrix of order[first .. first + count) from the counter a.qctr until the
+// queue is empty (every workgroup exits once the counter passes count).  The
+// order is largest first, so the long matrices start first and the short
+// ones fill the slots the long ones leave, across size bins.
+template <bool FAST, int NT = 128>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) zqrw2_queue_kernel(BatchArgs a) {
+  __shared__ zd win[ZWL * ZW + 4 * ZKC + ZWL + 1];
+  __shared__ int redi[NT / 64 > 4 ? NT / 64 : 4];
+  __shared__ int slot;
+  if (!(a.flags & 0x4000) && threadIdx.x < 64) __builtin_amdgcn_s_setprio(2);
+  for (;;) {
+    if (threadIdx.x == 0) slot = atomicAdd(a.qctr, 1);
+    __syncthreads();
+    const int q = slot;
+    __syncthreads();
+    if (q >= a.count) break;
+    zqrw2_matrix<FAST, NT>(a, a.order[a.first + q], win, redi);
+    __syncthreads();
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -1407,6 +1436,27 @@ hipError_t launch_zqrw(const BatchArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((zqrw2_kernel<false>), dim3(a.count), dim3(128), 0, s, a);
   else
     hipLaunchKernelGGL((zqrw2_kernel<true>), dim3(a.count), dim3(128), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_zqrw_queue(const BatchArgs& a, int slots, hipStream_t s) {
+  if (a.count <= 0) return hipSuccess;
+  const int grid = std::max(1, std::min(slots, a.count));
+  if (a.flags & 0x1000)
+    hipLaunchKernelGGL((zqrw2_queue_kernel<false>), dim3(grid), dim3(128), 0, s, a);
+  else
+    hipLaunchKernelGGL((zqrw2_queue_kernel<true>), dim3(grid), dim3(128), 0, s, a);
+  return hipGetLastError();
+}
+
+__global__ void fill_i32_kernel(int* p, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+hipError_t launch_fill_i32(int* p, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
   return hipGetLastError();
 }
 

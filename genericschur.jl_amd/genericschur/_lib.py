@@ -36,6 +36,7 @@ EXPORTS = (
     "gs_trevc_z", "gs_trevc_d", "gs_trevc_batched_z", "gs_trevc_batched_d",
     "gs_eig_vjp_batched_d", "gs_last_stats", "gs_last_launches", "gs_dgemm", "gs_schur_batched_z_noalloc",
     "gs_balance_z", "gs_balance_d", "gs_balance_apply", "gs_triangularize_d", "gs_enormalize", "gs_eig_rcond",
+    "gs_set_tuning",
 )
 
 _lib = None
@@ -111,6 +112,7 @@ def lib():
         L.gs_schur_batched_d.argtypes = sig_b
         L.gs_schur_batched_z_noalloc.argtypes = sig_b
         L.gs_batched_max_n.argtypes = [c_int]
+        L.gs_set_tuning.argtypes = [c_vp, c_int, c_vp, c_int]
         sig_tv = [c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_int, c_int, c_int]
         L.gs_trevc_z.argtypes = sig_tv
         L.gs_trevc_d.argtypes = sig_tv
@@ -147,6 +149,10 @@ def handle(device=0):
                 raise GSError(f"gs_create(device={device}) failed with status {rc}: no usable GPU")
             h = hp
             _handles[device] = h
+            # the committed launch-parameter table of the autotuner (GSCHUR_TUNING
+            # names another table, or "none" for the built-in defaults)
+            from . import tuning
+            tuning.apply(h, tuning.load())
     return h
 
 

@@ -135,6 +135,23 @@ int gs_schur_batched_d(gs_handle* h, int64_t count, const int32_t* n, const int6
                        double* T, double* Z, double* W, int32_t* info, int flags);
 int gs_batched_max_n(int is_complex);
 
+/* Launch parameters of this handle's batched calls for one element type
+ * (no counterpart in the reference), as written by the autotuner
+ * (genericschur/autotune.py).  rows holds nrows entries of four int32
+ * {kind, a, b, value}:
+ *   kind 0: a = 1 size-bin width (4..64), 2 leading Hessenberg launches
+ *           (0..3), 3 one work-queue QR launch over all bins (complex; 0/1),
+ *           4 workgroups per CU of that launch (1..16);
+ *   kind 1: Hessenberg + Q workgroup size (64 up to n = 32, 256, 512 or
+ *           1024; 0 = default) of bins whose largest order lies in [a, b];
+ *   kind 2: Z log of those bins' QR launches (1 on, 0 off, -1 default).
+ * nrows = 0 restores the defaults.  None of these changes the results of a
+ * matrix between its batched and single-matrix calls on one handle (the
+ * single-matrix LDS path takes the same Hessenberg workgroup size); a
+ * workgroup size does change the rounding of the Hessenberg reduction.
+ * Cached plans and graphs are dropped. */
+int gs_set_tuning(gs_handle* h, int is_complex, const int32_t* rows, int nrows);
+
 /* Batched ComplexF64 Schur without device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

@@ -154,3 +154,19 @@ def test_single_hip_runtime_with_torch(gs):
             if "libamdhip64" in path:
                 hips.add(os.path.realpath(path))
     assert len(hips) == 1, hips
+
+
+def test_tuning_rows_format():
+    """The tuning table's conversion into gs_set_tuning rows {kind, a, b,
+    value}, and the committed table's bins covering disjoint order ranges."""
+    from genericschur import tuning
+    r = tuning.rows({"bin_w": 24, "queue": 1, "bins": [{"nlo": 25, "nhi": 48, "nt_hq": 1024, "zlog": 0},
+                                                       {"nlo": 49, "nhi": 72, "zlog": -1}]})
+    assert r.tolist() == [[0, 1, 0, 24], [0, 3, 0, 1], [1, 25, 48, 1024], [2, 25, 48, 0]]
+    table = tuning.load(tuning.DEFAULT_PATH)
+    if table is None:
+        return
+    for key in ("complex", "real"):
+        bins = sorted((b["nlo"], b["nhi"]) for b in (table.get(key) or {}).get("bins", []))
+        for (a0, b0), (a1, b1) in zip(bins, bins[1:]):
+            assert a0 <= b0 < a1 <= b1
