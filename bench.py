@@ -189,9 +189,9 @@ def main():
     qr = [r for r in recs if r["kind"] == "qr"]
     zl = [r for r in recs if r["kind"] == "zlog"]
     hq = [r for r in recs if r["kind"] == "hq"]
-    # one QR record over the whole batch: the work-queue launch of the tuned
-    # configuration; else one per size bin
-    queue = len(qr) == 1 and int(qr[0]["count"]) == len(ns) and len(hq) > 1
+    # fewer QR records than size bins: the work-queue launch of the tuned
+    # configuration (one for the batch); else one per size bin
+    queue = 0 < len(qr) < len(hq)
     qr_flops = (100.0 - 56.0 / 3.0) * sum(r["sum_n3"] for r in qr)
     qr_us = sum(r["us"] for r in qr) + sum(r["us"] for r in zl)
     achieved = qr_flops / (qr_us * 1e-6) if qr_us > 0 else 0.0
@@ -245,7 +245,7 @@ def main():
             "traffic": traffic["bytes_per_launch"] if traffic else None,
             "kernel": ("QR phase of the batch: gs::zqrw2_queue_kernel<true> (two-wavefront windowed QR sweeps, "
                        "one work-queue launch over all size bins, largest matrix first) then gs::zlog_kernel "
-                       "(the Z log of the matrices above n = 80 applied with Z in LDS, one launch per bin)"
+                       "(the Z log of the largest matrices applied with Z in LDS, one launch per logging bin)"
                        if queue else
                        "QR phase of a size bin: gs::zqrw2_kernel<true> (two-wavefront windowed QR sweeps) "
                        "then, for the bin above n = 80, gs::zlog_kernel (the sweeps' Z log applied with Z in LDS)"),

@@ -509,6 +509,93 @@ static bool bin_zlog(bool cplx, const Bin& b, int kflags) {
   return b.nmax >= (cplx ? kZlogMinN : kDlogMinN);
 }
 
+// Work-queue form of a complex batch (tuning queue = 1): the bins'
+// Hessenberg launches (the leading bins' first, alone, on branch 0, the
+// others on branches 1 .. 3), then one persistent QR launch over the whole
+// batch, largest matrix first, and the Z-log replays.  Every matrix of order
+// >= zmin logs its Z updates, zmin the smallest order of the logging bins,
+// so that the replays of the bins at or above zmin cover exactly the
+// logging matrices.  (Measured: a second QR launch for the leading bins,
+// started beside the other bins' Hessenberg launches, ran slower, 57.4
+// against 56.4 ms per batch: the smaller matrices' workgroups then took
+// slots before the largest matrices had started.)
+static int queue_zmin(const std::vector<Bin>& bins, bool zl, bool cplx, int flags) {
+  int zmin = 1 << 30;
+  for (const Bin& b : bins)
+    if (zl && bin_zlog(cplx, b, flags)) zmin = std::min(zmin, b.nmin);
+  if ((flags & GS_ZLOG_TINY) && zmin < (1 << 30)) zmin = 0;
+  return zmin;
+}
+// events of the QR launch in the timed pass (after the bins' six each)
+static size_t queue_event(size_t nbins) { return 6 * nbins; }
+
+static hipError_t launch_plan_queue(gs_handle* h, const PlanDev& pd, BatchArgs a, int flags, double* zlog, bool timed,
+                                    const Tuning& tn) {
+  const std::vector<Bin>& bins = *pd.bins;
+  const int nb = (int)bins.size();
+  const int lead = std::min(hq_lead(tn, true), nb);
+  const int zmin = queue_zmin(bins, zlog != nullptr, true, flags);
+  hipError_t e;
+  auto st = [&](int i) { return timed ? h->stream : h->branch[i]; };
+  if (!timed) {
+    if ((e = hipEventRecord(h->evfork, h->stream)) != hipSuccess) return e;
+    for (int i = 0; i < kBranches; ++i)
+      if ((e = hipStreamWaitEvent(h->branch[i], h->evfork, 0)) != hipSuccess) return e;
+  }
+  auto hq = [&](int bi, hipStream_t s) -> hipError_t {
+    const Bin& b = bins[bi];
+    a.first = b.first;
+    a.count = b.count;
+    hipError_t e2;
+    if (timed && (e2 = hipEventRecord(h->tev[6 * bi], s)) != hipSuccess) return e2;
+    if ((e2 = launch_zhq(a, b.nmax, b.nt_hq, s)) != hipSuccess) return e2;
+    if (timed && (e2 = hipEventRecord(h->tev[6 * bi + 1], s)) != hipSuccess) return e2;
+    return hipSuccess;
+  };
+  for (int bi = 0; bi < lead; ++bi)
+    if ((e = hq(bi, st(0))) != hipSuccess) return e;
+  if (!timed && lead > 0 && (e = hipEventRecord(h->evlead[0], st(0))) != hipSuccess) return e;
+  bool usedb[kBranches] = {true, false, false, false};
+  for (int bi = lead; bi < nb; ++bi) {
+    const int br = 1 + (bi - lead) % (kBranches - 1);
+    if (!timed && !usedb[br] && lead > 0 && (e = hipStreamWaitEvent(st(br), h->evlead[0], 0)) != hipSuccess)
+      return e;
+    usedb[br] = true;
+    if ((e = hq(bi, st(br))) != hipSuccess) return e;
+  }
+  // every Hessenberg launch done: the QR launch and the replays on h->stream
+  if (!timed)
+    for (int i = 0; i < kBranches; ++i) {
+      if ((e = hipEventRecord(h->evjoin[i], h->branch[i])) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(h->stream, h->evjoin[i], 0)) != hipSuccess) return e;
+    }
+  int64_t tot = 0;
+  for (const Bin& b : bins) tot += b.count;
+  if (tot == 0) return hipSuccess;
+  a.first = 0;
+  a.count = (int)tot;
+  a.zlog = zmin < (1 << 30) ? zlog : nullptr;
+  a.zlog_min = zmin;
+  a.qctr = pd.qctr;
+  if ((e = launch_fill_i32(a.qctr, 1, h->stream)) != hipSuccess) return e;
+  if (timed && (e = hipEventRecord(h->tev[queue_event(nb)], h->stream)) != hipSuccess) return e;
+  if ((e = launch_zqrw_queue(a, qr_queue_slots(tn), h->stream)) != hipSuccess) return e;
+  if (timed && (e = hipEventRecord(h->tev[queue_event(nb) + 1], h->stream)) != hipSuccess) return e;
+  a.qctr = nullptr;
+  a.zlog_min = 0;
+  for (int bi = 0; bi < nb; ++bi) {
+    const Bin& b = bins[bi];
+    if (!(zmin < (1 << 30) && b.nmin >= zmin)) continue;
+    a.first = b.first;
+    a.count = b.count;
+    a.zlog = zlog;
+    if (timed && (e = hipEventRecord(h->tev[6 * bi + 4], h->stream)) != hipSuccess) return e;
+    if ((e = launch_zlog(a, b.nmax, h->stream)) != hipSuccess) return e;
+    if (timed && (e = hipEventRecord(h->tev[6 * bi + 5], h->stream)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 // With timed set, every launch runs alone, in order on the handle's stream,
 // between two events (eager launches only, never under capture): events
 // 6b .. 6b+5 for bin b (Hessenberg, QR, Z log), so each record is the
@@ -543,14 +630,15 @@ static hipError_t launch_plan(gs_handle* h, const PlanDev& pd, const double* A, 
   // fork: every bin runs its Hessenberg launch then its QR launch on one of
   // kBranches streams, so the LDS left free by a large bin's workgroups is
   // filled by the smaller bins' workgroups
+  const Tuning& tn = h->tune[p->is_complex ? 1 : 0];
+  if (pd.qctr && !(flags & GS_QR_BAND) && qr_queue(tn, p->is_complex))
+    return launch_plan_queue(h, pd, a, flags, zl ? zlog : nullptr, timed, tn);
   hipError_t e = hipEventRecord(h->evfork, h->stream);
   if (e != hipSuccess) return e;
   bool used[kBranches] = {false, false, false, false};
   for (const Bin& b : bins) used[b.branch] = true;
   for (int i = 0; i < kBranches; ++i)
     if (used[i] && (e = hipStreamWaitEvent(h->branch[i], h->evfork, 0)) != hipSuccess) return e;
-  const Tuning& tn = h->tune[p->is_complex ? 1 : 0];
-  const bool queue = pd.qctr && !(flags & GS_QR_BAND) && qr_queue(tn, p->is_complex);
   // the leading (largest) bins' Hessenberg launches run first, alone: the
   // largest bin's chain (Hessenberg, QR, Z log) is the batch's longest, and
   // started later it would finish last with the GPU half idle
@@ -584,7 +672,6 @@ static hipError_t launch_plan(gs_handle* h, const PlanDev& pd, const double* A, 
       }
       if ((e = launch_hq(bi)) != hipSuccess) return e;
     }
-    if (queue) continue;
     a.first = b.first;
     a.count = b.count;
     // the Z log pays off where Z's rows are long (its replay holds all of Z
@@ -605,42 +692,6 @@ static hipError_t launch_plan(gs_handle* h, const PlanDev& pd, const double* A, 
     if (!used[i]) continue;
     if ((e = hipEventRecord(h->evjoin[i], h->branch[i])) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(h->stream, h->evjoin[i], 0)) != hipSuccess) return e;
-  }
-  if (queue) {
-    // every bin's Hessenberg launch is done: one persistent QR launch over
-    // the whole batch (order[] is largest bin first, largest matrix first
-    // within a bin), then the Z-log replays of the logging bins
-    int64_t tot = 0;
-    int zmin = 1 << 30;
-    for (const Bin& b : bins) {
-      tot += b.count;
-      if (zl && bin_zlog(p->is_complex, b, flags)) zmin = std::min(zmin, b.nmin);
-    }
-    a.first = 0;
-    a.count = (int)tot;
-    a.zlog = zmin < (1 << 30) ? zlog : nullptr;
-    a.zlog_min = (flags & GS_ZLOG_TINY) ? 0 : zmin;
-    a.qctr = pd.qctr;
-    if ((e = launch_fill_i32(pd.qctr, 1, h->stream)) != hipSuccess) return e;
-    const size_t qev = 6 * bins.size();
-    if (timed && (e = hipEventRecord(h->tev[qev], h->stream)) != hipSuccess) return e;
-    if ((e = launch_zqrw_queue(a, qr_queue_slots(tn), h->stream)) != hipSuccess) return e;
-    if (timed && (e = hipEventRecord(h->tev[qev + 1], h->stream)) != hipSuccess) return e;
-    a.qctr = nullptr;
-    // the queue logged every matrix of order >= zlog_min: replay the logs of
-    // exactly those bins (bins are ranges of orders)
-    const int zmin_used = a.zlog_min;
-    a.zlog_min = 0;
-    for (size_t bi = 0; bi < bins.size(); ++bi) {
-      const Bin& b = bins[bi];
-      if (!(a.zlog && b.nmin >= zmin_used)) continue;
-      a.first = b.first;
-      a.count = b.count;
-      a.zlog = zlog;
-      if (timed && (e = hipEventRecord(h->tev[6 * bi + 4], h->stream)) != hipSuccess) return e;
-      if ((e = launch_zlog(a, b.nmax, h->stream)) != hipSuccess) return e;
-      if (timed && (e = hipEventRecord(h->tev[6 * bi + 5], h->stream)) != hipSuccess) return e;
-    }
   }
   return hipSuccess;
 }
@@ -663,13 +714,9 @@ static int run_timed(gs_handle* h, Plan* p, const double* A, double* T, double* 
   h->launch_recs.clear();
   // the work-queue form: one QR record for the whole batch, and the Z-log
   // replays of the bins at or above the smallest logging order
-  const bool queue = !(kflags & GS_QR_BAND) && qr_queue(h->tune[p->is_complex ? 1 : 0], p->is_complex);
-  int zmin = 1 << 30;
-  if (queue) {
-    for (const Bin& b : p->bins)
-      if (zlog && bin_zlog(p->is_complex, b, kflags)) zmin = std::min(zmin, b.nmin);
-    if (kflags & GS_ZLOG_TINY) zmin = zmin < (1 << 30) ? 0 : zmin;
-  }
+  const Tuning& tn = h->tune[p->is_complex ? 1 : 0];
+  const bool queue = !(kflags & GS_QR_BAND) && qr_queue(tn, p->is_complex);
+  const int zmin = queue ? queue_zmin(p->bins, zlog && uses_zlog(kflags), p->is_complex, kflags) : (1 << 30);
   auto put = [&](int kind, int nmin, int nmax, int64_t cnt, double work, hipEvent_t e0, hipEvent_t e1) -> int {
     float ms = 0.f;
     HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
@@ -679,25 +726,23 @@ static int run_timed(gs_handle* h, Plan* p, const double* A, double* T, double* 
     return GS_OK;
   };
   int rc = GS_OK;
+  int nmin = 1 << 30, nmax = 0;
+  int64_t cnt = 0;
+  double work = 0;
   for (size_t bi = 0; bi < p->bins.size() && rc == GS_OK; ++bi) {
     const Bin& b = p->bins[bi];
+    nmin = std::min(nmin, b.nmin);
+    nmax = std::max(nmax, b.nmax);
+    cnt += b.count;
+    work += b.work;
     // kinds 0 (Hessenberg + Q), 1 (QR sweeps), 4 (Z log applied)
     rc = put(0, b.nmin, b.nmax, b.count, b.work, h->tev[6 * bi], h->tev[6 * bi + 1]);
     if (rc == GS_OK && !queue) rc = put(1, b.nmin, b.nmax, b.count, b.work, h->tev[6 * bi + 2], h->tev[6 * bi + 3]);
-    const bool zb = queue ? (zlog && b.nmin >= zmin) : (zlog && bin_zlog(p->is_complex, b, kflags));
+    const bool zb = queue ? (zmin < (1 << 30) && b.nmin >= zmin) : (zlog && bin_zlog(p->is_complex, b, kflags));
     if (rc == GS_OK && zb) rc = put(4, b.nmin, b.nmax, b.count, b.work, h->tev[6 * bi + 4], h->tev[6 * bi + 5]);
   }
-  if (rc == GS_OK && queue && !p->bins.empty()) {
-    int nmin = 1 << 30, nmax = 0;
-    int64_t cnt = 0;
-    double work = 0;
-    for (const Bin& b : p->bins) {
-      nmin = std::min(nmin, b.nmin);
-      nmax = std::max(nmax, b.nmax);
-      cnt += b.count;
-      work += b.work;
-    }
-    const size_t qev = 6 * p->bins.size();
+  if (rc == GS_OK && queue && cnt > 0) {
+    const size_t qev = queue_event(p->bins.size());
     rc = put(1, nmin, nmax, cnt, work, h->tev[qev], h->tev[qev + 1]);
   }
   return rc;
