@@ -1,6 +1,7 @@
 // Host entry points of the eigen! pieces around the Schur factorization
 // (gs_balance_*, gs_balance_apply, gs_triangularize_d, gs_enormalize,
-// gs_eig_rcond), see include/gschur_mi355x.h.  Each is one matrix (or one
+// gs_eig_rcond) and of the reordering and conditioning of a complex Schur
+// form (gs_ordschur_z, gs_eigcond_z), see include/gschur_mi355x.h.  Each is one matrix (or one
 // set of columns) on the handle's stream; host pointers are staged through
 // the handle's buffers, device pointers (GS_DEVICE_PTRS) used directly.
 #include <hip/hip_runtime.h>
@@ -235,6 +236,77 @@ int gs_eig_rcond(gs_handle* h, int64_t n, int64_t m, const double* VL, int64_t l
   ECHK(launch_eig_rcond((int)n, (int)m, dL, a, dR, b, dC, s));
   if (!dev) ECHK(hipMemcpyAsync(rconde, dC, 8 * m, hipMemcpyDeviceToHost, s));
   if (!dev || !(flags & GS_ASYNC)) ECHK(hipStreamSynchronize(s));
+  return GS_OK;
+}
+
+int gs_ordschur_z(gs_handle* h, int64_t n, double* T, int64_t ldt, double* Z, int64_t ldz, const int32_t* select,
+                  double* W, int flags) {
+  if (!h) return GS_E_ARG;
+  const int64_t n1 = std::max<int64_t>(n, 1);
+  if (n < 0 || n > INT32_MAX || ldt < n1 || (Z && ldz < n1)) return handle_fail(h, GS_E_DIM, "bad dimension");
+  if (n == 0) return GS_OK;
+  if (!T || !select || !W) return handle_fail(h, GS_E_ARG, "null pointer");
+  Lock lk(h);
+  ECHK(hipSetDevice(handle_device(h)));
+  hipStream_t s = handle_stream(h);
+  const bool dev = (flags & GS_DEVICE_PTRS) != 0;
+  const size_t nn = (size_t)n * n;
+  hipError_t e;
+  // staging: select (and, with host pointers, T, Z, W)
+  char* base = static_cast<char*>(stage(h, 0, 4 * (size_t)n + 256 + (dev ? 0 : 16 * (2 * nn + (size_t)n)), e));
+  ECHK(e);
+  int32_t* dsel = reinterpret_cast<int32_t*>(base);
+  double* dT = T;
+  double* dZ = Z;
+  double* dW = W;
+  int64_t a = ldt, b = ldz;
+  ECHK(hipMemcpyAsync(dsel, select, 4 * (size_t)n, hipMemcpyHostToDevice, s));
+  if (!dev) {
+    double* sb = reinterpret_cast<double*>(base + ((4 * (size_t)n + 255) / 256) * 256);
+    dT = sb;
+    dZ = Z ? sb + 2 * nn : nullptr;
+    dW = sb + 4 * nn;
+    ECHK(hipMemcpy2DAsync(dT, 16 * n, T, 16 * ldt, 16 * n, n, hipMemcpyHostToDevice, s));
+    if (Z) ECHK(hipMemcpy2DAsync(dZ, 16 * n, Z, 16 * ldz, 16 * n, n, hipMemcpyHostToDevice, s));
+    a = b = n;
+  }
+  ECHK(launch_ordschur((int)n, dT, a, dZ, b, Z != nullptr, dsel, dW, s));
+  if (!dev) {
+    ECHK(hipMemcpy2DAsync(T, 16 * ldt, dT, 16 * n, 16 * n, n, hipMemcpyDeviceToHost, s));
+    if (Z) ECHK(hipMemcpy2DAsync(Z, 16 * ldz, dZ, 16 * n, 16 * n, n, hipMemcpyDeviceToHost, s));
+    ECHK(hipMemcpyAsync(W, dW, 16 * n, hipMemcpyDeviceToHost, s));
+  }
+  // select is read from the staging buffer until the launch is done
+  ECHK(hipStreamSynchronize(s));
+  return GS_OK;
+}
+
+int gs_eigcond_z(gs_handle* h, int64_t n, const double* T, int64_t ldt, int64_t nsub, double* out2, int flags) {
+  if (!h) return GS_E_ARG;
+  if (n < 2 || n > INT32_MAX || ldt < n) return handle_fail(h, GS_E_DIM, "bad dimension");
+  if (nsub < 1 || nsub >= n) return handle_fail(h, GS_E_ARG, "nsub must lie in 1 .. n-1");
+  if (!T || !out2) return handle_fail(h, GS_E_ARG, "null pointer");
+  Lock lk(h);
+  ECHK(hipSetDevice(handle_device(h)));
+  hipStream_t s = handle_stream(h);
+  const bool dev = (flags & GS_DEVICE_PTRS) != 0;
+  const size_t nn = (size_t)n * n, N = (size_t)nsub * (size_t)(n - nsub);
+  hipError_t e;
+  double* base = static_cast<double*>(stage(h, 0, 16 * (3 * N + (dev ? 0 : nn)) + 64, e));
+  ECHK(e);
+  double* work = base;
+  double* dout = base + 6 * N;
+  const double* dT = T;
+  int64_t a = ldt;
+  if (!dev) {
+    double* sT = dout + 8;
+    ECHK(hipMemcpy2DAsync(sT, 16 * n, T, 16 * ldt, 16 * n, n, hipMemcpyHostToDevice, s));
+    dT = sT;
+    a = n;
+  }
+  ECHK(launch_eigcond((int)n, dT, a, (int)nsub, work, dout, s));
+  ECHK(hipMemcpyAsync(out2, dout, 16, hipMemcpyDeviceToHost, s));
+  ECHK(hipStreamSynchronize(s));
   return GS_OK;
 }
 

@@ -167,6 +167,12 @@ hipError_t launch_balback(int n, int m, const int32_t* iit, const int32_t* sp, c
 hipError_t launch_triangularize(int n, const double* T, int64_t ldt, const double* Z, int64_t ldz, double* Tc,
                                 int64_t ldtc, double* Zc, int64_t ldzc, double* W, hipStream_t s);
 hipError_t launch_enormalize(int n, int m, double* V, int64_t ldv, hipStream_t s);
+// gordschur! on a complex Schur form (T, Z: ld ldt, ldz; select: n ints)
+// and eigvalscond / subspacesep (work: 3 nsub (n - nsub) complex entries,
+// out: s, sep), one wavefront each
+hipError_t launch_ordschur(int n, double* T, int64_t ldt, double* Z, int64_t ldz, bool wantz, const int32_t* select,
+                           double* W, hipStream_t s);
+hipError_t launch_eigcond(int n, const double* T, int64_t ldt, int nsub, double* work, double* out, hipStream_t s);
 hipError_t launch_eig_rcond(int n, int m, const double* VL, int64_t ldl, const double* VR, int64_t ldr, double* rc,
                             hipStream_t s);
 

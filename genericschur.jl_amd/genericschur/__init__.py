@@ -9,7 +9,8 @@ include/gschur_mi355x.h.
 from .api import (ArgumentError, DimensionMismatch, Schur, UnconvergedException, eigvals_, eigvecs, geigvecs,
                   gschur, gschur_, gschur_batched, gschur_batched_device, gschur_hessenberg_, pack, schur_,
                   synchronize, unpack, eig_offsets, set_stream, gschur_device, dgemm_device, batched_max_n,
-                  Balancer, Eigen, balance_, lmul_, ldiv_, triangularize, enormalize_, eigen_)
+                  Balancer, Eigen, balance_, lmul_, ldiv_, triangularize, enormalize_, eigen_, ordschur_, eigcond,
+                  eigvalscond, subspacesep)
 from ._lib import GSError, last_stats, last_launches
 
 __all__ = [
@@ -18,4 +19,5 @@ __all__ = [
     "gschur_batched", "gschur_batched_device", "pack", "unpack", "eig_offsets", "synchronize", "set_stream", "last_stats",
     "gschur_device", "dgemm_device", "batched_max_n", "last_launches",
     "Balancer", "Eigen", "balance_", "lmul_", "ldiv_", "triangularize", "enormalize_", "eigen_",
+    "ordschur_", "eigcond", "eigvalscond", "subspacesep",
 ]

@@ -36,7 +36,7 @@ EXPORTS = (
     "gs_trevc_z", "gs_trevc_d", "gs_trevc_batched_z", "gs_trevc_batched_d",
     "gs_eig_vjp_batched_d", "gs_last_stats", "gs_last_launches", "gs_dgemm", "gs_schur_batched_z_noalloc",
     "gs_balance_z", "gs_balance_d", "gs_balance_apply", "gs_triangularize_d", "gs_enormalize", "gs_eig_rcond",
-    "gs_set_tuning",
+    "gs_set_tuning", "gs_ordschur_z", "gs_eigcond_z",
 )
 
 _lib = None
@@ -113,6 +113,8 @@ def lib():
         L.gs_schur_batched_z_noalloc.argtypes = sig_b
         L.gs_batched_max_n.argtypes = [c_int]
         L.gs_set_tuning.argtypes = [c_vp, c_int, c_vp, c_int]
+        L.gs_ordschur_z.argtypes = [c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_int]
+        L.gs_eigcond_z.argtypes = [c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_int]
         sig_tv = [c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_int, c_int, c_int]
         L.gs_trevc_z.argtypes = sig_tv
         L.gs_trevc_d.argtypes = sig_tv

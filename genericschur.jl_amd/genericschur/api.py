@@ -474,6 +474,54 @@ def triangularize(S, device=0):
     return Schur(np.ascontiguousarray(Tc), np.ascontiguousarray(Zc), W)
 
 
+# ------------------------------------------- reordering and conditioning
+def ordschur_(S, select, device=0):
+    """gordschur!(S, select) for a complex Schur factorization
+    (ordschur.jl:23-73): the selected eigenvalues first, in order.  Returns a
+    new Schur (T exactly upper triangular, Z updated, values = diag(T))."""
+    T = np.asarray(S.T)
+    n = _checksquare(T)
+    if T.dtype != np.complex128:
+        raise TypeError("ordschur_ takes a complex Schur factorization (real reordering is not built)")
+    sel = np.ascontiguousarray(np.asarray(select, dtype=bool).astype(np.int32))
+    if sel.shape != (n,):
+        raise ValueError("select must have length n")
+    t = np.array(T, order="F", copy=True)
+    Z = np.asarray(S.Z)
+    z = None if Z.size == 0 else np.array(Z, dtype=np.complex128, order="F", copy=True)
+    W = np.zeros(n, dtype=np.complex128)
+    h = _lib.handle(device)
+    ld = max(n, 1)
+    rc = _lib.lib().gs_ordschur_z(h, n, _ptr(t), ld, None if z is None else _ptr(z), ld, _ptr(sel), _ptr(W), 0)
+    _lib.check(rc, h, "gs_ordschur_z")
+    return Schur(np.ascontiguousarray(t), np.ascontiguousarray(z) if z is not None else np.zeros((0, 0), complex), W)
+
+
+def eigcond(S, nsub, device=0):
+    """(eigvalscond(S, nsub), subspacesep(S, nsub)) of a complex Schur
+    factorization (ordschur.jl:85-131), 1 <= nsub <= n - 1."""
+    T = np.asarray(S.T)
+    n = _checksquare(T)
+    if T.dtype != np.complex128:
+        raise TypeError("eigcond takes a complex Schur factorization")
+    t = np.array(T, order="F", copy=True)
+    out = np.zeros(2)
+    h = _lib.handle(device)
+    rc = _lib.lib().gs_eigcond_z(h, n, _ptr(t), max(n, 1), int(nsub), _ptr(out), 0)
+    _lib.check(rc, h, "gs_eigcond_z")
+    return float(out[0]), float(out[1])
+
+
+def eigvalscond(S, nsub, device=0):
+    """eigvalscond(S, nsub) (ordschur.jl:85-98)."""
+    return eigcond(S, nsub, device)[0]
+
+
+def subspacesep(S, nsub, device=0):
+    """subspacesep(S, nsub) (ordschur.jl:111-131)."""
+    return eigcond(S, nsub, device)[1]
+
+
 def enormalize_(V, device=0):
     """_enormalize!(V) (util.jl:572-592): unit 2-norm columns, the first
     largest entry of each real and positive.  Returns the normalised copy."""

@@ -241,6 +241,25 @@ int gs_enormalize(gs_handle* h, int64_t n, int64_t m, double* V, int64_t ldv, in
 int gs_eig_rcond(gs_handle* h, int64_t n, int64_t m, const double* VL, int64_t ldvl, const double* VR, int64_t ldvr,
                  double* rconde, int flags);
 
+/* ---- reordering and conditioning of a complex Schur form ---- */
+
+/* gordschur!(T, Z, select) — ordschur.jl:23-73 (complex): the eigenvalues
+ * with select[k] != 0 (n int32, host memory) are moved to the front of the
+ * triangular T (n x n complex, ldt), in their order, by adjacent swaps with
+ * Givens rotations (_trexchange!, LinearAlgebra.givensAlgorithm), Z (ldz)
+ * updated alike (NULL: no Z), T made exactly upper triangular, W (n complex)
+ * = diag(T).  T and Z are overwritten. */
+int gs_ordschur_z(gs_handle* h, int64_t n, double* T, int64_t ldt, double* Z, int64_t ldz, const int32_t* select,
+                  double* W, int flags);
+/* eigvalscond(S, nsub) and subspacesep(S, nsub) — ordschur.jl:85-131 with
+ * trsylvester!/adjtrsylvester! (sylvester.jl:24-139) and norm1est!
+ * (norm1est.jl:21-100): out2[0] = the reciprocal condition number of the
+ * nsub leading eigenvalues of the triangular T, out2[1] = the estimated
+ * separation of its leading invariant subspace; 1 <= nsub <= n - 1
+ * (GS_E_ARG otherwise; the reference's estimator is undefined there).
+ * out2 is host memory. */
+int gs_eigcond_z(gs_handle* h, int64_t n, const double* T, int64_t ldt, int64_t nsub, double* out2, int flags);
+
 /* Statistics of the last Schur call on this handle: out[0] = QR sweeps,
  * out[1] = bulge steps, out[2] = kernel time in microseconds (events on the
  * handle's stream, 0 when unavailable).  Returns the number written. */

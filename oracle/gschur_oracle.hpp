@@ -1895,4 +1895,337 @@ template <class R> void balance_apply(long n, long m, long ilo, long ihi, const 
   }
 }
 
+// ======================================== reordering and conditioning
+// ordschur.jl:23-73 (gordschur!, _trexchange!), sylvester.jl:24-139
+// (trsylvester!, adjtrsylvester!), norm1est.jl:21-100 (norm1est!,
+// _findamax), ordschur.jl:75-131 (eigvalscond, subspacesep); complex
+// Schur forms.
+
+// Julia's floatmin2: 2^trunc(log2(floatmin / eps) / 2) (2^-485 for Float64)
+template <class R> R floatmin2() {
+  R x(1.0);
+  const R lim = rt<R>::fmin() / rt<R>::eps();
+  // the exponent of floatmin / eps, halved toward zero
+  int e = 0;
+  R y(1.0);
+  while (y > lim) {
+    y = y * R(0.5);
+    ++e;
+  }
+  for (int i = 0; i < e / 2; ++i) x = x * R(0.5);
+  return x;
+}
+
+// LinearAlgebra.givensAlgorithm(f, g) for complex arguments (the xLARTG
+// scheme of LAPACK 3.x with power-of-two scaling; its abs1 is
+// max(|re|, |im|)): cs real, sn complex, r = cs f + sn g.
+template <class R> void givens_cx(cx<R> f, cx<R> g, R& cs, cx<R>& sn, cx<R>& r) {
+  typedef cx<R> C;
+  auto am = [](const C& z) { return rmax(rt<R>::abs(z.re), rt<R>::abs(z.im)); };
+  const R one(1.0);
+  const R safmin = rt<R>::fmin();
+  const R safmn2 = floatmin2<R>();
+  const R safmx2 = one / safmn2;
+  R scalepar = rmax(am(f), am(g));
+  C fs = f, gs = g;
+  int count = 0;
+  if (scalepar >= safmx2) {
+    while (true) {
+      count += 1;
+      fs = fs * safmn2;
+      gs = gs * safmn2;
+      scalepar = scalepar * safmn2;
+      if (scalepar < safmx2 || count >= 20) break;
+    }
+  } else if (scalepar <= safmn2) {
+    if (g == C()) {
+      cs = one;
+      sn = C();
+      r = f;
+      return;
+    }
+    while (true) {
+      count -= 1;
+      fs = fs * safmx2;
+      gs = gs * safmx2;
+      scalepar = scalepar * safmx2;
+      if (scalepar > safmn2) break;
+    }
+  }
+  const R f2 = abs2(fs), g2 = abs2(gs);
+  if (f2 <= rmax(g2, one) * safmin) {
+    // F is very small
+    if (f == C()) {
+      cs = R(0.0);
+      r = C(rt<R>::hypot(g.re, g.im));
+      const R d = rt<R>::hypot(gs.re, gs.im);
+      sn = C(gs.re / d, -gs.im / d);
+      return;
+    }
+    const R f2s = rt<R>::hypot(fs.re, fs.im);
+    const R g2s = rt<R>::sqrt(g2);
+    cs = f2s / g2s;
+    C ff;
+    if (am(f) > one) {
+      const R d = rt<R>::hypot(f.re, f.im);
+      ff = C(f.re / d, f.im / d);
+    } else {
+      const R dr = safmx2 * f.re, di = safmx2 * f.im;
+      const R d = rt<R>::hypot(dr, di);
+      ff = C(dr / d, di / d);
+    }
+    sn = ff * C(gs.re / g2s, -gs.im / g2s);
+    r = cs * f + sn * g;
+  } else {
+    // the common case
+    const R f2s = rt<R>::sqrt(one + g2 / f2);
+    r = C(f2s * fs.re, f2s * fs.im);
+    cs = one / f2s;
+    const R d = f2 + g2;
+    sn = C(r.re / d, r.im / d);
+    sn = sn * conj(gs);
+    if (count > 0)
+      for (int i = 0; i < count; ++i) r = r * safmx2;
+    else
+      for (int i = 0; i < -count; ++i) r = r * safmn2;
+  }
+}
+
+// _trexchange!(T, Z, iold, inew) (ordschur.jl:52-73; Golub & Van Loan
+// Algorithm 7.6.1): adjacent swaps by Givens rotations, lmul!(G, T),
+// rmul!(T, G'), rmul!(Z, G').
+template <class R> void trexchange(long n, cx<R>* tp, long ldt, cx<R>* zp, long ldz, bool wantz, long iold,
+                                   long inew) {
+  typedef cx<R> C;
+  Mat<C> T{tp, ldt};
+  Mat<C> Z{zp, ldz};
+  if (n < 1 || iold == inew) return;
+  const long step = iold < inew ? 1 : -1;
+  const long k0 = iold < inew ? iold : iold - 1;
+  const long k1 = iold < inew ? inew - 1 : inew;
+  for (long k = k0;; k += step) {
+    const C Tkk = T(k, k), Tpp = T(k + 1, k + 1);
+    R c;
+    C sn, r;
+    givens_cx<R>(T(k, k + 1), Tpp - Tkk, c, sn, r);
+    for (long j = 1; j <= n; ++j) {
+      const C a1 = T(k, j), a2 = T(k + 1, j);
+      T(k, j) = c * a1 + sn * a2;
+      T(k + 1, j) = -conj(sn) * a1 + c * a2;
+    }
+    // G' = Givens(c, -sn): column k gets a1 c - a2 conj(-sn), column k+1
+    // a1 (-sn) + a2 c
+    const C ms = -sn;
+    for (long i = 1; i <= n; ++i) {
+      const C a1 = T(i, k), a2 = T(i, k + 1);
+      T(i, k) = a1 * c - a2 * conj(ms);
+      T(i, k + 1) = a1 * ms + a2 * c;
+    }
+    if (wantz)
+      for (long i = 1; i <= n; ++i) {
+        const C a1 = Z(i, k), a2 = Z(i, k + 1);
+        Z(i, k) = a1 * c - a2 * conj(ms);
+        Z(i, k + 1) = a1 * ms + a2 * c;
+      }
+    if (k == k1) break;
+  }
+}
+
+// gordschur!(T, Z, select) (ordschur.jl:23-43): the selected eigenvalues to
+// the front, in order; T made exactly upper triangular; w = diag(T).
+template <class R> void gordschur(long n, cx<R>* tp, long ldt, cx<R>* zp, long ldz, bool wantz, const int* select,
+                                  cx<R>* w) {
+  typedef cx<R> C;
+  Mat<C> T{tp, ldt};
+  long ks = 0;
+  for (long k = 1; k <= n; ++k)
+    if (select[k - 1]) {
+      ks += 1;
+      if (k != ks) trexchange<R>(n, tp, ldt, zp, ldz, wantz, k, ks);
+    }
+  for (long j = 1; j <= n; ++j)
+    for (long i = j + 1; i <= n; ++i) T(i, j) = C();
+  for (long j = 1; j <= n; ++j) w[j - 1] = T(j, j);
+}
+
+// the entrywise max |a| of Julia's norm(A, Inf) for an m x m block
+template <class R> R maxabs(long m, const cx<R>* ap, long lda) {
+  R mx(0.0);
+  for (long j = 0; j < m; ++j)
+    for (long i = 0; i < m; ++i) mx = rmax(mx, absv(ap[i + j * lda]));
+  return mx;
+}
+
+// trsylvester!(A, B, C) (sylvester.jl:24-75): A X - X B = scale C for
+// upper triangular A (m x m) and B (n x n); C (m x n, ldc) overwritten by
+// X.  Returns scale; perturbed set when a diagonal sum was replaced by smin.
+template <class R> R trsylvester(long m, const cx<R>* ap, long lda, long n, const cx<R>* bp, long ldb, cx<R>* cp,
+                                 long ldc, bool& perturbed) {
+  typedef cx<R> C;
+  Mat<const C> A{ap, lda};
+  Mat<const C> B{bp, ldb};
+  Mat<C> X{cp, ldc};
+  R scale(1.0);
+  const R tiny = rt<R>::eps();
+  const R small = safemin<R>() * R((double)m) * R((double)n) / tiny;
+  const R bignum = R(1.0) / small;
+  const R smin = rmax(small, tiny * maxabs<R>(m, ap, lda) * maxabs<R>(n, bp, ldb));
+  perturbed = false;
+  for (long l = 1; l <= n; ++l)
+    for (long k = m; k >= 1; --k) {
+      C suml, sumr;
+      for (long j = k + 1; j <= m; ++j) suml += A(k, j) * X(j, l);
+      for (long j = 1; j <= l - 1; ++j) sumr += X(k, j) * B(j, l);
+      const C v = X(k, l) - (suml - sumr);  // isgn = -1
+      R scaloc(1.0);
+      C a11 = A(k, k) - B(l, l);
+      R da11 = abs1(a11);
+      if (da11 <= smin) {
+        a11 = C(smin);
+        da11 = smin;
+        perturbed = true;
+      }
+      const R db = abs1(v);
+      if (da11 < R(1.0) && db > R(1.0) && db > bignum * da11) scaloc = R(1.0) / db;
+      const C x11 = (v * scaloc) / a11;
+      if (scaloc != R(1.0)) {
+        for (long jj = 1; jj <= n; ++jj)
+          for (long ii = 1; ii <= m; ++ii) X(ii, jj) = scaloc * X(ii, jj);
+        scale = scale * scaloc;
+      }
+      X(k, l) = x11;
+    }
+  return scale;
+}
+
+// adjtrsylvester!(A, B, C) (sylvester.jl:86-139): A^H X - X B^H = scale C.
+template <class R> R adjtrsylvester(long m, const cx<R>* ap, long lda, long n, const cx<R>* bp, long ldb, cx<R>* cp,
+                                    long ldc, bool& perturbed) {
+  typedef cx<R> C;
+  Mat<const C> A{ap, lda};
+  Mat<const C> B{bp, ldb};
+  Mat<C> X{cp, ldc};
+  R scale(1.0);
+  const R tiny = rt<R>::eps();
+  const R small = safemin<R>() * R((double)m) * R((double)n) / tiny;
+  const R bignum = R(1.0) / small;
+  const R smin = rmax(small, tiny * maxabs<R>(m, ap, lda) * maxabs<R>(n, bp, ldb));
+  perturbed = false;
+  for (long l = n; l >= 1; --l)
+    for (long k = 1; k <= m; ++k) {
+      C suml, sumr;
+      for (long j = 1; j <= k - 1; ++j) suml += conj(A(j, k)) * X(j, l);
+      for (long j = l + 1; j <= n; ++j) sumr += conj(X(k, j)) * B(l, j);
+      const C v = X(k, l) - (suml - conj(sumr));  // isgn = -1
+      R scaloc(1.0);
+      C a11 = conj(A(k, k) - B(l, l));
+      R da11 = abs1(a11);
+      if (da11 < smin) {
+        a11 = C(smin);
+        da11 = smin;
+        perturbed = true;
+      }
+      const R db = abs1(v);
+      if (da11 < R(1.0) && db > R(1.0) && db > bignum * da11) scaloc = R(1.0) / db;
+      const C x11 = (v * scaloc) / a11;
+      if (scaloc != R(1.0)) {
+        for (long jj = 1; jj <= n; ++jj)
+          for (long ii = 1; ii <= m; ++ii) X(ii, jj) = scaloc * X(ii, jj);
+        scale = scale * scaloc;
+      }
+      X(k, l) = x11;
+    }
+  return scale;
+}
+
+// _findamax (norm1est.jl:86-100), complex: sqrt of the first largest |x|^2
+template <class R> R findamax(long n, const cx<R>* x, long& i0) {
+  R ax0 = abs2(x[0]);
+  i0 = 0;
+  for (long i = 1; i < n; ++i) {
+    const R ax = abs2(x[i]);
+    if (ax > ax0) {
+      ax0 = ax;
+      i0 = i;
+    }
+  }
+  return rt<R>::sqrt(ax0);
+}
+
+template <class R> R norm1(long n, const cx<R>* x) {
+  R s(0.0);
+  for (long i = 0; i < n; ++i) s = s + absv(x[i]);
+  return s;
+}
+
+// norm1est!(applyA!, applyAH!, y) (norm1est.jl:21-75), Hager's estimator
+// with Higham's modifications; the operators act in place on x.
+template <class R, class FA, class FH> R norm1est(long n, FA applyA, FH applyAH, int maxiter = 5) {
+  typedef cx<R> C;
+  std::vector<C> x((size_t)n, C(R(1.0) / R((double)n))), y((size_t)n);
+  applyA(x.data());
+  if (n == 1) return absv(x[0]);
+  R gamma = norm1<R>(n, x.data());
+  const R tiny = safemin<R>();
+  auto sgn = [&]() {
+    for (long i = 0; i < n; ++i) {
+      const R a = absv(x[i]);
+      x[i] = a > tiny ? x[i] / a : C(R(1.0));
+    }
+  };
+  sgn();
+  applyAH(x.data());
+  long j0;
+  R ax0 = findamax<R>(n, x.data(), j0);
+  for (int iter = 1; iter <= maxiter; ++iter) {
+    for (long i = 0; i < n; ++i) x[i] = C();
+    x[j0] = C(R(1.0));
+    applyA(x.data());
+    y = x;
+    const R oldg = gamma;
+    gamma = norm1<R>(n, y.data());
+    if (gamma <= oldg) break;
+    sgn();
+    applyAH(x.data());
+    const long jlast = j0;
+    ax0 = findamax<R>(n, x.data(), j0);
+    if (absv(x[jlast]) == ax0) break;
+  }
+  // alternative estimate (Higham 1990)
+  R isign(1.0);
+  for (long i = 0; i < n; ++i) {
+    x[i] = C(isign * (R(1.0) + R((double)i) / R((double)(n - 1))));
+    isign = -isign;
+  }
+  applyA(x.data());
+  const R t = R(2.0) * norm1<R>(n, x.data()) / (R(3.0) * R((double)n));
+  return rmax(t, gamma);
+}
+
+// eigvalscond(S, nsub) (ordschur.jl:85-98) and subspacesep(S, nsub)
+// (ordschur.jl:111-131) of the upper triangular T (n x n); 0 < nsub < n.
+template <class R> R eigvalscond(long n, const cx<R>* tp, long ldt, long nsub) {
+  typedef cx<R> C;
+  const long m = nsub, n2 = n - nsub;
+  std::vector<C> X((size_t)(m * n2));
+  for (long j = 0; j < n2; ++j)
+    for (long i = 0; i < m; ++i) X[i + j * m] = tp[i + (m + j) * ldt];
+  bool pert;
+  const R scale = trsylvester<R>(m, tp, ldt, n2, tp + m + m * ldt, ldt, X.data(), m, pert);
+  R rn(0.0);
+  for (const C& v : X) rn = rn + abs2(v);
+  rn = rt<R>::sqrt(rn);
+  return rn > R(0.0) ? scale / (rt<R>::sqrt(scale * scale / rn + rn) * rt<R>::sqrt(rn)) : R(1.0);
+}
+
+template <class R> R subspacesep(long n, const cx<R>* tp, long ldt, long nsub) {
+  const long m = nsub, n2 = n - nsub;
+  R scale(1.0);
+  bool pert;
+  auto fA = [&](cx<R>* x) { scale = trsylvester<R>(m, tp, ldt, n2, tp + m + m * ldt, ldt, x, m, pert); };
+  auto fH = [&](cx<R>* x) { scale = adjtrsylvester<R>(m, tp, ldt, n2, tp + m + m * ldt, ldt, x, m, pert); };
+  const R est = norm1est<R>(m * n2, fA, fH);
+  return scale / est;
+}
+
 }  // namespace orc

@@ -238,4 +238,33 @@ void orc_balance_apply(long n, long m, const long* ilo_ihi_triv, const long* sp,
   balance_apply<double>(n, m, ilo_ihi_triv[0], ilo_ihi_triv[1], sp, d, ilo_ihi_triv[2] != 0, (zc*)v, ldv,
                         inverse != 0);
 }
+// gordschur! (complex; select: n ints), eigvalscond, subspacesep
+void orc_ordschur_z(long n, double* t, long ldt, double* z, long ldz, int wantz, const int* select, double* w) {
+  gordschur<double>(n, (zc*)t, ldt, (zc*)z, ldz, wantz != 0, select, (zc*)w);
+}
+double orc_eigvalscond_z(long n, const double* t, long ldt, long nsub) {
+  return eigvalscond<double>(n, (const zc*)t, ldt, nsub);
+}
+double orc_subspacesep_z(long n, const double* t, long ldt, long nsub) {
+  return subspacesep<double>(n, (const zc*)t, ldt, nsub);
+}
+// trsylvester! (adj = 0) / adjtrsylvester! (adj = 1) on complex A, B, C
+double orc_trsylvester_z(long m, const double* a, long lda, long n, const double* b, long ldb, double* c, long ldc,
+                         int adj, int* perturbed) {
+  bool p = false;
+  const double s = adj ? adjtrsylvester<double>(m, (const zc*)a, lda, n, (const zc*)b, ldb, (zc*)c, ldc, p)
+                       : trsylvester<double>(m, (const zc*)a, lda, n, (const zc*)b, ldb, (zc*)c, ldc, p);
+  *perturbed = p ? 1 : 0;
+  return s;
+}
+void orc_givens_z(const double* f, const double* g, double* out5) {
+  double cs;
+  zc sn, r;
+  givens_cx<double>(zc(f[0], f[1]), zc(g[0], g[1]), cs, sn, r);
+  out5[0] = cs;
+  out5[1] = sn.re;
+  out5[2] = sn.im;
+  out5[3] = r.re;
+  out5[4] = r.im;
+}
 }  // extern "C"

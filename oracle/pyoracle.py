@@ -58,6 +58,16 @@ def lib():
         L.orc_balance_z.argtypes = [c_long, c_dp, c_long, c_int, c_int, c_lp, c_lp, c_dp]
         L.orc_balance_d.argtypes = [c_long, c_dp, c_long, c_int, c_int, c_lp, c_lp, c_dp]
         L.orc_balance_apply.argtypes = [c_long, c_long, c_lp, c_lp, c_dp, c_dp, c_long, c_int]
+        L.orc_ordschur_z.argtypes = [c_long, c_dp, c_long, c_dp, c_long, c_int, c_ip, c_dp]
+        L.orc_eigvalscond_z.argtypes = [c_long, c_dp, c_long, c_long]
+        L.orc_eigvalscond_z.restype = ctypes.c_double
+        L.orc_subspacesep_z.argtypes = [c_long, c_dp, c_long, c_long]
+        L.orc_subspacesep_z.restype = ctypes.c_double
+        L.orc_trsylvester_z.argtypes = [c_long, c_dp, c_long, c_long, c_dp, c_long, c_dp, c_long, c_int, c_ip]
+        L.orc_trsylvester_z.restype = ctypes.c_double
+        L.orc_givens_z.argtypes = [c_dp, c_dp, c_dp]
+        L.orc_ordschur_z.restype = None
+        L.orc_givens_z.restype = None
         L.orc_balance_apply.restype = None
         L.orc_eig_hp_z.argtypes = [c_long, c_dp, c_dp, c_int]
         L.orc_eig_hp_d.argtypes = [c_long, c_dp, c_dp, c_int]
@@ -322,3 +332,50 @@ def eigen(A, permute=True, scale=True, sort=True, jvl=False, jce=False):
         if rc is not None:
             rc = rc[p]
     return w, v, (vl if jvl else None), rc
+
+
+def ordschur(T, Z, select):
+    """gordschur!(T, Z, select) for a complex Schur form (ordschur.jl:23-43):
+    returns (T2, Z2, w)."""
+    n = T.shape[0]
+    t = _fort(T, np.complex128)
+    z = _fort(Z, np.complex128) if Z is not None else np.zeros((1, 1), dtype=np.complex128, order="F")
+    sel = np.ascontiguousarray(np.asarray(select, dtype=bool).astype(np.int32))
+    w = np.zeros(max(n, 1), dtype=np.complex128)
+    lib().orc_ordschur_z(n, _p(t), max(n, 1), _p(z), max(z.shape[0], 1), int(Z is not None),
+                         sel.ctypes.data_as(c_ip), _p(w))
+    return t, (z if Z is not None else None), w[:n]
+
+
+def eigvalscond(T, nsub):
+    """eigvalscond(S, nsub) (ordschur.jl:85-98) of the complex triangular T."""
+    t = _fort(T, np.complex128)
+    return float(lib().orc_eigvalscond_z(t.shape[0], _p(t), t.shape[0], int(nsub)))
+
+
+def subspacesep(T, nsub):
+    """subspacesep(S, nsub) (ordschur.jl:111-131) of the complex triangular T."""
+    t = _fort(T, np.complex128)
+    return float(lib().orc_subspacesep_z(t.shape[0], _p(t), t.shape[0], int(nsub)))
+
+
+def trsylvester(A, B, C, adjoint=False):
+    """trsylvester!(A, B, C) / adjtrsylvester!(A, B, C) (sylvester.jl:24-139),
+    complex: returns (X, scale, perturbed)."""
+    a = _fort(A, np.complex128)
+    b = _fort(B, np.complex128)
+    c = _fort(C, np.complex128)
+    m, n = c.shape
+    pert = np.zeros(1, dtype=np.int32)
+    sc = lib().orc_trsylvester_z(m, _p(a), max(m, 1), n, _p(b), max(n, 1), _p(c), max(m, 1), int(adjoint),
+                                 pert.ctypes.data_as(c_ip))
+    return c, float(sc), bool(pert[0])
+
+
+def givens(f, g):
+    """LinearAlgebra.givensAlgorithm(f, g), complex: (c, s, r)."""
+    fa = np.array([complex(f).real, complex(f).imag])
+    ga = np.array([complex(g).real, complex(g).imag])
+    out = np.zeros(5)
+    lib().orc_givens_z(_p(fa), _p(ga), _p(out))
+    return out[0], complex(out[1], out[2]), complex(out[3], out[4])

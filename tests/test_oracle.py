@@ -548,3 +548,91 @@ def test_balance_nan_error(oracle):
     (balance.jl:160-162) rather than a sweep loop that never settles."""
     A = np.array([[1.0, 2.0, 0.5], [np.nan, 3.0, 1.0], [4.0, 1.0, 2.0]])
     assert oracle.balance(A)[4] == -3
+
+
+# ------------------------------------------ reordering and conditioning
+def test_givens_matches_zlartg(oracle):
+    """LinearAlgebra.givensAlgorithm (the rotation of _trexchange!,
+    ordschur.jl:62) against LAPACK's xLARTG: the same c, s, r to rounding,
+    [c s; -conj(s) c] [f; g] = [r; 0], c^2 + |s|^2 = 1, including the
+    scaled ranges and the zero cases."""
+    r = np.random.default_rng(31)
+    cases = [(1 + 2j, 3 - 1j), (0j, 2 + 1j), (2 - 1j, 0j), (1e-300 + 1e-300j, 3e-300j), (1e300 - 2e300j, 5e299j),
+             (1e-160j, 1.0 + 0j), (3.0 + 0j, 1e-170 + 0j)]
+    cases += [complex(*r.standard_normal(2)) * 10.0 ** r.uniform(-5, 5) for _ in range(20)]
+    cases = [(c, complex(*r.standard_normal(2))) if not isinstance(c, tuple) else c for c in cases]
+    for f, g in cases:
+        c, s, rr = oracle.givens(f, g)
+        cl, sl, rl = sla.lapack.zlartg(f, g)
+        scale = max(abs(f), abs(g))
+        assert abs(c * c + abs(s) ** 2 - 1) <= 8 * EPS
+        assert abs(rr - (c * f + s * g)) <= 8 * EPS * scale
+        assert abs(-np.conj(s) * f + c * g) <= 8 * EPS * scale
+        assert abs(c - cl) <= 8 * EPS and abs(s - sl) <= 8 * EPS and abs(rr - rl) <= 8 * EPS * scale
+
+
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_trsylvester_residual(oracle, adjoint):
+    """trsylvester! / adjtrsylvester! (sylvester.jl:24-139): the residual of
+    A X - X B = scale C (or A^H X - X B^H = scale C) at rounding level, and
+    X against scipy's solve_sylvester on the same well-separated triangular
+    blocks."""
+    r = np.random.default_rng(32)
+    for m, n in ((1, 1), (3, 5), (12, 7), (20, 20)):
+        A = np.triu(r.standard_normal((m, m)) + 1j * r.standard_normal((m, m))) + 4 * np.eye(m)
+        B = np.triu(r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))) - 4 * np.eye(n)
+        C = r.standard_normal((m, n)) + 1j * r.standard_normal((m, n))
+        X, sc, pert = oracle.trsylvester(A, B, C, adjoint=adjoint)
+        assert sc == 1.0 and not pert
+        Aop, Bop = (A.conj().T, B.conj().T) if adjoint else (A, B)
+        res = np.linalg.norm(Aop @ X - X @ Bop - sc * C) / (np.linalg.norm(Aop) + np.linalg.norm(Bop)) / np.linalg.norm(X)
+        assert res <= 20 * max(m, n) * EPS
+        Xs = sla.solve_sylvester(Aop, -Bop, C)
+        assert np.linalg.norm(X - Xs) <= 1e-10 * np.linalg.norm(Xs)
+
+
+def _ztrsen(T, Z, select):
+    n = T.shape[0]
+    m = int(np.count_nonzero(select))
+    out = sla.lapack.ztrsen(np.asarray(select, dtype=np.int32), T, Z, job="B", lwork=max(1, 2 * m * (n - m)))
+    ts, qs, w, mm, s, sep, info = out
+    assert info == 0
+    return ts, qs, w, s, sep
+
+
+@pytest.mark.parametrize("n", [6, 32])
+def test_ordschur_matches_ztrsen(oracle, n):
+    """gordschur! on a complex Schur form (the reference's checkord,
+    test/ordschur.jl:1-29: triangular, backward error, unitary Z, the selected
+    eigenvalues first) and against LAPACK's xTRSEN on the same (T, Z): the
+    same diagonal order and T, Z equal up to diagonal phases; eigvalscond and
+    subspacesep against xTRSEN's s and sep (test/ordschur.jl:31-56)."""
+    r = np.random.default_rng(33 + n)
+    A = r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))
+    T, Z, w, info, _ = oracle.gschur(A)
+    assert info == 0
+    for trial in range(3):
+        if trial == 0:
+            select = np.ones(n, dtype=bool)
+            select[: n // 2] = False
+        else:
+            select = np.zeros(n, dtype=bool)
+            select[r.choice(n, size=max(1, n // 4), replace=False)] = True
+        T2, Z2, w2 = oracle.ordschur(T, Z, select)
+        assert np.all(np.tril(T2, -1) == 0)
+        assert np.linalg.norm(Z2 @ T2 @ Z2.conj().T - A) / (n * np.linalg.norm(A) * EPS) < 10
+        assert np.linalg.norm(np.eye(n) - Z2 @ Z2.conj().T) / (n * EPS) < 10
+        k = int(select.sum())
+        want = w[select]
+        assert np.all(np.abs(want - w2[:k]) <= 10 * EPS * np.abs(w).max())
+        ts, qs, wl, s, sep = _ztrsen(np.asfortranarray(T), np.asfortranarray(Z), select)
+        assert np.all(np.abs(wl - w2) <= 100 * n * EPS * np.abs(w).max())
+        # up to diagonal phases: T2 = D^H Ts D, Z2 = Qs D
+        ph = np.array([np.vdot(qs[:, j], Z2[:, j]) for j in range(n)])
+        ph = ph / np.abs(ph)
+        assert np.abs(Z2 - qs * ph).max() <= 1e-10
+        assert np.abs(T2 - np.conj(ph)[:, None] * ts * ph[None, :]).max() <= 1e-10 * np.abs(T).max()
+        s2 = oracle.eigvalscond(T2, k)
+        sep2 = oracle.subspacesep(T2, k)
+        assert abs(s2 - s) <= 1e-8 * s
+        assert abs(sep2 - sep) <= 1e-8 * sep
