@@ -18,6 +18,7 @@
 #include "chains.h"
 #include "dchase.h"
 #include "dev_common.h"
+#include "formq.h"
 #include "gs2x2.h"
 #include "internal.h"
 #include "layout.h"
@@ -241,57 +242,6 @@ template <int NT, int RM = 0> GS_DEV void d_hessenberg(double* H, int64_t ld, in
     }
     __syncthreads();
   }
-}
-
-template <int NT, int RM = 0> GS_DEV void d_form_q_inplace(double* H, int64_t ld, int n, const double* tau, double* wk) {
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  for (int k = n - 2; k >= 0; --k) {
-    const int c = k + 1;
-    const double t = tau[k];
-    // s_j = t (Q[c, j] + sum_{r>c} v_r Q[r, j]); column j -= s_j [1; v]
-    ddots4<NT>(c + 1, n, c + 1, n, [&](int r, int j) { return H[r + (int64_t)k * ld] * H[r + (int64_t)j * ld]; }, wk);
-    __syncthreads();
-    for (int j = c + 1 + tid; j < n; j += NT) wk[j] = t * (H[c + (int64_t)j * ld] + wk[j]);
-    __syncthreads();
-    if constexpr (RM > 0) {
-      double vr[RM];
-#pragma unroll
-      for (int q = 0; q < RM; ++q) {
-        const int r = c + lane + 64 * q;
-        vr[q] = (r < n) ? ((r == c) ? 1.0 : H[r + (int64_t)k * ld]) : 0.0;
-      }
-      #pragma unroll 4
-      for (int j = c + 1 + wave; j < n; j += NT / 64) {
-        const double sj = wk[j];
-#pragma unroll
-        for (int q = 0; q < RM; ++q) {
-          const int r = c + lane + 64 * q;
-          if (r < n) H[r + (int64_t)j * ld] -= sj * vr[q];
-        }
-      }
-    } else {
-      #pragma unroll 4
-      for (int j = c + 1 + wave; j < n; j += NT / 64) {
-        const double sj = wk[j];
-        for (int r = c + lane; r < n; r += 64) {
-          const double h = H[r + (int64_t)j * ld];
-          H[r + (int64_t)j * ld] = (r == c) ? h - sj : h - sj * H[r + (int64_t)k * ld];
-        }
-      }
-    }
-    __syncthreads();
-    for (int r = tid; r < n; r += NT) {
-      double q;
-      if (r < c) q = 0.0;
-      else if (r == c) q = 1.0 - t;
-      else q = -(t * H[r + (int64_t)k * ld]);
-      H[r + (int64_t)c * ld] = q;
-    }
-    __syncthreads();
-  }
-  for (int r = tid; r < n; r += NT) H[r] = (r == 0) ? 1.0 : 0.0;
-  __syncthreads();
 }
 
 // Chain application of a double-shift sweep's reflectors (steps k0..k1) to Z.

@@ -170,6 +170,43 @@ hipError_t launch_enormalize(int n, int m, double* V, int64_t ldv, hipStream_t s
 // gordschur! on a complex Schur form (T, Z: ld ldt, ldz; select: n ints)
 // and eigvalscond / subspacesep (work: 3 nsub (n - nsub) complex entries,
 // out: s, sep), one wavefront each
+// Hermitian / real symmetric eigenproblem (heev.hip).  mode 0: eigenvalues
+// W (and eigenvectors Z when non-null); mode 2: the tridiagonal matrix (W =
+// d, E = e) and Q (Z when non-null).  A: lower (upper = 0) or upper triangle
+// of each Hermitian matrix (lda, stride sA elements).
+struct HeevArgs {
+  int count = 0, n = 0, upper = 0, mode = 0, sort = 0, maxsweeps = 0;
+  const void* A = nullptr;
+  int64_t lda = 0, sA = 0;
+  double* W = nullptr;
+  int64_t sW = 0;
+  double* E = nullptr;
+  int64_t sE = 0;
+  void* Z = nullptr;
+  int64_t ldz = 0, sZ = 0;
+  int* info = nullptr;
+  int* sweeps = nullptr;
+};
+struct SteqrArgs {
+  int n = 0, zrows = 0, maxsweeps = 0;
+  double* d = nullptr;
+  double* e = nullptr;
+  void* Z = nullptr;
+  int64_t ldz = 0;
+  int* info = nullptr;
+  int* sweeps = nullptr;
+};
+// LDS shared by the reduction's vectors (tau, v, w) and the rotation log
+constexpr size_t heev_union_bytes(int n, size_t es) {
+  return ((3 * (size_t)n * es > 56 * (size_t)n ? 3 * (size_t)n * es : 56 * (size_t)n) + 15) / 16 * 16;
+}
+constexpr int heev_max_n(bool cplx) { return cplx ? 96 : 128; }
+constexpr int steqr_max_n() { return 2048; }
+size_t heev_lds_bytes(int n, bool cplx);
+size_t steqr_lds_bytes(int n);
+hipError_t launch_heev(const HeevArgs& a, bool cplx, hipStream_t s);
+hipError_t launch_steqr(const SteqrArgs& a, bool zcplx, hipStream_t s);
+
 hipError_t launch_ordschur(int n, double* T, int64_t ldt, double* Z, int64_t ldz, bool wantz, const int32_t* select,
                            double* W, hipStream_t s);
 hipError_t launch_eigcond(int n, const double* T, int64_t ldt, int nsub, double* work, double* out, hipStream_t s);

@@ -1,0 +1,526 @@
+// Symmetric tridiagonal QL/QR iteration with accumulation of the rotations,
+// following GenericSchur.jl's _gschur!(A::SymTridiagonal, QRIteration(), Z)
+// (symtridiag.jl:77-172, a translation of LAPACK xSTEQR) with _st_QLIter!
+// (:175-262), _st_QRIter! (:265-352), _st_findsplit_fwd! (:354-372) and
+// _dgz_sym2x2 (:375-427).
+//
+// Work split on the GPU: one lane (the producer) runs the scalar recurrence
+// on d and e in LDS and appends every rotation it makes to a log in LDS;
+// the other wavefronts of the workgroup (the appliers) apply the log to the
+// rows of Z, each lane one row at a time, with the rotations of a sweep
+// chained through a register.  Two log buffers alternate, so the appliers
+// work on round k's rotations while the producer makes round k+1's; one
+// workgroup barrier separates rounds.  Each element of Z receives exactly
+// the reference's operations in the reference's order
+// (rmul!(Z, Givens(i, i+1, c, s)'): Z[:, i] = c a1 + s a2,
+// Z[:, i+1] = -s a1 + c a2), and the recurrence is evaluated without
+// contraction, so d, e and Z match the CPU restatement bit for bit.
+//
+// The reference's sweep counter never advances (symtridiag.jl:141: the
+// statement `niter + - nit` discards its value), so it has no iteration
+// limit; here the producer stops after `maxsweeps` QL/QR sweeps in all
+// (30 n, LAPACK's limit) and reports the number of nonzero off-diagonal
+// entries left, so that every launch ends.
+#pragma once
+#include "dev_common.h"
+
+namespace gs {
+
+// Julia's Base.Math hypot(x, y) for Float64: the fused multiply-add branch,
+// correctly rounded.
+GS_DEV double jl_hypot(double x, double y) {
+#pragma clang fp contract(off)
+  double ax = fabs(x), ay = fabs(y);
+  if (__builtin_isinf(ax) || __builtin_isinf(ay)) return __builtin_huge_val();
+  if (ay > ax) {
+    const double t = ax;
+    ax = ay;
+    ay = t;
+  }
+  if (ay <= ax * sqrt(GS_EPS / 2.0)) return ax;
+  double scale = GS_EPS * sqrt(GS_FMIN);
+  if (ax > sqrt(GS_FMAX / 2.0)) {
+    ax = ax * scale;
+    ay = ay * scale;
+    scale = 1.0 / scale;
+  } else if (ay < sqrt(GS_FMIN)) {
+    ax = ax / scale;
+    ay = ay / scale;
+  } else {
+    scale = 1.0;
+  }
+  double h = sqrt(__builtin_fma(ax, ax, ay * ay));
+  const double hsq = h * h, axsq = ax * ax;
+  h -= (__builtin_fma(-ay, ay, hsq - axsq) + __builtin_fma(h, h, -hsq) - __builtin_fma(ax, ax, -axsq)) / (2.0 * h);
+  return h * scale;
+}
+
+// LinearAlgebra.givensAlgorithm(f, g) for real f, g (LAPACK 3.1 xLARTG):
+// floatmin2 = 2^-485 scaling and the sign fix when |f| > |g| and cs < 0.
+GS_DEV void givens_re(double f, double g, double& cs, double& sn, double& r) {
+#pragma clang fp contract(off)
+  const double safmn2 = 0x1p-485, safmx2 = 0x1p485;
+  if (g == 0.0) {
+    cs = 1.0;
+    sn = 0.0;
+    r = f;
+    return;
+  }
+  if (f == 0.0) {
+    cs = 0.0;
+    sn = 1.0;
+    r = g;
+    return;
+  }
+  double f1 = f, g1 = g;
+  double sc = fmax(fabs(f1), fabs(g1));
+  if (sc >= safmx2) {
+    int count = 0;
+    while (true) {
+      count += 1;
+      f1 = f1 * safmn2;
+      g1 = g1 * safmn2;
+      sc = fmax(fabs(f1), fabs(g1));
+      if (sc < safmx2 || count >= 20) break;
+    }
+    r = sqrt(f1 * f1 + g1 * g1);
+    cs = f1 / r;
+    sn = g1 / r;
+    for (int i = 0; i < count; ++i) r = r * safmx2;
+  } else if (sc <= safmn2) {
+    int count = 0;
+    // the scaled value exceeds safmn2 after at most 3 steps for finite
+    // nonzero input; the cap keeps a NaN from looping
+    while (true) {
+      count += 1;
+      f1 = f1 * safmx2;
+      g1 = g1 * safmx2;
+      sc = fmax(fabs(f1), fabs(g1));
+      if (sc > safmn2 || count >= 20) break;
+    }
+    r = sqrt(f1 * f1 + g1 * g1);
+    cs = f1 / r;
+    sn = g1 / r;
+    for (int i = 0; i < count; ++i) r = r * safmn2;
+  } else {
+    r = sqrt(f1 * f1 + g1 * g1);
+    cs = f1 / r;
+    sn = g1 / r;
+  }
+  if (fabs(f) > fabs(g) && cs < 0.0) {
+    cs = -cs;
+    sn = -sn;
+    r = -r;
+  }
+}
+
+// _dgz_sym2x2(a, b, c, wantvec) (symtridiag.jl:375-427; LAPACK xLAEV2)
+GS_DEV void sym2x2(double a, double b, double c, bool wantvec, double& rt1, double& rt2, double& cs1, double& sn1) {
+#pragma clang fp contract(off)
+  const double sm = a + c, df = a - c, adf = fabs(df), tb = 2.0 * b, ab = fabs(tb);
+  double acmx, acmn;
+  if (fabs(a) > fabs(c)) {
+    acmx = a;
+    acmn = c;
+  } else {
+    acmx = c;
+    acmn = a;
+  }
+  const double rt = jl_hypot(ab, adf);
+  int sgn1;
+  if (sm < 0.0) {
+    rt1 = (sm - rt) / 2.0;
+    sgn1 = -1;
+    rt2 = (acmx / rt1) * acmn - (b / rt1) * b;
+  } else if (sm > 0.0) {
+    rt1 = (sm + rt) / 2.0;
+    sgn1 = 1;
+    rt2 = (acmx / rt1) * acmn - (b / rt1) * b;
+  } else {
+    rt1 = rt / 2.0;
+    rt2 = -rt / 2.0;
+    sgn1 = 1;
+  }
+  cs1 = 0.0;
+  sn1 = 0.0;
+  if (!wantvec) return;
+  double cs;
+  int sgn2;
+  if (df > 0.0) {
+    cs = df + rt;
+    sgn2 = 1;
+  } else {
+    cs = df - rt;
+    sgn2 = -1;
+  }
+  if (fabs(cs) > ab) {
+    const double ct = -tb / cs;
+    sn1 = 1.0 / sqrt(1.0 + ct * ct);
+    cs1 = ct * sn1;
+  } else if (ab == 0.0) {
+    cs1 = 1.0;
+    sn1 = 0.0;
+  } else {
+    const double tn = -cs / tb;
+    cs1 = 1.0 / sqrt(1.0 + tn * tn);
+    sn1 = tn * cs1;
+  }
+  if (sgn1 == sgn2) {
+    const double tn = cs1;
+    cs1 = -sn1;
+    sn1 = tn;
+  }
+}
+
+// safescale!(x, cfrom, cto) (util.jl:41-80) on len entries of x
+GS_DEV void st_safescale(double* x, int len, double cfrom, double cto) {
+  const double smlnum = GS_FMIN, bignum = 1.0 / smlnum;
+  double cfromc = cfrom, ctoc = cto;
+  bool done = false;
+  for (int guard = 0; !done && guard < 64; ++guard) {
+    double mul;
+    const double cfrom1 = cfromc * smlnum;
+    if (cfrom1 == cfromc) {
+      mul = ctoc / cfromc;
+      done = true;
+    } else {
+      const double cto1 = ctoc / bignum;
+      if (cto1 == ctoc) {
+        mul = cto;
+        done = true;
+        cfromc = 1.0;
+      } else if (fabs(cfrom1) > fabs(ctoc) && ctoc != 0.0) {
+        mul = smlnum;
+        cfromc = 1.0;
+      } else if (fabs(cto1) > fabs(cfromc)) {
+        mul = bignum;
+        cfromc = cfrom1;
+      } else {
+        mul = ctoc / cfromc;
+        done = true;
+      }
+    }
+    for (int i = 0; i < len; ++i) x[i] = x[i] * mul;
+  }
+}
+
+// Rotation log: segment s covers rotations rot[off, off + cnt) on column
+// pairs (i, i+1) (1-based) with i = i0, i0 + dir, ...; rotation (c, s)
+// maps Z[:, i] = c a1 + s a2, Z[:, i+1] = -s a1 + c a2.
+struct StLog {
+  double* c;   // cap
+  double* s;   // cap
+  int* seg;    // 3 * cap: i0, cnt, dir
+  int nrot, nseg;
+};
+
+// Producer state (symtridiag.jl:77-172 unrolled into resumable steps).
+struct StState {
+  int n, l1, l, lend, lsave, lendsave, iscale, sweeps, maxsweeps, info;
+  bool inblk, ql, stop, done;
+  double anorm;
+};
+
+GS_DEV void st_init(StState& S, int n, int maxsweeps) {
+  S.n = n;
+  S.l1 = 1;
+  S.l = S.lend = S.lsave = S.lendsave = 0;
+  S.iscale = 0;
+  S.sweeps = 0;
+  S.maxsweeps = maxsweeps;
+  S.info = 0;
+  S.inblk = S.ql = S.stop = false;
+  S.done = n <= 0;
+  S.anorm = 0.0;
+}
+
+// Upper bound on the rotations the next st_step can log.
+GS_DEV int st_next_bound(const StState& S) { return S.inblk ? abs(S.lend - S.l) : S.n - 1; }
+
+// One step of the producer: the set-up of the next unreduced block, one QL
+// or QR iteration, or the end of a block.  d, e: 0-based arrays; D(i) =
+// d[i-1] is the reference's 1-based d[i].  Rotations go to lg when non-null.
+GS_DEV void st_step(StState& S, double* d, double* e, StLog* lg) {
+#pragma clang fp contract(off)
+  const int n = S.n;
+  const double eps = GS_EPS, eps2 = GS_EPS * GS_EPS, safmin = GS_FMIN;
+  const double ssfmax = sqrt(1.0 / GS_FMIN) / 3.0;
+  const double ssfmin = sqrt(GS_FMIN) / eps2;
+#define D(i) d[(i) - 1]
+#define E(i) e[(i) - 1]
+  if (!S.inblk) {
+    if (S.l1 > n) {
+      S.done = true;
+      return;
+    }
+    if (S.l1 > 1) E(S.l1 - 1) = 0.0;
+    // _st_findsplit_fwd!
+    int msplit = n;
+    for (int m = S.l1; m <= n - 1; ++m) {
+      const double tst = fabs(E(m));
+      if (tst == 0.0) {
+        msplit = m;
+        break;
+      }
+      if (tst <= sqrt(fabs(D(m))) * sqrt(fabs(D(m + 1))) * eps) {
+        E(m) = 0.0;
+        msplit = m;
+        break;
+      }
+    }
+    S.l = S.l1;
+    S.lsave = S.l;
+    S.lend = msplit;
+    S.lendsave = msplit;
+    S.l1 = msplit + 1;
+    if (S.lend == S.l) return;
+    // opnorm(SymTridiagonal(d[l:lend], e[l:lend-1]), Inf): largest row sum
+    double anorm = 0.0;
+    for (int i = S.l; i <= S.lend; ++i) {
+      double s = 0.0;
+      if (i > S.l) s = s + fabs(E(i - 1));
+      s = s + fabs(D(i));
+      if (i < S.lend) s = s + fabs(E(i));
+      anorm = fmax(anorm, s);
+    }
+    S.anorm = anorm;
+    S.iscale = 0;
+    if (anorm == 0.0) return;
+    if (anorm > ssfmax) {
+      S.iscale = 1;
+      st_safescale(&D(S.l), S.lend - S.l + 1, anorm, ssfmax);
+      st_safescale(&E(S.l), S.lend - S.l, anorm, ssfmax);
+    } else if (anorm < ssfmin) {
+      S.iscale = 2;
+      st_safescale(&D(S.l), S.lend - S.l + 1, anorm, ssfmin);
+      st_safescale(&E(S.l), S.lend - S.l, anorm, ssfmin);
+    }
+    if (fabs(D(S.lend)) < fabs(D(S.l))) {
+      S.lend = S.lsave;
+      S.l = S.lendsave;
+    }
+    S.ql = S.lend > S.l;
+    S.inblk = true;
+    return;
+  }
+  const bool more = !S.stop && (S.ql ? (S.l <= S.lend) : (S.l >= S.lend));
+  if (!more) {
+    // end of the block: undo the scaling
+    const int ls = S.lsave, le = S.lendsave;
+    if (S.iscale == 1) {
+      st_safescale(&D(ls), le - ls + 1, ssfmax, S.anorm);
+      st_safescale(&E(ls), le - ls, ssfmax, S.anorm);
+    } else if (S.iscale == 2) {
+      st_safescale(&D(ls), le - ls + 1, ssfmin, S.anorm);
+      st_safescale(&E(ls), le - ls, ssfmin, S.anorm);
+    }
+    S.inblk = false;
+    if (S.stop) {
+      int nbad = 0;
+      for (int i = 1; i <= n - 1; ++i) nbad += (E(i) != 0.0) ? 1 : 0;
+      S.info = nbad > 0 ? nbad : 1;
+      S.done = true;
+    }
+    return;
+  }
+  int l = S.l;
+  const int lend = S.lend;
+  const bool wantz = lg != nullptr;
+  if (S.ql) {
+    // _st_QLIter! loop body
+    int msplit = lend;
+    for (int m = l; m <= lend - 1; ++m) {
+      if (E(m) * E(m) <= (eps2 * fabs(D(m))) * fabs(D(m + 1)) + safmin) {
+        msplit = m;
+        break;
+      }
+    }
+    if (msplit < lend) E(msplit) = 0.0;
+    double p = D(l);
+    if (msplit != l) {
+      if (msplit == l + 1) {
+        double rt1, rt2, cs, sn;
+        sym2x2(D(l), E(l), D(l + 1), wantz, rt1, rt2, cs, sn);
+        if (wantz) {
+          lg->c[lg->nrot] = cs;
+          lg->s[lg->nrot] = sn;
+          lg->seg[3 * lg->nseg] = l;
+          lg->seg[3 * lg->nseg + 1] = 1;
+          lg->seg[3 * lg->nseg + 2] = 1;
+          lg->nrot += 1;
+          lg->nseg += 1;
+        }
+        D(l) = rt1;
+        D(l + 1) = rt2;
+        E(l) = 0.0;
+        S.l = l + 2;
+        return;
+      }
+      if (S.sweeps >= S.maxsweeps) {
+        S.stop = true;
+        return;
+      }
+      S.sweeps += 1;
+      double g = (D(l + 1) - p) / (2.0 * E(l));
+      double r = jl_hypot(g, 1.0);
+      g = D(msplit) - p + (E(l) / (g + copysign(r, g)));
+      double s = 1.0, c = 1.0;
+      p = 0.0;
+      const int r0 = wantz ? lg->nrot : 0;
+      for (int i = msplit - 1; i >= l; --i) {
+        const double f = s * E(i);
+        const double b = c * E(i);
+        givens_re(g, f, c, s, r);
+        if (i < msplit - 1) E(i + 1) = r;
+        g = D(i + 1) - p;
+        r = (D(i) - g) * s + 2.0 * c * b;
+        p = s * r;
+        D(i + 1) = g + p;
+        g = c * r - b;
+        if (wantz) {
+          const int k = r0 + (msplit - 1 - i);
+          lg->c[k] = c;
+          lg->s[k] = -s;
+        }
+      }
+      if (wantz) {
+        lg->seg[3 * lg->nseg] = msplit - 1;
+        lg->seg[3 * lg->nseg + 1] = msplit - l;
+        lg->seg[3 * lg->nseg + 2] = -1;
+        lg->nrot += msplit - l;
+        lg->nseg += 1;
+      }
+      D(l) = D(l) - p;
+      E(l) = g;
+    } else {
+      D(l) = p;
+      S.l = l + 1;
+    }
+  } else {
+    // _st_QRIter! loop body
+    int msplit = lend;
+    for (int m = l; m >= lend + 1; --m) {
+      if (E(m - 1) * E(m - 1) <= (eps2 * fabs(D(m))) * fabs(D(m - 1)) + safmin) {
+        msplit = m;
+        break;
+      }
+    }
+    if (msplit > lend) E(msplit - 1) = 0.0;
+    double p = D(l);
+    if (msplit != l) {
+      if (msplit == l - 1) {
+        double rt1, rt2, cs, sn;
+        sym2x2(D(l - 1), E(l - 1), D(l), wantz, rt1, rt2, cs, sn);
+        if (wantz) {
+          lg->c[lg->nrot] = cs;
+          lg->s[lg->nrot] = sn;
+          lg->seg[3 * lg->nseg] = l - 1;
+          lg->seg[3 * lg->nseg + 1] = 1;
+          lg->seg[3 * lg->nseg + 2] = 1;
+          lg->nrot += 1;
+          lg->nseg += 1;
+        }
+        D(l - 1) = rt1;
+        D(l) = rt2;
+        E(l - 1) = 0.0;
+        S.l = l - 2;
+        return;
+      }
+      if (S.sweeps >= S.maxsweeps) {
+        S.stop = true;
+        return;
+      }
+      S.sweeps += 1;
+      double g = (D(l - 1) - p) / (2.0 * E(l - 1));
+      double r = jl_hypot(g, 1.0);
+      g = D(msplit) - p + (E(l - 1) / (g + copysign(r, g)));
+      double s = 1.0, c = 1.0;
+      p = 0.0;
+      const int r0 = wantz ? lg->nrot : 0;
+      for (int i = msplit; i <= l - 1; ++i) {
+        const double f = s * E(i);
+        const double b = c * E(i);
+        givens_re(g, f, c, s, r);
+        if (i != msplit) E(i - 1) = r;
+        g = D(i) - p;
+        r = (D(i + 1) - g) * s + 2.0 * c * b;
+        p = s * r;
+        D(i) = g + p;
+        g = c * r - b;
+        if (wantz) {
+          const int k = r0 + (i - msplit);
+          lg->c[k] = c;
+          lg->s[k] = s;
+        }
+      }
+      if (wantz) {
+        lg->seg[3 * lg->nseg] = msplit;
+        lg->seg[3 * lg->nseg + 1] = l - msplit;
+        lg->seg[3 * lg->nseg + 2] = 1;
+        lg->nrot += l - msplit;
+        lg->nseg += 1;
+      }
+      D(l) = D(l) - p;
+      E(l - 1) = g;
+    } else {
+      D(l) = p;
+      S.l = l - 1;
+    }
+  }
+#undef D
+#undef E
+}
+
+// Runs the producer until the log cannot take another step's rotations or
+// the recurrence is done.
+GS_DEV void st_produce(StState& S, double* d, double* e, StLog* lg, int cap) {
+  while (!S.done) {
+    if (lg && lg->nrot + st_next_bound(S) > cap) break;
+    st_step(S, d, e, lg);
+  }
+}
+
+// Real / complex Z element helpers for the appliers
+GS_DEV double st_rot1(double a1, double a2, double c, double s) {
+#pragma clang fp contract(off)
+  return a1 * c + a2 * s;
+}
+GS_DEV double st_rot2(double a1, double a2, double c, double s) {
+#pragma clang fp contract(off)
+  return (-a1) * s + a2 * c;
+}
+GS_DEV zd st_rot1(zd a1, zd a2, double c, double s) { return zmk(st_rot1(a1.x, a2.x, c, s), st_rot1(a1.y, a2.y, c, s)); }
+GS_DEV zd st_rot2(zd a1, zd a2, double c, double s) { return zmk(st_rot2(a1.x, a2.x, c, s), st_rot2(a1.y, a2.y, c, s)); }
+
+// Applies the log to one row of Z: z(k) is the element of the row in the
+// reference's (1-based) column k.
+template <class TZ, class ZA> GS_DEV void st_apply_row(const StLog& lg, ZA z) {
+  int off = 0;
+  for (int q = 0; q < lg.nseg; ++q) {
+    const int i0 = lg.seg[3 * q], cnt = lg.seg[3 * q + 1], dir = lg.seg[3 * q + 2];
+    if (dir < 0) {
+      TZ cur = z(i0 + 1);
+      for (int k = 0; k < cnt; ++k) {
+        const int i = i0 - k;
+        const double c = lg.c[off + k], s = lg.s[off + k];
+        const TZ a1 = z(i);
+        z(i + 1) = st_rot2(a1, cur, c, s);
+        cur = st_rot1(a1, cur, c, s);
+      }
+      z(i0 - cnt + 1) = cur;
+    } else {
+      TZ cur = z(i0);
+      for (int k = 0; k < cnt; ++k) {
+        const int i = i0 + k;
+        const double c = lg.c[off + k], s = lg.s[off + k];
+        const TZ a2 = z(i + 1);
+        z(i) = st_rot1(cur, a2, c, s);
+        cur = st_rot2(cur, a2, c, s);
+      }
+      z(i0 + cnt) = cur;
+    }
+    off += cnt;
+  }
+}
+
+}  // namespace gs

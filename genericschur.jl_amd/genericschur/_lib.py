@@ -21,6 +21,7 @@ GS_TIMING = 0x80
 GS_QR_BAND = 0x100
 GS_ZDIRECT = 0x200
 GS_ZLOG_TINY = 0x400
+GS_SORT = 0x800
 GS_LAUNCH_REC = 6
 
 GS_OK = 0
@@ -37,6 +38,7 @@ EXPORTS = (
     "gs_eig_vjp_batched_d", "gs_last_stats", "gs_last_launches", "gs_dgemm", "gs_schur_batched_z_noalloc",
     "gs_balance_z", "gs_balance_d", "gs_balance_apply", "gs_triangularize_d", "gs_enormalize", "gs_eig_rcond",
     "gs_set_tuning", "gs_ordschur_z", "gs_eigcond_z",
+    "gs_heev_z", "gs_heev_d", "gs_heev_max_n", "gs_hetrd_z", "gs_hetrd_d", "gs_steqr",
 )
 
 _lib = None
@@ -115,6 +117,14 @@ def lib():
         L.gs_set_tuning.argtypes = [c_vp, c_int, c_vp, c_int]
         L.gs_ordschur_z.argtypes = [c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_int]
         L.gs_eigcond_z.argtypes = [c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_int]
+        sig_heev = [c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_int, c_vp, c_vp, c_i64, c_i64, c_vp, c_int]
+        L.gs_heev_z.argtypes = sig_heev
+        L.gs_heev_d.argtypes = sig_heev
+        L.gs_heev_max_n.argtypes = [c_int]
+        sig_hetrd = [c_vp, c_i64, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_i64, c_int]
+        L.gs_hetrd_z.argtypes = sig_hetrd
+        L.gs_hetrd_d.argtypes = sig_hetrd
+        L.gs_steqr.argtypes = [c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_vp, c_int]
         sig_tv = [c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_int, c_int, c_int]
         L.gs_trevc_z.argtypes = sig_tv
         L.gs_trevc_d.argtypes = sig_tv

@@ -595,3 +595,158 @@ def eigen_(A, permute=True, scale=True, sortby="eigsortby", jvl=False, jvr=True,
         if rconde is not None:
             rconde = rconde[p]
     return Eigen(vals, v, vl, rconde)
+
+
+# ------------------------------------------------- Hermitian / real symmetric
+
+
+def _uplo(uplo):
+    if uplo not in ("L", "U"):
+        raise ArgumentError("uplo must be 'L' or 'U'")
+    return int(uplo == "U")
+
+
+def heev_max_n(complex_input):
+    """Largest order of the LDS-resident Hermitian eigensolver."""
+    return int(_lib.lib().gs_heev_max_n(int(bool(complex_input))))
+
+
+def hessenberg_hermitian(A, uplo="L", wantQ=True, device=0):
+    """_hessenberg!(Hermitian(A, uplo)) and _materializeQ (hessenberg.jl:19-95,
+    133-232): the real symmetric tridiagonal matrix (d, e) and the unitary Q
+    with A = Q SymTridiagonal(d, e) Q^H, from the uplo triangle of A."""
+    A = np.asarray(A)
+    n = _checksquare(A)
+    cplx = np.iscomplexobj(A)
+    dt = np.complex128 if cplx else np.float64
+    a = np.array(A, dtype=dt, order="F", copy=True)
+    d = np.zeros(max(n, 1))
+    e = np.zeros(max(n - 1, 1))
+    q = np.zeros((max(n, 1), max(n, 1)), dtype=dt, order="F") if wantQ else None
+    h = _lib.handle(device)
+    f = _lib.lib().gs_hetrd_z if cplx else _lib.lib().gs_hetrd_d
+    rc = f(h, n, _ptr(a), max(n, 1), _uplo(uplo), _ptr(d), _ptr(e), None if q is None else _ptr(q), max(n, 1), 0)
+    _lib.check(rc, h, "gs_hetrd")
+    return d[:n], e[:max(n - 1, 0)], (np.ascontiguousarray(q[:n, :n]) if wantQ else None)
+
+
+def _sort_host(w, V, sortby):
+    """sorteig!(λ, V, sortby) for a callable sortby (stable, as Julia's)."""
+    keys = [sortby(x) for x in w]
+    p = sorted(range(len(w)), key=lambda i: keys[i])
+    w = w[p]
+    return w, (V[:, p] if V is not None else None)
+
+
+def heev_batched(A, uplo="L", wantV=True, sortby="eigsortby", device=0, raise_on_info=True):
+    """geigen! (wantV) / geigvals! with QRIteration (GenericSchur.jl:124-146)
+    over a stack A (count, n, n) of Hermitian (complex) or real symmetric
+    matrices, one GPU launch: returns (W (count, n), V (count, n, n) or None,
+    info).  sortby: "eigsortby" (ascending, on the GPU), None, or a callable
+    (applied on the host afterwards)."""
+    A = np.asarray(A)
+    if A.ndim != 3 or A.shape[1] != A.shape[2]:
+        raise DimensionMismatch("A must be (count, n, n)")
+    count, n = A.shape[0], A.shape[1]
+    cplx = np.iscomplexobj(A)
+    dt = np.complex128 if cplx else np.float64
+    a = np.ascontiguousarray(np.transpose(A, (0, 2, 1)), dtype=dt)  # column-major per matrix
+    W = np.zeros((count, n))
+    V = np.zeros((count, n, n), dtype=dt) if wantV else None
+    info = np.zeros(max(count, 1), dtype=np.int32)
+    flags = (_lib.GS_WANTZ if wantV else 0) | (_lib.GS_SORT if sortby == "eigsortby" else 0)
+    h = _lib.handle(device)
+    f = _lib.lib().gs_heev_z if cplx else _lib.lib().gs_heev_d
+    rc = f(h, count, n, _ptr(a), max(n, 1), n * n, _uplo(uplo), _ptr(W), None if V is None else _ptr(V), max(n, 1),
+           n * n, _ptr(info), flags)
+    _lib.check(rc, h, "gs_heev")
+    if raise_on_info and count and info[:count].any():
+        raise UnconvergedException(f"{int(info[:count].max())} eigenvalues failed to converge in {30 * n} sweeps")
+    V = np.transpose(V, (0, 2, 1)) if wantV else None
+    if callable(sortby):
+        for k in range(count):
+            w2, v2 = _sort_host(W[k], V[k] if wantV else None, sortby)
+            W[k] = w2
+            if wantV:
+                V[k] = v2
+    return W, V, info[:count]
+
+
+def geigen_hermitian(A, uplo="L", sortby="eigsortby", device=0):
+    """geigen!(Hermitian(A, uplo), QRIteration(); sortby) (GenericSchur.jl:
+    124-135): Eigen(values, vectors)."""
+    A = np.asarray(A)
+    _checksquare(A)
+    W, V, _ = heev_batched(A[None], uplo=uplo, wantV=True, sortby=sortby, device=device)
+    return Eigen(W[0], V[0])
+
+
+def geigvals_hermitian(A, uplo="L", sortby="eigsortby", device=0):
+    """geigvals!(Hermitian(A, uplo), QRIteration(); sortby) (GenericSchur.jl:
+    137-146)."""
+    A = np.asarray(A)
+    _checksquare(A)
+    W, _, _ = heev_batched(A[None], uplo=uplo, wantV=False, sortby=sortby, device=device)
+    return W[0]
+
+
+def steqr_(d, e, Z=None, device=0):
+    """_gschur!(SymTridiagonal(d, e), QRIteration(), Z) (symtridiag.jl:77-172):
+    returns (eigenvalues unsorted, Z times the rotations or None, e after the
+    iteration, info).  Z (zrows x n) may be real or complex."""
+    d = np.array(d, dtype=np.float64, copy=True).reshape(-1)
+    n = d.shape[0]
+    e1 = np.zeros(max(n, 1))
+    ev = np.asarray(e, dtype=np.float64).reshape(-1)
+    if ev.shape[0] != max(n - 1, 0):
+        raise DimensionMismatch("e must have length n - 1")
+    e1[:ev.shape[0]] = ev
+    z = None
+    zc = 0
+    if Z is not None:
+        Z = np.asarray(Z)
+        if Z.ndim != 2 or Z.shape[1] != n:
+            raise DimensionMismatch("second dimension of Z must match tridiagonal A")
+        zc = int(np.iscomplexobj(Z))
+        z = np.array(Z, dtype=np.complex128 if zc else np.float64, order="F", copy=True)
+    info = np.zeros(1, dtype=np.int32)
+    h = _lib.handle(device)
+    zr = z.shape[0] if z is not None else 0
+    rc = _lib.lib().gs_steqr(h, n, _ptr(d) if n else None, _ptr(e1), None if z is None else _ptr(z), max(zr, 1), zr,
+                             zc, _ptr(info), 0)
+    _lib.check(rc, h, "gs_steqr")
+    return d, (np.ascontiguousarray(z) if z is not None else None), e1[:max(n - 1, 0)], int(info[0])
+
+
+def gschur_symtridiag(d, e, Z=None, device=0):
+    """gschur!(H::Hessenberg{<:Any, <:SymTridiagonal}, Z) (symtridiag.jl:14-41)
+    on the tridiagonal (d, e) with the transformation Z: Schur(diagm(values),
+    Z updated, values) (T and Z empty when Z is None)."""
+    w, z, _, info = steqr_(d, e, Z, device=device)
+    if info:
+        raise UnconvergedException(f"{info} eigenvalues failed to converge")
+    if z is None:
+        return Schur(np.zeros((0, 0)), np.zeros((0, 0)), w)
+    T = np.zeros((w.shape[0], w.shape[0]), dtype=z.dtype)
+    T[np.diag_indices(w.shape[0])] = w
+    return Schur(T, z, w)
+
+
+def geigen_symtridiag(d, e, device=0):
+    """geigen!(A::SymTridiagonal, QRIteration()) (symtridiag.jl:43-49):
+    Eigen(values ascending, vectors)."""
+    n = np.asarray(d).reshape(-1).shape[0]
+    w, V, _, info = steqr_(d, e, np.eye(n), device=device)
+    if info:
+        raise UnconvergedException(f"{info} eigenvalues failed to converge")
+    w2, V2 = _sort_host(w, V, lambda x: x)
+    return Eigen(w2, V2)
+
+
+def geigvals_symtridiag(d, e, device=0):
+    """geigvals!(A::SymTridiagonal, QRIteration()) (symtridiag.jl:51-55):
+    the eigenvalues in the order the iteration leaves them."""
+    w, _, _, info = steqr_(d, e, None, device=device)
+    if info:
+        raise UnconvergedException(f"{info} eigenvalues failed to converge")
+    return w

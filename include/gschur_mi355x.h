@@ -61,6 +61,8 @@ typedef struct gs_handle gs_handle;
 #define GS_ZLOG_TINY 0x400   /* batched complex, testing: a Z log of one sweep */
                              /* per matrix, so that the QR kernel applies the  */
                              /* log to Z itself before almost every sweep      */
+#define GS_SORT 0x800       /* Hermitian eigensolver: eigenvalues ascending, */
+                             /* eigenvectors alike (sorteig! with eigsortby)  */
 #define GS_TIMING 0x80       /* record launch durations for gs_last_launches:  */
                              /* batched with GS_DEVICE_PTRS (launched eagerly, */
                              /* no graph, events around every kernel on its    */
@@ -259,6 +261,48 @@ int gs_ordschur_z(gs_handle* h, int64_t n, double* T, int64_t ldt, double* Z, in
  * (GS_E_ARG otherwise; the reference's estimator is undefined there).
  * out2 is host memory. */
 int gs_eigcond_z(gs_handle* h, int64_t n, const double* T, int64_t ldt, int64_t nsub, double* out2, int flags);
+
+/* ---- Hermitian / real symmetric eigenproblem (QR iteration) ---- */
+
+/* geigen!(A::Hermitian, QRIteration()) / geigvals!(A) — GenericSchur.jl:
+ * 124-146: _hehessenberg! (hessenberg.jl:27-95, LAPACK xHETD2 with
+ * rank2update!), _materializeQ (hessenberg.jl:133-232), gschur!(H, V)
+ * (symtridiag.jl:14-41) with the QL/QR iteration _gschur!(SymTridiagonal,
+ * QRIteration(), V) (symtridiag.jl:77-427, LAPACK xSTEQR), and, with
+ * GS_SORT, sorteig!(λ, V, eigsortby) (a stable ascending sort).
+ * `count` matrices of order n (A: lda, strideA elements between matrices;
+ * only the lower (upper = 0) or upper (upper = 1) triangle is read; A is not
+ * modified).  W receives count x n eigenvalues (stride n doubles); with
+ * GS_WANTZ, V (ldv, strideV) the eigenvectors (complex for _z).  info[k]
+ * (may be NULL): 0, or the number of unconverged off-diagonal entries when
+ * the iteration stopped after 30 n sweeps (the reference's loop has no
+ * limit).  One workgroup per matrix, the matrix in LDS: n <= gs_heev_max_n
+ * (GS_E_UNSUPPORTED above).  Host pointers are staged through device
+ * buffers; with GS_DEVICE_PTRS, A, W, V and info are device memory. */
+int gs_heev_z(gs_handle* h, int64_t count, int64_t n, const double* A, int64_t lda, int64_t strideA, int upper,
+              double* W, double* V, int64_t ldv, int64_t strideV, int32_t* info, int flags);
+int gs_heev_d(gs_handle* h, int64_t count, int64_t n, const double* A, int64_t lda, int64_t strideA, int upper,
+              double* W, double* V, int64_t ldv, int64_t strideV, int32_t* info, int flags);
+int gs_heev_max_n(int is_complex);
+
+/* _hehessenberg!(Hermitian(A, uplo)) + _materializeQ(H) — hessenberg.jl:
+ * 19-95, 133-232: the real symmetric tridiagonal matrix (d: n, e: n - 1) and
+ * Q (n x n, ldq; NULL: not formed) with A = Q T Q^H, from the lower (upper =
+ * 0) or upper triangle of A (not modified).  n <= gs_heev_max_n. */
+int gs_hetrd_z(gs_handle* h, int64_t n, const double* A, int64_t lda, int upper, double* d, double* e, double* Q,
+               int64_t ldq, int flags);
+int gs_hetrd_d(gs_handle* h, int64_t n, const double* A, int64_t lda, int upper, double* d, double* e, double* Q,
+               int64_t ldq, int flags);
+
+/* _gschur!(SymTridiagonal(d, e), QRIteration(), Z) — symtridiag.jl:77-427
+ * (the step of gschur!(H::Hessenberg{SymTridiagonal}, Z), symtridiag.jl:
+ * 14-41, and of geigen!(::SymTridiagonal), :43-49): d (n) is overwritten by
+ * the eigenvalues (unsorted), e (n - 1) destroyed, and the zrows x n matrix Z
+ * (ldz; complex when zcomplex; NULL: eigenvalues only) multiplied by the
+ * rotations, bit for bit as the reference's operation order gives them.
+ * info[0] as for gs_heev_z.  n <= 2048. */
+int gs_steqr(gs_handle* h, int64_t n, double* d, double* e, double* Z, int64_t ldz, int64_t zrows, int zcomplex,
+             int32_t* info, int flags);
 
 /* Statistics of the last Schur call on this handle: out[0] = QR sweeps,
  * out[1] = bulge steps, out[2] = kernel time in microseconds (events on the

@@ -2228,4 +2228,570 @@ template <class R> R subspacesep(long n, const cx<R>* tp, long ldt, long nsub) {
   return scale / est;
 }
 
+
+// ===================================================================
+// Hermitian / real symmetric path: _hehessenberg! (hessenberg.jl:19-95)
+// with rank2update! (:97-130), _materializeQ (:133-232), and the
+// symmetric tridiagonal QL/QR iteration (symtridiag.jl:14-427).
+// ===================================================================
+
+// Julia's Base.Math hypot(x, y) for Float64 (the fused multiply-add branch,
+// correctly rounded); other real types use their library hypot.
+template <class R> R jhypot(R x, R y) { return rt<R>::hypot(x, y); }
+template <> inline double jhypot<double>(double x, double y) {
+  double ax = std::fabs(x), ay = std::fabs(y);
+  if (std::isinf(ax) || std::isinf(ay)) return HUGE_VAL;
+  if (ay > ax) std::swap(ax, ay);
+  if (ay <= ax * std::sqrt(std::numeric_limits<double>::epsilon() / 2.0)) return ax;
+  double scale = std::numeric_limits<double>::epsilon() * std::sqrt(std::numeric_limits<double>::min());
+  if (ax > std::sqrt(std::numeric_limits<double>::max() / 2.0)) {
+    ax = ax * scale;
+    ay = ay * scale;
+    scale = 1.0 / scale;
+  } else if (ay < std::sqrt(std::numeric_limits<double>::min())) {
+    ax = ax / scale;
+    ay = ay / scale;
+  } else {
+    scale = 1.0;
+  }
+  double h = std::sqrt(std::fma(ax, ax, ay * ay));
+  const double hsq = h * h, axsq = ax * ax;
+  h -= (std::fma(-ay, ay, hsq - axsq) + std::fma(h, h, -hsq) - std::fma(ax, ax, -axsq)) / (2.0 * h);
+  return h * scale;
+}
+
+// LinearAlgebra.givensAlgorithm(f, g) for real arguments (LAPACK 3.1 xLARTG:
+// safmn2 = floatmin2 scaling, and the sign fix when |f| > |g| and cs < 0).
+template <class R> void givens_re(R f, R g, R& cs, R& sn, R& r) {
+  const R safmn2 = floatmin2<R>();
+  const R safmx2 = R(1.0) / safmn2;
+  if (g == R(0.0)) {
+    cs = R(1.0);
+    sn = R(0.0);
+    r = f;
+    return;
+  }
+  if (f == R(0.0)) {
+    cs = R(0.0);
+    sn = R(1.0);
+    r = g;
+    return;
+  }
+  R f1 = f, g1 = g;
+  R sc = rmax(rt<R>::abs(f1), rt<R>::abs(g1));
+  if (sc >= safmx2) {
+    int count = 0;
+    while (true) {
+      count += 1;
+      f1 = f1 * safmn2;
+      g1 = g1 * safmn2;
+      sc = rmax(rt<R>::abs(f1), rt<R>::abs(g1));
+      if (sc < safmx2 || count >= 20) break;
+    }
+    r = rt<R>::sqrt(f1 * f1 + g1 * g1);
+    cs = f1 / r;
+    sn = g1 / r;
+    for (int i = 0; i < count; ++i) r = r * safmx2;
+  } else if (sc <= safmn2) {
+    int count = 0;
+    while (true) {
+      count += 1;
+      f1 = f1 * safmx2;
+      g1 = g1 * safmx2;
+      sc = rmax(rt<R>::abs(f1), rt<R>::abs(g1));
+      if (sc > safmn2) break;
+    }
+    r = rt<R>::sqrt(f1 * f1 + g1 * g1);
+    cs = f1 / r;
+    sn = g1 / r;
+    for (int i = 0; i < count; ++i) r = r * safmn2;
+  } else {
+    r = rt<R>::sqrt(f1 * f1 + g1 * g1);
+    cs = f1 / r;
+    sn = g1 / r;
+  }
+  if (rt<R>::abs(f) > rt<R>::abs(g) && cs < R(0.0)) {
+    cs = -cs;
+    sn = -sn;
+    r = -r;
+  }
+}
+
+// rank2update!(A, -1, x, y, Val(:L)) (hessenberg.jl:115-130) on the m x m
+// block at a: A <- A - x y^H - y x^H over the lower triangle, diagonal kept
+// real.
+template <class T> void rank2update_l(long m, T* a, long lda, const T* x, const T* y) {
+  Mat<T> A{a, lda};
+  for (long j = 1; j <= m; ++j) {
+    if (x[j - 1] == T() && y[j - 1] == T()) {
+      A(j, j) = T(real(A(j, j)));
+      continue;
+    }
+    const T t1 = -conj(y[j - 1]);
+    const T t2 = conj(-x[j - 1]);
+    A(j, j) = T(real(A(j, j)) + real(x[j - 1] * t1 + y[j - 1] * t2));
+    for (long i = j + 1; i <= m; ++i) A(i, j) = A(i, j) + (x[i - 1] * t1 + y[i - 1] * t2);
+  }
+}
+// rank2update!(A, -1, x, y, Val(:U)) (hessenberg.jl:97-113)
+template <class T> void rank2update_u(long m, T* a, long lda, const T* x, const T* y) {
+  Mat<T> A{a, lda};
+  for (long j = 1; j <= m; ++j) {
+    if (x[j - 1] == T() && y[j - 1] == T()) {
+      A(j, j) = T(real(A(j, j)));
+      continue;
+    }
+    const T t1 = -conj(y[j - 1]);
+    const T t2 = conj(-x[j - 1]);
+    for (long i = 1; i <= j - 1; ++i) A(i, j) = A(i, j) + (x[i - 1] * t1 + y[i - 1] * t2);
+    A(j, j) = T(real(A(j, j)) + real(x[j - 1] * t1 + y[j - 1] * t2));
+  }
+}
+
+// w = tau * A x for the Hermitian m x m block at a given by its lower
+// (upper = false) or upper triangle, in the order of the reference BLAS
+// xHEMV / xSYMV (the reference calls mul!(w, Hermitian(A, uplo), x, tau,
+// false), which BLAS evaluates in its own order).
+template <class T> void hemv(long m, const T* a, long lda, bool upper, T tau, const T* x, T* w) {
+  Mat<const T> A{a, lda};
+  for (long i = 0; i < m; ++i) w[i] = T();
+  for (long j = 1; j <= m; ++j) {
+    const T t1 = tau * x[j - 1];
+    T t2 = T();
+    if (upper) {
+      for (long i = 1; i <= j - 1; ++i) {
+        w[i - 1] = w[i - 1] + t1 * A(i, j);
+        t2 = t2 + conj(A(i, j)) * x[i - 1];
+      }
+      w[j - 1] = w[j - 1] + t1 * real(A(j, j)) + tau * t2;
+    } else {
+      w[j - 1] = w[j - 1] + t1 * real(A(j, j));
+      for (long i = j + 1; i <= m; ++i) {
+        w[i - 1] = w[i - 1] + t1 * A(i, j);
+        t2 = t2 + conj(A(i, j)) * x[i - 1];
+      }
+      w[j - 1] = w[j - 1] + tau * t2;
+    }
+  }
+}
+
+template <class T> T dotc(long m, const T* x, const T* y) {
+  T s = T();
+  for (long i = 0; i < m; ++i) s = s + conj(x[i]) * y[i];
+  return s;
+}
+
+// _hehessenberg!(Hermitian(A, uplo)) (hessenberg.jl:27-95; LAPACK xHETD2):
+// A (n x n, lda) holds the Hermitian matrix in its lower (upper = false) or
+// upper triangle.  On exit d (n) and e (n-1) hold the real symmetric
+// tridiagonal matrix, tau (n-1) the reflector scalars, and A the reflectors
+// (L: reflector i in A[i+2:n, i]; U: in A[1:i-1, i+1]).
+template <class T> void hetrd(long n, T* a, long lda, bool upper, T* tau, typename real_of<T>::type* d,
+                              typename real_of<T>::type* e) {
+  typedef typename real_of<T>::type R;
+  Mat<T> A{a, lda};
+  if (n == 0) return;
+  std::vector<T> xi((size_t)n), w((size_t)n);
+  if (!upper) {
+    A(1, 1) = T(real(A(1, 1)));
+    for (long i = 1; i <= n - 1; ++i) {
+      const long m = n - i;
+      T* x = &A(i + 1, i);
+      const T ti = reflector(m, x, 1);
+      e[i - 1] = real(x[0]);
+      if (!(ti == T())) {
+        x[0] = T(1.0);
+        hemv<T>(m, &A(i + 1, i + 1), lda, false, ti, x, w.data());
+        const T alpha = -(ti / R(2.0)) * dotc<T>(m, w.data(), x);
+        for (long k = 0; k < m; ++k) w[k] = w[k] + alpha * x[k];
+        rank2update_l<T>(m, &A(i + 1, i + 1), lda, x, w.data());
+      } else {
+        A(i + 1, i + 1) = T(real(A(i + 1, i + 1)));
+      }
+      A(i + 1, i) = T(e[i - 1]);
+      d[i - 1] = real(A(i, i));
+      tau[i - 1] = ti;
+    }
+    d[n - 1] = real(A(n, n));
+  } else {
+    A(n, n) = T(real(A(n, n)));
+    for (long i = n - 1; i >= 1; --i) {
+      xi[0] = A(i, i + 1);
+      for (long k = 1; k <= i - 1; ++k) xi[k] = A(k, i + 1);
+      const T ti = reflector(i, xi.data(), 1);
+      e[i - 1] = real(xi[0]);
+      for (long k = 1; k <= i - 1; ++k) A(k, i + 1) = xi[k];
+      if (!(ti == T())) {
+        A(i, i + 1) = T(1.0);
+        const T* x = &A(1, i + 1);
+        hemv<T>(i, &A(1, 1), lda, true, ti, x, w.data());
+        const T alpha = -(ti / R(2.0)) * dotc<T>(i, w.data(), x);
+        for (long k = 0; k < i; ++k) w[k] = w[k] + alpha * x[k];
+        rank2update_u<T>(i, &A(1, 1), lda, x, w.data());
+      } else {
+        A(i, i) = T(real(A(i, i)));
+      }
+      A(i, i + 1) = T(e[i - 1]);
+      d[i] = real(A(i + 1, i + 1));
+      tau[i - 1] = ti;
+    }
+    d[0] = real(A(1, 1));
+  }
+}
+
+// _materializeQ(H, Val(:U)) (hessenberg.jl:198-232): Q with A = Q T Q^H
+// from the reflectors of the upper-triangle reduction.
+template <class T> void materialize_q_u(long n, const T* a, long lda, const T* tau, T* q, long ldq) {
+  Mat<const T> F{a, lda};
+  Mat<T> A{q, ldq};
+  for (long j = 1; j <= n; ++j)
+    for (long i = 1; i <= n; ++i) A(i, j) = F(i, j);
+  for (long j = 1; j <= n - 1; ++j) {
+    for (long i = 1; i <= j - 1; ++i) A(i, j) = A(i, j + 1);
+    for (long i = j; i <= n; ++i) A(i, j) = T();
+  }
+  for (long i = 1; i <= n - 1; ++i) A(i, n) = T();
+  A(n, n) = T(1.0);
+  std::vector<T> w((size_t)std::max<long>(n - 1, 1));
+  for (long i = 1; i <= n - 1; ++i) {
+    A(i, i) = T(1.0);
+    const T ti = tau[i - 1];
+    for (long j = 1; j <= i - 1; ++j) w[j - 1] = dotc<T>(i, &A(1, j), &A(1, i));
+    for (long j = 1; j <= i - 1; ++j) {
+      const T t = ti * conj(w[j - 1]);
+      for (long k = 1; k <= i; ++k) A(k, j) = A(k, j) - A(k, i) * t;
+    }
+    for (long k = 1; k <= i - 1; ++k) A(k, i) = A(k, i) * (-ti);
+    A(i, i) = T(1.0) - ti;
+    for (long k = i + 1; k <= n; ++k) A(k, i) = T();
+  }
+}
+
+// _dgz_sym2x2(a, b, c, wantvec) (symtridiag.jl:375-427; LAPACK xLAEV2):
+// eigenvalues rt1 (larger modulus), rt2 and the rotation (cs, sn) of
+// [a b; b c].
+template <class R> void sym2x2(R a, R b, R c, bool wantvec, R& rt1, R& rt2, R& cs1, R& sn1) {
+  const R sm = a + c, df = a - c, adf = rt<R>::abs(df), tb = R(2.0) * b, ab = rt<R>::abs(tb);
+  R acmx, acmn;
+  if (rt<R>::abs(a) > rt<R>::abs(c)) {
+    acmx = a;
+    acmn = c;
+  } else {
+    acmx = c;
+    acmn = a;
+  }
+  const R rt_ = jhypot<R>(ab, adf);
+  int sgn1;
+  if (sm < R(0.0)) {
+    rt1 = (sm - rt_) / R(2.0);
+    sgn1 = -1;
+    rt2 = (acmx / rt1) * acmn - (b / rt1) * b;
+  } else if (sm > R(0.0)) {
+    rt1 = (sm + rt_) / R(2.0);
+    sgn1 = 1;
+    rt2 = (acmx / rt1) * acmn - (b / rt1) * b;
+  } else {
+    rt1 = rt_ / R(2.0);
+    rt2 = -rt_ / R(2.0);
+    sgn1 = 1;
+  }
+  cs1 = R(0.0);
+  sn1 = R(0.0);
+  if (!wantvec) return;
+  R cs;
+  int sgn2;
+  if (df > R(0.0)) {
+    cs = df + rt_;
+    sgn2 = 1;
+  } else {
+    cs = df - rt_;
+    sgn2 = -1;
+  }
+  const R acs = rt<R>::abs(cs);
+  if (acs > ab) {
+    const R ct = -tb / cs;
+    sn1 = R(1.0) / rt<R>::sqrt(R(1.0) + ct * ct);
+    cs1 = ct * sn1;
+  } else {
+    if (ab == R(0.0)) {
+      cs1 = R(1.0);
+      sn1 = R(0.0);
+    } else {
+      const R tn = -cs / tb;
+      cs1 = R(1.0) / rt<R>::sqrt(R(1.0) + tn * tn);
+      sn1 = tn * cs1;
+    }
+  }
+  if (sgn1 == sgn2) {
+    const R tn = cs1;
+    cs1 = -sn1;
+    sn1 = tn;
+  }
+}
+
+// rmul!(Z, Givens(i, i+1, c, s)') on columns i, i+1 (1-based) of the
+// zrows x n matrix Z: Z[:, i] = c a1 + s a2, Z[:, i+1] = -s a1 + c a2.
+template <class R, class TZ> void rot_cols(TZ* z, long ldz, long zrows, long i, R c, R s) {
+  Mat<TZ> Z{z, ldz};
+  for (long r = 1; r <= zrows; ++r) {
+    const TZ a1 = Z(r, i), a2 = Z(r, i + 1);
+    Z(r, i) = a1 * c + a2 * s;
+    Z(r, i + 1) = (-a1) * s + a2 * c;
+  }
+}
+
+// _st_findsplit_fwd! (symtridiag.jl:354-372)
+template <class R> long st_findsplit(R* d, R* e, long l1, long n) {
+  long msplit = n;
+  for (long m = l1; m <= n - 1; ++m) {
+    const R tst = rt<R>::abs(e[m - 1]);
+    if (tst == R(0.0)) {
+      msplit = m;
+      break;
+    }
+    if (tst <= rt<R>::sqrt(rt<R>::abs(d[m - 1])) * rt<R>::sqrt(rt<R>::abs(d[m])) * rt<R>::eps()) {
+      e[m - 1] = R(0.0);
+      msplit = m;
+      break;
+    }
+  }
+  return msplit;
+}
+
+// _gschur!(A::SymTridiagonal, QRIteration(), Z) (symtridiag.jl:77-172;
+// LAPACK xSTEQR) with _st_QLIter! (:175-262) and _st_QRIter! (:265-352):
+// d (n) is overwritten with the eigenvalues (unsorted), e (n-1) destroyed,
+// and the zrows x n matrix Z (real or complex; null: none) multiplied by
+// the rotations.  The reference's iteration counter never advances
+// (symtridiag.jl:141), so it has no limit; this restatement stops after
+// maxsweeps QL/QR sweeps in all (30 n, LAPACK's limit, by default) and
+// returns the number of nonzero off-diagonal entries then left (LAPACK's
+// info), 0 on convergence.  *sweeps receives the sweep count.
+template <class R, class TZ>
+long steqr(long n, R* d, R* e, TZ* z, long ldz, long zrows, long maxsweeps, long* sweeps) {
+  const bool wantz = z != nullptr && zrows > 0;
+  const R epsT = rt<R>::eps(), eps2 = epsT * epsT;
+  const R safmin = rt<R>::fmin(), safmax = R(1.0) / safmin;
+  const R ssfmax = rt<R>::sqrt(safmax) / R(3.0);
+  const R ssfmin = rt<R>::sqrt(safmin) / eps2;
+  long niter = 0;
+  if (maxsweeps < 0) maxsweeps = 30 * n;
+  if (sweeps) *sweeps = 0;
+  std::vector<R> gc((size_t)std::max<long>(n, 1)), gs((size_t)std::max<long>(n, 1));
+  // 1-based views
+  auto D = [&](long i) -> R& { return d[i - 1]; };
+  auto E = [&](long i) -> R& { return e[i - 1]; };
+  long l1 = 1;
+  while (l1 <= n) {
+    if (l1 > 1) E(l1 - 1) = R(0.0);
+    const long msplit0 = st_findsplit<R>(d, e, l1, n);
+    long l = l1, lsave = l, lend = msplit0, lendsave = lend;
+    l1 = msplit0 + 1;
+    if (lend == l) continue;
+    // opnorm(SymTridiagonal(d[l:lend], e[l:lend-1]), Inf): largest row sum
+    R anorm(0.0);
+    for (long i = l; i <= lend; ++i) {
+      R s(0.0);
+      if (i > l) s = s + rt<R>::abs(E(i - 1));
+      s = s + rt<R>::abs(D(i));
+      if (i < lend) s = s + rt<R>::abs(E(i));
+      anorm = rmax(anorm, s);
+    }
+    int iscale = 0;
+    if (anorm == R(0.0)) continue;
+    if (anorm > ssfmax) {
+      iscale = 1;
+      safescale<R>(lend - l + 1, &D(l), anorm, ssfmax);
+      safescale<R>(lend - l, &E(l), anorm, ssfmax);
+    } else if (anorm < ssfmin) {
+      iscale = 2;
+      safescale<R>(lend - l + 1, &D(l), anorm, ssfmin);
+      safescale<R>(lend - l, &E(l), anorm, ssfmin);
+    }
+    if (rt<R>::abs(D(lend)) < rt<R>::abs(D(l))) {
+      lend = lsave;
+      l = lendsave;
+    }
+    bool stop = false;
+    if (lend > l) {
+      // QL iteration
+      while (l <= lend) {
+        long msplit = lend;
+        for (long m = l; m <= lend - 1; ++m) {
+          if (E(m) * E(m) <= (eps2 * rt<R>::abs(D(m))) * rt<R>::abs(D(m + 1)) + safmin) {
+            msplit = m;
+            break;
+          }
+        }
+        if (msplit < lend) E(msplit) = R(0.0);
+        R p = D(l);
+        if (msplit != l) {
+          if (msplit == l + 1) {
+            R rt1, rt2, cs, sn;
+            sym2x2<R>(D(l), E(l), D(l + 1), wantz, rt1, rt2, cs, sn);
+            if (wantz) rot_cols<R, TZ>(z, ldz, zrows, l, cs, sn);
+            D(l) = rt1;
+            D(l + 1) = rt2;
+            E(l) = R(0.0);
+            l += 2;
+            continue;
+          }
+          if (niter >= maxsweeps) {
+            stop = true;
+            break;
+          }
+          niter += 1;
+          R g = (D(l + 1) - p) / (R(2.0) * E(l));
+          R r = jhypot<R>(g, R(1.0));
+          g = D(msplit) - p + (E(l) / (g + rt<R>::copysign(r, g)));
+          R s(1.0), c(1.0);
+          p = R(0.0);
+          for (long i = msplit - 1; i >= l; --i) {
+            const R f = s * E(i);
+            const R b = c * E(i);
+            givens_re<R>(g, f, c, s, r);
+            if (i < msplit - 1) E(i + 1) = r;
+            g = D(i + 1) - p;
+            r = (D(i) - g) * s + R(2.0) * c * b;
+            p = s * r;
+            D(i + 1) = g + p;
+            g = c * r - b;
+            gc[i - 1] = c;
+            gs[i - 1] = -s;
+          }
+          if (wantz)
+            for (long i = msplit - 1; i >= l; --i) rot_cols<R, TZ>(z, ldz, zrows, i, gc[i - 1], gs[i - 1]);
+          D(l) = D(l) - p;
+          E(l) = g;
+        } else {
+          D(l) = p;
+          l += 1;
+        }
+      }
+    } else {
+      // QR iteration
+      while (l >= lend) {
+        long msplit = lend;
+        for (long m = l; m >= lend + 1; --m) {
+          if (E(m - 1) * E(m - 1) <= (eps2 * rt<R>::abs(D(m))) * rt<R>::abs(D(m - 1)) + safmin) {
+            msplit = m;
+            break;
+          }
+        }
+        if (msplit > lend) E(msplit - 1) = R(0.0);
+        R p = D(l);
+        if (msplit != l) {
+          if (msplit == l - 1) {
+            R rt1, rt2, cs, sn;
+            sym2x2<R>(D(l - 1), E(l - 1), D(l), wantz, rt1, rt2, cs, sn);
+            if (wantz) rot_cols<R, TZ>(z, ldz, zrows, l - 1, cs, sn);
+            D(l - 1) = rt1;
+            D(l) = rt2;
+            E(l - 1) = R(0.0);
+            l -= 2;
+            continue;
+          }
+          if (niter >= maxsweeps) {
+            stop = true;
+            break;
+          }
+          niter += 1;
+          R g = (D(l - 1) - p) / (R(2.0) * E(l - 1));
+          R r = jhypot<R>(g, R(1.0));
+          g = D(msplit) - p + (E(l - 1) / (g + rt<R>::copysign(r, g)));
+          R s(1.0), c(1.0);
+          p = R(0.0);
+          for (long i = msplit; i <= l - 1; ++i) {
+            const R f = s * E(i);
+            const R b = c * E(i);
+            givens_re<R>(g, f, c, s, r);
+            if (i != msplit) E(i - 1) = r;
+            g = D(i) - p;
+            r = (D(i + 1) - g) * s + R(2.0) * c * b;
+            p = s * r;
+            D(i) = g + p;
+            g = c * r - b;
+            gc[i - 1] = c;
+            gs[i - 1] = s;
+          }
+          if (wantz)
+            for (long i = msplit; i <= l - 1; ++i) rot_cols<R, TZ>(z, ldz, zrows, i, gc[i - 1], gs[i - 1]);
+          D(l) = D(l) - p;
+          E(l - 1) = g;
+        } else {
+          D(l) = p;
+          l -= 1;
+        }
+      }
+    }
+    if (iscale == 1) {
+      safescale<R>(lendsave - lsave + 1, &D(lsave), ssfmax, anorm);
+      safescale<R>(lendsave - lsave, &E(lsave), ssfmax, anorm);
+    } else if (iscale == 2) {
+      safescale<R>(lendsave - lsave + 1, &D(lsave), ssfmin, anorm);
+      safescale<R>(lendsave - lsave, &E(lsave), ssfmin, anorm);
+    }
+    if (stop) {
+      if (sweeps) *sweeps = niter;
+      long nbad = 0;
+      for (long i = 1; i <= n - 1; ++i) nbad += (E(i) != R(0.0)) ? 1 : 0;
+      return nbad > 0 ? nbad : 1;
+    }
+  }
+  if (sweeps) *sweeps = niter;
+  return 0;
+}
+
+// Julia's isless on floats: a total order with -0.0 < 0.0 and NaN last.
+template <class R> bool jl_isless(R a, R b) {
+  const bool an = rt<R>::isnan(a), bn = rt<R>::isnan(b);
+  if (an || bn) return !an && bn;
+  if (a == b) return std::signbit(rt<R>::to_double(a)) && !std::signbit(rt<R>::to_double(b));
+  return a < b;
+}
+
+// sorteig!(λ, V, eigsortby) for real eigenvalues (LinearAlgebra; called by
+// geigen!, GenericSchur.jl:133): a stable ascending sort under isless,
+// the columns of V (vrows x n) permuted alike; no change when sorted.
+template <class R, class TV> void sorteig(long n, R* w, TV* v, long ldv, long vrows) {
+  std::vector<long> p((size_t)n);
+  for (long i = 0; i < n; ++i) p[i] = i;
+  std::stable_sort(p.begin(), p.end(), [&](long a, long b) { return jl_isless<R>(w[a], w[b]); });
+  std::vector<R> w2((size_t)n);
+  for (long i = 0; i < n; ++i) w2[i] = w[p[i]];
+  for (long i = 0; i < n; ++i) w[i] = w2[i];
+  if (!v) return;
+  std::vector<TV> tmp((size_t)(vrows * n));
+  for (long j = 0; j < n; ++j)
+    for (long r = 0; r < vrows; ++r) tmp[r + j * vrows] = v[r + p[j] * ldv];
+  for (long j = 0; j < n; ++j)
+    for (long r = 0; r < vrows; ++r) v[r + j * ldv] = tmp[r + j * vrows];
+}
+
+// geigen! / geigvals! for a Hermitian (or real symmetric) matrix with
+// QRIteration (GenericSchur.jl:124-146): _hehessenberg!, _materializeQ,
+// gschur!(H, V) (symtridiag.jl:14-41) and sorteig! (when sort).  A (n x n)
+// is overwritten by the reduction; w (n) receives the eigenvalues, v (n x n,
+// null: eigenvalues only) the eigenvectors.  Returns steqr's info.
+template <class T> long heev(long n, T* a, long lda, bool upper, typename real_of<T>::type* w, T* v, long ldv,
+                             bool sort, long* sweeps) {
+  typedef typename real_of<T>::type R;
+  if (sweeps) *sweeps = 0;
+  if (n == 0) return 0;
+  std::vector<T> tau((size_t)std::max<long>(n - 1, 1));
+  std::vector<R> e((size_t)std::max<long>(n - 1, 1));
+  hetrd<T>(n, a, lda, upper, tau.data(), w, e.data());
+  if (v) {
+    if (upper) materialize_q_u<T>(n, a, lda, tau.data(), v, ldv);
+    else materialize_q<T>(n, a, lda, tau.data(), v, ldv);
+  }
+  const long info = steqr<R, T>(n, w, e.data(), v, ldv, v ? n : 0, -1, sweeps);
+  if (sort) {
+    bool sorted = true;
+    for (long i = 1; i < n; ++i) sorted = sorted && !jl_isless<R>(w[i], w[i - 1]);
+    if (!sorted) sorteig<R, T>(n, w, v, ldv, v ? n : 0);
+  }
+  return info;
+}
+
 }  // namespace orc

@@ -16,7 +16,7 @@ typedef cx<double> zc;
 
 extern "C" {
 
-int orc_abi_version() { return 3; }
+int orc_abi_version() { return 4; }
 
 // gschur!(A::StridedMatrix{ComplexF64}) — GenericSchur.jl:350-372
 int orc_schur_z(long n, double* a, long lda, double* t, long ldt, double* z, long ldz, double* w, int wantz,
@@ -267,4 +267,58 @@ void orc_givens_z(const double* f, const double* g, double* out5) {
   out5[3] = r.re;
   out5[4] = r.im;
 }
+
+// ---- Hermitian / real symmetric path ----
+// _hehessenberg! (hessenberg.jl:27-95) + _materializeQ (:133-232): on exit a
+// holds the reflectors, d/e the tridiagonal, tau the scalars, q (n x n, may
+// be NULL) the transformation.  upper selects the triangle of a that is read.
+void orc_hetrd_z(long n, double* a, long lda, int upper, double* d, double* e, double* tau, double* q, long ldq) {
+  hetrd<zc>(n, (zc*)a, lda, upper != 0, (zc*)tau, d, e);
+  if (q) {
+    if (upper) materialize_q_u<zc>(n, (zc*)a, lda, (zc*)tau, (zc*)q, ldq);
+    else materialize_q<zc>(n, (zc*)a, lda, (zc*)tau, (zc*)q, ldq);
+  }
+}
+void orc_hetrd_d(long n, double* a, long lda, int upper, double* d, double* e, double* tau, double* q, long ldq) {
+  hetrd<double>(n, a, lda, upper != 0, tau, d, e);
+  if (q) {
+    if (upper) materialize_q_u<double>(n, a, lda, tau, q, ldq);
+    else materialize_q<double>(n, a, lda, tau, q, ldq);
+  }
+}
+// _gschur!(SymTridiagonal(d, e), QRIteration(), Z) (symtridiag.jl:77-172);
+// z (zrows x n, complex when zcplx) may be NULL; maxsweeps < 0: 30 n.
+long orc_steqr(long n, double* d, double* e, double* z, long ldz, long zrows, int zcplx, long maxsweeps,
+               long* sweeps) {
+  if (zcplx) return steqr<double, zc>(n, d, e, (zc*)z, ldz, z ? zrows : 0, maxsweeps, sweeps);
+  return steqr<double, double>(n, d, e, z, ldz, z ? zrows : 0, maxsweeps, sweeps);
+}
+// geigen! / geigvals! (GenericSchur.jl:124-146) on a Hermitian / real
+// symmetric matrix; v may be NULL.
+long orc_heev_z(long n, double* a, long lda, int upper, double* w, double* v, long ldv, int sort, long* sweeps) {
+  return heev<zc>(n, (zc*)a, lda, upper != 0, w, (zc*)v, ldv, sort != 0, sweeps);
+}
+long orc_heev_d(long n, double* a, long lda, int upper, double* w, double* v, long ldv, int sort, long* sweeps) {
+  return heev<double>(n, a, lda, upper != 0, w, v, ldv, sort != 0, sweeps);
+}
+// heev over a batch of uniform order n (column-major, stride n*n), OpenMP
+// over matrices; the CPU baseline of the batched Hermitian eigensolver.
+void orc_heev_batched(long count, long n, int cplx, double* a, double* w, double* v, int sort, int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  const long es = cplx ? 2 : 1;
+#pragma omp parallel for schedule(dynamic)
+  for (long k = 0; k < count; ++k) {
+    double* ak = a + k * n * n * es;
+    double* vk = v ? v + k * n * n * es : nullptr;
+    if (cplx) heev<zc>(n, (zc*)ak, n, false, w + k * n, (zc*)vk, n, sort != 0, nullptr);
+    else heev<double>(n, ak, n, false, w + k * n, vk, n, sort != 0, nullptr);
+  }
+}
+void orc_givens_d(double f, double g, double* out3) {
+  givens_re<double>(f, g, out3[0], out3[1], out3[2]);
+}
+void orc_sym2x2(double a, double b, double c, double* out4) {
+  sym2x2<double>(a, b, c, true, out4[0], out4[1], out4[2], out4[3]);
+}
+double orc_jhypot(double x, double y) { return jhypot<double>(x, y); }
 }  // extern "C"

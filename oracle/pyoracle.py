@@ -71,6 +71,22 @@ def lib():
         L.orc_balance_apply.restype = None
         L.orc_eig_hp_z.argtypes = [c_long, c_dp, c_dp, c_int]
         L.orc_eig_hp_d.argtypes = [c_long, c_dp, c_dp, c_int]
+        L.orc_hetrd_z.argtypes = [c_long, c_dp, c_long, c_int, c_dp, c_dp, c_dp, c_dp, c_long]
+        L.orc_hetrd_d.argtypes = [c_long, c_dp, c_long, c_int, c_dp, c_dp, c_dp, c_dp, c_long]
+        L.orc_hetrd_z.restype = L.orc_hetrd_d.restype = None
+        L.orc_steqr.argtypes = [c_long, c_dp, c_dp, c_dp, c_long, c_long, c_int, c_long, c_lp]
+        L.orc_steqr.restype = c_long
+        L.orc_heev_z.argtypes = [c_long, c_dp, c_long, c_int, c_dp, c_dp, c_long, c_int, c_lp]
+        L.orc_heev_d.argtypes = [c_long, c_dp, c_long, c_int, c_dp, c_dp, c_long, c_int, c_lp]
+        L.orc_heev_z.restype = L.orc_heev_d.restype = c_long
+        L.orc_heev_batched.argtypes = [c_long, c_long, c_int, c_dp, c_dp, c_dp, c_int, c_int]
+        L.orc_heev_batched.restype = None
+        L.orc_givens_d.argtypes = [ctypes.c_double, ctypes.c_double, c_dp]
+        L.orc_givens_d.restype = None
+        L.orc_sym2x2.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, c_dp]
+        L.orc_sym2x2.restype = None
+        L.orc_jhypot.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_jhypot.restype = ctypes.c_double
         for f in ("orc_qd_sqrt",):
             getattr(L, f).argtypes = [c_dp, c_dp]
             getattr(L, f).restype = None
@@ -379,3 +395,93 @@ def givens(f, g):
     out = np.zeros(5)
     lib().orc_givens_z(_p(fa), _p(ga), _p(out))
     return out[0], complex(out[1], out[2]), complex(out[3], out[4])
+
+
+# ---------------------------------------------------------------- Hermitian
+
+
+def hetrd(A, uplo="L", wantq=True):
+    """_hehessenberg!(Hermitian(A, uplo)) (hessenberg.jl:27-95) and
+    _materializeQ (hessenberg.jl:133-232): returns (d, e, Q or None); only
+    the uplo triangle of A is read."""
+    A = np.asarray(A)
+    cplx = np.iscomplexobj(A)
+    dt = np.complex128 if cplx else np.float64
+    n = A.shape[0]
+    a = _fort(A, dt)
+    d = np.zeros(max(n, 1))
+    e = np.zeros(max(n - 1, 1))
+    tau = np.zeros(max(n - 1, 1), dtype=dt)
+    q = np.zeros((max(n, 1), max(n, 1)), dtype=dt, order="F") if wantq else None
+    f = lib().orc_hetrd_z if cplx else lib().orc_hetrd_d
+    f(n, _p(a), max(n, 1), int(uplo == "U"), _p(d), _p(e), _p(tau), _p(q), max(n, 1))
+    return d[:n], e[:max(n - 1, 0)], (q[:n, :n] if wantq else None)
+
+
+def steqr(d, e, Z=None, maxsweeps=-1):
+    """_gschur!(SymTridiagonal(d, e), QRIteration(), Z) (symtridiag.jl:77-172):
+    returns (eigenvalues unsorted, Z updated or None, info, sweeps)."""
+    d = np.array(d, dtype=np.float64, copy=True)
+    n = d.shape[0]
+    e = np.array(e, dtype=np.float64, copy=True).reshape(-1)
+    e2 = np.zeros(max(n, 1))
+    e2[:e.shape[0]] = e
+    z = None
+    zc = 0
+    if Z is not None:
+        zc = int(np.iscomplexobj(Z))
+        z = _fort(Z, np.complex128 if zc else np.float64)
+    sw = c_long(0)
+    info = lib().orc_steqr(n, _p(d) if n else None, _p(e2), _p(z), max(z.shape[0], 1) if z is not None else 1,
+                           z.shape[0] if z is not None else 0, zc, int(maxsweeps), ctypes.byref(sw))
+    return d, z, int(info), int(sw.value)
+
+
+def heev(A, uplo="L", wantv=True, sort=True):
+    """geigen! / geigvals! with QRIteration (GenericSchur.jl:124-146) of a
+    Hermitian or real symmetric matrix: returns (w, V or None, info)."""
+    A = np.asarray(A)
+    cplx = np.iscomplexobj(A)
+    dt = np.complex128 if cplx else np.float64
+    n = A.shape[0]
+    a = _fort(A, dt)
+    w = np.zeros(max(n, 1))
+    v = np.zeros((max(n, 1), max(n, 1)), dtype=dt, order="F") if wantv else None
+    f = lib().orc_heev_z if cplx else lib().orc_heev_d
+    sw = c_long(0)
+    info = f(n, _p(a), max(n, 1), int(uplo == "U"), _p(w), _p(v), max(n, 1), int(sort), ctypes.byref(sw))
+    return w[:n], (v[:n, :n] if wantv else None), int(info)
+
+
+def heev_batched(A, sort=True, wantv=True, nthreads=0):
+    """heev over a (count, n, n) stack (lower triangles), OpenMP over
+    matrices: returns (W (count, n), V (count, n, n) or None)."""
+    A = np.asarray(A)
+    cplx = np.iscomplexobj(A)
+    dt = np.complex128 if cplx else np.float64
+    count, n = A.shape[0], A.shape[1]
+    a = np.ascontiguousarray(np.transpose(A, (0, 2, 1)).astype(dt))
+    w = np.zeros((count, n))
+    v = np.zeros((count, n, n), dtype=dt) if wantv else None
+    lib().orc_heev_batched(count, n, int(cplx), a.ctypes.data_as(c_dp), w.ctypes.data_as(c_dp),
+                           v.ctypes.data_as(c_dp) if wantv else None, int(sort), int(nthreads))
+    return w, (np.transpose(v, (0, 2, 1)) if wantv else None)
+
+
+def givens_re(f, g):
+    """LinearAlgebra.givensAlgorithm(f, g) for real f, g: (c, s, r)."""
+    out = np.zeros(3)
+    lib().orc_givens_d(float(f), float(g), _p(out))
+    return float(out[0]), float(out[1]), float(out[2])
+
+
+def sym2x2(a, b, c):
+    """_dgz_sym2x2(a, b, c, true) (symtridiag.jl:375-427): (rt1, rt2, cs, sn)."""
+    out = np.zeros(4)
+    lib().orc_sym2x2(float(a), float(b), float(c), _p(out))
+    return tuple(float(x) for x in out)
+
+
+def jhypot(x, y):
+    """Julia's hypot(x, y) for Float64."""
+    return float(lib().orc_jhypot(float(x), float(y)))

@@ -636,3 +636,202 @@ def test_ordschur_matches_ztrsen(oracle, n):
         sep2 = oracle.subspacesep(T2, k)
         assert abs(s2 - s) <= 1e-8 * s
         assert abs(sep2 - sep) <= 1e-8 * sep
+
+
+# ------------------------------------------------------ Hermitian path pins
+# The Hermitian / real symmetric path (hessenberg.jl:19-232, symtridiag.jl):
+# LAPACK xSYTRD/xHETRD (the algorithm _hehessenberg! translates) pins d, e,
+# tau and the reflector layout of both triangles; the reference's own
+# symmetric tridiagonal test classes and error measures (test/symtridiag.jl)
+# and scipy's tridiagonal eigensolver pin the QL/QR iteration; mpmath pins
+# Julia's hypot; the old/new xLARTG sign relation pins givensAlgorithm.
+
+import symtri_cases as sc
+
+
+def _lapack_q(c, tau, lower):
+    """Q of xHETRD's reflectors (LAPACK's documented layout)."""
+    n = c.shape[0]
+    Q = np.eye(n, dtype=c.dtype)
+    if lower:
+        for i in range(n - 1):
+            v = np.zeros(n, dtype=c.dtype)
+            v[i + 1] = 1
+            v[i + 2:] = c[i + 2:, i]
+            Q = Q @ (np.eye(n) - tau[i] * np.outer(v, v.conj()))
+    else:
+        for i in range(n - 2, -1, -1):
+            v = np.zeros(n, dtype=c.dtype)
+            v[i] = 1
+            v[:i] = c[:i, i + 1]
+            Q = Q @ (np.eye(n) - tau[i] * np.outer(v, v.conj()))
+    return Q
+
+
+def test_jhypot_correctly_rounded(oracle):
+    """Julia's hypot (the fma branch) is correctly rounded: compare with
+    mpmath at 200 bits over random, widely varying, near-overflow and
+    subnormal operands (a dropped correction term or scaling fails)."""
+    import mpmath
+    mpmath.mp.prec = 200
+    r = np.random.default_rng(40)
+    cases = [(3.0, 4.0), (0.0, 0.0), (0.0, -2.5), (1e308, 1e308), (1e-310, 3e-310), (5e-324, 5e-324),
+             (1.0, 1e-9), (1.0, 1e-8), (np.inf, 1.0), (-2.0, np.inf)]
+    for _ in range(300):
+        x, y = r.standard_normal(2) * 10.0 ** r.uniform(-300, 300, 2)
+        cases.append((x, y))
+        cases.append((x, x * (1 + r.uniform(-1e-3, 1e-3))))
+    for x, y in cases:
+        got = oracle.jhypot(x, y)
+        if np.isinf(x) or np.isinf(y):
+            assert got == np.inf
+            continue
+        want = float(mpmath.sqrt(mpmath.mpf(x) ** 2 + mpmath.mpf(y) ** 2))
+        assert got == want, (x, y, got, want)
+
+
+def test_givens_re_matches_dlartg(oracle):
+    """givensAlgorithm(f, g) for real input (LAPACK 3.1 xLARTG): the rotation
+    identities, c > 0 when |f| > |g|, r > 0 otherwise, and LAPACK's current
+    xLARTG (c >= 0, sign(r) = sign(f)) equal up to that sign convention,
+    including the floatmin2-scaled ranges and the zero cases."""
+    r = np.random.default_rng(41)
+    cases = [(1.0, 2.0), (-1.0, 2.0), (2.0, -1.0), (-2.0, 1.0), (0.0, 3.0), (3.0, 0.0), (0.0, -3.0), (-3.0, 0.0),
+             (1e300, -3e300), (-2e-300, 1e-300), (1e-320, 2e-320), (5e299, 5e-200)]
+    for _ in range(200):
+        f, g = r.standard_normal(2) * 10.0 ** r.uniform(-200, 200, 2)
+        cases.append((f, g))
+    for f, g in cases:
+        c, s, rr = oracle.givens_re(f, g)
+        scale = max(abs(f), abs(g))
+        assert abs(c * c + s * s - 1) <= 4 * EPS
+        assert abs(c * f + s * g - rr) <= 4 * EPS * scale
+        assert abs(-s * f + c * g) <= 4 * EPS * scale
+        if g == 0:
+            assert (c, s, rr) == (1.0, 0.0, f)
+            continue
+        if f == 0:
+            assert (c, s, rr) == (0.0, 1.0, g)
+            continue
+        if abs(f) > abs(g):
+            assert c > 0
+        else:
+            assert rr > 0
+        cl, sl, rl = lap.dlartg(f, g)
+        sign = -1.0 if (abs(f) <= abs(g) and f < 0) else 1.0
+        assert abs(c - sign * cl) <= 4 * EPS and abs(s - sign * sl) <= 4 * EPS
+        assert abs(rr - sign * rl) <= 4 * EPS * scale
+
+
+def test_sym2x2_closed_form(oracle):
+    """_dgz_sym2x2 (symtridiag.jl:375-427; xLAEV2): rt1 the eigenvalue of
+    larger modulus, rt2 the other, (cs, sn) the unit eigenvector of rt1 --
+    against numpy's eigvalsh, including b = 0, a = c and a + c = 0."""
+    r = np.random.default_rng(42)
+    cases = [(1.0, 0.0, 2.0), (2.0, 0.0, 1.0), (1.0, 1.0, 1.0), (1.0, 2.0, -1.0), (0.0, 1.0, 0.0),
+             (-3.0, 1e-9, -3.0), (1e-200, 1e-200, 2e-200), (5.0, 0.0, -5.0)]
+    cases += [tuple(r.standard_normal(3) * 10.0 ** r.uniform(-3, 3)) for _ in range(100)]
+    for a, b, c in cases:
+        rt1, rt2, cs, sn = oracle.sym2x2(a, b, c)
+        M = np.array([[a, b], [b, c]])
+        ev = np.linalg.eigvalsh(M)
+        scale = np.abs(M).max()
+        assert abs(rt1) >= abs(rt2) - 4 * EPS * scale
+        assert np.abs(np.sort([rt1, rt2]) - ev).max() <= 8 * EPS * scale
+        assert abs(cs * cs + sn * sn - 1) <= 4 * EPS
+        x = np.array([cs, sn])
+        assert np.abs(M @ x - rt1 * x).max() <= 8 * EPS * scale
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_hetrd_matches_lapack(oracle, cplx, uplo):
+    """_hehessenberg! (hessenberg.jl:27-95) and _materializeQ (:133-232)
+    against LAPACK xSYTRD/xHETRD on the same triangle: d, e and Q to
+    rounding (the reflector conventions and the U form's reversed sweep),
+    plus hesstest's decomposition and orthogonality bounds
+    (test/complex.jl:63-83, tol 10) at scale 1 and 100 floatmin."""
+    r = np.random.default_rng(43)
+    for n in (1, 2, 3, 7, 24, 40):
+        A = r.standard_normal((n, n)) + (1j * r.standard_normal((n, n)) if cplx else 0)
+        A = A + A.conj().T
+        d, e, Q = oracle.hetrd(A, uplo)
+        f = lap.zhetrd if cplx else lap.dsytrd
+        c, dl, el, tau, info = f(A, lower=int(uplo == "L"))
+        assert info == 0
+        scale = np.abs(A).max()
+        assert np.abs(d - dl).max() <= 64 * n * EPS * scale
+        if n > 1:
+            assert np.abs(e - el).max() <= 64 * n * EPS * scale
+            Ql = _lapack_q(c, tau, uplo == "L")
+            assert np.abs(Q - Ql).max() <= 64 * n * EPS
+        for s in (1.0, 100 * TINY):
+            As = A * s
+            d2, e2, Q2 = oracle.hetrd(As, uplo)
+            T = np.diag(d2) + np.diag(e2, 1) + np.diag(e2, -1)
+            # norms taken of the rescaled matrices (numpy's norm squares entries)
+            nrm = np.linalg.norm(As / s)
+            assert np.linalg.norm((As - Q2 @ T @ Q2.conj().T) / s) / (n * nrm * EPS) < 10
+            assert np.linalg.norm(np.eye(n) - Q2 @ Q2.conj().T) / (n * EPS) < 10
+
+
+def test_steqr_reference_classes(oracle):
+    """_gschur!(SymTridiagonal, QRIteration(), Z) on the reference's own test
+    classes (test/symtridiag.jl:47-190, n = 32): runtest's d_err and v_err
+    below 50, and the eigenvalues against scipy's tridiagonal solver (a
+    dropped shift term, a wrong rotation sign or a lost scaling fails)."""
+    r = np.random.default_rng(44)
+    tri = lambda A: oracle.hetrd(A, "L", wantq=False)[:2]
+    mats = [sc.class1(32, t, r, tri)[:2] for t in range(1, 10)] + [sc.class2(32, t)[:2] for t in range(0, 9)]
+    for d, e in mats:
+        n = d.shape[0]
+        w, Z, info, sweeps = oracle.steqr(d, e, np.eye(n))
+        assert info == 0
+        de, ve = sc.errors(d, e, w, Z)
+        assert de < 50 and ve < 50
+        ref = sla.eigh_tridiagonal(d, e, eigvals_only=True)
+        scale = max(np.abs(d).max(), np.abs(e).max() if e.size else 0.0)
+        assert np.abs(np.sort(w) - ref).max() <= 16 * n * EPS * scale
+        # eigenvalues alone: the same values, bit for bit (Z does not steer the iteration)
+        w2, _, _, _ = oracle.steqr(d, e, None)
+        assert np.array_equal(w, w2)
+
+
+def test_steqr_sweep_limit(oracle):
+    """The iteration stops after maxsweeps sweeps in all (the reference's own
+    counter never advances, symtridiag.jl:141) and reports the unconverged
+    off-diagonal entries, LAPACK's info."""
+    d, e, _ = sc.class2(20, 2)
+    w, Z, info, sweeps = oracle.steqr(d, e, np.eye(20), maxsweeps=3)
+    assert sweeps == 3 and info > 0
+    w, Z, info, sweeps = oracle.steqr(d, e, np.eye(20))
+    assert info == 0 and sweeps > 3
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_heev_eigensystem(oracle, cplx):
+    """geigen! (GenericSchur.jl:124-135): eigenvalues ascending against
+    numpy's eigh, A V = V diag(w), V unitary, both triangles giving the same
+    eigenvalues, the iteration bit-identical with and without vectors, and
+    sorteig!'s stable order on repeated eigenvalues (identity, diagonal)."""
+    r = np.random.default_rng(45)
+    for n in (1, 2, 5, 33, 64):
+        A = r.standard_normal((n, n)) + (1j * r.standard_normal((n, n)) if cplx else 0)
+        A = A + A.conj().T
+        w, V, info = oracle.heev(A, "L")
+        assert info == 0
+        scale = np.abs(A).max()
+        assert np.all(np.diff(w) >= 0)
+        assert np.abs(w - np.linalg.eigvalsh(A)).max() <= 16 * n * EPS * scale
+        assert np.abs(A @ V - V * w).max() <= 16 * n * EPS * scale
+        assert np.abs(V.conj().T @ V - np.eye(n)).max() <= 16 * n * EPS
+        wu, _, _ = oracle.heev(A, "U")
+        assert np.abs(wu - w).max() <= 16 * n * EPS * scale
+        w0, _, _ = oracle.heev(A, "L", wantv=False)
+        assert np.array_equal(w0, w)
+    w, V, _ = oracle.heev(np.eye(4), "L")
+    assert np.array_equal(w, np.ones(4)) and np.array_equal(V, np.eye(4))
+    D = np.diag([3.0, -1.0, 2.0, -1.0])
+    w, V, _ = oracle.heev(D, "L")
+    assert np.array_equal(w, [-1.0, -1.0, 2.0, 3.0])
+    assert np.array_equal(np.abs(V), np.eye(4)[:, [1, 3, 2, 0]])
