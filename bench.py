@@ -100,9 +100,12 @@ def main():
     ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--workload", type=int, default=2, choices=[2, 3, 4, 5],
+    ap.add_argument("--workload", type=int, default=2, choices=[2, 3, 4, 5, 6],
                     help="2: batched complex (the headline line); 3: batched real mix; 4: one large real matrix "
-                         "(blocked path vs plain kernel); 5: eigenvalue-loss forward + backward")
+                         "(blocked path vs plain kernel); 5: eigenvalue-loss forward + backward; 6: batched "
+                         "Hermitian eigensystems (QR iteration)")
+    ap.add_argument("--herm-n", type=int, default=64, help="order of the Hermitian matrices of workload 6")
+    ap.add_argument("--herm-real", action="store_true", help="workload 6 on real symmetric matrices")
     ap.add_argument("--large-n", type=int, default=4096)
     ap.add_argument("--plain-n", type=int, default=1024,
                     help="order at which the plain kernel is timed (scaled by n^3 to --large-n)")
@@ -126,6 +129,8 @@ def main():
         return bench_workload4(args)
     if args.workload == 5:
         return bench_workload5(args)
+    if args.workload == 6:
+        return bench_workload6(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (batches shard with no exchange; "
               "run one process per GPU on a slice of the batch)", file=sys.stderr)
@@ -634,6 +639,89 @@ def bench_workload5(args):
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_batched(mats, False, count)
         out["cpu_baseline"]["note"] = "forward (Schur) only: the reference has no eigenvalue rrule"
+    print(json.dumps(out), flush=True)
+    if not out["ok"]:
+        sys.exit(1)
+
+
+def bench_workload6(args):
+    """Batched Hermitian eigensystems: geigen!(Hermitian(A)) with QRIteration
+    (eigenvalues ascending and eigenvectors) of 4096 ComplexF64 Hermitian
+    matrices of order 64 (--herm-real: Float64 symmetric), one launch."""
+    import genericschur as gs
+    from genericschur import _lib
+    import torch
+    count = 4096 if args.count == 8192 else args.count
+    n = args.herm_n
+    cplx = not args.herm_real
+    rng = np.random.default_rng(args.seed)
+    G = rng.standard_normal((count, n, n))
+    if cplx:
+        G = G + 1j * rng.standard_normal((count, n, n))
+    mats = G + np.conj(np.transpose(G, (0, 2, 1)))
+    dt_ = np.complex128 if cplx else np.float64
+    flat = np.ascontiguousarray(np.transpose(mats, (0, 2, 1)), dtype=dt_)  # column-major per matrix
+    dev = torch.device("cuda:0")
+    A = torch.from_numpy(flat.reshape(-1).view(np.float64)).to(dev)
+    V = torch.zeros_like(A)
+    W = torch.zeros(count * n, dtype=torch.float64, device=dev)
+    info = torch.zeros(count, dtype=torch.int32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    gs.set_stream(stream.cuda_stream)
+    h = _lib.handle()
+    f = _lib.lib().gs_heev_z if cplx else _lib.lib().gs_heev_d
+    a = lambda t: ctypes.c_void_p(t.data_ptr())
+    flags = _lib.GS_WANTZ | _lib.GS_SORT | _lib.GS_DEVICE_PTRS | _lib.GS_ASYNC
+
+    def step():
+        rc = f(h, count, n, a(A), n, n * n, 0, a(W), a(V), n, n * n, a(info), flags)
+        _lib.check(rc, h, "gs_heev")
+
+    wall, kern_s = _timed(stream, step, args.steps, args.warmup)
+    gs.set_stream(None)
+    kind = "ComplexF64 Hermitian" if cplx else "Float64 symmetric"
+    out = _common(args, count * args.steps / wall, "matrices/s", 1e3 * wall / args.steps,
+                  f"batched Hermitian eigensystems geigen!(Hermitian(A), QRIteration()) (values ascending and "
+                  f"vectors): {count} {kind} matrices G + G^H of order {n}, G Gaussian; seeded numpy",
+                  {"count": count, "n": n, "complex": cplx, "seed": args.seed})
+    out["metric"] = "Hermitian eigensystems/s (batched, QR iteration)"
+    # model flops per matrix: tridiagonalization and Q formation (xHETRD +
+    # xUNGTR: 16/3 n^3 each complex, 4/3 real) and about n^2 rotations of
+    # 12 n (complex Z) / 6 n (real Z) flops
+    fl = (16.0 / 3 + 16.0 / 3 + 12.0) if cplx else (4.0 / 3 + 4.0 / 3 + 6.0)
+    flops = fl * n ** 3 * count
+    out["roofline"] = {"bound": "mfma", "engine": "valu-fp64", "achieved": flops / kern_s / 1e12, "peak": 78.6,
+                       "unit": "TFLOP/s", "frac": flops / kern_s / PEAK_FP64, "traffic": None,
+                       "note": f"{fl:.2f} n^3 model flops per matrix over the device time of the launch"}
+    Wh = W.cpu().numpy().reshape(count, n)
+    Vh = V.cpu().numpy().view(dt_).reshape(count, n, n).transpose(0, 2, 1)
+    eps = np.finfo(np.float64).eps
+    rb = ro = 0.0
+    for k in np.linspace(0, count - 1, min(64, count)).astype(int):
+        Ak, Vk, wk = mats[k], Vh[k], Wh[k]
+        rb = max(rb, np.linalg.norm(Ak @ Vk - Vk * wk) / np.linalg.norm(Ak) / (n * eps))
+        ro = max(ro, np.linalg.norm(Vk.conj().T @ Vk - np.eye(n)) / (n * eps))
+    out["check"] = {"sample": int(min(64, count)), "max_residual_over_n_eps": rb, "max_orth_over_n_eps": ro,
+                    "info_nonzero": int((info.cpu().numpy() != 0).sum()), "bound_over_n_eps": 50.0}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        m = min(args.cpu_sample, count)
+        cores = min(16, os.cpu_count() or 1)
+        tc = time.perf_counter()
+        pyoracle.heev_batched(mats[:m], nthreads=cores)
+        dtc = time.perf_counter() - tc
+        m1 = max(1, m // 16)
+        tc = time.perf_counter()
+        pyoracle.heev_batched(mats[:m1], nthreads=1)
+        dt1 = time.perf_counter() - tc
+        out["cpu_baseline"] = {"value": m / dtc, "unit": "matrices/s", "cores": cores, "kind": "port",
+                               "sample": f"first {m} matrices; C++ restatement of geigen! (oracle/), OpenMP over "
+                                         f"matrices, {cores} threads, {dtc:.1f}s",
+                               "one_thread_value": m1 / dt1, "one_thread_sample": f"first {m1} matrices, 1 thread"}
+    c = out["check"]
+    out["ok"] = bool(c["info_nonzero"] == 0 and c["max_residual_over_n_eps"] <= 50.0
+                     and c["max_orth_over_n_eps"] <= 50.0)
     print(json.dumps(out), flush=True)
     if not out["ok"]:
         sys.exit(1)

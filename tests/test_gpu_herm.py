@@ -5,9 +5,11 @@ against the CPU oracle.
     real and complex Z, on random tridiagonals and the reference's own test
     classes (test/symtridiag.jl), including the floatmin/ulp- and
     floatmax*ulp-scaled blocks, with runtest's bounds (< 50) on top.
-  * gs_hetrd (hessenberg.jl:19-232): d, e, Q against the oracle to
-    rounding (the GPU's parallel sums round differently), both triangles,
-    plus hesstest's bounds (test/complex.jl:63-83) at 1 and 100 floatmin.
+  * gs_hetrd (hessenberg.jl:19-232): d, e, Q against the oracle entry by
+    entry for n <= 9, the spectrum of (d, e) against the oracle's for all
+    n, both triangles, and hesstest's bounds (test/complex.jl:63-83) at 1
+    and 100 floatmin (single tridiagonal entries are not forward-stable
+    under the GPU's different summation order; the factorization is).
   * gs_heev (GenericSchur.jl:124-146): eigenvalues against the oracle to
     16 n eps |A|, eigenvectors column by column after aligning the unit
     factor (the iteration's signs follow the algorithm, not a convention),
@@ -44,8 +46,7 @@ def test_steqr_bitwise_random(gs, oracle, zkind):
             Z = r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))
         print(f"steqr n={n} z={zkind}", flush=True)
         wo, zo, info_o, _ = oracle.steqr(d, e, Z)
-        eo = None
-        wg, zg, eg, info_g = gs.steqr_(d, e, Z)
+        wg, zg, _, info_g = gs.steqr_(d, e, Z)
         assert info_g == info_o == 0
         assert np.array_equal(wg, wo), n
         if Z is not None:
@@ -85,11 +86,19 @@ def test_hetrd_parity(gs, oracle, cplx, uplo):
         d, e, Q = gs.hessenberg_hermitian(A, uplo)
         do, eo, Qo = oracle.hetrd(A, uplo)
         scale = np.abs(A).max()
-        tol = 32 * n * EPS
-        assert np.abs(d - do).max() <= tol * scale
-        if n > 1:
-            assert np.abs(e - eo).max() <= tol * scale
-        assert np.abs(Q - Qo).max() <= tol
+        if n <= 9:
+            # entrywise only for small orders: single entries of the
+            # tridiagonal form are not forward-stable under a different
+            # rounding (the GPU sums in parallel), the factorization is
+            tol = 32 * n * EPS
+            assert np.abs(d - do).max() <= tol * scale
+            if n > 1:
+                assert np.abs(e - eo).max() <= tol * scale
+            assert np.abs(Q - Qo).max() <= tol
+        # the same spectrum as the oracle's tridiagonal form
+        lam = np.linalg.eigvalsh(np.diag(d) + np.diag(e, 1) + np.diag(e, -1))
+        lamo = np.linalg.eigvalsh(np.diag(do) + np.diag(eo, 1) + np.diag(eo, -1))
+        assert np.abs(lam - lamo).max() <= 16 * n * EPS * scale
         for s in (1.0, 100 * TINY):
             As = A * s
             d2, e2, Q2 = gs.hessenberg_hermitian(As, uplo)
