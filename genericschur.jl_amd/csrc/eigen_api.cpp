@@ -1,7 +1,7 @@
 // Host entry points of the eigen! pieces around the Schur factorization
 // (gs_balance_*, gs_balance_apply, gs_triangularize_d, gs_enormalize,
 // gs_eig_rcond) and of the reordering and conditioning of a complex Schur
-// form (gs_ordschur_z, gs_eigcond_z), see include/gschur_mi355x.h.  Each is one matrix (or one
+// form (gs_ordschur_z, gs_ordschur_d, gs_eigcond_z), see include/gschur_mi355x.h.  Each is one matrix (or one
 // set of columns) on the handle's stream; host pointers are staged through
 // the handle's buffers, device pointers (GS_DEVICE_PTRS) used directly.
 #include <hip/hip_runtime.h>
@@ -277,6 +277,52 @@ int gs_ordschur_z(gs_handle* h, int64_t n, double* T, int64_t ldt, double* Z, in
     ECHK(hipMemcpyAsync(W, dW, 16 * n, hipMemcpyDeviceToHost, s));
   }
   // select is read from the staging buffer until the launch is done
+  ECHK(hipStreamSynchronize(s));
+  return GS_OK;
+}
+
+int gs_ordschur_d(gs_handle* h, int64_t n, double* T, int64_t ldt, double* Z, int64_t ldz, const int32_t* select,
+                  double* W, int32_t* info, int flags) {
+  if (!h) return GS_E_ARG;
+  const int64_t n1 = std::max<int64_t>(n, 1);
+  if (n < 0 || n > INT32_MAX || ldt < n1 || (Z && ldz < n1)) return handle_fail(h, GS_E_DIM, "bad dimension");
+  if (!info) return handle_fail(h, GS_E_ARG, "null pointer");
+  info[0] = 0;
+  if (n == 0) return GS_OK;
+  if (!T || !select || !W) return handle_fail(h, GS_E_ARG, "null pointer");
+  Lock lk(h);
+  ECHK(hipSetDevice(handle_device(h)));
+  hipStream_t s = handle_stream(h);
+  const bool dev = (flags & GS_DEVICE_PTRS) != 0;
+  const size_t nn = (size_t)n * n;
+  hipError_t e;
+  // staging: select and info (and, with host pointers, T, Z, W)
+  const size_t hdr = ((4 * (size_t)n + 4 + 255) / 256) * 256;
+  char* base = static_cast<char*>(stage(h, 0, hdr + (dev ? 0 : 8 * (2 * nn + 2 * (size_t)n)), e));
+  ECHK(e);
+  int32_t* dsel = reinterpret_cast<int32_t*>(base);
+  int32_t* dinfo = dsel + n;
+  double* dT = T;
+  double* dZ = Z;
+  double* dW = W;
+  int64_t a = ldt, b = ldz;
+  ECHK(hipMemcpyAsync(dsel, select, 4 * (size_t)n, hipMemcpyHostToDevice, s));
+  if (!dev) {
+    double* sb = reinterpret_cast<double*>(base + hdr);
+    dT = sb;
+    dZ = Z ? sb + nn : nullptr;
+    dW = sb + 2 * nn;
+    ECHK(hipMemcpy2DAsync(dT, 8 * n, T, 8 * ldt, 8 * n, n, hipMemcpyHostToDevice, s));
+    if (Z) ECHK(hipMemcpy2DAsync(dZ, 8 * n, Z, 8 * ldz, 8 * n, n, hipMemcpyHostToDevice, s));
+    a = b = n;
+  }
+  ECHK(launch_ordschur_d((int)n, dT, a, dZ, b, Z != nullptr, dsel, dW, dinfo, s));
+  if (!dev) {
+    ECHK(hipMemcpy2DAsync(T, 8 * ldt, dT, 8 * n, 8 * n, n, hipMemcpyDeviceToHost, s));
+    if (Z) ECHK(hipMemcpy2DAsync(Z, 8 * ldz, dZ, 8 * n, 8 * n, n, hipMemcpyDeviceToHost, s));
+    ECHK(hipMemcpyAsync(W, dW, 16 * (size_t)n, hipMemcpyDeviceToHost, s));
+  }
+  ECHK(hipMemcpyAsync(info, dinfo, 4, hipMemcpyDeviceToHost, s));
   ECHK(hipStreamSynchronize(s));
   return GS_OK;
 }

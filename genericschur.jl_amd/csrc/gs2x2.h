@@ -4,7 +4,7 @@
 
 namespace gs {
 
-// _gs2x2! (GenericSchur.jl:716-803)
+// _gs2x2! (GenericSchur.jl:716-803); hypot is Julia's (jl_hypot)
 GS_DEV void gs2x2_dev(double& a, double& b, double& c, double& d, double& cs, double& sn, zd& w1, zd& w2) {
   auto sgn = [](double x) { return (x < 0.0) ? -1.0 : 1.0; };
   const double half = 0.5;
@@ -34,14 +34,14 @@ GS_DEV void gs2x2_dev(double& a, double& b, double& c, double& d, double& cs, do
       z = p + sqrt(scale) * sqrt(z) * sgn(p);
       a = d + z;
       d = d - (bcmax / z) * bcmis;
-      const double tau = hypot(c, z);
+      const double tau = jl_hypot(c, z);
       cs = z / tau;
       sn = c / tau;
       b = b - c;
       c = 0.0;
     } else {
       const double sigma = b + c;
-      double tau = hypot(sigma, asubd);
+      double tau = jl_hypot(sigma, asubd);
       cs = sqrt(half * (1.0 + fabs(sigma) / tau));
       sn = -(p / (tau * cs)) * sgn(sigma);
       const double aa = a * cs + b * sn;

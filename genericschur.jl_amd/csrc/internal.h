@@ -207,6 +207,8 @@ size_t steqr_lds_bytes(int n);
 hipError_t launch_heev(const HeevArgs& a, bool cplx, hipStream_t s);
 hipError_t launch_steqr(const SteqrArgs& a, bool zcplx, hipStream_t s);
 
+hipError_t launch_ordschur_d(int n, double* T, int64_t ldt, double* Z, int64_t ldz, bool wantz, const int32_t* select,
+                             double* W, int32_t* info, hipStream_t s);
 hipError_t launch_ordschur(int n, double* T, int64_t ldt, double* Z, int64_t ldz, bool wantz, const int32_t* select,
                            double* W, hipStream_t s);
 hipError_t launch_eigcond(int n, const double* T, int64_t ldt, int nsub, double* work, double* out, hipStream_t s);

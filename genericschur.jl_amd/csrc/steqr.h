@@ -26,35 +26,6 @@
 
 namespace gs {
 
-// Julia's Base.Math hypot(x, y) for Float64: the fused multiply-add branch,
-// correctly rounded.
-GS_DEV double jl_hypot(double x, double y) {
-#pragma clang fp contract(off)
-  double ax = fabs(x), ay = fabs(y);
-  if (__builtin_isinf(ax) || __builtin_isinf(ay)) return __builtin_huge_val();
-  if (ay > ax) {
-    const double t = ax;
-    ax = ay;
-    ay = t;
-  }
-  if (ay <= ax * sqrt(GS_EPS / 2.0)) return ax;
-  double scale = GS_EPS * sqrt(GS_FMIN);
-  if (ax > sqrt(GS_FMAX / 2.0)) {
-    ax = ax * scale;
-    ay = ay * scale;
-    scale = 1.0 / scale;
-  } else if (ay < sqrt(GS_FMIN)) {
-    ax = ax / scale;
-    ay = ay / scale;
-  } else {
-    scale = 1.0;
-  }
-  double h = sqrt(__builtin_fma(ax, ax, ay * ay));
-  const double hsq = h * h, axsq = ax * ax;
-  h -= (__builtin_fma(-ay, ay, hsq - axsq) + __builtin_fma(h, h, -hsq) - __builtin_fma(ax, ax, -axsq)) / (2.0 * h);
-  return h * scale;
-}
-
 // LinearAlgebra.givensAlgorithm(f, g) for real f, g (LAPACK 3.1 xLARTG):
 // floatmin2 = 2^-485 scaling and the sign fix when |f| > |g| and cs < 0.
 GS_DEV void givens_re(double f, double g, double& cs, double& sn, double& r) {

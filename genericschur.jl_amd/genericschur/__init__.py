@@ -12,7 +12,8 @@ from .api import (ArgumentError, DimensionMismatch, Schur, UnconvergedException,
                   synchronize, unpack, eig_offsets, set_stream, gschur_device, dgemm_device, batched_max_n,
                   Balancer, Eigen, balance_, lmul_, ldiv_, triangularize, enormalize_, eigen_, ordschur_, eigcond,
                   eigvalscond, subspacesep, heev_max_n, hessenberg_hermitian, heev_batched, geigen_hermitian,
-                  geigvals_hermitian, steqr_, gschur_symtridiag, geigen_symtridiag, geigvals_symtridiag)
+                  geigvals_hermitian, steqr_, gschur_symtridiag, geigen_symtridiag, geigvals_symtridiag,
+                  IllConditionException)
 from ._lib import GSError, last_stats, last_launches
 
 __all__ = [
@@ -23,5 +24,5 @@ __all__ = [
     "Balancer", "Eigen", "balance_", "lmul_", "ldiv_", "triangularize", "enormalize_", "eigen_",
     "ordschur_", "eigcond", "eigvalscond", "subspacesep",
     "heev_max_n", "hessenberg_hermitian", "heev_batched", "geigen_hermitian", "geigvals_hermitian", "steqr_",
-    "gschur_symtridiag", "geigen_symtridiag", "geigvals_symtridiag",
+    "gschur_symtridiag", "geigen_symtridiag", "geigvals_symtridiag", "IllConditionException",
 ]

@@ -37,7 +37,7 @@ EXPORTS = (
     "gs_trevc_z", "gs_trevc_d", "gs_trevc_batched_z", "gs_trevc_batched_d",
     "gs_eig_vjp_batched_d", "gs_last_stats", "gs_last_launches", "gs_dgemm", "gs_schur_batched_z_noalloc",
     "gs_balance_z", "gs_balance_d", "gs_balance_apply", "gs_triangularize_d", "gs_enormalize", "gs_eig_rcond",
-    "gs_set_tuning", "gs_ordschur_z", "gs_eigcond_z",
+    "gs_set_tuning", "gs_ordschur_z", "gs_ordschur_d", "gs_eigcond_z",
     "gs_heev_z", "gs_heev_d", "gs_heev_max_n", "gs_hetrd_z", "gs_hetrd_d", "gs_steqr",
 )
 
@@ -117,6 +117,7 @@ def lib():
         L.gs_set_tuning.argtypes = [c_vp, c_int, c_vp, c_int]
         L.gs_ordschur_z.argtypes = [c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_int]
         L.gs_eigcond_z.argtypes = [c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_int]
+        L.gs_ordschur_d.argtypes = [c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_int]
         sig_heev = [c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_int, c_vp, c_vp, c_i64, c_i64, c_vp, c_int]
         L.gs_heev_z.argtypes = sig_heev
         L.gs_heev_d.argtypes = sig_heev

@@ -475,17 +475,47 @@ def triangularize(S, device=0):
 
 
 # ------------------------------------------- reordering and conditioning
+class IllConditionException(Exception):
+    """A block swap of the real reordering was refused (rordschur.jl:32;
+    GenericSchur.IllConditionException): k is the 1-based position."""
+
+    def __init__(self, k):
+        super().__init__(f"reordering refused at block {k}")
+        self.k = k
+
+
+def _ordschur_real(S, sel, n, device):
+    t = np.array(S.T, dtype=np.float64, order="F", copy=True)
+    Z = np.asarray(S.Z)
+    z = None if Z.size == 0 else np.array(Z, dtype=np.float64, order="F", copy=True)
+    W = np.zeros(n, dtype=np.complex128)
+    info = np.zeros(1, dtype=np.int32)
+    h = _lib.handle(device)
+    ld = max(n, 1)
+    rc = _lib.lib().gs_ordschur_d(h, n, _ptr(t), ld, None if z is None else _ptr(z), ld, _ptr(sel), _ptr(W),
+                                  _ptr(info), 0)
+    _lib.check(rc, h, "gs_ordschur_d")
+    if info[0]:
+        raise IllConditionException(int(info[0]))
+    return Schur(np.ascontiguousarray(t), np.ascontiguousarray(z) if z is not None else np.zeros((0, 0)), W)
+
+
 def ordschur_(S, select, device=0):
-    """gordschur!(S, select) for a complex Schur factorization
-    (ordschur.jl:23-73): the selected eigenvalues first, in order.  Returns a
-    new Schur (T exactly upper triangular, Z updated, values = diag(T))."""
+    """gordschur!(S, select): the selected eigenvalues first, in order.
+    Complex Schur factorization (ordschur.jl:23-73): a new Schur with T
+    exactly upper triangular, Z updated, values = diag(T).  Real (rordschur.jl:
+    3-45): blocks move whole (a 2 x 2 pair is selected when either member
+    is), the moved 2 x 2 blocks are restandardized; raises
+    IllConditionException(k) when a swap is refused."""
     T = np.asarray(S.T)
     n = _checksquare(T)
-    if T.dtype != np.complex128:
-        raise TypeError("ordschur_ takes a complex Schur factorization (real reordering is not built)")
     sel = np.ascontiguousarray(np.asarray(select, dtype=bool).astype(np.int32))
     if sel.shape != (n,):
         raise ValueError("select must have length n")
+    if T.dtype == np.float64:
+        return _ordschur_real(S, sel, n, device)
+    if T.dtype != np.complex128:
+        raise TypeError("ordschur_ takes a Float64 or ComplexF64 Schur factorization")
     t = np.array(T, order="F", copy=True)
     Z = np.asarray(S.Z)
     z = None if Z.size == 0 else np.array(Z, dtype=np.complex128, order="F", copy=True)

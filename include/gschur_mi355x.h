@@ -253,6 +253,20 @@ int gs_eig_rcond(gs_handle* h, int64_t n, int64_t m, const double* VL, int64_t l
  * = diag(T).  T and Z are overwritten. */
 int gs_ordschur_z(gs_handle* h, int64_t n, double* T, int64_t ldt, double* Z, int64_t ldz, const int32_t* select,
                   double* W, int flags);
+/* gordschur!(T, Z, select) for a real Schur form — rordschur.jl:3-45, with
+ * _trexchange! (:54-165, LAPACK xTREXC), _swap1or2 (:168-297, xLAEXC) and
+ * _syl1or2 (:300-466, xLASY2): the blocks (1 x 1, or standard 2 x 2 pairs,
+ * selected when either member is) with select[k] != 0 (n int32, host
+ * memory) moved to the front of the quasi-triangular T (n x n, ldt) by
+ * adjacent block swaps, Z (ldz; NULL: none) updated alike, the swapped
+ * 2 x 2 blocks restandardized (_gs2x2!); W (n complex) the eigenvalues.
+ * info[0] (host memory) = 0, or k when the swap moving the block at k was
+ * refused as too ill-conditioned (IllConditionException(k)); T and Z then
+ * hold the swaps done before it and W is not written.  Bit for bit the
+ * reference's operation order (its BLAS calls in the reference BLAS
+ * order). */
+int gs_ordschur_d(gs_handle* h, int64_t n, double* T, int64_t ldt, double* Z, int64_t ldz, const int32_t* select,
+                  double* W, int32_t* info, int flags);
 /* eigvalscond(S, nsub) and subspacesep(S, nsub) — ordschur.jl:85-131 with
  * trsylvester!/adjtrsylvester! (sylvester.jl:24-139) and norm1est!
  * (norm1est.jl:21-100): out2[0] = the reciprocal condition number of the

@@ -90,6 +90,31 @@ template <class R> R hypot3(R x, R y, R z) {
   return w * rt<R>::sqrt(a * a + b * b + c * c);
 }
 
+// Julia's Base.Math hypot(x, y) for Float64 (the fused multiply-add branch,
+// correctly rounded); other real types use their library hypot.
+template <class R> R jhypot(R x, R y) { return rt<R>::hypot(x, y); }
+template <> inline double jhypot<double>(double x, double y) {
+  double ax = std::fabs(x), ay = std::fabs(y);
+  if (std::isinf(ax) || std::isinf(ay)) return HUGE_VAL;
+  if (ay > ax) std::swap(ax, ay);
+  if (ay <= ax * std::sqrt(std::numeric_limits<double>::epsilon() / 2.0)) return ax;
+  double scale = std::numeric_limits<double>::epsilon() * std::sqrt(std::numeric_limits<double>::min());
+  if (ax > std::sqrt(std::numeric_limits<double>::max() / 2.0)) {
+    ax = ax * scale;
+    ay = ay * scale;
+    scale = 1.0 / scale;
+  } else if (ay < std::sqrt(std::numeric_limits<double>::min())) {
+    ax = ax / scale;
+    ay = ay / scale;
+  } else {
+    scale = 1.0;
+  }
+  double h = std::sqrt(std::fma(ax, ax, ay * ay));
+  const double hsq = h * h, axsq = ax * ax;
+  h -= (std::fma(-ay, ay, hsq - axsq) + std::fma(h, h, -hsq) - std::fma(ax, ax, -axsq)) / (2.0 * h);
+  return h * scale;
+}
+
 // --------------------------------------- householder.jl:12-54 (real xLARFG)
 template <class R> R reflector(long n, R* x, long inc) {
   if (n <= 1) return R(0.0);
@@ -485,14 +510,14 @@ template <class R> void gs2x2(R& a, R& b, R& c, R& d, R& cs, R& sn, cx<R>& w1, c
       z = p + rt<R>::sqrt(scale) * rt<R>::sqrt(z) * sgn(p);
       a = d + z;
       d = d - (bcmax / z) * bcmis;
-      R tau = rt<R>::hypot(c, z);
+      R tau = jhypot<R>(c, z);
       cs = z / tau;
       sn = c / tau;
       b = b - c;
       c = R(0.0);
     } else {
       R sigma = b + c;
-      R tau = rt<R>::hypot(sigma, asubd);
+      R tau = jhypot<R>(sigma, asubd);
       cs = rt<R>::sqrt(half * (R(1.0) + rt<R>::abs(sigma) / tau));
       sn = -(p / (tau * cs)) * sgn(sigma);
       R aa = a * cs + b * sn;
@@ -2235,31 +2260,6 @@ template <class R> R subspacesep(long n, const cx<R>* tp, long ldt, long nsub) {
 // symmetric tridiagonal QL/QR iteration (symtridiag.jl:14-427).
 // ===================================================================
 
-// Julia's Base.Math hypot(x, y) for Float64 (the fused multiply-add branch,
-// correctly rounded); other real types use their library hypot.
-template <class R> R jhypot(R x, R y) { return rt<R>::hypot(x, y); }
-template <> inline double jhypot<double>(double x, double y) {
-  double ax = std::fabs(x), ay = std::fabs(y);
-  if (std::isinf(ax) || std::isinf(ay)) return HUGE_VAL;
-  if (ay > ax) std::swap(ax, ay);
-  if (ay <= ax * std::sqrt(std::numeric_limits<double>::epsilon() / 2.0)) return ax;
-  double scale = std::numeric_limits<double>::epsilon() * std::sqrt(std::numeric_limits<double>::min());
-  if (ax > std::sqrt(std::numeric_limits<double>::max() / 2.0)) {
-    ax = ax * scale;
-    ay = ay * scale;
-    scale = 1.0 / scale;
-  } else if (ay < std::sqrt(std::numeric_limits<double>::min())) {
-    ax = ax / scale;
-    ay = ay / scale;
-  } else {
-    scale = 1.0;
-  }
-  double h = std::sqrt(std::fma(ax, ax, ay * ay));
-  const double hsq = h * h, axsq = ax * ax;
-  h -= (std::fma(-ay, ay, hsq - axsq) + std::fma(h, h, -hsq) - std::fma(ax, ax, -axsq)) / (2.0 * h);
-  return h * scale;
-}
-
 // LinearAlgebra.givensAlgorithm(f, g) for real arguments (LAPACK 3.1 xLARTG:
 // safmn2 = floatmin2 scaling, and the sign fix when |f| > |g| and cs < 0).
 template <class R> void givens_re(R f, R g, R& cs, R& sn, R& r) {
@@ -2794,4 +2794,492 @@ template <class T> long heev(long n, T* a, long lda, bool upper, typename real_o
   return info;
 }
 
+
+// ===================================================================
+// Reordering of a real Schur form: gordschur! (rordschur.jl:3-45),
+// _trexchange! (:54-165, LAPACK xTREXC), _swap1or2 (:168-297, xLAEXC) and
+// _syl1or2 (:300-466, xLASY2).  BLAS calls of the reference on length-2
+// and length-3 vectors are evaluated in the order of the reference BLAS
+// (xDOT, xAXPY, xGEMV, xGER); LinearAlgebra's reflector! (stdlib) and its
+// short-vector norm (generic_norm2) are restated.
+// ===================================================================
+
+// jmax (defined with the balancing code) is Julia's NaN-propagating max.
+
+// LinearAlgebra.generic_norm2 (the norm of vectors shorter than 32):
+// unscaled sum of squares unless n maxabs^2 overflows or maxabs^2 underflows
+template <class R> R jl_norm2_short(long n, const R* x) {
+  if (n == 0) return R(0.0);
+  R maxabs = rt<R>::abs(x[0]);
+  for (long i = 1; i < n; ++i) maxabs = jmax(maxabs, rt<R>::abs(x[i]));
+  const bool inf = !rt<R>::isnan(maxabs) && !(maxabs <= rt<R>::fmax());
+  if (maxabs == R(0.0) || inf) return maxabs;
+  const R t = (R(double(n)) * maxabs) * maxabs;
+  const bool finite = !rt<R>::isnan(t) && t <= rt<R>::fmax();
+  if (finite && !(maxabs * maxabs == R(0.0))) {
+    R sum = x[0] * x[0];
+    for (long i = 1; i < n; ++i) sum = sum + x[i] * x[i];
+    return rt<R>::sqrt(sum);
+  }
+  R y = rt<R>::abs(x[0]) / maxabs;
+  R sum = y * y;
+  for (long i = 1; i < n; ++i) {
+    y = rt<R>::abs(x[i]) / maxabs;
+    sum = sum + y * y;
+  }
+  return maxabs * rt<R>::sqrt(sum);
+}
+
+// LinearAlgebra.reflector!(x) (stdlib): x[0] <- -nu, x[1:] <- x[1:] / (x0 + nu),
+// returns tau = (x0 + nu) / nu with nu = copysign(norm(x), x0).
+template <class R> R jl_reflector(long n, R* x) {
+  if (n == 0) return R(0.0);
+  R xi1 = x[0];
+  const R normu = jl_norm2_short<R>(n, x);
+  if (normu == R(0.0)) return R(0.0);
+  const R nu = rt<R>::copysign(normu, xi1);
+  xi1 = xi1 + nu;
+  x[0] = -nu;
+  for (long i = 1; i < n; ++i) x[i] = x[i] / xi1;
+  return xi1 / nu;
+}
+
+// _syl1or2(Tl, Tr, B) (rordschur.jl:300-466; xLASY2 with isgn = -1): X with
+// Tl X - X Tr = scale B, orders n1, n2 in {1, 2}; matrices 2 x 2 column-major
+// (element (i, j) at [i + 2 j]).  Returns scale.
+template <class R> R syl1or2(int n1, int n2, const R* tl, const R* tr, const R* b, R* x) {
+  const R epst = rt<R>::eps();
+  const R smallnum = rt<R>::fmin() / epst;
+  auto TL = [&](int i, int j) { return tl[(i - 1) + 2 * (j - 1)]; };
+  auto TR = [&](int i, int j) { return tr[(i - 1) + 2 * (j - 1)]; };
+  auto B = [&](int i, int j) { return b[(i - 1) + 2 * (j - 1)]; };
+  R scale(1.0);
+  if (n1 == 1 && n2 == 1) {
+    R tau1 = TL(1, 1) - TR(1, 1);
+    R beta = rt<R>::abs(tau1);
+    if (beta <= smallnum) {
+      tau1 = smallnum;
+      beta = smallnum;
+    }
+    const R gamma = rt<R>::abs(B(1, 1));
+    if (smallnum * gamma > beta) scale = R(1.0) / gamma;
+    x[0] = (B(1, 1) * scale) / tau1;
+    return scale;
+  }
+  if (n1 == 2 && n2 == 2) {
+    R smin = jmax(jmax(jmax(rt<R>::abs(TR(1, 1)), rt<R>::abs(TR(2, 1))), jmax(rt<R>::abs(TR(1, 2)), rt<R>::abs(TR(2, 2)))),
+                  jmax(jmax(rt<R>::abs(TL(1, 1)), rt<R>::abs(TL(2, 1))), jmax(rt<R>::abs(TL(1, 2)), rt<R>::abs(TL(2, 2)))));
+    smin = jmax(epst * smin, smallnum);
+    R TT[4][4] = {};
+    TT[0][0] = TL(1, 1) - TR(1, 1);
+    TT[1][1] = TL(2, 2) - TR(1, 1);
+    TT[2][2] = TL(1, 1) - TR(2, 2);
+    TT[3][3] = TL(2, 2) - TR(2, 2);
+    TT[0][1] = TL(1, 2);
+    TT[1][0] = TL(2, 1);
+    TT[2][3] = TL(1, 2);
+    TT[3][2] = TL(2, 1);
+    TT[0][2] = -TR(2, 1);
+    TT[1][3] = -TR(2, 1);
+    TT[2][0] = -TR(1, 2);
+    TT[3][1] = -TR(1, 2);
+    R bv[4] = {B(1, 1), B(2, 1), B(1, 2), B(2, 2)};
+    int ipsv = 0, jpsv = 0;
+    int jpiv[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 3; ++i) {
+      R xmax(0.0);
+      for (int ip = i; ip < 4; ++ip)
+        for (int jp = i; jp < 4; ++jp)
+          if (rt<R>::abs(TT[ip][jp]) > xmax) {
+            xmax = rt<R>::abs(TT[ip][jp]);
+            ipsv = ip;
+            jpsv = jp;
+          }
+      if (ipsv != i) {
+        for (int k = 0; k < 4; ++k) std::swap(TT[ipsv][k], TT[i][k]);
+        std::swap(bv[i], bv[ipsv]);
+      }
+      if (jpsv != i)
+        for (int k = 0; k < 4; ++k) std::swap(TT[k][jpsv], TT[k][i]);
+      jpiv[i] = jpsv;
+      if (rt<R>::abs(TT[i][i]) < smin) TT[i][i] = smin;
+      for (int j = i + 1; j < 4; ++j) {
+        TT[j][i] = TT[j][i] / TT[i][i];
+        bv[j] = bv[j] - TT[j][i] * bv[i];
+        for (int k = i + 1; k < 4; ++k) TT[j][k] = TT[j][k] - TT[j][i] * TT[i][k];
+      }
+    }
+    if (rt<R>::abs(TT[3][3]) < smin) TT[3][3] = smin;
+    const R a = R(8.0) * smallnum;
+    if ((a * rt<R>::abs(bv[0]) > TT[0][0]) || (a * rt<R>::abs(bv[1]) > TT[1][1]) || (a * rt<R>::abs(bv[2]) > TT[2][2]) ||
+        (a * rt<R>::abs(bv[3]) > TT[3][3])) {
+      const R bmax = jmax(jmax(rt<R>::abs(bv[0]), rt<R>::abs(bv[1])), jmax(rt<R>::abs(bv[2]), rt<R>::abs(bv[3])));
+      scale = (R(1.0) / R(8.0)) / bmax;
+      for (int k = 0; k < 4; ++k) bv[k] = bv[k] * scale;
+    }
+    R xt[4] = {};
+    for (int i = 1; i <= 4; ++i) {
+      const int k = 4 - i;
+      const R t = R(1.0) / TT[k][k];
+      xt[k] = bv[k] * t;
+      for (int j = k + 1; j < 4; ++j) xt[k] = xt[k] - (t * TT[k][j]) * xt[j];
+    }
+    for (int i = 1; i <= 3; ++i) {
+      const int jj = jpiv[3 - i];
+      if (jj != 3 - i) std::swap(xt[3 - i], xt[jj]);
+    }
+    x[0] = xt[0];
+    x[1] = xt[1];
+    x[2] = xt[2];
+    x[3] = xt[3];
+    return scale;
+  }
+  R smin, tmp[4], btmp[2];
+  if (n1 == 1) {
+    smin = jmax(epst * jmax(rt<R>::abs(TL(1, 1)),
+                            jmax(jmax(rt<R>::abs(TR(1, 1)), rt<R>::abs(TR(2, 1))), jmax(rt<R>::abs(TR(1, 2)), rt<R>::abs(TR(2, 2))))),
+                smallnum);
+    tmp[0] = TL(1, 1) - TR(1, 1);
+    tmp[1] = -TR(1, 2);
+    tmp[2] = -TR(2, 1);
+    tmp[3] = TL(1, 1) - TR(2, 2);
+    btmp[0] = B(1, 1);
+    btmp[1] = B(1, 2);
+  } else {
+    smin = jmax(epst * jmax(rt<R>::abs(TR(1, 1)),
+                            jmax(jmax(rt<R>::abs(TL(1, 1)), rt<R>::abs(TL(2, 1))), jmax(rt<R>::abs(TL(1, 2)), rt<R>::abs(TL(2, 2))))),
+                smallnum);
+    tmp[0] = TL(1, 1) - TR(1, 1);
+    tmp[1] = TL(2, 1);
+    tmp[2] = TL(1, 2);
+    tmp[3] = TL(2, 2) - TR(1, 1);
+    btmp[0] = B(1, 1);
+    btmp[1] = B(2, 1);
+  }
+  // findmax(abs.(tmp)): the first index of the largest value
+  int ipiv = 0;
+  R amax = rt<R>::abs(tmp[0]);
+  for (int k = 1; k < 4; ++k)
+    if (rt<R>::abs(tmp[k]) > amax) {
+      amax = rt<R>::abs(tmp[k]);
+      ipiv = k;
+    }
+  R u11 = tmp[ipiv];
+  if (amax <= smin) u11 = smin;
+  static const int inds[4][3] = {{2, 1, 3}, {3, 0, 2}, {0, 3, 1}, {1, 2, 0}};
+  static const bool xsw[4] = {false, false, true, true}, bsw[4] = {false, true, false, true};
+  const R u12 = tmp[inds[ipiv][0]];
+  const R l21 = tmp[inds[ipiv][1]] / u11;
+  R u22 = tmp[inds[ipiv][2]] - u12 * l21;
+  if (rt<R>::abs(u22) <= smin) u22 = smin;
+  R b1, b2;
+  if (bsw[ipiv]) {
+    b1 = btmp[1];
+    b2 = btmp[0] - l21 * btmp[1];
+  } else {
+    b1 = btmp[0];
+    b2 = btmp[1] - l21 * btmp[0];
+  }
+  if (((R(2.0) * smallnum) * rt<R>::abs(b2) > rt<R>::abs(u22)) || ((R(2.0) * smallnum) * rt<R>::abs(b1) > rt<R>::abs(u11))) {
+    scale = (R(1.0) / R(2.0)) / jmax(rt<R>::abs(b1), rt<R>::abs(b2));
+    b1 = scale * b1;
+    b2 = scale * b2;
+  }
+  const R x22 = b2 / u22;
+  R x1 = b1 / u11 - (u12 / u11) * x22, x2 = x22;
+  if (xsw[ipiv]) std::swap(x1, x2);
+  if (n1 == 1) {  // X is 1 x 2
+    x[0] = x1;
+    x[2] = x2;
+  } else {  // X is 2 x 1
+    x[0] = x1;
+    x[1] = x2;
+  }
+  return scale;
+}
+
+// Householder H = I - tau [1; v1; v2][1; v1; v2]^T (householder.jl:105-172):
+// lmul!(H', A[r0:r0+2, c0:c1]) and rmul!(A[r0:r1, c0:c0+2], H); 1-based.
+template <class R> void hh3_lmul(Mat<R> A, long r0, long c0, long c1, R v1, R v2, R tau) {
+  for (long j = c0; j <= c1; ++j) {
+    R va = A(r0, j);
+    va = va + (v1 * A(r0 + 1, j) + v2 * A(r0 + 2, j));
+    va = tau * va;
+    A(r0, j) = A(r0, j) - va;
+    A(r0 + 1, j) = A(r0 + 1, j) + (-va) * v1;
+    A(r0 + 2, j) = A(r0 + 2, j) + (-va) * v2;
+  }
+}
+template <class R> void hh3_rmul(Mat<R> A, long r0, long r1, long c0, R v1, R v2, R tau) {
+  for (long i = r0; i <= r1; ++i) {
+    R x = A(i, c0 + 1) * v1 + A(i, c0 + 2) * v2;
+    x = x + A(i, c0);
+    A(i, c0) = A(i, c0) + (-tau) * x;
+    A(i, c0 + 1) = A(i, c0 + 1) + x * ((-tau) * v1);
+    A(i, c0 + 2) = A(i, c0 + 2) + x * ((-tau) * v2);
+  }
+}
+// the shifted reflector [v1; v2; 1] of the n1 = 1, n2 = 2 swap
+// (_reflectorYappl!, rordschur.jl:511-540: A <- H' A on rows r0..r0+2;
+// _reflectorYlppa!, :485-509: A <- A H on columns c0..c0+2)
+template <class R> void hy3_lmul(Mat<R> A, long r0, long c0, long c1, R v1, R v2, R tau) {
+  for (long j = c0; j <= c1; ++j) {
+    R va = A(r0 + 2, j);
+    va = va + v1 * A(r0, j);
+    va = va + v2 * A(r0 + 1, j);
+    va = tau * va;
+    A(r0 + 2, j) = A(r0 + 2, j) - va;
+    A(r0, j) = A(r0, j) - v1 * va;
+    A(r0 + 1, j) = A(r0 + 1, j) - v2 * va;
+  }
+}
+template <class R> void hy3_rmul(Mat<R> A, long r0, long r1, long c0, R v1, R v2, R tau) {
+  for (long i = r0; i <= r1; ++i) {
+    R x = A(i, c0 + 2);
+    x = x + A(i, c0) * v1;
+    x = x + A(i, c0 + 1) * v2;
+    x = tau * x;
+    A(i, c0 + 2) = A(i, c0 + 2) - x;
+    A(i, c0) = A(i, c0) - x * v1;
+    A(i, c0 + 1) = A(i, c0 + 1) - x * v2;
+  }
+}
+// rows (i, i+1) <- G (i, i+1) for columns c0..c1 and columns (i, i+1) <- (i, i+1) G'
+// for rows r0..r1, G = Givens(i, i+1, c, s)
+template <class R> void g2_lmul(Mat<R> A, long i, long c0, long c1, R c, R s) {
+  for (long j = c0; j <= c1; ++j) {
+    const R a1 = A(i, j), a2 = A(i + 1, j);
+    A(i, j) = c * a1 + s * a2;
+    A(i + 1, j) = -s * a1 + c * a2;
+  }
+}
+template <class R> void g2_rmul(Mat<R> A, long r0, long r1, long i, R c, R s) {
+  for (long r = r0; r <= r1; ++r) {
+    const R a1 = A(r, i), a2 = A(r, i + 1);
+    A(r, i) = a1 * c + a2 * s;
+    A(r, i + 1) = -a1 * s + a2 * c;
+  }
+}
+
+// _swap1or2(T, Z, j1, n1, n2) (rordschur.jl:168-297; LAPACK xLAEXC)
+template <class R> bool swap1or2(long n, R* t, long ldt, R* z, long ldz, long j1, int n1, int n2) {
+  Mat<R> T{t, ldt};
+  Mat<R> Z{z, ldz};
+  if (j1 + n1 > n) return true;
+  const long j2 = j1 + 1, j3 = j1 + 2, j4 = j1 + 3;
+  if (n1 == 1 && n2 == 1) {
+    const R T11 = T(j1, j1), T22 = T(j2, j2);
+    R c, s, r;
+    givens_re<R>(T(j1, j2), T22 - T11, c, s, r);
+    if (j3 <= n) g2_lmul<R>(T, j1, j3, n, c, s);
+    g2_rmul<R>(T, 1, j1 - 1, j1, c, s);
+    T(j1, j1) = T22;
+    T(j2, j2) = T11;
+    if (z) g2_rmul<R>(Z, 1, n, j1, c, s);
+    return true;
+  }
+  const int nd = n1 + n2;
+  R Dm[16];
+  Mat<R> D{Dm, 4};
+  R dnorm(0.0);
+  for (int j = 1; j <= nd; ++j)
+    for (int i = 1; i <= nd; ++i) {
+      D(i, j) = T(j1 + i - 1, j1 + j - 1);
+      dnorm = (i == 1 && j == 1) ? rt<R>::abs(D(i, j)) : jmax(dnorm, rt<R>::abs(D(i, j)));
+    }
+  const R smallnum = rt<R>::fmin() / rt<R>::eps();
+  const R thresh = jmax(R(10.0) * rt<R>::eps() * dnorm, smallnum);
+  R tl[4] = {}, tr[4] = {}, bb[4] = {}, X[4] = {};
+  for (int j = 0; j < n1; ++j)
+    for (int i = 0; i < n1; ++i) tl[i + 2 * j] = D(i + 1, j + 1);
+  for (int j = 0; j < n2; ++j)
+    for (int i = 0; i < n2; ++i) tr[i + 2 * j] = D(n1 + i + 1, n1 + j + 1);
+  for (int j = 0; j < n2; ++j)
+    for (int i = 0; i < n1; ++i) bb[i + 2 * j] = D(i + 1, n1 + j + 1);
+  const R scale = syl1or2<R>(n1, n2, tl, tr, bb, X);
+  auto x = [&](int i, int j) { return X[(i - 1) + 2 * (j - 1)]; };
+  if (n1 == 1 && n2 == 2) {
+    R u[3] = {x(1, 2), scale, x(1, 1)};
+    const R tau = jl_reflector<R>(3, u);
+    const R T11 = T(j1, j1);
+    hy3_lmul<R>(D, 1, 1, 3, u[1], u[2], tau);
+    hy3_rmul<R>(D, 1, 3, 1, u[1], u[2], tau);
+    if (jmax(jmax(rt<R>::abs(D(3, 1)), rt<R>::abs(D(3, 2))), rt<R>::abs(D(3, 3) - T11)) > thresh) return false;
+    hy3_lmul<R>(T, j1, j1, n, u[1], u[2], tau);
+    hy3_rmul<R>(T, 1, j2, j1, u[1], u[2], tau);
+    T(j3, j1) = R(0.0);
+    T(j3, j2) = R(0.0);
+    T(j3, j3) = T11;
+    if (z) hy3_rmul<R>(Z, 1, n, j1, u[1], u[2], tau);
+  } else if (n1 == 2 && n2 == 1) {
+    R u[3] = {-x(1, 1), -x(2, 1), scale};
+    const R tau = jl_reflector<R>(3, u);
+    u[0] = R(1.0);
+    const R T33 = T(j3, j3);
+    hh3_lmul<R>(D, 1, 1, 3, u[1], u[2], tau);
+    hh3_rmul<R>(D, 1, 3, 1, u[1], u[2], tau);
+    if (jmax(jmax(rt<R>::abs(D(1, 1) - T33), rt<R>::abs(D(2, 1))), rt<R>::abs(D(3, 1))) > thresh) return false;
+    hh3_rmul<R>(T, 1, j3, j1, u[1], u[2], tau);
+    hh3_lmul<R>(T, j1, j2, n, u[1], u[2], tau);
+    T(j1, j1) = T33;
+    T(j2, j1) = R(0.0);
+    T(j3, j1) = R(0.0);
+    if (z) hh3_rmul<R>(Z, 1, n, j1, u[1], u[2], tau);
+  } else {
+    R u1[3] = {-x(1, 1), -x(2, 1), scale};
+    const R tau1 = jl_reflector<R>(3, u1);
+    const R tt = -tau1 * (x(1, 2) + u1[1] * x(2, 2));
+    R u2[3] = {-tt * u1[1] - x(2, 2), -tt * u1[2], scale};
+    const R tau2 = jl_reflector<R>(3, u2);
+    hh3_lmul<R>(D, 1, 1, 4, u1[1], u1[2], tau1);
+    hh3_rmul<R>(D, 1, 4, 1, u1[1], u1[2], tau1);
+    hh3_lmul<R>(D, 2, 1, 4, u2[1], u2[2], tau2);
+    hh3_rmul<R>(D, 1, 4, 2, u2[1], u2[2], tau2);
+    if (jmax(jmax(rt<R>::abs(D(3, 1)), rt<R>::abs(D(3, 2))), jmax(rt<R>::abs(D(4, 1)), rt<R>::abs(D(4, 2)))) > thresh)
+      return false;
+    hh3_lmul<R>(T, j1, j1, n, u1[1], u1[2], tau1);
+    hh3_rmul<R>(T, 1, j4, j1, u1[1], u1[2], tau1);
+    hh3_lmul<R>(T, j2, j1, n, u2[1], u2[2], tau2);
+    hh3_rmul<R>(T, 1, j4, j2, u2[1], u2[2], tau2);
+    T(j3, j1) = R(0.0);
+    T(j3, j2) = R(0.0);
+    T(j4, j1) = R(0.0);
+    T(j4, j2) = R(0.0);
+    if (z) {
+      hh3_rmul<R>(Z, 1, n, j1, u1[1], u1[2], tau1);
+      hh3_rmul<R>(Z, 1, n, j2, u2[1], u2[2], tau2);
+    }
+  }
+  auto standardize = [&](long k) {
+    R a = T(k, k), b = T(k, k + 1), c = T(k + 1, k), d = T(k + 1, k + 1), cs, sn;
+    cx<R> w1, w2;
+    gs2x2<R>(a, b, c, d, cs, sn, w1, w2);
+    g2_lmul<R>(T, k, k + 2, n, cs, sn);
+    g2_rmul<R>(T, 1, k - 1, k, cs, sn);
+    T(k, k) = a;
+    T(k, k + 1) = b;
+    T(k + 1, k) = c;
+    T(k + 1, k + 1) = d;
+    if (z) g2_rmul<R>(Z, 1, n, k, cs, sn);
+  };
+  if (n2 == 2) standardize(j1);
+  if (n1 == 2) standardize(j1 + n2);
+  return true;
+}
+
+// _trexchange!(T, Z, iold, inew) (rordschur.jl:54-165; LAPACK xTREXC):
+// returns false when a swap was refused; inew receives the final position.
+template <class R> bool trexchange_re(long n, R* t, long ldt, R* z, long ldz, long iold, long& inew) {
+  Mat<R> T{t, ldt};
+  if (n <= 1 || iold == inew) return true;
+  if (iold > 1 && T(iold, iold - 1) != R(0.0)) iold -= 1;
+  auto nbsize = [&](long j) { return (j < n && T(j + 1, j) != R(0.0)) ? 2 : 1; };
+  int nbold = nbsize(iold);
+  if (inew > 1 && T(inew, inew - 1) != R(0.0)) inew -= 1;
+  const int nbnew = nbsize(inew);
+  if (iold == inew) return true;
+  long ihere = iold;
+  bool ok = true;
+  if (iold < inew) {
+    if (nbold == 2 && nbnew == 1) inew -= 1;
+    else if (nbold == 1 && nbnew == 2) inew += 1;
+    while (ihere < inew) {
+      if (nbold == 1 || nbold == 2) {
+        const int nbnext = nbsize(ihere + nbold);
+        ok = swap1or2<R>(n, t, ldt, z, ldz, ihere, nbold, nbnext);
+        if (!ok) { inew = ihere; return false; }
+        ihere += nbnext;
+        if (nbold == 2 && T(ihere + 1, ihere) == R(0.0)) nbold = 3;
+      } else {
+        int nbnext = nbsize(ihere + 1);
+        ok = swap1or2<R>(n, t, ldt, z, ldz, ihere + 1, 1, nbnext);
+        if (!ok) { inew = ihere; return false; }
+        if (nbnext == 1) {
+          ok = swap1or2<R>(n, t, ldt, z, ldz, ihere, 1, nbnext);
+          ihere += 1;
+        } else {
+          if (T(ihere + 2, ihere + 1) == R(0.0)) nbnext = 1;
+          if (nbnext == 2) {
+            ok = swap1or2<R>(n, t, ldt, z, ldz, ihere, 1, nbnext);
+            if (!ok) { inew = ihere; return false; }
+            ihere += 2;
+          } else {
+            ok = swap1or2<R>(n, t, ldt, z, ldz, ihere, 1, 1);
+            ok = swap1or2<R>(n, t, ldt, z, ldz, ihere + 1, 1, 1);
+            ihere += 2;
+          }
+        }
+      }
+    }
+  } else {
+    while (ihere > inew) {
+      if (nbold == 1 || nbold == 2) {
+        const int nbnext = (ihere >= 3 && T(ihere - 1, ihere - 2) != R(0.0)) ? 2 : 1;
+        ok = swap1or2<R>(n, t, ldt, z, ldz, ihere - nbnext, nbnext, nbold);
+        if (!ok) { inew = ihere; return false; }
+        ihere -= nbnext;
+        if (nbold == 2 && T(ihere + 1, ihere) == R(0.0)) nbold = 3;
+      } else {
+        int nbnext = (ihere >= 3 && T(ihere - 1, ihere - 2) != R(0.0)) ? 2 : 1;
+        ok = swap1or2<R>(n, t, ldt, z, ldz, ihere - nbnext, nbnext, 1);
+        if (!ok) { inew = ihere; return false; }
+        if (nbnext == 1) {
+          ok = swap1or2<R>(n, t, ldt, z, ldz, ihere, nbnext, 1);
+          ihere -= 1;
+        } else {
+          if (T(ihere, ihere - 1) == R(0.0)) nbnext = 1;
+          if (nbnext == 2) {
+            ok = swap1or2<R>(n, t, ldt, z, ldz, ihere - 1, 2, 1);
+            if (!ok) { inew = ihere; return false; }
+            ihere -= 2;
+          } else {
+            ok = swap1or2<R>(n, t, ldt, z, ldz, ihere, 1, 1);
+            ok = swap1or2<R>(n, t, ldt, z, ldz, ihere - 1, 1, 1);
+            ihere -= 2;
+          }
+        }
+      }
+    }
+  }
+  inew = ihere;
+  return ok;
+}
+
+// gordschur!(T, Z, select) for a real Schur form (rordschur.jl:3-45):
+// returns 0, or k (1-based) when the swap moving block k was refused
+// (IllConditionException(k)); w (n complex) receives the eigenvalues.
+template <class R> long gordschur_re(long n, R* t, long ldt, R* z, long ldz, const int* select, cx<R>* w) {
+  Mat<R> T{t, ldt};
+  long ks = 0;
+  bool pair = false;
+  for (long k = 1; k <= n; ++k) {
+    if (pair) {
+      pair = false;
+      continue;
+    }
+    bool swap = select[k - 1] != 0;
+    if (k < n && T(k + 1, k) != R(0.0)) {
+      pair = true;
+      swap = swap || select[k] != 0;
+    }
+    if (swap) {
+      ks += 1;
+      if (k != ks) {
+        long inew = ks;
+        const bool ok = trexchange_re<R>(n, t, ldt, z, ldz, k, inew);
+        ks = inew;
+        if (!ok) return k;
+      }
+      if (pair) ks += 1;
+    }
+  }
+  for (long k = 1; k <= n; ++k) w[k - 1] = cx<R>(T(k, k));
+  for (long k = 1; k <= n - 1; ++k)
+    if (T(k + 1, k) != R(0.0)) {
+      const R y = rt<R>::sqrt(rt<R>::abs(T(k, k + 1))) * rt<R>::sqrt(rt<R>::abs(T(k + 1, k)));
+      w[k - 1] = w[k - 1] + cx<R>(R(0.0), y);
+      w[k] = w[k] - cx<R>(R(0.0), y);
+    }
+  return 0;
+}
 }  // namespace orc

@@ -321,4 +321,15 @@ void orc_sym2x2(double a, double b, double c, double* out4) {
   sym2x2<double>(a, b, c, true, out4[0], out4[1], out4[2], out4[3]);
 }
 double orc_jhypot(double x, double y) { return jhypot<double>(x, y); }
+
+// gordschur!(T, Z, select) for a real Schur form (rordschur.jl:3-45): T, Z
+// (may be NULL) overwritten, w (n complex) the eigenvalues; returns 0 or the
+// 1-based index k of IllConditionException(k).
+long orc_ordschur_d(long n, double* t, long ldt, double* z, long ldz, const int* select, double* w) {
+  return gordschur_re<double>(n, t, ldt, z, ldz, select, (zc*)w);
+}
+// _syl1or2(Tl, Tr, B) (rordschur.jl:300-466): x (2 x 2 column-major) and scale
+double orc_syl1or2(int n1, int n2, const double* tl, const double* tr, const double* b, double* x) {
+  return syl1or2<double>(n1, n2, tl, tr, b, x);
+}
 }  // extern "C"

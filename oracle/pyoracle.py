@@ -85,6 +85,10 @@ def lib():
         L.orc_givens_d.restype = None
         L.orc_sym2x2.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, c_dp]
         L.orc_sym2x2.restype = None
+        L.orc_ordschur_d.argtypes = [c_long, c_dp, c_long, c_dp, c_long, c_ip, c_dp]
+        L.orc_ordschur_d.restype = c_long
+        L.orc_syl1or2.argtypes = [c_int, c_int, c_dp, c_dp, c_dp, c_dp]
+        L.orc_syl1or2.restype = ctypes.c_double
         L.orc_jhypot.argtypes = [ctypes.c_double, ctypes.c_double]
         L.orc_jhypot.restype = ctypes.c_double
         for f in ("orc_qd_sqrt",):
@@ -485,3 +489,27 @@ def sym2x2(a, b, c):
 def jhypot(x, y):
     """Julia's hypot(x, y) for Float64."""
     return float(lib().orc_jhypot(float(x), float(y)))
+
+
+def ordschur_re(T, Z, select):
+    """gordschur!(T, Z, select) for a real Schur form (rordschur.jl:3-45):
+    returns (T2, Z2, w, info) with info = k of IllConditionException(k) or 0."""
+    n = T.shape[0]
+    t = _fort(T, np.float64)
+    z = _fort(Z, np.float64) if Z is not None else None
+    sel = np.ascontiguousarray(np.asarray(select, dtype=bool).astype(np.int32))
+    w = np.zeros(max(n, 1), dtype=np.complex128)
+    info = lib().orc_ordschur_d(n, _p(t), max(n, 1), _p(z), max(n, 1), sel.ctypes.data_as(c_ip), _p(w))
+    return t, z, w[:n], int(info)
+
+
+def syl1or2(Tl, Tr, B):
+    """_syl1or2(Tl, Tr, B) (rordschur.jl:300-466): (X, scale) with
+    Tl X - X Tr = scale B."""
+    n1, n2 = Tl.shape[0], Tr.shape[0]
+    tl = np.zeros((2, 2), order="F"); tl[:n1, :n1] = Tl
+    tr = np.zeros((2, 2), order="F"); tr[:n2, :n2] = Tr
+    b = np.zeros((2, 2), order="F"); b[:n1, :n2] = B
+    x = np.zeros((2, 2), order="F")
+    sc = lib().orc_syl1or2(n1, n2, _p(tl), _p(tr), _p(b), _p(x))
+    return x[:n1, :n2].copy(), float(sc)

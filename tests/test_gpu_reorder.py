@@ -77,3 +77,79 @@ def test_eigcond_rejects_bad_nsub(gs):
     for k in (0, 6):
         with pytest.raises(gs.GSError):
             gs.eigcond(S, k)
+
+
+# ------------------------------------------------ real Schur forms
+# gordschur! for real T (rordschur.jl:3-466) against the oracle bit for bit
+# (the kernel evaluates the oracle's operations in its order, without
+# contraction), with the reference's real checkord (test/ordschur.jl:89-133).
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 20, 64, 150])
+def test_ordschur_real_bitwise(gs, oracle, n):
+    import checks
+    r = np.random.default_rng(70 + n)
+    A = r.standard_normal((n, n))
+    S = gs.gschur(A)
+    T, Z = np.asarray(S.T), np.asarray(S.Z)
+    sels = [np.zeros(n, dtype=bool), np.ones(n, dtype=bool)]
+    for _ in range(3):
+        sels.append(r.random(n) < 0.4)
+    s_last = np.zeros(n, dtype=bool)
+    s_last[-1] = True
+    sels.append(s_last)
+    for k, sel in enumerate(sels):
+        print(f"case ordschur_d n={n} sel={k}", flush=True)
+        S2 = gs.ordschur_(S, sel)
+        T2, Z2, w2, info = oracle.ordschur_re(T, Z, sel)
+        assert info == 0
+        assert np.array_equal(S2.T, T2), (n, k)
+        assert np.array_equal(S2.Z, Z2), (n, k)
+        assert np.array_equal(S2.values, w2), (n, k)
+        if n > 1:
+            assert np.all(np.tril(S2.T, -2) == 0)
+            assert checks.checkblocks(S2.T)
+            assert np.linalg.norm(S2.Z @ S2.T @ S2.Z.T - A) / (n * np.linalg.norm(A) * EPS) < 10
+            assert np.linalg.norm(np.eye(n) - S2.Z @ S2.Z.T) / (n * EPS) < 10
+    # without Z: the same T
+    S0 = gs.Schur(T, np.zeros((0, 0)), S.values)
+    sel = sels[2]
+    assert np.array_equal(gs.ordschur_(S0, sel).T, gs.ordschur_(S, sel).T)
+
+
+def test_ordschur_real_checkord(gs, oracle):
+    """test/ordschur.jl:89-133 on the GPU: two reals, a pair, a real and a
+    pair, two pairs moved to the front of an n = 20 real Schur form."""
+    import checks
+    n = 20
+    r = np.random.default_rng(48)
+    A = r.random((n, n))
+    S = gs.gschur(A)
+    wv = np.asarray(S.values)
+    for nr, ni in ((2, 0), (0, 2), (1, 2), (0, 4)):
+        select = np.zeros(n, dtype=bool)
+        i = n
+        while nr + ni > 0 and i > 1:
+            i -= 1
+            if nr > 0 and wv[i].imag == 0:
+                select[i] = True
+                nr -= 1
+            if ni > 0:
+                iwi = wv[i].imag
+                if abs(iwi) > 1000 * EPS and wv[i - 1].imag == -iwi:
+                    select[i] = True
+                    select[i - 1] = True
+                    ni -= 2
+                    i -= 1
+        if nr + ni > 0:
+            continue
+        S2 = gs.ordschur_(S, select)
+        T2, Z2, w2 = np.asarray(S2.T), np.asarray(S2.Z), np.asarray(S2.values)
+        assert np.all(np.tril(T2, -2) == 0)
+        assert np.linalg.norm(Z2 @ T2 @ Z2.T - A) / (n * np.linalg.norm(A) * EPS) < 10
+        assert np.linalg.norm(np.eye(n) - Z2 @ Z2.T) / (n * EPS) < 10
+        assert checks.checkeigvals(T2, w2, 10)
+        want = wv[select]
+        wscale = np.abs(w2).max()
+        for k in range(int(select.sum())):
+            assert np.min(np.abs(want - w2[k])) / (EPS + wscale) < 10

@@ -835,3 +835,102 @@ def test_heev_eigensystem(oracle, cplx):
     w, V, _ = oracle.heev(D, "L")
     assert np.array_equal(w, [-1.0, -1.0, 2.0, 3.0])
     assert np.array_equal(np.abs(V), np.eye(4)[:, [1, 3, 2, 0]])
+
+
+# ------------------------------------------------ real reordering pins
+# gordschur! for real T (rordschur.jl:3-466): LAPACK xTRSEN (whose xTREXC /
+# xLAEXC / xLASY2 the reference translates) on the same (T, Z) must give the
+# same eigenvalue order and the same T and Z up to the signs the two xLARTG
+# conventions give; the reference's own checkord (test/ordschur.jl:89-133);
+# _syl1or2 against the Sylvester residual and scipy's solver.
+
+
+def _real_schur_pairs(A):
+    T, Z, *_ = sla.schur(A)
+    return np.asfortranarray(T), np.asfortranarray(Z)
+
+
+@pytest.mark.parametrize("n1,n2", [(1, 2), (2, 1), (2, 2)])
+def test_syl1or2_residual(oracle, n1, n2):
+    """_syl1or2 (xLASY2): Tl X - X Tr = scale B at rounding level and X
+    against scale times scipy's solve_sylvester (a transposed operand, a
+    wrong pivot table entry or a sign slip fails).  scale = 1 for the 1 x 2
+    and 2 x 1 forms on well-scaled data; the 4 x 4 form compares 8 smallnum
+    |b| with the signed pivot (rordschur.jl:406-411, where xLASY2 takes its
+    absolute value), so a negative pivot gives scale = 1/(8 max |b|)."""
+    r = np.random.default_rng(50 + 3 * n1 + n2)
+    for _ in range(200):
+        Tl = r.standard_normal((n1, n1)) * 2
+        Tr = r.standard_normal((n2, n2)) * 2 + 6 * np.eye(n2)
+        B = r.standard_normal((n1, n2))
+        X, scale = oracle.syl1or2(Tl, Tr, B)
+        res = Tl @ X - X @ Tr - scale * B
+        assert np.abs(res).max() <= 64 * EPS * (np.abs(Tl).max() + np.abs(Tr).max()) * np.abs(X).max() + 64 * EPS
+        if n1 + n2 == 3:
+            assert scale == 1.0
+        else:
+            assert 0.0 < scale <= 1.0
+        Xs = sla.solve_sylvester(Tl, -Tr, B)
+        assert np.abs(X - scale * Xs).max() <= 1e-10 * max(1.0, np.abs(Xs).max())
+
+
+@pytest.mark.parametrize("n", [6, 20, 40])
+def test_ordschur_real_matches_dtrsen(oracle, n):
+    """The real gordschur! against LAPACK xTRSEN: the same eigenvalue order
+    and |T|, |Z| equal to rounding (LAPACK 3.10's xLARTG differs only in the
+    rotation's sign), eigenvalues as the reference computes them from the
+    standard blocks."""
+    r = np.random.default_rng(46 + n)
+    A = r.standard_normal((n, n))
+    T, Z = _real_schur_pairs(A)
+    for trial in range(4):
+        sel = r.random(n) < 0.35
+        T2, Z2, w, info = oracle.ordschur_re(T, Z, sel)
+        assert info == 0
+        out = lap.dtrsen(sel.astype(np.int32), T, Z, job="N", lwork=max(1, n), liwork=1)
+        Tl, Zl, wr, wi, info_l = out[0], out[1], out[2], out[3], out[-1]
+        assert info_l == 0
+        assert np.abs(w - (wr + 1j * wi)).max() <= 100 * n * EPS * np.abs(T).max()
+        assert np.abs(np.abs(T2) - np.abs(Tl)).max() <= 1e-12 * np.abs(T).max()
+        assert np.abs(np.abs(Z2) - np.abs(Zl)).max() <= 1e-12
+
+
+def test_ordschur_real_checkord(oracle):
+    """test/ordschur.jl:89-133 (n = 20, Float64): two reals, one complex pair,
+    a real and a pair, two pairs moved to the front; quasi-triangular,
+    backward error and orthogonality below 10, checkeigvals, standard blocks,
+    the wanted eigenvalues first."""
+    ulp = EPS
+    r = np.random.default_rng(47)
+    n = 20
+    A = r.random((n, n))
+    T, Z, wv, info, _ = oracle.gschur(A)
+    assert info == 0
+    for nr, ni in ((2, 0), (0, 2), (1, 2), (0, 4)):
+        select = np.zeros(n, dtype=bool)
+        i = n
+        while nr + ni > 0 and i > 1:
+            i -= 1
+            if nr > 0 and wv[i].imag == 0:
+                select[i] = True
+                nr -= 1
+            if ni > 0:
+                iwi = wv[i].imag
+                if abs(iwi) > 1000 * ulp and wv[i - 1].imag == -iwi:
+                    select[i] = True
+                    select[i - 1] = True
+                    ni -= 2
+                    i -= 1
+        if nr + ni > 0:
+            continue
+        T2, Z2, w2, info = oracle.ordschur_re(T, Z, select)
+        assert info == 0
+        assert np.all(np.tril(T2, -2) == 0)
+        assert np.linalg.norm(Z2 @ T2 @ Z2.T - A) / (n * np.linalg.norm(A) * ulp) < 10
+        assert np.linalg.norm(np.eye(n) - Z2 @ Z2.T) / (n * ulp) < 10
+        assert checks.checkeigvals(T2, w2, 10)
+        assert checks.checkblocks(T2)
+        want = wv[select]
+        wscale = np.abs(w2).max()
+        for k in range(int(select.sum())):
+            assert np.min(np.abs(want - w2[k])) / (ulp + wscale) < 10
