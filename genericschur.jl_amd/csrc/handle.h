@@ -39,7 +39,7 @@ struct DevBuf {
 
 struct Plan;
 constexpr int kBranches = 4;
-constexpr int kAuxBufs = 13;
+constexpr int kAuxBufs = 14;
 
 // Launch parameters of batched calls set by gs_set_tuning (a table written
 // by the autotuner); zero or -1 fields keep the built-in defaults.

@@ -250,27 +250,19 @@ GS_DEV StMem st_mem(void* base, int cap, int* ctrl) {
 }
 constexpr size_t st_log_bytes(int cap) { return (size_t)cap * 56; }
 
-// The QL/QR iteration on d, e (LDS) with the rotations applied to nrows
-// rows of Z by waves 1 .. NT/64 - 1; zat(r, k) is the element of row r in
-// the reference's column k (1-based).  Returns info (thread 0's value is
-// the one that counts; every thread returns it).
+// The QL/QR iteration on d, e (LDS, stride 1) from the producer state S
+// (thread 0's copy counts) with the rotations applied to nrows rows of Z by
+// waves 1 .. NT/64 - 1; zat(r, k) is the element of row r in the
+// reference's column k (1-based).  On return thread 0's S is done.
 template <int NT, class TZ, class ZAt>
-GS_DEV void st_solve(int n, double* d, double* e, bool wantz, int nrows, ZAt zat, const StMem& mem, int maxsweeps,
-                     int* info_out, int* sweeps_out) {
+GS_DEV void st_solve(StState& S, double* d, double* e, bool wantz, int nrows, ZAt zat, const StMem& mem) {
   const int tid = threadIdx.x;
-  StState S;
+  const StVec dv{d, 1}, ev{e, 1};
   if (!wantz) {
-    if (tid == 0) {
-      st_init(S, n, maxsweeps);
-      st_produce(S, d, e, nullptr, 0);
-      *info_out = S.info;
-      *sweeps_out = S.sweeps;
-    }
+    if (tid == 0) st_produce<StLog>(S, dv, ev, nullptr);
     __syncthreads();
     return;
   }
-  const int cap = n;
-  if (tid == 0) st_init(S, n, maxsweeps);
   auto apply = [&](int bj) {
     StLog L;
     L.c = mem.c(bj);
@@ -278,6 +270,7 @@ GS_DEV void st_solve(int n, double* d, double* e, bool wantz, int nrows, ZAt zat
     L.seg = mem.seg(bj);
     L.nrot = mem.ctrl[bj];
     L.nseg = mem.ctrl[2 + bj];
+    L.cap = mem.cap;
     for (int r = tid - 64; r < nrows; r += NT - 64) st_apply_row<TZ>(L, [&](int k) -> TZ& { return zat(r, k); });
   };
   for (int k = 0;; ++k) {
@@ -289,7 +282,8 @@ GS_DEV void st_solve(int n, double* d, double* e, bool wantz, int nrows, ZAt zat
       L.seg = mem.seg(bi);
       L.nrot = 0;
       L.nseg = 0;
-      st_produce(S, d, e, &L, cap);
+      L.cap = mem.cap;
+      st_produce(S, dv, ev, &L);
       mem.ctrl[bi] = L.nrot;
       mem.ctrl[2 + bi] = L.nseg;
       mem.ctrl[4 + bi] = S.done ? 1 : 0;
@@ -303,19 +297,16 @@ GS_DEV void st_solve(int n, double* d, double* e, bool wantz, int nrows, ZAt zat
       break;
     }
   }
-  if (tid == 0) {
-    *info_out = S.info;
-    *sweeps_out = S.sweeps;
-  }
-  __syncthreads();
 }
 
 }  // namespace
 
 // ------------------------------------------------------------------------
-// geigen! / geigvals! / _hehessenberg! + _materializeQ of `count` matrices
-// of order n, one workgroup per matrix.
-template <class T, int NT, int RM> __global__ void __launch_bounds__(NT) heev_kernel(HeevArgs a) {
+// Phase 1: _hehessenberg! + _materializeQ of `count` matrices of order n,
+// one workgroup per matrix, the matrix in LDS.  Writes d (W, stride sW), e
+// (E, stride sE) in the reference's order and, when Z is non-null, Q (Z,
+// ld ldz, stride sZ) in the reference's coordinates.
+template <class T, int NT, int RM> __global__ void __launch_bounds__(NT) heev_tq_kernel(HeevArgs a) {
   extern __shared__ double smem[];
   const int b = blockIdx.x;
   const int n = a.n, ld = n | 1, tid = threadIdx.x;
@@ -325,13 +316,9 @@ template <class T, int NT, int RM> __global__ void __launch_bounds__(NT) heev_ke
   T* tau = reinterpret_cast<T*>(ub);
   T* v = tau + n;
   T* w = v + n;
-  const size_t ubytes = heev_union_bytes(n, sizeof(T));
-  double* d = reinterpret_cast<double*>(ub + ubytes);
+  double* d = reinterpret_cast<double*>(ub + heev_union_bytes(n, sizeof(T)));
   double* e = d + n;
   double* red = e + n;
-  int* ctrl = reinterpret_cast<int*>(red + 16);
-  int* sh = ctrl + 8;  // info, sweeps
-
   // lower triangle of the working matrix (J A J for the upper form)
   const T* Ag = static_cast<const T*>(a.A) + (int64_t)b * a.sA;
   for (int idx = tid; idx < n * n; idx += NT) {
@@ -342,40 +329,92 @@ template <class T, int NT, int RM> __global__ void __launch_bounds__(NT) heev_ke
   he_tridiag<NT, T>(M, ld, n, tau, v, w, d, e, red);
   const bool wantq = a.Z != nullptr;
   if (wantq) form_q<NT, RM>(M, ld, n, tau, w);
-  if (upper) {
-    for (int i = tid; i < n / 2; i += NT) {
-      const double t = d[i];
-      d[i] = d[n - 1 - i];
-      d[n - 1 - i] = t;
-    }
-    for (int i = tid; i < (n - 1) / 2; i += NT) {
-      const double t = e[i];
-      e[i] = e[n - 2 - i];
-      e[n - 2 - i] = t;
+  // d, e in the reference's order (reversed for the upper form)
+  for (int i = tid; i < n; i += NT) a.W[(int64_t)b * a.sW + i] = upper ? d[n - 1 - i] : d[i];
+  for (int i = tid; i < n - 1; i += NT) a.E[(int64_t)b * a.sE + i] = upper ? e[n - 2 - i] : e[i];
+  if (wantq) {
+    T* Zg = static_cast<T*>(a.Z) + (int64_t)b * a.sZ;
+    for (int idx = tid; idx < n * n; idx += NT) {
+      const int r = idx % n, j = idx / n;
+      Zg[r + (int64_t)j * a.ldz] = upper ? M[(n - 1 - r) + (int64_t)(n - 1 - j) * ld] : M[r + (int64_t)j * ld];
     }
   }
+}
+
+// Phase 2: the QL/QR recurrence of 64 matrices per workgroup, one per lane
+// (symtridiag.jl:77-427 via st_step), d and e interleaved in LDS (entry i of
+// lane l at [i * 64 + l]), each lane's rotations to its record stream in
+// HBM (LOG, cap records per matrix).  A lane whose stream would overflow
+// stops at a step boundary and leaves its state for phase 3.
+__global__ void __launch_bounds__(64) steqr_lanes_kernel(HeevArgs a) {
+  extern __shared__ double smem[];
+  const int lane = threadIdx.x, n = a.n;
+  const int b = blockIdx.x * 64 + lane;
+  if (b >= a.count) return;
+  const StVec d{smem + lane, 64}, e{smem + (size_t)64 * n + lane, 64};
+  for (int i = 0; i < n; ++i) d[i] = a.W[(int64_t)b * a.sW + i];
+  for (int i = 0; i < n - 1; ++i) e[i] = a.E[(int64_t)b * a.sE + i];
+  StState S;
+  st_init(S, n, a.maxsweeps);
+  StState* states = static_cast<StState*>(a.state);
+  if (a.log) {
+    StLogG L;
+    L.rec = static_cast<double2*>(a.log) + (int64_t)b * a.logcap;
+    L.pos = 0;
+    L.cap = a.logcap;
+    st_produce(S, d, e, &L);
+    a.logpos[b] = L.pos;
+  } else {
+    st_produce<StLogG>(S, d, e, nullptr);
+  }
+  for (int i = 0; i < n; ++i) a.W[(int64_t)b * a.sW + i] = d[i];
+  for (int i = 0; i < n - 1; ++i) a.E[(int64_t)b * a.sE + i] = e[i];
+  states[b] = S;
+}
+
+// Phase 3: per matrix, the record stream replayed onto Q's rows (Q in LDS,
+// one row per thread), the recurrence finished in place for a matrix whose
+// stream overflowed (producer lane and appliers as for one matrix), the
+// stable sort (sorteig!) and the write-out of W, V and info.
+template <class T, int NT> __global__ void __launch_bounds__(NT) heev_apply_kernel(HeevArgs a) {
+  extern __shared__ double smem[];
+  const int b = blockIdx.x;
+  const int n = a.n, ld = n | 1, tid = threadIdx.x;
+  const bool wantz = a.Z != nullptr;
+  T* M = reinterpret_cast<T*>(smem);
+  char* ub = reinterpret_cast<char*>(M + (wantz ? (size_t)ld * n : 0));
+  double* d = reinterpret_cast<double*>(ub);
+  double* e = d + n;
+  int* ctrl = reinterpret_cast<int*>(e + n);
+  int* perm = ctrl + 8;
+  void* logb = static_cast<void*>(perm + ((n + 3) / 4) * 4);
+  T* Zg = static_cast<T*>(a.Z) + (wantz ? (int64_t)b * a.sZ : 0);
+  for (int i = tid; i < n; i += NT) d[i] = a.W[(int64_t)b * a.sW + i];
+  for (int i = tid; i < n - 1; i += NT) e[i] = a.E[(int64_t)b * a.sE + i];
+  if (wantz)
+    for (int idx = tid; idx < n * n; idx += NT) {
+      const int r = idx % n, j = idx / n;
+      M[r + (int64_t)j * ld] = Zg[r + (int64_t)j * a.ldz];
+    }
   __syncthreads();
-  // storage row / column of the reference's row r (0-based), column k (1-based)
-  auto srow = [&](int r) { return upper ? n - 1 - r : r; };
-  auto scol = [&](int k) { return upper ? n - k : k - 1; };
-  T* Zg = static_cast<T*>(a.Z);
-  if (a.mode == 2) {
-    // _hehessenberg! + _materializeQ: d, e, Q
-    for (int i = tid; i < n; i += NT) a.W[(int64_t)b * a.sW + i] = d[i];
-    for (int i = tid; i < n - 1; i += NT) a.E[(int64_t)b * a.sE + i] = e[i];
-    if (wantq)
-      for (int idx = tid; idx < n * n; idx += NT) {
-        const int r = idx % n, j = idx / n;
-        Zg[(int64_t)b * a.sZ + r + (int64_t)j * a.ldz] = M[srow(r) + (int64_t)scol(j + 1) * ld];
-      }
-    return;
+  StState* states = static_cast<StState*>(a.state);
+  StState S = states[b];
+  if (wantz) {
+    const double2* rec = static_cast<const double2*>(a.log) + (int64_t)b * a.logcap;
+    const int npos = a.logpos[b];
+    for (int r = tid; r < n; r += NT)
+      st_replay_row<T>(rec, npos, [&](int k) -> T& { return M[r + (int64_t)(k - 1) * ld]; });
+    __syncthreads();
   }
-  const StMem mem = st_mem(ub, n, ctrl);
-  st_solve<NT, T>(
-      n, d, e, wantq, n, [&](int r, int k) -> T& { return M[r + (int64_t)scol(k) * ld]; }, mem, a.maxsweeps, sh,
-      sh + 1);
+  if (!S.done) {
+    // the stream overflowed: finish the recurrence here
+    const StMem mem = st_mem(logb, n, ctrl);
+    st_solve<NT, T>(S, d, e, wantz, n, [&](int r, int k) -> T& { return M[r + (int64_t)(k - 1) * ld]; }, mem);
+    if (tid == 0) states[b] = S;
+    __syncthreads();
+    S = states[b];
+  }
   // sorteig!: stable ascending order under isless; perm[j] = source column
-  int* perm = reinterpret_cast<int*>(ub);
   for (int i = tid; i < n; i += NT) {
     int rank = i;
     if (a.sort) {
@@ -390,14 +429,14 @@ template <class T, int NT, int RM> __global__ void __launch_bounds__(NT) heev_ke
   }
   __syncthreads();
   for (int j = tid; j < n; j += NT) a.W[(int64_t)b * a.sW + j] = d[perm[j]];
-  if (wantq)
+  if (wantz)
     for (int idx = tid; idx < n * n; idx += NT) {
       const int r = idx % n, j = idx / n;
-      Zg[(int64_t)b * a.sZ + r + (int64_t)j * a.ldz] = M[srow(r) + (int64_t)scol(perm[j] + 1) * ld];
+      Zg[r + (int64_t)j * a.ldz] = M[r + (int64_t)perm[j] * ld];
     }
   if (tid == 0) {
-    if (a.info) a.info[b] = sh[0];
-    if (a.sweeps) a.sweeps[b] = sh[1];
+    if (a.info) a.info[b] = S.info;
+    if (a.sweeps) a.sweeps[b] = S.sweeps;
   }
 }
 
@@ -409,44 +448,63 @@ template <class TZ, int NT> __global__ void __launch_bounds__(NT) steqr_kernel(S
   double* d = smem;
   double* e = d + n;
   int* ctrl = reinterpret_cast<int*>(e + n);
-  int* sh = ctrl + 8;
   void* logb = static_cast<void*>(ctrl + 16);
   for (int i = tid; i < n; i += NT) d[i] = a.d[i];
   for (int i = tid; i < n - 1; i += NT) e[i] = a.e[i];
   __syncthreads();
   TZ* Z = static_cast<TZ*>(a.Z);
   const StMem mem = st_mem(logb, n, ctrl);
-  st_solve<NT, TZ>(
-      n, d, e, Z != nullptr, a.zrows, [&](int r, int k) -> TZ& { return Z[r + (int64_t)(k - 1) * a.ldz]; }, mem,
-      a.maxsweeps, sh, sh + 1);
+  StState S;
+  if (tid == 0) st_init(S, n, a.maxsweeps);
+  st_solve<NT, TZ>(S, d, e, Z != nullptr, a.zrows, [&](int r, int k) -> TZ& { return Z[r + (int64_t)(k - 1) * a.ldz]; },
+                   mem);
   for (int i = tid; i < n; i += NT) a.d[i] = d[i];
   for (int i = tid; i < n - 1; i += NT) a.e[i] = e[i];
   if (tid == 0) {
-    if (a.info) a.info[0] = sh[0];
-    if (a.sweeps) a.sweeps[0] = sh[1];
+    if (a.info) a.info[0] = S.info;
+    if (a.sweeps) a.sweeps[0] = S.sweeps;
   }
 }
 
-size_t heev_lds_bytes(int n, bool cplx) {
+size_t heev_tq_lds_bytes(int n, bool cplx) {
   const size_t es = cplx ? 16 : 8;
   const size_t ld = (size_t)(n | 1);
-  return ld * n * es + heev_union_bytes(n, es) + 16 * (size_t)n + 16 * 8 + 16 * 4 + 64;
+  return ld * n * es + heev_union_bytes(n, es) + 16 * (size_t)n + 16 * 8 + 64;
+}
+
+size_t heev_apply_lds_bytes(int n, bool cplx, bool wantz) {
+  const size_t es = cplx ? 16 : 8;
+  const size_t ld = (size_t)(n | 1);
+  return (wantz ? ld * n * es : 0) + 16 * (size_t)n + 32 + 4 * (size_t)((n + 3) / 4 * 4) + st_log_bytes(n) + 64;
 }
 
 size_t steqr_lds_bytes(int n) { return 16 * (size_t)n + 16 * 4 + st_log_bytes(n) + 64; }
 
-template <class T, int NT, int RM> static hipError_t launch_heev_t(const HeevArgs& a, size_t lds, hipStream_t s) {
-  (void)hipFuncSetAttribute((const void*)heev_kernel<T, NT, RM>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((heev_kernel<T, NT, RM>), dim3(a.count), dim3(NT), lds, s, a);
+size_t heev_log_records(int n) { return 2 * (size_t)n * n + 8 * (size_t)n; }
+
+size_t heev_state_bytes() { return (sizeof(StState) + 15) / 16 * 16; }
+
+template <class T, int NT, int RM> static hipError_t launch_heev_t(const HeevArgs& a, bool cplx, hipStream_t s) {
+  const size_t l1 = heev_tq_lds_bytes(a.n, cplx);
+  (void)hipFuncSetAttribute((const void*)heev_tq_kernel<T, NT, RM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)l1);
+  hipLaunchKernelGGL((heev_tq_kernel<T, NT, RM>), dim3(a.count), dim3(NT), l1, s, a);
+  if (a.mode == 2) return hipGetLastError();
+  const size_t l2 = (size_t)1024 * a.n;
+  (void)hipFuncSetAttribute((const void*)steqr_lanes_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l2);
+  hipLaunchKernelGGL(steqr_lanes_kernel, dim3((a.count + 63) / 64), dim3(64), l2, s, a);
+  const size_t l3 = heev_apply_lds_bytes(a.n, cplx, a.Z != nullptr);
+  (void)hipFuncSetAttribute((const void*)heev_apply_kernel<T, 128>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)l3);
+  hipLaunchKernelGGL((heev_apply_kernel<T, 128>), dim3(a.count), dim3(128), l3, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_heev(const HeevArgs& a, bool cplx, hipStream_t s) {
   if (a.count <= 0 || a.n <= 0) return hipSuccess;
   if (a.n > heev_max_n(cplx)) return hipErrorInvalidValue;
-  const size_t lds = heev_lds_bytes(a.n, cplx);
-  if (cplx) return launch_heev_t<zd, 256, 2>(a, lds, s);
-  return launch_heev_t<double, 256, 2>(a, lds, s);
+  if (cplx) return launch_heev_t<zd, 256, 2>(a, true, s);
+  return launch_heev_t<double, 256, 2>(a, false, s);
 }
 
 hipError_t launch_steqr(const SteqrArgs& a, bool zcplx, hipStream_t s) {

@@ -1,10 +1,12 @@
 // C entry points of the Hermitian / real symmetric eigenproblem (gs_heev_z,
 // gs_heev_d, gs_hetrd_z, gs_hetrd_d, gs_steqr), declared in
 // include/gschur_mi355x.h.  Host pointers are staged through handle buffer
-// 12, device pointers (GS_DEVICE_PTRS) used directly.
+// 12, device pointers (GS_DEVICE_PTRS) used directly; buffer 13 holds the
+// eigensolver's workspace (e, the per-matrix rotation streams and states).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "../../include/gschur_mi355x.h"
@@ -47,6 +49,23 @@ int heev_impl(gs_handle* h, bool cplx, int64_t count, int64_t n, const double* A
   a.sort = (flags & GS_SORT) ? 1 : 0;
   a.maxsweeps = 30 * (int)n;
   a.sW = n;
+  // workspace: e (count x n), stream lengths, states, streams (with vectors)
+  {
+    const size_t bE = up256(cnt * (size_t)n * 8), bP = up256(cnt * 4), bS = up256(cnt * heev_state_bytes());
+    size_t cap = heev_log_records((int)n);
+    // testing: a shorter stream forces the in-place finish of phase 3
+    if (const char* ev = std::getenv("GSCHUR_HEEV_LOGCAP")) cap = std::max<size_t>(1, std::strtoul(ev, nullptr, 10));
+    const size_t bL = wantv ? cnt * cap * 16 : 0;
+    DevBuf& ws = handle_buf(h, 13);
+    HCHK(ws.ensure(bE + bP + bS + bL));
+    char* q = static_cast<char*>(ws.p);
+    a.E = reinterpret_cast<double*>(q);
+    a.sE = n;
+    a.logpos = reinterpret_cast<int*>(q + bE);
+    a.state = q + bE + bP;
+    a.log = wantv ? static_cast<void*>(q + bE + bP + bS) : nullptr;
+    a.logcap = (int)cap;
+  }
   if (dev) {
     a.A = A;
     a.lda = lda;

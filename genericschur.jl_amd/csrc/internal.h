@@ -171,8 +171,8 @@ hipError_t launch_enormalize(int n, int m, double* V, int64_t ldv, hipStream_t s
 // and eigvalscond / subspacesep (work: 3 nsub (n - nsub) complex entries,
 // out: s, sep), one wavefront each
 // Hermitian / real symmetric eigenproblem (heev.hip).  mode 0: eigenvalues
-// W (and eigenvectors Z when non-null); mode 2: the tridiagonal matrix (W =
-// d, E = e) and Q (Z when non-null).  A: lower (upper = 0) or upper triangle
+// W (and eigenvectors Z when non-null; E is workspace of count x n); mode 2:
+// the tridiagonal matrix (W = d, E = e) and Q (Z when non-null).  A: lower (upper = 0) or upper triangle
 // of each Hermitian matrix (lda, stride sA elements).
 struct HeevArgs {
   int count = 0, n = 0, upper = 0, mode = 0, sort = 0, maxsweeps = 0;
@@ -186,6 +186,12 @@ struct HeevArgs {
   int64_t ldz = 0, sZ = 0;
   int* info = nullptr;
   int* sweeps = nullptr;
+  // mode 0 workspace: per matrix a rotation record stream (logcap records
+  // of 16 bytes; null: eigenvalues only), its length, the QL/QR state
+  void* log = nullptr;
+  int logcap = 0;
+  int* logpos = nullptr;
+  void* state = nullptr;
 };
 struct SteqrArgs {
   int n = 0, zrows = 0, maxsweeps = 0;
@@ -202,8 +208,9 @@ constexpr size_t heev_union_bytes(int n, size_t es) {
 }
 constexpr int heev_max_n(bool cplx) { return cplx ? 96 : 128; }
 constexpr int steqr_max_n() { return 2048; }
-size_t heev_lds_bytes(int n, bool cplx);
 size_t steqr_lds_bytes(int n);
+size_t heev_log_records(int n);   // records per matrix of the mode 0 stream
+size_t heev_state_bytes();        // bytes per matrix of the QL/QR state
 hipError_t launch_heev(const HeevArgs& a, bool cplx, hipStream_t s);
 hipError_t launch_steqr(const SteqrArgs& a, bool zcplx, hipStream_t s);
 

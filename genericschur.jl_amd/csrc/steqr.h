@@ -143,8 +143,17 @@ GS_DEV void sym2x2(double a, double b, double c, bool wantvec, double& rt1, doub
   }
 }
 
-// safescale!(x, cfrom, cto) (util.jl:41-80) on len entries of x
-GS_DEV void st_safescale(double* x, int len, double cfrom, double cto) {
+// d or e of the recurrence: entry i at p[i * stride] (stride 1 in LDS for
+// one matrix, the lane count where each lane of a wavefront owns a matrix
+// and the arrays are interleaved)
+struct StVec {
+  double* p;
+  int stride;
+  GS_DEV double& operator[](int i) const { return p[i * stride]; }
+};
+
+// safescale!(x, cfrom, cto) (util.jl:41-80) on x[i0 .. i0 + len - 1]
+GS_DEV void st_safescale(StVec x, int i0, int len, double cfrom, double cto) {
   const double smlnum = GS_FMIN, bignum = 1.0 / smlnum;
   double cfromc = cfrom, ctoc = cto;
   bool done = false;
@@ -171,18 +180,50 @@ GS_DEV void st_safescale(double* x, int len, double cfrom, double cto) {
         done = true;
       }
     }
-    for (int i = 0; i < len; ++i) x[i] = x[i] * mul;
+    for (int i = 0; i < len; ++i) x[i0 + i] = x[i0 + i] * mul;
   }
 }
 
-// Rotation log: segment s covers rotations rot[off, off + cnt) on column
-// pairs (i, i+1) (1-based) with i = i0, i0 + dir, ...; rotation (c, s)
-// maps Z[:, i] = c a1 + s a2, Z[:, i+1] = -s a1 + c a2.
+// Rotation logs.  A segment is cnt rotations on column pairs (i, i+1)
+// (1-based) with i = i0, i0 + dir, ...; rotation (c, s) maps
+// Z[:, i] = c a1 + s a2, Z[:, i+1] = -s a1 + c a2.  The producer calls
+// begin() before a segment's rotations, put() for each, end() after.
+//
+// StLog: in LDS, rotations in c[] and s[], segments in seg[] (i0, cnt, dir).
 struct StLog {
   double* c;   // cap
   double* s;   // cap
   int* seg;    // 3 * cap: i0, cnt, dir
-  int nrot, nseg;
+  int nrot, nseg, cap;
+  GS_DEV bool room(int bound) const { return nrot + bound <= cap; }
+  GS_DEV int begin() const { return nrot; }
+  GS_DEV void put(int base, int k, double cc, double ss) {
+    c[base + k] = cc;
+    s[base + k] = ss;
+  }
+  GS_DEV void end(int base, int i0, int cnt, int dir) {
+    (void)base;
+    seg[3 * nseg] = i0;
+    seg[3 * nseg + 1] = cnt;
+    seg[3 * nseg + 2] = dir;
+    nrot += cnt;
+    nseg += 1;
+  }
+};
+
+// StLogG: one matrix's record stream in global memory, 16-byte records: a
+// segment header (i0, dir * cnt, as doubles) followed by its rotations
+// (c, s).  pos counts the records written.
+struct StLogG {
+  double2* rec;
+  int pos, cap;
+  GS_DEV bool room(int bound) const { return pos + 1 + bound <= cap; }
+  GS_DEV int begin() const { return pos + 1; }
+  GS_DEV void put(int base, int k, double cc, double ss) { rec[base + k] = make_double2(cc, ss); }
+  GS_DEV void end(int base, int i0, int cnt, int dir) {
+    rec[base - 1] = make_double2((double)i0, (double)(dir * cnt));
+    pos = base + cnt;
+  }
 };
 
 // Producer state (symtridiag.jl:77-172 unrolled into resumable steps).
@@ -209,9 +250,9 @@ GS_DEV void st_init(StState& S, int n, int maxsweeps) {
 GS_DEV int st_next_bound(const StState& S) { return S.inblk ? abs(S.lend - S.l) : S.n - 1; }
 
 // One step of the producer: the set-up of the next unreduced block, one QL
-// or QR iteration, or the end of a block.  d, e: 0-based arrays; D(i) =
-// d[i-1] is the reference's 1-based d[i].  Rotations go to lg when non-null.
-GS_DEV void st_step(StState& S, double* d, double* e, StLog* lg) {
+// or QR iteration, or the end of a block.  d, e: 0-based; D(i) = d[i-1] is
+// the reference's 1-based d[i].  Rotations go to lg when non-null.
+template <class Log> GS_DEV void st_step(StState& S, StVec d, StVec e, Log* lg) {
 #pragma clang fp contract(off)
   const int n = S.n;
   const double eps = GS_EPS, eps2 = GS_EPS * GS_EPS, safmin = GS_FMIN;
@@ -259,12 +300,12 @@ GS_DEV void st_step(StState& S, double* d, double* e, StLog* lg) {
     if (anorm == 0.0) return;
     if (anorm > ssfmax) {
       S.iscale = 1;
-      st_safescale(&D(S.l), S.lend - S.l + 1, anorm, ssfmax);
-      st_safescale(&E(S.l), S.lend - S.l, anorm, ssfmax);
+      st_safescale(d, S.l - 1, S.lend - S.l + 1, anorm, ssfmax);
+      st_safescale(e, S.l - 1, S.lend - S.l, anorm, ssfmax);
     } else if (anorm < ssfmin) {
       S.iscale = 2;
-      st_safescale(&D(S.l), S.lend - S.l + 1, anorm, ssfmin);
-      st_safescale(&E(S.l), S.lend - S.l, anorm, ssfmin);
+      st_safescale(d, S.l - 1, S.lend - S.l + 1, anorm, ssfmin);
+      st_safescale(e, S.l - 1, S.lend - S.l, anorm, ssfmin);
     }
     if (fabs(D(S.lend)) < fabs(D(S.l))) {
       S.lend = S.lsave;
@@ -279,11 +320,11 @@ GS_DEV void st_step(StState& S, double* d, double* e, StLog* lg) {
     // end of the block: undo the scaling
     const int ls = S.lsave, le = S.lendsave;
     if (S.iscale == 1) {
-      st_safescale(&D(ls), le - ls + 1, ssfmax, S.anorm);
-      st_safescale(&E(ls), le - ls, ssfmax, S.anorm);
+      st_safescale(d, ls - 1, le - ls + 1, ssfmax, S.anorm);
+      st_safescale(e, ls - 1, le - ls, ssfmax, S.anorm);
     } else if (S.iscale == 2) {
-      st_safescale(&D(ls), le - ls + 1, ssfmin, S.anorm);
-      st_safescale(&E(ls), le - ls, ssfmin, S.anorm);
+      st_safescale(d, ls - 1, le - ls + 1, ssfmin, S.anorm);
+      st_safescale(e, ls - 1, le - ls, ssfmin, S.anorm);
     }
     S.inblk = false;
     if (S.stop) {
@@ -313,13 +354,9 @@ GS_DEV void st_step(StState& S, double* d, double* e, StLog* lg) {
         double rt1, rt2, cs, sn;
         sym2x2(D(l), E(l), D(l + 1), wantz, rt1, rt2, cs, sn);
         if (wantz) {
-          lg->c[lg->nrot] = cs;
-          lg->s[lg->nrot] = sn;
-          lg->seg[3 * lg->nseg] = l;
-          lg->seg[3 * lg->nseg + 1] = 1;
-          lg->seg[3 * lg->nseg + 2] = 1;
-          lg->nrot += 1;
-          lg->nseg += 1;
+          const int base = lg->begin();
+          lg->put(base, 0, cs, sn);
+          lg->end(base, l, 1, 1);
         }
         D(l) = rt1;
         D(l + 1) = rt2;
@@ -337,7 +374,7 @@ GS_DEV void st_step(StState& S, double* d, double* e, StLog* lg) {
       g = D(msplit) - p + (E(l) / (g + copysign(r, g)));
       double s = 1.0, c = 1.0;
       p = 0.0;
-      const int r0 = wantz ? lg->nrot : 0;
+      const int r0 = wantz ? lg->begin() : 0;
       for (int i = msplit - 1; i >= l; --i) {
         const double f = s * E(i);
         const double b = c * E(i);
@@ -348,19 +385,9 @@ GS_DEV void st_step(StState& S, double* d, double* e, StLog* lg) {
         p = s * r;
         D(i + 1) = g + p;
         g = c * r - b;
-        if (wantz) {
-          const int k = r0 + (msplit - 1 - i);
-          lg->c[k] = c;
-          lg->s[k] = -s;
-        }
+        if (wantz) lg->put(r0, msplit - 1 - i, c, -s);
       }
-      if (wantz) {
-        lg->seg[3 * lg->nseg] = msplit - 1;
-        lg->seg[3 * lg->nseg + 1] = msplit - l;
-        lg->seg[3 * lg->nseg + 2] = -1;
-        lg->nrot += msplit - l;
-        lg->nseg += 1;
-      }
+      if (wantz) lg->end(r0, msplit - 1, msplit - l, -1);
       D(l) = D(l) - p;
       E(l) = g;
     } else {
@@ -383,13 +410,9 @@ GS_DEV void st_step(StState& S, double* d, double* e, StLog* lg) {
         double rt1, rt2, cs, sn;
         sym2x2(D(l - 1), E(l - 1), D(l), wantz, rt1, rt2, cs, sn);
         if (wantz) {
-          lg->c[lg->nrot] = cs;
-          lg->s[lg->nrot] = sn;
-          lg->seg[3 * lg->nseg] = l - 1;
-          lg->seg[3 * lg->nseg + 1] = 1;
-          lg->seg[3 * lg->nseg + 2] = 1;
-          lg->nrot += 1;
-          lg->nseg += 1;
+          const int base = lg->begin();
+          lg->put(base, 0, cs, sn);
+          lg->end(base, l - 1, 1, 1);
         }
         D(l - 1) = rt1;
         D(l) = rt2;
@@ -407,7 +430,7 @@ GS_DEV void st_step(StState& S, double* d, double* e, StLog* lg) {
       g = D(msplit) - p + (E(l - 1) / (g + copysign(r, g)));
       double s = 1.0, c = 1.0;
       p = 0.0;
-      const int r0 = wantz ? lg->nrot : 0;
+      const int r0 = wantz ? lg->begin() : 0;
       for (int i = msplit; i <= l - 1; ++i) {
         const double f = s * E(i);
         const double b = c * E(i);
@@ -418,19 +441,9 @@ GS_DEV void st_step(StState& S, double* d, double* e, StLog* lg) {
         p = s * r;
         D(i) = g + p;
         g = c * r - b;
-        if (wantz) {
-          const int k = r0 + (i - msplit);
-          lg->c[k] = c;
-          lg->s[k] = s;
-        }
+        if (wantz) lg->put(r0, i - msplit, c, s);
       }
-      if (wantz) {
-        lg->seg[3 * lg->nseg] = msplit;
-        lg->seg[3 * lg->nseg + 1] = l - msplit;
-        lg->seg[3 * lg->nseg + 2] = 1;
-        lg->nrot += l - msplit;
-        lg->nseg += 1;
-      }
+      if (wantz) lg->end(r0, msplit, l - msplit, 1);
       D(l) = D(l) - p;
       E(l - 1) = g;
     } else {
@@ -444,9 +457,9 @@ GS_DEV void st_step(StState& S, double* d, double* e, StLog* lg) {
 
 // Runs the producer until the log cannot take another step's rotations or
 // the recurrence is done.
-GS_DEV void st_produce(StState& S, double* d, double* e, StLog* lg, int cap) {
+template <class Log> GS_DEV void st_produce(StState& S, StVec d, StVec e, Log* lg) {
   while (!S.done) {
-    if (lg && lg->nrot + st_next_bound(S) > cap) break;
+    if (lg && !lg->room(st_next_bound(S))) break;
     st_step(S, d, e, lg);
   }
 }
@@ -491,6 +504,39 @@ template <class TZ, class ZA> GS_DEV void st_apply_row(const StLog& lg, ZA z) {
       z(i0 + cnt) = cur;
     }
     off += cnt;
+  }
+}
+
+// Replays a record stream (StLogG) of npos records onto one row of Z.
+template <class TZ, class ZA> GS_DEV void st_replay_row(const double2* __restrict__ rec, int npos, ZA z) {
+  int q = 0;
+  while (q < npos) {
+    const double2 h = rec[q];
+    const int i0 = (int)h.x, dc = (int)h.y;
+    const int cnt = dc < 0 ? -dc : dc;
+    const double2* r = rec + q + 1;
+    if (dc < 0) {
+      TZ cur = z(i0 + 1);
+      for (int k = 0; k < cnt; ++k) {
+        const int i = i0 - k;
+        const double2 cs = r[k];
+        const TZ a1 = z(i);
+        z(i + 1) = st_rot2(a1, cur, cs.x, cs.y);
+        cur = st_rot1(a1, cur, cs.x, cs.y);
+      }
+      z(i0 - cnt + 1) = cur;
+    } else {
+      TZ cur = z(i0);
+      for (int k = 0; k < cnt; ++k) {
+        const int i = i0 + k;
+        const double2 cs = r[k];
+        const TZ a2 = z(i + 1);
+        z(i) = st_rot1(cur, a2, cs.x, cs.y);
+        cur = st_rot2(cur, a2, cs.x, cs.y);
+      }
+      z(i0 + cnt) = cur;
+    }
+    q += 1 + cnt;
   }
 }
 

@@ -19,8 +19,22 @@ this module                 reference
 ``lmul_(B, V)``, ``ldiv_``  ``lmul!/ldiv!(B::Balancer, V)`` (balance.jl:201-261)
 ``triangularize(S)``        ``triangularize(S::Schur{<:Real})`` (triang.jl:9-43)
 ``enormalize_(V)``          ``_enormalize!(V)`` (util.jl:572-592)
+``ordschur_(S, select)``    ``gordschur!`` / ``ordschur!`` (ordschur.jl:23-73
+                            complex, rordschur.jl:3-45 real)
+``eigvalscond``,            ``eigvalscond(S, nsub)``, ``subspacesep(S, nsub)``
+``subspacesep``             (ordschur.jl:85-131)
+``geigen_hermitian(A)``     ``geigen!(A::Hermitian, QRIteration())``
+                            (GenericSchur.jl:124-135), ``geigvals_hermitian``
+                            (:137-146), ``heev_batched`` (a batch of them)
+``hessenberg_hermitian``    ``_hessenberg!(Hermitian(A))`` + ``_materializeQ``
+                            (hessenberg.jl:19-232)
+``steqr_(d, e, Z)``         ``_gschur!(SymTridiagonal, QRIteration(), Z)``
+                            (symtridiag.jl:77-427); ``gschur_symtridiag``,
+                            ``geigen_symtridiag``, ``geigvals_symtridiag``
+                            (symtridiag.jl:14-55)
 ``Schur``                   ``LinearAlgebra.Schur`` (fields T, Z, values)
 ``UnconvergedException``    ``GenericSchur.UnconvergedException``
+``IllConditionException``   ``GenericSchur.IllConditionException``
 ``DimensionMismatch``       ``Base.DimensionMismatch``
 ``ArgumentError``           ``Base.ArgumentError``
 ==========================  ====================================================
@@ -528,13 +542,18 @@ def ordschur_(S, select, device=0):
 
 
 def eigcond(S, nsub, device=0):
-    """(eigvalscond(S, nsub), subspacesep(S, nsub)) of a complex Schur
-    factorization (ordschur.jl:85-131), 1 <= nsub <= n - 1."""
+    """(eigvalscond(S, nsub), subspacesep(S, nsub)) of a Schur factorization
+    (ordschur.jl:85-131), 1 <= nsub <= n - 1.  The reference's solvers
+    (trsylvester!, adjtrsylvester!, norm1est!) are generic in the element
+    type and read T as triangular: a Float64 T goes through the same kernel
+    as ComplexF64 with zero imaginary parts, whose complex products, Smith
+    divisions, moduli and sign steps then reduce exactly to the real
+    operations."""
     T = np.asarray(S.T)
     n = _checksquare(T)
-    if T.dtype != np.complex128:
-        raise TypeError("eigcond takes a complex Schur factorization")
-    t = np.array(T, order="F", copy=True)
+    if T.dtype not in (np.complex128, np.float64):
+        raise TypeError("eigcond takes a Float64 or ComplexF64 Schur factorization")
+    t = np.array(T, dtype=np.complex128, order="F", copy=True)
     out = np.zeros(2)
     h = _lib.handle(device)
     rc = _lib.lib().gs_eigcond_z(h, n, _ptr(t), max(n, 1), int(nsub), _ptr(out), 0)

@@ -189,3 +189,29 @@ def test_heev_edge_cases(gs, oracle, cplx):
     # above the LDS order: a clear error, no fallback
     with pytest.raises(gs.GSError):
         gs.geigen_hermitian(np.eye(gs.heev_max_n(cplx) + 1, dtype=dt))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_heev_phases_bitwise(gs, cplx, monkeypatch):
+    """The batched solver (tridiagonalization + Q, the lane-parallel QL/QR
+    recurrence writing rotation streams, their replay onto Q, the sort)
+    against the same steps one call each (gs_hetrd, then gs_steqr on
+    (d, e, Q), then a stable sort): bit for bit.  Again with streams too
+    short for the iteration, so that phase 3 finishes it in place."""
+    r = np.random.default_rng(66)
+    n, count = 40, 24
+    A = np.stack([_herm(r, n, cplx) for _ in range(count)])
+    W, V, info = gs.heev_batched(A)
+    assert not info.any()
+    for k in range(count):
+        d, e, Q = gs.hessenberg_hermitian(A[k])
+        w, Z, _, inf1 = gs.steqr_(d, e, Q)
+        assert inf1 == 0
+        p = np.argsort(w, kind="stable")
+        assert np.array_equal(W[k], w[p]), k
+        assert np.array_equal(V[k], Z[:, p]), k
+    for cap in ("1", "300"):
+        monkeypatch.setenv("GSCHUR_HEEV_LOGCAP", cap)
+        W2, V2, info2 = gs.heev_batched(A)
+        assert not info2.any()
+        assert np.array_equal(W2, W) and np.array_equal(V2, V), cap

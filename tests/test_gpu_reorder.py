@@ -153,3 +153,29 @@ def test_ordschur_real_checkord(gs, oracle):
         wscale = np.abs(w2).max()
         for k in range(int(select.sum())):
             assert np.min(np.abs(want - w2[k])) / (EPS + wscale) < 10
+
+
+def test_eigcond_real(gs, oracle):
+    """eigvalscond / subspacesep of a real Schur factorization: with real
+    eigenvalues (T triangular) against LAPACK xTRSEN's s and sep (the
+    reference's checkecond, test/ordschur.jl:31-56); with complex pairs the
+    reference's generic solvers read T as triangular, as the oracle does on
+    the same T in complex form."""
+    import scipy.linalg.lapack as lap
+    r = np.random.default_rng(49)
+    n, k = 24, 5
+    T = np.triu(r.standard_normal((n, n))) + np.diag(np.arange(n) * 0.5)
+    S = gs.Schur(T, np.eye(n), np.diag(T).astype(complex))
+    s, sep = gs.eigcond(S, k)
+    sel = np.zeros(n, dtype=np.int32)
+    sel[:k] = 1
+    out = lap.dtrsen(sel, T, np.eye(n), job="B", lwork=max(1, 2 * k * (n - k)), liwork=max(1, k * (n - k)))
+    s_l, sep_l, info = out[5], out[6], out[-1]
+    assert info == 0
+    assert abs(s - s_l) <= 1e-8 * s_l and abs(sep - sep_l) <= 1e-8 * sep_l
+    A = r.standard_normal((n, n))
+    Sq = gs.gschur(A)
+    Tq = np.asarray(Sq.T)
+    s2, sep2 = gs.eigcond(Sq, k)
+    assert s2 == oracle.eigvalscond(Tq.astype(complex), k)
+    assert abs(sep2 - oracle.subspacesep(Tq.astype(complex), k)) <= 1e-12 * sep2
