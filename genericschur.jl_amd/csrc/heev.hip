@@ -376,6 +376,7 @@ __global__ void __launch_bounds__(64) steqr_lanes_kernel(HeevArgs a) {
 // one row per thread), the recurrence finished in place for a matrix whose
 // stream overflowed (producer lane and appliers as for one matrix), the
 // stable sort (sorteig!) and the write-out of W, V and info.
+static_assert(heev_max_n(false) <= 128 && heev_max_n(true) <= 128, "phase 3 replays one row per thread");
 template <class T, int NT> __global__ void __launch_bounds__(NT) heev_apply_kernel(HeevArgs a) {
   extern __shared__ double smem[];
   const int b = blockIdx.x;
@@ -402,8 +403,9 @@ template <class T, int NT> __global__ void __launch_bounds__(NT) heev_apply_kern
   if (wantz) {
     const double2* rec = static_cast<const double2*>(a.log) + (int64_t)b * a.logcap;
     const int npos = a.logpos[b];
-    for (int r = tid; r < n; r += NT)
-      st_replay_row<T>(rec, npos, [&](int k) -> T& { return M[r + (int64_t)(k - 1) * ld]; });
+    // one row per thread (n <= NT)
+    const int r = tid;
+    if (r < n) st_replay_row<T>(rec, npos, [&](int k) -> T& { return M[r + (int64_t)(k - 1) * ld]; });
     __syncthreads();
   }
   if (!S.done) {

@@ -335,121 +335,78 @@ template <class Log> GS_DEV void st_step(StState& S, StVec d, StVec e, Log* lg) 
     }
     return;
   }
-  int l = S.l;
-  const int lend = S.lend;
+  // _st_QLIter! (dir = +1: l < lend, the block swept towards l from its far
+  // end) and _st_QRIter! (dir = -1) loop bodies as one: the two are mirror
+  // images, with the same operations on mirrored indices, so lanes of a
+  // wavefront in either iteration share the path.
+  const int l = S.l, lend = S.lend;
+  const int dir = S.ql ? 1 : -1;
   const bool wantz = lg != nullptr;
-  if (S.ql) {
-    // _st_QLIter! loop body
-    int msplit = lend;
-    for (int m = l; m <= lend - 1; ++m) {
-      if (E(m) * E(m) <= (eps2 * fabs(D(m))) * fabs(D(m + 1)) + safmin) {
-        msplit = m;
-        break;
-      }
+  // search for a small off-diagonal entry: pairs (m, m + dir), e index
+  // m (QL) or m - 1 (QR)
+  int msplit = lend;
+  for (int m = l; m != lend; m += dir) {
+    const int em = dir > 0 ? m : m - 1;
+    if (E(em) * E(em) <= (eps2 * fabs(D(m))) * fabs(D(m + dir)) + safmin) {
+      msplit = m;
+      break;
     }
-    if (msplit < lend) E(msplit) = 0.0;
-    double p = D(l);
-    if (msplit != l) {
-      if (msplit == l + 1) {
-        double rt1, rt2, cs, sn;
-        sym2x2(D(l), E(l), D(l + 1), wantz, rt1, rt2, cs, sn);
-        if (wantz) {
-          const int base = lg->begin();
-          lg->put(base, 0, cs, sn);
-          lg->end(base, l, 1, 1);
-        }
-        D(l) = rt1;
-        D(l + 1) = rt2;
-        E(l) = 0.0;
-        S.l = l + 2;
-        return;
+  }
+  if (msplit != lend) E(dir > 0 ? msplit : msplit - 1) = 0.0;
+  double p = D(l);
+  if (msplit != l) {
+    if (msplit == l + dir) {
+      // the remaining 2 x 2 block (lo, lo + 1)
+      const int lo = dir > 0 ? l : l - 1;
+      double rt1, rt2, cs, sn;
+      sym2x2(D(lo), E(lo), D(lo + 1), wantz, rt1, rt2, cs, sn);
+      if (wantz) {
+        const int base = lg->begin();
+        lg->put(base, 0, cs, sn);
+        lg->end(base, lo, 1, 1);
       }
-      if (S.sweeps >= S.maxsweeps) {
-        S.stop = true;
-        return;
-      }
-      S.sweeps += 1;
-      double g = (D(l + 1) - p) / (2.0 * E(l));
-      double r = jl_hypot(g, 1.0);
-      g = D(msplit) - p + (E(l) / (g + copysign(r, g)));
-      double s = 1.0, c = 1.0;
-      p = 0.0;
-      const int r0 = wantz ? lg->begin() : 0;
-      for (int i = msplit - 1; i >= l; --i) {
-        const double f = s * E(i);
-        const double b = c * E(i);
-        givens_re(g, f, c, s, r);
-        if (i < msplit - 1) E(i + 1) = r;
-        g = D(i + 1) - p;
-        r = (D(i) - g) * s + 2.0 * c * b;
-        p = s * r;
-        D(i + 1) = g + p;
-        g = c * r - b;
-        if (wantz) lg->put(r0, msplit - 1 - i, c, -s);
-      }
-      if (wantz) lg->end(r0, msplit - 1, msplit - l, -1);
-      D(l) = D(l) - p;
-      E(l) = g;
-    } else {
-      D(l) = p;
-      S.l = l + 1;
+      D(lo) = rt1;
+      D(lo + 1) = rt2;
+      E(lo) = 0.0;
+      S.l = l + 2 * dir;
+      return;
     }
+    if (S.sweeps >= S.maxsweeps) {
+      S.stop = true;
+      return;
+    }
+    S.sweeps += 1;
+    const int el = dir > 0 ? l : l - 1;
+    double g = (D(l + dir) - p) / (2.0 * E(el));
+    double r = jl_hypot(g, 1.0);
+    g = D(msplit) - p + (E(el) / (g + copysign(r, g)));
+    double s = 1.0, c = 1.0;
+    p = 0.0;
+    // rotations j = j0, j0 - dir, ..., j1 on pairs (j, j + 1): QL updates
+    // d[j + 1] against d[j], QR d[j] against d[j + 1]
+    const int j0 = dir > 0 ? msplit - 1 : msplit;
+    const int cnt = dir > 0 ? msplit - l : l - msplit;
+    const int r0 = wantz ? lg->begin() : 0;
+    for (int k = 0; k < cnt; ++k) {
+      const int jj = j0 - dir * k;
+      const int up = dir > 0 ? jj + 1 : jj, far = dir > 0 ? jj : jj + 1;
+      const double f = s * E(jj);
+      const double b = c * E(jj);
+      givens_re(g, f, c, s, r);
+      if (k > 0) E(jj + dir) = r;
+      g = D(up) - p;
+      r = (D(far) - g) * s + 2.0 * c * b;
+      p = s * r;
+      D(up) = g + p;
+      g = c * r - b;
+      if (wantz) lg->put(r0, k, c, dir > 0 ? -s : s);
+    }
+    if (wantz) lg->end(r0, j0, cnt, -dir);
+    D(l) = D(l) - p;
+    E(el) = g;
   } else {
-    // _st_QRIter! loop body
-    int msplit = lend;
-    for (int m = l; m >= lend + 1; --m) {
-      if (E(m - 1) * E(m - 1) <= (eps2 * fabs(D(m))) * fabs(D(m - 1)) + safmin) {
-        msplit = m;
-        break;
-      }
-    }
-    if (msplit > lend) E(msplit - 1) = 0.0;
-    double p = D(l);
-    if (msplit != l) {
-      if (msplit == l - 1) {
-        double rt1, rt2, cs, sn;
-        sym2x2(D(l - 1), E(l - 1), D(l), wantz, rt1, rt2, cs, sn);
-        if (wantz) {
-          const int base = lg->begin();
-          lg->put(base, 0, cs, sn);
-          lg->end(base, l - 1, 1, 1);
-        }
-        D(l - 1) = rt1;
-        D(l) = rt2;
-        E(l - 1) = 0.0;
-        S.l = l - 2;
-        return;
-      }
-      if (S.sweeps >= S.maxsweeps) {
-        S.stop = true;
-        return;
-      }
-      S.sweeps += 1;
-      double g = (D(l - 1) - p) / (2.0 * E(l - 1));
-      double r = jl_hypot(g, 1.0);
-      g = D(msplit) - p + (E(l - 1) / (g + copysign(r, g)));
-      double s = 1.0, c = 1.0;
-      p = 0.0;
-      const int r0 = wantz ? lg->begin() : 0;
-      for (int i = msplit; i <= l - 1; ++i) {
-        const double f = s * E(i);
-        const double b = c * E(i);
-        givens_re(g, f, c, s, r);
-        if (i != msplit) E(i - 1) = r;
-        g = D(i) - p;
-        r = (D(i + 1) - g) * s + 2.0 * c * b;
-        p = s * r;
-        D(i) = g + p;
-        g = c * r - b;
-        if (wantz) lg->put(r0, i - msplit, c, s);
-      }
-      if (wantz) lg->end(r0, msplit, l - msplit, 1);
-      D(l) = D(l) - p;
-      E(l - 1) = g;
-    } else {
-      D(l) = p;
-      S.l = l - 1;
-    }
+    D(l) = p;
+    S.l = l + dir;
   }
 #undef D
 #undef E
@@ -507,7 +464,11 @@ template <class TZ, class ZA> GS_DEV void st_apply_row(const StLog& lg, ZA z) {
   }
 }
 
-// Replays a record stream (StLogG) of npos records onto one row of Z.
+// Replays a record stream (StLogG) of npos records onto one row of Z.  The
+// rotations of a segment are taken four at a time with the four columns
+// they read loaded first: within a segment each column is read by one
+// rotation before the next rotation writes it, so the element operations
+// and their order are those of one rotation after another.
 template <class TZ, class ZA> GS_DEV void st_replay_row(const double2* __restrict__ rec, int npos, ZA z) {
   int q = 0;
   while (q < npos) {
@@ -515,9 +476,23 @@ template <class TZ, class ZA> GS_DEV void st_replay_row(const double2* __restric
     const int i0 = (int)h.x, dc = (int)h.y;
     const int cnt = dc < 0 ? -dc : dc;
     const double2* r = rec + q + 1;
+    int k = 0;
     if (dc < 0) {
       TZ cur = z(i0 + 1);
-      for (int k = 0; k < cnt; ++k) {
+      for (; k + 4 <= cnt; k += 4) {
+        const int i = i0 - k;
+        const TZ a0 = z(i), a1 = z(i - 1), a2 = z(i - 2), a3 = z(i - 3);
+        const double2 c0 = r[k], c1 = r[k + 1], c2 = r[k + 2], c3 = r[k + 3];
+        z(i + 1) = st_rot2(a0, cur, c0.x, c0.y);
+        cur = st_rot1(a0, cur, c0.x, c0.y);
+        z(i) = st_rot2(a1, cur, c1.x, c1.y);
+        cur = st_rot1(a1, cur, c1.x, c1.y);
+        z(i - 1) = st_rot2(a2, cur, c2.x, c2.y);
+        cur = st_rot1(a2, cur, c2.x, c2.y);
+        z(i - 2) = st_rot2(a3, cur, c3.x, c3.y);
+        cur = st_rot1(a3, cur, c3.x, c3.y);
+      }
+      for (; k < cnt; ++k) {
         const int i = i0 - k;
         const double2 cs = r[k];
         const TZ a1 = z(i);
@@ -527,7 +502,20 @@ template <class TZ, class ZA> GS_DEV void st_replay_row(const double2* __restric
       z(i0 - cnt + 1) = cur;
     } else {
       TZ cur = z(i0);
-      for (int k = 0; k < cnt; ++k) {
+      for (; k + 4 <= cnt; k += 4) {
+        const int i = i0 + k;
+        const TZ a0 = z(i + 1), a1 = z(i + 2), a2 = z(i + 3), a3 = z(i + 4);
+        const double2 c0 = r[k], c1 = r[k + 1], c2 = r[k + 2], c3 = r[k + 3];
+        z(i) = st_rot1(cur, a0, c0.x, c0.y);
+        cur = st_rot2(cur, a0, c0.x, c0.y);
+        z(i + 1) = st_rot1(cur, a1, c1.x, c1.y);
+        cur = st_rot2(cur, a1, c1.x, c1.y);
+        z(i + 2) = st_rot1(cur, a2, c2.x, c2.y);
+        cur = st_rot2(cur, a2, c2.x, c2.y);
+        z(i + 3) = st_rot1(cur, a3, c3.x, c3.y);
+        cur = st_rot2(cur, a3, c3.x, c3.y);
+      }
+      for (; k < cnt; ++k) {
         const int i = i0 + k;
         const double2 cs = r[k];
         const TZ a2 = z(i + 1);
