@@ -390,6 +390,9 @@ template <int NT> __global__ void __launch_bounds__(NT) lctrl_kernel(LQArgs a) {
   const double eps = GS_EPS;
   const double smallnum = GS_FMIN * ((double)n / eps);
   LState s = *a.st;
+  // launched again after the iteration ended (the host runs a few sweeps
+  // ahead of the state it has seen): nothing to do
+  if (s.status != 0) return;
   while (true) {
     if (s.phase == 0) {
       if (s.iend < 1) {
@@ -591,6 +594,34 @@ struct LChunk {
   int istart, iend, mx;
 };
 
+// Chunk c (0-based) of the sweep the state record describes: bulge steps
+// mx + c LKC .. (through iend - 1), the last one (c = number of chunks) the
+// flush of the previous chunk's updates.  Returns false when the iteration
+// has ended or c lies beyond the flush.
+GS_DEV bool chunk_of(const LState* st, int c, LChunk& ck) {
+  if (st->status != 0) return false;
+  ck.istart = st->istart;
+  ck.iend = st->iend;
+  ck.mx = st->mx;
+  const int nch = (ck.iend - ck.mx + LKC - 1) / LKC;  // steps mx .. iend - 1
+  if (c > nch) return false;
+  if (c < nch) {
+    ck.k0 = ck.mx + c * LKC;
+    ck.kend = min(ck.k0 + LKC - 1, ck.iend - 1);
+  } else {
+    ck.k0 = 1;
+    ck.kend = 0;
+  }
+  if (c > 0) {
+    ck.pk0 = ck.mx + (c - 1) * LKC;
+    ck.pkend = min(ck.pk0 + LKC - 1, ck.iend - 1);
+  } else {
+    ck.pk0 = 1;
+    ck.pkend = 0;
+  }
+  return true;
+}
+
 // One chunk of a sweep.  Workgroup 0 (one wavefront): apply the previous
 // chunk's left update to the columns this window adds, load the window
 // H(wb:we, wb:we) into LDS, chase the bulge through steps k0 .. kend
@@ -598,7 +629,9 @@ struct LChunk {
 // reflectors.  Workgroups 1..: the previous chunk's updates outside its
 // window: columns right of the current window, rows above the previous
 // window, and Z.  The regions are disjoint.
-__global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, LChunk ck) {
+__global__ void __launch_bounds__(64) lchunk_kernel(LQArgs a, int c) {
+  LChunk ck;
+  if (!chunk_of(a.st, c, ck)) return;
   __shared__ double win[LWL * LW];
   // the chunk's reflectors, logged in LDS during the chase and written to
   // the global log after it: a global store per step would make every
@@ -820,31 +853,53 @@ hipError_t large_schur_d(const LargeArgs& a, LargeWork& w, hipStream_t s, LargeT
   LCHK(hipMemcpyAsync(q.st, &st0, sizeof st0, hipMemcpyHostToDevice, s));
   const int nwork = (n + 63) / 64;
   const int grid = 1 + std::min(3 * nwork, 1024);
-  LState* hst = static_cast<LState*>(w.host_state);
-  int status = 0;
+  // Sweep control stays on the device: each sweep is lctrl_kernel and then
+  // chunk launches that read their steps from the state record it wrote.
+  // The host does not wait for a sweep's state: it issues up to kAhead
+  // sweeps ahead of the newest state it has seen (copied back with an event
+  // per sweep), launching for each the chunks an upper bound allows: iend
+  // never grows, so ceil((iend - 1) / LKC) chunks of a state seen earlier
+  // cover any later sweep (surplus launches return at once).  Launches after
+  // the iteration has ended return at once as well.
+  constexpr int kAhead = 4;
+  LState* hst = static_cast<LState*>(w.host_state);  // kAhead slots
+  hipEvent_t ev[kAhead];
+  for (int i = 0; i < kAhead; ++i) LCHK(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+  auto fail = [&](hipError_t err) {
+    for (int i = 0; i < kAhead; ++i) (void)hipEventDestroy(ev[i]);
+    return err;
+  };
+  int status = 0, seen_iend = n;
+  int64_t issued = 0, done = 0;  // sweeps issued / whose state was read
+  LState last;
+  std::memset(&last, 0, sizeof last);
   while (true) {
-    LLAUNCH(lctrl_kernel<1024>, dim3(1), dim3(1024), 0, s, q);
-    LCHK(hipMemcpyAsync(hst, q.st, sizeof(LState), hipMemcpyDeviceToHost, s));
-    LCHK(hipStreamSynchronize(s));
-    status = hst->status;
-    if (status != 0) break;
-    LChunk ck;
-    ck.istart = hst->istart;
-    ck.iend = hst->iend;
-    ck.mx = hst->mx;
-    ck.pk0 = 1;
-    ck.pkend = 0;
-    for (int k0 = ck.mx; k0 <= ck.iend - 1; k0 += LKC) {
-      ck.k0 = k0;
-      ck.kend = std::min(k0 + LKC - 1, ck.iend - 1);
-      LLAUNCH(lchunk_kernel, dim3(grid), dim3(64), 0, s, q, ck);
-      ck.pk0 = ck.k0;
-      ck.pkend = ck.kend;
+    if (issued - done == kAhead) {
+      // read the oldest outstanding state
+      const int slot = (int)(done % kAhead);
+      if ((e = hipEventSynchronize(ev[slot])) != hipSuccess) return fail(e);
+      last = hst[slot];
+      done += 1;
+      status = last.status;
+      seen_iend = std::min(seen_iend, last.iend);
+      if (status != 0) break;
     }
-    ck.k0 = 1;
-    ck.kend = 0;
-    LLAUNCH(lchunk_kernel, dim3(grid), dim3(64), 0, s, q, ck);
+    const int slot = (int)(issued % kAhead);
+    hipLaunchKernelGGL(lctrl_kernel<1024>, dim3(1), dim3(1024), 0, s, q);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    const int nch = (seen_iend - 1 + LKC - 1) / LKC;
+    for (int c = 0; c <= nch; ++c) {
+      hipLaunchKernelGGL(lchunk_kernel, dim3(grid), dim3(64), 0, s, q, c);
+      if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    }
+    if ((e = hipMemcpyAsync(hst + slot, q.st, sizeof(LState), hipMemcpyDeviceToHost, s)) != hipSuccess) return fail(e);
+    if ((e = hipEventRecord(ev[slot], s)) != hipSuccess) return fail(e);
+    issued += 1;
   }
+  // the sweeps issued after the last one read return at once; wait for them
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail(e);
+  for (int i = 0; i < kAhead; ++i) (void)hipEventDestroy(ev[i]);
+  hst[0] = last;
   if (tm) LCHK(tm->mark(s, 4));
   // 5. results
   const int32_t info = status == 2 ? (int32_t)std::min<int64_t>(hst->iter, 0x7fffffff) : 0;

@@ -928,7 +928,7 @@ static hipError_t run_large(gs_handle* h, int64_t n, double* dA, int64_t lda, do
   hipError_t e = h->lwork.ensure(8 * nd);
   if (e != hipSuccess) return e;
   if (!h->lhost) {
-    e = hipHostMalloc(&h->lhost, 256, hipHostMallocDefault);
+    e = hipHostMalloc(&h->lhost, 1024, hipHostMallocDefault);
     if (e != hipSuccess) {
       h->lhost = nullptr;
       return e;
