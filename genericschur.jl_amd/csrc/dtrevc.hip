@@ -15,6 +15,7 @@
 
 #include "dev_common.h"
 #include "internal.h"
+#include "trevc_big.h"
 #include "wavevec.h"
 
 namespace gs {
@@ -153,8 +154,8 @@ GS_DEV double xsolve_dev(int na, const double* Ae, zd bw, zd* B, double smin, do
 // Solution for column ki (0-based) with ip = 0 (real wr) or -1 (pair,
 // lambda = wr + i wi, ki the second row of the block).  Entries 0..ki hold
 // the vector on exit.
-template <int R, class TA> GS_DEV void dsolve_right(const TA& Tc, const double* sub, const double* tnorms, int ki, int ip,
-                                          double smallnum, double bignum, WaveVecZ<R>& v) {
+template <int R, class TA, class VEC> GS_DEV void dsolve_right(const TA& Tc, const double* sub, const double* tnorms, int ki,
+                                                     int ip, double smallnum, double bignum, VEC& v) {
   const int lane = threadIdx.x & 63;
   const double ulp = GS_EPS;
   const double wr = Tc(ki, ki);
@@ -248,8 +249,8 @@ template <int R, class TA> GS_DEV void dsolve_right(const TA& Tc, const double* 
 // block, lambda = wr - i wi in the solves).  Entries ki..n-1 hold it.
 // Forward substitution in axpy form; the reference's overflow guards
 // (vmax, vcrit) and the j1 = j + 1 quirk of vectors.jl:627 are kept.
-template <int R, class TA> GS_DEV void dsolve_left(const TA& Tc, const double* sub, const double* tnorms, int n, int ki,
-                                         int ip, double smallnum, double bignum, WaveVecZ<R>& v) {
+template <int R, class TA, class VEC> GS_DEV void dsolve_left(const TA& Tc, const double* sub, const double* tnorms, int n,
+                                                    int ki, int ip, double smallnum, double bignum, VEC& v) {
   const int lane = threadIdx.x & 63;
   const double ulp = GS_EPS;
   const double wr = Tc(ki, ki);
@@ -353,10 +354,10 @@ template <int R, class TA> GS_DEV void dsolve_left(const TA& Tc, const double* s
   }
 }
 
-template <int R> GS_DEV void enormalize_w(WaveVecZ<R>& v, int n);
+template <int R, class VEC> GS_DEV void enormalize_w(VEC& v, int n);
 
 // max-|.| normalisation of a finished column (maximum(abs, ...) or abs1)
-template <int R> GS_DEV void scale_by_max(WaveVecZ<R>& y, int lo, int hi, bool use_abs1) {
+template <int R, class VEC> GS_DEV void scale_by_max(VEC& y, int lo, int hi, bool use_abs1) {
   const int lane = threadIdx.x & 63;
   double m = 0.0;
 #pragma unroll
@@ -370,7 +371,7 @@ template <int R> GS_DEV void scale_by_max(WaveVecZ<R>& y, int lo, int hi, bool u
   for (int q = 0; q < R; ++q) y.v[q] = y.v[q] * s;
 }
 
-template <int R> GS_DEV void enormalize_w(WaveVecZ<R>& v, int n) {
+template <int R, class VEC> GS_DEV void enormalize_w(VEC& v, int n) {
   const int lane = threadIdx.x & 63;
   double am = 0.0;
 #pragma unroll
@@ -581,5 +582,224 @@ hipError_t launch_dtrevc(const TrevcArgsD& a, int nmax, hipStream_t s) {
   return hipGetLastError();
 }
 int dtrevc_max_n() { return 1024; }
+
+// ------------------------------------------------------- larger orders
+// One matrix of order 1024 < n <= trevc_big_max_n() over the whole GPU:
+// the block kinds and column norms in global memory, one solve per
+// workgroup (a single wavefront whose solution vector lives in LDS,
+// WaveVecL), the solutions in a planar workspace X = [Re | Im] (n x 2n),
+// the back-transform [Re | Im] of Z X as one FP64 MFMA GEMM, then one
+// wavefront per column scales, normalises and writes V.  The per-column
+// steps are those of dtrevc_kernel.
+struct DBig {
+  int n, left, normalize;
+  const double* T;
+  int64_t ldt;
+  const double* Tt;  // T's transpose (left solves), ld n
+  const double* Z;   // may be null
+  int64_t ldz;
+  zd* V;
+  int64_t ldv;
+  double* sub;
+  double* tnorms;
+  int* kind;
+  double* X;  // n x 2n, ld n
+  double* Y;  // n x 2n, ld n
+};
+
+// subdiagonal and block kinds (as in dtrevc_kernel), one workgroup
+__global__ void __launch_bounds__(1024) dtrevc_big_kinds(DBig a) {
+  extern __shared__ double smem[];
+  int* kl = reinterpret_cast<int*>(smem);
+  const int n = a.n, tid = threadIdx.x;
+  const bool left = a.left != 0;
+  for (int j = tid; j < n; j += 1024) {
+    const double sj = (j + 1 < n) ? a.T[(j + 1) + (int64_t)j * a.ldt] : 0.0;
+    const double sp = (j > 0) ? a.T[j + (int64_t)(j - 1) * a.ldt] : 0.0;
+    a.sub[j] = sj;
+    kl[j] = left ? (sj != 0.0 ? 1 : 0) : (sp != 0.0 ? -1 : 0);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    if (!left) {
+      for (int ki = n - 1; ki >= 0; --ki)
+        if (kl[ki] == -1) {
+          if (ki - 1 >= 0) kl[ki - 1] = 2;
+          --ki;
+        }
+    } else {
+      for (int ki = 0; ki < n; ++ki)
+        if (kl[ki] == 1) {
+          if (ki + 1 < n) kl[ki + 1] = 2;
+          ++ki;
+        }
+    }
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += 1024) a.kind[j] = kl[j];
+}
+
+// workgroup t solves column n-1-t (right) or t (left): the longest solves
+// are dispatched first
+template <int R, class TA> __global__ void __launch_bounds__(64) dtrevc_big_solve(DBig a, TA Tc) {
+  extern __shared__ double smem[];
+  const int n = a.n, lane = threadIdx.x & 63, t = blockIdx.x;
+  const bool left = a.left != 0;
+  const int ki = left ? t : n - 1 - t;
+  const int kd = a.kind[ki];
+  double* xr = a.X + (int64_t)ki * n;
+  double* xi = xr + (int64_t)n * n;
+  if (kd == 2) {
+    for (int i = lane; i < n; i += 64) {
+      xr[i] = 0.0;
+      xi[i] = 0.0;
+    }
+    return;
+  }
+  const double ulp = GS_EPS;
+  const double smallnum = GS_FMIN * ((double)n / ulp);  // vectors.jl:146,475
+  const double bignum = (1.0 - ulp) / smallnum;
+  WaveVecL<R> v{{reinterpret_cast<zd*>(smem)}};
+  if (!left) dsolve_right<R>(Tc, a.sub, a.tnorms, ki, kd, smallnum, bignum, v);
+  else dsolve_left<R>(Tc, a.sub, a.tnorms, n, ki, kd, smallnum, bignum, v);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = lane + 64 * q;
+    if (i < n) {
+      const zd z = v.v[q];
+      xr[i] = z.x;
+      xi[i] = z.y;
+    }
+  }
+}
+
+template <int R> __global__ void __launch_bounds__(64) dtrevc_big_finish(DBig a) {
+  extern __shared__ double smem[];
+  const int n = a.n, lane = threadIdx.x & 63, ki = blockIdx.x;
+  const bool left = a.left != 0;
+  const int kd = a.kind[ki];
+  if (kd == 2) return;
+  WaveVecL<R> y{{reinterpret_cast<zd*>(smem)}};
+  const int lo = left ? ki : 0, hi = left ? n : ki + 1;
+  if (a.Z) {
+    if ((!left && ki == 0 && kd == 0) || (left && ki == n - 1 && kd == 0)) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int r = lane + 64 * q;
+        y.v[q] = (r < n) ? zr(a.Z[r + (int64_t)ki * a.ldz]) : zr(0.0);
+      }
+    } else {
+      const double* yr = a.Y + (int64_t)ki * n;
+      const double* yi = yr + (int64_t)n * n;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int r = lane + 64 * q;
+        y.v[q] = (r < n) ? zmk(yr[r], yi[r]) : zr(0.0);
+      }
+      scale_by_max<R>(y, 0, n, false);
+      if (!left && kd == -1) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) y.v[q] = zconj(y.v[q]);
+      }
+    }
+  } else {
+    const double* xr = a.X + (int64_t)ki * n;
+    const double* xi = xr + (int64_t)n * n;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int i = lane + 64 * q;
+      y.v[q] = (i >= lo && i < hi) ? zmk(xr[i], xi[i]) : zr(0.0);
+    }
+    if (kd == 0) {
+      scale_by_max<R>(y, lo, hi, false);
+    } else {
+      if (!left) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) y.v[q] = zconj(y.v[q]);
+      }
+      scale_by_max<R>(y, lo, hi, true);
+    }
+  }
+  if (a.normalize) enormalize_w<R>(y, n);
+  const int pk = left ? ki + 1 : ki - 1;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int r = lane + 64 * q;
+    if (r < n) {
+      const zd z = y.v[q];
+      a.V[r + (int64_t)ki * a.ldv] = z;
+      if (kd != 0) a.V[r + (int64_t)pk * a.ldv] = zconj(z);
+    }
+  }
+}
+
+size_t dtrevc_big_ws_bytes(int n) {
+  const size_t nn = (size_t)n * n;
+  return 2 * trevc_seg(8 * (size_t)n) + trevc_seg(4 * (size_t)n) + trevc_seg(8 * nn) + 2 * trevc_seg(16 * nn);
+}
+
+template <int R> static hipError_t launch_dbig_r(const DBig& b, hipStream_t s) {
+  const int n = b.n;
+  const size_t lds = 64 * R * sizeof(zd);
+  if (b.left) {
+    auto k = dtrevc_big_solve<R, TransG<double>>;
+    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, dim3(n), dim3(64), lds, s, b, TransG<double>{b.Tt, n});
+  } else {
+    auto k = dtrevc_big_solve<R, GlobD>;
+    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, dim3(n), dim3(64), lds, s, b, GlobD{b.T, b.ldt});
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (b.Z) {
+    e = launch_dgemm(false, false, n, 2 * n, n, 1.0, b.Z, b.ldz, b.X, n, 0.0, b.Y, n, s);
+    if (e != hipSuccess) return e;
+  }
+  (void)hipFuncSetAttribute((const void*)dtrevc_big_finish<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(dtrevc_big_finish<R>, dim3(n), dim3(64), lds, s, b);
+  return hipGetLastError();
+}
+
+// a.T, a.Z, a.V point at the matrix (no offsets), ld from a.ldt/ldz/ldv;
+// ws holds dtrevc_big_ws_bytes(n)
+hipError_t launch_dtrevc_big(const TrevcArgsD& a, int n, void* ws, hipStream_t s) {
+  if (n <= dtrevc_max_n() || n > trevc_big_max_n() || !ws) return hipErrorInvalidValue;
+  const size_t nn = (size_t)n * n;
+  char* p = static_cast<char*>(ws);
+  DBig b;
+  b.n = n;
+  b.left = a.left;
+  b.normalize = a.normalize;
+  b.T = a.T;
+  b.ldt = a.ldt > 0 ? a.ldt : n;
+  b.Z = a.Z;
+  b.ldz = a.ldz > 0 ? a.ldz : n;
+  b.V = reinterpret_cast<zd*>(a.V);
+  b.ldv = a.ldv > 0 ? a.ldv : n;
+  b.sub = reinterpret_cast<double*>(p);
+  p += trevc_seg(8 * (size_t)n);
+  b.tnorms = reinterpret_cast<double*>(p);
+  p += trevc_seg(8 * (size_t)n);
+  b.kind = reinterpret_cast<int*>(p);
+  p += trevc_seg(4 * (size_t)n);
+  double* tt = reinterpret_cast<double*>(p);
+  b.Tt = tt;
+  p += trevc_seg(8 * nn);
+  b.X = reinterpret_cast<double*>(p);
+  p += trevc_seg(16 * nn);
+  b.Y = reinterpret_cast<double*>(p);
+  hipLaunchKernelGGL(dtrevc_big_kinds, dim3(1), dim3(1024), 4 * (size_t)n, s, b);
+  hipLaunchKernelGGL(trevc_colnorm_kernel<double>, dim3((n + 3) / 4), dim3(256), 0, s, n, b.T, b.ldt, false,
+                     b.tnorms);
+  if (b.left)
+    hipLaunchKernelGGL(trevc_transpose_kernel<double>, dim3((n + 31) / 32, (n + 31) / 32), dim3(256), 0, s, n, b.T,
+                       b.ldt, tt);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return n <= 2048 ? launch_dbig_r<32>(b, s) : launch_dbig_r<64>(b, s);
+}
+
+int trevc_big_max_n() { return 4096; }
 
 }  // namespace gs

@@ -72,11 +72,11 @@ static int balance_impl(gs_handle* h, bool cplx, int64_t n, double* A, int64_t l
     dsp = dout + 4;
     dD = reinterpret_cast<double*>(meta + 16 + ((4 * (size_t)n + 7) & ~size_t(7)));
     dlda = n;
-    ECHK(hipMemcpy2DAsync(dA, es * n, A, es * lda, es * n, n, hipMemcpyHostToDevice, s));
+    ECHK(copy_cols(dA, es * n, A, es * lda, es * n, n, hipMemcpyHostToDevice, s));
   }
   ECHK(launch_balance(cplx, dA, dlda, (int)n, scale, permute, dout, dsp, dD, s));
   if (!dev) {
-    ECHK(hipMemcpy2DAsync(A, es * lda, dA, es * n, es * n, n, hipMemcpyDeviceToHost, s));
+    ECHK(copy_cols(A, es * lda, dA, es * n, es * n, n, hipMemcpyDeviceToHost, s));
     ECHK(hipMemcpyAsync(out4, dout, 16, hipMemcpyDeviceToHost, s));
     ECHK(hipMemcpyAsync(sp, dsp, 4 * n, hipMemcpyDeviceToHost, s));
     ECHK(hipMemcpyAsync(D, dD, 8 * n, hipMemcpyDeviceToHost, s));
@@ -126,14 +126,14 @@ int gs_balance_apply(gs_handle* h, int64_t n, int64_t m, const int32_t* bal3, co
     ECHK(hipMemcpyAsync(mb, bal3, 12, hipMemcpyHostToDevice, s));
     ECHK(hipMemcpyAsync(mb + 4, sp, 4 * n, hipMemcpyHostToDevice, s));
     ECHK(hipMemcpyAsync(md, D, 8 * n, hipMemcpyHostToDevice, s));
-    ECHK(hipMemcpy2DAsync(dV, 16 * n, V, 16 * ldv, 16 * n, m, hipMemcpyHostToDevice, s));
+    ECHK(copy_cols(dV, 16 * n, V, 16 * ldv, 16 * n, m, hipMemcpyHostToDevice, s));
     dbal = mb;
     dsp = mb + 4;
     dD = md;
     dldv = n;
   }
   ECHK(launch_balback((int)n, (int)m, dbal, dsp, dD, dV, dldv, inverse, s));
-  if (!dev) ECHK(hipMemcpy2DAsync(V, 16 * ldv, dV, 16 * n, 16 * n, m, hipMemcpyDeviceToHost, s));
+  if (!dev) ECHK(copy_cols(V, 16 * ldv, dV, 16 * n, 16 * n, m, hipMemcpyDeviceToHost, s));
   if (!dev || !(flags & GS_ASYNC)) ECHK(hipStreamSynchronize(s));
   return GS_OK;
 }
@@ -163,8 +163,8 @@ int gs_triangularize_d(gs_handle* h, int64_t n, const double* T, int64_t ldt, co
     dTc = base + 2 * nn;
     dZc = base + 4 * nn;
     dW = base + 6 * nn;
-    ECHK(hipMemcpy2DAsync(sT, 8 * n, T, 8 * ldt, 8 * n, n, hipMemcpyHostToDevice, s));
-    if (Z) ECHK(hipMemcpy2DAsync(sZ, 8 * n, Z, 8 * ldz, 8 * n, n, hipMemcpyHostToDevice, s));
+    ECHK(copy_cols(sT, 8 * n, T, 8 * ldt, 8 * n, n, hipMemcpyHostToDevice, s));
+    if (Z) ECHK(copy_cols(sZ, 8 * n, Z, 8 * ldz, 8 * n, n, hipMemcpyHostToDevice, s));
     dT = sT;
     dZ = Z ? sZ : nullptr;
     a = b = c = d = n;
@@ -172,8 +172,8 @@ int gs_triangularize_d(gs_handle* h, int64_t n, const double* T, int64_t ldt, co
   if (!Z) dZc = dTc;  // not written
   ECHK(launch_triangularize((int)n, dT, a, dZ, b, dTc, c, dZ ? dZc : nullptr, d, dW, s));
   if (!dev) {
-    ECHK(hipMemcpy2DAsync(Tc, 16 * ldtc, dTc, 16 * n, 16 * n, n, hipMemcpyDeviceToHost, s));
-    if (Z) ECHK(hipMemcpy2DAsync(Zc, 16 * ldzc, dZc, 16 * n, 16 * n, n, hipMemcpyDeviceToHost, s));
+    ECHK(copy_cols(Tc, 16 * ldtc, dTc, 16 * n, 16 * n, n, hipMemcpyDeviceToHost, s));
+    if (Z) ECHK(copy_cols(Zc, 16 * ldzc, dZc, 16 * n, 16 * n, n, hipMemcpyDeviceToHost, s));
     ECHK(hipMemcpyAsync(W, dW, 16 * n, hipMemcpyDeviceToHost, s));
   }
   if (!dev || !(flags & GS_ASYNC)) ECHK(hipStreamSynchronize(s));
@@ -196,11 +196,11 @@ int gs_enormalize(gs_handle* h, int64_t n, int64_t m, double* V, int64_t ldv, in
   if (!dev) {
     dV = static_cast<double*>(stage(h, 0, 16 * (size_t)n * m, e));
     ECHK(e);
-    ECHK(hipMemcpy2DAsync(dV, 16 * n, V, 16 * ldv, 16 * n, m, hipMemcpyHostToDevice, s));
+    ECHK(copy_cols(dV, 16 * n, V, 16 * ldv, 16 * n, m, hipMemcpyHostToDevice, s));
     dl = n;
   }
   ECHK(launch_enormalize((int)n, (int)m, dV, dl, s));
-  if (!dev) ECHK(hipMemcpy2DAsync(V, 16 * ldv, dV, 16 * n, 16 * n, m, hipMemcpyDeviceToHost, s));
+  if (!dev) ECHK(copy_cols(V, 16 * ldv, dV, 16 * n, 16 * n, m, hipMemcpyDeviceToHost, s));
   if (!dev || !(flags & GS_ASYNC)) ECHK(hipStreamSynchronize(s));
   return GS_OK;
 }
@@ -227,8 +227,8 @@ int gs_eig_rcond(gs_handle* h, int64_t n, int64_t m, const double* VL, int64_t l
     double* sL = base;
     double* sR = base + 2 * (size_t)n * m;
     dC = base + 4 * (size_t)n * m;
-    ECHK(hipMemcpy2DAsync(sL, 16 * n, VL, 16 * ldvl, 16 * n, m, hipMemcpyHostToDevice, s));
-    ECHK(hipMemcpy2DAsync(sR, 16 * n, VR, 16 * ldvr, 16 * n, m, hipMemcpyHostToDevice, s));
+    ECHK(copy_cols(sL, 16 * n, VL, 16 * ldvl, 16 * n, m, hipMemcpyHostToDevice, s));
+    ECHK(copy_cols(sR, 16 * n, VR, 16 * ldvr, 16 * n, m, hipMemcpyHostToDevice, s));
     dL = sL;
     dR = sR;
     a = b = n;
@@ -266,14 +266,14 @@ int gs_ordschur_z(gs_handle* h, int64_t n, double* T, int64_t ldt, double* Z, in
     dT = sb;
     dZ = Z ? sb + 2 * nn : nullptr;
     dW = sb + 4 * nn;
-    ECHK(hipMemcpy2DAsync(dT, 16 * n, T, 16 * ldt, 16 * n, n, hipMemcpyHostToDevice, s));
-    if (Z) ECHK(hipMemcpy2DAsync(dZ, 16 * n, Z, 16 * ldz, 16 * n, n, hipMemcpyHostToDevice, s));
+    ECHK(copy_cols(dT, 16 * n, T, 16 * ldt, 16 * n, n, hipMemcpyHostToDevice, s));
+    if (Z) ECHK(copy_cols(dZ, 16 * n, Z, 16 * ldz, 16 * n, n, hipMemcpyHostToDevice, s));
     a = b = n;
   }
   ECHK(launch_ordschur((int)n, dT, a, dZ, b, Z != nullptr, dsel, dW, s));
   if (!dev) {
-    ECHK(hipMemcpy2DAsync(T, 16 * ldt, dT, 16 * n, 16 * n, n, hipMemcpyDeviceToHost, s));
-    if (Z) ECHK(hipMemcpy2DAsync(Z, 16 * ldz, dZ, 16 * n, 16 * n, n, hipMemcpyDeviceToHost, s));
+    ECHK(copy_cols(T, 16 * ldt, dT, 16 * n, 16 * n, n, hipMemcpyDeviceToHost, s));
+    if (Z) ECHK(copy_cols(Z, 16 * ldz, dZ, 16 * n, 16 * n, n, hipMemcpyDeviceToHost, s));
     ECHK(hipMemcpyAsync(W, dW, 16 * n, hipMemcpyDeviceToHost, s));
   }
   // select is read from the staging buffer until the launch is done
@@ -312,14 +312,14 @@ int gs_ordschur_d(gs_handle* h, int64_t n, double* T, int64_t ldt, double* Z, in
     dT = sb;
     dZ = Z ? sb + nn : nullptr;
     dW = sb + 2 * nn;
-    ECHK(hipMemcpy2DAsync(dT, 8 * n, T, 8 * ldt, 8 * n, n, hipMemcpyHostToDevice, s));
-    if (Z) ECHK(hipMemcpy2DAsync(dZ, 8 * n, Z, 8 * ldz, 8 * n, n, hipMemcpyHostToDevice, s));
+    ECHK(copy_cols(dT, 8 * n, T, 8 * ldt, 8 * n, n, hipMemcpyHostToDevice, s));
+    if (Z) ECHK(copy_cols(dZ, 8 * n, Z, 8 * ldz, 8 * n, n, hipMemcpyHostToDevice, s));
     a = b = n;
   }
   ECHK(launch_ordschur_d((int)n, dT, a, dZ, b, Z != nullptr, dsel, dW, dinfo, s));
   if (!dev) {
-    ECHK(hipMemcpy2DAsync(T, 8 * ldt, dT, 8 * n, 8 * n, n, hipMemcpyDeviceToHost, s));
-    if (Z) ECHK(hipMemcpy2DAsync(Z, 8 * ldz, dZ, 8 * n, 8 * n, n, hipMemcpyDeviceToHost, s));
+    ECHK(copy_cols(T, 8 * ldt, dT, 8 * n, 8 * n, n, hipMemcpyDeviceToHost, s));
+    if (Z) ECHK(copy_cols(Z, 8 * ldz, dZ, 8 * n, 8 * n, n, hipMemcpyDeviceToHost, s));
     ECHK(hipMemcpyAsync(W, dW, 16 * (size_t)n, hipMemcpyDeviceToHost, s));
   }
   ECHK(hipMemcpyAsync(info, dinfo, 4, hipMemcpyDeviceToHost, s));
@@ -346,7 +346,7 @@ int gs_eigcond_z(gs_handle* h, int64_t n, const double* T, int64_t ldt, int64_t 
   int64_t a = ldt;
   if (!dev) {
     double* sT = dout + 8;
-    ECHK(hipMemcpy2DAsync(sT, 16 * n, T, 16 * ldt, 16 * n, n, hipMemcpyHostToDevice, s));
+    ECHK(copy_cols(sT, 16 * n, T, 16 * ldt, 16 * n, n, hipMemcpyHostToDevice, s));
     dT = sT;
     a = n;
   }

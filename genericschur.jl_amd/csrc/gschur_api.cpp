@@ -1060,9 +1060,9 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
     dW = static_cast<double*>(h->bW.p);
     dInfo = static_cast<int32_t*>(h->bInfo.p);
     dlda = dldt = dldz = n;
-    HIPCHK(h, hipMemcpy2DAsync(dA, es * n, A, es * lda, es * n, n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, copy_cols(dA, es * n, A, es * lda, es * n, n, hipMemcpyHostToDevice, h->stream));
     if ((flags & GS_HESSENBERG) && (flags & GS_WANTZ))
-      HIPCHK(h, hipMemcpy2DAsync(dZ, es * n, Z, es * ldz, es * n, n, hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, copy_cols(dZ, es * n, Z, es * ldz, es * n, n, hipMemcpyHostToDevice, h->stream));
   }
   const int nlds = cplx ? zschur_max_lds_n() : dschur_max_lds_n();
   const bool use_lds = !(flags & GS_PLAIN) && n <= nlds && dlda == n && dldt == n && dldz == n;
@@ -1188,9 +1188,9 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
   HIPCHK(h, hipMemcpyAsync(info, dInfo, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
   if (!dev) {
-    HIPCHK(h, hipMemcpy2DAsync(T, es * ldt, dT, es * n, es * n, n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, copy_cols(T, es * ldt, dT, es * n, es * n, n, hipMemcpyDeviceToHost, h->stream));
     if (flags & GS_WANTZ)
-      HIPCHK(h, hipMemcpy2DAsync(Z, es * ldz, dZ, es * n, es * n, n, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, copy_cols(Z, es * ldz, dZ, es * n, es * n, n, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipMemcpyAsync(W, dW, 16 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -126,6 +126,15 @@ inline int check_spans(gs_handle* h, int64_t count, const int32_t* n, const int6
   return GS_OK;
 }
 
+// Host <-> device copy of a column-major block (cols columns of width
+// bytes, pitches in bytes): one linear copy when both sides are packed,
+// the pitched copy otherwise (which moves pageable memory column by column).
+inline hipError_t copy_cols(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t cols,
+                            hipMemcpyKind kind, hipStream_t s) {
+  if (dpitch == width && spitch == width) return hipMemcpyAsync(dst, src, width * cols, kind, s);
+  return hipMemcpy2DAsync(dst, dpitch, src, spitch, width, cols, kind, s);
+}
+
 inline std::mutex& handle_mutex(gs_handle* h) { return h->mu; }
 inline int handle_device(gs_handle* h) { return h->device; }
 inline hipStream_t handle_stream(gs_handle* h) { return h->stream; }

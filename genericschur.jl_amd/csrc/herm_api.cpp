@@ -90,7 +90,7 @@ int heev_impl(gs_handle* h, bool cplx, int64_t count, int64_t n, const double* A
       HCHK(hipMemcpyAsync(dA, A, cnt * nn * es, hipMemcpyHostToDevice, s));
     } else {
       for (int64_t k = 0; k < count; ++k)
-        HCHK(hipMemcpy2DAsync(dA + k * nn * ed, n * es, A + k * sA * ed, lda * es, n * es, n, hipMemcpyHostToDevice,
+        HCHK(copy_cols(dA + k * nn * ed, n * es, A + k * sA * ed, lda * es, n * es, n, hipMemcpyHostToDevice,
                               s));
     }
     a.A = dA;
@@ -108,7 +108,7 @@ int heev_impl(gs_handle* h, bool cplx, int64_t count, int64_t n, const double* A
         HCHK(hipMemcpyAsync(V, dV, cnt * nn * es, hipMemcpyDeviceToHost, s));
       } else {
         for (int64_t k = 0; k < count; ++k)
-          HCHK(hipMemcpy2DAsync(V + k * sV * ed, ldv * es, dV + k * nn * ed, n * es, n * es, n,
+          HCHK(copy_cols(V + k * sV * ed, ldv * es, dV + k * nn * ed, n * es, n * es, n,
                                 hipMemcpyDeviceToHost, s));
       }
     }
@@ -159,7 +159,7 @@ int hetrd_impl(gs_handle* h, bool cplx, int64_t n, const double* A, int64_t lda,
   double* dd = reinterpret_cast<double*>(p + bA);
   double* de = reinterpret_cast<double*>(p + bA + bd);
   double* dQ = Q ? reinterpret_cast<double*>(p + bA + 2 * bd) : nullptr;
-  HCHK(hipMemcpy2DAsync(dA, n * es, A, lda * es, n * es, n, hipMemcpyHostToDevice, s));
+  HCHK(copy_cols(dA, n * es, A, lda * es, n * es, n, hipMemcpyHostToDevice, s));
   a.A = dA;
   a.lda = n;
   a.W = dd;
@@ -169,7 +169,7 @@ int hetrd_impl(gs_handle* h, bool cplx, int64_t n, const double* A, int64_t lda,
   HCHK(launch_heev(a, cplx, s));
   HCHK(hipMemcpyAsync(d, dd, (size_t)n * 8, hipMemcpyDeviceToHost, s));
   if (n > 1) HCHK(hipMemcpyAsync(e, de, (size_t)(n - 1) * 8, hipMemcpyDeviceToHost, s));
-  if (Q) HCHK(hipMemcpy2DAsync(Q, ldq * es, dQ, n * es, n * es, n, hipMemcpyDeviceToHost, s));
+  if (Q) HCHK(copy_cols(Q, ldq * es, dQ, n * es, n * es, n, hipMemcpyDeviceToHost, s));
   HCHK(hipStreamSynchronize(s));
   return GS_OK;
 }
@@ -241,7 +241,7 @@ int gs_steqr(gs_handle* h, int64_t n, double* d, double* e, double* Z, int64_t l
   double* dZ = wantz ? reinterpret_cast<double*>(p + 2 * bd + 256) : nullptr;
   HCHK(hipMemcpyAsync(dd, d, (size_t)n * 8, hipMemcpyHostToDevice, s));
   if (n > 1) HCHK(hipMemcpyAsync(de, e, (size_t)(n - 1) * 8, hipMemcpyHostToDevice, s));
-  if (wantz) HCHK(hipMemcpy2DAsync(dZ, zrows * es, Z, ldz * es, zrows * es, n, hipMemcpyHostToDevice, s));
+  if (wantz) HCHK(copy_cols(dZ, zrows * es, Z, ldz * es, zrows * es, n, hipMemcpyHostToDevice, s));
   a.d = dd;
   a.e = de;
   a.Z = dZ;
@@ -250,7 +250,7 @@ int gs_steqr(gs_handle* h, int64_t n, double* d, double* e, double* Z, int64_t l
   HCHK(launch_steqr(a, zcomplex != 0, s));
   HCHK(hipMemcpyAsync(d, dd, (size_t)n * 8, hipMemcpyDeviceToHost, s));
   if (n > 1) HCHK(hipMemcpyAsync(e, de, (size_t)(n - 1) * 8, hipMemcpyDeviceToHost, s));
-  if (wantz) HCHK(hipMemcpy2DAsync(Z, ldz * es, dZ, zrows * es, zrows * es, n, hipMemcpyDeviceToHost, s));
+  if (wantz) HCHK(copy_cols(Z, ldz * es, dZ, zrows * es, zrows * es, n, hipMemcpyDeviceToHost, s));
   if (info) HCHK(hipMemcpyAsync(info, dI, 4, hipMemcpyDeviceToHost, s));
   HCHK(hipStreamSynchronize(s));
   return GS_OK;

@@ -131,6 +131,16 @@ int ztrevc_max_n();
 int dtrevc_max_n();
 hipError_t launch_ztrevc(const TrevcArgs& a, int nmax, hipStream_t s);
 hipError_t launch_dtrevc(const TrevcArgsD& a, int nmax, hipStream_t s);
+// One matrix of order z/dtrevc_max_n() < n <= trevc_big_max_n() over the
+// whole GPU (solves one per workgroup with the vector in LDS, the
+// back-transform as FP64 MFMA GEMMs).  a.T, a.Z, a.V point at the matrix,
+// a.ld* its leading dimensions (0: n); order/count/n/off are not read.
+// ws: z/dtrevc_big_ws_bytes(n) bytes of device memory.
+int trevc_big_max_n();
+size_t dtrevc_big_ws_bytes(int n);
+size_t ztrevc_big_ws_bytes(int n);
+hipError_t launch_dtrevc_big(const TrevcArgsD& a, int n, void* ws, hipStream_t s);
+hipError_t launch_ztrevc_big(const TrevcArgs& a, int n, void* ws, hipStream_t s);
 
 // Eigenvalue VJP assembly: VR, VL (complex) at complex offset off[m], gbar
 // (complex) at woff[m], Abar (real) at off[m].

@@ -26,8 +26,10 @@ int run_trevc(gs_handle* h, bool cplx, int64_t count, const int32_t* n, const in
   int nmax = 0;
   for (int64_t i = 0; i < count; ++i) nmax = std::max(nmax, (int)n[i]);
   if (nmax == 0) return GS_OK;
+  // orders above the one-workgroup kernels' limit run one matrix at a time
+  // through the whole-GPU path, up to trevc_big_max_n()
   const int lim = cplx ? ztrevc_max_n() : dtrevc_max_n();
-  if (nmax > lim) return handle_fail(h, GS_E_UNSUPPORTED, "order above the eigenvector kernel limit");
+  if (nmax > trevc_big_max_n()) return handle_fail(h, GS_E_UNSUPPORTED, "order above the eigenvector kernel limit");
   std::lock_guard<std::mutex> lk(handle_mutex(h));
   hipError_t e = hipSetDevice(handle_device(h));
   if (e != hipSuccess) return handle_fail_hip(h, e, "hipSetDevice");
@@ -70,7 +72,7 @@ int run_trevc(gs_handle* h, bool cplx, int64_t count, const int32_t* n, const in
     // a single strided operand occupies ld (n - 1) + n elements of the
     // caller's buffer: copy column by column, never past its last column
     if (single)
-      e = hipMemcpy2DAsync(bT.p, es * ldt, T, es * ldt, es * n[0], n[0], hipMemcpyHostToDevice, s);
+      e = copy_cols(bT.p, es * ldt, T, es * ldt, es * n[0], n[0], hipMemcpyHostToDevice, s);
     else
       e = hipMemcpyAsync(bT.p, T, es * spanT, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return handle_fail_hip(h, e, "hipMemcpyAsync");
@@ -78,7 +80,7 @@ int run_trevc(gs_handle* h, bool cplx, int64_t count, const int32_t* n, const in
     if (Z) {
       if ((e = bZ.ensure(es * spanZ)) != hipSuccess) return handle_fail_hip(h, e, "hipMalloc");
       if (single)
-        e = hipMemcpy2DAsync(bZ.p, es * ldz, Z, es * ldz, es * n[0], n[0], hipMemcpyHostToDevice, s);
+        e = copy_cols(bZ.p, es * ldz, Z, es * ldz, es * n[0], n[0], hipMemcpyHostToDevice, s);
       else
         e = hipMemcpyAsync(bZ.p, Z, es * spanZ, hipMemcpyHostToDevice, s);
       if (e != hipSuccess) return handle_fail_hip(h, e, "hipMemcpyAsync");
@@ -99,11 +101,36 @@ int run_trevc(gs_handle* h, bool cplx, int64_t count, const int32_t* n, const in
   a.ldt = single ? ldt : 0;
   a.ldz = single ? ldz : 0;
   a.ldv = single ? ldv : 0;
-  e = cplx ? launch_ztrevc(a, nmax, s) : launch_dtrevc(a, nmax, s);
-  if (e != hipSuccess) return handle_fail_hip(h, e, "kernel launch");
+  // the order array lists the matrices largest first: those above lim lead
+  int64_t nbig = 0;
+  while (nbig < count && n[order[nbig]] > lim) ++nbig;
+  if (nbig > 0) {
+    DevBuf& ws = handle_buf(h, 4);
+    const size_t wsb = cplx ? ztrevc_big_ws_bytes(nmax) : dtrevc_big_ws_bytes(nmax);
+    if ((e = ws.ensure(wsb)) != hipSuccess) return handle_fail_hip(h, e, "hipMalloc");
+    const int64_t ew = cplx ? 2 : 1;  // doubles per element of T and Z
+    for (int64_t b = 0; b < nbig; ++b) {
+      const int32_t m = order[b];
+      TrevcArgs one = a;
+      one.T = dT + ew * off[m];
+      one.Z = dZ ? dZ + ew * off[m] : nullptr;
+      one.V = dV + 2 * off[m];
+      one.ldt = single ? ldt : n[m];
+      one.ldz = single ? ldz : n[m];
+      one.ldv = single ? ldv : n[m];
+      e = cplx ? launch_ztrevc_big(one, n[m], ws.p, s) : launch_dtrevc_big(one, n[m], ws.p, s);
+      if (e != hipSuccess) return handle_fail_hip(h, e, "kernel launch");
+    }
+  }
+  if (nbig < count && n[order[nbig]] > 0) {
+    a.order = d_order + nbig;
+    a.count = (int)(count - nbig);
+    e = cplx ? launch_ztrevc(a, n[order[nbig]], s) : launch_dtrevc(a, n[order[nbig]], s);
+    if (e != hipSuccess) return handle_fail_hip(h, e, "kernel launch");
+  }
   if (!dev) {
     if (single) {
-      e = hipMemcpy2DAsync(V, 16 * ldv, dV, 16 * ldv, 16 * n[0], n[0], hipMemcpyDeviceToHost, s);
+      e = copy_cols(V, 16 * ldv, dV, 16 * ldv, 16 * n[0], n[0], hipMemcpyDeviceToHost, s);
       if (e != hipSuccess) return handle_fail_hip(h, e, "hipMemcpy2DAsync");
     } else {
       std::vector<double> hv(2 * (size_t)span);

@@ -34,5 +34,26 @@ template <int R> struct WaveVecZ {
   }
 };
 
+// The same vector kept in LDS (64 R entries at p, entry i at p[i]), for
+// orders whose vectors would not fit the registers.  v[q] is lane l's
+// entry l + 64 q as in WaveVecZ; get() reads the entry every lane sees
+// (the wavefront's LDS accesses complete in order, so a lane reads what
+// another lane of the wavefront stored before).
+struct LdsLaneRef {
+  zd* p;
+  GS_DEV zd& operator[](int q) const { return p[(threadIdx.x & 63) + 64 * q]; }
+};
+template <int R> struct WaveVecL {
+  LdsLaneRef v;
+  GS_DEV zd get(int i) const { return v.p[i]; }
+  GS_DEV void set(int i, zd val) {
+    if ((threadIdx.x & 63) == 0) v.p[i] = val;
+  }
+  GS_DEV void scale(double s) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = v[k] * s;
+  }
+};
+
 // wave-wide maximum over lanes (all lanes receive it)
 GS_DEV double wmax(double x) { return wave_max(x); }

@@ -15,6 +15,7 @@
 
 #include "dev_common.h"
 #include "internal.h"
+#include "trevc_big.h"
 #include "wavevec.h"
 
 namespace gs {
@@ -46,8 +47,8 @@ template <class TA> GS_DEV zd shifted_diag(const TA& Tc, int j, zd lam, double s
 }
 
 // _usolve! on entries 0..m-1 of x (A = T shifted by lam, leading m x m).
-template <int R, class TA> GS_DEV double usolve_w(const TA& Tc, const double* cnorm, int m, zd lam, double smin,
-                                                  WaveVecZ<R>& x) {
+template <int R, class TA, class VEC> GS_DEV double usolve_w(const TA& Tc, const double* cnorm, int m, zd lam,
+                                                             double smin, VEC& x) {
   const int lane = threadIdx.x & 63;
   const double half = 0.5;
   const double smallnum = GS_FMIN / GS_EPS;
@@ -175,8 +176,8 @@ template <int R, class TA> GS_DEV double usolve_w(const TA& Tc, const double* cn
 // _cusolve! on entries o..o+m-1 of x: (A)^H y = s b with
 // A(i, j) = T(o+i, o+j) for i < j and the shifted, clamped diagonal; cnorm
 // is indexed from 0 (the reference passes the full tnorms, vectors.jl:417-420).
-template <int R, class TA> GS_DEV double cusolve_w(const TA& Tc, const double* cnorm, int o, int m, zd lam,
-                                                   double smin, WaveVecZ<R>& x) {
+template <int R, class TA, class VEC> GS_DEV double cusolve_w(const TA& Tc, const double* cnorm, int o, int m, zd lam,
+                                                              double smin, VEC& x) {
   const int lane = threadIdx.x & 63;
   const double half = 0.5;
   const double smallnum = GS_FMIN / GS_EPS;
@@ -313,7 +314,7 @@ template <int R, class TA> GS_DEV double cusolve_w(const TA& Tc, const double* c
 
 // max abs1 scaling then _enormalize! of one column held in a WaveVecZ
 // (entries 0..n-1); normalize = 0 stops after the max-abs1 scaling.
-template <int R> GS_DEV void normalize_col(WaveVecZ<R>& v, int n, bool enorm) {
+template <int R, class VEC> GS_DEV void normalize_col(VEC& v, int n, bool enorm) {
   const int lane = threadIdx.x & 63;
   double t0 = 0.0;
 #pragma unroll
@@ -494,5 +495,167 @@ hipError_t launch_ztrevc(const TrevcArgs& a, int nmax, hipStream_t s) {
   return hipGetLastError();
 }
 int ztrevc_max_n() { return 1024; }
+
+// ------------------------------------------------------- larger orders
+// One matrix of order 1024 < n <= trevc_big_max_n() over the whole GPU:
+// column norms in global memory, one solve per workgroup (a single
+// wavefront, solution vector in LDS), solutions in planar X = [Re | Im]
+// (n x 2n), the back-transform Z X as three FP64 MFMA GEMMs on Z's planar
+// copy, then one wavefront per column normalises and writes V.  The
+// per-column steps are those of ztrevc_kernel.
+struct ZBig {
+  int n, left, normalize;
+  const zd* T;
+  int64_t ldt;
+  const zd* Tt;  // T's transpose (left solves), ld n
+  const zd* Z;   // may be null
+  int64_t ldz;
+  zd* V;
+  int64_t ldv;
+  double* tnorms;
+  double* Zp;  // [Re Z | Im Z], n x 2n, ld n
+  double* X;   // n x 2n, ld n
+  double* Y;   // n x 2n, ld n
+};
+
+// workgroup t solves column n-1-t (right) or t (left), longest first
+template <int R, class TA> __global__ void __launch_bounds__(64) ztrevc_big_solve(ZBig a, TA Tc) {
+  extern __shared__ double smem[];
+  const int n = a.n, lane = threadIdx.x & 63, t = blockIdx.x;
+  const bool left = a.left != 0;
+  const int kk = left ? t : n - 1 - t;
+  const zd lam = Tc(kk, kk);
+  const double smin = fmax(GS_EPS * zabs1(lam), GS_FMIN * (double)n);  // vectors.jl:61,386
+  WaveVecL<R> x{{reinterpret_cast<zd*>(smem)}};
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = lane + 64 * q;
+    zd v = zr(0.0);
+    if (!left) {
+      if (i < kk) v = -Tc(i, kk);
+    } else {
+      if (i > kk && i < n) v = -zconj(Tc(kk, i));
+    }
+    x.v[q] = v;
+  }
+  double vscale;
+  if (!left) vscale = (kk > 0) ? usolve_w<R>(Tc, a.tnorms, kk, lam, smin, x) : 1.0;
+  else vscale = (kk < n - 1) ? cusolve_w<R>(Tc, a.tnorms, kk + 1, n - 1 - kk, lam, smin, x) : 1.0;
+  x.set(kk, zr(vscale));
+  double* xr = a.X + (int64_t)kk * n;
+  double* xi = xr + (int64_t)n * n;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = lane + 64 * q;
+    if (i < n) {
+      const zd z = (left ? (i >= kk) : (i <= kk)) ? x.v[q] : zr(0.0);
+      xr[i] = z.x;
+      xi[i] = z.y;
+    }
+  }
+}
+
+// Zp = [Re Z | Im Z]
+__global__ void __launch_bounds__(256) ztrevc_big_planar(int n, const zd* Z, int64_t ldz, double* Zp) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nn; e += (int64_t)gridDim.x * 256) {
+    const int64_t c = e / n, r = e - c * n;
+    const zd z = Z[r + c * ldz];
+    Zp[e] = z.x;
+    Zp[e + nn] = z.y;
+  }
+}
+
+template <int R> __global__ void __launch_bounds__(64) ztrevc_big_finish(ZBig a) {
+  extern __shared__ double smem[];
+  const int n = a.n, lane = threadIdx.x & 63, kk = blockIdx.x;
+  WaveVecL<R> y{{reinterpret_cast<zd*>(smem)}};
+  const double* src = a.Z ? a.Y : a.X;
+  const double* yr = src + (int64_t)kk * n;
+  const double* yi = yr + (int64_t)n * n;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int r = lane + 64 * q;
+    y.v[q] = (r < n) ? zmk(yr[r], yi[r]) : zr(0.0);
+  }
+  normalize_col<R>(y, n, a.normalize != 0);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int r = lane + 64 * q;
+    if (r < n) a.V[r + (int64_t)kk * a.ldv] = y.v[q];
+  }
+}
+
+size_t ztrevc_big_ws_bytes(int n) {
+  const size_t nn = (size_t)n * n;
+  return trevc_seg(8 * (size_t)n) + 4 * trevc_seg(16 * nn);
+}
+
+template <int R> static hipError_t launch_zbig_r(const ZBig& b, hipStream_t s) {
+  const int n = b.n;
+  const size_t lds = 64 * R * sizeof(zd);
+  if (b.left) {
+    auto k = ztrevc_big_solve<R, TransG<zd>>;
+    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, dim3(n), dim3(64), lds, s, b, TransG<zd>{b.Tt, n});
+  } else {
+    auto k = ztrevc_big_solve<R, GlobZ>;
+    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, dim3(n), dim3(64), lds, s, b, GlobZ{b.T, b.ldt});
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (b.Z) {
+    // [Yr | Yi] = Re Z [Xr | Xi]; Yr -= Im Z Xi; Yi += Im Z Xr
+    const int64_t nn = (int64_t)n * n;
+    const double *Zr = b.Zp, *Zi = b.Zp + nn, *Xr = b.X, *Xi = b.X + nn;
+    double *Yr = b.Y, *Yi = b.Y + nn;
+    if ((e = launch_dgemm(false, false, n, 2 * n, n, 1.0, Zr, n, Xr, n, 0.0, Yr, n, s)) != hipSuccess) return e;
+    if ((e = launch_dgemm(false, false, n, n, n, -1.0, Zi, n, Xi, n, 1.0, Yr, n, s)) != hipSuccess) return e;
+    if ((e = launch_dgemm(false, false, n, n, n, 1.0, Zi, n, Xr, n, 1.0, Yi, n, s)) != hipSuccess) return e;
+  }
+  (void)hipFuncSetAttribute((const void*)ztrevc_big_finish<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(ztrevc_big_finish<R>, dim3(n), dim3(64), lds, s, b);
+  return hipGetLastError();
+}
+
+// a.T, a.Z, a.V point at the matrix (complex interleaved), ld from
+// a.ldt/ldz/ldv; ws holds ztrevc_big_ws_bytes(n)
+hipError_t launch_ztrevc_big(const TrevcArgs& a, int n, void* ws, hipStream_t s) {
+  if (n <= ztrevc_max_n() || n > trevc_big_max_n() || !ws) return hipErrorInvalidValue;
+  const size_t nn = (size_t)n * n;
+  char* p = static_cast<char*>(ws);
+  ZBig b;
+  b.n = n;
+  b.left = a.left;
+  b.normalize = a.normalize;
+  b.T = reinterpret_cast<const zd*>(a.T);
+  b.ldt = a.ldt > 0 ? a.ldt : n;
+  b.Z = reinterpret_cast<const zd*>(a.Z);
+  b.ldz = a.ldz > 0 ? a.ldz : n;
+  b.V = reinterpret_cast<zd*>(a.V);
+  b.ldv = a.ldv > 0 ? a.ldv : n;
+  b.tnorms = reinterpret_cast<double*>(p);
+  p += trevc_seg(8 * (size_t)n);
+  zd* tt = reinterpret_cast<zd*>(p);
+  b.Tt = tt;
+  p += trevc_seg(16 * nn);
+  b.Zp = reinterpret_cast<double*>(p);
+  p += trevc_seg(16 * nn);
+  b.X = reinterpret_cast<double*>(p);
+  p += trevc_seg(16 * nn);
+  b.Y = reinterpret_cast<double*>(p);
+  // strictly-upper column norms: abs for right (vectors.jl:70-75), abs1 for
+  // left (vectors.jl:395-400)
+  hipLaunchKernelGGL(trevc_colnorm_kernel<zd>, dim3((n + 3) / 4), dim3(256), 0, s, n, b.T, b.ldt, b.left != 0,
+                     b.tnorms);
+  if (b.left)
+    hipLaunchKernelGGL(trevc_transpose_kernel<zd>, dim3((n + 31) / 32, (n + 31) / 32), dim3(256), 0, s, n, b.T,
+                       b.ldt, tt);
+  if (b.Z) hipLaunchKernelGGL(ztrevc_big_planar, dim3(2048), dim3(256), 0, s, n, b.Z, b.ldz, b.Zp);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return n <= 2048 ? launch_zbig_r<32>(b, s) : launch_zbig_r<64>(b, s);
+}
 
 }  // namespace gs

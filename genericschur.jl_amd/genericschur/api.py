@@ -153,8 +153,7 @@ def _single(A, wantZ, scale, maxiter, maxinner, hess=False, Z0=None, checksd=Fal
         raise DimensionMismatch("bad dimension")
     _lib.check(rc, h, "gs_schur")
     _raise_info(info.value, maxiter, n)
-    T = np.ascontiguousarray(T)
-    Zo = np.ascontiguousarray(Z) if wantZ else np.zeros((0, 0), dtype=dt)
+    Zo = Z if wantZ else np.zeros((0, 0), dtype=dt)
     return Schur(T, Zo, W)
 
 
@@ -260,7 +259,7 @@ def geigvecs(S, left=False, normalize=True, device=0):
     ld = max(n, 1)
     rc = f(h, n, _ptr(t), ld, None if z is None else _ptr(z), ld, _ptr(V), ld, int(bool(left)), int(bool(normalize)), 0)
     _lib.check(rc, h, "gs_trevc")
-    return np.ascontiguousarray(V)
+    return V
 
 
 def eigvecs(S, left=False, device=0):
@@ -436,7 +435,7 @@ def balance_(A, scale=True, permute=True, device=0):
         raise RuntimeError("NaN encountered while balancing")
     if out[3] != 0:
         raise GSError(f"balancing did not settle (status {out[3]})")
-    return np.ascontiguousarray(a), Balancer(int(out[0]), int(out[1]), sp[:n].copy(), D[:n].copy(), bool(out[2]))
+    return a, Balancer(int(out[0]), int(out[1]), sp[:n].copy(), D[:n].copy(), bool(out[2]))
 
 
 def _bal_apply(B, V, inverse, device):
@@ -453,7 +452,7 @@ def _bal_apply(B, V, inverse, device):
     h = _lib.handle(device)
     rc = _lib.lib().gs_balance_apply(h, n, m, _ptr(b3), _ptr(sp), _ptr(D), _ptr(v), max(n, 1), int(inverse), 0)
     _lib.check(rc, h, "gs_balance_apply")
-    return np.ascontiguousarray(v)
+    return v
 
 
 def lmul_(B, V, device=0):
@@ -485,7 +484,7 @@ def triangularize(S, device=0):
     rc = _lib.lib().gs_triangularize_d(h, n, _ptr(t), ld, None if z is None else _ptr(z), ld, _ptr(Tc), ld,
                                        None if z is None else _ptr(Zc), ld, _ptr(W), 0)
     _lib.check(rc, h, "gs_triangularize_d")
-    return Schur(np.ascontiguousarray(Tc), np.ascontiguousarray(Zc), W)
+    return Schur(Tc, Zc, W)
 
 
 # ------------------------------------------- reordering and conditioning
@@ -511,7 +510,7 @@ def _ordschur_real(S, sel, n, device):
     _lib.check(rc, h, "gs_ordschur_d")
     if info[0]:
         raise IllConditionException(int(info[0]))
-    return Schur(np.ascontiguousarray(t), np.ascontiguousarray(z) if z is not None else np.zeros((0, 0)), W)
+    return Schur(t, z if z is not None else np.zeros((0, 0)), W)
 
 
 def ordschur_(S, select, device=0):
@@ -538,7 +537,7 @@ def ordschur_(S, select, device=0):
     ld = max(n, 1)
     rc = _lib.lib().gs_ordschur_z(h, n, _ptr(t), ld, None if z is None else _ptr(z), ld, _ptr(sel), _ptr(W), 0)
     _lib.check(rc, h, "gs_ordschur_z")
-    return Schur(np.ascontiguousarray(t), np.ascontiguousarray(z) if z is not None else np.zeros((0, 0), complex), W)
+    return Schur(t, z if z is not None else np.zeros((0, 0), complex), W)
 
 
 def eigcond(S, nsub, device=0):
@@ -580,7 +579,7 @@ def enormalize_(V, device=0):
     h = _lib.handle(device)
     rc = _lib.lib().gs_enormalize(h, n, m, _ptr(v), max(n, 1), 0)
     _lib.check(rc, h, "gs_enormalize")
-    return np.ascontiguousarray(v)
+    return v
 
 
 def _eig_rcond(VL, VR, device):
@@ -676,7 +675,7 @@ def hessenberg_hermitian(A, uplo="L", wantQ=True, device=0):
     f = _lib.lib().gs_hetrd_z if cplx else _lib.lib().gs_hetrd_d
     rc = f(h, n, _ptr(a), max(n, 1), _uplo(uplo), _ptr(d), _ptr(e), None if q is None else _ptr(q), max(n, 1), 0)
     _lib.check(rc, h, "gs_hetrd")
-    return d[:n], e[:max(n - 1, 0)], (np.ascontiguousarray(q[:n, :n]) if wantQ else None)
+    return d[:n], e[:max(n - 1, 0)], (q[:n, :n] if wantQ else None)
 
 
 def _sort_host(w, V, sortby):
@@ -764,7 +763,7 @@ def steqr_(d, e, Z=None, device=0):
     rc = _lib.lib().gs_steqr(h, n, _ptr(d) if n else None, _ptr(e1), None if z is None else _ptr(z), max(zr, 1), zr,
                              zc, _ptr(info), 0)
     _lib.check(rc, h, "gs_steqr")
-    return d, (np.ascontiguousarray(z) if z is not None else None), e1[:max(n - 1, 0)], int(info[0])
+    return d, (z if z is not None else None), e1[:max(n - 1, 0)], int(info[0])
 
 
 def gschur_symtridiag(d, e, Z=None, device=0):

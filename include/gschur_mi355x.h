@@ -180,8 +180,11 @@ int gs_schur_batched_z_noalloc(gs_handle* h, int64_t count, const int32_t* n, co
  * util.jl:128-461; real right/left: vectors.jl:133-362 / :462-698 with
  * _xsolve/_safe_lu_solve! util.jl:603-751).  Z may be NULL (vectors of T).
  * V is n x n complex (ldv).  normalize = 0 returns the max-abs1-scaled
- * vectors before _enormalize!.  Orders up to 1024 (GS_E_UNSUPPORTED above):
- * T in LDS up to 128 (complex) / 192 (real), read from global memory above. */
+ * vectors before _enormalize!.  Orders up to 4096 (GS_E_UNSUPPORTED above):
+ * one workgroup per matrix up to 1024 (T in LDS up to 128 complex / 192
+ * real, read from global memory above); above 1024 one matrix at a time
+ * over the whole GPU, with n x n workspaces (real 40 n^2 bytes, complex
+ * 64 n^2) held by the handle. */
 int gs_trevc_z(gs_handle* h, int64_t n, const double* T, int64_t ldt, const double* Z, int64_t ldz, double* V,
                int64_t ldv, int left, int normalize, int flags);
 int gs_trevc_d(gs_handle* h, int64_t n, const double* T, int64_t ldt, const double* Z, int64_t ldz, double* V,

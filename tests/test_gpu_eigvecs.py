@@ -139,3 +139,82 @@ def test_batched_vectors(gs, oracle):
                 Vg = V[o:o + m * m].reshape((m, m), order="F")
                 Vo = oracle.eigvecs(T, Z, left=bool(left))
                 assert np.abs(Vg - Vo).max() <= 1e-7
+
+
+def _schur_lapack(n, cplx, seed):
+    """A random matrix and its Schur factors from LAPACK (scipy): the inputs
+    of the orders above 1024, where the oracle's own Schur factorization
+    would dominate the test's time.  The comparison isolates the
+    back-substitution either way (T and Z go to both sides)."""
+    import scipy.linalg as sl
+    r = np.random.default_rng(seed)
+    A = r.standard_normal((n, n))
+    if cplx:
+        A = A + 1j * r.standard_normal((n, n))
+        T, Z = sl.schur(A, output="complex")
+        return A, T, Z, np.diag(T).copy()
+    T, Z = sl.schur(A)
+    w = np.diag(T).astype(np.complex128)
+    j = 0
+    while j < n - 1:
+        if T[j + 1, j] != 0.0:
+            wi = np.sqrt(abs(T[j + 1, j])) * np.sqrt(abs(T[j, j + 1]))
+            w[j] = T[j, j] + 1j * wi
+            w[j + 1] = T[j, j] - 1j * wi
+            j += 2
+        else:
+            j += 1
+    return A, T, Z, w
+
+
+@pytest.mark.parametrize("n,cplx", [(1100, False), (2100, False), (1100, True)])
+def test_vectors_large_orders(gs, oracle, n, cplx):
+    """Orders above 1024 run one matrix at a time over the whole GPU (one
+    solve per workgroup with the vector in LDS, the back-transform as FP64
+    GEMMs); n = 2100 takes the 4096-entry vectors.  Entrywise against the
+    oracle and the reference's vectest residuals."""
+    A, T, Z, w = _schur_lapack(n, cplx, 300 + n)
+    _compare(oracle, gs, A, T, Z, w, tol=1e-7)
+
+
+def test_vectors_large_complex_4096_entry(gs, oracle):
+    """Complex n = 2100: the 4096-entry vectors of the complex solves."""
+    A, T, Z, w = _schur_lapack(2100, True, 2400)
+    _compare(oracle, gs, A, T, Z, w, tol=1e-7)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_vectors_large_no_z(gs, oracle, cplx):
+    """Eigenvectors of T itself (no Z) above order 1024: the finish takes the
+    solutions directly, with the support-limited max scaling."""
+    A, T, Z, w = _schur_lapack(1030, cplx, 77)
+    _compare(oracle, gs, T, T, Z, w, tol=1e-7, use_z=False)
+
+
+def test_batched_vectors_with_large_order(gs, oracle):
+    """A batch holding one matrix above 1024 among small ones: the large one
+    runs through the whole-GPU path, the rest through the batched kernel."""
+    import ctypes
+    from genericschur import _lib
+    for cplx in (False, True):
+        dt = np.complex128 if cplx else np.float64
+        Ts, Zs = [], []
+        for k, m in enumerate((40, 1050, 7, 300)):
+            _, T, Z, _ = _schur_lapack(m, cplx, 900 + k)
+            Ts.append(T)
+            Zs.append(Z)
+        ns, off, tflat = gs.pack(Ts, dt)
+        _, _, zflat = gs.pack(Zs, dt)
+        V = np.zeros(len(tflat), dtype=np.complex128)
+        h = _lib.handle()
+        f = _lib.lib().gs_trevc_batched_z if cplx else _lib.lib().gs_trevc_batched_d
+        p = lambda x: ctypes.c_void_p(x.ctypes.data)
+        for left in (0, 1):
+            print(f"batched large cplx={cplx} left={left}", flush=True)
+            rc = f(h, len(ns), p(ns), p(off), p(tflat), p(zflat), p(V), left, 1, 0)
+            _lib.check(rc, h, "gs_trevc_batched")
+            for k, (T, Z) in enumerate(zip(Ts, Zs)):
+                m, o = int(ns[k]), int(off[k])
+                Vg = V[o:o + m * m].reshape((m, m), order="F")
+                Vo = oracle.eigvecs(T, Z, left=bool(left))
+                assert np.abs(Vg - Vo).max() <= 1e-7, (cplx, left, m)
