@@ -20,6 +20,8 @@
 //
 // steqr_kernel: the QL/QR iteration alone on (d, e) with Z in global memory
 // (gschur!(H, Z) / _gschur!(A::SymTridiagonal, QRIteration(), Z)).
+#include <algorithm>
+
 #include "dev_common.h"
 #include "formq.h"
 #include "internal.h"
@@ -468,6 +470,223 @@ template <class TZ, int NT> __global__ void __launch_bounds__(NT) steqr_kernel(S
   }
 }
 
+// ------------------------------------------------------------------------
+// Orders above the LDS-resident kernels (heev_max_n < n <= heev_big_max_n),
+// one matrix at a time over the whole GPU, the same steps as heev_tq_kernel
+// + steqr + sorteig!: _hehessenberg! on the lower triangle (hessenberg.jl:
+// 27-60), Q by backward accumulation (:150-166), the QL/QR iteration
+// (symtridiag.jl:77-427) on Q in HBM, the stable sort.  The working matrix
+// stays in HBM in full Hermitian storage (the upper triangle the exact
+// conjugate of the lower, the diagonal real), so the product with the
+// trailing block is a column-major gemv.  Each reflector step is four
+// launches (reflector, gemv partials, combine, rank-2 update), each
+// reflector of Q's accumulation two.
+template <class T> struct HbArgs {
+  int n = 0, p = 0, ks = 1, kchunk = 0;
+  T* M = nullptr;    // n x n, ld n
+  T* tau = nullptr;  // n
+  T* v = nullptr;    // n: the current reflector [1; tail]
+  T* w = nullptr;    // n
+  T* P = nullptr;    // ks x n gemv partials (ld n)
+  T* Q = nullptr;    // n x n, ld n
+  T* u = nullptr;    // n: Q accumulation dots
+  double* d = nullptr;
+  double* e = nullptr;
+};
+
+GS_DEV double twsum(double v) { return wave_sum(v); }
+GS_DEV zd twsum(zd v) { return wave_zsum(v); }
+
+// M = the lower triangle of A (of J A J for the upper form, J the exchange
+// matrix) with its conjugate mirrored above and the diagonal real
+template <class T> __global__ void __launch_bounds__(256) hb_load_kernel(HbArgs<T> a, const T* A, int64_t lda, int upper) {
+  const int n = a.n;
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    const int li = i >= j ? i : j, lj = i >= j ? j : i;
+    T x = upper ? A[(n - 1 - li) + (int64_t)(n - 1 - lj) * lda] : A[li + (int64_t)lj * lda];
+    if (i == j) x = tfrom<T>(tre(x));
+    else if (i < j) x = tconj(x);
+    a.M[idx] = x;
+  }
+}
+
+// reflector of step p on M[p+1 :, p]; v, tau[p], e[p], d[p]
+template <class T> __global__ void __launch_bounds__(64) hb_reflect_kernel(HbArgs<T> a) {
+  const int n = a.n, p = a.p, m = n - 1 - p, lane = threadIdx.x;
+  T* x = a.M + (p + 1) + (int64_t)p * n;
+  const T tp = he_reflector(x, m, lane);
+  // each lane reads back the tail entries it scaled
+  for (int k = 1 + lane; k < m; k += 64) a.v[k] = x[k];
+  if (lane == 0) {
+    a.v[0] = tfrom<T>(1.0);
+    a.tau[p] = tp;
+    a.e[p] = tre(x[0]);
+    a.d[p] = tre(a.M[p + (int64_t)p * n]);
+  }
+}
+
+// P[y][i] = sum over column chunk y of B(i, k) v[k], B the trailing block
+template <class T> __global__ void __launch_bounds__(256) hb_gemv_kernel(HbArgs<T> a) {
+  __shared__ double raw[4 * 64 * 2];
+  T* red = reinterpret_cast<T*>(raw);
+  const int n = a.n, p = a.p, m = n - 1 - p;
+  if (tzero(a.tau[p])) return;
+  const T* B = a.M + (p + 1) + (int64_t)(p + 1) * n;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  const int k0 = blockIdx.y * a.kchunk, k1 = min(m, k0 + a.kchunk);
+  T acc = tfrom<T>(0.0);
+  if (i < m)
+    for (int k = k0 + wv; k < k1; k += 4) acc = acc + B[i + (int64_t)k * n] * a.v[k];
+  red[wv * 64 + lane] = acc;
+  __syncthreads();
+  if (wv == 0 && i < m)
+    a.P[(int64_t)blockIdx.y * n + i] = ((red[lane] + red[64 + lane]) + red[128 + lane]) + red[192 + lane];
+}
+
+// w = tau B v; w += -(tau / 2) (w^H v) v
+template <class T> __global__ void __launch_bounds__(1024) hb_combine_kernel(HbArgs<T> a) {
+  __shared__ double red[16];
+  const int n = a.n, p = a.p, m = n - 1 - p, tid = threadIdx.x;
+  const T tp = a.tau[p];
+  if (tzero(tp)) return;
+  T part = tfrom<T>(0.0);
+  for (int i = tid; i < m; i += 1024) {
+    T s = a.P[i];
+    for (int q = 1; q < a.ks; ++q) s = s + a.P[(int64_t)q * n + i];
+    const T wi = tp * s;
+    a.w[i] = wi;
+    part = part + tconj(wi) * a.v[i];
+  }
+  const T dot = tblock_sum<1024>(part, red);
+  const T alpha = -(tp * 0.5) * dot;
+  for (int i = tid; i < m; i += 1024) a.w[i] = a.w[i] + alpha * a.v[i];
+}
+
+// rank2update!(B, -1, v, w, Val(:L)) on the lower triangle, 32 x 32 tiles,
+// each lower tile's conjugate written to its mirror
+template <class T> __global__ void __launch_bounds__(256) hb_rank2_kernel(HbArgs<T> a) {
+  __shared__ double raw[32 * 33 * 2];
+  T* tile = reinterpret_cast<T*>(raw);
+  const int n = a.n, p = a.p, m = n - 1 - p;
+  const int I0 = blockIdx.x * 32, J0 = blockIdx.y * 32;
+  if (I0 < J0) return;
+  const T tp = a.tau[p];
+  if (tzero(tp)) return;  // B unchanged; its diagonal is real already
+  T* B = a.M + (p + 1) + (int64_t)(p + 1) * n;
+  const int t = threadIdx.x, rl = t & 31, cg = t >> 5;
+  const int r = I0 + rl;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int cl = cg + 8 * s, c = J0 + cl;
+    if (r < m && c < m && r >= c) {
+      const T vc = a.v[c], wc = a.w[c];
+      const T old = B[r + (int64_t)c * n];
+      T nw;
+      if (tzero(vc) && tzero(wc)) {
+        nw = (r == c) ? tfrom<T>(tre(old)) : old;
+      } else {
+        const T t1 = -tconj(wc), t2 = tconj(-vc);
+        const T upd = a.v[r] * t1 + a.w[r] * t2;
+        nw = (r == c) ? tfrom<T>(tre(old) + tre(upd)) : old + upd;
+      }
+      B[r + (int64_t)c * n] = nw;
+      tile[cl * 33 + rl] = nw;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int rl2 = cg + 8 * s, cl2 = t & 31;
+    const int rr = I0 + rl2, cc = J0 + cl2;
+    if (rr < m && cc < m && rr > cc) B[cc + (int64_t)rr * n] = tconj(tile[cl2 * 33 + rl2]);
+  }
+}
+
+template <class T> __global__ void __launch_bounds__(64) hb_tail_kernel(HbArgs<T> a) {
+  if (threadIdx.x == 0) a.d[a.n - 1] = tre(a.M[(int64_t)(a.n - 1) * (a.n + 1)]);
+}
+
+template <class T> __global__ void __launch_bounds__(256) hb_identity_kernel(int n, T* Q) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256)
+    Q[idx] = tfrom<T>((idx % n) == (idx / n) ? 1.0 : 0.0);
+}
+
+// reflector k = a.p applied to Q(c:, c+1:), c = k + 1: the dots
+// u[j] = tau (Q[c, j] + sum_{r > c} conj(v_r) Q[r, j]), one wavefront per column
+template <class T> __global__ void __launch_bounds__(256) hq_dots_kernel(HbArgs<T> a) {
+  const int n = a.n, k = a.p, c = k + 1;
+  const int j = c + 1 + blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (j >= n) return;
+  T acc = tfrom<T>(0.0);
+  for (int r = c + 1 + lane; r < n; r += 64) acc = acc + tconj(a.M[r + (int64_t)k * n]) * a.Q[r + (int64_t)j * n];
+  acc = twsum(acc);
+  if (lane == 0) a.u[j] = a.tau[k] * (a.Q[c + (int64_t)j * n] + acc);
+}
+
+// Q(r, j) -= u_j [1; v]_r for j > c; column c = e_c - tau [1; v]
+template <class T> __global__ void __launch_bounds__(256) hq_update_kernel(HbArgs<T> a) {
+  const int n = a.n, k = a.p, c = k + 1;
+  const int r = c + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = c + blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (r >= n || j >= n) return;
+  const T t = a.tau[k];
+  T* q = a.Q + r + (int64_t)j * n;
+  if (j == c) {
+    *q = (r == c) ? tfrom<T>(1.0) - t : -(t * a.M[r + (int64_t)k * n]);
+  } else {
+    const T h = *q, sj = a.u[j];
+    *q = (r == c) ? h - sj : h - sj * a.M[r + (int64_t)k * n];
+  }
+}
+
+// outputs in the reference's coordinates: d, e (reversed for the upper
+// form) and Q (J Q J for the upper form) into Zo (ld ldz)
+template <class T> __global__ void __launch_bounds__(256) hb_out_kernel(HbArgs<T> a, int upper, double* dref,
+                                                                       double* eref, T* Zo, int64_t ldz) {
+  const int n = a.n;
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x, G = (int64_t)gridDim.x * 256;
+  for (int64_t i = g; i < n; i += G) dref[i] = upper ? a.d[n - 1 - i] : a.d[i];
+  for (int64_t i = g; i < n - 1; i += G) eref[i] = upper ? a.e[n - 2 - i] : a.e[i];
+  if (Zo) {
+    const int64_t nn = (int64_t)n * n;
+    for (int64_t idx = g; idx < nn; idx += G) {
+      const int r = (int)(idx % n), j = (int)(idx / n);
+      Zo[r + (int64_t)j * ldz] = upper ? a.Q[(n - 1 - r) + (int64_t)(n - 1 - j) * n] : a.Q[idx];
+    }
+  }
+}
+
+// sorteig! (stable, isless) of d into W; perm[j] = source column
+__global__ void __launch_bounds__(1024) hb_sort_kernel(int n, int sort, const double* d, int* perm, double* W) {
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    int rank = i;
+    if (sort) {
+      const double di = d[i];
+      rank = 0;
+      for (int j = 0; j < n; ++j) {
+        const double dj = d[j];
+        rank += (jl_isless(dj, di) || (j < i && !jl_isless(di, dj))) ? 1 : 0;
+      }
+    }
+    perm[rank] = i;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += 1024) W[j] = d[perm[j]];
+}
+
+template <class T> __global__ void __launch_bounds__(256) hb_permute_kernel(int n, const T* Zs, const int* perm, T* V,
+                                                                           int64_t ldv) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int r = (int)(idx % n), j = (int)(idx / n);
+    V[r + (int64_t)j * ldv] = Zs[r + (int64_t)perm[j] * n];
+  }
+}
+
 size_t heev_tq_lds_bytes(int n, bool cplx) {
   const size_t es = cplx ? 16 : 8;
   const size_t ld = (size_t)(n | 1);
@@ -507,6 +726,112 @@ hipError_t launch_heev(const HeevArgs& a, bool cplx, hipStream_t s) {
   if (a.n > heev_max_n(cplx)) return hipErrorInvalidValue;
   if (cplx) return launch_heev_t<zd, 256, 2>(a, true, s);
   return launch_heev_t<double, 256, 2>(a, false, s);
+}
+
+static int hb_blocks(int64_t elems) { return (int)std::min<int64_t>(4096, (elems + 255) / 256); }
+
+size_t heev_big_ws_bytes(int n, bool cplx) {
+  const size_t es = cplx ? 16 : 8, nn = (size_t)n * n;
+  auto seg = [](size_t b) { return (b + 255) / 256 * 256; };
+  return 3 * seg(nn * es) + 4 * seg((size_t)n * es) + seg(16 * (size_t)n * es) + 4 * seg(8 * (size_t)n) +
+         seg(4 * (size_t)n);
+}
+
+template <class T> static hipError_t launch_heev_big_t(const HeevArgs& a, bool cplx, void* ws, hipStream_t s) {
+  const int n = a.n;
+  const size_t es = sizeof(T), nn = (size_t)n * n;
+  auto seg = [](size_t b) { return (b + 255) / 256 * 256; };
+  char* p = static_cast<char*>(ws);
+  HbArgs<T> h;
+  h.n = n;
+  h.M = reinterpret_cast<T*>(p);
+  p += seg(nn * es);
+  h.Q = reinterpret_cast<T*>(p);
+  p += seg(nn * es);
+  T* Zr = reinterpret_cast<T*>(p);  // Q in the reference's coordinates
+  p += seg(nn * es);
+  h.tau = reinterpret_cast<T*>(p);
+  p += seg((size_t)n * es);
+  h.v = reinterpret_cast<T*>(p);
+  p += seg((size_t)n * es);
+  h.w = reinterpret_cast<T*>(p);
+  p += seg((size_t)n * es);
+  h.u = reinterpret_cast<T*>(p);
+  p += seg((size_t)n * es);
+  h.P = reinterpret_cast<T*>(p);
+  p += seg(16 * (size_t)n * es);
+  h.d = reinterpret_cast<double*>(p);
+  p += seg(8 * (size_t)n);
+  h.e = reinterpret_cast<double*>(p);
+  p += seg(8 * (size_t)n);
+  double* dref = reinterpret_cast<double*>(p);
+  p += seg(8 * (size_t)n);
+  double* eref = reinterpret_cast<double*>(p);
+  p += seg(8 * (size_t)n);
+  int* perm = reinterpret_cast<int*>(p);
+  const bool wantz = a.Z != nullptr;
+  for (int b = 0; b < a.count; ++b) {
+    const T* A = static_cast<const T*>(a.A) + (int64_t)b * a.sA;
+    hipLaunchKernelGGL(hb_load_kernel<T>, dim3(hb_blocks((int64_t)nn)), dim3(256), 0, s, h, A, a.lda, a.upper);
+    for (int st = 0; st + 2 <= n; ++st) {
+      const int m = n - 1 - st;
+      h.p = st;
+      const int nbr = (m + 63) / 64;
+      h.ks = std::max(1, std::min(16, (512 + nbr - 1) / nbr));
+      h.kchunk = (m + h.ks - 1) / h.ks;
+      h.ks = (m + h.kchunk - 1) / h.kchunk;
+      const int nt = (m + 31) / 32;
+      hipLaunchKernelGGL(hb_reflect_kernel<T>, dim3(1), dim3(64), 0, s, h);
+      hipLaunchKernelGGL(hb_gemv_kernel<T>, dim3(nbr, h.ks), dim3(256), 0, s, h);
+      hipLaunchKernelGGL(hb_combine_kernel<T>, dim3(1), dim3(1024), 0, s, h);
+      hipLaunchKernelGGL(hb_rank2_kernel<T>, dim3(nt, nt), dim3(256), 0, s, h);
+    }
+    hipLaunchKernelGGL(hb_tail_kernel<T>, dim3(1), dim3(64), 0, s, h);
+    const bool formq = wantz;
+    if (formq) {
+      hipLaunchKernelGGL(hb_identity_kernel<T>, dim3(hb_blocks((int64_t)nn)), dim3(256), 0, s, n, h.Q);
+      for (int k = n - 2; k >= 0; --k) {
+        const int c = k + 1;
+        h.p = k;
+        if (n - 1 - c > 0) hipLaunchKernelGGL(hq_dots_kernel<T>, dim3((n - 1 - c + 3) / 4), dim3(256), 0, s, h);
+        hipLaunchKernelGGL(hq_update_kernel<T>, dim3((n - c + 63) / 64, (n - c + 3) / 4), dim3(256), 0, s, h);
+      }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (a.mode == 2) {
+      // tridiagonal form and Q only
+      T* Zo = wantz ? static_cast<T*>(a.Z) + (int64_t)b * a.sZ : nullptr;
+      hipLaunchKernelGGL(hb_out_kernel<T>, dim3(hb_blocks((int64_t)nn)), dim3(256), 0, s, h, a.upper,
+                         a.W + (int64_t)b * a.sW, a.E + (int64_t)b * a.sE, Zo, a.ldz);
+      continue;
+    }
+    hipLaunchKernelGGL(hb_out_kernel<T>, dim3(hb_blocks((int64_t)nn)), dim3(256), 0, s, h, a.upper, dref, eref,
+                       wantz ? Zr : nullptr, (int64_t)n);
+    SteqrArgs q;
+    q.n = n;
+    q.zrows = wantz ? n : 0;
+    q.maxsweeps = a.maxsweeps;
+    q.d = dref;
+    q.e = eref;
+    q.Z = wantz ? static_cast<void*>(Zr) : nullptr;
+    q.ldz = n;
+    q.info = a.info ? a.info + b : nullptr;
+    q.sweeps = a.sweeps ? a.sweeps + b : nullptr;
+    if ((e = launch_steqr(q, cplx, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(hb_sort_kernel, dim3(1), dim3(1024), 0, s, n, a.sort, dref, perm, a.W + (int64_t)b * a.sW);
+    if (wantz)
+      hipLaunchKernelGGL(hb_permute_kernel<T>, dim3(hb_blocks((int64_t)nn)), dim3(256), 0, s, n, Zr, perm,
+                         static_cast<T*>(a.Z) + (int64_t)b * a.sZ, a.ldz);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_heev_big(const HeevArgs& a, bool cplx, void* ws, hipStream_t s) {
+  if (a.count <= 0 || a.n <= 0) return hipSuccess;
+  if (a.n > heev_big_max_n() || !ws) return hipErrorInvalidValue;
+  return cplx ? launch_heev_big_t<zd>(a, true, ws, s) : launch_heev_big_t<double>(a, false, ws, s);
 }
 
 hipError_t launch_steqr(const SteqrArgs& a, bool zcplx, hipStream_t s) {

@@ -34,8 +34,9 @@ int heev_impl(gs_handle* h, bool cplx, int64_t count, int64_t n, const double* A
   if (lda < n || (count > 1 && sA < lda * n) || (wantv && (ldv < n || (count > 1 && sV < ldv * n))))
     return handle_fail(h, GS_E_DIM, "bad leading dimension or stride");
   if (!A || !W || (wantv && !V)) return handle_fail(h, GS_E_ARG, "null pointer");
-  if (n > heev_max_n(cplx))
-    return handle_fail(h, GS_E_UNSUPPORTED, "order above gs_heev_max_n (the matrix must fit in LDS)");
+  if (n > heev_big_max_n()) return handle_fail(h, GS_E_UNSUPPORTED, "order above gs_heev_max_n");
+  // above the LDS-resident orders: one matrix at a time over the whole GPU
+  const bool big = n > heev_max_n(cplx);
   std::lock_guard<std::mutex> lk(handle_mutex(h));
   HCHK(hipSetDevice(handle_device(h)));
   hipStream_t s = handle_stream(h);
@@ -49,8 +50,14 @@ int heev_impl(gs_handle* h, bool cplx, int64_t count, int64_t n, const double* A
   a.sort = (flags & GS_SORT) ? 1 : 0;
   a.maxsweeps = 30 * (int)n;
   a.sW = n;
-  // workspace: e (count x n), stream lengths, states, streams (with vectors)
-  {
+  // workspace: e (count x n), stream lengths, states, streams (with
+  // vectors); for the large orders the working matrices of one matrix
+  void* bigws = nullptr;
+  if (big) {
+    DevBuf& ws = handle_buf(h, 13);
+    HCHK(ws.ensure(heev_big_ws_bytes((int)n, cplx)));
+    bigws = ws.p;
+  } else {
     const size_t bE = up256(cnt * (size_t)n * 8), bP = up256(cnt * 4), bS = up256(cnt * heev_state_bytes());
     size_t cap = heev_log_records((int)n);
     // testing: a shorter stream forces the in-place finish of phase 3
@@ -101,7 +108,7 @@ int heev_impl(gs_handle* h, bool cplx, int64_t count, int64_t n, const double* A
     a.ldz = n;
     a.sZ = (int64_t)nn;
     a.info = dI;
-    HCHK(launch_heev(a, cplx, s));
+    HCHK(big ? launch_heev_big(a, cplx, bigws, s) : launch_heev(a, cplx, s));
     HCHK(hipMemcpyAsync(W, dW, cnt * n * 8, hipMemcpyDeviceToHost, s));
     if (wantv) {
       if (ldv == n && (count == 1 || sV == (int64_t)nn)) {
@@ -116,7 +123,7 @@ int heev_impl(gs_handle* h, bool cplx, int64_t count, int64_t n, const double* A
     HCHK(hipStreamSynchronize(s));
     return GS_OK;
   }
-  HCHK(launch_heev(a, cplx, s));
+  HCHK(big ? launch_heev_big(a, cplx, bigws, s) : launch_heev(a, cplx, s));
   if (!(flags & GS_ASYNC)) HCHK(hipStreamSynchronize(s));
   return GS_OK;
 }
@@ -128,13 +135,19 @@ int hetrd_impl(gs_handle* h, bool cplx, int64_t n, const double* A, int64_t lda,
     return handle_fail(h, GS_E_DIM, "bad dimension");
   if (n == 0) return GS_OK;
   if (!A || !d || (n > 1 && !e)) return handle_fail(h, GS_E_ARG, "null pointer");
-  if (n > heev_max_n(cplx))
-    return handle_fail(h, GS_E_UNSUPPORTED, "order above gs_heev_max_n (the matrix must fit in LDS)");
+  if (n > heev_big_max_n()) return handle_fail(h, GS_E_UNSUPPORTED, "order above gs_heev_max_n");
+  const bool big = n > heev_max_n(cplx);
   std::lock_guard<std::mutex> lk(handle_mutex(h));
   HCHK(hipSetDevice(handle_device(h)));
   hipStream_t s = handle_stream(h);
   const bool dev = (flags & GS_DEVICE_PTRS) != 0;
   const size_t es = cplx ? 16 : 8, nn = (size_t)n * n;
+  void* bigws = nullptr;
+  if (big) {
+    DevBuf& ws = handle_buf(h, 13);
+    HCHK(ws.ensure(heev_big_ws_bytes((int)n, cplx)));
+    bigws = ws.p;
+  }
   HeevArgs a;
   a.count = 1;
   a.n = (int)n;
@@ -147,7 +160,7 @@ int hetrd_impl(gs_handle* h, bool cplx, int64_t n, const double* A, int64_t lda,
     a.E = e;
     a.Z = Q;
     a.ldz = ldq;
-    HCHK(launch_heev(a, cplx, s));
+    HCHK(big ? launch_heev_big(a, cplx, bigws, s) : launch_heev(a, cplx, s));
     if (!(flags & GS_ASYNC)) HCHK(hipStreamSynchronize(s));
     return GS_OK;
   }
@@ -166,7 +179,7 @@ int hetrd_impl(gs_handle* h, bool cplx, int64_t n, const double* A, int64_t lda,
   a.E = de;
   a.Z = dQ;
   a.ldz = n;
-  HCHK(launch_heev(a, cplx, s));
+  HCHK(big ? launch_heev_big(a, cplx, bigws, s) : launch_heev(a, cplx, s));
   HCHK(hipMemcpyAsync(d, dd, (size_t)n * 8, hipMemcpyDeviceToHost, s));
   if (n > 1) HCHK(hipMemcpyAsync(e, de, (size_t)(n - 1) * 8, hipMemcpyDeviceToHost, s));
   if (Q) HCHK(copy_cols(Q, ldq * es, dQ, n * es, n * es, n, hipMemcpyDeviceToHost, s));
@@ -188,7 +201,10 @@ int gs_heev_d(gs_handle* h, int64_t count, int64_t n, const double* A, int64_t l
   return heev_impl(h, false, count, n, A, lda, strideA, upper, W, V, ldv, strideV, info, flags);
 }
 
-int gs_heev_max_n(int is_complex) { return heev_max_n(is_complex != 0); }
+int gs_heev_max_n(int is_complex) {
+  (void)is_complex;
+  return heev_big_max_n();
+}
 
 int gs_hetrd_z(gs_handle* h, int64_t n, const double* A, int64_t lda, int upper, double* d, double* e, double* Q,
                int64_t ldq, int flags) {

@@ -222,6 +222,13 @@ size_t steqr_lds_bytes(int n);
 size_t heev_log_records(int n);   // records per matrix of the mode 0 stream
 size_t heev_state_bytes();        // bytes per matrix of the QL/QR state
 hipError_t launch_heev(const HeevArgs& a, bool cplx, hipStream_t s);
+// Orders heev_max_n < n <= heev_big_max_n, one matrix at a time over the
+// whole GPU (the working matrix, Q and the eigenvectors in HBM); modes 0
+// and 2 as launch_heev (E is output only in mode 2).  ws holds
+// heev_big_ws_bytes(n, cplx) bytes of device memory.
+constexpr int heev_big_max_n() { return steqr_max_n(); }
+size_t heev_big_ws_bytes(int n, bool cplx);
+hipError_t launch_heev_big(const HeevArgs& a, bool cplx, void* ws, hipStream_t s);
 hipError_t launch_steqr(const SteqrArgs& a, bool zcplx, hipStream_t s);
 
 hipError_t launch_ordschur_d(int n, double* T, int64_t ldt, double* Z, int64_t ldz, bool wantz, const int32_t* select,

@@ -27,6 +27,11 @@ EPS = np.finfo(np.float64).eps
 TINY = np.finfo(np.float64).tiny
 
 
+# largest orders of the LDS-resident kernels (one workgroup per matrix);
+# above, up to gs.heev_max_n(), one matrix at a time over the whole GPU
+LDS_MAX = {True: 96, False: 128}
+
+
 def _herm(r, n, cplx, scale=1.0):
     A = r.standard_normal((n, n)) + (1j * r.standard_normal((n, n)) if cplx else 0)
     return (A + A.conj().T) * scale
@@ -79,7 +84,7 @@ def test_steqr_reference_classes_bitwise(gs, oracle):
 @pytest.mark.parametrize("uplo", ["L", "U"])
 def test_hetrd_parity(gs, oracle, cplx, uplo):
     r = np.random.default_rng(62)
-    nmax = gs.heev_max_n(cplx)
+    nmax = LDS_MAX[cplx]
     for n in (1, 2, 3, 9, 40, 64, nmax):
         A = _herm(r, n, cplx)
         print(f"hetrd n={n} cplx={cplx} uplo={uplo}", flush=True)
@@ -134,7 +139,7 @@ def _check_eig(A, w, V, wo, Vo):
 @pytest.mark.parametrize("uplo", ["L", "U"])
 def test_heev_parity(gs, oracle, cplx, uplo):
     r = np.random.default_rng(63)
-    nmax = gs.heev_max_n(cplx)
+    nmax = LDS_MAX[cplx]
     for n in (1, 2, 3, 10, 33, 64, nmax):
         A = _herm(r, n, cplx)
         print(f"heev n={n} cplx={cplx} uplo={uplo}", flush=True)
@@ -186,7 +191,8 @@ def test_heev_edge_cases(gs, oracle, cplx):
     assert np.array_equal(gs.geigvals_hermitian(B, "L"), gs.geigvals_hermitian(A, "L"))
     C = np.triu(A) + np.tril(r.standard_normal((12, 12)), -1)
     assert np.array_equal(gs.geigvals_hermitian(C, "U"), gs.geigvals_hermitian(A, "U"))
-    # above the LDS order: a clear error, no fallback
+    # above the largest order: a clear error, no fallback
+    assert gs.heev_max_n(cplx) == 2048
     with pytest.raises(gs.GSError):
         gs.geigen_hermitian(np.eye(gs.heev_max_n(cplx) + 1, dtype=dt))
 
@@ -215,3 +221,55 @@ def test_heev_phases_bitwise(gs, cplx, monkeypatch):
         W2, V2, info2 = gs.heev_batched(A)
         assert not info2.any()
         assert np.array_equal(W2, W) and np.array_equal(V2, V), cap
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_heev_large_orders(gs, oracle, cplx, uplo):
+    """Orders above the LDS-resident kernels: the working matrix in HBM,
+    each reflector step and each Q accumulation step as grid-wide launches,
+    the QL/QR iteration on Q in HBM.  Against the oracle's heev as for the
+    small orders (values, residual, unitarity, aligned columns), hetrd's
+    factorization bounds, and values-only calls."""
+    r = np.random.default_rng(70)
+    for n in (LDS_MAX[cplx] + 1, 300):
+        A = _herm(r, n, cplx)
+        print(f"heev large n={n} cplx={cplx} uplo={uplo}", flush=True)
+        E = gs.geigen_hermitian(A, uplo)
+        wo, Vo, info = oracle.heev(A, uplo)
+        assert info == 0
+        _check_eig(A, E.values, E.vectors, wo, Vo)
+        w = gs.geigvals_hermitian(A, uplo)
+        assert np.abs(w - E.values).max() <= 16 * n * EPS * np.abs(A).max()
+        d, e, Q = gs.hessenberg_hermitian(A, uplo)
+        T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+        nrm = np.linalg.norm(A)
+        assert np.linalg.norm(A - Q @ T @ Q.conj().T) / (n * nrm * EPS) < 10
+        assert np.linalg.norm(np.eye(n) - Q @ Q.conj().T) / (n * EPS) < 10
+        do, eo, _ = oracle.hetrd(A, uplo)
+        lam = np.linalg.eigvalsh(T)
+        lamo = np.linalg.eigvalsh(np.diag(do) + np.diag(eo, 1) + np.diag(eo, -1))
+        assert np.abs(lam - lamo).max() <= 16 * n * EPS * np.abs(A).max()
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_heev_large_batched_and_structured(gs, oracle, cplx):
+    """A batch of two large matrices (one after the other on the GPU), and
+    structured large inputs: diagonal (every reflector trivial: tau = 0)
+    and a matrix with an exactly zero column below the diagonal."""
+    r = np.random.default_rng(71)
+    n = LDS_MAX[cplx] + 40
+    As = np.stack([_herm(r, n, cplx) for _ in range(2)])
+    Eb = gs.heev_batched(As, wantV=True)
+    for k in range(2):
+        wo, Vo, info = oracle.heev(As[k], "L")
+        _check_eig(As[k], Eb[0][k], Eb[1][k], wo, Vo)
+    D = np.diag(r.standard_normal(n)).astype(As.dtype)
+    E = gs.geigen_hermitian(D)
+    assert np.array_equal(E.values, np.sort(np.diag(D).real))
+    B = _herm(r, n, cplx)
+    B[5:, 3] = 0
+    B[3, 5:] = 0
+    wo, Vo, info = oracle.heev(B, "L")
+    E = gs.geigen_hermitian(B)
+    _check_eig(B, E.values, E.vectors, wo, Vo)
