@@ -660,6 +660,67 @@ template <class T> __global__ void __launch_bounds__(256) hb_out_kernel(HbArgs<T
   }
 }
 
+// The QL/QR recurrence of one large matrix by one thread (d, e staged in
+// LDS), its rotations to a record stream in HBM; a full stream stops it at
+// a step boundary with its state left for hb_finish_kernel.
+__global__ void __launch_bounds__(64) hb_produce_kernel(int n, int maxsweeps, double* d, double* e, double2* rec,
+                                                       int cap, int* npos, StState* state) {
+  extern __shared__ double smem[];
+  double* dl = smem;
+  double* el = smem + n;
+  for (int i = threadIdx.x; i < n; i += 64) dl[i] = d[i];
+  for (int i = threadIdx.x; i < n - 1; i += 64) el[i] = e[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    StState S;
+    st_init(S, n, maxsweeps);
+    StLogG L;
+    L.rec = rec;
+    L.pos = 0;
+    L.cap = cap;
+    st_produce(S, StVec{dl, 1}, StVec{el, 1}, &L);
+    *npos = L.pos;
+    *state = S;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 64) d[i] = dl[i];
+  for (int i = threadIdx.x; i < n - 1; i += 64) e[i] = el[i];
+}
+
+// the record stream replayed onto Z's rows (ld n), one row per thread
+template <class T> __global__ void __launch_bounds__(64) hb_replay_kernel(int n, const double2* rec, const int* npos,
+                                                                         T* Z) {
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  if (r >= n) return;
+  st_replay_row<T>(rec, *npos, [&](int k) -> T& { return Z[r + (int64_t)(k - 1) * n]; });
+}
+
+// finishes a recurrence whose stream overflowed (producer thread and row
+// appliers of one workgroup, as steqr_kernel); info and sweeps
+template <class T, int NT> __global__ void __launch_bounds__(NT) hb_finish_kernel(int n, double* d, double* e, T* Z,
+                                                                                 const StState* state, int* info,
+                                                                                 int* sweeps) {
+  extern __shared__ double smem[];
+  double* dl = smem;
+  double* el = dl + n;
+  int* ctrl = reinterpret_cast<int*>(el + n);
+  void* logb = static_cast<void*>(ctrl + 16);
+  StState S = *state;
+  if (!S.done) {
+    for (int i = threadIdx.x; i < n; i += NT) dl[i] = d[i];
+    for (int i = threadIdx.x; i < n - 1; i += NT) el[i] = e[i];
+    __syncthreads();
+    const StMem mem = st_mem(logb, n, ctrl);
+    st_solve<NT, T>(S, dl, el, true, n, [&](int r, int k) -> T& { return Z[r + (int64_t)(k - 1) * n]; }, mem);
+    for (int i = threadIdx.x; i < n; i += NT) d[i] = dl[i];
+    for (int i = threadIdx.x; i < n - 1; i += NT) e[i] = el[i];
+  }
+  if (threadIdx.x == 0) {
+    if (info) *info = S.info;
+    if (sweeps) *sweeps = S.sweeps;
+  }
+}
+
 // sorteig! (stable, isless) of d into W; perm[j] = source column
 __global__ void __launch_bounds__(1024) hb_sort_kernel(int n, int sort, const double* d, int* perm, double* W) {
   for (int i = threadIdx.x; i < n; i += 1024) {
@@ -734,7 +795,7 @@ size_t heev_big_ws_bytes(int n, bool cplx) {
   const size_t es = cplx ? 16 : 8, nn = (size_t)n * n;
   auto seg = [](size_t b) { return (b + 255) / 256 * 256; };
   return 3 * seg(nn * es) + 4 * seg((size_t)n * es) + seg(16 * (size_t)n * es) + 4 * seg(8 * (size_t)n) +
-         seg(4 * (size_t)n);
+         seg(4 * (size_t)n) + seg(16 * heev_log_records(n)) + seg(4) + seg(sizeof(StState));
 }
 
 template <class T> static hipError_t launch_heev_big_t(const HeevArgs& a, bool cplx, void* ws, hipStream_t s) {
@@ -769,6 +830,13 @@ template <class T> static hipError_t launch_heev_big_t(const HeevArgs& a, bool c
   double* eref = reinterpret_cast<double*>(p);
   p += seg(8 * (size_t)n);
   int* perm = reinterpret_cast<int*>(p);
+  p += seg(4 * (size_t)n);
+  double2* rec = reinterpret_cast<double2*>(p);
+  const int cap = (a.logcap > 0 && (size_t)a.logcap <= heev_log_records(n)) ? a.logcap : (int)heev_log_records(n);
+  p += seg(16 * heev_log_records(n));
+  int* npos = reinterpret_cast<int*>(p);
+  p += seg(4);
+  StState* state = reinterpret_cast<StState*>(p);
   const bool wantz = a.Z != nullptr;
   for (int b = 0; b < a.count; ++b) {
     const T* A = static_cast<const T*>(a.A) + (int64_t)b * a.sA;
@@ -808,17 +876,30 @@ template <class T> static hipError_t launch_heev_big_t(const HeevArgs& a, bool c
     }
     hipLaunchKernelGGL(hb_out_kernel<T>, dim3(hb_blocks((int64_t)nn)), dim3(256), 0, s, h, a.upper, dref, eref,
                        wantz ? Zr : nullptr, (int64_t)n);
-    SteqrArgs q;
-    q.n = n;
-    q.zrows = wantz ? n : 0;
-    q.maxsweeps = a.maxsweeps;
-    q.d = dref;
-    q.e = eref;
-    q.Z = wantz ? static_cast<void*>(Zr) : nullptr;
-    q.ldz = n;
-    q.info = a.info ? a.info + b : nullptr;
-    q.sweeps = a.sweeps ? a.sweeps + b : nullptr;
-    if ((e = launch_steqr(q, cplx, s)) != hipSuccess) return e;
+    int* infob = a.info ? a.info + b : nullptr;
+    int* sweepsb = a.sweeps ? a.sweeps + b : nullptr;
+    if (wantz) {
+      // the recurrence by one thread writing its rotations to HBM, their
+      // replay onto Q's rows over the whole GPU, an in-place finish when the
+      // stream overflowed: the element operations of steqr_kernel
+      hipLaunchKernelGGL(hb_produce_kernel, dim3(1), dim3(64), 16 * (size_t)n, s, n, a.maxsweeps, dref, eref, rec,
+                         cap, npos, state);
+      hipLaunchKernelGGL(hb_replay_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, s, n, rec, npos, Zr);
+      const size_t lf = steqr_lds_bytes(n);
+      (void)hipFuncSetAttribute((const void*)hb_finish_kernel<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lf);
+      hipLaunchKernelGGL((hb_finish_kernel<T, 256>), dim3(1), dim3(256), lf, s, n, dref, eref, Zr, state, infob,
+                         sweepsb);
+    } else {
+      SteqrArgs q;
+      q.n = n;
+      q.maxsweeps = a.maxsweeps;
+      q.d = dref;
+      q.e = eref;
+      q.info = infob;
+      q.sweeps = sweepsb;
+      if ((e = launch_steqr(q, cplx, s)) != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(hb_sort_kernel, dim3(1), dim3(1024), 0, s, n, a.sort, dref, perm, a.W + (int64_t)b * a.sW);
     if (wantz)
       hipLaunchKernelGGL(hb_permute_kernel<T>, dim3(hb_blocks((int64_t)nn)), dim3(256), 0, s, n, Zr, perm,

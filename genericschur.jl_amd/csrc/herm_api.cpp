@@ -57,6 +57,8 @@ int heev_impl(gs_handle* h, bool cplx, int64_t count, int64_t n, const double* A
     DevBuf& ws = handle_buf(h, 13);
     HCHK(ws.ensure(heev_big_ws_bytes((int)n, cplx)));
     bigws = ws.p;
+    // testing: a shorter stream forces the in-place finish
+    if (const char* ev = std::getenv("GSCHUR_HEEV_LOGCAP")) a.logcap = (int)std::strtoul(ev, nullptr, 10);
   } else {
     const size_t bE = up256(cnt * (size_t)n * 8), bP = up256(cnt * 4), bS = up256(cnt * heev_state_bytes());
     size_t cap = heev_log_records((int)n);

@@ -273,3 +273,29 @@ def test_heev_large_batched_and_structured(gs, oracle, cplx):
     wo, Vo, info = oracle.heev(B, "L")
     E = gs.geigen_hermitian(B)
     _check_eig(B, E.values, E.vectors, wo, Vo)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_heev_large_phases_bitwise(gs, cplx, monkeypatch):
+    """Above the LDS orders the eigenvectors come from a record stream
+    written by one producer thread and replayed onto Q's rows over the whole
+    GPU; against the same steps one call each (gs_hetrd, gs_steqr's
+    one-workgroup iteration on (d, e, Q), a stable sort): bit for bit, also
+    with streams too short for the iteration (the in-place finish)."""
+    r = np.random.default_rng(72)
+    n = LDS_MAX[cplx] + 70
+    A = np.stack([_herm(r, n, cplx) for _ in range(2)])
+    W, V, info = gs.heev_batched(A)
+    assert not info.any()
+    for k in range(2):
+        d, e, Q = gs.hessenberg_hermitian(A[k])
+        w, Z, _, inf1 = gs.steqr_(d, e, Q)
+        assert inf1 == 0
+        p = np.argsort(w, kind="stable")
+        assert np.array_equal(W[k], w[p]), k
+        assert np.array_equal(V[k], Z[:, p]), k
+    for cap in ("1", "5000"):
+        monkeypatch.setenv("GSCHUR_HEEV_LOGCAP", cap)
+        W2, V2, info2 = gs.heev_batched(A)
+        assert not info2.any()
+        assert np.array_equal(W2, W) and np.array_equal(V2, V), cap
