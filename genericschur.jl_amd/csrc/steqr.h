@@ -344,13 +344,30 @@ template <class Log> GS_DEV void st_step(StState& S, StVec d, StVec e, Log* lg) 
   const bool wantz = lg != nullptr;
   // search for a small off-diagonal entry: pairs (m, m + dir), e index
   // m (QL) or m - 1 (QR)
-  int msplit = lend;
-  for (int m = l; m != lend; m += dir) {
+  // (four tests at a time with their loads issued together; the first
+  // pair that passes, as one test after another)
+  auto small = [&](int m) {
     const int em = dir > 0 ? m : m - 1;
-    if (E(em) * E(em) <= (eps2 * fabs(D(m))) * fabs(D(m + dir)) + safmin) {
-      msplit = m;
-      break;
+    return E(em) * E(em) <= (eps2 * fabs(D(m))) * fabs(D(m + dir)) + safmin;
+  };
+  int msplit = lend;
+  {
+    const int len = dir > 0 ? lend - l : l - lend;
+    int t = 0;
+    for (; t + 4 <= len; t += 4) {
+      const int m0 = l + dir * t;
+      const bool h0 = small(m0), h1 = small(m0 + dir), h2 = small(m0 + 2 * dir), h3 = small(m0 + 3 * dir);
+      if (h0 || h1 || h2 || h3) {
+        msplit = h0 ? m0 : (h1 ? m0 + dir : (h2 ? m0 + 2 * dir : m0 + 3 * dir));
+        t = len + 1;
+        break;
+      }
     }
+    for (; t < len; ++t)
+      if (small(l + dir * t)) {
+        msplit = l + dir * t;
+        break;
+      }
   }
   if (msplit != lend) E(dir > 0 ? msplit : msplit - 1) = 0.0;
   double p = D(l);
@@ -387,15 +404,29 @@ template <class Log> GS_DEV void st_step(StState& S, StVec d, StVec e, Log* lg) 
     const int j0 = dir > 0 ? msplit - 1 : msplit;
     const int cnt = dir > 0 ? msplit - l : l - msplit;
     const int r0 = wantz ? lg->begin() : 0;
+    // the entries of rotation k + 1 (e[jj - dir], d[up - dir], d[far - dir])
+    // are loaded during rotation k, which writes only e[jj + dir], d[up]
+    double ej = 0.0, dup = 0.0, dfar = 0.0;
+    if (cnt > 0) {
+      ej = E(j0);
+      dup = D(dir > 0 ? j0 + 1 : j0);
+      dfar = D(dir > 0 ? j0 : j0 + 1);
+    }
     for (int k = 0; k < cnt; ++k) {
       const int jj = j0 - dir * k;
       const int up = dir > 0 ? jj + 1 : jj, far = dir > 0 ? jj : jj + 1;
-      const double f = s * E(jj);
-      const double b = c * E(jj);
+      const double ek = ej, dupk = dup, dfark = dfar;
+      if (k + 1 < cnt) {
+        ej = E(jj - dir);
+        dup = D(up - dir);
+        dfar = D(far - dir);
+      }
+      const double f = s * ek;
+      const double b = c * ek;
       givens_re(g, f, c, s, r);
       if (k > 0) E(jj + dir) = r;
-      g = D(up) - p;
-      r = (D(far) - g) * s + 2.0 * c * b;
+      g = dupk - p;
+      r = (dfark - g) * s + 2.0 * c * b;
       p = s * r;
       D(up) = g + p;
       g = c * r - b;
