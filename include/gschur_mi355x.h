@@ -293,9 +293,12 @@ int gs_eigcond_z(gs_handle* h, int64_t n, const double* T, int64_t ldt, int64_t 
  * GS_WANTZ, V (ldv, strideV) the eigenvectors (complex for _z).  info[k]
  * (may be NULL): 0, or the number of unconverged off-diagonal entries when
  * the iteration stopped after 30 n sweeps (the reference's loop has no
- * limit).  One workgroup per matrix, the matrix in LDS: n <= gs_heev_max_n
- * (GS_E_UNSUPPORTED above).  Host pointers are staged through device
- * buffers; with GS_DEVICE_PTRS, A, W, V and info are device memory. */
+ * limit).  n <= gs_heev_max_n() = 2048 (GS_E_UNSUPPORTED above): one
+ * workgroup per matrix with the matrix in LDS up to 96 (complex) / 128
+ * (real); above, one matrix at a time over the whole GPU with the working
+ * matrices in device memory held by the handle.  Host pointers are staged
+ * through device buffers; with GS_DEVICE_PTRS, A, W, V and info are device
+ * memory. */
 int gs_heev_z(gs_handle* h, int64_t count, int64_t n, const double* A, int64_t lda, int64_t strideA, int upper,
               double* W, double* V, int64_t ldv, int64_t strideV, int32_t* info, int flags);
 int gs_heev_d(gs_handle* h, int64_t count, int64_t n, const double* A, int64_t lda, int64_t strideA, int upper,
@@ -305,7 +308,7 @@ int gs_heev_max_n(int is_complex);
 /* _hehessenberg!(Hermitian(A, uplo)) + _materializeQ(H) — hessenberg.jl:
  * 19-95, 133-232: the real symmetric tridiagonal matrix (d: n, e: n - 1) and
  * Q (n x n, ldq; NULL: not formed) with A = Q T Q^H, from the lower (upper =
- * 0) or upper triangle of A (not modified).  n <= gs_heev_max_n. */
+ * 0) or upper triangle of A (not modified).  n <= gs_heev_max_n(). */
 int gs_hetrd_z(gs_handle* h, int64_t n, const double* A, int64_t lda, int upper, double* d, double* e, double* Q,
                int64_t ldq, int flags);
 int gs_hetrd_d(gs_handle* h, int64_t n, const double* A, int64_t lda, int upper, double* d, double* e, double* Q,
