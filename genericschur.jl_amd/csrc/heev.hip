@@ -661,10 +661,11 @@ template <class T> __global__ void __launch_bounds__(256) hb_out_kernel(HbArgs<T
 }
 
 // The QL/QR recurrence of one large matrix by one thread (d, e staged in
-// LDS), its rotations to a record stream in HBM; a full stream stops it at
-// a step boundary with its state left for hb_finish_kernel.
+// LDS), from the start or (resume) from the state record; with rec its
+// rotations go to a record stream in HBM, and a full stream stops it at a
+// step boundary with its state left for the next round.
 __global__ void __launch_bounds__(64) hb_produce_kernel(int n, int maxsweeps, double* d, double* e, double2* rec,
-                                                       int cap, int* npos, StState* state) {
+                                                       int cap, int* npos, StState* state, int resume) {
   extern __shared__ double smem[];
   double* dl = smem;
   double* el = smem + n;
@@ -673,13 +674,19 @@ __global__ void __launch_bounds__(64) hb_produce_kernel(int n, int maxsweeps, do
   __syncthreads();
   if (threadIdx.x == 0) {
     StState S;
-    st_init(S, n, maxsweeps);
-    StLogG L;
-    L.rec = rec;
-    L.pos = 0;
-    L.cap = cap;
-    st_produce(S, StVec{dl, 1}, StVec{el, 1}, &L);
-    *npos = L.pos;
+    if (resume) S = *state;
+    else st_init(S, n, maxsweeps);
+    if (rec) {
+      StLogG L;
+      L.rec = rec;
+      L.pos = 0;
+      L.cap = cap;
+      st_produce(S, StVec{dl, 1}, StVec{el, 1}, &L);
+      *npos = L.pos;
+    } else {
+      st_produce<StLogG>(S, StVec{dl, 1}, StVec{el, 1}, nullptr);
+      *npos = 0;
+    }
     *state = S;
   }
   __syncthreads();
@@ -687,37 +694,19 @@ __global__ void __launch_bounds__(64) hb_produce_kernel(int n, int maxsweeps, do
   for (int i = threadIdx.x; i < n - 1; i += 64) e[i] = el[i];
 }
 
-// the record stream replayed onto Z's rows (ld n), one row per thread
-template <class T> __global__ void __launch_bounds__(64) hb_replay_kernel(int n, const double2* rec, const int* npos,
-                                                                         T* Z) {
+// the record stream replayed onto rows 0 .. nrows-1 of Z (ld ldz), one row
+// per thread
+template <class T> __global__ void __launch_bounds__(64) hb_replay_kernel(int nrows, int64_t ldz, const double2* rec,
+                                                                         const int* npos, T* Z) {
   const int r = blockIdx.x * 64 + threadIdx.x;
-  if (r >= n) return;
-  st_replay_row<T>(rec, *npos, [&](int k) -> T& { return Z[r + (int64_t)(k - 1) * n]; });
+  if (r >= nrows) return;
+  st_replay_row<T>(rec, *npos, [&](int k) -> T& { return Z[r + (int64_t)(k - 1) * ldz]; });
 }
 
-// finishes a recurrence whose stream overflowed (producer thread and row
-// appliers of one workgroup, as steqr_kernel); info and sweeps
-template <class T, int NT> __global__ void __launch_bounds__(NT) hb_finish_kernel(int n, double* d, double* e, T* Z,
-                                                                                 const StState* state, int* info,
-                                                                                 int* sweeps) {
-  extern __shared__ double smem[];
-  double* dl = smem;
-  double* el = dl + n;
-  int* ctrl = reinterpret_cast<int*>(el + n);
-  void* logb = static_cast<void*>(ctrl + 16);
-  StState S = *state;
-  if (!S.done) {
-    for (int i = threadIdx.x; i < n; i += NT) dl[i] = d[i];
-    for (int i = threadIdx.x; i < n - 1; i += NT) el[i] = e[i];
-    __syncthreads();
-    const StMem mem = st_mem(logb, n, ctrl);
-    st_solve<NT, T>(S, dl, el, true, n, [&](int r, int k) -> T& { return Z[r + (int64_t)(k - 1) * n]; }, mem);
-    for (int i = threadIdx.x; i < n; i += NT) d[i] = dl[i];
-    for (int i = threadIdx.x; i < n - 1; i += NT) e[i] = el[i];
-  }
+__global__ void __launch_bounds__(64) hb_info_kernel(const StState* state, int* info, int* sweeps) {
   if (threadIdx.x == 0) {
-    if (info) *info = S.info;
-    if (sweeps) *sweeps = S.sweeps;
+    if (info) *info = state->info;
+    if (sweeps) *sweeps = state->sweeps;
   }
 }
 
@@ -796,6 +785,53 @@ size_t heev_big_ws_bytes(int n, bool cplx) {
   auto seg = [](size_t b) { return (b + 255) / 256 * 256; };
   return 3 * seg(nn * es) + 4 * seg((size_t)n * es) + seg(16 * (size_t)n * es) + 4 * seg(8 * (size_t)n) +
          seg(4 * (size_t)n) + seg(16 * heev_log_records(n)) + seg(4) + seg(sizeof(StState));
+}
+
+// The QL/QR iteration of one matrix (d, e in device memory, n <= 4096)
+// with its rotations replayed onto rows 0 .. zrows-1 of Z (ld ldz; null:
+// values only) in rounds of at most cap records: the producer thread, then
+// the replay over the whole GPU, until the recurrence is done (the host
+// reads the state record after each round; one round unless a stream
+// fills).  The element operations are those of steqr_kernel.
+template <class T> static hipError_t st_rounds(int n, int maxsweeps, double* d, double* e, T* Z, int zrows,
+                                               int64_t ldz, double2* rec, int cap, int* npos, StState* state,
+                                               int* info, int* sweeps, hipStream_t s) {
+  const bool wantz = Z != nullptr && zrows > 0;
+  cap = std::max(cap, n + 2);  // room for any one step: every round progresses
+  for (int round = 0;; ++round) {
+    hipLaunchKernelGGL(hb_produce_kernel, dim3(1), dim3(64), 16 * (size_t)n, s, n, maxsweeps, d, e,
+                       wantz ? rec : nullptr, cap, npos, state, round > 0 ? 1 : 0);
+    if (!wantz) break;
+    hipLaunchKernelGGL(hb_replay_kernel<T>, dim3((zrows + 63) / 64), dim3(64), 0, s, zrows, ldz, rec, npos, Z);
+    StState hs;
+    hipError_t e2 = hipMemcpyAsync(&hs, state, sizeof hs, hipMemcpyDeviceToHost, s);
+    if (e2 == hipSuccess) e2 = hipStreamSynchronize(s);
+    if (e2 != hipSuccess) return e2;
+    if (hs.done) break;
+  }
+  hipLaunchKernelGGL(hb_info_kernel, dim3(1), dim3(64), 0, s, state, info, sweeps);
+  return hipGetLastError();
+}
+
+size_t steqr_big_ws_bytes(int n) {
+  return (16 * heev_log_records(n) + 255) / 256 * 256 + 256 + (sizeof(StState) + 255) / 256 * 256;
+}
+
+// gs_steqr above the LDS log's range: the rounds of st_rounds on a.Z
+hipError_t launch_steqr_big(const SteqrArgs& a, bool zcplx, void* ws, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  if (a.n > heev_big_max_n() || !ws) return hipErrorInvalidValue;
+  char* p = static_cast<char*>(ws);
+  double2* rec = reinterpret_cast<double2*>(p);
+  p += (16 * heev_log_records(a.n) + 255) / 256 * 256;
+  int* npos = reinterpret_cast<int*>(p);
+  StState* state = reinterpret_cast<StState*>(p + 256);
+  const int cap = (int)heev_log_records(a.n);
+  if (zcplx)
+    return st_rounds<zd>(a.n, a.maxsweeps, a.d, a.e, static_cast<zd*>(a.Z), a.zrows, a.ldz, rec, cap, npos, state,
+                         a.info, a.sweeps, s);
+  return st_rounds<double>(a.n, a.maxsweeps, a.d, a.e, static_cast<double*>(a.Z), a.zrows, a.ldz, rec, cap, npos,
+                           state, a.info, a.sweeps, s);
 }
 
 template <class T> static hipError_t launch_heev_big_t(const HeevArgs& a, bool cplx, void* ws, hipStream_t s) {
@@ -878,28 +914,11 @@ template <class T> static hipError_t launch_heev_big_t(const HeevArgs& a, bool c
                        wantz ? Zr : nullptr, (int64_t)n);
     int* infob = a.info ? a.info + b : nullptr;
     int* sweepsb = a.sweeps ? a.sweeps + b : nullptr;
-    if (wantz) {
-      // the recurrence by one thread writing its rotations to HBM, their
-      // replay onto Q's rows over the whole GPU, an in-place finish when the
-      // stream overflowed: the element operations of steqr_kernel
-      hipLaunchKernelGGL(hb_produce_kernel, dim3(1), dim3(64), 16 * (size_t)n, s, n, a.maxsweeps, dref, eref, rec,
-                         cap, npos, state);
-      hipLaunchKernelGGL(hb_replay_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, s, n, rec, npos, Zr);
-      const size_t lf = steqr_lds_bytes(n);
-      (void)hipFuncSetAttribute((const void*)hb_finish_kernel<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lf);
-      hipLaunchKernelGGL((hb_finish_kernel<T, 256>), dim3(1), dim3(256), lf, s, n, dref, eref, Zr, state, infob,
-                         sweepsb);
-    } else {
-      SteqrArgs q;
-      q.n = n;
-      q.maxsweeps = a.maxsweeps;
-      q.d = dref;
-      q.e = eref;
-      q.info = infob;
-      q.sweeps = sweepsb;
-      if ((e = launch_steqr(q, cplx, s)) != hipSuccess) return e;
-    }
+    // the recurrence by one thread writing its rotations to HBM, their replay
+    // onto Q's rows over the whole GPU (values only: the recurrence alone)
+    if ((e = st_rounds<T>(n, a.maxsweeps, dref, eref, wantz ? Zr : nullptr, n, (int64_t)n, rec, cap, npos, state,
+                          infob, sweepsb, s)) != hipSuccess)
+      return e;
     hipLaunchKernelGGL(hb_sort_kernel, dim3(1), dim3(1024), 0, s, n, a.sort, dref, perm, a.W + (int64_t)b * a.sW);
     if (wantz)
       hipLaunchKernelGGL(hb_permute_kernel<T>, dim3(hb_blocks((int64_t)nn)), dim3(256), 0, s, n, Zr, perm,

@@ -227,7 +227,7 @@ int gs_steqr(gs_handle* h, int64_t n, double* d, double* e, double* Z, int64_t l
     return GS_OK;
   }
   if (!d || (n > 1 && !e)) return handle_fail(h, GS_E_ARG, "null pointer");
-  if (n > steqr_max_n()) return handle_fail(h, GS_E_UNSUPPORTED, "order above 2048 (d, e and the log must fit in LDS)");
+  if (n > heev_big_max_n()) return handle_fail(h, GS_E_UNSUPPORTED, "order above 4096");
   if (zrows > INT32_MAX) return handle_fail(h, GS_E_DIM, "too many rows of Z");
   std::lock_guard<std::mutex> lk(handle_mutex(h));
   HCHK(hipSetDevice(handle_device(h)));
@@ -239,13 +239,21 @@ int gs_steqr(gs_handle* h, int64_t n, double* d, double* e, double* Z, int64_t l
   a.n = (int)n;
   a.zrows = wantz ? (int)zrows : 0;
   a.maxsweeps = 30 * (int)n;
+  // above the LDS log's range: the rotations through a stream in HBM
+  void* bigws = nullptr;
+  if (n > steqr_max_n()) {
+    DevBuf& ws = handle_buf(h, 13);
+    HCHK(ws.ensure(steqr_big_ws_bytes((int)n)));
+    bigws = ws.p;
+  }
+  auto run = [&]() { return bigws ? launch_steqr_big(a, zcomplex != 0, bigws, s) : launch_steqr(a, zcomplex != 0, s); };
   if (dev) {
     a.d = d;
     a.e = e;
     a.Z = wantz ? Z : nullptr;
     a.ldz = ldz;
     a.info = info;
-    HCHK(launch_steqr(a, zcomplex != 0, s));
+    HCHK(run());
     if (!(flags & GS_ASYNC)) HCHK(hipStreamSynchronize(s));
     return GS_OK;
   }
@@ -265,7 +273,7 @@ int gs_steqr(gs_handle* h, int64_t n, double* d, double* e, double* Z, int64_t l
   a.Z = dZ;
   a.ldz = zrows;
   a.info = dI;
-  HCHK(launch_steqr(a, zcomplex != 0, s));
+  HCHK(run());
   HCHK(hipMemcpyAsync(d, dd, (size_t)n * 8, hipMemcpyDeviceToHost, s));
   if (n > 1) HCHK(hipMemcpyAsync(e, de, (size_t)(n - 1) * 8, hipMemcpyDeviceToHost, s));
   if (wantz) HCHK(copy_cols(Z, ldz * es, dZ, zrows * es, zrows * es, n, hipMemcpyDeviceToHost, s));

@@ -226,9 +226,14 @@ hipError_t launch_heev(const HeevArgs& a, bool cplx, hipStream_t s);
 // whole GPU (the working matrix, Q and the eigenvectors in HBM); modes 0
 // and 2 as launch_heev (E is output only in mode 2).  ws holds
 // heev_big_ws_bytes(n, cplx) bytes of device memory.
-constexpr int heev_big_max_n() { return steqr_max_n(); }
+constexpr int heev_big_max_n() { return 4096; }
 size_t heev_big_ws_bytes(int n, bool cplx);
 hipError_t launch_heev_big(const HeevArgs& a, bool cplx, void* ws, hipStream_t s);
+// gs_steqr for steqr_max_n < n <= heev_big_max_n: the recurrence's
+// rotations through a record stream in HBM (ws: steqr_big_ws_bytes(n)),
+// replayed onto Z over the whole GPU; the host waits once per stream round
+size_t steqr_big_ws_bytes(int n);
+hipError_t launch_steqr_big(const SteqrArgs& a, bool zcplx, void* ws, hipStream_t s);
 hipError_t launch_steqr(const SteqrArgs& a, bool zcplx, hipStream_t s);
 
 hipError_t launch_ordschur_d(int n, double* T, int64_t ldt, double* Z, int64_t ldz, bool wantz, const int32_t* select,

@@ -293,7 +293,7 @@ int gs_eigcond_z(gs_handle* h, int64_t n, const double* T, int64_t ldt, int64_t 
  * GS_WANTZ, V (ldv, strideV) the eigenvectors (complex for _z).  info[k]
  * (may be NULL): 0, or the number of unconverged off-diagonal entries when
  * the iteration stopped after 30 n sweeps (the reference's loop has no
- * limit).  n <= gs_heev_max_n() = 2048 (GS_E_UNSUPPORTED above): one
+ * limit).  n <= gs_heev_max_n() = 4096 (GS_E_UNSUPPORTED above): one
  * workgroup per matrix with the matrix in LDS up to 96 (complex) / 128
  * (real); above, one matrix at a time over the whole GPU with the working
  * matrices in device memory held by the handle.  Host pointers are staged
@@ -320,7 +320,10 @@ int gs_hetrd_d(gs_handle* h, int64_t n, const double* A, int64_t lda, int upper,
  * the eigenvalues (unsorted), e (n - 1) destroyed, and the zrows x n matrix Z
  * (ldz; complex when zcomplex; NULL: eigenvalues only) multiplied by the
  * rotations, bit for bit as the reference's operation order gives them.
- * info[0] as for gs_heev_z.  n <= 2048. */
+ * info[0] as for gs_heev_z.  n <= 4096: above 2048 the rotations pass
+ * through a record stream in device memory held by the handle, replayed
+ * onto Z's rows over the whole GPU (the host waits once per stream round;
+ * the results are the same bit for bit). */
 int gs_steqr(gs_handle* h, int64_t n, double* d, double* e, double* Z, int64_t ldz, int64_t zrows, int zcomplex,
              int32_t* info, int flags);
 

@@ -192,7 +192,7 @@ def test_heev_edge_cases(gs, oracle, cplx):
     C = np.triu(A) + np.tril(r.standard_normal((12, 12)), -1)
     assert np.array_equal(gs.geigvals_hermitian(C, "U"), gs.geigvals_hermitian(A, "U"))
     # above the largest order: a clear error, no fallback
-    assert gs.heev_max_n(cplx) == 2048
+    assert gs.heev_max_n(cplx) == 4096
     with pytest.raises(gs.GSError):
         gs.geigen_hermitian(np.eye(gs.heev_max_n(cplx) + 1, dtype=dt))
 
@@ -299,3 +299,36 @@ def test_heev_large_phases_bitwise(gs, cplx, monkeypatch):
         W2, V2, info2 = gs.heev_batched(A)
         assert not info2.any()
         assert np.array_equal(W2, W) and np.array_equal(V2, V), cap
+
+
+def test_steqr_above_lds_log_bitwise(gs, oracle):
+    """gs_steqr above order 2048 (its LDS log's range): the rotations go
+    through a record stream in HBM and are replayed onto Z's rows over the
+    whole GPU, bit for bit the oracle's d and Z (a few rows of Z keep the
+    oracle's time down)."""
+    r = np.random.default_rng(73)
+    n = 2100
+    d = r.standard_normal(n)
+    e = r.standard_normal(n - 1)
+    Z = r.standard_normal((5, n))
+    wo, zo, info_o, _ = oracle.steqr(d, e, Z)
+    wg, zg, _, info_g = gs.steqr_(d, e, Z)
+    assert info_g == info_o == 0
+    assert np.array_equal(wg, wo)
+    assert np.array_equal(zg, zo)
+    wv, _, _, info_v = gs.steqr_(d, e, None)
+    assert info_v == 0 and np.array_equal(wv, wo)
+
+
+def test_heev_above_2048(gs):
+    """A real symmetric eigensystem of order 2100 (above the one-workgroup
+    log's range): residual, orthogonality and the sorted order."""
+    r = np.random.default_rng(74)
+    n = 2100
+    A = _herm(r, n, False)
+    E = gs.geigen_hermitian(A)
+    scale = np.abs(A).max()
+    assert np.all(np.diff(E.values) >= 0)
+    assert np.abs(A @ E.vectors - E.vectors * E.values).max() <= 16 * n * EPS * scale
+    assert np.abs(E.vectors.T @ E.vectors - np.eye(n)).max() <= 16 * n * EPS
+    assert np.abs(np.linalg.eigvalsh(A) - E.values).max() <= 16 * n * EPS * scale
