@@ -150,3 +150,37 @@ def test_workload4_n512_vs_quad_double(gs, oracle):
     print(f"n=512 vs quad-double: max |dlambda| = {err.max():.3e}, max |dlambda|/|lambda| = {rel.max():.3e}, "
           f"median |dlambda|/|lambda| = {np.median(rel):.3e}, max err/bound = {(err / bound).max():.3e}", flush=True)
     checks.eigen_parity(A, S.values, ref, cond)
+
+
+class _SchurTZ:
+    def __init__(self, T, Z, values):
+        self.T, self.Z, self.values = T, Z, values
+
+
+def test_large_workload4_full_size(gs):
+    """The large-matrix workload itself: one 4096 x 4096 Q (Lambda + U) Q^T.
+    Standard-form quasi-triangular T, the acceptance bounds on the backward
+    error and Z's orthogonality (50 n eps), and every computed eigenvalue
+    within 10 n eps ||A||_F cond(lambda) of the nearest exact one (the
+    conditions from this build's own eigenvectors at n = 4096; ||A||_F bounds
+    ||A||_2)."""
+    n = 4096
+    A, M, exact = gen.workload4_parts(seed=0, n=n)
+    print(f"case workload4 n={n}", flush=True)
+    S = gs.gschur(A)
+    eps = np.finfo(np.float64).eps
+    T, Z = np.asarray(S.T), np.asarray(S.Z)
+    assert np.all(np.tril(T, -2) == 0) and checks.checkblocks(T)
+    assert checks.rel_backward(A, T, Z) <= 50 * n * eps
+    assert checks.rel_orth(Z) <= 50 * n * eps
+    VR = gs.geigvecs(_SchurTZ(T, Z, S.values), left=False)
+    VL = gs.geigvecs(_SchurTZ(T, Z, S.values), left=True)
+    cond = checks.eig_conditions(VR, VL)
+    w = np.asarray(S.values)
+    d = np.abs(w[:, None] - exact[None, :])
+    near = d.argmin(axis=1)
+    err = d[np.arange(n), near]
+    bound = 10 * n * eps * np.linalg.norm(A) * cond
+    # (U makes most eigenvalues ill-conditioned: a small eigenvalue's error
+    # disc can hold its neighbours, so the nearest exact value is not unique)
+    assert np.all(err <= bound), float(np.max(err / bound))
