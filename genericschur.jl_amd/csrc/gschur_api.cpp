@@ -1036,7 +1036,7 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
   HIPCHK(h, h->bInfo.ensure(sizeof(int32_t)));
   HIPCHK(h, h->bStats.ensure(2 * sizeof(int64_t)));
   HIPCHK(h, h->bScl.ensure(2 * sizeof(double)));
-  HIPCHK(h, h->bWs.ensure(es * 5 * (size_t)n + 64));
+  HIPCHK(h, h->bWs.ensure(std::max(es * 5 * (size_t)n + 64, 8 * zhq_global_ws_doubles(n))));
   if (dev) {
     dA = A;
     dT = T;

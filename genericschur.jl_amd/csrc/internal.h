@@ -100,6 +100,8 @@ hipError_t launch_zschur_global(const SingleArgs& a, int nt, hipStream_t s);
 // one matrix above the LDS-resident orders: Hessenberg + Q in global memory
 // (H in a.T, Q in a.Z, ld = n), then launch_zqrw on a batch of one
 hipError_t launch_zhq_global(const SingleArgs& a, double* scl, hipStream_t s);
+// doubles of a.ws launch_zhq_global needs (tau, two vectors, gemv partials)
+size_t zhq_global_ws_doubles(int64_t n);
 // the windowed QR kernel with nt threads (128 .. 1024): one chasing
 // wavefront, nt / 64 - 1 chain wavefronts
 hipError_t launch_zqrw_single(const BatchArgs& a, int nt, hipStream_t s);

@@ -35,6 +35,80 @@ namespace gs {
 #define HH(i, j) H[lay.idx((i), (j))]
 #define ZZ(i, j) Zg[((i) - 1) + (int64_t)((j) - 1) * ldz]
 
+// Reflector on x = H[p+1 : n-1, p] (householder.jl:56-102) by one
+// wavefront (its reductions need no workgroup barrier): tau[p], beta in
+// H[p+1, p], the scaled tail below.
+GS_DEV void z_hess_reflect(zd* H, int64_t ld, int n, int p, zd* tau) {
+  const int lane = threadIdx.x & 63;
+  const double sfmin = GS_FMIN / GS_EPS;
+  const int r0 = p + 2;
+  double amax = 0.0;
+  for (int r = r0 + lane; r < n; r += 64) {
+    const zd v = H[r + p * ld];
+    amax = fmax(amax, fmax(fabs(v.x), fabs(v.y)));
+  }
+  amax = wave_max(amax);
+  double xnorm = 0.0;
+  if (amax > 0.0) {
+    const double ra = 1.0 / amax;
+    double s = 0.0;
+    for (int r = r0 + lane; r < n; r += 64) {
+      const zd v = H[r + p * ld];
+      const double a = v.x * ra, b = v.y * ra;
+      s += a * a + b * b;
+    }
+    s = wave_sum(s);
+    xnorm = amax * sqrt(s);
+  }
+  zd alpha = H[(p + 1) + p * ld];
+  zd t_ = zr(0.0);
+  if (!(xnorm == 0.0 && alpha.y == 0.0)) {
+    double ar = alpha.x, ai = alpha.y;
+    double beta = -copysign(hypot3(ar, ai, xnorm), ar);
+    int kount = 0;
+    if (fabs(beta) < sfmin) {
+      const double rsfmin = 1.0 / sfmin;
+      bool smallb = true;
+      while (smallb) {
+        kount += 1;
+        wsync();
+        for (int r = r0 + lane; r < n; r += 64) H[r + p * ld] = H[r + p * ld] * rsfmin;
+        beta *= rsfmin;
+        ar *= rsfmin;
+        ai *= rsfmin;
+        smallb = (fabs(beta) < sfmin) && (kount < 20);
+      }
+      wsync();
+      double am2 = 0.0;
+      for (int r = r0 + lane; r < n; r += 64) {
+        const zd v = H[r + p * ld];
+        am2 = fmax(am2, fmax(fabs(v.x), fabs(v.y)));
+      }
+      am2 = wave_max(am2);
+      xnorm = 0.0;
+      if (am2 > 0.0) {
+        const double ra = 1.0 / am2;
+        double s = 0.0;
+        for (int r = r0 + lane; r < n; r += 64) {
+          const zd v = H[r + p * ld];
+          const double a = v.x * ra, b = v.y * ra;
+          s += a * a + b * b;
+        }
+        s = wave_sum(s);
+        xnorm = am2 * sqrt(s);
+      }
+      alpha = zmk(ar, ai);
+      beta = -copysign(hypot3(ar, ai, xnorm), ar);
+    }
+    t_ = zmk((beta - ar) / beta, -ai / beta);
+    const zd sc = zdiv(zr(1.0), alpha - zr(beta));
+    for (int r = r0 + lane; r < n; r += 64) H[r + p * ld] = H[r + p * ld] * sc;
+    for (int j = 0; j < kount; ++j) beta *= sfmin;
+    if (lane == 0) H[(p + 1) + p * ld] = zr(beta);
+  }
+  if (lane == 0) tau[p] = t_;
+}
+
 // ------------------------------------------------------------------------
 // Hessenberg reduction in place (hessenberg.jl:3-17), full layout, 0-based.
 // Reflector p is stored below the subdiagonal of column p, tau[p] its scalar.
@@ -45,81 +119,12 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
   (void)red;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const double sfmin = GS_FMIN / GS_EPS;
   // RM > 0: wavefront 0 computes reflector p + 1 (column p + 1, final once
   // the right update's dot products are done) while the other wavefronts
   // apply reflector p's right update to columns p + 2 .. n - 1
   constexpr bool OVL = RM > 0 && NT >= 128;
-  // reflector on x = H[p+1 : n-1, p] (householder.jl:56-102), computed by
-  // wavefront 0 alone: its reductions need no workgroup barrier
-  auto reflect = [&](const int p) {
-    const int r0 = p + 2;
-    double amax = 0.0;
-    for (int r = r0 + lane; r < n; r += 64) {
-      const zd v = H[r + p * ld];
-      amax = fmax(amax, fmax(fabs(v.x), fabs(v.y)));
-    }
-    amax = wave_max(amax);
-    double xnorm = 0.0;
-    if (amax > 0.0) {
-      const double ra = 1.0 / amax;
-      double s = 0.0;
-      for (int r = r0 + lane; r < n; r += 64) {
-        const zd v = H[r + p * ld];
-        const double a = v.x * ra, b = v.y * ra;
-        s += a * a + b * b;
-      }
-      s = wave_sum(s);
-      xnorm = amax * sqrt(s);
-    }
-    zd alpha = H[(p + 1) + p * ld];
-    zd t_ = zr(0.0);
-    if (!(xnorm == 0.0 && alpha.y == 0.0)) {
-      double ar = alpha.x, ai = alpha.y;
-      double beta = -copysign(hypot3(ar, ai, xnorm), ar);
-      int kount = 0;
-      if (fabs(beta) < sfmin) {
-        const double rsfmin = 1.0 / sfmin;
-        bool smallb = true;
-        while (smallb) {
-          kount += 1;
-          wsync();
-          for (int r = r0 + lane; r < n; r += 64) H[r + p * ld] = H[r + p * ld] * rsfmin;
-          beta *= rsfmin;
-          ar *= rsfmin;
-          ai *= rsfmin;
-          smallb = (fabs(beta) < sfmin) && (kount < 20);
-        }
-        wsync();
-        double am2 = 0.0;
-        for (int r = r0 + lane; r < n; r += 64) {
-          const zd v = H[r + p * ld];
-          am2 = fmax(am2, fmax(fabs(v.x), fabs(v.y)));
-        }
-        am2 = wave_max(am2);
-        xnorm = 0.0;
-        if (am2 > 0.0) {
-          const double ra = 1.0 / am2;
-          double s = 0.0;
-          for (int r = r0 + lane; r < n; r += 64) {
-            const zd v = H[r + p * ld];
-            const double a = v.x * ra, b = v.y * ra;
-            s += a * a + b * b;
-          }
-          s = wave_sum(s);
-          xnorm = am2 * sqrt(s);
-        }
-        alpha = zmk(ar, ai);
-        beta = -copysign(hypot3(ar, ai, xnorm), ar);
-      }
-      t_ = zmk((beta - ar) / beta, -ai / beta);
-      const zd sc = zdiv(zr(1.0), alpha - zr(beta));
-      for (int r = r0 + lane; r < n; r += 64) H[r + p * ld] = H[r + p * ld] * sc;
-      for (int j = 0; j < kount; ++j) beta *= sfmin;
-      if (lane == 0) H[(p + 1) + p * ld] = zr(beta);
-    }
-    if (lane == 0) tau[p] = t_;
-  };
+  // reflector p by wavefront 0 alone
+  auto reflect = [&](const int p) { z_hess_reflect(H, ld, n, p, tau); };
   if (OVL) {
     if (wave == 0 && n >= 2) reflect(0);
   }
@@ -1266,7 +1271,7 @@ template <int NT> __global__ void __launch_bounds__(NT) zschur_global_kernel(Sin
 // caller's Z) in Z, (cscale, anrm) of _scale! in scl[0], scl[1] and the
 // status in info.  The same steps as zschur_global_kernel before its QR
 // phase; zqrw2_kernel then runs the sweeps with T and Z in HBM.
-template <int NT> __global__ void __launch_bounds__(NT) zhq_global_kernel(SingleArgs a, double* scl) {
+template <int NT, bool PREP_ONLY> __global__ void __launch_bounds__(NT) zhq_global_kernel(SingleArgs a, double* scl) {
   __shared__ double red[NT / 64 + 1];
   const int n = (int)a.n;
   zd* A = reinterpret_cast<zd*>(a.A);
@@ -1283,7 +1288,7 @@ template <int NT> __global__ void __launch_bounds__(NT) zhq_global_kernel(Single
     scl[0] = cscale;
     scl[1] = anrm;
   }
-  if (st0 != 0) return;
+  if (st0 != 0 || PREP_ONLY) return;
   if (!hessin) {
     z_hessenberg<NT>(T, ldt, n, tau, red, tau + 4 * n);
     if (wantz) {
@@ -1309,8 +1314,214 @@ template <int NT> __global__ void __launch_bounds__(NT) zhq_global_kernel(Single
   }
 }
 
+// ------------------------------------------------------------------------
+// The same reduction and Q for orders from kZhbMinN up, over the whole GPU:
+// per column p the reflector (one wavefront), the left update's column dots
+// and rank-1 update, the right update's gemv partials (64-row blocks times
+// column chunks), their sum with column p+1's update, and the right rank-1
+// update of columns p+2..; then Q's backward accumulation (z_form_q_inplace's
+// steps, two launches per reflector).  The element operations are those of
+// z_hessenberg and z_form_q_inplace; only the dot products' summation
+// order differs.  Every launch first checks the status z_prep wrote.
+constexpr int kZhbMinN = 192;
+struct ZhbArgs {
+  int n = 0, p = 0, ks = 1, kchunk = 0;
+  zd* H = nullptr;
+  int64_t ld = 0;
+  zd* Q = nullptr;
+  int64_t ldq = 0;
+  zd* tau = nullptr;
+  zd* wk = nullptr;   // n
+  zd* wk2 = nullptr;  // n
+  zd* P = nullptr;    // 16 x n
+  const int32_t* info = nullptr;
+};
+
+__global__ void __launch_bounds__(64) zhb_reflect_kernel(ZhbArgs a) {
+  if (*a.info != 0) return;
+  z_hess_reflect(a.H, a.ld, a.n, a.p, a.tau);
+}
+
+// lmul!(H', A[p+1:n, p+1:n]): va_j = conj(tau) (H[p+1, j] + sum_{r > p+1}
+// conj(v_r) H[r, j]), one wavefront per column j > p
+__global__ void __launch_bounds__(256) zhb_ldots_kernel(ZhbArgs a) {
+  if (*a.info != 0) return;
+  const int n = a.n, p = a.p, lane = threadIdx.x & 63;
+  const int j = p + 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;
+  const zd* H = a.H;
+  const int64_t ld = a.ld;
+  zd acc = zr(0.0);
+  for (int r = p + 2 + lane; r < n; r += 64) acc = acc + zconj(H[r + p * ld]) * H[r + (int64_t)j * ld];
+  acc = wave_zsum(acc);
+  if (lane == 0) a.wk[j] = zconj(a.tau[p]) * (H[(p + 1) + (int64_t)j * ld] + acc);
+}
+
+// column j -= va_j [1; v] over rows p+1 ..
+__global__ void __launch_bounds__(256) zhb_lrank1_kernel(ZhbArgs a) {
+  if (*a.info != 0) return;
+  const int n = a.n, p = a.p;
+  const int r = p + 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = p + 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (r >= n || j >= n) return;
+  zd* H = a.H;
+  const int64_t ld = a.ld;
+  const zd h = H[r + (int64_t)j * ld], va = a.wk[j];
+  H[r + (int64_t)j * ld] = (r == p + 1) ? h - va : h - va * H[r + p * ld];
+}
+
+// rmul!(A[:, p+1:n], H): P[y][r] = sum over chunk y of columns c > p+1 of
+// H[r, c] v_c, all rows
+__global__ void __launch_bounds__(256) zhb_rgemv_kernel(ZhbArgs a) {
+  __shared__ double raw[4 * 64 * 2];
+  zd* red = reinterpret_cast<zd*>(raw);
+  if (*a.info != 0) return;
+  const int n = a.n, p = a.p, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
+  const int c0 = p + 2 + blockIdx.y * a.kchunk, c1 = min(n, c0 + a.kchunk);
+  const zd* H = a.H;
+  const int64_t ld = a.ld;
+  zd acc = zr(0.0);
+  if (r < n)
+    for (int c = c0 + wv; c < c1; c += 4) acc = acc + H[r + (int64_t)c * ld] * H[c + p * ld];
+  red[wv * 64 + lane] = acc;
+  __syncthreads();
+  if (wv == 0 && r < n) a.P[(int64_t)blockIdx.y * n + r] = ((red[lane] + red[64 + lane]) + red[128 + lane]) + red[192 + lane];
+}
+
+// x_r = H[r, p+1] + sum_y P[y][r]; H[r, p+1] -= tau x_r; wk2[r] = -tau x_r
+__global__ void __launch_bounds__(256) zhb_rcombine_kernel(ZhbArgs a) {
+  if (*a.info != 0) return;
+  const int n = a.n, p = a.p;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  zd acc = zr(0.0);
+  for (int y = 0; y < a.ks; ++y) acc = acc + a.P[(int64_t)y * n + r];
+  zd* h = a.H + r + (int64_t)(p + 1) * a.ld;
+  const zd t = a.tau[p];
+  const zd x = *h + acc;
+  *h = *h - t * x;
+  a.wk2[r] = x * (-t);
+}
+
+// H[r, c] += wk2[r] conj(v_c), all rows, columns p+2 ..
+__global__ void __launch_bounds__(256) zhb_rrank1_kernel(ZhbArgs a) {
+  if (*a.info != 0) return;
+  const int n = a.n, p = a.p;
+  const int r = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int c = p + 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (r >= n || c >= n) return;
+  zd* H = a.H;
+  const int64_t ld = a.ld;
+  H[r + (int64_t)c * ld] = H[r + (int64_t)c * ld] + a.wk2[r] * zconj(H[c + p * ld]);
+}
+
+__global__ void __launch_bounds__(256) zhb_identity_kernel(ZhbArgs a) {
+  if (*a.info != 0) return;
+  const int64_t nn = (int64_t)a.n * a.n;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nn; e += (int64_t)gridDim.x * 256) {
+    const int c = (int)(e / a.n), r = (int)(e - (int64_t)c * a.n);
+    a.Q[r + c * a.ldq] = zr(r == c ? 1.0 : 0.0);
+  }
+}
+
+// reflector k = a.p applied to Q(c:, c+1:), c = k + 1: s_j = tau (Q[c, j] +
+// sum_{r > c} conj(v_r) Q[r, j]), one wavefront per column
+__global__ void __launch_bounds__(256) zhb_qdots_kernel(ZhbArgs a) {
+  if (*a.info != 0) return;
+  const int n = a.n, k = a.p, c = k + 1, lane = threadIdx.x & 63;
+  const int j = c + 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;
+  zd acc = zr(0.0);
+  for (int r = c + 1 + lane; r < n; r += 64) acc = acc + zconj(a.H[r + k * a.ld]) * a.Q[r + (int64_t)j * a.ldq];
+  acc = wave_zsum(acc);
+  if (lane == 0) a.wk[j] = a.tau[k] * (a.Q[c + (int64_t)j * a.ldq] + acc);
+}
+
+// column j > c -= s_j [1; v]; column c = e_c - tau [1; v]
+__global__ void __launch_bounds__(256) zhb_qupdate_kernel(ZhbArgs a) {
+  if (*a.info != 0) return;
+  const int n = a.n, k = a.p, c = k + 1;
+  const int r = c + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = c + blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (r >= n || j >= n) return;
+  const zd t = a.tau[k];
+  zd* q = a.Q + r + (int64_t)j * a.ldq;
+  if (j == c) {
+    *q = (r == c) ? zr(1.0) - t : -(t * a.H[r + k * a.ld]);
+  } else {
+    const zd h = *q, sj = a.wk[j];
+    *q = (r == c) ? h - sj : h - sj * a.H[r + k * a.ld];
+  }
+}
+
+// T = triu(H, -1)
+__global__ void __launch_bounds__(256) zhb_zero_kernel(ZhbArgs a) {
+  if (*a.info != 0) return;
+  const int64_t nn = (int64_t)a.n * a.n;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nn; e += (int64_t)gridDim.x * 256) {
+    const int c = (int)(e / a.n), r = (int)(e - (int64_t)c * a.n);
+    if (r > c + 1) a.H[r + c * a.ld] = zr(0.0);
+  }
+}
+
+static hipError_t launch_zhb(const SingleArgs& g, hipStream_t s) {
+  const int n = (int)g.n;
+  ZhbArgs a;
+  a.n = n;
+  a.H = reinterpret_cast<zd*>(g.T);
+  a.ld = g.ldt;
+  a.Q = reinterpret_cast<zd*>(g.Z);
+  a.ldq = g.ldz;
+  a.tau = reinterpret_cast<zd*>(g.ws);
+  a.wk = a.tau + n;
+  a.wk2 = a.wk + n;
+  a.P = a.wk2 + n;
+  a.info = g.info;
+  const int nrb = (n + 63) / 64;
+  const int gridn = std::min(4096, (int)(((int64_t)n * n + 255) / 256));
+  for (int p = 0; p + 2 <= n; ++p) {
+    a.p = p;
+    const int m = n - 1 - p;   // rows / columns p+1 ..
+    const int m2 = n - 2 - p;  // columns p+2 ..
+    hipLaunchKernelGGL(zhb_reflect_kernel, dim3(1), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(zhb_ldots_kernel, dim3((m + 3) / 4), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(zhb_lrank1_kernel, dim3((m + 63) / 64, (m + 3) / 4), dim3(256), 0, s, a);
+    a.ks = 1;
+    a.kchunk = 0;
+    if (m2 > 0) {
+      a.ks = std::max(1, std::min(16, (512 + nrb - 1) / nrb));
+      a.kchunk = (m2 + a.ks - 1) / a.ks;
+      a.ks = (m2 + a.kchunk - 1) / a.kchunk;
+    }
+    hipLaunchKernelGGL(zhb_rgemv_kernel, dim3(nrb, a.ks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(zhb_rcombine_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a);
+    if (m2 > 0) hipLaunchKernelGGL(zhb_rrank1_kernel, dim3(nrb, (m2 + 3) / 4), dim3(256), 0, s, a);
+  }
+  if (g.flags & 0x01) {
+    hipLaunchKernelGGL(zhb_identity_kernel, dim3(gridn), dim3(256), 0, s, a);
+    for (int k = n - 2; k >= 0; --k) {
+      const int c = k + 1;
+      a.p = k;
+      if (n - 1 - c > 0) hipLaunchKernelGGL(zhb_qdots_kernel, dim3((n - 1 - c + 3) / 4), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(zhb_qupdate_kernel, dim3((n - c + 63) / 64, (n - c + 3) / 4), dim3(256), 0, s, a);
+    }
+  }
+  hipLaunchKernelGGL(zhb_zero_kernel, dim3(gridn), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+size_t zhq_global_ws_doubles(int64_t n) { return 2 * (19 * (size_t)n) + 8; }
+
 hipError_t launch_zhq_global(const SingleArgs& a, double* scl, hipStream_t s) {
-  hipLaunchKernelGGL(zhq_global_kernel<1024>, dim3(1), dim3(1024), 0, s, a, scl);
+  const bool hessin = (a.flags & 0x04) != 0;
+  if (!hessin && a.n >= kZhbMinN) {
+    hipLaunchKernelGGL((zhq_global_kernel<1024, true>), dim3(1), dim3(1024), 0, s, a, scl);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_zhb(a, s);
+  }
+  hipLaunchKernelGGL((zhq_global_kernel<1024, false>), dim3(1), dim3(1024), 0, s, a, scl);
   return hipGetLastError();
 }
 

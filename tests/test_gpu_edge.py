@@ -82,10 +82,11 @@ def test_families_single(gs, oracle, cplx, sizes):
     report(f"single {'complex' if cplx else 'real'}", tally)
 
 
-@pytest.mark.parametrize("n", [100, 130, 200])
+@pytest.mark.parametrize("n", [100, 130, 200, 260])
 def test_families_single_complex_windowed(gs, oracle, n):
     """Complex orders above the LDS-resident range: Hessenberg + Q in global
-    memory, then the windowed QR kernel with H and Z in HBM."""
+    memory (one workgroup below order 192, grid-wide launches per column
+    from 192 up), then the windowed QR kernel with H and Z in HBM."""
     assert n > gs.batched_max_n(True)
     tally = {}
     for name, A, normal in families(n, True, seed=4):
