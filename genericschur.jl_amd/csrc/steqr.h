@@ -495,66 +495,114 @@ template <class TZ, class ZA> GS_DEV void st_apply_row(const StLog& lg, ZA z) {
   }
 }
 
+// A zero the compiler cannot see through: an index that makes a load of a
+// uniform address a vector load, counted apart from the LDS accesses (a
+// scalar load would share their counter, so every LDS wait would wait for
+// it too).
+GS_DEV int st_opaque_zero() {
+  int z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+
 // Replays a record stream (StLogG) of npos records onto one row of Z.  The
 // rotations of a segment are taken four at a time with the four columns
 // they read loaded first: within a segment each column is read by one
 // rotation before the next rotation writes it, so the element operations
-// and their order are those of one rotation after another.
+// and their order are those of one rotation after another.  The (c, s)
+// records are vector loads kept four groups ahead in four register
+// buffers, so that their latency overlaps the rotations of the groups
+// before (indices clamped into the segment; a clamped load is not used).
 template <class TZ, class ZA> GS_DEV void st_replay_row(const double2* __restrict__ rec, int npos, ZA z) {
+  const int vz = st_opaque_zero();
   int q = 0;
   while (q < npos) {
     const double2 h = rec[q];
     const int i0 = (int)h.x, dc = (int)h.y;
     const int cnt = dc < 0 ? -dc : dc;
-    const double2* r = rec + q + 1;
-    int k = 0;
-    if (dc < 0) {
-      TZ cur = z(i0 + 1);
-      for (; k + 4 <= cnt; k += 4) {
+    const double2* r = rec + q + 1 + vz;
+    const bool back = dc < 0;
+    TZ cur = back ? z(i0 + 1) : z(i0);
+    // four rotations k .. k + 3 with records c[0..3]
+    auto grp = [&](int k, const double2* c) {
+      if (back) {
         const int i = i0 - k;
         const TZ a0 = z(i), a1 = z(i - 1), a2 = z(i - 2), a3 = z(i - 3);
-        const double2 c0 = r[k], c1 = r[k + 1], c2 = r[k + 2], c3 = r[k + 3];
-        z(i + 1) = st_rot2(a0, cur, c0.x, c0.y);
-        cur = st_rot1(a0, cur, c0.x, c0.y);
-        z(i) = st_rot2(a1, cur, c1.x, c1.y);
-        cur = st_rot1(a1, cur, c1.x, c1.y);
-        z(i - 1) = st_rot2(a2, cur, c2.x, c2.y);
-        cur = st_rot1(a2, cur, c2.x, c2.y);
-        z(i - 2) = st_rot2(a3, cur, c3.x, c3.y);
-        cur = st_rot1(a3, cur, c3.x, c3.y);
+        z(i + 1) = st_rot2(a0, cur, c[0].x, c[0].y);
+        cur = st_rot1(a0, cur, c[0].x, c[0].y);
+        z(i) = st_rot2(a1, cur, c[1].x, c[1].y);
+        cur = st_rot1(a1, cur, c[1].x, c[1].y);
+        z(i - 1) = st_rot2(a2, cur, c[2].x, c[2].y);
+        cur = st_rot1(a2, cur, c[2].x, c[2].y);
+        z(i - 2) = st_rot2(a3, cur, c[3].x, c[3].y);
+        cur = st_rot1(a3, cur, c[3].x, c[3].y);
+      } else {
+        const int i = i0 + k;
+        const TZ a0 = z(i + 1), a1 = z(i + 2), a2 = z(i + 3), a3 = z(i + 4);
+        z(i) = st_rot1(cur, a0, c[0].x, c[0].y);
+        cur = st_rot2(cur, a0, c[0].x, c[0].y);
+        z(i + 1) = st_rot1(cur, a1, c[1].x, c[1].y);
+        cur = st_rot2(cur, a1, c[1].x, c[1].y);
+        z(i + 2) = st_rot1(cur, a2, c[2].x, c[2].y);
+        cur = st_rot2(cur, a2, c[2].x, c[2].y);
+        z(i + 3) = st_rot1(cur, a3, c[3].x, c[3].y);
+        cur = st_rot2(cur, a3, c[3].x, c[3].y);
       }
-      for (; k < cnt; ++k) {
+    };
+    auto ld4 = [&](double2* b, int kk) {
+      const int last = cnt - 1;
+      b[0] = r[min(kk, last)];
+      b[1] = r[min(kk + 1, last)];
+      b[2] = r[min(kk + 2, last)];
+      b[3] = r[min(kk + 3, last)];
+    };
+    int k = 0;
+    if (cnt >= 4) {
+      double2 b0[4], b1[4], b2[4], b3[4];
+      ld4(b0, 0);
+      ld4(b1, 4);
+      ld4(b2, 8);
+      ld4(b3, 12);
+      for (; k + 16 <= cnt; k += 16) {
+        grp(k, b0);
+        ld4(b0, k + 16);
+        grp(k + 4, b1);
+        ld4(b1, k + 20);
+        grp(k + 8, b2);
+        ld4(b2, k + 24);
+        grp(k + 12, b3);
+        ld4(b3, k + 28);
+      }
+      // at most three more full groups, already in b0, b1, b2
+      if (k + 4 <= cnt) {
+        grp(k, b0);
+        k += 4;
+      }
+      if (k + 4 <= cnt) {
+        grp(k, b1);
+        k += 4;
+      }
+      if (k + 4 <= cnt) {
+        grp(k, b2);
+        k += 4;
+      }
+    }
+    for (; k < cnt; ++k) {
+      const double2 cs = r[k];
+      if (back) {
         const int i = i0 - k;
-        const double2 cs = r[k];
         const TZ a1 = z(i);
         z(i + 1) = st_rot2(a1, cur, cs.x, cs.y);
         cur = st_rot1(a1, cur, cs.x, cs.y);
-      }
-      z(i0 - cnt + 1) = cur;
-    } else {
-      TZ cur = z(i0);
-      for (; k + 4 <= cnt; k += 4) {
+      } else {
         const int i = i0 + k;
-        const TZ a0 = z(i + 1), a1 = z(i + 2), a2 = z(i + 3), a3 = z(i + 4);
-        const double2 c0 = r[k], c1 = r[k + 1], c2 = r[k + 2], c3 = r[k + 3];
-        z(i) = st_rot1(cur, a0, c0.x, c0.y);
-        cur = st_rot2(cur, a0, c0.x, c0.y);
-        z(i + 1) = st_rot1(cur, a1, c1.x, c1.y);
-        cur = st_rot2(cur, a1, c1.x, c1.y);
-        z(i + 2) = st_rot1(cur, a2, c2.x, c2.y);
-        cur = st_rot2(cur, a2, c2.x, c2.y);
-        z(i + 3) = st_rot1(cur, a3, c3.x, c3.y);
-        cur = st_rot2(cur, a3, c3.x, c3.y);
-      }
-      for (; k < cnt; ++k) {
-        const int i = i0 + k;
-        const double2 cs = r[k];
         const TZ a2 = z(i + 1);
         z(i) = st_rot1(cur, a2, cs.x, cs.y);
         cur = st_rot2(cur, a2, cs.x, cs.y);
       }
-      z(i0 + cnt) = cur;
     }
+    if (back) z(i0 - cnt + 1) = cur;
+    else z(i0 + cnt) = cur;
     q += 1 + cnt;
   }
 }
