@@ -155,26 +155,40 @@ template <int NT, int RM> GS_DEV void form_q(zd* M, int ld, int n, const zd* tau
 // _hehessenberg!(Hermitian(M, :L)) (hessenberg.jl:27-60) on the n x n matrix
 // M in LDS (leading dimension ld; lower triangle read).  d (n), e (n-1)
 // receive the tridiagonal matrix, tau (n-1) the scalars; reflector p stays
-// in M[p+2 :, p].  v, w: LDS vectors of n elements.
+// in M[p+2 :, p].  vb: LDS for two reflector vectors of n elements
+// (alternating by step), w: LDS vector of n elements.  With two or more
+// wavefronts, wavefront 0 updates the trailing block's first column and
+// then forms the next reflector from it while the other wavefronts finish
+// the rank-2 update (the next step's input column is final once its own
+// update is done); the element operations are those of the step-by-step
+// order.
 template <int NT, class T>
-GS_DEV void he_tridiag(T* M, int ld, int n, T* tau, T* v, T* w, double* d, double* e, double* red) {
+GS_DEV void he_tridiag(T* M, int ld, int n, T* tau, T* vb, T* w, double* d, double* e, double* red) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   constexpr int NW = NT / 64;
+  constexpr bool OVL = NW >= 2;
+  // reflector of step p into the vector buffer of p's parity (wavefront 0)
+  auto reflect = [&](int p) {
+    const int m = n - 1 - p;
+    T* x = M + (p + 1) + (int64_t)p * ld;
+    T* v = vb + (p & 1) * n;
+    const T tp = he_reflector(x, m, lane);
+    // each lane reads back the tail entries it scaled (x[1 + lane + 64 j])
+    for (int k = 1 + lane; k < m; k += 64) v[k] = x[k];
+    if (lane == 0) {
+      v[0] = tfrom<T>(1.0);
+      tau[p] = tp;
+    }
+  };
   if (tid == 0 && n > 0) M[0] = tfrom<T>(tre(M[0]));
   __syncthreads();
+  if (OVL && wave == 0 && n >= 2) reflect(0);
   for (int p = 0; p <= n - 2; ++p) {
     const int m = n - 1 - p;
     T* x = M + (p + 1) + (int64_t)p * ld;
     T* B = M + (p + 1) + (int64_t)(p + 1) * ld;  // trailing block, m x m
-    if (wave == 0) {
-      const T tp = he_reflector(x, m, lane);
-      // each lane reads back the tail entries it scaled (x[1 + lane + 64 j])
-      for (int k = 1 + lane; k < m; k += 64) v[k] = x[k];
-      if (lane == 0) {
-        v[0] = tfrom<T>(1.0);
-        tau[p] = tp;
-      }
-    }
+    const T* v = vb + (p & 1) * n;
+    if (!OVL && wave == 0) reflect(p);
     __syncthreads();
     const T tp = tau[p];
     if (!tzero(tp)) {
@@ -197,22 +211,35 @@ GS_DEV void he_tridiag(T* M, int ld, int n, T* tau, T* v, T* w, double* d, doubl
       const T alpha = -(tp * 0.5) * dotwv;
       for (int k = tid; k < m; k += NT) w[k] = w[k] + alpha * v[k];
       __syncthreads();
-      // rank2update!(B, -1, v, w, Val(:L))
-      for (int c = wave; c < m; c += NW) {
+      // rank2update!(B, -1, v, w, Val(:L)); with OVL wavefront 0 takes
+      // column 0 alone, then the next reflector
+      auto upd = [&](int c) {
         const T vc = v[c], wc = w[c];
         T* col = B + (int64_t)c * ld;
         if (tzero(vc) && tzero(wc)) {
           if (lane == 0) col[c] = tfrom<T>(tre(col[c]));
-          continue;
+          return;
         }
         const T t1 = -tconj(wc), t2 = tconj(-vc);
         for (int r = c + lane; r < m; r += 64) {
-          const T upd = v[r] * t1 + w[r] * t2;
-          col[r] = (r == c) ? tfrom<T>(tre(col[r]) + tre(upd)) : col[r] + upd;
+          const T u = v[r] * t1 + w[r] * t2;
+          col[r] = (r == c) ? tfrom<T>(tre(col[r]) + tre(u)) : col[r] + u;
         }
+      };
+      if (OVL) {
+        if (wave == 0) {
+          upd(0);
+          if (p + 1 <= n - 2) reflect(p + 1);
+        } else {
+          // columns 1 .. m-1 over wavefronts 1 .. NW-1
+          for (int c = wave; c < m; c += NW - 1) upd(c);
+        }
+      } else {
+        for (int c = wave; c < m; c += NW) upd(c);
       }
-    } else if (tid == 0) {
-      B[0] = tfrom<T>(tre(B[0]));
+    } else {
+      if (tid == 0) B[0] = tfrom<T>(tre(B[0]));
+      if (OVL && wave == 0 && p + 1 <= n - 2) reflect(p + 1);
     }
     if (tid == 0) {
       e[p] = tre(x[0]);
@@ -316,8 +343,8 @@ template <class T, int NT, int RM> __global__ void __launch_bounds__(NT) heev_tq
   T* M = reinterpret_cast<T*>(smem);
   char* ub = reinterpret_cast<char*>(M + (size_t)ld * n);
   T* tau = reinterpret_cast<T*>(ub);
-  T* v = tau + n;
-  T* w = v + n;
+  T* v = tau + n;  // two reflector buffers
+  T* w = v + 2 * n;
   double* d = reinterpret_cast<double*>(ub + heev_union_bytes(n, sizeof(T)));
   double* e = d + n;
   double* red = e + n;

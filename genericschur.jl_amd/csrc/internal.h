@@ -214,9 +214,9 @@ struct SteqrArgs {
   int* info = nullptr;
   int* sweeps = nullptr;
 };
-// LDS shared by the reduction's vectors (tau, v, w) and the rotation log
+// LDS shared by the reduction's vectors (tau, two reflector buffers, w) and the rotation log
 constexpr size_t heev_union_bytes(int n, size_t es) {
-  return ((3 * (size_t)n * es > 56 * (size_t)n ? 3 * (size_t)n * es : 56 * (size_t)n) + 15) / 16 * 16;
+  return ((4 * (size_t)n * es > 56 * (size_t)n ? 4 * (size_t)n * es : 56 * (size_t)n) + 15) / 16 * 16;
 }
 constexpr int heev_max_n(bool cplx) { return cplx ? 96 : 128; }
 constexpr int steqr_max_n() { return 2048; }
