@@ -686,12 +686,37 @@ def _sort_host(w, V, sortby):
     return w, (V[:, p] if V is not None else None)
 
 
-def heev_batched(A, uplo="L", wantV=True, sortby="eigsortby", device=0, raise_on_info=True):
+_MRRR_WARNED = [False]
+
+
+def _check_alg(alg):
+    """The Hermitian / tridiagonal algorithm argument (GenericSchur.jl:
+    118-123, symtridiag.jl:14-29): "QRIteration" runs; "RobustRepresentations"
+    falls back to QR iteration with a one-time warning, as the reference does
+    when no MRRR implementation is registered (symtridiag.jl:21-28);
+    "DivideAndConquer" (symtri_dc.jl) is not provided by this library."""
+    if alg in (None, "QRIteration"):
+        return
+    if alg == "RobustRepresentations":
+        if not _MRRR_WARNED[0]:
+            import warnings
+            warnings.warn("GenericSchur does not currently provide alg=RobustRepresentations; "
+                          "falling back to QRIteration")
+            _MRRR_WARNED[0] = True
+        return
+    if alg == "DivideAndConquer":
+        raise NotImplementedError("alg=DivideAndConquer (symtri_dc.jl) is not provided by this library; "
+                                  "use QRIteration")
+    raise ArgumentError(f"unknown algorithm {alg!r}")
+
+
+def heev_batched(A, uplo="L", wantV=True, sortby="eigsortby", device=0, raise_on_info=True, alg="QRIteration"):
     """geigen! (wantV) / geigvals! with QRIteration (GenericSchur.jl:124-146)
     over a stack A (count, n, n) of Hermitian (complex) or real symmetric
     matrices, one GPU launch: returns (W (count, n), V (count, n, n) or None,
     info).  sortby: "eigsortby" (ascending, on the GPU), None, or a callable
     (applied on the host afterwards)."""
+    _check_alg(alg)
     A = np.asarray(A)
     if A.ndim != 3 or A.shape[1] != A.shape[2]:
         raise DimensionMismatch("A must be (count, n, n)")
@@ -720,21 +745,21 @@ def heev_batched(A, uplo="L", wantV=True, sortby="eigsortby", device=0, raise_on
     return W, V, info[:count]
 
 
-def geigen_hermitian(A, uplo="L", sortby="eigsortby", device=0):
+def geigen_hermitian(A, uplo="L", sortby="eigsortby", device=0, alg="QRIteration"):
     """geigen!(Hermitian(A, uplo), QRIteration(); sortby) (GenericSchur.jl:
     124-135): Eigen(values, vectors)."""
     A = np.asarray(A)
     _checksquare(A)
-    W, V, _ = heev_batched(A[None], uplo=uplo, wantV=True, sortby=sortby, device=device)
+    W, V, _ = heev_batched(A[None], uplo=uplo, wantV=True, sortby=sortby, device=device, alg=alg)
     return Eigen(W[0], V[0])
 
 
-def geigvals_hermitian(A, uplo="L", sortby="eigsortby", device=0):
+def geigvals_hermitian(A, uplo="L", sortby="eigsortby", device=0, alg="QRIteration"):
     """geigvals!(Hermitian(A, uplo), QRIteration(); sortby) (GenericSchur.jl:
     137-146)."""
     A = np.asarray(A)
     _checksquare(A)
-    W, _, _ = heev_batched(A[None], uplo=uplo, wantV=False, sortby=sortby, device=device)
+    W, _, _ = heev_batched(A[None], uplo=uplo, wantV=False, sortby=sortby, device=device, alg=alg)
     return W[0]
 
 
@@ -766,10 +791,11 @@ def steqr_(d, e, Z=None, device=0):
     return d, (z if z is not None else None), e1[:max(n - 1, 0)], int(info[0])
 
 
-def gschur_symtridiag(d, e, Z=None, device=0):
+def gschur_symtridiag(d, e, Z=None, device=0, alg="QRIteration"):
     """gschur!(H::Hessenberg{<:Any, <:SymTridiagonal}, Z) (symtridiag.jl:14-41)
     on the tridiagonal (d, e) with the transformation Z: Schur(diagm(values),
     Z updated, values) (T and Z empty when Z is None)."""
+    _check_alg(alg)
     w, z, _, info = steqr_(d, e, Z, device=device)
     if info:
         raise UnconvergedException(f"{info} eigenvalues failed to converge")
@@ -780,9 +806,10 @@ def gschur_symtridiag(d, e, Z=None, device=0):
     return Schur(T, z, w)
 
 
-def geigen_symtridiag(d, e, device=0):
+def geigen_symtridiag(d, e, device=0, alg="QRIteration"):
     """geigen!(A::SymTridiagonal, QRIteration()) (symtridiag.jl:43-49):
     Eigen(values ascending, vectors)."""
+    _check_alg(alg)
     n = np.asarray(d).reshape(-1).shape[0]
     w, V, _, info = steqr_(d, e, np.eye(n), device=device)
     if info:
@@ -791,9 +818,10 @@ def geigen_symtridiag(d, e, device=0):
     return Eigen(w2, V2)
 
 
-def geigvals_symtridiag(d, e, device=0):
+def geigvals_symtridiag(d, e, device=0, alg="QRIteration"):
     """geigvals!(A::SymTridiagonal, QRIteration()) (symtridiag.jl:51-55):
     the eigenvalues in the order the iteration leaves them."""
+    _check_alg(alg)
     w, _, _, info = steqr_(d, e, None, device=device)
     if info:
         raise UnconvergedException(f"{info} eigenvalues failed to converge")

@@ -170,3 +170,22 @@ def test_tuning_rows_format():
         bins = sorted((b["nlo"], b["nhi"]) for b in (table.get(key) or {}).get("bins", []))
         for (a0, b0), (a1, b1) in zip(bins, bins[1:]):
             assert a0 <= b0 < a1 <= b1
+
+
+def test_hermitian_algorithm_argument():
+    """alg mirrors the reference: QRIteration runs, RobustRepresentations
+    falls back to it with a one-time warning (symtridiag.jl:21-28), and
+    DivideAndConquer (symtri_dc.jl) is refused; checked before any GPU work."""
+    import warnings
+    from genericschur import api
+    api._check_alg("QRIteration")
+    api._MRRR_WARNED[0] = False
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        api._check_alg("RobustRepresentations")
+        api._check_alg("RobustRepresentations")
+    assert len(w) == 1
+    with pytest.raises(NotImplementedError):
+        api.geigen_symtridiag(np.ones(3), np.ones(2), alg="DivideAndConquer")
+    with pytest.raises(api.ArgumentError):
+        api._check_alg("Bisection")
