@@ -100,10 +100,13 @@ def main():
     ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--workload", type=int, default=2, choices=[2, 3, 4, 5, 6],
+    ap.add_argument("--workload", type=int, default=2, choices=[2, 3, 4, 5, 6, 7, 8],
                     help="2: batched complex (the headline line); 3: batched real mix; 4: one large real matrix "
                          "(blocked path vs plain kernel); 5: eigenvalue-loss forward + backward; 6: batched "
-                         "Hermitian eigensystems (QR iteration)")
+                         "Hermitian eigensystems (QR iteration); 7: one large Hermitian eigensystem; 8: the "
+                         "eigenvectors of one large real Schur factorization")
+    ap.add_argument("--herm-big-n", type=int, default=2048, help="order of the Hermitian matrix of workload 7")
+    ap.add_argument("--vec-n", type=int, default=4096, help="order of the Schur factorization of workload 8")
     ap.add_argument("--herm-n", type=int, default=64, help="order of the Hermitian matrices of workload 6")
     ap.add_argument("--herm-real", action="store_true", help="workload 6 on real symmetric matrices")
     ap.add_argument("--large-n", type=int, default=4096)
@@ -131,6 +134,10 @@ def main():
         return bench_workload5(args)
     if args.workload == 6:
         return bench_workload6(args)
+    if args.workload == 7:
+        return bench_workload7(args)
+    if args.workload == 8:
+        return bench_workload8(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (batches shard with no exchange; "
               "run one process per GPU on a slice of the batch)", file=sys.stderr)
@@ -639,6 +646,125 @@ def bench_workload5(args):
     if not args.no_cpu:
         out["cpu_baseline"] = _cpu_batched(mats, False, count)
         out["cpu_baseline"]["note"] = "forward (Schur) only: the reference has no eigenvalue rrule"
+    print(json.dumps(out), flush=True)
+    if not out["ok"]:
+        sys.exit(1)
+
+
+def bench_workload7(args):
+    """One large Hermitian eigensystem: geigen!(Hermitian(A)) with
+    QRIteration, values ascending and vectors, of one ComplexF64 G + G^H of
+    order --herm-big-n (--herm-real: Float64), device-resident; a step is one
+    solve (at most 3 timed)."""
+    import genericschur as gs
+    from genericschur import _lib
+    import torch
+    n = args.herm_big_n
+    cplx = not args.herm_real
+    rng = np.random.default_rng(args.seed)
+    G = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0)
+    Ah = G + G.conj().T
+    dt_ = np.complex128 if cplx else np.float64
+    dev = torch.device("cuda:0")
+    A = torch.from_numpy(np.asfortranarray(Ah, dtype=dt_).reshape(-1, order="F").view(np.float64)).to(dev)
+    V = torch.zeros_like(A)
+    W = torch.zeros(n, dtype=torch.float64, device=dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    gs.set_stream(stream.cuda_stream)
+    h = _lib.handle()
+    f = _lib.lib().gs_heev_z if cplx else _lib.lib().gs_heev_d
+    a = lambda t: ctypes.c_void_p(t.data_ptr())
+    flags = _lib.GS_WANTZ | _lib.GS_SORT | _lib.GS_DEVICE_PTRS
+
+    def step():
+        rc = f(h, 1, n, a(A), n, n * n, 0, a(W), a(V), n, n * n, a(info), flags)
+        _lib.check(rc, h, "gs_heev")
+
+    steps = max(1, min(args.steps, 3))
+    wall, kern_s = _timed(stream, step, steps, min(args.warmup, 1))
+    gs.set_stream(None)
+    kind = "ComplexF64 Hermitian" if cplx else "Float64 symmetric"
+    out = _common(args, steps / wall, "solves/s", 1e3 * wall / steps,
+                  f"one Hermitian eigensystem geigen!(Hermitian(A), QRIteration()) (values ascending and vectors): "
+                  f"one {kind} matrix G + G^H of order {n}, G Gaussian; seeded numpy",
+                  {"n": n, "complex": cplx, "seed": args.seed})
+    out["metric"] = "Hermitian eigensystem solves/s (one large matrix, QR iteration)"
+    out["steps"] = steps
+    Wh = W.cpu().numpy()
+    Vh = V.cpu().numpy().view(dt_).reshape(n, n).T
+    eps = np.finfo(np.float64).eps
+    idx = np.linspace(0, n - 1, 32).astype(int)
+    res = np.linalg.norm(Ah @ Vh[:, idx] - Vh[:, idx] * Wh[idx]) / np.linalg.norm(Ah) / (n * eps)
+    orth = np.abs(Vh[:, idx].conj().T @ Vh - np.eye(n)[idx]).max() / (n * eps)
+    out["check"] = {"columns": 32, "residual_over_n_eps": float(res), "orth_over_n_eps": float(orth),
+                    "info": int(info.cpu().item()), "sorted": bool(np.all(np.diff(Wh) >= 0))}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        nc = min(512, n)
+        Gc = rng.standard_normal((nc, nc)) + (1j * rng.standard_normal((nc, nc)) if cplx else 0)
+        tc = time.perf_counter()
+        pyoracle.heev(Gc + Gc.conj().T, "L")
+        dtc = time.perf_counter() - tc
+        est = dtc * (n / nc) ** 3
+        out["cpu_baseline"] = {"value": 1.0 / est, "unit": "solves/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle geigen! (C++ restatement) on one {nc}x{nc} matrix of the same kind, "
+                                         f"1 thread, {dtc:.2f}s, scaled by ({n}/{nc})^3 = {est:.1f}s per solve"}
+    c = out["check"]
+    out["ok"] = bool(c["info"] == 0 and c["sorted"] and c["residual_over_n_eps"] <= 50.0
+                     and c["orth_over_n_eps"] <= 50.0)
+    print(json.dumps(out), flush=True)
+    if not out["ok"]:
+        sys.exit(1)
+
+
+def bench_workload8(args):
+    """The eigenvectors of one large real Schur factorization: T, Z of the
+    large-matrix workload's construction at order --vec-n from the large path,
+    then geigvecs right and left (gs_trevc_d), device-resident; a step is the
+    right and the left vectors (at most 3 timed)."""
+    import genericschur as gs
+    from genericschur import _lib, gen
+    import torch
+    n = args.vec_n
+    dev = torch.device("cuda:0")
+    A = gen.workload4(seed=args.seed, n=n)
+    Ad = torch.from_numpy(np.asfortranarray(A).ravel(order="F")).to(dev)
+    T = torch.zeros_like(Ad)
+    Z = torch.zeros_like(Ad)
+    W = torch.zeros(2 * n, dtype=torch.float64, device=dev)
+    info = gs.gschur_device(n, Ad, T, Z, W, False)
+    gs.synchronize()
+    VR = torch.zeros(2 * n * n, dtype=torch.float64, device=dev)
+    VL = torch.zeros_like(VR)
+    stream = torch.cuda.Stream(device=dev)
+    gs.set_stream(stream.cuda_stream)
+    h = _lib.handle()
+    f = _lib.lib().gs_trevc_d
+    a = lambda t: ctypes.c_void_p(t.data_ptr())
+
+    def step():
+        for left, V in ((0, VR), (1, VL)):
+            rc = f(h, n, a(T), n, a(Z), n, a(V), n, left, 1, _lib.GS_DEVICE_PTRS | _lib.GS_ASYNC)
+            _lib.check(rc, h, "gs_trevc_d")
+
+    steps = max(1, min(args.steps, 3))
+    wall, kern_s = _timed(stream, step, steps, min(args.warmup, 1))
+    gs.set_stream(None)
+    out = _common(args, steps / wall, "solves/s", 1e3 * wall / steps,
+                  f"geigvecs (right and left, normalised) of one real Schur factorization of order {n}: T, Z of "
+                  f"Q (Lambda + U) Q^T from the large path", {"n": n, "seed": args.seed})
+    out["metric"] = "eigenvector sets/s (right + left, one large real Schur factorization)"
+    out["steps"] = steps
+    # residual of a sample of right vectors against A
+    eps = np.finfo(np.float64).eps
+    w = W.cpu().numpy().view(np.complex128)
+    Vr = VR.cpu().numpy().view(np.complex128).reshape(n, n).T
+    idx = np.linspace(0, n - 1, 32).astype(int)
+    res = np.linalg.norm(A @ Vr[:, idx] - Vr[:, idx] * w[idx]) / np.linalg.norm(A) / (n * eps)
+    out["check"] = {"columns": 32, "right_residual_over_n_eps": float(res), "schur_info": int(info)}
+    out["ok"] = bool(info == 0 and np.isfinite(res))
     print(json.dumps(out), flush=True)
     if not out["ok"]:
         sys.exit(1)
