@@ -332,3 +332,40 @@ def test_heev_above_2048(gs):
     assert np.abs(A @ E.vectors - E.vectors * E.values).max() <= 16 * n * EPS * scale
     assert np.abs(E.vectors.T @ E.vectors - np.eye(n)).max() <= 16 * n * EPS
     assert np.abs(np.linalg.eigvalsh(A) - E.values).max() <= 16 * n * EPS * scale
+
+
+def _revsort(x):
+    """test/wrappers.jl: revsort(λ::Real) = -λ; revsort(λ::Complex) = (-real, -imag)."""
+    return (-x.real, -x.imag) if np.iscomplexobj(x) else -x
+
+
+@pytest.mark.parametrize("kind", ["bare", "hermitian", "symmetric"])
+def test_wrappers_sortby(gs, kind):
+    """test/wrappers.jl's existence-and-sanity checks: eigen, eigvecs and
+    eigvals with sortby = eigsortby and with revsort agree with each other;
+    the reversed order reverses values and vectors (up to phases)."""
+    r = np.random.default_rng(75)
+    n = 24
+    if kind == "bare":
+        A = r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))
+        E = gs.eigen_(A)
+        Er = gs.eigen_(A, sortby=_revsort)
+        lam = gs.eigvals_(A)
+        lamr = gs.eigvals_(A, sortby=_revsort)
+        V, Vr, w, wr = E.vectors, Er.vectors, E.values, Er.values
+    else:
+        cplx = kind == "hermitian"
+        A = _herm(r, n, cplx)
+        E = gs.geigen_hermitian(A)
+        Er = gs.geigen_hermitian(A, sortby=_revsort)
+        lam = gs.geigvals_hermitian(A)
+        lamr = gs.geigvals_hermitian(A, sortby=_revsort)
+        V, Vr, w, wr = E.vectors, Er.vectors, E.values, Er.values
+    assert np.array_equal(lam, w) or np.abs(lam - w).max() <= 1e3 * n * EPS * np.abs(A).max()
+    assert np.abs(lamr - lam[::-1]).max() <= 1e3 * n * EPS * np.abs(A).max()
+    assert np.abs(wr - w[::-1]).max() <= 1e3 * n * EPS * np.abs(A).max()
+    for j in range(n):
+        a, b = V[:, j], Vr[:, n - 1 - j]
+        p = np.vdot(a, b)
+        assert abs(abs(p) - np.linalg.norm(a) * np.linalg.norm(b)) <= 1e-8, j
+    assert np.abs(A @ V - V * w).max() <= 1e3 * n * EPS * np.abs(A).max() * max(1.0, np.abs(V).max())
