@@ -1117,11 +1117,21 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
         le = cplx ? launch_zqrw(a, h->stream) : launch_dqrw(a, h->stream);
     }
   } else if (use_win) {
-    HIPCHK(h, h->bMeta.ensure(64));
+    HIPCHK(h, h->bMeta.ensure(128));
+    // Z log from order 768 up (unless GS_ZDIRECT; GS_ZLOG_TINY forces it for
+    // tests): the chain wavefronts then update H only, and the log is
+    // replayed onto Z's rows over the whole GPU after the sweeps (measured:
+    // order 1024 2.84 -> 2.50 s; orders 256 and 512 slower, 0.216 -> 0.242
+    // and 0.434 -> 0.471 s, the replay's row chains outweighing the chains
+    // saved)
+    const bool zlg = (kflags & GS_WANTZ) && !(flags & GS_ZDIRECT) && (n >= 768 || (flags & GS_ZLOG_TINY));
+    const int64_t lcap = zlg ? zlog_capacity((int)n, (flags & GS_ZLOG_TINY) != 0) : 0;
+    if (zlg) HIPCHK(h, h->bLog.ensure(16 * (size_t)lcap));
     struct Meta {
       int32_t order, n;
       int64_t off, woff;
-    } meta = {0, (int32_t)n, 0, 0};
+      int64_t loff[2], llen;
+    } meta = {0, (int32_t)n, 0, 0, {0, lcap}, 0};
     HIPCHK(h, hipMemcpyAsync(h->bMeta.p, &meta, sizeof meta, hipMemcpyHostToDevice, h->stream));
     SingleArgs g;
     g.n = n;
@@ -1158,12 +1168,16 @@ static int run_single(gs_handle* h, bool cplx, int64_t n, double* A, int64_t lda
       a.flags = kflags;
       a.maxiter = maxiter;
       a.maxinner = maxinner;
-      a.zlog = nullptr;
-      a.loff = nullptr;
-      a.llen = nullptr;
+      a.zlog = zlg ? static_cast<double*>(h->bLog.p) : nullptr;
+      a.loff = zlg ? reinterpret_cast<const int64_t*>(mp + offsetof(Meta, loff)) : nullptr;
+      a.llen = zlg ? reinterpret_cast<int64_t*>(mp + offsetof(Meta, llen)) : nullptr;
+      a.zlog_min = 0;
       // chain wavefronts in proportion to the lines a chunk updates (about
       // 2n rows and columns): GSCHUR_WIN_NT overrides (128 .. 1024)
       le = launch_zqrw_single(a, win_nt((int)n), h->stream);
+      if (le == hipSuccess && zlg)
+        le = launch_zlog_rows((int)n, dZ, n, static_cast<const double*>(h->bLog.p),
+                              reinterpret_cast<const int64_t*>(mp + offsetof(Meta, llen)), h->stream);
     }
   } else {
     SingleArgs a;

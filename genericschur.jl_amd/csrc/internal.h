@@ -105,6 +105,9 @@ size_t zhq_global_ws_doubles(int64_t n);
 // the windowed QR kernel with nt threads (128 .. 1024): one chasing
 // wavefront, nt / 64 - 1 chain wavefronts
 hipError_t launch_zqrw_single(const BatchArgs& a, int nt, hipStream_t s);
+// the Z log of one matrix (llen entries at lg) replayed onto Z's rows in
+// HBM (ld ld) over the whole GPU
+hipError_t launch_zlog_rows(int n, double* Z, int64_t ld, const double* lg, const int64_t* llen, hipStream_t s);
 hipError_t launch_dschur_global(const SingleArgs& a, int nt, hipStream_t s);
 
 // Eigenvector kernels (one workgroup per matrix).  T, Z at element offset

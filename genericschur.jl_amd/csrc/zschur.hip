@@ -1314,6 +1314,65 @@ template <int NT, bool PREP_ONLY> __global__ void __launch_bounds__(NT) zhq_glob
   }
 }
 
+// The Z log of one matrix above the LDS range replayed onto Z's rows in
+// HBM (ld ld), one row per thread over the whole GPU: zlog_row's
+// operations, taken in groups of eight steps whose row entries are read
+// before any of the group's writes (as zlog_kernel does in LDS), so that a
+// group waits for memory once.
+__global__ void __launch_bounds__(64) zlog_rows_kernel(int n, zd* Z, int64_t ld, const zd* __restrict__ lg,
+                                                       const int64_t* llen) {
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  if (r >= n) return;
+  zd* row = Z + r;
+  const int64_t len = *llen;
+  for (int64_t pos = 0; pos < len;) {
+    const zd h0 = lg[pos];
+    const int i1 = (int)h0.x, ie = (int)h0.y;
+    const zd ctt = lg[pos + 1];
+    const zd* e = lg + pos + 2;
+    zd c0 = row[(int64_t)(i1 - 1) * ld];
+    {  // first step, then the phase scaling when logged
+      zd x1 = row[(int64_t)i1 * ld];
+      zrot_right_fixed(c0, x1, e[0], e[1]);
+      row[(int64_t)(i1 - 1) * ld] = c0;
+      c0 = x1;
+      if (!(ctt.x == 0.0 && ctt.y == 0.0)) {
+        row[(int64_t)(i1 - 1) * ld] = zmul_fixed(row[(int64_t)(i1 - 1) * ld], ctt);
+        for (int c = i1 + 2; c <= ie; ++c) row[(int64_t)(c - 1) * ld] = zmul_fixed(row[(int64_t)(c - 1) * ld], ctt);
+      }
+    }
+    int k = i1 + 1;
+    for (; k + 7 < ie; k += 8) {
+      zd xa[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) xa[j] = row[(int64_t)(k + j) * ld];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        zrot_right_fixed(c0, xa[j], e[2 * (k + j - i1)], e[2 * (k + j - i1) + 1]);
+        row[(int64_t)(k + j - 1) * ld] = c0;
+        c0 = xa[j];
+      }
+    }
+    for (; k < ie; ++k) {
+      zd x1 = row[(int64_t)k * ld];
+      zrot_right_fixed(c0, x1, e[2 * (k - i1)], e[2 * (k - i1) + 1]);
+      row[(int64_t)(k - 1) * ld] = c0;
+      c0 = x1;
+    }
+    row[(int64_t)(ie - 1) * ld] = c0;
+    const zd t = e[2 * (ie - i1)];
+    if (!(t.x == 0.0 && t.y == 0.0)) row[(int64_t)(ie - 1) * ld] = zmul_fixed(row[(int64_t)(ie - 1) * ld], t);
+    pos += 3 + 2 * (int64_t)(ie - i1);
+  }
+}
+
+hipError_t launch_zlog_rows(int n, double* Z, int64_t ld, const double* lg, const int64_t* llen, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zlog_rows_kernel, dim3((n + 63) / 64), dim3(64), 0, s, n, reinterpret_cast<zd*>(Z), ld,
+                     reinterpret_cast<const zd*>(lg), llen);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------------
 // The same reduction and Q for orders from kZhbMinN up, over the whole GPU:
 // per column p the reflector (one wavefront), the left update's column dots

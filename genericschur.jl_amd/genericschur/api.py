@@ -126,7 +126,7 @@ def _raise_info(info, maxiter, n):
 
 
 def _single(A, wantZ, scale, maxiter, maxinner, hess=False, Z0=None, checksd=False, plain=False, device=0,
-            timing=False):
+            timing=False, extra=0):
     cplx = _kind(A)
     n = A.shape[0]
     dt = A.dtype
@@ -135,7 +135,7 @@ def _single(A, wantZ, scale, maxiter, maxinner, hess=False, Z0=None, checksd=Fal
     Z = np.zeros((n, n), dtype=dt, order="F") if Z0 is None else np.array(Z0, dtype=dt, order="F", copy=True)
     W = np.zeros(n, dtype=np.complex128)
     info = ctypes.c_int32(0)
-    flags = (GS_WANTZ if wantZ else 0) | (GS_SCALE if scale else 0)
+    flags = (GS_WANTZ if wantZ else 0) | (GS_SCALE if scale else 0) | extra
     if hess:
         flags |= GS_HESSENBERG
         if checksd:
@@ -157,18 +157,23 @@ def _single(A, wantZ, scale, maxiter, maxinner, hess=False, Z0=None, checksd=Fal
     return Schur(T, Zo, W)
 
 
-def gschur_(A, wantZ=True, scale=True, maxiter=None, maxinner=None, plain=False, device=0, timing=False):
+def gschur_(A, wantZ=True, scale=True, maxiter=None, maxinner=None, plain=False, device=0, timing=False,
+            zdirect=False, zlog_tiny=False):
     """gschur!(A) -> Schur (GenericSchur.jl:350-372 / :805-835).
 
     Real matrices above the LDS-resident orders take the blocked large-matrix
     path unless plain=True (the unblocked single-workgroup kernel).  With
     timing=True the large path records per-phase and per-GEMM durations
-    (genericschur._lib.last_launches())."""
+    (genericschur._lib.last_launches()).  Complex matrices above the
+    LDS-resident orders log Z's updates and apply the log after the sweeps;
+    zdirect=True updates Z during the sweeps instead, zlog_tiny=True makes
+    the log hold one sweep (comparisons; the results are the same)."""
     if _is_torch(A):
         raise TypeError("use gschur_device / gschur_batched_device for device tensors")
     A = np.asarray(A)
     _checksquare(A)
-    return _single(A, wantZ, scale, maxiter, maxinner, plain=plain, device=device, timing=timing)
+    extra = (_lib.GS_ZDIRECT if zdirect else 0) | (_lib.GS_ZLOG_TINY if zlog_tiny else 0)
+    return _single(A, wantZ, scale, maxiter, maxinner, plain=plain, device=device, timing=timing, extra=extra)
 
 
 def gschur_device(n, A, T, Z, W, complex_input, wantZ=True, scale=True, plain=False, timing=False, maxiter=None,

@@ -329,3 +329,25 @@ def test_batched_bad_offsets_rejected(gs):
         assert rc == _lib.GS_E_ARG and np.all(Ab == 7.0)
     msg = L.gs_last_error(h).decode()
     assert "offset" in msg or "overlap" in msg
+
+
+@pytest.mark.parametrize("n", [200, 800])
+def test_single_complex_zlog_matches_direct(gs, oracle, n):
+    """Complex orders from 768 up log Z's updates during the sweeps and
+    replay the log onto Z's rows over the whole GPU afterwards; Z updated
+    during the sweeps (the default below 768, zdirect above) and a log of
+    one sweep (zlog_tiny: the QR kernel applies the log itself before almost
+    every sweep, and the rest is replayed over the GPU) give the same T and Z
+    bit for bit, and the logged form passes the invariants."""
+    A = gen.ginibre_complex(n, gen.stream(31, 1, n))
+    S = gs.gschur(A)
+    Sd = gs.gschur(A, zdirect=True)
+    St = gs.gschur(A, zlog_tiny=True)
+    assert np.array_equal(S.T, Sd.T) and np.array_equal(S.T, St.T)
+    assert np.array_equal(S.Z, Sd.Z) and np.array_equal(S.Z, St.Z)
+    if n <= 200:
+        check_case(oracle, "ginibre", A, S, False, {})
+    else:
+        eps = np.finfo(np.float64).eps
+        assert checks.rel_backward(A, S.T, S.Z) <= 50 * n * eps
+        assert checks.rel_orth(S.Z) <= 50 * n * eps
