@@ -53,14 +53,16 @@ typedef struct gs_handle gs_handle;
                              /* (up to 80 KB per matrix) instead of the        */
                              /* default windowed kernel (H in HBM, a 32 x 32   */
                              /* chase window in LDS); same arithmetic           */
-#define GS_ZDIRECT 0x200     /* batched complex: apply each chunk's reflectors */
-                             /* to Z in HBM during the QR sweeps instead of    */
-                             /* logging them and applying the log to Z, held   */
-                             /* in LDS, after the sweeps; same arithmetic and  */
-                             /* order (comparisons)                            */
-#define GS_ZLOG_TINY 0x400   /* batched complex, testing: a Z log of one sweep */
-                             /* per matrix, so that the QR kernel applies the  */
-                             /* log to Z itself before almost every sweep      */
+#define GS_ZDIRECT 0x200     /* complex (batched, and single from order 768):  */
+                             /* apply each chunk's reflectors to Z in HBM      */
+                             /* during the QR sweeps instead of logging them   */
+                             /* and applying the log to Z after the sweeps     */
+                             /* (batched: Z in LDS; single: Z's rows over the  */
+                             /* GPU); same arithmetic and order (comparisons)  */
+#define GS_ZLOG_TINY 0x400   /* complex, testing: a Z log of one sweep per     */
+                             /* matrix, so that the QR kernel applies the log  */
+                             /* to Z itself before almost every sweep (single: */
+                             /* the log at any order above the LDS range)      */
 #define GS_SORT 0x800       /* Hermitian eigensolver: eigenvalues ascending, */
                              /* eigenvectors alike (sorteig! with eigsortby)  */
 #define GS_TIMING 0x80       /* record launch durations for gs_last_launches:  */
