@@ -8,15 +8,22 @@
 //   QL/QR iteration        symtridiag.jl:77-427 (steqr.h)
 //   ordering               sorteig!(λ, V, eigsortby)
 //
-// heev_kernel: one workgroup per matrix, the matrix resident in LDS.  The
-// upper-triangle form is reduced as the lower-triangle form of J A J (J the
-// reversal permutation): the reference's U reduction applies, at each step,
-// the same reflector as the L reduction of J A J with its tail in reverse
-// order, so d, e and Q = J Q_L J follow by reversing indices (the tail's
-// order only changes the rounding of its norm, which a parallel sum changes
-// anyway).  Then the QL/QR iteration with one producer lane and the other
-// wavefronts applying the rotation log to Q's rows, the stable sort and the
-// write-out.
+// Batches of LDS-resident orders run in three launches: heev_tq_kernel
+// (one workgroup per matrix, the matrix in LDS: the tridiagonalization and
+// Q), steqr_lanes_kernel (the QL/QR recurrence of 64 matrices per
+// wavefront, one per lane, each writing its rotations to a record stream in
+// HBM) and heev_apply_kernel (the streams replayed onto Q's rows in LDS, an
+// in-place finish for a stream that overflowed, the stable sort, the
+// write-out).  The upper-triangle form is reduced as the lower-triangle form
+// of J A J (J the reversal permutation): the reference's U reduction
+// applies, at each step, the same reflector as the L reduction of J A J with
+// its tail in reverse order, so d, e and Q = J Q_L J follow by reversing
+// indices (the tail's order only changes the rounding of its norm, which a
+// parallel sum changes anyway).
+//
+// Orders above the LDS range run one matrix at a time over the whole GPU
+// (the hb_* kernels: the reduction and Q as grid launches per column, the
+// recurrence's record stream replayed onto Q's rows in rounds).
 //
 // steqr_kernel: the QL/QR iteration alone on (d, e) with Z in global memory
 // (gschur!(H, Z) / _gschur!(A::SymTridiagonal, QRIteration(), Z)).
