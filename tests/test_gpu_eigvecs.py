@@ -218,3 +218,17 @@ def test_batched_vectors_with_large_order(gs, oracle):
                 Vg = V[o:o + m * m].reshape((m, m), order="F")
                 Vo = oracle.eigvecs(T, Z, left=bool(left))
                 assert np.abs(Vg - Vo).max() <= 1e-7, (cplx, left, m)
+
+
+@pytest.mark.parametrize("n,cplx", [(50, False), (50, True), (1100, False), (1100, True)])
+def test_vectors_unnormalized(gs, oracle, n, cplx):
+    """normalize = 0: the solutions back-transformed and scaled to largest
+    entry 1 (max abs, or abs1 for the real pairs and the complex kernel),
+    before _enormalize! — the form eigen! feeds to the balancing
+    back-transform (pirates.jl:86-110).  Small orders on the one-workgroup
+    kernels, 1100 on the whole-GPU path; entrywise against the oracle."""
+    A, T, Z, w = _schur_lapack(n, cplx, 500 + n)
+    for left in (False, True):
+        Vg = gs.geigvecs(S(T, Z, w), left=left, normalize=False)
+        Vo = oracle.eigvecs(T, Z, left=left, normalize=False)
+        assert np.abs(Vg - Vo).max() <= (1e-9 if n <= 64 else 1e-7), left
