@@ -169,3 +169,29 @@ def test_eigen_matches_oracle(gs, oracle, cplx, n):
         # A x = lambda x for the returned right vectors
         r = np.linalg.norm(A @ E.vectors - E.vectors * E.values, axis=0)
         assert np.all(r <= 1e3 * n * EPS * np.linalg.norm(A, 2) * np.maximum(1.0, cond))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_eigen_large_order(gs, cplx):
+    """eigen!(A; jvl, jce) above order 1024, where every piece takes its
+    whole-GPU path (the large real Schur path or the complex windowed path,
+    the order > 1024 eigenvector solves, balancing and its back-transforms):
+    unit right and left vectors, A x = lambda x and y^H A = lambda y^H within
+    the conditions the call returns, and the eigenvalues against LAPACK's."""
+    n = 1100
+    rng = gen.stream(8, 92, n + int(cplx))
+    A = rng.standard_normal((n, n))
+    if cplx:
+        A = A + 1j * rng.standard_normal((n, n))
+    print(f"case eigen n={n} cplx={cplx}", flush=True)
+    E = gs.eigen_(A, jvl=True, jce=True)
+    cond = 1.0 / E.rconde
+    nrm = np.linalg.norm(A)  # Frobenius: bounds ||A||_2
+    for V in (E.vectors, E.vectorsl):
+        assert np.all(np.abs(np.linalg.norm(V, axis=0) - 1) < 1e-12)
+    r = np.linalg.norm(A @ E.vectors - E.vectors * E.values, axis=0)
+    assert np.all(r <= 1e3 * n * EPS * nrm * np.maximum(1.0, cond))
+    rl = np.linalg.norm(E.vectorsl.conj().T @ A - E.values[:, None] * E.vectorsl.conj().T, axis=1)
+    assert np.all(rl <= 1e3 * n * EPS * nrm * np.maximum(1.0, cond))
+    perm, d = checks.match_eigenvalues(E.values, np.linalg.eigvals(A))
+    assert np.all(d <= 1e3 * n * EPS * nrm * np.maximum(1.0, cond))
