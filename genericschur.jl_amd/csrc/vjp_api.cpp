@@ -29,7 +29,7 @@ extern "C" int gs_eig_vjp_batched_d(gs_handle* h, int64_t count, const int32_t* 
     woff[i] = wtot;
     wtot += n[i];
   }
-  if (nmax > dtrevc_max_n()) return handle_fail(h, GS_E_UNSUPPORTED, "order above the VJP kernel limit");
+  if (nmax > trevc_big_max_n()) return handle_fail(h, GS_E_UNSUPPORTED, "order above the VJP kernel limit");
   std::lock_guard<std::mutex> lk(handle_mutex(h));
   hipError_t e = hipSetDevice(handle_device(h));
   if (e != hipSuccess) return handle_fail_hip(h, e, "hipSetDevice");
@@ -91,12 +91,39 @@ extern "C" int gs_eig_vjp_batched_d(gs_handle* h, int64_t count, const int32_t* 
   t.Z = dZ;
   t.normalize = 1;
   t.ldt = t.ldz = t.ldv = 0;
-  t.V = static_cast<double*>(bVR.p);
-  t.left = 0;
-  if ((e = launch_dtrevc(t, nmax, s)) != hipSuccess) return handle_fail_hip(h, e, "eigenvector launch");
-  t.V = static_cast<double*>(bVL.p);
-  t.left = 1;
-  if ((e = launch_dtrevc(t, nmax, s)) != hipSuccess) return handle_fail_hip(h, e, "eigenvector launch");
+  // right and left eigenvectors: matrices above the one-workgroup kernels'
+  // orders (first in the descending order) one at a time on the whole-GPU
+  // path, the rest in one launch per side
+  int64_t nbigv = 0;
+  while (nbigv < count && n[order[nbigv]] > dtrevc_max_n()) ++nbigv;
+  if (nbigv > 0) {
+    DevBuf& ws = handle_buf(h, 13);
+    if ((e = ws.ensure(dtrevc_big_ws_bytes(nmax))) != hipSuccess) return handle_fail_hip(h, e, "hipMalloc");
+    for (int64_t b = 0; b < nbigv; ++b) {
+      const int32_t m = order[b];
+      TrevcArgs one = t;
+      one.T = dT + off[m];
+      one.Z = dZ + off[m];
+      one.ldt = one.ldz = one.ldv = n[m];
+      for (int left = 0; left < 2; ++left) {
+        one.left = left;
+        one.V = static_cast<double*>(left ? bVL.p : bVR.p) + 2 * off[m];
+        if ((e = launch_dtrevc_big(one, n[m], ws.p, s)) != hipSuccess)
+          return handle_fail_hip(h, e, "eigenvector launch");
+      }
+    }
+  }
+  if (nbigv < count) {
+    t.order = d_order + nbigv;
+    t.count = (int)(count - nbigv);
+    const int nsmall = n[order[nbigv]];
+    t.V = static_cast<double*>(bVR.p);
+    t.left = 0;
+    if ((e = launch_dtrevc(t, nsmall, s)) != hipSuccess) return handle_fail_hip(h, e, "eigenvector launch");
+    t.V = static_cast<double*>(bVL.p);
+    t.left = 1;
+    if ((e = launch_dtrevc(t, nsmall, s)) != hipSuccess) return handle_fail_hip(h, e, "eigenvector launch");
+  }
   // orders whose two factors fit LDS: one workgroup per matrix; larger
   // ones (first in the descending order): c_k and the planes of each
   // matrix, then Abar = P Q^T as one FP64 MFMA GEMM of n x n x 2n

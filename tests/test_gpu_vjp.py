@@ -193,3 +193,24 @@ def test_large_orders_gemm_assembly(gs, oracle):
         worst = max(worst, abs(fd - an) / max(1.0, abs(an)))
     print("n=100 worst FD relative error", worst, flush=True)
     assert worst <= 1e-6
+
+
+def test_order_above_1024(gs, oracle):
+    """An order above 1024: the right and left eigenvectors recomputed on the
+    whole-GPU path, Abar as the FP64 GEMM of the split factors; the trace
+    invariant (gbar = 1 gives the identity) and the oracle-eigenvector
+    formula."""
+    n = 1100
+    A = gen.ginibre_real(n, gen.stream(12, 1, n))
+    S = gs.gschur(A)
+    print(f"case vjp n={n}", flush=True)
+    Ab = _vjp(gs, [A], [S], [np.ones(n, dtype=complex)])[0]
+    assert np.abs(Ab - np.eye(n)).max() <= 1e-8
+    rng = np.random.default_rng(13)
+    g = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    Ab = _vjp(gs, [A], [S], [g])[0]
+    X = oracle.eigvecs(S.T, S.Z, left=False)
+    Y = oracle.eigvecs(S.T, S.Z, left=True)
+    d = np.sum(Y.conj() * X, axis=0)
+    ref = np.real((Y * (g / np.conj(d))) @ X.conj().T)
+    assert np.abs(Ab - ref).max() <= 1e-7 * max(1.0, np.abs(ref).max())
