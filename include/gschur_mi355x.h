@@ -207,9 +207,10 @@ int gs_trevc_batched_d(gs_handle* h, int64_t count, const int32_t* n, const int6
  * = Re( X^{-H} diag(gbar) X^H ), with the right (x_k) and left (y_k)
  * eigenvectors recomputed from T and Z on the device.  Abar is real at
  * off[i].  info[i] = 0, or 1 when some y_k^H x_k vanished (defective).
- * Orders up to 1024: up to 70 the assembly runs with both eigenvector
+ * Orders up to 4096: up to 70 the assembly runs with both eigenvector
  * factors in LDS, above as the FP64 MFMA GEMM Abar = P Q^T with
- * P = [Re(Y C) | Im(Y C)], Q = [Re X | Im X], C = diag(c_k). */
+ * P = [Re(Y C) | Im(Y C)], Q = [Re X | Im X], C = diag(c_k); above 1024
+ * the eigenvectors are recomputed one matrix at a time over the GPU. */
 int gs_eig_vjp_batched_d(gs_handle* h, int64_t count, const int32_t* n, const int64_t* off, const double* T,
                          const double* Z, const double* gbar, double* Abar, int32_t* info, int flags);
 
