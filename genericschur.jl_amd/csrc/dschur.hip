@@ -178,19 +178,41 @@ template <int NT, int RM = 0> GS_DEV void d_hessenberg(double* H, int64_t ld, in
     for (int j = p + 1 + tid; j < n; j += NT) wk[j] = t_ * (H[(p + 1) + (int64_t)j * ld] + wk[j]);
     __syncthreads();
     if constexpr (RM > 0) {
-      double vr[RM];
-#pragma unroll
-      for (int q = 0; q < RM; ++q) {
-        const int r = p + 1 + lane + 64 * q;
-        vr[q] = (r < n) ? ((r == p + 1) ? 1.0 : H[r + p * ld]) : 0.0;
-      }
-      #pragma unroll 4
-      for (int j = p + 1 + wave; j < n; j += NT / 64) {
-        const double va = wk[j];
-#pragma unroll
-        for (int q = 0; q < RM; ++q) {
-          const int r = p + 1 + lane + 64 * q;
-          if (r < n) H[r + (int64_t)j * ld] -= va * vr[q];
+      // The left update fused with the right update's dot products: the
+      // thread that sums row r over columns c >= p + 2 applies
+      // H[r, c] -= va_c v_r to each element as it reads it (rows r >= p + 1),
+      // so the trailing block is read once for both; the lane that finishes
+      // x_r applies the left update to H[r, p + 1] itself.  x_r goes to
+      // wk[n + r], since wk[0 .. n) still holds the va_c.
+      double* xw = wk + n;
+      const int s4 = tid & 3, il = tid >> 2;
+      const int seg = (n - r0 + 3) >> 2;
+      const int ks = r0 + s4 * seg, ke = min(ks + seg, n);
+      for (int ib = 0; ib < n; ib += NT / 4) {
+        const int r = ib + il;
+        double acc = 0.0;
+        if (r < n) {
+          if (r >= p + 1) {
+            const double vr = (r == p + 1) ? 1.0 : H[r + p * ld];
+#pragma unroll 4
+            for (int c = ks; c < ke; ++c) {
+              const double h = H[r + (int64_t)c * ld] - wk[c] * vr;
+              H[r + (int64_t)c * ld] = h;
+              acc += h * H[c + p * ld];
+            }
+          } else {
+#pragma unroll 4
+            for (int c = ks; c < ke; ++c) acc += H[r + (int64_t)c * ld] * H[c + p * ld];
+          }
+        }
+        acc += __shfl_xor(acc, 1, 64);
+        acc += __shfl_xor(acc, 2, 64);
+        if (s4 == 0 && r < n) {
+          double h1 = H[r + (int64_t)(p + 1) * ld];
+          if (r >= p + 1) h1 = h1 - wk[p + 1] * ((r == p + 1) ? 1.0 : H[r + p * ld]);
+          const double x = h1 + acc;
+          H[r + (int64_t)(p + 1) * ld] = h1 - t_ * x;
+          xw[r] = -t_ * x;
         }
       }
     } else {
@@ -202,17 +224,17 @@ template <int NT, int RM = 0> GS_DEV void d_hessenberg(double* H, int64_t ld, in
           H[r + (int64_t)j * ld] = (r == p + 1) ? h - va : h - va * H[r + p * ld];
         }
       }
-    }
-    __syncthreads();
-    // rmul!(A[:, p+1:n], H, x) with BLAS ger for the rank-1 part:
-    // x_r = H[r, p+1] + sum_c H[r, c] v_c; H[r, p+1] -= tau x_r;
-    // H[r, c] += (-tau x_r) v_c
-    ddots4<NT>(0, n, r0, n, [&](int c, int r) { return H[r + (int64_t)c * ld] * H[c + p * ld]; }, wk);
-    __syncthreads();
-    for (int r = tid; r < n; r += NT) {
-      const double x = H[r + (int64_t)(p + 1) * ld] + wk[r];
-      H[r + (int64_t)(p + 1) * ld] -= t_ * x;
-      wk[r] = -t_ * x;
+      __syncthreads();
+      // rmul!(A[:, p+1:n], H, x) with BLAS ger for the rank-1 part:
+      // x_r = H[r, p+1] + sum_c H[r, c] v_c; H[r, p+1] -= tau x_r;
+      // H[r, c] += (-tau x_r) v_c
+      ddots4<NT>(0, n, r0, n, [&](int c, int r) { return H[r + (int64_t)c * ld] * H[c + p * ld]; }, wk);
+      __syncthreads();
+      for (int r = tid; r < n; r += NT) {
+        const double x = H[r + (int64_t)(p + 1) * ld] + wk[r];
+        H[r + (int64_t)(p + 1) * ld] -= t_ * x;
+        wk[r] = -t_ * x;
+      }
     }
     __syncthreads();
     if constexpr (RM > 0) {
@@ -221,7 +243,7 @@ template <int NT, int RM = 0> GS_DEV void d_hessenberg(double* H, int64_t ld, in
       } else {
         double xr[RM];
 #pragma unroll
-        for (int q = 0; q < RM; ++q) xr[q] = (lane + 64 * q < n) ? wk[lane + 64 * q] : 0.0;
+        for (int q = 0; q < RM; ++q) xr[q] = (lane + 64 * q < n) ? wk[n + lane + 64 * q] : 0.0;
         constexpr int NW = OVL ? NT / 64 - 1 : NT / 64;
 #pragma unroll 4
         for (int c = r0 + wave - (OVL ? 1 : 0); c < n; c += NW) {
@@ -788,7 +810,7 @@ template <int NT> __global__ void __launch_bounds__(NT) dhq_kernel(BatchArgs a) 
   double* H = smem;
   double* tau = H + ld * n;
   double* wk = tau + n;
-  double* red = wk + n;
+  double* red = wk + 2 * n;
   const double* A = a.A + o;
   double* T = a.T + o;
   double* Z = a.Z + o;
@@ -1069,7 +1091,7 @@ hipError_t launch_dlog(const BatchArgs& a, int nmax, hipStream_t s) {
 
 size_t dhq_lds_bytes(int n, int nt) {
   const size_t ld = (size_t)(n | 1);
-  return 8 * (ld * n + 2 * (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
+  return 8 * (ld * n + 3 * (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
 }
 size_t dqr_lds_bytes(int n, int nt) {
   return 8 * ((size_t)BandLayout<3>::size(n) + 3 * (size_t)(n > 0 ? n : 1)) + 4 * (size_t)(nt / 64 + 1);

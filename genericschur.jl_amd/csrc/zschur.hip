@@ -141,20 +141,45 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
     zdots4e<NT>(p + 1, n, r0, n, [&](int r, int j) { return zconj(H[r + p * ld]) * H[r + (int64_t)j * ld]; },
                 [&](int j, zd acc) { wk[j] = ct * (H[(p + 1) + (int64_t)j * ld] + acc); });
     __syncthreads();
+    const zd mt = -t_;
     if constexpr (RM > 0) {
-      zd vr[RM];
-#pragma unroll
-      for (int q = 0; q < RM; ++q) {
-        const int r = p + 1 + lane + 64 * q;
-        vr[q] = (r < n) ? ((r == p + 1) ? zr(1.0) : H[r + p * ld]) : zr(0.0);
-      }
+      // The left update fused with the right update's dot products: the
+      // thread that sums row r over columns c >= p + 2 applies
+      // H[r, c] -= va_c v_r to each element as it reads it (rows r >= p + 1),
+      // so the trailing block is read once for both; the lane that finishes
+      // x_r applies the left update to H[r, p + 1] itself.  x_r goes to
+      // wk[n + r], since wk[0 .. n) still holds the va_c.
+      zd* xw = wk + n;
+      const int s4 = tid & 3, il = tid >> 2;
+      const int seg = (n - r0 + 3) >> 2;
+      const int ks = r0 + s4 * seg, ke = min(ks + seg, n);
+      for (int ib = 0; ib < n; ib += NT / 4) {
+        const int r = ib + il;
+        zd acc = zr(0.0);
+        if (r < n) {
+          if (r >= p + 1) {
+            const zd vr = (r == p + 1) ? zr(1.0) : H[r + p * ld];
 #pragma unroll 4
-      for (int j = p + 1 + wave; j < n; j += NT / 64) {
-        const zd va = wk[j];
-#pragma unroll
-        for (int q = 0; q < RM; ++q) {
-          const int r = p + 1 + lane + 64 * q;
-          if (r < n) H[r + (int64_t)j * ld] = H[r + (int64_t)j * ld] - va * vr[q];
+            for (int c = ks; c < ke; ++c) {
+              const zd h = H[r + (int64_t)c * ld] - wk[c] * vr;
+              H[r + (int64_t)c * ld] = h;
+              acc = acc + h * H[c + p * ld];
+            }
+          } else {
+#pragma unroll 4
+            for (int c = ks; c < ke; ++c) acc = acc + H[r + (int64_t)c * ld] * H[c + p * ld];
+          }
+        }
+        acc.x += __shfl_xor(acc.x, 1, 64);
+        acc.y += __shfl_xor(acc.y, 1, 64);
+        acc.x += __shfl_xor(acc.x, 2, 64);
+        acc.y += __shfl_xor(acc.y, 2, 64);
+        if (s4 == 0 && r < n) {
+          zd h1 = H[r + (int64_t)(p + 1) * ld];
+          if (r >= p + 1) h1 = h1 - wk[p + 1] * ((r == p + 1) ? zr(1.0) : H[r + p * ld]);
+          const zd x = h1 + acc;
+          H[r + (int64_t)(p + 1) * ld] = h1 - t_ * x;
+          xw[r] = x * mt;
         }
       }
     } else {
@@ -166,19 +191,18 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
           H[r + (int64_t)j * ld] = (r == p + 1) ? h - va : h - va * H[r + p * ld];
         }
       }
+      __syncthreads();
+      // rmul!(A[:, p+1:n], H, x)     (householder.jl:140-155, util.jl:103-114):
+      // x_r = H[r, p+1] + sum_c H[r, c] v_c; H[r, p+1] -= tau x_r;
+      // H[r, c] += (-tau x_r) conj(v_c) (the dot products leave column p+1
+      // alone, so the lane holding x_r's sum finishes it at once)
+      zdots4e<NT>(0, n, r0, n, [&](int c, int r) { return H[r + (int64_t)c * ld] * H[c + p * ld]; },
+                  [&](int r, zd acc) {
+                    const zd x = H[r + (int64_t)(p + 1) * ld] + acc;
+                    H[r + (int64_t)(p + 1) * ld] = H[r + (int64_t)(p + 1) * ld] - t_ * x;
+                    wk[r] = x * mt;
+                  });
     }
-    __syncthreads();
-    // rmul!(A[:, p+1:n], H, x)     (householder.jl:140-155, util.jl:103-114):
-    // x_r = H[r, p+1] + sum_c H[r, c] v_c; H[r, p+1] -= tau x_r;
-    // H[r, c] += (-tau x_r) conj(v_c) (the dot products leave column p+1
-    // alone, so the lane holding x_r's sum finishes it at once)
-    const zd mt = -t_;
-    zdots4e<NT>(0, n, r0, n, [&](int c, int r) { return H[r + (int64_t)c * ld] * H[c + p * ld]; },
-                [&](int r, zd acc) {
-                  const zd x = H[r + (int64_t)(p + 1) * ld] + acc;
-                  H[r + (int64_t)(p + 1) * ld] = H[r + (int64_t)(p + 1) * ld] - t_ * x;
-                  wk[r] = x * mt;
-                });
     __syncthreads();
     if constexpr (RM > 0) {
       if (OVL && wave == 0) {
@@ -186,7 +210,7 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
       } else {
       zd xr[RM];
 #pragma unroll
-      for (int q = 0; q < RM; ++q) xr[q] = (lane + 64 * q < n) ? wk[lane + 64 * q] : zr(0.0);
+      for (int q = 0; q < RM; ++q) xr[q] = (lane + 64 * q < n) ? wk[n + lane + 64 * q] : zr(0.0);
       constexpr int NW = OVL ? NT / 64 - 1 : NT / 64;
 #pragma unroll 4
       for (int c = r0 + wave - (OVL ? 1 : 0); c < n; c += NW) {
@@ -1058,7 +1082,7 @@ template <int NT> __global__ void __launch_bounds__(NT) zhq_kernel(BatchArgs a) 
   zd* H = reinterpret_cast<zd*>(smem);
   zd* tau = H + ld * n;
   zd* wk = tau + n;
-  double* red = reinterpret_cast<double*>(wk + n);
+  double* red = reinterpret_cast<double*>(wk + 2 * n);
   const zd* A = reinterpret_cast<const zd*>(a.A) + o;
   zd* T = reinterpret_cast<zd*>(a.T) + o;
   zd* Z = reinterpret_cast<zd*>(a.Z) + o;
@@ -1587,7 +1611,7 @@ hipError_t launch_zhq_global(const SingleArgs& a, double* scl, hipStream_t s) {
 // ------------------------------------------------------------------------
 size_t zhq_lds_bytes(int n, int nt) {
   const size_t ld = (size_t)(n | 1);
-  return 16 * (ld * n + 2 * (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
+  return 16 * (ld * n + 3 * (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
 }
 size_t zqr_lds_bytes(int n, int nt) {
   return 16 * ((size_t)BandLayout<2>::size(n) + 2 * (size_t)(n > 0 ? n : 1)) + 4 * (size_t)(nt / 64 + 1);
