@@ -110,6 +110,19 @@ GS_DEV void z_hess_reflect(zd* H, int64_t ld, int n, int p, zd* tau) {
 }
 
 // ------------------------------------------------------------------------
+// LDS bytes of the batched Hessenberg kernel for one matrix of order n:
+// H (leading dimension n | 1), tau, two work vectors, and, when they fit in
+// the 160 KiB of a CU, four partial-sum vectors for the fused left update
+// and right dot products (zhq_colwise); then the reduction slots.
+__host__ __device__ inline size_t zhq_need_bytes(int n, int nt, bool cw) {
+  const size_t ld = (size_t)(n | 1), m = (size_t)(n > 0 ? n : 1);
+  return 16 * (ld * (size_t)n + 3 * m + (cw ? 4 * m : 0)) + 8 * (size_t)(nt / 64 + 1);
+}
+// decided for the largest workgroup (1024 threads), so that every
+// workgroup size takes the same path for a given order
+__host__ __device__ inline bool zhq_colwise(int n) { return zhq_need_bytes(n, 1024, true) <= 160 * 1024; }
+
+// ------------------------------------------------------------------------
 // Hessenberg reduction in place (hessenberg.jl:3-17), full layout, 0-based.
 // Reflector p is stored below the subdiagonal of column p, tau[p] its scalar.
 // RM > 0: n <= 64 RM (LDS-resident matrices), so each lane keeps the
@@ -142,7 +155,49 @@ template <int NT, int RM = 0> GS_DEV void z_hessenberg(zd* H, int64_t ld, int n,
                 [&](int j, zd acc) { wk[j] = ct * (H[(p + 1) + (int64_t)j * ld] + acc); });
     __syncthreads();
     const zd mt = -t_;
-    if constexpr (RM > 0) {
+    if (RM > 0 && zhq_colwise(n)) {
+      // The left update fused with the right update's dot products, by
+      // units of (column class g, row block h): columns c = r0 + g + 4 k
+      // (g < 4), rows r = lane + 64 h (h < 2 above n = 64, else h = 0),
+      // with v_r in a register and each column's va_c and v_c uniform
+      // loads.  Per element: one read and one write of H.  Unit (g, h)
+      // sums its rows over its columns in increasing c into
+      // wk[2 n + g n + r]; the four classes are then added in class order,
+      // so the result does not depend on the workgroup size.
+      constexpr int NWA = NT / 64;
+      zd* xw = wk + n;
+      zd* xp = wk + 2 * n;
+      const int nunits = n > 64 ? 8 : 4;
+      for (int u = wave; u < nunits; u += NWA) {
+        const int g = u & 3, r = lane + 64 * (u >> 2);
+        const bool upd = r < n && r >= p + 1;
+        const zd vr = (r == p + 1) ? zr(1.0) : (upd ? H[r + p * ld] : zr(0.0));
+        zd xa = zr(0.0);
+        if (r < n) {
+#pragma unroll 4
+          for (int c = r0 + g; c < n; c += 4) {
+            const zd va = wk[c];
+            const zd vc = H[c + p * ld];
+            zd h = H[r + (int64_t)c * ld];
+            if (upd) {
+              h = h - va * vr;
+              H[r + (int64_t)c * ld] = h;
+            }
+            xa = xa + h * vc;
+          }
+          xp[g * n + r] = xa;
+        }
+      }
+      __syncthreads();
+      for (int r = tid; r < n; r += NT) {
+        const zd acc = ((xp[r] + xp[n + r]) + xp[2 * n + r]) + xp[3 * n + r];
+        zd h1 = H[r + (int64_t)(p + 1) * ld];
+        if (r >= p + 1) h1 = h1 - wk[p + 1] * ((r == p + 1) ? zr(1.0) : H[r + p * ld]);
+        const zd x = h1 + acc;
+        H[r + (int64_t)(p + 1) * ld] = h1 - t_ * x;
+        xw[r] = x * mt;
+      }
+    } else if constexpr (RM > 0) {
       // The left update fused with the right update's dot products: the
       // thread that sums row r over columns c >= p + 2 applies
       // H[r, c] -= va_c v_r to each element as it reads it (rows r >= p + 1),
@@ -1082,7 +1137,7 @@ template <int NT> __global__ void __launch_bounds__(NT) zhq_kernel(BatchArgs a) 
   zd* H = reinterpret_cast<zd*>(smem);
   zd* tau = H + ld * n;
   zd* wk = tau + n;
-  double* red = reinterpret_cast<double*>(wk + 2 * n);
+  double* red = reinterpret_cast<double*>(wk + 2 * n + (zhq_colwise(n) ? 4 * n : 0));
   const zd* A = reinterpret_cast<const zd*>(a.A) + o;
   zd* T = reinterpret_cast<zd*>(a.T) + o;
   zd* Z = reinterpret_cast<zd*>(a.Z) + o;
@@ -1610,8 +1665,12 @@ hipError_t launch_zhq_global(const SingleArgs& a, double* scl, hipStream_t s) {
 
 // ------------------------------------------------------------------------
 size_t zhq_lds_bytes(int n, int nt) {
-  const size_t ld = (size_t)(n | 1);
-  return 16 * (ld * n + 3 * (size_t)(n > 0 ? n : 1)) + 8 * (size_t)(nt / 64 + 1);
+  // the most any order up to n needs (the partial sums drop out above the
+  // orders where they fit)
+  size_t b = zhq_need_bytes(n, nt, false);
+  for (int k = 1; k <= n; ++k)
+    if (zhq_colwise(k)) b = std::max(b, zhq_need_bytes(k, nt, true));
+  return b;
 }
 size_t zqr_lds_bytes(int n, int nt) {
   return 16 * ((size_t)BandLayout<2>::size(n) + 2 * (size_t)(n > 0 ? n : 1)) + 4 * (size_t)(nt / 64 + 1);
