@@ -189,3 +189,19 @@ def test_hermitian_algorithm_argument():
         api.geigen_symtridiag(np.ones(3), np.ones(2), alg="DivideAndConquer")
     with pytest.raises(api.ArgumentError):
         api._check_alg("Bisection")
+
+
+def test_lds_resident_order_limits():
+    """The largest orders whose Hessenberg phase fits the 160 KiB of LDS of
+    a CU: H with leading dimension n | 1 (odd), tau and two work vectors.
+    Complex: 16 (99 * 99 + 3 * 99) + reduction slots <= 163840 < 16 (101 * 100
+    + 3 * 100); real: 8 (141 * 141 + 3 * 141) + slots <= 163840 <
+    8 (143 * 142 + 3 * 142).  The Hessenberg pass's partial sums only take
+    LDS at orders where they fit, so they do not lower either limit."""
+    def need(n, es):
+        ld = n | 1
+        return es * (ld * n + 3 * n) + 8 * 5
+
+    assert gs.batched_max_n(True) == 99 and gs.batched_max_n(False) == 141
+    assert need(99, 16) <= 160 * 1024 < need(100, 16)
+    assert need(141, 8) <= 160 * 1024 < need(142, 8)
